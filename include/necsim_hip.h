@@ -1,0 +1,298 @@
+/*
+ * necsim_hip.h -- C ABI of the MI355X (gfx950) Independent coalescence
+ * algorithm.
+ *
+ * The library is a drop-in for necsim-rust's CPU `IndependentAlgorithm`
+ * (rustcoalescence/algorithms/independent/src/lib.rs:39-173) and the existing
+ * GPU `CudaAlgorithm` (rustcoalescence/algorithms/cuda/src/lib.rs:66-593):
+ * a Rust glue crate implements the `Algorithm` trait
+ * (rustcoalescence/algorithms/src/lib.rs:55-112) on top of these entry points.
+ *
+ * Conventions
+ *  - Every function returns 0 (NH_OK) on success and a negative NH_E* code on
+ *    failure; nh_last_error() then describes the failure of the calling thread.
+ *  - All pointers are plain host pointers; the caller owns every host buffer.
+ *    Device memory is owned by the library and freed by the nh_*_destroy calls.
+ *  - A context owns one HIP stream on one device.  Handles are not
+ *    thread-safe: use each context (and the objects created from it) from one
+ *    thread at a time.  Concurrency comes from batching jobs inside one
+ *    persistent kernel (the server API), not from concurrent calls.
+ *  - Struct layouts of nh_lineage, nh_speciation_event, nh_alias_range,
+ *    nh_self_dispersal and nh_alias_atom equal the reference's #[repr(C)]
+ *    types on 64-bit targets, so the glue passes its own buffers unchanged.
+ */
+#ifndef NECSIM_HIP_H
+#define NECSIM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NH_ABI_VERSION 1
+
+/* ---- return codes ------------------------------------------------------- */
+enum {
+    NH_OK = 0,
+    NH_E_INVALID = -1,     /* invalid argument (null pointer, bad enum, bad size) */
+    NH_E_HIP = -2,         /* a HIP runtime call failed (no device, fault, ...) */
+    NH_E_NOMEM = -3,       /* device or host allocation failed */
+    NH_E_STATE = -4,       /* call not valid in the object's current state */
+    NH_E_CONTRACT = -5,    /* scenario violates the in-memory dispersal contract */
+    NH_E_OUT_OF_DEME = -6, /* a lineage lies outside the habitat or its deme */
+    NH_E_CAPACITY = -7,    /* a caller buffer or a server limit is too small */
+};
+
+/* ---- plain data types (reference #[repr(C)] layouts) --------------------- */
+
+/* necsim/core/src/lineage.rs:97-109 `Lineage` (32 B) */
+typedef struct {
+    uint64_t global_reference;
+    double last_event_time;
+    uint32_t x, y, index;
+    uint32_t _pad;
+} nh_lineage;
+
+/* necsim/core/src/event.rs:58-71 `SpeciationEvent` (40 B) */
+typedef struct {
+    uint64_t global_reference;
+    double prior_time;
+    double event_time;
+    uint32_t x, y, index;
+    uint32_t _pad;
+} nh_speciation_event;
+
+/* necsim/impls/no-std/src/alias/packed.rs:11-15 `AliasMethodSamplerAtom<usize>` (24 B) */
+typedef struct {
+    double u;
+    uint64_t e;
+    uint64_t k;
+} nh_alias_atom;
+
+/* .../dispersal_sampler/in_memory/packed_alias/mod.rs `AliasSamplerRange` (16 B) */
+typedef struct {
+    uint64_t start;
+    uint64_t end;
+} nh_alias_range;
+
+/* .../in_memory/packed_separable_alias/mod.rs:35-44 `SeparableAliasSelfDispersal` (16 B) */
+typedef struct {
+    double self_dispersal;
+    uint64_t non_self_dispersal_event;
+} nh_self_dispersal;
+
+/* ---- scenarios ------------------------------------------------------------ */
+enum {
+    /* NonSpatialHabitat + NonSpatialDispersalSampler
+     * (rustcoalescence/scenarios/src/non_spatial.rs:39-89) */
+    NH_SCENARIO_NON_SPATIAL = 0,
+    /* AlmostInfiniteHabitat + AlmostInfiniteNormalDispersalSampler
+     * (rustcoalescence/scenarios/src/almost_infinite/normal.rs:43-94) */
+    NH_SCENARIO_ALMOST_INFINITE_NORMAL = 1,
+    /* InMemoryHabitat + InMemoryPackedSeparableAliasDispersalSampler
+     * (rustcoalescence/scenarios/src/spatially_explicit/turnover/uniform.rs:49-117) */
+    NH_SCENARIO_SPATIALLY_EXPLICIT = 2,
+    /* InMemoryHabitat + InMemoryPackedAliasDispersalSampler
+     * (necsim/impls/no-std/src/cogs/dispersal_sampler/in_memory/packed_alias/mod.rs:49-119) */
+    NH_SCENARIO_SPATIALLY_EXPLICIT_PACKED = 3,
+};
+
+/* The scenario cogs of rustcoalescence_scenarios::ScenarioCogs
+ * (rustcoalescence/scenarios/src/lib.rs:44-95) with uniform turnover and
+ * speciation.  For the in-memory kinds the caller hands over the habitat and
+ * the already built alias tables (row-major over the habitat extent). */
+typedef struct {
+    int32_t kind;
+    uint32_t width, height;       /* extent (non-spatial, in-memory) */
+    uint32_t origin_x, origin_y;  /* extent origin (non-spatial, in-memory) */
+    uint32_t deme;                /* non-spatial deme size */
+    double sigma;                 /* almost-infinite Gaussian sigma */
+    /* in-memory only */
+    const uint32_t *habitat;                 /* width*height */
+    const nh_alias_range *alias_ranges;      /* width*height */
+    const nh_self_dispersal *self_dispersal; /* width*height, separable kind only */
+    const nh_alias_atom *atoms;
+    size_t n_atoms;
+} nh_scenario;
+
+/* ---- algorithm arguments -------------------------------------------------- */
+enum { NH_DEDUP_NONE = 0, NH_DEDUP_RELATIVE = 1, NH_DEDUP_ABSOLUTE = 2 };
+enum { NH_SLICE_RELATIVE = 1, NH_SLICE_ABSOLUTE = 2 };
+enum {
+    /* one kernel launch per step slice, lineages processed in the reference's
+     * water-level queue order (bit-exact with DedupCache::Relative/Absolute) */
+    NH_FORM_PLAIN = 0,
+    /* persistent lane-refilling kernel per water-level round (bit-exact with
+     * DedupCache::None; with dedup on, which of two coalesced duplicates
+     * survives may differ, while species and unique events do not) */
+    NH_FORM_PERSISTENT = 1,
+};
+
+/* IndependentArguments (rustcoalescence/algorithms/independent/src/arguments.rs:79-146)
+ * plus the simulate call's rng / pause_before / resume_after inputs and the
+ * device launch knobs. */
+typedef struct {
+    uint64_t rng_seed;             /* WyHash seed (= seed_from_u64 of the RON Seed) */
+    double speciation_probability; /* UniformSpeciationProbability nu */
+    double turnover_rate;          /* UniformTurnoverRate lambda (0.5 by default) */
+    double delta_t;                /* default 2.0 */
+    uint64_t step_slice;           /* default 10 */
+    int32_t dedup_mode;            /* default NH_DEDUP_RELATIVE, factor 1.0 */
+    double dedup_factor;
+    uint64_t dedup_capacity;
+    int32_t event_slice_mode;      /* default NH_SLICE_RELATIVE, factor 2.0 */
+    double event_slice_factor;
+    uint64_t event_slice_capacity;
+    int32_t has_pause_before;
+    double pause_before;
+    double resume_after;           /* min event time; 0.0 for a fresh simulation */
+    int32_t form;                  /* NH_FORM_* */
+    uint32_t block_size;           /* threads per block, 0 = tuned default */
+    uint32_t blocks_per_cu;        /* persistent form: resident blocks per CU, 0 = tuned */
+    uint32_t use_graph;            /* persistent form: capture each round in a hipGraph */
+} nh_job_params;
+
+typedef struct {
+    int32_t paused;      /* Status::Paused (lineages remain) */
+    double time;         /* SimulationOutcome time: max event time */
+    uint64_t steps;      /* SimulationOutcome steps: completed events */
+    uint64_t n_events;   /* speciation events produced so far */
+    uint64_t n_remaining;/* lineages left (paused) */
+    uint64_t rounds;     /* water-level rounds run */
+    uint64_t launches;   /* step-kernel launches */
+} nh_outcome;
+
+typedef struct nh_ctx nh_ctx;
+typedef struct nh_tables nh_tables;
+typedef struct nh_job nh_job;
+typedef struct nh_server nh_server;
+
+/* ---- library / context ---------------------------------------------------- */
+int nh_abi_version(void);
+/* Describe the last failure of the calling thread (NUL-terminated, truncated). */
+int nh_last_error(char *buf, size_t cap);
+/* rustcoalescence/algorithms/cuda/src/cuda.rs:11-37 (`with_initialised_cuda`):
+ * select the device and create the context's stream. */
+int nh_init(int device, nh_ctx **out);
+int nh_ctx_destroy(nh_ctx *ctx);
+/* number of compute units and device name */
+int nh_device_info(nh_ctx *ctx, uint32_t *compute_units, char *name, size_t name_cap);
+
+/* ---- scenario tables ------------------------------------------------------ */
+/* Upload the scenario cogs (the `LendRustToCuda` borrow of habitat and
+ * dispersal sampler in the CUDA algorithm, .../packed_separable_alias/mod.rs:47-48)
+ * into resident device tables. */
+int nh_tables_create(nh_ctx *ctx, const nh_scenario *scenario, nh_tables **out);
+int nh_tables_destroy(nh_tables *tables);
+/* device bytes held by the tables */
+int nh_tables_footprint(const nh_tables *tables, uint64_t *bytes);
+
+/* ---- one simulation (Algorithm::initialise_and_simulate / resume_and_simulate) ---- */
+int nh_job_create(nh_ctx *ctx, const nh_tables *tables, const nh_job_params *params, nh_job **out);
+/* Append lineages in origin-sampler order (the order defines the water-level
+ * queue, parallelisation/independent/monolithic/mod.rs:94-103).  Lineages
+ * outside the habitat or their deme fail with NH_E_OUT_OF_DEME
+ * (active_lineage_sampler/independent/mod.rs:98-117). */
+int nh_job_add_lineages(nh_job *job, const nh_lineage *lineages, size_t n);
+/* Run one water-level round (monolithic/mod.rs:135-255).  *finished is set to
+ * 1 when no lineage is left or pause_before has been reached. */
+int nh_job_step_round(nh_job *job, int32_t *finished);
+/* Run rounds until finished (parallelisation::independent::monolithic::simulate). */
+int nh_job_simulate(nh_job *job);
+/* Speciation events produced since the last drain, sorted by the PackedEvent
+ * order (necsim/core/src/event.rs:256-280), i.e. what the live water-level
+ * reporter (monolithic/reporter/live.rs:88-115) hands to the reporter.  Call
+ * with buf == NULL to query the count in *n_out. */
+int nh_job_drain_events(nh_job *job, nh_speciation_event *buf, size_t cap, size_t *n_out);
+int nh_job_outcome(const nh_job *job, nh_outcome *out);
+/* Lineages remaining after a paused run (SimulationOutcome::Paused.lineages) */
+int nh_job_take_lineages(nh_job *job, nh_lineage *buf, size_t cap, size_t *n_out);
+int nh_job_destroy(nh_job *job);
+/* Mean device time per step-kernel launch (ms) over the run, from HIP events
+ * recorded on the job's stream, and the number of timed launches. */
+int nh_job_kernel_time(const nh_job *job, double *total_ms, uint64_t *launches);
+
+/* ---- serving: continuous batching of simulations on resident tables ------- */
+typedef struct {
+    uint64_t max_lineages;      /* lane-state capacity shared by requests in flight */
+    uint64_t hbm_budget_bytes;  /* admission limit for per-request device memory */
+    uint64_t tick_steps;        /* max steps per lineage per kernel tick (latency knob) */
+    uint32_t block_size;        /* 0 = default */
+    uint32_t blocks_per_cu;     /* 0 = default */
+} nh_server_params;
+
+typedef struct {
+    uint64_t rng_seed;             /* WyHash seed */
+    double speciation_probability;
+    double turnover_rate;
+    double delta_t;
+    uint64_t step_slice;
+    int32_t dedup_mode;
+    double dedup_factor;
+    uint64_t dedup_capacity;
+    int32_t event_slice_mode;
+    double event_slice_factor;
+    uint64_t event_slice_capacity;
+    int32_t has_pause_before;
+    double pause_before;
+    const nh_lineage *lineages;    /* copied at submission */
+    size_t n_lineages;
+} nh_request;
+
+typedef struct {
+    uint64_t request_id;
+    int32_t status;                /* NH_OK or the request's error */
+    int32_t paused;
+    double time;
+    uint64_t steps;
+    uint64_t n_events;
+    double submit_s, admit_s, done_s; /* host monotonic clock (s) */
+} nh_request_result;
+
+int nh_server_create(nh_ctx *ctx, const nh_tables *tables, const nh_server_params *params,
+                     nh_server **out);
+/* Queue a simulation; returns immediately with its id. */
+int nh_server_submit(nh_server *srv, const nh_request *req, uint64_t *request_id);
+/* Advance the server by one kernel tick: admit queued requests that fit,
+ * run the shared persistent kernel over every admitted request, retire
+ * finished ones.  *n_active / *n_queued report the load afterwards. */
+int nh_server_tick(nh_server *srv, uint64_t *n_active, uint64_t *n_queued);
+/* Pop one finished request (status NH_E_STATE if none is ready). */
+int nh_server_poll(nh_server *srv, nh_request_result *result);
+/* Sorted speciation events of a finished, polled request. */
+int nh_server_request_events(nh_server *srv, uint64_t request_id, nh_speciation_event *buf,
+                             size_t cap, size_t *n_out);
+int nh_server_release(nh_server *srv, uint64_t request_id);
+int nh_server_destroy(nh_server *srv);
+
+/* ---- host helpers (scenario construction for standalone C/C++/Python users;
+ *      the Rust glue uses the reference's own scenario code instead) ------- */
+
+/* SeedableRng::seed_from_u64 + WyHash::from_seed (necsim/core/src/cogs/rng.rs:33-63) */
+uint64_t nh_seed_from_u64(uint64_t seed);
+/* InMemoryPackedSeparableAliasDispersalSampler::new (packed_separable_alias/mod.rs:64-203)
+ * or InMemoryPackedAliasDispersalSampler::new (packed_alias/mod.rs:58-118) from a
+ * sparse dispersal map: row r's entries (col ascending) are
+ * col[row_ptr[r]..row_ptr[r+1]], weight[...]; entries absent from a row are 0.0.
+ * Writes alias_ranges[width*height], self_dispersal[width*height] (separable
+ * only, may be NULL otherwise) and atoms (capacity atoms_cap); *n_atoms
+ * receives the atom count (query with atoms == NULL). */
+int nh_build_alias_tables(uint32_t width, uint32_t height, const uint32_t *habitat,
+                          const uint64_t *row_ptr, const uint32_t *col, const double *weight,
+                          int32_t separable, nh_alias_range *alias_ranges,
+                          nh_self_dispersal *self_dispersal, nh_alias_atom *atoms,
+                          size_t atoms_cap, size_t *n_atoms);
+/* OriginPreSampler::all().percentage(p).partition(rank, size) fed into the
+ * scenario's origin sampler (non_spatial.rs:89-112, in_memory.rs:89-119,
+ * singleton_demes/circle.rs:123-156).  Query with out == NULL. */
+int nh_sample_origins(const nh_scenario *scenario, double percentage, uint32_t rank,
+                      uint32_t size, uint32_t circle_centre_x, uint32_t circle_centre_y,
+                      uint32_t circle_radius, nh_lineage *out, size_t cap, size_t *n_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NECSIM_HIP_H */

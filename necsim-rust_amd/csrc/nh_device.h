@@ -1,0 +1,335 @@
+// nh_device.h -- device data layout and the per-lineage step of the
+// Independent algorithm, shared by the plain and the persistent kernels.
+//
+// One lineage per lane.  A step is one event of
+// Simulation::simulate_incremental_early_stop (necsim/core/src/simulation/mod.rs:53-134):
+//   1. next event time weakly after max(last_event_time, min_event_time)
+//      (IndependentActiveLineageSampler::pop_active_lineage_and_event_time,
+//       active_lineage_sampler/independent/sampler.rs:69-116, with the
+//       PoissonEventTimeSampler, event_time_sampler/poisson.rs:30-109);
+//   2. the speciation draw and min-speciation-sample tracking
+//      (event_sampler/independent.rs:112-172, event_sampler/tracking.rs:39-56);
+//   3. otherwise dispersal, the coalescence draw and the indexed target
+//      (event_sampler/independent.rs:174-221, coalescence_sampler/independent.rs:38-76).
+#pragma once
+
+#include "nh_rng.h"
+
+namespace nh {
+
+enum : int {
+    K_NON_SPATIAL = 0,
+    K_ALMOST_INFINITE = 1,
+    K_SEPARABLE = 2,
+    K_PACKED = 3,
+};
+
+// One 32-byte record per habitat cell: everything a step needs about the
+// lineage's current cell (injective-id prefix and deme for RNG priming and the
+// coalescence draw, alias range and self-dispersal for the next dispersal).
+struct __attribute__((aligned(32))) CellRec {
+    uint64_t prefix;     // InMemoryHabitat u64_injection[cell] (habitat/in_memory.rs:151-160)
+    uint32_t habitat;    // deme size
+    uint32_t atom_start; // alias range start
+    uint32_t atom_len;   // alias range length
+    uint32_t non_self;   // SeparableAliasSelfDispersal::non_self_dispersal_event
+    double self_p;       // SeparableAliasSelfDispersal::self_dispersal
+};
+static_assert(sizeof(CellRec) == 32, "cell record is 32 B");
+
+// AliasMethodSamplerAtom<usize> with u32 events (16 B instead of 24 B)
+struct __attribute__((aligned(16))) Atom {
+    double u;
+    uint32_t e;
+    uint32_t k;
+};
+static_assert(sizeof(Atom) == 16, "atom is 16 B");
+
+struct DevScenario {
+    int kind;
+    uint32_t width, height, ox, oy, deme;
+    uint64_t area;   // width * height (non-spatial dispersal index range)
+    double area_f;   // (double)area
+    double deme_f;   // (double)deme
+    double sigma;
+    const CellRec *cells;
+    const Atom *atoms;
+};
+
+constexpr int RATIO_N = 256; // Poisson inversion ratios lambda*dt/k cached for k < RATIO_N
+
+// Per-simulation constants
+struct DevParams {
+    uint64_t seed;          // WyHash seed
+    double nu;              // speciation probability
+    double lambda_dt;       // lambda * delta_t (poisson.rs:41)
+    double p0;              // exp(-lambda_dt) (poisson.rs:42)
+    double lambda_dt_sqrt;  // sqrt(lambda_dt) for the normal fallback (poisson.rs:70)
+    double delta_t;
+    double inv_delta_t;     // 1/delta_t when delta_t is a power of two
+    int dt_pow2;
+    double min_event_time;  // resume_after
+    uint64_t step_slice;
+    const double *ratio;    // ratio[k] = lambda_dt / k, k in 1..RATIO_N-1 (device, global)
+};
+
+// Lineage state held in HBM between launches (SoA, coalesced per field).
+struct LineageSoA {
+    uint64_t *ref;      // global_reference
+    double *t_last;     // last_event_time
+    uint32_t *x, *y, *idx;
+    double *next_t;     // next event time (pessimistic lower bound, or exact if flag bit 0)
+    uint64_t *next_st;  // RNG state after the final prime of next_t (exact only)
+    uint8_t *flags;     // bit 0: next_t / next_st are exact
+};
+
+// Speciation event, the reference's SpeciationEvent layout (40 B)
+struct DevEvent {
+    uint64_t ref;
+    double prior_time;
+    double event_time;
+    uint32_t x, y, index, pad;
+};
+static_assert(sizeof(DevEvent) == 40, "event is 40 B");
+
+// Lane-resident lineage.  a/b hold (x, y) for the almost-infinite habitat and
+// (cell index, unused) for finite extents; `id` is the injective location id
+// used to prime the RNG.
+template <int KIND> struct Lane {
+    uint64_t ref;
+    double t_last;
+    uint32_t a, b, i;
+    uint64_t id;
+    CellRec rec;        // finite in-memory kinds: record of the current cell
+    double next_t;
+    uint64_t next_st;
+    bool have_next;
+    // min speciation sample of the current slice
+    bool ms_some;
+    double ms_u, ms_t;
+    uint32_t ms_a, ms_b, ms_i;
+};
+
+NH_HD bool is_cell_kind(int kind) { return kind != K_ALMOST_INFINITE; }
+
+// ---------------------------------------------------------------------------
+// location helpers
+template <int KIND>
+__device__ __forceinline__ void location_xy(const DevScenario &S, uint32_t a, uint32_t b, uint32_t &x,
+                                            uint32_t &y) {
+    if (KIND == K_ALMOST_INFINITE) {
+        x = a;
+        y = b;
+    } else {
+        x = S.ox + a % S.width;
+        y = S.oy + a / S.width;
+    }
+}
+
+// Load a lineage's location into lane form.  Returns false if the location is
+// outside the habitat extent.
+template <int KIND>
+__device__ __forceinline__ void lane_set_location(Lane<KIND> &L, const DevScenario &S, uint32_t x,
+                                                  uint32_t y, uint32_t i) {
+    L.i = i;
+    if (KIND == K_ALMOST_INFINITE) {
+        L.a = x;
+        L.b = y;
+        L.id = ((uint64_t)y << 32) | (uint64_t)x; // habitat/almost_infinite.rs:67-71
+    } else {
+        uint32_t cell = (y - S.oy) * S.width + (x - S.ox);
+        L.a = cell;
+        L.b = 0;
+        if (KIND == K_NON_SPATIAL) {
+            // habitat/non_spatial.rs:93-109
+            L.id = (uint64_t)cell * (uint64_t)S.deme + (uint64_t)i;
+        } else {
+            L.rec = S.cells[cell];
+            L.id = L.rec.prefix + (uint64_t)i; // habitat/in_memory.rs:71-81
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PoissonEventTimeSampler::next_event_time_at_indexed_location_weakly_after
+// (event_time_sampler/poisson.rs:30-109).  Returns the event time and leaves
+// `state` primed for the event's own draws.
+__device__ __forceinline__ double poisson_next(const DevParams &P, const double *ratio, uint64_t id,
+                                               double time, uint64_t &state) {
+    uint64_t ts = as_u64(floor_(P.dt_pow2 ? time * P.inv_delta_t : time / P.delta_t));
+    double best = 0.0;
+    uint32_t best_index = 0;
+    for (;;) {
+        state = prime(P.seed, id, ts);
+        uint32_t number;
+        if (P.p0 > 0.0) {
+            uint32_t k = 0;
+            double prod = P.p0;
+            double acc = P.p0;
+            double u = uniform_closed_open(state);
+            while (u > acc && prod > 0.0) {
+                k += 1;
+                prod *= (k < RATIO_N) ? ratio[k] : P.lambda_dt / (double)k;
+                acc += prod;
+            }
+            number = k;
+        } else {
+            double a, b;
+            sample_2d_normal(state, P.lambda_dt, P.lambda_dt_sqrt, a, b);
+            number = as_u32(a > 0.0 || a != a ? (a != a ? 0.0 : a) : 0.0);
+        }
+        bool found = false;
+        for (uint32_t e = 0; e < number; e++) {
+            double t = ((double)ts + uniform_closed_open(state)) * P.delta_t;
+            if (t > time && (!found || best > t)) {
+                best = t;
+                best_index = e;
+                found = true;
+            }
+        }
+        if (found) break;
+        ts += 1;
+    }
+    state = prime(P.seed, id, ts + INV_PHI * (uint64_t)(best_index + 1));
+    return best;
+}
+
+// Make sure L.next_t / L.next_st hold the next event (the `pop` peek).
+template <int KIND>
+__device__ __forceinline__ void lane_peek(Lane<KIND> &L, const DevParams &P, const double *ratio) {
+    if (L.have_next) return;
+    // NonNegativeF64::max of last_event_time and min_event_time (sampler.rs:87)
+    double before = P.min_event_time > L.t_last ? P.min_event_time : L.t_last;
+    uint64_t st;
+    double et = poisson_next(P, ratio, L.id, before, st);
+    L.next_t = max_after(before, et);
+    L.next_st = st;
+    L.have_next = true;
+}
+
+enum : int { STEP_DISPERSED = 0, STEP_SPECIATED = 1 };
+
+// Execute the peeked event (L.have_next must hold and L.next_t < level).
+// On speciation the event is written to *ev.
+template <int KIND>
+__device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S, const DevParams &P,
+                                            DevEvent &ev) {
+    double net = L.next_t;
+    uint64_t st = L.next_st;
+    L.have_next = false;
+
+    double u_spec = uniform_closed_open(st);
+    // SpeciationSample::update_min keeps the first minimum (tracking.rs:46-55)
+    if (!(L.ms_some && L.ms_u <= u_spec)) {
+        L.ms_some = true;
+        L.ms_u = u_spec;
+        L.ms_t = net;
+        L.ms_a = L.a;
+        L.ms_b = L.b;
+        L.ms_i = L.i;
+    }
+    if (u_spec < P.nu) {
+        uint32_t x, y;
+        location_xy<KIND>(S, L.a, L.b, x, y);
+        ev.ref = L.ref;
+        ev.prior_time = L.t_last;
+        ev.event_time = net;
+        ev.x = x;
+        ev.y = y;
+        ev.index = L.i;
+        ev.pad = 0;
+        L.t_last = net;
+        return STEP_SPECIATED;
+    }
+
+    if (KIND == K_NON_SPATIAL) {
+        // NonSpatialDispersalSampler (dispersal_sampler/non_spatial.rs:41-70):
+        // sample_index_u64(W*H) (rng.rs:132-144).  The target cell index is the
+        // index itself, so no division is needed here.
+        double u = uniform_closed_open(st);
+        uint64_t v = as_u64(floor_(u * S.area_f));
+        uint64_t cell = v < S.area - 1 ? v : S.area - 1;
+        double uc = uniform_closed_open(st);
+        uint32_t ci = as_u32(floor_(uc * S.deme_f));
+        L.a = (uint32_t)cell;
+        L.i = ci;
+        L.id = cell * (uint64_t)S.deme + (uint64_t)ci;
+    } else if (KIND == K_ALMOST_INFINITE) {
+        // AlmostInfiniteNormalDispersalSampler + clamp_round_dispersal
+        // (almost_infinite_normal.rs:52-69, habitat/almost_infinite.rs:98-117).
+        // ((x + dx) % 2^32 + 2^32) % 2^32 is the low word of x + dx.
+        double dx, dy;
+        sample_2d_normal(st, 0.0, S.sigma, dx, dy);
+        int64_t ix = as_i64(round_(dx));
+        int64_t iy = as_i64(round_(dy));
+        (void)uniform_closed_open(st); // CoalescenceRngSample::new, unused by singleton demes
+        L.a = (uint32_t)((uint64_t)L.a + (uint64_t)ix);
+        L.b = (uint32_t)((uint64_t)L.b + (uint64_t)iy);
+        L.i = 0;
+        L.id = ((uint64_t)L.b << 32) | (uint64_t)L.a;
+    } else {
+        // InMemoryPackedSeparableAliasDispersalSampler (packed_separable_alias/dispersal.rs:16-77)
+        // and InMemoryPackedAliasDispersalSampler (packed_alias/dispersal.rs:16-52)
+        uint32_t cell = L.a;
+        uint32_t target;
+        bool alias;
+        if (KIND == K_SEPARABLE) {
+            if (L.rec.self_p == 1.0) {
+                alias = false;
+            } else if (L.rec.self_p == 0.0 || L.rec.non_self != cell) {
+                alias = true;
+            } else {
+                alias = uniform_closed_open(st) >= L.rec.self_p;
+            }
+        } else {
+            alias = true;
+        }
+        if (alias) {
+            // AliasMethodSamplerAtom::sample_event_with_cdf_limit, limit 1.0 (alias/packed.rs:135-163)
+            double f = uniform_closed_open(st) * 1.0 * (double)L.rec.atom_len;
+            uint64_t k = as_u64(floor_(f));
+            double yv = f - (double)k;
+            Atom at = S.atoms[L.rec.atom_start + (uint32_t)k];
+            target = yv < at.u ? at.e : at.k;
+        } else {
+            target = cell;
+        }
+        CellRec t = S.cells[target];
+        double uc = uniform_closed_open(st);
+        uint32_t ci = as_u32(floor_(uc * (double)t.habitat));
+        L.a = target;
+        L.i = ci;
+        L.rec = t;
+        L.id = t.prefix + (uint64_t)ci;
+    }
+    L.t_last = net;
+    return STEP_DISPERSED;
+}
+
+// ---------------------------------------------------------------------------
+// Dedup cache buckets: 4 x u64 (time bits, sample bits, x | y << 32,
+// index | check << 32).  An all-zero bucket is empty (event times are > 0).
+struct Sample {
+    double u, t;
+    uint32_t x, y, i;
+};
+
+NH_HD uint32_t sample_check(const Sample &s) {
+    uint64_t h = wymum(bits(s.t) ^ 0x9e3779b97f4a7c15ull, bits(s.u) ^ (((uint64_t)s.x << 32) | s.y));
+    h = wymum(h ^ s.i, 0xeb44accab455d165ull);
+    return (uint32_t)(h >> 32) | 1u;
+}
+
+__device__ __forceinline__ bool bucket_equals(const uint64_t *b, const Sample &s) {
+    return b[0] == bits(s.t) && b[1] == bits(s.u) && b[2] == (((uint64_t)s.y << 32) | s.x) &&
+           b[3] == (((uint64_t)sample_check(s) << 32) | s.i);
+}
+
+__device__ __forceinline__ void bucket_store(uint64_t *b, const Sample &s) {
+    b[0] = bits(s.t);
+    b[1] = bits(s.u);
+    b[2] = ((uint64_t)s.y << 32) | s.x;
+    b[3] = ((uint64_t)sample_check(s) << 32) | s.i;
+}
+
+} // namespace nh

@@ -1,0 +1,105 @@
+// nh_kernels.h -- launch interface of the gfx950 kernels (internal to the library).
+#pragma once
+
+#include "nh_device.h"
+
+namespace nh {
+
+// Device-side counters of one simulation (or one server).
+struct Counters {
+    unsigned long long head;          // persistent work-queue head
+    unsigned long long n_out;         // lineages written to the next round
+    unsigned long long n_events;      // speciation events appended
+    unsigned long long steps;         // completed events
+    unsigned long long max_time_bits; // max event time (positive f64 bits)
+    unsigned long long overflow;      // plain dedup: a bucket list exceeded the device limit
+    unsigned long long pad[2];
+};
+
+// Per-round inputs read by the kernels from device memory (so that a captured
+// graph can be replayed with new values).
+struct RoundDesc {
+    double level;
+    unsigned long long n_in;
+};
+
+// plain-form element status (one byte per pass element)
+enum : uint8_t {
+    ST_SKIP = 0,     // not processed: pessimistic next event >= water level
+    ST_SPEC = 1,     // speciated
+    ST_SLOW = 2,     // stopped by the step slice, next event below the level
+    ST_FAST = 3,     // next event at or above the level
+    ST_DUP = 4,      // dropped by the dedup cache
+    ST_KIND_MASK = 7,
+    ST_HAS_SAMPLE = 8, // executed >= 1 step (has a min speciation sample)
+};
+
+struct PersistArgs {
+    DevScenario S;
+    DevParams P;
+    LineageSoA in, out;
+    const RoundDesc *round;
+    uint64_t *cache;          // 4 u64 per bucket, null if dedup is off
+    unsigned long long cache_cap;
+    DevEvent *events;
+    Counters *ctr;
+    unsigned long long tick_steps; // 0: unlimited (a lineage runs to the level)
+};
+
+struct PlainArgs {
+    DevScenario S;
+    DevParams P;
+    LineageSoA L;
+    const uint32_t *pass;     // lineage slots in processing order
+    unsigned long long m;
+    const RoundDesc *round;
+    uint8_t *status;          // [m]
+    // min speciation samples of the slice, per pass element
+    double *s_u, *s_t;
+    uint32_t *s_x, *s_y, *s_i;
+    unsigned long long *s_bucket;
+    unsigned long long cache_cap; // 0: dedup off
+    DevEvent *events;
+    Counters *ctr;
+};
+
+// kernel launchers (all asynchronous on `stream`)
+void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream);
+int persistent_max_blocks_per_cu(int kind, int threads);
+void launch_plain(int kind, const PlainArgs &a, int threads, hipStream_t stream);
+
+void launch_dedup_push(const uint8_t *status, const unsigned long long *bucket, uint32_t *head,
+                       uint32_t *link, uint64_t m, hipStream_t stream);
+void launch_dedup_resolve(uint8_t *status, const unsigned long long *bucket, const double *s_u,
+                          const double *s_t, const uint32_t *s_x, const uint32_t *s_y,
+                          const uint32_t *s_i, uint32_t *head, const uint32_t *link,
+                          uint64_t *cache, uint64_t m, Counters *ctr, hipStream_t stream);
+
+// Ordered two-way compaction of pass elements: slots with status ST_SLOW go to
+// out_a (next pass), status ST_FAST to out_b + *b_base (appended).  Counts are
+// written to counts[0] (a) and counts[1] (b).
+struct CompactBuffers {
+    uint32_t *blk_a, *blk_b; // [nblocks + 1]
+    unsigned int *counts;    // [2] device
+};
+uint32_t compact_blocks(uint64_t m);
+void launch_compact(const uint8_t *status, const uint32_t *pass, uint64_t m, uint32_t *out_a,
+                    uint32_t *out_b, uint64_t b_base, CompactBuffers &cb, hipStream_t stream);
+
+// Event sort: sorts events[0..n) by the PackedEvent order into out[0..n).
+struct SortBuffers {
+    unsigned long long *keys[2];
+    uint32_t *vals[2];
+    uint32_t *hist;         // [256 * nblocks + 1]
+    unsigned long long *minmax; // [2] device
+    uint64_t capacity;
+};
+uint32_t sort_blocks(uint64_t n);
+void sort_events(const DevEvent *events, uint64_t n, DevEvent *out, SortBuffers &sb,
+                 hipStream_t stream, bool fixed_passes);
+
+void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream);
+void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream);
+void launch_reset_counters(Counters *c, hipStream_t stream);
+
+} // namespace nh
