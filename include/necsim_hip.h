@@ -291,6 +291,16 @@ int nh_sample_origins(const nh_scenario *scenario, double percentage, uint32_t r
                       uint32_t size, uint32_t circle_centre_x, uint32_t circle_centre_y,
                       uint32_t circle_radius, nh_lineage *out, size_t cap, size_t *n_out);
 
+/* ---- diagnostics ---------------------------------------------------------
+ * Evaluate one device primitive elementwise on the GPU (no counterpart in the
+ * reference: used to pin the device MathsCore and RNG against the CPU).
+ * op: 0 ln, 1 exp, 2 sin, 3 cos, 4 sqrt, 5 floor, 6 round, 7 divide
+ * (in[2i] / in[2i+1]), 8 wymum (u64 pairs), 9 seahash_diffuse (u64),
+ * 10 prime (in[3i..3i+2] = seed, location, time; u64), 11 uniform_closed_open
+ * after prime (same inputs, f64 out).  in / out are host buffers of 8-byte
+ * words; n is the number of outputs. */
+int nh_debug_eval(nh_ctx *ctx, int32_t op, const uint64_t *in, uint64_t *out, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

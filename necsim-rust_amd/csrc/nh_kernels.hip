@@ -1,0 +1,679 @@
+// nh_kernels.hip -- gfx950 kernels of the Independent algorithm.
+//
+//  * k_persistent<KIND>: one water-level round.  Wave64 lanes each hold one
+//    lineage in registers and refill from a device work queue (one atomic per
+//    wave, ballot/mbcnt distribution) whenever their lineage speciates, is
+//    dropped as a duplicate, or reaches the water level.  Dedup inserts happen
+//    in-kernel at every step-slice boundary.
+//  * k_plain<KIND>: one launch per step slice over the reference's queue
+//    order (parallelisation/independent/monolithic/mod.rs:183-251); dedup and
+//    reclassification follow in separate order-preserving kernels.
+//  * ordered dedup, ordered compaction and an LSD radix sort (wave ballot
+//    ranking) for the water-level event sort (monolithic/reporter/live.rs:88-115).
+#include "nh_kernels.h"
+
+namespace nh {
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+// ---------------------------------------------------------------------------
+// lineage load / store
+template <int KIND>
+__device__ __forceinline__ void lane_load(Lane<KIND> &L, const DevScenario &S, const LineageSoA &s,
+                                          uint64_t slot) {
+    L.ref = s.ref[slot];
+    L.t_last = s.t_last[slot];
+    lane_set_location<KIND>(L, S, s.x[slot], s.y[slot], s.idx[slot]);
+    uint8_t f = s.flags[slot];
+    L.have_next = (f & 1) != 0;
+    L.next_t = s.next_t[slot];
+    L.next_st = L.have_next ? s.next_st[slot] : 0;
+    L.ms_some = false;
+}
+
+template <int KIND>
+__device__ __forceinline__ void lane_store(const Lane<KIND> &L, const DevScenario &S,
+                                           const LineageSoA &s, uint64_t slot) {
+    uint32_t x, y;
+    location_xy<KIND>(S, L.a, L.b, x, y);
+    s.ref[slot] = L.ref;
+    s.t_last[slot] = L.t_last;
+    s.x[slot] = x;
+    s.y[slot] = y;
+    s.idx[slot] = L.i;
+    s.next_t[slot] = L.next_t;
+    s.next_st[slot] = L.next_st;
+    s.flags[slot] = L.have_next ? 1 : 0;
+}
+
+template <int KIND>
+__device__ __forceinline__ Sample lane_sample(const Lane<KIND> &L, const DevScenario &S) {
+    Sample s;
+    s.u = L.ms_u;
+    s.t = L.ms_t;
+    location_xy<KIND>(S, L.ms_a, L.ms_b, s.x, s.y);
+    s.i = L.ms_i;
+    return s;
+}
+
+// DirectMappedCache::insert (necsim/impls/no-std/src/cache.rs:39-54):
+// returns true if the bucket already held an equal sample (a duplicate).
+__device__ __forceinline__ bool cache_insert_concurrent(uint64_t *cache, unsigned long long cap,
+                                                        const Sample &s) {
+    uint64_t h = speciation_sample_hash(s.u, s.t, s.x, s.y, s.i);
+    uint64_t *b = cache + 4 * (h % cap);
+    bool dup = bucket_equals(b, s);
+    bucket_store(b, s);
+    return dup;
+}
+
+__device__ __forceinline__ void load_ratio(double *s_ratio, const double *ratio) {
+    for (int k = threadIdx.x; k < RATIO_N; k += blockDim.x) s_ratio[k] = ratio[k];
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Persistent round kernel
+template <int KIND>
+__global__ void __launch_bounds__(256) k_persistent(PersistArgs A) {
+    __shared__ double s_ratio[RATIO_N];
+    load_ratio(s_ratio, A.P.ratio);
+
+    const double level = A.round->level;
+    const unsigned long long n_in = A.round->n_in;
+    const unsigned lane = lane_id();
+
+    Lane<KIND> L;
+    L.have_next = false;
+    L.ms_some = false;
+    bool has = false;
+    bool exhausted = false;
+    uint64_t slice_steps = 0;
+    uint64_t tick = 0;
+    unsigned long long my_steps = 0;
+    double my_max = 0.0;
+
+    for (;;) {
+        if (!exhausted) {
+            uint64_t need = __ballot(!has);
+            if (need) {
+                uint32_t cnt = __popcll(need);
+                uint32_t leader = __ffsll((unsigned long long)need) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(&A.ctr->head, (unsigned long long)cnt);
+                base = __shfl(base, leader);
+                if (base + cnt >= n_in) exhausted = true;
+                if (!has) {
+                    unsigned long long pos = base + __popcll(need & lanemask_lt());
+                    if (pos < n_in) {
+                        lane_load<KIND>(L, A.S, A.in, pos);
+                        has = true;
+                        slice_steps = 0;
+                        tick = 0;
+                    }
+                }
+            }
+        }
+        if (!__any(has)) break;
+
+        bool emit = false, out = false;
+        DevEvent ev;
+        if (has) {
+            lane_peek<KIND>(L, A.P, s_ratio);
+            if (L.next_t >= level || (A.tick_steps && tick >= A.tick_steps)) {
+                // The lineage leaves this launch: its (partial) slice ends and
+                // the slice sample is inserted like on reclassification
+                // (monolithic/mod.rs:205-219).  A tick-limited lineage starts
+                // a fresh slice when it is resumed.
+                bool dup = false;
+                if (A.cache && L.ms_some)
+                    dup = cache_insert_concurrent(A.cache, A.cache_cap, lane_sample<KIND>(L, A.S));
+                has = false;
+                out = !dup;
+            } else if (slice_steps >= A.P.step_slice) {
+                bool dup = false;
+                if (A.cache && L.ms_some)
+                    dup = cache_insert_concurrent(A.cache, A.cache_cap, lane_sample<KIND>(L, A.S));
+                L.ms_some = false;
+                slice_steps = 0;
+                if (dup) has = false;
+            } else {
+                int r = lane_execute<KIND>(L, A.S, A.P, ev);
+                my_steps++;
+                slice_steps++;
+                tick++;
+                my_max = L.t_last > my_max ? L.t_last : my_max;
+                if (r == STEP_SPECIATED) {
+                    has = false;
+                    emit = true;
+                }
+            }
+        }
+        // wave-aggregated appends (events are rare, round-end write-outs are not)
+        uint64_t em = __ballot(emit);
+        if (em) {
+            uint32_t leader = __ffsll((unsigned long long)em) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&A.ctr->n_events, (unsigned long long)__popcll(em));
+            base = __shfl(base, leader);
+            if (emit) A.events[base + __popcll(em & lanemask_lt())] = ev;
+        }
+        uint64_t om = __ballot(out);
+        if (om) {
+            uint32_t leader = __ffsll((unsigned long long)om) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&A.ctr->n_out, (unsigned long long)__popcll(om));
+            base = __shfl(base, leader);
+            if (out) lane_store<KIND>(L, A.S, A.out, base + __popcll(om & lanemask_lt()));
+        }
+    }
+
+    // block reduction of steps and max time
+    __shared__ unsigned long long s_steps;
+    __shared__ unsigned long long s_max;
+    if (threadIdx.x == 0) {
+        s_steps = 0;
+        s_max = 0;
+    }
+    __syncthreads();
+    if (my_steps) atomicAdd(&s_steps, my_steps);
+    if (my_max > 0.0) atomicMax(&s_max, (unsigned long long)bits(my_max));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_steps) atomicAdd(&A.ctr->steps, s_steps);
+        if (s_max) atomicMax(&A.ctr->max_time_bits, s_max);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain slice kernel: element q of the pass runs one step slice
+template <int KIND>
+__global__ void __launch_bounds__(256) k_plain(PlainArgs A) {
+    __shared__ double s_ratio[RATIO_N];
+    load_ratio(s_ratio, A.P.ratio);
+    const unsigned long long q = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const double level = A.round->level;
+    unsigned long long my_steps = 0;
+    double my_max = 0.0;
+    bool emit = false;
+    DevEvent ev;
+    if (q < A.m) {
+        uint32_t slot = A.pass[q];
+        // the pop loop's check on the stored (pessimistic) next event time
+        // (monolithic/mod.rs:186-196)
+        double stored_next = A.L.next_t[slot];
+        if (!(stored_next < level)) {
+            A.status[q] = ST_SKIP;
+        } else {
+            Lane<KIND> L;
+            lane_load<KIND>(L, A.S, A.L, slot);
+            uint8_t st;
+            for (;;) {
+                lane_peek<KIND>(L, A.P, s_ratio);
+                if (L.next_t >= level) {
+                    st = ST_FAST;
+                    break;
+                }
+                if (my_steps >= A.P.step_slice) {
+                    st = ST_SLOW;
+                    break;
+                }
+                int r = lane_execute<KIND>(L, A.S, A.P, ev);
+                my_steps++;
+                if (r == STEP_SPECIATED) {
+                    st = ST_SPEC;
+                    emit = true;
+                    break;
+                }
+            }
+            if (my_steps) my_max = L.t_last;
+            if (st != ST_SPEC) {
+                lane_store<KIND>(L, A.S, A.L, slot);
+                if (L.ms_some) {
+                    st |= ST_HAS_SAMPLE;
+                    if (A.cache_cap) {
+                        Sample s = lane_sample<KIND>(L, A.S);
+                        A.s_u[q] = s.u;
+                        A.s_t[q] = s.t;
+                        A.s_x[q] = s.x;
+                        A.s_y[q] = s.y;
+                        A.s_i[q] = s.i;
+                        A.s_bucket[q] = speciation_sample_hash(s.u, s.t, s.x, s.y, s.i) % A.cache_cap;
+                    }
+                }
+            }
+            A.status[q] = st;
+        }
+    }
+    const unsigned lane = lane_id();
+    uint64_t em = __ballot(emit);
+    if (em) {
+        uint32_t leader = __ffsll((unsigned long long)em) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(&A.ctr->n_events, (unsigned long long)__popcll(em));
+        base = __shfl(base, leader);
+        if (emit) A.events[base + __popcll(em & lanemask_lt())] = ev;
+    }
+    __shared__ unsigned long long s_steps;
+    __shared__ unsigned long long s_max;
+    if (threadIdx.x == 0) {
+        s_steps = 0;
+        s_max = 0;
+    }
+    __syncthreads();
+    if (my_steps) atomicAdd(&s_steps, my_steps);
+    if (my_max > 0.0) atomicMax(&s_max, (unsigned long long)bits(my_max));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_steps) atomicAdd(&A.ctr->steps, s_steps);
+        if (s_max) atomicMax(&A.ctr->max_time_bits, s_max);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ordered dedup of one pass (exact DirectMappedCache semantics in pass order)
+constexpr uint32_t NO_LINK = 0xffffffffu;
+constexpr int DEDUP_LIST_MAX = 64;
+
+__global__ void k_dedup_push(const uint8_t *status, const unsigned long long *bucket, uint32_t *head,
+                             uint32_t *link, uint64_t m) {
+    uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    if (!(status[q] & ST_HAS_SAMPLE)) return;
+    uint32_t old = atomicExch(&head[bucket[q]], (uint32_t)q);
+    link[q] = old;
+}
+
+// The last pusher of a bucket owns it: it orders the bucket's inserts by
+// pass position and replays them sequentially against the cache bucket.
+__global__ void k_dedup_resolve(uint8_t *status, const unsigned long long *bucket, const double *s_u,
+                                const double *s_t, const uint32_t *s_x, const uint32_t *s_y,
+                                const uint32_t *s_i, uint32_t *head, const uint32_t *link,
+                                uint64_t *cache, uint64_t m, Counters *ctr) {
+    uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    if (!(status[q] & ST_HAS_SAMPLE)) return;
+    unsigned long long b = bucket[q];
+    if (head[b] != (uint32_t)q) return;
+    uint32_t list[DEDUP_LIST_MAX];
+    int n = 0;
+    for (uint32_t e = (uint32_t)q; e != NO_LINK; e = link[e]) {
+        if (n == DEDUP_LIST_MAX) {
+            atomicOr(&ctr->overflow, 1ull);
+            return; // the host replays this pass
+        }
+        list[n++] = e;
+    }
+    for (int a = 1; a < n; a++) { // insertion sort by pass position
+        uint32_t v = list[a];
+        int c = a - 1;
+        while (c >= 0 && list[c] > v) {
+            list[c + 1] = list[c];
+            c--;
+        }
+        list[c + 1] = v;
+    }
+    uint64_t *cb = cache + 4 * b;
+    uint64_t w0 = cb[0], w1 = cb[1], w2 = cb[2], w3 = cb[3];
+    for (int a = 0; a < n; a++) {
+        uint32_t e = list[a];
+        Sample s{s_u[e], s_t[e], s_x[e], s_y[e], s_i[e]};
+        uint64_t n0 = bits(s.t), n1 = bits(s.u), n2 = ((uint64_t)s.y << 32) | s.x;
+        uint64_t n3 = ((uint64_t)sample_check(s) << 32) | s.i;
+        if (w0 == n0 && w1 == n1 && w2 == n2 && w3 == n3) {
+            status[e] = (uint8_t)((status[e] & ~ST_KIND_MASK) | ST_DUP);
+        }
+        w0 = n0; w1 = n1; w2 = n2; w3 = n3;
+    }
+    cb[0] = w0; cb[1] = w1; cb[2] = w2; cb[3] = w3;
+    head[b] = NO_LINK;
+}
+
+// ---------------------------------------------------------------------------
+// Ordered two-way compaction
+constexpr int CT_THREADS = 256;
+constexpr int CT_ITEMS = 8;
+constexpr int CT_TILE = CT_THREADS * CT_ITEMS;
+
+__global__ void __launch_bounds__(CT_THREADS) k_compact_count(const uint8_t *status, uint64_t m,
+                                                              uint32_t *blk_a, uint32_t *blk_b) {
+    __shared__ uint32_t ca, cbn;
+    if (threadIdx.x == 0) { ca = 0; cbn = 0; }
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * CT_TILE;
+    uint32_t a = 0, b = 0;
+    for (int j = 0; j < CT_ITEMS; j++) {
+        uint64_t q = base + (uint64_t)j * CT_THREADS + threadIdx.x;
+        if (q < m) {
+            uint8_t k = status[q] & ST_KIND_MASK;
+            a += k == ST_SLOW;
+            b += k == ST_FAST;
+        }
+    }
+    if (a) atomicAdd(&ca, a);
+    if (b) atomicAdd(&cbn, b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        blk_a[blockIdx.x] = ca;
+        blk_b[blockIdx.x] = cbn;
+    }
+}
+
+// exclusive scan of n values in one block; data[n] receives the total
+__global__ void __launch_bounds__(1024) k_scan_exclusive(uint32_t *data, uint64_t n) {
+    __shared__ uint32_t partial[1024];
+    const int T = blockDim.x;
+    uint64_t per = (n + T - 1) / T;
+    uint64_t lo = per * threadIdx.x, hi = lo + per < n ? lo + per : n;
+    uint32_t sum = 0;
+    for (uint64_t j = lo; j < hi; j++) sum += data[j];
+    partial[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < T; off <<= 1) {
+        uint32_t v = threadIdx.x >= (unsigned)off ? partial[threadIdx.x - off] : 0;
+        __syncthreads();
+        partial[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? partial[threadIdx.x - 1] : 0;
+    for (uint64_t j = lo; j < hi; j++) {
+        uint32_t v = data[j];
+        data[j] = run;
+        run += v;
+    }
+    if (threadIdx.x == T - 1) data[n] = partial[T - 1];
+}
+
+__global__ void __launch_bounds__(CT_THREADS) k_compact_scatter(const uint8_t *status, const uint32_t *pass,
+                                                                uint64_t m, const uint32_t *off_a,
+                                                                const uint32_t *off_b, uint32_t *out_a,
+                                                                uint32_t *out_b, uint64_t b_base) {
+    __shared__ uint32_t wa[CT_THREADS / 64], wb[CT_THREADS / 64];
+    __shared__ uint32_t run_a, run_b;
+    const unsigned lane = lane_id();
+    const unsigned wave = threadIdx.x / 64;
+    if (threadIdx.x == 0) {
+        run_a = off_a[blockIdx.x];
+        run_b = off_b[blockIdx.x];
+    }
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * CT_TILE;
+    for (int j = 0; j < CT_ITEMS; j++) {
+        uint64_t q = base + (uint64_t)j * CT_THREADS + threadIdx.x;
+        uint8_t k = q < m ? (status[q] & ST_KIND_MASK) : 0xff;
+        bool ia = k == ST_SLOW, ib = k == ST_FAST;
+        uint64_t ma = __ballot(ia), mb = __ballot(ib);
+        if (lane == 0) {
+            wa[wave] = __popcll(ma);
+            wb[wave] = __popcll(mb);
+        }
+        __syncthreads();
+        uint32_t pa = run_a, pb = run_b;
+        for (unsigned w = 0; w < wave; w++) {
+            pa += wa[w];
+            pb += wb[w];
+        }
+        if (ia) out_a[pa + __popcll(ma & lanemask_lt())] = pass[q];
+        if (ib) out_b[b_base + pb + __popcll(mb & lanemask_lt())] = pass[q];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (unsigned w = 0; w < CT_THREADS / 64; w++) {
+                run_a += wa[w];
+                run_b += wb[w];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_compact_totals(const uint32_t *off_a, const uint32_t *off_b, uint32_t nb,
+                                 unsigned int *counts) {
+    counts[0] = off_a[nb];
+    counts[1] = off_b[nb];
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort of (u64 key, u32 value) by 8-bit digits, stable
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+
+__global__ void k_event_keys(const DevEvent *ev, uint64_t n, unsigned long long *keys, uint32_t *vals,
+                             unsigned long long *minmax) {
+    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    // positive f64 (PositiveF64 total order) sorts like its bit pattern
+    unsigned long long k = bits(ev[p].event_time) & 0x7fffffffffffffffull;
+    keys[p] = k;
+    vals[p] = (uint32_t)p;
+    atomicMin(&minmax[0], k);
+    atomicMax(&minmax[1], k);
+}
+
+__global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const unsigned long long *keys, uint64_t n,
+                                                           int shift, uint32_t *hist, uint32_t nb) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    for (int j = 0; j < RS_ITEMS; j++) {
+        uint64_t p = base + (uint64_t)j * RS_THREADS + threadIdx.x;
+        if (p < n) atomicAdd(&h[(keys[p] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const unsigned long long *kin,
+                                                              const uint32_t *vin,
+                                                              unsigned long long *kout, uint32_t *vout,
+                                                              uint64_t n, int shift,
+                                                              const uint32_t *offsets, uint32_t nb) {
+    constexpr int W = RS_THREADS / 64;
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t wcnt[W][256];
+    const unsigned lane = lane_id();
+    const unsigned wave = threadIdx.x / 64;
+    running[threadIdx.x] = offsets[(uint64_t)threadIdx.x * nb + blockIdx.x];
+    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    for (int j = 0; j < RS_ITEMS; j++) {
+        for (int w = 0; w < W; w++) wcnt[w][threadIdx.x] = 0;
+        __syncthreads();
+        uint64_t p = base + (uint64_t)j * RS_THREADS + threadIdx.x;
+        bool valid = p < n;
+        unsigned long long key = valid ? kin[p] : 0;
+        uint32_t val = valid ? vin[p] : 0;
+        uint32_t digit = (uint32_t)(key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+        for (int bit = 0; bit < 8; bit++) {
+            bool set = (digit >> bit) & 1u;
+            uint64_t mset = __ballot(valid && set);
+            peers &= set ? mset : ~mset;
+        }
+        uint32_t rank = __popcll(peers & lanemask_lt());
+        if (valid && rank == 0) wcnt[wave][digit] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = running[digit] + rank;
+            for (unsigned w = 0; w < wave; w++) pos += wcnt[w][digit];
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < W; w++) add += wcnt[w][threadIdx.x];
+        running[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+// PackedEvent order for speciation events (necsim/core/src/event.rs:256-280):
+// event_time, origin (x, y, index), target (= origin), prior_time, reference
+__device__ __forceinline__ bool event_less(const DevEvent &a, const DevEvent &b) {
+    uint64_t ta = bits(a.event_time) & 0x7fffffffffffffffull, tb = bits(b.event_time) & 0x7fffffffffffffffull;
+    if (ta != tb) return ta < tb;
+    if (a.x != b.x) return a.x < b.x;
+    if (a.y != b.y) return a.y < b.y;
+    if (a.index != b.index) return a.index < b.index;
+    uint64_t pa = bits(a.prior_time) & 0x7fffffffffffffffull, pb = bits(b.prior_time) & 0x7fffffffffffffffull;
+    if (pa != pb) return pa < pb;
+    return a.ref < b.ref;
+}
+
+// Equal event times are rare (only duplicate lineages share them): order
+// each run of equal keys by the full PackedEvent order.
+__global__ void k_fix_ties(const unsigned long long *keys, uint32_t *vals, const DevEvent *ev, uint64_t n) {
+    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p + 1 >= n) return;
+    if (keys[p] != keys[p + 1]) return;
+    if (p > 0 && keys[p - 1] == keys[p]) return; // not the run start
+    uint64_t e = p + 1;
+    while (e < n && keys[e] == keys[p]) e++;
+    for (uint64_t a = p + 1; a < e; a++) {
+        uint32_t v = vals[a];
+        uint64_t c = a;
+        while (c > p && event_less(ev[v], ev[vals[c - 1]])) {
+            vals[c] = vals[c - 1];
+            c--;
+        }
+        vals[c] = v;
+    }
+}
+
+__global__ void k_gather_events(const DevEvent *ev, const uint32_t *vals, DevEvent *out, uint64_t n) {
+    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) out[p] = ev[vals[p]];
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_reset_counters(Counters *c) {
+    c->head = 0;
+    c->n_out = 0;
+}
+__global__ void k_set_minmax(unsigned long long *mm) {
+    mm[0] = ~0ull;
+    mm[1] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+static inline unsigned grid_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream) {
+    switch (kind) {
+    case K_NON_SPATIAL: hipLaunchKernelGGL(k_persistent<K_NON_SPATIAL>, dim3(blocks), dim3(threads), 0, stream, a); break;
+    case K_ALMOST_INFINITE: hipLaunchKernelGGL(k_persistent<K_ALMOST_INFINITE>, dim3(blocks), dim3(threads), 0, stream, a); break;
+    case K_SEPARABLE: hipLaunchKernelGGL(k_persistent<K_SEPARABLE>, dim3(blocks), dim3(threads), 0, stream, a); break;
+    default: hipLaunchKernelGGL(k_persistent<K_PACKED>, dim3(blocks), dim3(threads), 0, stream, a); break;
+    }
+}
+
+int persistent_max_blocks_per_cu(int kind, int threads) {
+    int n = 0;
+    switch (kind) {
+    case K_NON_SPATIAL: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_NON_SPATIAL>, threads, 0); break;
+    case K_ALMOST_INFINITE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_ALMOST_INFINITE>, threads, 0); break;
+    case K_SEPARABLE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_SEPARABLE>, threads, 0); break;
+    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_PACKED>, threads, 0); break;
+    }
+    return n > 0 ? n : 1;
+}
+
+void launch_plain(int kind, const PlainArgs &a, int threads, hipStream_t stream) {
+    unsigned g = grid_for(a.m, threads);
+    if (!g) return;
+    switch (kind) {
+    case K_NON_SPATIAL: hipLaunchKernelGGL(k_plain<K_NON_SPATIAL>, dim3(g), dim3(threads), 0, stream, a); break;
+    case K_ALMOST_INFINITE: hipLaunchKernelGGL(k_plain<K_ALMOST_INFINITE>, dim3(g), dim3(threads), 0, stream, a); break;
+    case K_SEPARABLE: hipLaunchKernelGGL(k_plain<K_SEPARABLE>, dim3(g), dim3(threads), 0, stream, a); break;
+    default: hipLaunchKernelGGL(k_plain<K_PACKED>, dim3(g), dim3(threads), 0, stream, a); break;
+    }
+}
+
+void launch_dedup_push(const uint8_t *status, const unsigned long long *bucket, uint32_t *head,
+                       uint32_t *link, uint64_t m, hipStream_t stream) {
+    if (!m) return;
+    hipLaunchKernelGGL(k_dedup_push, dim3(grid_for(m, 256)), dim3(256), 0, stream, status, bucket, head, link, m);
+}
+
+void launch_dedup_resolve(uint8_t *status, const unsigned long long *bucket, const double *s_u,
+                          const double *s_t, const uint32_t *s_x, const uint32_t *s_y,
+                          const uint32_t *s_i, uint32_t *head, const uint32_t *link,
+                          uint64_t *cache, uint64_t m, Counters *ctr, hipStream_t stream) {
+    if (!m) return;
+    hipLaunchKernelGGL(k_dedup_resolve, dim3(grid_for(m, 256)), dim3(256), 0, stream, status, bucket, s_u,
+                       s_t, s_x, s_y, s_i, head, link, cache, m, ctr);
+}
+
+uint32_t compact_blocks(uint64_t m) { return (uint32_t)((m + CT_TILE - 1) / CT_TILE); }
+
+void launch_compact(const uint8_t *status, const uint32_t *pass, uint64_t m, uint32_t *out_a,
+                    uint32_t *out_b, uint64_t b_base, CompactBuffers &cb, hipStream_t stream) {
+    uint32_t nb = compact_blocks(m);
+    if (nb == 0) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, (uint32_t *)cb.counts, 0u, (uint64_t)2);
+        return;
+    }
+    hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(CT_THREADS), 0, stream, status, m, cb.blk_a, cb.blk_b);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, stream, cb.blk_a, (uint64_t)nb);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, stream, cb.blk_b, (uint64_t)nb);
+    hipLaunchKernelGGL(k_compact_scatter, dim3(nb), dim3(CT_THREADS), 0, stream, status, pass, m, cb.blk_a,
+                       cb.blk_b, out_a, out_b, b_base);
+    hipLaunchKernelGGL(k_compact_totals, dim3(1), dim3(1), 0, stream, cb.blk_a, cb.blk_b, nb, cb.counts);
+}
+
+uint32_t sort_blocks(uint64_t n) { return (uint32_t)((n + RS_TILE - 1) / RS_TILE); }
+
+void sort_events(const DevEvent *events, uint64_t n, DevEvent *out, SortBuffers &sb, hipStream_t stream,
+                 bool fixed_passes) {
+    if (n == 0) return;
+    uint32_t nb = sort_blocks(n);
+    hipLaunchKernelGGL(k_set_minmax, dim3(1), dim3(1), 0, stream, sb.minmax);
+    hipLaunchKernelGGL(k_event_keys, dim3(grid_for(n, 256)), dim3(256), 0, stream, events, n, sb.keys[0],
+                       sb.vals[0], sb.minmax);
+    int first = 0, last = 7;
+    if (!fixed_passes) {
+        unsigned long long mm[2];
+        (void)hipMemcpyAsync(mm, sb.minmax, sizeof(mm), hipMemcpyDeviceToHost, stream);
+        (void)hipStreamSynchronize(stream);
+        unsigned long long diff = mm[0] ^ mm[1];
+        last = -1;
+        for (int d = 7; d >= 0; d--)
+            if ((diff >> (8 * d)) & 255ull) {
+                last = d;
+                break;
+            }
+        // digits above the highest differing bit are equal for all keys
+        // (every key lies between min and max)
+    }
+    int cur = 0;
+    for (int d = first; d <= last; d++) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], n, 8 * d, sb.hist, nb);
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, stream, sb.hist, (uint64_t)256 * nb);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sb.vals[cur],
+                           sb.keys[cur ^ 1], sb.vals[cur ^ 1], n, 8 * d, sb.hist, nb);
+        cur ^= 1;
+    }
+    hipLaunchKernelGGL(k_fix_ties, dim3(grid_for(n, 256)), dim3(256), 0, stream, sb.keys[cur], sb.vals[cur], events, n);
+    hipLaunchKernelGGL(k_gather_events, dim3(grid_for(n, 256)), dim3(256), 0, stream, events, sb.vals[cur], out, n);
+}
+
+void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream) {
+    if (n) hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, stream, p, v, n);
+}
+void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream) {
+    if (n) hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, stream, p, v, n);
+}
+void launch_reset_counters(Counters *c, hipStream_t stream) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, stream, c);
+}
+
+} // namespace nh

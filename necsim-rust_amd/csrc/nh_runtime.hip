@@ -1,0 +1,813 @@
+// nh_runtime.hip -- C ABI of the HIP Independent algorithm: contexts,
+// resident scenario tables and simulation jobs (the water-level loop of
+// necsim/impls/no-std/src/parallelisation/independent/monolithic/mod.rs:43-270
+// driven on the host, every per-lineage step on the device).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/necsim_hip.h"
+#include "nh_internal.h"
+#include "nh_kernels.h"
+
+using namespace nh;
+
+#define NH_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return set_error(NH_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                             __FILE__, __LINE__);                                             \
+    } while (0)
+
+#define NH_TRY(call)             \
+    do {                         \
+        int r_ = (call);         \
+        if (r_ != NH_OK) return r_; \
+    } while (0)
+
+struct nh_tables {
+    nh_ctx *ctx = nullptr;
+    int kind = 0;
+    uint32_t width = 0, height = 0, ox = 0, oy = 0, deme = 1;
+    double sigma = 0.0;
+    std::vector<uint32_t> habitat; // host copy for lineage validation
+    CellRec *d_cells = nullptr;
+    Atom *d_atoms = nullptr;
+    uint64_t bytes = 0;
+    DevScenario dev{};
+};
+
+namespace {
+
+template <class T> int dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess)
+        return set_error(NH_E_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+    return NH_OK;
+}
+template <class T> void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+int kind_of(int32_t scenario_kind) {
+    switch (scenario_kind) {
+    case NH_SCENARIO_NON_SPATIAL: return K_NON_SPATIAL;
+    case NH_SCENARIO_ALMOST_INFINITE_NORMAL: return K_ALMOST_INFINITE;
+    case NH_SCENARIO_SPATIALLY_EXPLICIT: return K_SEPARABLE;
+    case NH_SCENARIO_SPATIALLY_EXPLICIT_PACKED: return K_PACKED;
+    default: return -1;
+    }
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+bool is_pow2_double(double x) {
+    if (!(x > 0.0) || std::isinf(x)) return false;
+    uint64_t b = bits(x);
+    uint64_t exp = (b >> 52) & 0x7ff;
+    return exp != 0 && exp != 0x7ff && (b & 0xfffffffffffffull) == 0;
+}
+
+struct LineageBuffers {
+    LineageSoA s{};
+    uint64_t cap = 0;
+    int alloc(uint64_t n) {
+        cap = n;
+        NH_TRY(dalloc(&s.ref, n));
+        NH_TRY(dalloc(&s.t_last, n));
+        NH_TRY(dalloc(&s.x, n));
+        NH_TRY(dalloc(&s.y, n));
+        NH_TRY(dalloc(&s.idx, n));
+        NH_TRY(dalloc(&s.next_t, n));
+        NH_TRY(dalloc(&s.next_st, n));
+        NH_TRY(dalloc(&s.flags, n));
+        return NH_OK;
+    }
+    void release() {
+        dfree(s.ref); dfree(s.t_last); dfree(s.x); dfree(s.y); dfree(s.idx);
+        dfree(s.next_t); dfree(s.next_st); dfree(s.flags);
+    }
+    static uint64_t bytes_per_lineage() { return 8 + 8 + 4 + 4 + 4 + 8 + 8 + 1; }
+};
+
+// upload host lineages (reference layout) into SoA slots [0, n)
+int upload_lineages(const std::vector<nh_lineage> &v, LineageBuffers &b, hipStream_t st) {
+    size_t n = v.size();
+    if (!n) return NH_OK;
+    std::vector<uint64_t> ref(n), nst(n, 0);
+    std::vector<double> t(n);
+    std::vector<uint32_t> x(n), y(n), idx(n);
+    std::vector<uint8_t> fl(n, 0);
+    for (size_t j = 0; j < n; j++) {
+        ref[j] = v[j].global_reference;
+        t[j] = v[j].last_event_time;
+        x[j] = v[j].x;
+        y[j] = v[j].y;
+        idx[j] = v[j].index;
+    }
+    // the pessimistic next event time of a fresh lineage is its
+    // last_event_time (monolithic/mod.rs:94-103)
+    NH_HIP(hipMemcpyAsync(b.s.ref, ref.data(), n * 8, hipMemcpyHostToDevice, st));
+    NH_HIP(hipMemcpyAsync(b.s.t_last, t.data(), n * 8, hipMemcpyHostToDevice, st));
+    NH_HIP(hipMemcpyAsync(b.s.next_t, t.data(), n * 8, hipMemcpyHostToDevice, st));
+    NH_HIP(hipMemcpyAsync(b.s.next_st, nst.data(), n * 8, hipMemcpyHostToDevice, st));
+    NH_HIP(hipMemcpyAsync(b.s.x, x.data(), n * 4, hipMemcpyHostToDevice, st));
+    NH_HIP(hipMemcpyAsync(b.s.y, y.data(), n * 4, hipMemcpyHostToDevice, st));
+    NH_HIP(hipMemcpyAsync(b.s.idx, idx.data(), n * 4, hipMemcpyHostToDevice, st));
+    NH_HIP(hipMemcpyAsync(b.s.flags, fl.data(), n, hipMemcpyHostToDevice, st));
+    NH_HIP(hipStreamSynchronize(st));
+    return NH_OK;
+}
+
+// download SoA slots (listed by `slots`, or [0, n) if null) into host lineages
+int download_lineages(const LineageBuffers &b, const uint32_t *d_slots, uint64_t n,
+                      std::vector<nh_lineage> &out, hipStream_t st) {
+    out.resize(n);
+    if (!n) return NH_OK;
+    std::vector<uint64_t> ref(b.cap);
+    std::vector<double> t(b.cap);
+    std::vector<uint32_t> x(b.cap), y(b.cap), idx(b.cap), slots(n);
+    uint64_t m = d_slots ? b.cap : n;
+    NH_HIP(hipMemcpyAsync(ref.data(), b.s.ref, m * 8, hipMemcpyDeviceToHost, st));
+    NH_HIP(hipMemcpyAsync(t.data(), b.s.t_last, m * 8, hipMemcpyDeviceToHost, st));
+    NH_HIP(hipMemcpyAsync(x.data(), b.s.x, m * 4, hipMemcpyDeviceToHost, st));
+    NH_HIP(hipMemcpyAsync(y.data(), b.s.y, m * 4, hipMemcpyDeviceToHost, st));
+    NH_HIP(hipMemcpyAsync(idx.data(), b.s.idx, m * 4, hipMemcpyDeviceToHost, st));
+    if (d_slots) NH_HIP(hipMemcpyAsync(slots.data(), d_slots, n * 4, hipMemcpyDeviceToHost, st));
+    NH_HIP(hipStreamSynchronize(st));
+    for (uint64_t j = 0; j < n; j++) {
+        uint64_t s = d_slots ? slots[j] : j;
+        out[j] = nh_lineage{ref[s], t[s], x[s], y[s], idx[s], 0};
+    }
+    return NH_OK;
+}
+
+} // namespace
+
+// ===========================================================================
+// context
+extern "C" int nh_abi_version(void) { return NH_ABI_VERSION; }
+
+extern "C" int nh_last_error(char *buf, size_t cap) {
+    if (!buf || !cap) return NH_E_INVALID;
+    std::snprintf(buf, cap, "%s", nh::g_last_error);
+    return NH_OK;
+}
+
+extern "C" int nh_init(int device, nh_ctx **out) {
+    if (!out) return set_error(NH_E_INVALID, "nh_init: out is null");
+    *out = nullptr;
+    int count = 0;
+    NH_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return set_error(NH_E_INVALID, "device %d of %d", device, count);
+    NH_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    NH_HIP(hipGetDeviceProperties(&prop, device));
+    nh_ctx *c = new nh_ctx();
+    c->device = device;
+    c->cus = (uint32_t)prop.multiProcessorCount;
+    c->name = prop.gcnArchName;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return set_error(NH_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return NH_OK;
+}
+
+extern "C" int nh_ctx_destroy(nh_ctx *ctx) {
+    if (!ctx) return NH_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return NH_OK;
+}
+
+extern "C" int nh_device_info(nh_ctx *ctx, uint32_t *cus, char *name, size_t cap) {
+    if (!ctx) return set_error(NH_E_INVALID, "null context");
+    if (cus) *cus = ctx->cus;
+    if (name && cap) std::snprintf(name, cap, "%s", ctx->name.c_str());
+    return NH_OK;
+}
+
+// ===========================================================================
+// tables
+extern "C" int nh_tables_create(nh_ctx *ctx, const nh_scenario *s, nh_tables **out) {
+    if (!ctx || !s || !out) return set_error(NH_E_INVALID, "nh_tables_create: null argument");
+    *out = nullptr;
+    int kind = kind_of(s->kind);
+    if (kind < 0) return set_error(NH_E_INVALID, "unknown scenario kind %d", s->kind);
+    NH_HIP(hipSetDevice(ctx->device));
+    nh_tables *t = new nh_tables();
+    t->ctx = ctx;
+    t->kind = kind;
+    t->width = s->width;
+    t->height = s->height;
+    t->ox = s->origin_x;
+    t->oy = s->origin_y;
+    t->deme = s->deme;
+    t->sigma = s->sigma;
+    DevScenario &d = t->dev;
+    d.kind = kind;
+    d.width = s->width;
+    d.height = s->height;
+    d.ox = s->origin_x;
+    d.oy = s->origin_y;
+    d.deme = s->deme;
+    d.sigma = s->sigma;
+    d.area = (uint64_t)s->width * s->height;
+    d.area_f = (double)d.area;
+    d.deme_f = (double)s->deme;
+    auto fail = [&](int code) {
+        nh_tables_destroy(t);
+        return code;
+    };
+    if (kind == K_NON_SPATIAL) {
+        if (!s->width || !s->height || !s->deme) return fail(set_error(NH_E_INVALID, "empty non-spatial habitat"));
+        if (d.area > 0xffffffffull) return fail(set_error(NH_E_INVALID, "non-spatial extent above 2^32 cells"));
+    } else if (kind == K_ALMOST_INFINITE) {
+        if (!(s->sigma >= 0.0)) return fail(set_error(NH_E_INVALID, "negative sigma"));
+    } else {
+        uint64_t cells = d.area;
+        if (!cells || cells > 0xffffffffull || !s->habitat || !s->alias_ranges)
+            return fail(set_error(NH_E_INVALID, "in-memory scenario needs habitat and alias ranges"));
+        if (kind == K_SEPARABLE && !s->self_dispersal)
+            return fail(set_error(NH_E_INVALID, "separable scenario needs self-dispersal"));
+        if (s->n_atoms > 0xffffffffull) return fail(set_error(NH_E_INVALID, "more than 2^32 atoms"));
+        t->habitat.assign(s->habitat, s->habitat + cells);
+        std::vector<CellRec> rec(cells);
+        uint64_t prefix = 0;
+        for (uint64_t c = 0; c < cells; c++) {
+            const nh_alias_range &r = s->alias_ranges[c];
+            if (r.end < r.start || r.end > s->n_atoms)
+                return fail(set_error(NH_E_INVALID, "alias range of cell %llu out of bounds", (unsigned long long)c));
+            if (s->habitat[c] > 0 && r.end == r.start &&
+                !(kind == K_SEPARABLE && s->self_dispersal[c].self_dispersal == 1.0))
+                return fail(set_error(NH_E_CONTRACT, "habitable cell %llu without dispersal", (unsigned long long)c));
+            rec[c].prefix = prefix;
+            rec[c].habitat = s->habitat[c];
+            rec[c].atom_start = (uint32_t)r.start;
+            rec[c].atom_len = (uint32_t)(r.end - r.start);
+            if (kind == K_SEPARABLE) {
+                rec[c].self_p = s->self_dispersal[c].self_dispersal;
+                uint64_t nse = s->self_dispersal[c].non_self_dispersal_event;
+                if (nse >= cells) return fail(set_error(NH_E_INVALID, "non-self dispersal event out of bounds"));
+                rec[c].non_self = (uint32_t)nse;
+            } else {
+                rec[c].self_p = 0.0;
+                rec[c].non_self = (uint32_t)c;
+            }
+            prefix += s->habitat[c];
+        }
+        if (prefix == 0) return fail(set_error(NH_E_INVALID, "habitat map is empty"));
+        std::vector<Atom> at(s->n_atoms);
+        for (size_t j = 0; j < s->n_atoms; j++) {
+            if (s->atoms[j].e >= cells || s->atoms[j].k >= cells)
+                return fail(set_error(NH_E_INVALID, "alias event out of bounds"));
+            at[j] = Atom{s->atoms[j].u, (uint32_t)s->atoms[j].e, (uint32_t)s->atoms[j].k};
+        }
+        int r = dalloc(&t->d_cells, cells);
+        if (r) return fail(r);
+        r = dalloc(&t->d_atoms, at.size());
+        if (r) return fail(r);
+        hipError_t e = hipMemcpy(t->d_cells, rec.data(), cells * sizeof(CellRec), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !at.empty())
+            e = hipMemcpy(t->d_atoms, at.data(), at.size() * sizeof(Atom), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(set_error(NH_E_HIP, "table upload: %s", hipGetErrorString(e)));
+        t->bytes = cells * sizeof(CellRec) + at.size() * sizeof(Atom);
+        d.cells = t->d_cells;
+        d.atoms = t->d_atoms;
+    }
+    *out = t;
+    return NH_OK;
+}
+
+extern "C" int nh_tables_destroy(nh_tables *t) {
+    if (!t) return NH_OK;
+    dfree(t->d_cells);
+    dfree(t->d_atoms);
+    delete t;
+    return NH_OK;
+}
+
+extern "C" int nh_tables_footprint(const nh_tables *t, uint64_t *bytes) {
+    if (!t || !bytes) return set_error(NH_E_INVALID, "null argument");
+    *bytes = t->bytes;
+    return NH_OK;
+}
+
+// check a lineage against IndependentActiveLineageSampler's resume filters
+// (active_lineage_sampler/independent/mod.rs:98-117)
+static bool lineage_valid(const nh_tables *t, const nh_lineage &l) {
+    if (t->kind == K_ALMOST_INFINITE) return l.index == 0;
+    uint32_t cx = l.x - t->ox, cy = l.y - t->oy;
+    if (cx >= t->width || cy >= t->height) return false;
+    uint32_t deme = t->kind == K_NON_SPATIAL ? t->deme : t->habitat[(uint64_t)cy * t->width + cx];
+    return l.index < deme;
+}
+
+// ===========================================================================
+// jobs
+struct nh_job {
+    nh_ctx *ctx = nullptr;
+    const nh_tables *tables = nullptr;
+    nh_job_params p{};
+    DevParams P{};
+    double *d_ratio = nullptr;
+    std::vector<nh_lineage> pending;
+    bool started = false, finished = false;
+    uint64_t n_initial = 0;
+    uint64_t event_slice = 1;
+    uint64_t cache_cap = 0;
+    double level_time = 0.0;
+    double max_time = 0.0;
+    uint64_t rounds = 0, launches = 0;
+    uint64_t steps = 0;
+
+    LineageBuffers lin[2];
+    int cur = 0;
+    uint64_t cur_n = 0;
+    Counters *d_ctr = nullptr;
+    RoundDesc *d_round = nullptr;
+    DevEvent *d_events = nullptr, *d_sorted = nullptr;
+    uint64_t n_events = 0;
+    std::vector<nh_speciation_event> ready;
+    size_t ready_pos = 0;
+    SortBuffers sb{};
+    uint64_t *d_cache = nullptr;
+
+    // plain form
+    uint32_t *d_S = nullptr, *d_F = nullptr, *d_pass[2] = {nullptr, nullptr};
+    uint8_t *d_status = nullptr;
+    double *s_u = nullptr, *s_t = nullptr;
+    uint32_t *s_x = nullptr, *s_y = nullptr, *s_i = nullptr;
+    unsigned long long *s_bucket = nullptr;
+    uint32_t *d_head = nullptr, *d_link = nullptr;
+    CompactBuffers cb{};
+
+    // persistent form
+    int threads = 256;
+    int blocks = 0;
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double kernel_ms = 0.0;
+    uint64_t timed = 0;
+    RoundDesc round_host{};
+
+    ~nh_job() {
+        for (auto &l : lin) l.release();
+        dfree(d_ratio); dfree(d_ctr); dfree(d_round); dfree(d_events); dfree(d_sorted);
+        dfree(sb.keys[0]); dfree(sb.keys[1]); dfree(sb.vals[0]); dfree(sb.vals[1]);
+        dfree(sb.hist); dfree(sb.minmax); dfree(d_cache);
+        dfree(d_S); dfree(d_F); dfree(d_pass[0]); dfree(d_pass[1]); dfree(d_status);
+        dfree(s_u); dfree(s_t); dfree(s_x); dfree(s_y); dfree(s_i); dfree(s_bucket);
+        dfree(d_head); dfree(d_link); dfree(cb.blk_a); dfree(cb.blk_b); dfree(cb.counts);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
+};
+
+extern "C" int nh_job_create(nh_ctx *ctx, const nh_tables *tables, const nh_job_params *p, nh_job **out) {
+    if (!ctx || !tables || !p || !out) return set_error(NH_E_INVALID, "nh_job_create: null argument");
+    *out = nullptr;
+    if (!(p->speciation_probability > 0.0 && p->speciation_probability <= 1.0))
+        return set_error(NH_E_INVALID, "speciation probability must lie in (0, 1]");
+    if (!(p->turnover_rate > 0.0) || std::isinf(p->turnover_rate))
+        return set_error(NH_E_INVALID, "turnover rate must be positive");
+    if (!(p->delta_t > 0.0) || std::isinf(p->delta_t)) return set_error(NH_E_INVALID, "delta_t must be positive");
+    if (p->step_slice == 0) return set_error(NH_E_INVALID, "step_slice must be positive");
+    if (p->form != NH_FORM_PLAIN && p->form != NH_FORM_PERSISTENT) return set_error(NH_E_INVALID, "unknown form");
+    if (p->dedup_mode < NH_DEDUP_NONE || p->dedup_mode > NH_DEDUP_ABSOLUTE) return set_error(NH_E_INVALID, "dedup mode");
+    if (p->dedup_mode == NH_DEDUP_RELATIVE && !(p->dedup_factor > 0.0)) return set_error(NH_E_INVALID, "dedup factor");
+    if (p->event_slice_mode == NH_SLICE_RELATIVE && !(p->event_slice_factor > 0.0))
+        return set_error(NH_E_INVALID, "event slice factor");
+    if (p->event_slice_mode == NH_SLICE_ABSOLUTE && p->event_slice_capacity == 0)
+        return set_error(NH_E_INVALID, "event slice capacity");
+    if (p->event_slice_mode != NH_SLICE_RELATIVE && p->event_slice_mode != NH_SLICE_ABSOLUTE)
+        return set_error(NH_E_INVALID, "event slice mode");
+    if (!(p->resume_after >= 0.0)) return set_error(NH_E_INVALID, "resume_after must be non-negative");
+    if (p->block_size && (p->block_size % 64 || p->block_size > 256))
+        return set_error(NH_E_INVALID, "block size must be 64, 128, 192 or 256");
+    NH_HIP(hipSetDevice(ctx->device));
+    nh_job *j = new nh_job();
+    j->ctx = ctx;
+    j->tables = tables;
+    j->p = *p;
+    *out = j;
+    return NH_OK;
+}
+
+extern "C" int nh_job_add_lineages(nh_job *j, const nh_lineage *l, size_t n) {
+    if (!j || (!l && n)) return set_error(NH_E_INVALID, "nh_job_add_lineages: null argument");
+    if (j->started) return set_error(NH_E_STATE, "lineages must be added before the first round");
+    for (size_t k = 0; k < n; k++) {
+        if (!lineage_valid(j->tables, l[k]))
+            return set_error(NH_E_OUT_OF_DEME, "lineage %zu at (%u, %u, %u) is outside the habitat or its deme", k,
+                             l[k].x, l[k].y, l[k].index);
+        if (!(l[k].last_event_time >= 0.0)) return set_error(NH_E_INVALID, "negative last_event_time");
+    }
+    j->pending.insert(j->pending.end(), l, l + n);
+    return NH_OK;
+}
+
+static int job_start(nh_job *j) {
+    hipStream_t st = j->ctx->stream;
+    const nh_job_params &p = j->p;
+    uint64_t n = j->pending.size();
+    j->n_initial = n;
+    j->started = true;
+
+    // DevParams: the Poisson constants are evaluated once, on the host, with
+    // the same MathsCore as the device (poisson.rs:39-42)
+    DevParams &P = j->P;
+    P.seed = p.rng_seed;
+    P.nu = p.speciation_probability;
+    P.lambda_dt = p.turnover_rate * p.delta_t;
+    P.p0 = nh::exp(-P.lambda_dt);
+    P.lambda_dt_sqrt = sqrt_(P.lambda_dt);
+    P.delta_t = p.delta_t;
+    P.dt_pow2 = is_pow2_double(p.delta_t);
+    P.inv_delta_t = P.dt_pow2 ? 1.0 / p.delta_t : 0.0; // exact for powers of two
+    P.min_event_time = p.resume_after;
+    P.step_slice = p.step_slice;
+    std::vector<double> ratio(RATIO_N, 0.0);
+    for (int k = 1; k < RATIO_N; k++) ratio[k] = P.lambda_dt / (double)k;
+    NH_TRY(dalloc(&j->d_ratio, RATIO_N));
+    NH_HIP(hipMemcpy(j->d_ratio, ratio.data(), RATIO_N * sizeof(double), hipMemcpyHostToDevice));
+    P.ratio = j->d_ratio;
+
+    // EventSlice::capacity and DedupCache::construct (parallelisation/independent/mod.rs:34-82)
+    if (p.event_slice_mode == NH_SLICE_ABSOLUTE) j->event_slice = p.event_slice_capacity;
+    else j->event_slice = std::max<uint64_t>(as_u64((double)n * p.event_slice_factor), 1);
+    if (p.dedup_mode == NH_DEDUP_RELATIVE) j->cache_cap = as_u64((double)n * p.dedup_factor);
+    else if (p.dedup_mode == NH_DEDUP_ABSOLUTE) j->cache_cap = p.dedup_capacity;
+    else j->cache_cap = 0;
+
+    // max_time / level_time start at the max / min last_event_time (monolithic/mod.rs:120-133)
+    if (n) {
+        double mx = j->pending[0].last_event_time, mn = mx;
+        for (auto &l : j->pending) {
+            mx = l.last_event_time > mx ? l.last_event_time : mx;
+            mn = l.last_event_time < mn ? l.last_event_time : mn;
+        }
+        j->max_time = mx;
+        j->level_time = mn;
+    }
+
+    NH_TRY(dalloc(&j->d_ctr, 1));
+    NH_TRY(dalloc(&j->d_round, 1));
+    NH_HIP(hipMemsetAsync(j->d_ctr, 0, sizeof(Counters), st));
+    NH_TRY(dalloc(&j->d_events, n));
+    NH_TRY(dalloc(&j->d_sorted, n));
+    j->sb.capacity = n;
+    NH_TRY(dalloc(&j->sb.keys[0], n));
+    NH_TRY(dalloc(&j->sb.keys[1], n));
+    NH_TRY(dalloc(&j->sb.vals[0], n));
+    NH_TRY(dalloc(&j->sb.vals[1], n));
+    NH_TRY(dalloc(&j->sb.hist, (size_t)256 * sort_blocks(n) + 1));
+    NH_TRY(dalloc(&j->sb.minmax, 2));
+    if (j->cache_cap) {
+        NH_TRY(dalloc(&j->d_cache, 4 * j->cache_cap));
+        launch_fill_u64(j->d_cache, 0, 4 * j->cache_cap, st);
+    }
+    NH_HIP(hipEventCreate(&j->ev0));
+    NH_HIP(hipEventCreate(&j->ev1));
+
+    NH_TRY(j->lin[0].alloc(n));
+    NH_TRY(upload_lineages(j->pending, j->lin[0], st));
+    j->cur = 0;
+    j->cur_n = n;
+    j->threads = p.block_size ? (int)p.block_size : 256;
+    if (p.form == NH_FORM_PERSISTENT) {
+        NH_TRY(j->lin[1].alloc(n));
+        int per_cu = persistent_max_blocks_per_cu(j->tables->kind, j->threads);
+        if (p.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)p.blocks_per_cu);
+        j->blocks = per_cu * (int)j->ctx->cus;
+    } else {
+        NH_TRY(dalloc(&j->d_S, n));
+        NH_TRY(dalloc(&j->d_F, n));
+        NH_TRY(dalloc(&j->d_pass[0], n));
+        NH_TRY(dalloc(&j->d_pass[1], n));
+        NH_TRY(dalloc(&j->d_status, n));
+        std::vector<uint32_t> iota(n);
+        for (uint64_t k = 0; k < n; k++) iota[k] = (uint32_t)k;
+        if (n) NH_HIP(hipMemcpy(j->d_S, iota.data(), n * 4, hipMemcpyHostToDevice));
+        uint32_t nb = compact_blocks(n);
+        NH_TRY(dalloc(&j->cb.blk_a, nb + 1));
+        NH_TRY(dalloc(&j->cb.blk_b, nb + 1));
+        NH_TRY(dalloc(&j->cb.counts, 2));
+        if (j->cache_cap) {
+            NH_TRY(dalloc(&j->s_u, n));
+            NH_TRY(dalloc(&j->s_t, n));
+            NH_TRY(dalloc(&j->s_x, n));
+            NH_TRY(dalloc(&j->s_y, n));
+            NH_TRY(dalloc(&j->s_i, n));
+            NH_TRY(dalloc(&j->s_bucket, n));
+            NH_TRY(dalloc(&j->d_link, n));
+            NH_TRY(dalloc(&j->d_head, j->cache_cap));
+            launch_fill_u32(j->d_head, 0xffffffffu, j->cache_cap, st);
+        }
+    }
+    j->pending.clear();
+    j->pending.shrink_to_fit();
+    NH_HIP(hipStreamSynchronize(st));
+    return NH_OK;
+}
+
+static int read_counters(nh_job *j, Counters &c) {
+    NH_HIP(hipMemcpyAsync(&c, j->d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, j->ctx->stream));
+    NH_HIP(hipStreamSynchronize(j->ctx->stream));
+    return NH_OK;
+}
+
+static void account_kernel_time(nh_job *j) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, j->ev0, j->ev1) == hipSuccess) {
+        j->kernel_ms += ms;
+        j->timed++;
+    }
+}
+
+// Sequential replay of one pass's dedup inserts on the host, for buckets whose
+// insert list exceeded the device limit (tiny caches only).
+static int host_dedup_replay(nh_job *j, uint64_t m) {
+    hipStream_t st = j->ctx->stream;
+    std::vector<uint8_t> status(m);
+    std::vector<double> u(m), t(m);
+    std::vector<uint32_t> x(m), y(m), i(m);
+    std::vector<unsigned long long> b(m);
+    std::vector<uint64_t> cache(4 * j->cache_cap);
+    NH_HIP(hipMemcpyAsync(status.data(), j->d_status, m, hipMemcpyDeviceToHost, st));
+    NH_HIP(hipMemcpyAsync(u.data(), j->s_u, m * 8, hipMemcpyDeviceToHost, st));
+    NH_HIP(hipMemcpyAsync(t.data(), j->s_t, m * 8, hipMemcpyDeviceToHost, st));
+    NH_HIP(hipMemcpyAsync(x.data(), j->s_x, m * 4, hipMemcpyDeviceToHost, st));
+    NH_HIP(hipMemcpyAsync(y.data(), j->s_y, m * 4, hipMemcpyDeviceToHost, st));
+    NH_HIP(hipMemcpyAsync(i.data(), j->s_i, m * 4, hipMemcpyDeviceToHost, st));
+    NH_HIP(hipMemcpyAsync(b.data(), j->s_bucket, m * 8, hipMemcpyDeviceToHost, st));
+    NH_HIP(hipMemcpyAsync(cache.data(), j->d_cache, cache.size() * 8, hipMemcpyDeviceToHost, st));
+    NH_HIP(hipStreamSynchronize(st));
+    // the device resolved some buckets already; their dup flags are final, but
+    // overflowed buckets were left untouched: replay only those (head != NONE)
+    std::vector<uint32_t> head(j->cache_cap);
+    NH_HIP(hipMemcpy(head.data(), j->d_head, j->cache_cap * 4, hipMemcpyDeviceToHost));
+    for (uint64_t q = 0; q < m; q++) {
+        if (!(status[q] & ST_HAS_SAMPLE)) continue;
+        unsigned long long bk = b[q];
+        if (head[bk] == 0xffffffffu) continue; // resolved on the device
+        Sample s{u[q], t[q], x[q], y[q], i[q]};
+        uint64_t *cb = &cache[4 * bk];
+        uint64_t n0 = bits(s.t), n1 = bits(s.u), n2 = ((uint64_t)s.y << 32) | s.x;
+        uint64_t n3 = ((uint64_t)sample_check(s) << 32) | s.i;
+        if (cb[0] == n0 && cb[1] == n1 && cb[2] == n2 && cb[3] == n3)
+            status[q] = (uint8_t)((status[q] & ~ST_KIND_MASK) | ST_DUP);
+        cb[0] = n0; cb[1] = n1; cb[2] = n2; cb[3] = n3;
+    }
+    NH_HIP(hipMemcpyAsync(j->d_status, status.data(), m, hipMemcpyHostToDevice, st));
+    NH_HIP(hipMemcpyAsync(j->d_cache, cache.data(), cache.size() * 8, hipMemcpyHostToDevice, st));
+    launch_fill_u32(j->d_head, 0xffffffffu, j->cache_cap, st);
+    NH_HIP(hipMemsetAsync(&j->d_ctr->overflow, 0, 8, st));
+    NH_HIP(hipStreamSynchronize(st));
+    return NH_OK;
+}
+
+// one water-level round of the plain form: one launch per step slice over the
+// reference's FIFO queue order (monolithic/mod.rs:183-251)
+static int plain_round(nh_job *j) {
+    hipStream_t st = j->ctx->stream;
+    uint64_t m = j->cur_n;
+    const uint32_t *plist = j->d_S;
+    uint64_t f_len = 0;
+    int pb = 0;
+    for (uint64_t pass = 1; m > 0; pass++) {
+        PlainArgs a{};
+        a.S = j->tables->dev;
+        a.P = j->P;
+        a.L = j->lin[0].s;
+        a.pass = plist;
+        a.m = m;
+        a.round = j->d_round;
+        a.status = j->d_status;
+        a.s_u = j->s_u; a.s_t = j->s_t; a.s_x = j->s_x; a.s_y = j->s_y; a.s_i = j->s_i;
+        a.s_bucket = j->s_bucket;
+        a.cache_cap = j->cache_cap;
+        a.events = j->d_events;
+        a.ctr = j->d_ctr;
+        NH_HIP(hipEventRecord(j->ev0, st));
+        launch_plain(j->tables->kind, a, j->threads, st);
+        NH_HIP(hipGetLastError());
+        NH_HIP(hipEventRecord(j->ev1, st));
+        j->launches++;
+        if (j->cache_cap) {
+            launch_dedup_push(j->d_status, j->s_bucket, j->d_head, j->d_link, m, st);
+            launch_dedup_resolve(j->d_status, j->s_bucket, j->s_u, j->s_t, j->s_x, j->s_y, j->s_i, j->d_head,
+                                 j->d_link, j->d_cache, m, j->d_ctr, st);
+            NH_HIP(hipGetLastError());
+            Counters c;
+            NH_TRY(read_counters(j, c));
+            if (c.overflow) NH_TRY(host_dedup_replay(j, m));
+        }
+        uint32_t *next = j->d_pass[pb];
+        uint64_t next_m;
+        if (pass == 1) {
+            // The first pass pops every lineage of the round: skipped ones go to
+            // the fast queue at once, a processed one only when the next
+            // processed lineage is popped (monolithic/mod.rs:186-219).
+            std::vector<uint8_t> status(m);
+            std::vector<uint32_t> slots(m), nextv, fv;
+            NH_HIP(hipMemcpyAsync(status.data(), j->d_status, m, hipMemcpyDeviceToHost, st));
+            NH_HIP(hipMemcpyAsync(slots.data(), plist, m * 4, hipMemcpyDeviceToHost, st));
+            NH_HIP(hipStreamSynchronize(st));
+            nextv.reserve(m);
+            fv.reserve(m);
+            int64_t pending = -1;
+            for (uint64_t q = 0; q < m; q++) {
+                uint8_t k = status[q] & ST_KIND_MASK;
+                if (k == ST_SKIP) {
+                    fv.push_back(slots[q]);
+                    continue;
+                }
+                if (pending >= 0 && (status[pending] & ST_KIND_MASK) == ST_FAST) fv.push_back(slots[pending]);
+                pending = (int64_t)q;
+                if (k == ST_SLOW) nextv.push_back(slots[q]);
+            }
+            if (pending >= 0 && (status[pending] & ST_KIND_MASK) == ST_FAST) fv.push_back(slots[pending]);
+            if (!nextv.empty())
+                NH_HIP(hipMemcpyAsync(next, nextv.data(), nextv.size() * 4, hipMemcpyHostToDevice, st));
+            if (!fv.empty()) NH_HIP(hipMemcpyAsync(j->d_F, fv.data(), fv.size() * 4, hipMemcpyHostToDevice, st));
+            NH_HIP(hipStreamSynchronize(st));
+            next_m = nextv.size();
+            f_len = fv.size();
+        } else {
+            launch_compact(j->d_status, plist, m, next, j->d_F, f_len, j->cb, st);
+            NH_HIP(hipGetLastError());
+            unsigned int counts[2];
+            NH_HIP(hipMemcpyAsync(counts, j->cb.counts, sizeof(counts), hipMemcpyDeviceToHost, st));
+            NH_HIP(hipStreamSynchronize(st));
+            next_m = counts[0];
+            f_len += counts[1];
+        }
+        account_kernel_time(j);
+        plist = next;
+        pb ^= 1;
+        m = next_m;
+    }
+    std::swap(j->d_S, j->d_F);
+    j->cur_n = f_len;
+    return NH_OK;
+}
+
+static int persistent_round(nh_job *j) {
+    hipStream_t st = j->ctx->stream;
+    PersistArgs a{};
+    a.S = j->tables->dev;
+    a.P = j->P;
+    a.in = j->lin[j->cur].s;
+    a.out = j->lin[j->cur ^ 1].s;
+    a.round = j->d_round;
+    a.cache = j->d_cache;
+    a.cache_cap = j->cache_cap;
+    a.events = j->d_events;
+    a.ctr = j->d_ctr;
+    a.tick_steps = 0;
+    launch_reset_counters(j->d_ctr, st);
+    NH_HIP(hipEventRecord(j->ev0, st));
+    launch_persistent(j->tables->kind, a, j->blocks, j->threads, st);
+    NH_HIP(hipGetLastError());
+    NH_HIP(hipEventRecord(j->ev1, st));
+    j->launches++;
+    Counters c;
+    NH_TRY(read_counters(j, c));
+    account_kernel_time(j);
+    j->cur ^= 1;
+    j->cur_n = c.n_out;
+    return NH_OK;
+}
+
+extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
+    if (!j) return set_error(NH_E_INVALID, "null job");
+    NH_HIP(hipSetDevice(j->ctx->device));
+    if (!j->started) NH_TRY(job_start(j));
+    const nh_job_params &p = j->p;
+    auto done = [&]() { return j->cur_n == 0 || (p.has_pause_before && !(j->level_time < p.pause_before)); };
+    if (!j->finished && done()) j->finished = true;
+    if (j->finished) {
+        if (finished) *finished = 1;
+        return NH_OK;
+    }
+    hipStream_t st = j->ctx->stream;
+    // speciation-only reporting: total rate = sum of lambda * nu over the slow
+    // lineages, folded in order (monolithic/mod.rs:150-164)
+    double total_rate = repeated_sum(p.turnover_rate * p.speciation_probability, j->cur_n);
+    j->level_time = j->level_time + (double)j->event_slice / total_rate;
+    if (p.has_pause_before && p.pause_before < j->level_time) j->level_time = p.pause_before;
+    j->round_host = RoundDesc{j->level_time, (unsigned long long)j->cur_n};
+    NH_HIP(hipMemcpyAsync(j->d_round, &j->round_host, sizeof(RoundDesc), hipMemcpyHostToDevice, st));
+
+    uint64_t ev_start = j->n_events;
+    if (p.form == NH_FORM_PLAIN) NH_TRY(plain_round(j));
+    else NH_TRY(persistent_round(j));
+
+    Counters c;
+    NH_TRY(read_counters(j, c));
+    j->n_events = c.n_events;
+    j->steps = c.steps;
+    double mt = from_bits(c.max_time_bits);
+    if (c.max_time_bits && mt > j->max_time) j->max_time = mt;
+    // every round ends with an empty simulate call whose time is
+    // max(0, min_event_time) (monolithic/mod.rs:198-242)
+    if (p.resume_after > j->max_time) j->max_time = p.resume_after;
+    j->rounds++;
+
+    // the live water-level reporter sorts each round's events (live.rs:88-115)
+    uint64_t ne = j->n_events - ev_start;
+    if (ne) {
+        sort_events(j->d_events + ev_start, ne, j->d_sorted, j->sb, st, false);
+        NH_HIP(hipGetLastError());
+        size_t old = j->ready.size();
+        j->ready.resize(old + ne);
+        NH_HIP(hipMemcpyAsync(j->ready.data() + old, j->d_sorted, ne * sizeof(DevEvent), hipMemcpyDeviceToHost, st));
+        NH_HIP(hipStreamSynchronize(st));
+    }
+    if (done()) j->finished = true;
+    if (finished) *finished = j->finished ? 1 : 0;
+    return NH_OK;
+}
+
+extern "C" int nh_job_simulate(nh_job *j) {
+    int32_t fin = 0;
+    while (!fin) NH_TRY(nh_job_step_round(j, &fin));
+    return NH_OK;
+}
+
+extern "C" int nh_job_drain_events(nh_job *j, nh_speciation_event *buf, size_t cap, size_t *n_out) {
+    if (!j || !n_out) return set_error(NH_E_INVALID, "null argument");
+    size_t avail = j->ready.size() - j->ready_pos;
+    if (!buf) {
+        *n_out = avail;
+        return NH_OK;
+    }
+    size_t n = std::min(avail, cap);
+    std::memcpy(buf, j->ready.data() + j->ready_pos, n * sizeof(nh_speciation_event));
+    j->ready_pos += n;
+    if (j->ready_pos == j->ready.size()) {
+        j->ready.clear();
+        j->ready_pos = 0;
+    }
+    *n_out = n;
+    return NH_OK;
+}
+
+extern "C" int nh_job_outcome(const nh_job *j, nh_outcome *o) {
+    if (!j || !o) return set_error(NH_E_INVALID, "null argument");
+    o->paused = j->started ? (j->cur_n > 0) : (j->pending.size() > 0);
+    o->time = j->max_time;
+    o->steps = j->steps;
+    o->n_events = j->n_events;
+    o->n_remaining = j->started ? j->cur_n : j->pending.size();
+    o->rounds = j->rounds;
+    o->launches = j->launches;
+    return NH_OK;
+}
+
+extern "C" int nh_job_take_lineages(nh_job *j, nh_lineage *buf, size_t cap, size_t *n_out) {
+    if (!j || !n_out) return set_error(NH_E_INVALID, "null argument");
+    if (!j->started) {
+        *n_out = j->pending.size();
+        if (buf) std::memcpy(buf, j->pending.data(), std::min(cap, j->pending.size()) * sizeof(nh_lineage));
+        return NH_OK;
+    }
+    *n_out = j->cur_n;
+    if (!buf) return NH_OK;
+    if (cap < j->cur_n) return set_error(NH_E_CAPACITY, "lineage buffer holds %zu of %llu", cap, (unsigned long long)j->cur_n);
+    NH_HIP(hipSetDevice(j->ctx->device));
+    std::vector<nh_lineage> v;
+    if (j->p.form == NH_FORM_PLAIN) NH_TRY(download_lineages(j->lin[0], j->d_S, j->cur_n, v, j->ctx->stream));
+    else NH_TRY(download_lineages(j->lin[j->cur], nullptr, j->cur_n, v, j->ctx->stream));
+    std::memcpy(buf, v.data(), v.size() * sizeof(nh_lineage));
+    return NH_OK;
+}
+
+extern "C" int nh_job_destroy(nh_job *j) {
+    if (!j) return NH_OK;
+    (void)hipSetDevice(j->ctx->device);
+    (void)hipStreamSynchronize(j->ctx->stream);
+    delete j;
+    return NH_OK;
+}
+
+extern "C" int nh_job_kernel_time(const nh_job *j, double *total_ms, uint64_t *launches) {
+    if (!j) return set_error(NH_E_INVALID, "null job");
+    if (total_ms) *total_ms = j->kernel_ms;
+    if (launches) *launches = j->timed;
+    return NH_OK;
+}

@@ -1,0 +1,18 @@
+"""MI355X (gfx950) Independent coalescence algorithm for necsim-rust scenarios.
+
+The simulation runs in libnecsim_hip.so (hand-written HIP kernels behind the C
+ABI of include/necsim_hip.h); this package is the Python host mirror of
+rustcoalescence's algorithm plugin interface and the synthetic workload
+generators used by the tests and the benchmark.
+"""
+from . import _ffi
+from ._ffi import EVENT_DTYPE, LINEAGE_DTYPE, NecsimHipError, lib
+from .algorithm import (PERSISTENT, PLAIN, AlmostInfiniteNormalDispersalScenario,
+                        BiodiversityReporter, Context, DedupCache, EventCollector, EventSlice,
+                        HipIndependentAlgorithm, IndependentArguments, Job, NonSpatialScenario,
+                        OriginPreSampler, ScenarioCogs, SimulationOutcome,
+                        SpatiallyExplicitUniformTurnoverScenario, Tables, WyHash,
+                        build_alias_tables)
+from .maps import generate_map
+
+__all__ = [n for n in dir() if not n.startswith("_")]

@@ -1,0 +1,407 @@
+"""Host-side mirror of rustcoalescence's algorithm plugin interface for the
+HIP Independent algorithm.
+
+The names follow the reference so that a caller reads the same flow as the
+Rust CLI:
+
+    rng = WyHash.seed_from_u64(42)                       # rng/mod.rs:60
+    cogs = NonSpatialScenario((100, 100), 1).new(1e-3)   # Scenario::new
+    outcome = HipIndependentAlgorithm.initialise_and_simulate(
+        IndependentArguments(), rng, cogs, OriginPreSampler.all(), None, reporter)
+
+rustcoalescence/algorithms/src/lib.rs:55-112 (Algorithm),
+rustcoalescence/algorithms/independent/src/arguments.rs:79-146 (arguments),
+rustcoalescence/algorithms/src/result.rs:11-26 (SimulationOutcome).
+All simulation work runs in libnecsim_hip.so on the GPU.
+"""
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import check, lib, ptr
+
+
+# ---------------------------------------------------------------------------
+# RNG
+@dataclass(frozen=True)
+class WyHash:
+    """WyHash PrimeableRng (necsim/impls/no-std/src/cogs/rng/wyhash.rs:19-24).
+
+    Only the seed matters: every event re-primes the state."""
+    seed: int
+
+    @staticmethod
+    def seed_from_u64(seed):
+        return WyHash(int(lib().nh_seed_from_u64(seed)))
+
+
+# ---------------------------------------------------------------------------
+# arguments (parallelisation/independent/mod.rs:15-82)
+@dataclass(frozen=True)
+class DedupCache:
+    mode: int
+    factor: float = 0.0
+    capacity: int = 0
+
+    @staticmethod
+    def Relative(factor=1.0):
+        return DedupCache(_ffi.DEDUP_RELATIVE, factor=factor)
+
+    @staticmethod
+    def Absolute(capacity):
+        return DedupCache(_ffi.DEDUP_ABSOLUTE, capacity=capacity)
+
+    @staticmethod
+    def none():
+        return DedupCache(_ffi.DEDUP_NONE)
+
+
+@dataclass(frozen=True)
+class EventSlice:
+    mode: int
+    factor: float = 0.0
+    capacity: int = 0
+
+    @staticmethod
+    def Relative(factor=2.0):
+        return EventSlice(_ffi.SLICE_RELATIVE, factor=factor)
+
+    @staticmethod
+    def Absolute(capacity):
+        return EventSlice(_ffi.SLICE_ABSOLUTE, capacity=capacity)
+
+
+PLAIN = _ffi.FORM_PLAIN
+PERSISTENT = _ffi.FORM_PERSISTENT
+
+
+@dataclass(frozen=True)
+class IndependentArguments:
+    """Independent(delta_t, step_slice, dedup_cache, parallelism_mode: Monolithic(event_slice))
+    with the reference defaults, plus the device form and launch knobs."""
+    delta_t: float = 2.0
+    step_slice: int = 10
+    dedup_cache: DedupCache = field(default_factory=lambda: DedupCache.Relative(1.0))
+    event_slice: EventSlice = field(default_factory=lambda: EventSlice.Relative(2.0))
+    form: int = PERSISTENT
+    block_size: int = 0
+    blocks_per_cu: int = 0
+    use_graph: bool = False
+    device: int = 0
+
+
+# ---------------------------------------------------------------------------
+# origin pre-sampler (origin_sampler/pre_sampler.rs:50-114)
+@dataclass(frozen=True)
+class OriginPreSampler:
+    proportion: float = 1.0
+    rank: int = 0
+    size: int = 1
+
+    @staticmethod
+    def all():
+        return OriginPreSampler()
+
+    def percentage(self, p):
+        return OriginPreSampler(p, self.rank, self.size)
+
+    def partition(self, rank, size):
+        return OriginPreSampler(self.proportion, rank, size)
+
+
+# ---------------------------------------------------------------------------
+# scenarios (rustcoalescence/scenarios/src)
+class ScenarioCogs:
+    """Habitat + dispersal sampler + uniform turnover / speciation."""
+
+    def __init__(self, desc, speciation_probability, turnover_rate=0.5, keep=(), circle=None):
+        self.desc = desc
+        self.speciation_probability = float(speciation_probability)
+        self.turnover_rate = float(turnover_rate)
+        self._keep = list(keep)
+        self.circle = circle
+
+    def sample_habitat(self, pre_sampler: OriginPreSampler):
+        cx, cy, r = self.circle if self.circle else (0, 0, 0)
+        n = ctypes.c_size_t(0)
+        args = (ctypes.byref(self.desc), pre_sampler.proportion, pre_sampler.rank, pre_sampler.size,
+                cx, cy, r)
+        check(lib().nh_sample_origins(*args, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, _ffi.LINEAGE_DTYPE)
+        check(lib().nh_sample_origins(*args, ptr(out), n.value, ctypes.byref(n)))
+        return out
+
+
+@dataclass
+class NonSpatialScenario:
+    """NonSpatial(area, deme) (rustcoalescence/scenarios/src/non_spatial.rs:39-89)"""
+    area: tuple
+    deme: int
+
+    def new(self, speciation_probability):
+        d = _ffi.Scenario(kind=_ffi.SCENARIO_NON_SPATIAL, width=self.area[0], height=self.area[1],
+                          deme=self.deme)
+        return ScenarioCogs(d, speciation_probability)
+
+
+CIRCLE_CENTRE = 2**32 // 2 - 1  # Sample::default_circle_sample_centre (u32::MAX / 2)
+
+
+@dataclass
+class AlmostInfiniteNormalDispersalScenario:
+    """AlmostInfinite(sample: Circle(centre, radius), dispersal: Normal(sigma))
+    (rustcoalescence/scenarios/src/almost_infinite/normal.rs:43-94, mod.rs:90-123)"""
+    radius: int
+    sigma: float
+    centre: tuple = (CIRCLE_CENTRE, CIRCLE_CENTRE)
+
+    def new(self, speciation_probability):
+        d = _ffi.Scenario(kind=_ffi.SCENARIO_ALMOST_INFINITE_NORMAL, sigma=self.sigma)
+        return ScenarioCogs(d, speciation_probability,
+                            circle=(self.centre[0], self.centre[1], self.radius))
+
+
+def build_alias_tables(width, height, habitat, row_ptr, col, weight, separable=True):
+    """InMemoryPacked(Separable)AliasDispersalSampler::new from a sparse dispersal map."""
+    habitat = np.ascontiguousarray(habitat, dtype=np.uint32).ravel()
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    weight = np.ascontiguousarray(weight, dtype=np.float64)
+    cells = width * height
+    ranges = np.zeros(cells, _ffi.RANGE_DTYPE)
+    selfd = np.zeros(cells, _ffi.SELF_DTYPE)
+    n = ctypes.c_size_t(0)
+    args = (width, height, ptr(habitat), ptr(row_ptr), ptr(col), ptr(weight), 1 if separable else 0,
+            ptr(ranges), ptr(selfd))
+    check(lib().nh_build_alias_tables(*args, None, 0, ctypes.byref(n)))
+    atoms = np.zeros(max(n.value, 1), _ffi.ATOM_DTYPE)
+    check(lib().nh_build_alias_tables(*args, ptr(atoms), n.value, ctypes.byref(n)))
+    return ranges, (selfd if separable else None), atoms[:n.value]
+
+
+@dataclass
+class SpatiallyExplicitUniformTurnoverScenario:
+    """SpatiallyExplicit(turnover: Uniform, mode: Strict) on an in-memory habitat
+    map with a sparse dispersal map (rows in ascending column order)
+    (rustcoalescence/scenarios/src/spatially_explicit/turnover/uniform.rs:49-117)."""
+    width: int
+    height: int
+    habitat: np.ndarray
+    row_ptr: np.ndarray
+    col: np.ndarray
+    weight: np.ndarray
+    turnover_rate: float = 0.5
+    packed: bool = False  # InMemoryPackedAliasDispersalSampler instead of the separable one
+    _tables: tuple = None
+
+    def tables(self):
+        if self._tables is None:
+            self._tables = build_alias_tables(self.width, self.height, self.habitat, self.row_ptr,
+                                              self.col, self.weight, separable=not self.packed)
+        return self._tables
+
+    def new(self, speciation_probability):
+        ranges, selfd, atoms = self.tables()
+        habitat = np.ascontiguousarray(self.habitat, dtype=np.uint32).ravel()
+        kind = (_ffi.SCENARIO_SPATIALLY_EXPLICIT_PACKED if self.packed
+                else _ffi.SCENARIO_SPATIALLY_EXPLICIT)
+        d = _ffi.Scenario(kind=kind, width=self.width, height=self.height, habitat=ptr(habitat),
+                          alias_ranges=ptr(ranges), self_dispersal=ptr(selfd), atoms=ptr(atoms),
+                          n_atoms=len(atoms))
+        return ScenarioCogs(d, speciation_probability, self.turnover_rate,
+                            keep=(habitat, ranges, selfd, atoms))
+
+
+# ---------------------------------------------------------------------------
+# outcome and reporters
+@dataclass
+class SimulationOutcome:
+    """SimulationOutcome::{Done, Paused} (rustcoalescence/algorithms/src/result.rs:13-26)"""
+    paused: bool
+    time: float
+    steps: int
+    lineages: Optional[np.ndarray] = None
+    rng: Optional[WyHash] = None
+    rounds: int = 0
+    launches: int = 0
+    kernel_ms: float = 0.0
+
+
+class BiodiversityReporter:
+    """necsim/plugins/common/src/biodiversity.rs:32-41: counts speciation
+    events, skipping one equal (same origin and event time) to the previous."""
+
+    def __init__(self):
+        self.biodiversity = 0
+        self._last = None
+
+    def report_speciation(self, events):
+        for e in events:
+            key = (int(e["x"]), int(e["y"]), int(e["index"]), float(e["event_time"]))
+            if key == self._last:
+                continue
+            self._last = key
+            self.biodiversity += 1
+
+
+class EventCollector:
+    """Keeps the reported (sorted) speciation stream."""
+
+    def __init__(self):
+        self.chunks = []
+
+    def report_speciation(self, events):
+        self.chunks.append(np.array(events, copy=True))
+
+    @property
+    def events(self):
+        if not self.chunks:
+            return np.zeros(0, _ffi.EVENT_DTYPE)
+        return np.concatenate(self.chunks)
+
+
+class Context:
+    """One device context (nh_init / nh_ctx_destroy)."""
+    _cache = {}
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().nh_init(device, ctypes.byref(h)))
+        self.handle = h
+
+    @classmethod
+    def get(cls, device=0):
+        if device not in cls._cache:
+            cls._cache[device] = Context(device)
+        return cls._cache[device]
+
+    def info(self):
+        cus = ctypes.c_uint32(0)
+        name = ctypes.create_string_buffer(128)
+        check(lib().nh_device_info(self.handle, ctypes.byref(cus), name, 128))
+        return cus.value, name.value.decode()
+
+
+class Tables:
+    """Resident device tables of one scenario (nh_tables_create)."""
+
+    def __init__(self, ctx, cogs):
+        h = ctypes.c_void_p()
+        check(lib().nh_tables_create(ctx.handle, ctypes.byref(cogs.desc), ctypes.byref(h)))
+        self.handle = h
+        self.cogs = cogs
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib().nh_tables_destroy(self.handle)
+            self.handle = None
+
+
+class Job:
+    """One simulation on the device (nh_job_*)."""
+
+    def __init__(self, ctx, tables, args: IndependentArguments, rng: WyHash, speciation_probability,
+                 turnover_rate, pause_before=None, resume_after=0.0):
+        p = _ffi.JobParams(rng_seed=rng.seed, speciation_probability=speciation_probability,
+                           turnover_rate=turnover_rate, delta_t=args.delta_t,
+                           step_slice=args.step_slice, dedup_mode=args.dedup_cache.mode,
+                           dedup_factor=args.dedup_cache.factor,
+                           dedup_capacity=args.dedup_cache.capacity,
+                           event_slice_mode=args.event_slice.mode,
+                           event_slice_factor=args.event_slice.factor,
+                           event_slice_capacity=args.event_slice.capacity,
+                           has_pause_before=0 if pause_before is None else 1,
+                           pause_before=0.0 if pause_before is None else pause_before,
+                           resume_after=resume_after, form=args.form, block_size=args.block_size,
+                           blocks_per_cu=args.blocks_per_cu, use_graph=1 if args.use_graph else 0)
+        h = ctypes.c_void_p()
+        check(lib().nh_job_create(ctx.handle, tables.handle, ctypes.byref(p), ctypes.byref(h)))
+        self.handle = h
+        self.rng = rng
+
+    def add_lineages(self, lineages):
+        lineages = np.ascontiguousarray(lineages, dtype=_ffi.LINEAGE_DTYPE)
+        check(lib().nh_job_add_lineages(self.handle, ptr(lineages), len(lineages)))
+
+    def step_round(self):
+        fin = ctypes.c_int32(0)
+        check(lib().nh_job_step_round(self.handle, ctypes.byref(fin)))
+        return bool(fin.value)
+
+    def drain_events(self):
+        n = ctypes.c_size_t(0)
+        check(lib().nh_job_drain_events(self.handle, None, 0, ctypes.byref(n)))
+        buf = np.zeros(n.value, _ffi.EVENT_DTYPE)
+        if n.value:
+            check(lib().nh_job_drain_events(self.handle, ptr(buf), n.value, ctypes.byref(n)))
+        return buf[:n.value]
+
+    def outcome(self):
+        o = _ffi.Outcome()
+        check(lib().nh_job_outcome(self.handle, ctypes.byref(o)))
+        return o
+
+    def take_lineages(self):
+        n = ctypes.c_size_t(0)
+        check(lib().nh_job_take_lineages(self.handle, None, 0, ctypes.byref(n)))
+        buf = np.zeros(n.value, _ffi.LINEAGE_DTYPE)
+        if n.value:
+            check(lib().nh_job_take_lineages(self.handle, ptr(buf), n.value, ctypes.byref(n)))
+        return buf
+
+    def kernel_time(self):
+        ms = ctypes.c_double(0)
+        k = ctypes.c_uint64(0)
+        check(lib().nh_job_kernel_time(self.handle, ctypes.byref(ms), ctypes.byref(k)))
+        return ms.value, k.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().nh_job_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+class HipIndependentAlgorithm:
+    """Algorithm::initialise_and_simulate / resume_and_simulate of the HIP
+    Independent algorithm (Monolithic parallelism mode)."""
+
+    @staticmethod
+    def _run(args, rng, cogs, lineages, pause_before, resume_after, reporter):
+        ctx = Context.get(args.device)
+        tables = Tables(ctx, cogs)
+        job = Job(ctx, tables, args, rng, cogs.speciation_probability, cogs.turnover_rate,
+                  pause_before, resume_after)
+        try:
+            job.add_lineages(lineages)
+            finished = False
+            while not finished:
+                finished = job.step_round()
+                ev = job.drain_events()
+                if len(ev) and reporter is not None:
+                    reporter.report_speciation(ev)
+            o = job.outcome()
+            ms, _ = job.kernel_time()
+            rem = job.take_lineages() if o.paused else None
+            return SimulationOutcome(paused=bool(o.paused), time=o.time, steps=int(o.steps),
+                                     lineages=rem, rng=rng if o.paused else None,
+                                     rounds=int(o.rounds), launches=int(o.launches), kernel_ms=ms)
+        finally:
+            job.close()
+
+    @classmethod
+    def initialise_and_simulate(cls, args: IndependentArguments, rng: WyHash, cogs: ScenarioCogs,
+                                pre_sampler: OriginPreSampler, pause_before=None, reporter=None):
+        lineages = cogs.sample_habitat(pre_sampler)
+        return cls._run(args, rng, cogs, lineages, pause_before, 0.0, reporter)
+
+    @classmethod
+    def resume_and_simulate(cls, args: IndependentArguments, rng: WyHash, cogs: ScenarioCogs,
+                            lineages, resume_after=None, pause_before=None, reporter=None):
+        return cls._run(args, rng, cogs, lineages, pause_before,
+                        0.0 if resume_after is None else resume_after, reporter)

@@ -1,0 +1,279 @@
+"""GPU parity: the HIP Independent algorithm against the CPU oracle.
+
+Every test runs the product through the C ABI (libnecsim_hip.so) on cuda:0 /
+the first HIP device and compares with the oracle on the same seeded inputs.
+"""
+import ctypes
+import struct
+
+import numpy as np
+import pytest
+
+import necsim_hip as nh
+import oracle as O
+from necsim_hip import _ffi
+from parity_utils import (assert_exact, assert_invariants, hip_run, map_scenarios, oracle_run,
+                          unique_keys)
+
+pytestmark = pytest.mark.gpu
+PLAIN, PERSISTENT = nh.PLAIN, nh.PERSISTENT
+
+
+def announce(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def device_eval(ctx, op, words, n):
+    words = np.ascontiguousarray(words, dtype=np.uint64)
+    out = np.zeros(n, np.uint64)
+    _ffi.check(_ffi.lib().nh_debug_eval(ctx.handle, op, _ffi.ptr(words), _ffi.ptr(out), n))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# device MathsCore and RNG primitives, bit-exact against the oracle
+def _inputs(rng, kind, n):
+    if kind == "unit_open":  # Box-Muller u0 in (0, 1]
+        return ((rng.integers(0, 2**53, n, dtype=np.uint64) + 1).astype(np.float64) * 2.0**-53)
+    if kind == "angle":  # -TAU * u1
+        return -6.283185307179586 * (rng.integers(0, 2**53, n, dtype=np.uint64).astype(np.float64) * 2.0**-53)
+    if kind == "wide":
+        return np.exp(rng.uniform(-700, 700, n))
+    if kind == "exp":
+        return rng.uniform(-740, 705, n)
+    if kind == "sqrt":
+        return -2.0 * np.log(((rng.integers(0, 2**53, n, dtype=np.uint64) + 1).astype(np.float64) * 2.0**-53))
+    if kind == "round":
+        return rng.normal(0, 50, n) * rng.choice([1.0, 1e-3, 1e3], n)
+    raise ValueError(kind)
+
+
+SPECIALS = [1.0, 2.0, 0.5, 2.0**-53, 1.0 - 2.0**-53, np.pi / 2, -np.pi / 2, -np.pi, -3 * np.pi / 2,
+            -2 * np.pi, -np.pi / 4, -6.283185307179586, 1e-300, 5e-324, 0.0, -0.0]
+
+
+@pytest.mark.parametrize("op,name,kind", [(0, "orc_ln", "unit_open"), (0, "orc_ln", "wide"),
+                                          (1, "orc_exp", "exp"), (2, "orc_sin", "angle"),
+                                          (3, "orc_cos", "angle"), (4, "orc_sqrt", "sqrt"),
+                                          (5, "orc_floor", "round"), (6, "orc_round", "round")])
+def test_device_maths_core(ctx, op, name, kind):
+    announce(f"device maths op {op} {name} {kind}")
+    rng = np.random.default_rng(op * 7 + len(kind))
+    x = np.concatenate([_inputs(rng, kind, 200_000), np.array(SPECIALS)])
+    if op == 0:
+        x = x[x > 0]
+    got = device_eval(ctx, op, x.view(np.uint64), len(x))
+    f = getattr(O.lib(), name)
+    want = np.array([f(float(v)) for v in x]).view(np.uint64)
+    bad = np.nonzero(got != want)[0]
+    assert len(bad) == 0, f"{name}: {len(bad)} mismatches, e.g. x={x[bad[:3]]}"
+
+
+def test_device_division_correctly_rounded(ctx):
+    announce("device f64 division")
+    rng = np.random.default_rng(3)
+    a = np.concatenate([rng.uniform(-2, 2, 200_000), rng.normal(0, 1, 1000) * 1e300])
+    b = np.concatenate([2.0 + rng.uniform(-0.3, 0.5, 200_000), rng.normal(0, 1, 1000) * 1e-300])
+    pairs = np.stack([a, b], axis=1).ravel()
+    got = device_eval(ctx, 7, pairs.view(np.uint64), len(a))
+    want = (a / b).view(np.uint64)  # numpy f64 division is IEEE correctly rounded
+    assert np.array_equal(got, want)
+
+
+def test_device_rng(ctx):
+    announce("device wymum / seahash / prime / uniform")
+    rng = np.random.default_rng(11)
+    a = rng.integers(0, 2**63, 50_000, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, 50_000, dtype=np.uint64)
+    b = rng.integers(0, 2**63, 50_000, dtype=np.uint64) * np.uint64(3)
+    got = device_eval(ctx, 8, np.stack([a, b], 1).ravel(), len(a))
+    want = np.array([O.lib().orc_wymum(int(x), int(y)) for x, y in zip(a, b)], dtype=np.uint64)
+    assert np.array_equal(got, want)
+    got = device_eval(ctx, 9, a, len(a))
+    want = np.array([O.lib().orc_seahash_diffuse(int(x)) for x in a], dtype=np.uint64)
+    assert np.array_equal(got, want)
+    triples = np.stack([a, b, a ^ b], 1)
+    got = device_eval(ctx, 10, triples.ravel(), len(a))
+    want = np.array([O.lib().orc_prime(int(s), int(l), int(t)) for s, l, t in triples], dtype=np.uint64)
+    assert np.array_equal(got, want)
+    got = device_eval(ctx, 11, triples.ravel(), len(a))
+    st = ctypes.c_uint64()
+    want = []
+    for s, l, t in triples:
+        st.value = O.lib().orc_prime(int(s), int(l), int(t))
+        want.append(O.lib().orc_uniform_closed_open(ctypes.byref(st)))
+    assert np.array_equal(got, np.array(want).view(np.uint64))
+
+
+# ---------------------------------------------------------------------------
+# config 1: NonSpatial 100x100, deme 1, nu 1e-3, seed 42, 100% sampled
+@pytest.fixture(scope="module")
+def config1():
+    cogs = nh.NonSpatialScenario((100, 100), 1).new(1e-3)
+    orc = O.Scenario.non_spatial(100, 100, 1)
+    lin = orc.origins(1.0)
+    return cogs, orc, lin, {d: oracle_run(orc, lin, 42, 1e-3, dedup=d) for d in ("none", 1.0)}
+
+
+@pytest.mark.parametrize("form", [PLAIN, PERSISTENT])
+def test_config1_dedup_none_bit_exact(config1, form):
+    announce(f"config1 dedup None form {form}")
+    cogs, orc, lin, ref = config1
+    out, ev = hip_run(cogs, lin, 42, form, dedup="none")
+    assert_exact(ref["none"], out, ev, "config1 none")
+    assert O.biodiversity(ev) == 67
+
+
+def test_config1_dedup_plain_bit_exact(config1):
+    announce("config1 dedup Relative(1.0) plain form")
+    cogs, orc, lin, ref = config1
+    out, ev = hip_run(cogs, lin, 42, PLAIN, dedup=1.0)
+    assert_exact(ref[1.0], out, ev, "config1 dedup plain")
+
+
+def test_config1_dedup_persistent_invariants(config1):
+    announce("config1 dedup Relative(1.0) persistent form")
+    cogs, orc, lin, ref = config1
+    out, ev = hip_run(cogs, lin, 42, PERSISTENT, dedup=1.0)
+    assert_invariants(ev, ref["none"]["events"], ref[1.0]["events"], "config1 dedup persistent")
+    assert out.steps <= ref["none"]["steps"]
+
+
+# ---------------------------------------------------------------------------
+# config 2: almost-infinite Gaussian sigma=10, circle r=100, nu 1e-4, seed 7
+@pytest.fixture(scope="module")
+def config2():
+    cogs = nh.AlmostInfiniteNormalDispersalScenario(100, 10.0).new(1e-4)
+    orc = O.Scenario.almost_infinite_normal(10.0)
+    lin = orc.origins(1.0, circle=(2**31 - 1, 2**31 - 1, 100))
+    assert len(lin) == 31417
+    return cogs, orc, lin
+
+
+@pytest.mark.parametrize("form", [PLAIN, PERSISTENT])
+def test_config2_window_dedup_none(config2, form):
+    announce(f"config2 pause 40 dedup None form {form}")
+    cogs, orc, lin = config2
+    ref = oracle_run(orc, lin, 7, 1e-4, dedup="none", pause_before=40.0)
+    out, ev = hip_run(cogs, lin, 7, form, dedup="none", pause_before=40.0)
+    assert_exact(ref, out, ev, "config2 window")
+    rem = np.sort(out.lineages, order=["global_reference"])
+    orem = np.sort(ref["remaining"], order=["global_reference"])
+    assert rem.tobytes() == orem.tobytes()
+
+
+def test_config2_window_dedup_plain_exact(config2):
+    announce("config2 pause 40 dedup plain")
+    cogs, orc, lin = config2
+    ref = oracle_run(orc, lin, 7, 1e-4, dedup=1.0, pause_before=40.0)
+    out, ev = hip_run(cogs, lin, 7, PLAIN, dedup=1.0, pause_before=40.0)
+    assert_exact(ref, out, ev, "config2 dedup plain")
+    # the plain form keeps the reference's queue order of remaining lineages
+    assert out.lineages.tobytes() == ref["remaining"].tobytes()
+
+
+# ---------------------------------------------------------------------------
+# spatially explicit maps (separable and packed alias)
+@pytest.mark.parametrize("packed", [False, True])
+def test_small_map_full_run(packed):
+    announce(f"8x8 map packed={packed}")
+    hip, orc = map_scenarios(8, 8, seed=5, packed=packed, sigma=1.5, cutoff=2.5)
+    cogs = hip.new(0.01)
+    lin = orc.origins(1.0)
+    for form in (PLAIN, PERSISTENT):
+        ref = oracle_run(orc, lin, 3, 0.01, dedup="none")
+        out, ev = hip_run(cogs, lin, 3, form, dedup="none")
+        assert_exact(ref, out, ev, f"8x8 form {form}")
+    ref = oracle_run(orc, lin, 3, 0.01, dedup=1.0)
+    out, ev = hip_run(cogs, lin, 3, PLAIN, dedup=1.0)
+    assert_exact(ref, out, ev, "8x8 dedup plain")
+
+
+@pytest.fixture(scope="module")
+def map64():
+    hip, orc = map_scenarios(64, 64, seed=42)
+    return hip.new(1e-3), orc, orc.origins(1.0)
+
+
+@pytest.mark.parametrize("form", [PLAIN, PERSISTENT])
+def test_map64_window(map64, form):
+    announce(f"64x64 map pause 30 form {form}")
+    cogs, orc, lin = map64
+    ref = oracle_run(orc, lin, 9, 1e-3, dedup="none", pause_before=30.0)
+    out, ev = hip_run(cogs, lin, 9, form, dedup="none", pause_before=30.0)
+    assert_exact(ref, out, ev, "map64 window")
+    refd = oracle_run(orc, lin, 9, 1e-3, dedup=1.0, pause_before=30.0)
+    outd, evd = hip_run(cogs, lin, 9, form, dedup=1.0, pause_before=30.0)
+    if form == PLAIN:
+        assert_exact(refd, outd, evd, "map64 dedup plain")
+    else:
+        assert_invariants(evd, ref["events"], refd["events"], "map64 dedup persistent")
+
+
+# ---------------------------------------------------------------------------
+# pause / resume continuity (docs/simulate.ron:45-54)
+@pytest.mark.parametrize("form", [PLAIN, PERSISTENT])
+def test_pause_resume_continuity(config1, form):
+    announce(f"pause/resume config1 form {form}")
+    cogs, orc, lin, ref = config1
+    out1, ev1 = hip_run(cogs, lin, 42, form, dedup="none", pause_before=500.0)
+    assert out1.paused and len(out1.lineages) > 0
+    out2, ev2 = hip_run(cogs, out1.lineages, 42, form, dedup="none", resume_after=500.0)
+    both = np.sort(np.concatenate([ev1, ev2]),
+                   order=["event_time", "x", "y", "index", "prior_time", "global_reference"])
+    want = ref["none"]["events"]
+    assert both.tobytes() == want.tobytes()
+    assert out1.steps + out2.steps == ref["none"]["steps"]
+
+
+# ---------------------------------------------------------------------------
+# edge cases
+def test_edge_empty_and_single(config1):
+    announce("edge: empty and single lineage")
+    cogs, orc, lin, ref = config1
+    for form in (PLAIN, PERSISTENT):
+        out, ev = hip_run(cogs, lin[:0], 42, form)
+        assert out.steps == 0 and len(ev) == 0 and not out.paused
+        one = lin[17:18]
+        r1 = oracle_run(orc, one, 42, 1e-3, dedup=1.0)
+        out, ev = hip_run(cogs, one, 42, form, dedup=1.0)
+        assert_exact(r1, out, ev, "single lineage")
+
+
+def test_edge_certain_speciation():
+    announce("edge: nu = 1")
+    cogs = nh.NonSpatialScenario((10, 10), 4).new(1.0)
+    orc = O.Scenario.non_spatial(10, 10, 4)
+    lin = orc.origins(1.0)
+    ref = oracle_run(orc, lin, 1, 1.0, dedup=1.0)
+    for form in (PLAIN, PERSISTENT):
+        out, ev = hip_run(cogs, lin, 1, form, dedup=1.0)
+        assert_exact(ref, out, ev, "nu=1")
+
+
+def test_edge_tiny_absolute_cache_host_replay():
+    announce("edge: Absolute(3) dedup cache (bucket lists beyond the device limit)")
+    cogs = nh.NonSpatialScenario((30, 30), 2).new(1e-2)
+    orc = O.Scenario.non_spatial(30, 30, 2)
+    lin = orc.origins(1.0)
+    ref = oracle_run(orc, lin, 5, 1e-2, dedup=("absolute", 3))
+    out, ev = hip_run(cogs, lin, 5, PLAIN, dedup=("absolute", 3))
+    assert_exact(ref, out, ev, "absolute(3)")
+
+
+def test_edge_out_of_deme_rejected(config1):
+    announce("edge: lineage outside its deme")
+    cogs, orc, lin, ref = config1
+    bad = lin[:4].copy()
+    bad[2]["index"] = 5
+    with pytest.raises(nh.NecsimHipError) as e:
+        hip_run(cogs, bad, 42, PLAIN)
+    assert e.value.code == -6
+
+
+def test_edge_general_delta_t_and_step_slice(config1):
+    announce("edge: delta_t 3.0 (division path), step_slice 1")
+    cogs, orc, lin, ref = config1
+    sub = lin[::7]
+    r = oracle_run(orc, sub, 42, 1e-3, dedup=1.0, delta_t=3.0, step_slice=1)
+    out, ev = hip_run(cogs, sub, 42, PLAIN, dedup=1.0, delta_t=3.0, step_slice=1)
+    assert_exact(r, out, ev, "delta_t 3, step_slice 1")

@@ -1,0 +1,328 @@
+"""Pins of the CPU oracle (the restatement of the reference's CPU Independent
+algorithm) against published vectors, closed forms, brute force and
+invariants.  CPU only."""
+import ctypes
+import math
+import struct
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+L = O.lib()
+
+
+def u64(state):
+    return ctypes.c_uint64(state)
+
+
+# ---------------------------------------------------------------------------
+# RNG: published known answers
+def test_seahash_diffuse_published_vectors():
+    # seahash 4.1.0 helper::diffuse unit test vectors (the function restated at
+    # necsim/impls/no-std/src/cogs/rng/wyhash.rs:99-119)
+    for x, want in [(94203824938, 17289265692384716055), (0xDEADBEEF, 12110756357096144265), (0, 0),
+                    (1, 15197155197312260123), (2, 1571904453004118546), (3, 16467633989910088880)]:
+        assert L.orc_seahash_diffuse(x) == want
+
+
+def test_seed_from_u64_value_breakage():
+    # rand_core's SeedableRng::seed_from_u64 value-breakage test: seed 0 fills
+    # the 8-byte seed to 5029875928683246316 (necsim/core/src/cogs/rng.rs:33-63)
+    assert L.orc_seed_from_u64(0) == 5029875928683246316
+
+
+def test_fnv1a_published_vectors():
+    # FNV-1a 64 test vectors (crate fnv 1.0.7, necsim/impls/no-std/src/cache.rs:4)
+    for s, want in [(b"", 0xcbf29ce484222325), (b"a", 0xaf63dc4c8601ec8c), (b"foobar", 0x85944171f73967e8)]:
+        assert L.orc_fnv1a64(s, len(s)) == want
+
+
+def test_speciation_sample_hash_byte_order():
+    # Hash stream of SpeciationSample: x, y, index (u32 LE), time bits, sample bits (u64 LE)
+    u, t, x, y, i = 0.25, 17.5, 3, 4, 5
+    b = struct.pack("<IIIQQ", x, y, i, struct.unpack("<Q", struct.pack("<d", t))[0],
+                    struct.unpack("<Q", struct.pack("<d", u))[0])
+    assert L.orc_speciation_sample_hash(u, t, x, y, i) == L.orc_fnv1a64(b, len(b))
+
+
+def test_wymum_is_xor_fold_of_128_bit_product():
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        a, b = (int(v) for v in rng.integers(0, 2**63, 2, dtype=np.uint64) * 2 + 1)
+        r = a * b
+        want = (a ^ (r & (2**64 - 1))) ^ (b ^ (r >> 64))
+        assert L.orc_wymum(a, b) == want
+
+
+def test_uniform_closed_forms():
+    # U[0,1) = (x >> 11) * 2^-53 and U(0,1] = ((x >> 11) + 1) * 2^-53 (rng.rs:75-93)
+    for s in [0, 1, 2**64 - 1, 0x123456789abcdef0]:
+        st = u64(s)
+        st2 = u64(s)
+        x = L.orc_sample_u64(ctypes.byref(st2))
+        assert L.orc_uniform_closed_open(ctypes.byref(st)) == (x >> 11) * 2.0**-53
+        st = u64(s)
+        assert L.orc_uniform_open_closed(ctypes.byref(st)) == ((x >> 11) + 1) * 2.0**-53
+
+
+def test_uniform_statistics():
+    # mean, variance and low-bit balance of U[0,1) over primed streams
+    vals = []
+    for k in range(20000):
+        st = u64(L.orc_prime(12345, k, k * 7))
+        vals.append(L.orc_uniform_closed_open(ctypes.byref(st)))
+    v = np.array(vals)
+    assert abs(v.mean() - 0.5) < 0.01 and abs(v.var() - 1 / 12) < 0.005
+    assert v.min() >= 0.0 and v.max() < 1.0
+
+
+# ---------------------------------------------------------------------------
+# MathsCore (crate libm 0.2.8): within 1 ulp of the correctly rounded value
+getcontext().prec = 60
+
+
+def _ulps(a, b):
+    ia = struct.unpack("<q", struct.pack("<d", a))[0]
+    ib = struct.unpack("<q", struct.pack("<d", b))[0]
+    return abs(ia - ib)
+
+
+def _round_decimal(d):
+    return float(d)  # Decimal -> nearest double
+
+
+def _dec_sin_cos(x):
+    x = Decimal(x)
+    s, c = Decimal(0), Decimal(0)
+    term = x
+    n = 1
+    while True:  # sin
+        s += term
+        term = -term * x * x / ((n + 1) * (n + 2))
+        n += 2
+        if abs(term) < Decimal(10) ** -55:
+            break
+    term = Decimal(1)
+    n = 0
+    while True:  # cos
+        c += term
+        term = -term * x * x / ((n + 1) * (n + 2))
+        n += 2
+        if abs(term) < Decimal(10) ** -55:
+            break
+    return s, c
+
+
+def test_ln_exp_sin_cos_within_one_ulp_of_exact():
+    rng = np.random.default_rng(7)
+    xs = ((rng.integers(0, 2**53, 600, dtype=np.uint64) + 1).astype(np.float64) * 2.0**-53)
+    for x in xs:
+        assert _ulps(L.orc_ln(float(x)), _round_decimal(Decimal(float(x)).ln())) <= 1
+    for x in rng.uniform(-30, 30, 300):
+        assert _ulps(L.orc_exp(float(x)), _round_decimal(Decimal(float(x)).exp())) <= 1
+    for x in -6.283185307179586 * rng.random(300):
+        s, c = _dec_sin_cos(float(x))
+        assert _ulps(L.orc_sin(float(x)), _round_decimal(s)) <= 1
+        assert _ulps(L.orc_cos(float(x)), _round_decimal(c)) <= 1
+
+
+def test_maths_special_values():
+    assert L.orc_ln(1.0) == 0.0
+    assert struct.pack("<d", L.orc_exp(-1.0)) == struct.pack("<Q", 0x3FD78B56362CEF38)  # e^-1 rounded
+    assert L.orc_exp(0.0) == 1.0
+    assert L.orc_sin(0.0) == 0.0 and L.orc_cos(0.0) == 1.0
+    for k in range(-60, 60):
+        # log(2^k) = k ln2_hi + k ln2_lo exactly as the e_log.c reduction evaluates it
+        assert abs(L.orc_ln(2.0**k) - k * math.log(2)) <= abs(k) * 2e-16 + 1e-300
+    assert math.isinf(L.orc_ln(0.0)) and L.orc_ln(0.0) < 0
+    assert math.isnan(L.orc_ln(-1.0))
+    assert L.orc_round(2.5) == 3.0 and L.orc_round(-2.5) == -3.0 and L.orc_round(0.49999999999999994) == 0.0
+
+
+def test_maths_agree_with_glibc_almost_everywhere():
+    rng = np.random.default_rng(9)
+    for f, ref, xs in [(L.orc_ln, math.log, rng.random(20000) + 1e-9),
+                       (L.orc_sin, math.sin, -6.283185307179586 * rng.random(20000)),
+                       (L.orc_cos, math.cos, -6.283185307179586 * rng.random(20000))]:
+        d = [_ulps(f(float(x)), ref(float(x))) for x in xs]
+        assert max(d) <= 1 and sum(1 for v in d if v) < 0.15 * len(d)
+
+
+# ---------------------------------------------------------------------------
+# alias method (necsim/impls/no-std/src/alias/packed.rs)
+def _alias(weights):
+    n = len(weights)
+    ev = np.arange(n, dtype=np.uint64) * 3 + 1
+    w = np.asarray(weights, dtype=np.float64)
+    u = np.zeros(n)
+    e = np.zeros(n, np.uint64)
+    k = np.zeros(n, np.uint64)
+    L.orc_alias_create(ev.ctypes.data, w.ctypes.data, n, u.ctypes.data, e.ctypes.data, k.ctypes.data)
+    return ev, w, u, e, k
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_alias_tables_reproduce_weights(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 9))
+    weights = rng.random(n) * rng.choice([1.0, 1e-3, 7.0], n)
+    ev, w, u, e, k = _alias(weights)
+    assert list(e) == list(ev)  # stores exactly the input events (packed.rs:48-51)
+    assert all((uu != 1.0) or (ee == kk) for uu, ee, kk in zip(u, e, k))  # full buckets (:52-58)
+    # exact probability mass: bucket i contributes u_i/n to e_i and (1-u_i)/n to k_i
+    mass = {int(x): 0.0 for x in ev}
+    for uu, ee, kk in zip(u, e, k):
+        mass[int(ee)] += uu / n
+        mass[int(kk)] += (1 - uu) / n
+    for x, ww in zip(ev, w):
+        assert abs(mass[int(x)] - ww / w.sum()) < 1e-12
+    # sampling on a fine grid of U realises the same masses
+    grid = 4096 * n
+    counts = {int(x): 0 for x in ev}
+    for j in range(grid):
+        counts[int(L.orc_alias_sample(u.ctypes.data, e.ctypes.data, k.ctypes.data, n, 1.0, j / grid))] += 1
+    for x, ww in zip(ev, w):
+        assert abs(counts[int(x)] / grid - ww / w.sum()) <= 1.0 / 4096 + 1e-12
+
+
+def test_separable_alias_self_dispersal_branches():
+    # an 8x8 map exercises both separable branches (self included / excluded)
+    import necsim_hip as nh
+    h, rp, c, w = nh.generate_map(8, 8, seed=5, sigma=1.5, cutoff=2.5)
+    s = O.Scenario.spatially_explicit(8, 8, h, rp, c, w)
+    t = s.tables()
+    included = excluded = 0
+    for cell in range(64):
+        if h[cell] == 0:
+            continue
+        a, b = int(t["range_start"][cell]), int(t["range_end"][cell])
+        if t["self_event"][cell] != cell:
+            included += 1
+            # self-dispersal sits in the last atom's k slot (mod.rs:150-168)
+            assert t["atom_k"][b - 1] == cell and t["atom_e"][b - 1] != cell
+        else:
+            excluded += 1
+            assert cell not in set(t["atom_e"][a:b]) | set(t["atom_k"][a:b])
+        # empirical self-dispersal frequency matches self_p
+        row = slice(int(rp[cell]), int(rp[cell + 1]))
+        wt = w[row] * h[c[row]]
+        p_self = wt[c[row] == cell].sum() / wt.sum()
+        assert abs(t["self_p"][cell] - p_self) < 1e-12
+        hits = 0
+        trials = 4000
+        for j in range(trials):
+            st = u64(L.orc_prime(99, cell, j))
+            tx, ty = ctypes.c_uint32(), ctypes.c_uint32()
+            L.orc_sample_dispersal(s.handle, cell % 8, cell // 8, ctypes.byref(st), ctypes.byref(tx), ctypes.byref(ty))
+            hits += (ty.value * 8 + tx.value) == cell
+        assert abs(hits / trials - p_self) < 4 * math.sqrt(p_self * (1 - p_self) / trials) + 1e-9
+    assert included > 0 and excluded > 0
+
+
+# ---------------------------------------------------------------------------
+# Poisson event-time sampler (event_time_sampler/poisson.rs:30-109)
+def test_poisson_sampler_contract_and_rate():
+    gaps = []
+    for k in range(20000):
+        t = (k % 97) * 0.731
+        st = ctypes.c_uint64()
+        nxt = L.orc_poisson_next_event_time(77, k * 13 + 5, t, 0.5, 2.0, ctypes.byref(st))
+        assert nxt > t  # strictly after (sampler.rs:100 then requires > before)
+        assert math.floor(nxt / 2.0) >= math.floor(t / 2.0)
+        gaps.append(nxt - t)
+    # memoryless Poisson process with rate lambda = 0.5: mean gap 1/lambda
+    assert abs(np.mean(gaps) - 2.0) < 0.06
+
+
+def test_poisson_sampler_is_a_pure_function_of_location_and_time():
+    a = L.orc_poisson_next_event_time(5, 42, 10.0, 0.5, 2.0, None)
+    b = L.orc_poisson_next_event_time(5, 42, 10.0, 0.5, 2.0, None)
+    c = L.orc_poisson_next_event_time(5, 42, a - 1e-9, 0.5, 2.0, None)
+    assert a == b == c  # the next event after any time in [10, a) is a
+
+
+# ---------------------------------------------------------------------------
+# origin samplers / landscape extent order (necsim/core/src/landscape/extent.rs:77-100)
+def test_circle_origins_wrap_around_extent():
+    s = O.Scenario.almost_infinite_normal(1.0)
+    out = s.origins(1.0, circle=(0, 1, 2))
+    # brute force: the 5x5 box around (0, 1) row-major with wrapping coords
+    want = []
+    for dy in range(-2, 3):
+        for dx in range(-2, 3):
+            if dx * dx + dy * dy <= 4:
+                want.append(((0 + dx) % 2**32, (1 + dy) % 2**32))
+    assert [(int(o["x"]), int(o["y"])) for o in out] == want
+    assert all(int(o["global_reference"]) == (int(o["y"]) << 32) | int(o["x"]) for o in out)
+
+
+def test_non_spatial_origins_and_partition():
+    s = O.Scenario.non_spatial(7, 5, 3)
+    full = s.origins(1.0)
+    assert len(full) == 105
+    assert [int(v) for v in full["global_reference"]] == list(range(105))
+    parts = [s.origins(1.0, r, 4) for r in range(4)]
+    merged = np.sort(np.concatenate(parts), order="global_reference")
+    assert merged.tobytes() == full.tobytes()
+    sub = s.origins(0.3)
+    assert 15 < len(sub) < 50 and len(set(sub["global_reference"].tolist())) == len(sub)
+
+
+# ---------------------------------------------------------------------------
+# whole-algorithm invariants (config 1 at reduced size)
+@pytest.fixture(scope="module")
+def small_runs():
+    s = O.Scenario.non_spatial(40, 40, 1)
+    lin = s.origins(1.0)
+    runs = {}
+    for d in [None, ("relative", 1.0), ("relative", 0.1), ("absolute", 7)]:
+        runs[d] = O.simulate(s, O.make_params(42, 1e-2, dedup=d), lin)
+    return s, lin, runs
+
+
+def test_dedup_preserves_species_and_unique_events(small_runs):
+    s, lin, runs = small_runs
+    base = runs[None]
+    assert len(base["events"]) == len(lin)  # every lineage speciates exactly once
+    keys = lambda ev: set(zip(ev["event_time"].tolist(), ev["x"].tolist(), ev["y"].tolist(), ev["index"].tolist()))
+    full = set(e.tobytes() for e in base["events"])
+    for d, r in runs.items():
+        assert O.biodiversity(r["events"]) == O.biodiversity(base["events"])
+        assert keys(r["events"]) == keys(base["events"])
+        assert all(e.tobytes() in full for e in r["events"])
+        assert r["steps"] <= base["steps"]
+
+
+def test_stream_is_sorted_by_packed_event_order(small_runs):
+    _, _, runs = small_runs
+    ev = runs[None]["events"]
+    keys = list(zip(ev["event_time"], ev["x"], ev["y"], ev["index"], ev["prior_time"], ev["global_reference"]))
+    assert keys == sorted(keys)
+
+
+def test_pause_resume_continuity_oracle(small_runs):
+    s, lin, runs = small_runs
+    p1 = O.simulate(s, O.make_params(42, 1e-2, dedup=None, pause_before=150.0), lin)
+    assert p1["paused"] and all(p1["events"]["event_time"] < 150.0)
+    p2 = O.simulate(s, O.make_params(42, 1e-2, dedup=None, min_event_time=150.0), p1["remaining"])
+    both = np.sort(np.concatenate([p1["events"], p2["events"]]),
+                   order=["event_time", "x", "y", "index", "prior_time", "global_reference"])
+    assert both.tobytes() == runs[None]["events"].tobytes()
+    assert p1["steps"] + p2["steps"] == runs[None]["steps"]
+
+
+def test_water_level_slicing_does_not_change_dedup_none_output(small_runs):
+    s, lin, runs = small_runs
+    for step_slice, es in [(1, ("relative", 0.5)), (1000, ("absolute", 3)), (3, ("relative", 50.0))]:
+        r = O.simulate(s, O.make_params(42, 1e-2, dedup=None, step_slice=step_slice, event_slice=es), lin)
+        assert r["events"].tobytes() == runs[None]["events"].tobytes()
+        assert r["steps"] == runs[None]["steps"] and r["time"] == runs[None]["time"]
+
+
+def test_partitioned_baseline_union_matches_monolithic(small_runs):
+    s, lin, runs = small_runs
+    r = O.simulate_partitioned(s, O.make_params(42, 1e-2, dedup=None), 1.0, 4)
+    assert r["steps"] == runs[None]["steps"] and r["events"] == len(lin)
