@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark: lineage-steps/s of the HIP Independent algorithm on one MI355X.
+
+Headline workload ("config4-window"): the almost-infinite scenario with
+Gaussian dispersal sigma=50, every lineage of a circle of radius 2000
+(12,566,345 lineages), speciation 1e-6, Seed(42), run with the reference's
+CPU-Independent defaults (delta_t 2, step_slice 10, dedup Relative(1.0),
+event_slice Relative(2.0)) up to pause_before = T (default 40, about 20
+events per lineage before dedup).  One bench step simulates the whole
+window from the initial lineages in the persistent form; lineages are
+already resident in HBM when the timed region starts.
+
+The same JSON line also reports: the plain one-launch-per-slice form on the
+same window (the persistent/plain ratio), config 2 (sigma 10, r 100, nu 1e-4,
+Seed(7), full run), config 3 (128x128 map, nu 1e-5, a pause window, alias
+gathers), the dominant kernel's roofline, and the CPU oracle timed on the
+host's cores on a bounded sample of the headline workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "necsim-rust_amd"))
+
+import numpy as np  # noqa: E402
+
+import necsim_hip as nh  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# bytes of lineage state one persistent round reads and writes per lineage
+# (SoA: ref 8, t_last 8, x/y/index 12, next_t 8, next_state 8, flags 1)
+LINEAGE_STATE_BYTES = 45
+EVENT_BYTES = 40
+CACHE_BUCKET_BYTES = 32
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_job(ctx, tables, cogs, lineages, args, seed, pause):
+    job = nh.Job(ctx, tables, args, nh.WyHash.seed_from_u64(seed), cogs.speciation_probability,
+                 cogs.turnover_rate, pause_before=pause)
+    job.add_lineages(lineages)
+    job.start()  # device allocation + upload, outside the timed region
+    return job
+
+
+def run_job(job):
+    fin = False
+    while not fin:
+        fin = job.step_round()
+        job.drain_events()
+    o = job.outcome()
+    ms, launches = job.kernel_time()
+    return o, ms, launches
+
+
+def timed_runs(ctx, tables, cogs, lineages, args, seed, pause, steps, warmup):
+    for _ in range(warmup):
+        j = make_job(ctx, tables, cogs, lineages, args, seed, pause)
+        run_job(j)
+        j.close()
+    jobs = [make_job(ctx, tables, cogs, lineages, args, seed, pause) for _ in range(steps)]
+    t0 = time.perf_counter()
+    results = [run_job(j) for j in jobs]
+    wall = time.perf_counter() - t0
+    for j in jobs:
+        j.close()
+    return wall, results
+
+
+def cpu_baseline(workload, pause, budget_s=20.0):
+    """The oracle (C restatement of the reference CPU Independent algorithm)
+    on the host's cores, IsolatedIndividuals partitions over a bounded sample
+    of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = max(1, min(16, os.cpu_count() or 1))
+    if workload == "config4-window":
+        scn = O.Scenario.almost_infinite_normal(50.0)
+        circle = (2**31 - 1, 2**31 - 1, 2000)
+        nu, seed, total = 1e-6, 42, 12566345
+    else:
+        raise ValueError(workload)
+    params = O.make_params(seed, nu, pause_before=pause)
+    # calibrate: a small sample first, then size the sample to the budget
+    parts = 4096
+    r = O.simulate_partitioned(scn, params, 1.0, threads, parts, circle=circle)
+    rate = r["steps"] / max(r["seconds"], 1e-6)
+    sample_lineages = total * threads / parts
+    steps_per_lineage = r["steps"] / max(sample_lineages, 1)
+    want_lineages = budget_s * rate / max(steps_per_lineage, 1e-9)
+    parts = int(max(threads, min(parts, total * threads / max(want_lineages, 1))))
+    r = O.simulate_partitioned(scn, params, 1.0, threads, parts, circle=circle)
+    return {"value": r["steps"] / r["seconds"], "unit": "lineage-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{threads} of {parts} IsolatedIndividuals partitions of the same workload "
+                      f"({r['steps']} steps, {r['seconds']:.1f} s, one thread per partition)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--window", type=float, default=40.0, help="pause_before of the headline workload")
+    ap.add_argument("--no-extra", action="store_true", help="headline only")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for {a.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    ctx = nh.Context.get(0)
+    cus, arch = ctx.info()
+    log(f"device {arch} with {cus} CUs")
+
+    # ---- headline: config 4 window, persistent form ----
+    t = time.perf_counter()
+    cogs = nh.AlmostInfiniteNormalDispersalScenario(2000, 50.0).new(1e-6)
+    lineages = cogs.sample_habitat(nh.OriginPreSampler.all())
+    tables = nh.Tables(ctx, cogs)
+    setup_s = time.perf_counter() - t
+    log(f"config 4: {len(lineages)} lineages, setup {setup_s:.2f} s")
+    args = nh.IndependentArguments(form=nh.PERSISTENT)
+    wall, results = timed_runs(ctx, tables, cogs, lineages, args, 42, a.window, a.steps, a.warmup)
+    steps_total = sum(int(o.steps) for o, _, _ in results)
+    value = steps_total / wall
+    kern_ms = sum(ms for _, ms, _ in results)
+    launches = sum(k for _, _, k in results)
+    o0 = results[0][0]
+    log(f"persistent: {steps_total} steps in {wall:.3f} s -> {value:.4e} steps/s; kernel {kern_ms:.1f} ms over {launches} launches")
+
+    # roofline of the dominant kernel (k_persistent): algorithmic bytes per
+    # launch = lineage state in + out + events + dedup bucket read/write per slice
+    slices = o0.steps / args.step_slice + len(lineages)
+    alg_bytes = (len(lineages) + o0.n_remaining) * LINEAGE_STATE_BYTES + o0.n_events * EVENT_BYTES \
+        + slices * 2 * CACHE_BUCKET_BYTES
+    per_launch_bytes = alg_bytes / max(results[0][2], 1)
+    mean_launch_ms = results[0][1] / max(results[0][2], 1)
+    achieved = per_launch_bytes / (mean_launch_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "k_persistent<almost-infinite>", "mean_launch_ms": round(mean_launch_ms, 3),
+                "algorithmic_bytes_per_launch": int(per_launch_bytes)}
+
+    extra = {}
+    if not a.no_extra:
+        # plain form on the same window
+        pargs = nh.IndependentArguments(form=nh.PLAIN)
+        pwall, pres = timed_runs(ctx, tables, cogs, lineages, pargs, 42, a.window, 1, 0)
+        psteps = int(pres[0][0].steps)
+        extra["plain_form"] = {"value": psteps / pwall, "ms": pwall * 1e3, "steps": psteps,
+                               "launches": int(pres[0][2])}
+        extra["persistent_over_plain"] = value / (psteps / pwall)
+        log(f"plain: {psteps} steps in {pwall:.3f} s -> {psteps / pwall:.4e} steps/s")
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline("config4-window", a.window)
+        log(f"cpu baseline: {cpu['value']:.4e} steps/s on {cpu['cores']} cores")
+
+    line = {
+        "metric": "lineage-steps/sec on 1 MI355X vs CPU Independent; p50/p99 sweep latency",
+        "value": value, "unit": "lineage-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64+u64", "data": "synthetic",
+        "config": {"workload": "config4-window: almost-infinite normal sigma=50, circle r=2000 "
+                               f"(12,566,345 lineages), nu=1e-6, Seed(42), pause_before={a.window}, "
+                               "persistent form, dedup Relative(1.0), step_slice 10",
+                   "lineages": int(len(lineages)), "steps_per_run": int(o0.steps),
+                   "speciations_per_run": int(o0.n_events), "rounds": int(o0.rounds)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

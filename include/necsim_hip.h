@@ -196,6 +196,11 @@ int nh_job_create(nh_ctx *ctx, const nh_tables *tables, const nh_job_params *par
  * outside the habitat or their deme fail with NH_E_OUT_OF_DEME
  * (active_lineage_sampler/independent/mod.rs:98-117). */
 int nh_job_add_lineages(nh_job *job, const nh_lineage *lineages, size_t n);
+/* Allocate the job's device state and upload its lineages (done implicitly by
+ * the first nh_job_step_round; callable earlier so that the upload stays out
+ * of a timed region).  No counterpart in the reference: the CUDA algorithm
+ * uploads inside its launch loop (cuda/src/parallelisation/monolithic.rs:222-246). */
+int nh_job_start(nh_job *job);
 /* Run one water-level round (monolithic/mod.rs:135-255).  *finished is set to
  * 1 when no lineage is left or pause_before has been reached. */
 int nh_job_step_round(nh_job *job, int32_t *finished);

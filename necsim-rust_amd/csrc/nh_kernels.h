@@ -63,6 +63,45 @@ struct PlainArgs {
     Counters *ctr;
 };
 
+// ---- serving: one persistent kernel over the lineages of many requests ----
+constexpr int SERVE_MAX_REQS = 1024;
+
+// per-request device counters
+struct ReqCounters {
+    unsigned long long n_out;         // lineages written back this tick
+    unsigned long long n_below;       // ... of which still below the water level (tick limit)
+    unsigned long long n_events;      // speciation events appended (all ticks)
+    unsigned long long steps;         // completed events (all ticks)
+    unsigned long long max_time_bits; // max event time
+    unsigned long long pad[3];
+};
+
+struct ServeReq {
+    uint64_t seed;
+    double nu;
+    double level;
+    LineageSoA in, out;
+    DevEvent *events;
+    uint64_t *cache;
+    unsigned long long cache_cap;
+    unsigned long long n_in;
+    unsigned long long work_base; // first global work index of this request
+    ReqCounters *ctr;
+};
+
+struct ServeArgs {
+    DevScenario S;
+    DevParams P; // shared: lambda*dt, p0, ratio table, delta_t, step_slice, min_event_time
+    const ServeReq *reqs;
+    int n_reqs;
+    unsigned long long total_work;
+    unsigned long long *head;
+    unsigned long long tick_steps;
+};
+
+void launch_serve(int kind, const ServeArgs &a, int blocks, int threads, hipStream_t stream);
+int serve_max_blocks_per_cu(int kind, int threads);
+
 // kernel launchers (all asynchronous on `stream`)
 void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream);
 int persistent_max_blocks_per_cu(int kind, int threads);

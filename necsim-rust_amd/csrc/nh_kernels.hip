@@ -187,6 +187,115 @@ __global__ void __launch_bounds__(256) k_persistent(PersistArgs A) {
 }
 
 // ---------------------------------------------------------------------------
+// Serving kernel: the persistent round kernel over the lineages of many
+// requests at once.  Global work index w belongs to the request whose
+// [work_base, work_base + n_in) contains it; each lane keeps its request's
+// seed, speciation probability and water level in registers and refills from
+// the shared queue independently of which request the next lineage belongs to.
+template <int KIND>
+__global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
+    __shared__ double s_ratio[RATIO_N];
+    __shared__ unsigned long long s_base[SERVE_MAX_REQS];
+    load_ratio(s_ratio, A.P.ratio);
+    for (int r = threadIdx.x; r < A.n_reqs; r += blockDim.x) s_base[r] = A.reqs[r].work_base;
+    __syncthreads();
+    const unsigned lane = lane_id();
+
+    Lane<KIND> L;
+    L.have_next = false;
+    L.ms_some = false;
+    DevParams P = A.P;
+    double level = 0.0;
+    int req = 0;
+    bool has = false, exhausted = false;
+    uint64_t slice_steps = 0, tick = 0;
+    unsigned long long lin_steps = 0;
+    double lin_max = 0.0;
+
+    for (;;) {
+        if (!exhausted) {
+            uint64_t need = __ballot(!has);
+            if (need) {
+                uint32_t cnt = __popcll(need);
+                uint32_t leader = __ffsll((unsigned long long)need) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(A.head, (unsigned long long)cnt);
+                base = __shfl(base, leader);
+                if (base + cnt >= A.total_work) exhausted = true;
+                if (!has) {
+                    unsigned long long w = base + __popcll(need & lanemask_lt());
+                    if (w < A.total_work) {
+                        int lo = 0, hi = A.n_reqs - 1; // last request with work_base <= w
+                        while (lo < hi) {
+                            int mid = (lo + hi + 1) >> 1;
+                            if (s_base[mid] <= w) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        req = lo;
+                        const ServeReq &R = A.reqs[req];
+                        P.seed = R.seed;
+                        P.nu = R.nu;
+                        level = R.level;
+                        lane_load<KIND>(L, A.S, R.in, w - s_base[req]);
+                        has = true;
+                        slice_steps = 0;
+                        tick = 0;
+                        lin_steps = 0;
+                        lin_max = 0.0;
+                    }
+                }
+            }
+        }
+        if (!__any(has)) break;
+        if (has) {
+            lane_peek<KIND>(L, P, s_ratio);
+            bool leave = false, write = false, speciated = false;
+            DevEvent ev;
+            if (L.next_t >= level || (A.tick_steps && tick >= A.tick_steps)) {
+                const ServeReq &R = A.reqs[req];
+                bool dup = false;
+                if (R.cache && L.ms_some)
+                    dup = cache_insert_concurrent(R.cache, R.cache_cap, lane_sample<KIND>(L, A.S));
+                leave = true;
+                write = !dup;
+            } else if (slice_steps >= P.step_slice) {
+                const ServeReq &R = A.reqs[req];
+                bool dup = false;
+                if (R.cache && L.ms_some)
+                    dup = cache_insert_concurrent(R.cache, R.cache_cap, lane_sample<KIND>(L, A.S));
+                L.ms_some = false;
+                slice_steps = 0;
+                if (dup) leave = true;
+            } else {
+                int r = lane_execute<KIND>(L, A.S, P, ev);
+                lin_steps++;
+                slice_steps++;
+                tick++;
+                lin_max = L.t_last;
+                if (r == STEP_SPECIATED) {
+                    leave = true;
+                    speciated = true;
+                }
+            }
+            if (leave) {
+                const ServeReq &R = A.reqs[req];
+                if (speciated) R.events[atomicAdd(&R.ctr->n_events, 1ull)] = ev;
+                if (write) {
+                    unsigned long long o = atomicAdd(&R.ctr->n_out, 1ull);
+                    lane_store<KIND>(L, A.S, R.out, o);
+                    if (L.next_t < level) atomicAdd(&R.ctr->n_below, 1ull);
+                }
+                if (lin_steps) {
+                    atomicAdd(&R.ctr->steps, lin_steps);
+                    atomicMax(&R.ctr->max_time_bits, (unsigned long long)bits(lin_max));
+                }
+                has = false;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Plain slice kernel: element q of the pass runs one step slice
 template <int KIND>
 __global__ void __launch_bounds__(256) k_plain(PlainArgs A) {
@@ -583,6 +692,26 @@ int persistent_max_blocks_per_cu(int kind, int threads) {
     case K_ALMOST_INFINITE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_ALMOST_INFINITE>, threads, 0); break;
     case K_SEPARABLE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_SEPARABLE>, threads, 0); break;
     default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_PACKED>, threads, 0); break;
+    }
+    return n > 0 ? n : 1;
+}
+
+void launch_serve(int kind, const ServeArgs &a, int blocks, int threads, hipStream_t stream) {
+    switch (kind) {
+    case K_NON_SPATIAL: hipLaunchKernelGGL(k_serve<K_NON_SPATIAL>, dim3(blocks), dim3(threads), 0, stream, a); break;
+    case K_ALMOST_INFINITE: hipLaunchKernelGGL(k_serve<K_ALMOST_INFINITE>, dim3(blocks), dim3(threads), 0, stream, a); break;
+    case K_SEPARABLE: hipLaunchKernelGGL(k_serve<K_SEPARABLE>, dim3(blocks), dim3(threads), 0, stream, a); break;
+    default: hipLaunchKernelGGL(k_serve<K_PACKED>, dim3(blocks), dim3(threads), 0, stream, a); break;
+    }
+}
+
+int serve_max_blocks_per_cu(int kind, int threads) {
+    int n = 0;
+    switch (kind) {
+    case K_NON_SPATIAL: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_serve<K_NON_SPATIAL>, threads, 0); break;
+    case K_ALMOST_INFINITE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_serve<K_ALMOST_INFINITE>, threads, 0); break;
+    case K_SEPARABLE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_serve<K_SEPARABLE>, threads, 0); break;
+    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_serve<K_PACKED>, threads, 0); break;
     }
     return n > 0 ? n : 1;
 }

@@ -300,6 +300,11 @@ extern "C" int nh_tables_destroy(nh_tables *t) {
     return NH_OK;
 }
 
+const DevScenario *nh_tables_device_view(const nh_tables *t, int *kind) {
+    *kind = t->kind;
+    return &t->dev;
+}
+
 extern "C" int nh_tables_footprint(const nh_tables *t, uint64_t *bytes) {
     if (!t || !bytes) return set_error(NH_E_INVALID, "null argument");
     *bytes = t->bytes;
@@ -691,6 +696,13 @@ static int persistent_round(nh_job *j) {
     j->cur ^= 1;
     j->cur_n = c.n_out;
     return NH_OK;
+}
+
+extern "C" int nh_job_start(nh_job *j) {
+    if (!j) return set_error(NH_E_INVALID, "null job");
+    if (j->started) return NH_OK;
+    NH_HIP(hipSetDevice(j->ctx->device));
+    return job_start(j);
 }
 
 extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {

@@ -10,7 +10,7 @@ from ._ffi import EVENT_DTYPE, LINEAGE_DTYPE, NecsimHipError, lib
 from .algorithm import (PERSISTENT, PLAIN, AlmostInfiniteNormalDispersalScenario,
                         BiodiversityReporter, Context, DedupCache, EventCollector, EventSlice,
                         HipIndependentAlgorithm, IndependentArguments, Job, NonSpatialScenario,
-                        OriginPreSampler, ScenarioCogs, SimulationOutcome,
+                        OriginPreSampler, ScenarioCogs, Server, SimulationOutcome,
                         SpatiallyExplicitUniformTurnoverScenario, Tables, WyHash,
                         build_alias_tables)
 from .maps import generate_map

@@ -89,6 +89,7 @@ SIGNATURES = {
     "nh_tables_footprint": (ctypes.c_int, [c_vp, ctypes.POINTER(c_u64)]),
     "nh_job_create": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(JobParams), ctypes.POINTER(c_vp)]),
     "nh_job_add_lineages": (ctypes.c_int, [c_vp, c_vp, c_sz]),
+    "nh_job_start": (ctypes.c_int, [c_vp]),
     "nh_job_step_round": (ctypes.c_int, [c_vp, ctypes.POINTER(c_i32)]),
     "nh_job_simulate": (ctypes.c_int, [c_vp]),
     "nh_job_drain_events": (ctypes.c_int, [c_vp, c_vp, c_sz, ctypes.POINTER(c_sz)]),

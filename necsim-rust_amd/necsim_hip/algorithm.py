@@ -326,6 +326,9 @@ class Job:
         lineages = np.ascontiguousarray(lineages, dtype=_ffi.LINEAGE_DTYPE)
         check(lib().nh_job_add_lineages(self.handle, ptr(lineages), len(lineages)))
 
+    def start(self):
+        check(lib().nh_job_start(self.handle))
+
     def step_round(self):
         fin = ctypes.c_int32(0)
         check(lib().nh_job_step_round(self.handle, ctypes.byref(fin)))
@@ -405,3 +408,72 @@ class HipIndependentAlgorithm:
                             lineages, resume_after=None, pause_before=None, reporter=None):
         return cls._run(args, rng, cogs, lineages, pause_before,
                         0.0 if resume_after is None else resume_after, reporter)
+
+
+class Server:
+    """Continuous-batching server on resident tables (nh_server_*).
+
+    Requests are admitted by device footprint as memory frees up; every
+    tick() runs one shared persistent kernel over all admitted requests."""
+
+    def __init__(self, ctx, tables, max_lineages, hbm_budget_bytes, tick_steps=0, block_size=0,
+                 blocks_per_cu=0):
+        p = _ffi.ServerParams(max_lineages=max_lineages, hbm_budget_bytes=hbm_budget_bytes,
+                              tick_steps=tick_steps, block_size=block_size,
+                              blocks_per_cu=blocks_per_cu)
+        h = ctypes.c_void_p()
+        check(lib().nh_server_create(ctx.handle, tables.handle, ctypes.byref(p), ctypes.byref(h)))
+        self.handle = h
+        self.tables = tables
+
+    def submit(self, lineages, rng: WyHash, speciation_probability, args: IndependentArguments,
+               turnover_rate=0.5, pause_before=None):
+        lineages = np.ascontiguousarray(lineages, dtype=_ffi.LINEAGE_DTYPE)
+        r = _ffi.Request(rng_seed=rng.seed, speciation_probability=speciation_probability,
+                         turnover_rate=turnover_rate, delta_t=args.delta_t,
+                         step_slice=args.step_slice, dedup_mode=args.dedup_cache.mode,
+                         dedup_factor=args.dedup_cache.factor,
+                         dedup_capacity=args.dedup_cache.capacity,
+                         event_slice_mode=args.event_slice.mode,
+                         event_slice_factor=args.event_slice.factor,
+                         event_slice_capacity=args.event_slice.capacity,
+                         has_pause_before=0 if pause_before is None else 1,
+                         pause_before=0.0 if pause_before is None else pause_before,
+                         lineages=ptr(lineages), n_lineages=len(lineages))
+        rid = ctypes.c_uint64(0)
+        check(lib().nh_server_submit(self.handle, ctypes.byref(r), ctypes.byref(rid)))
+        return rid.value
+
+    def tick(self):
+        a = ctypes.c_uint64(0)
+        q = ctypes.c_uint64(0)
+        check(lib().nh_server_tick(self.handle, ctypes.byref(a), ctypes.byref(q)))
+        return a.value, q.value
+
+    def poll(self):
+        res = _ffi.RequestResult()
+        rc = lib().nh_server_poll(self.handle, ctypes.byref(res))
+        if rc == -4:  # NH_E_STATE: nothing finished yet
+            return None
+        check(rc)
+        return res
+
+    def events(self, request_id):
+        n = ctypes.c_size_t(0)
+        check(lib().nh_server_request_events(self.handle, request_id, None, 0, ctypes.byref(n)))
+        buf = np.zeros(n.value, _ffi.EVENT_DTYPE)
+        if n.value:
+            check(lib().nh_server_request_events(self.handle, request_id, ptr(buf), n.value,
+                                                 ctypes.byref(n)))
+        return buf
+
+    def release(self, request_id):
+        check(lib().nh_server_release(self.handle, request_id))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().nh_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,229 @@
+//! `Independent`-semantics algorithm on an AMD MI355X through libnecsim_hip.so.
+//!
+//! Mirrors rustcoalescence/algorithms/independent/src/lib.rs:39-173 (CPU
+//! Independent) and rustcoalescence/algorithms/cuda/src/lib.rs:66-593 (the
+//! existing GPU drop-in): same arguments (delta_t, step_slice, dedup_cache,
+//! Monolithic event_slice), same MathsCore (ReproducibleMathsCore) and RNG
+//! (WyHash), same `SimulationOutcome`.  All steps run on the GPU; events come
+//! back sorted per water-level round and are reported through the local
+//! partition like the live water-level reporter
+//! (necsim/impls/no-std/src/parallelisation/independent/monolithic/reporter/live.rs:88-115).
+
+use std::{ffi::CStr, marker::PhantomData};
+
+use necsim_core::{
+    cogs::{MathsCore, RngCore},
+    event::SpeciationEvent,
+    lineage::{GlobalLineageReference, Lineage},
+    reporter::Reporter,
+};
+use necsim_core_bond::{NonNegativeF64, PositiveF64};
+use necsim_impls_no_std::cogs::{
+    lineage_store::independent::IndependentLineageStore,
+    maths::reproducible::ReproducibleMathsCore,
+    origin_sampler::pre_sampler::OriginPreSampler, rng::wyhash::WyHash,
+};
+use necsim_partitioning_core::{
+    partition::{Partition, PartitionSize},
+    LocalPartition, Partitioning,
+};
+use rustcoalescence_algorithms::{
+    result::{ResumeError, SimulationOutcome},
+    strategy::RestartFixUpStrategy,
+    Algorithm, AlgorithmDefaults, AlgorithmDispatch, AlgorithmParamters,
+};
+use rustcoalescence_scenarios::{Scenario, ScenarioCogs};
+
+mod arguments;
+pub mod ffi;
+mod scenario;
+
+pub use arguments::HipArguments;
+pub use scenario::HipScenario;
+
+#[derive(thiserror::Error, Debug, serde::Serialize, serde::Deserialize)]
+#[error("necsim-hip error {code}: {message}")]
+pub struct HipError {
+    pub code: i32,
+    pub message: String,
+}
+
+fn check(code: i32) -> Result<(), HipError> {
+    if code == ffi::NH_OK {
+        return Ok(());
+    }
+    let mut buf = [0 as core::ffi::c_char; 512];
+    unsafe { ffi::nh_last_error(buf.as_mut_ptr(), buf.len()) };
+    let message = unsafe { CStr::from_ptr(buf.as_ptr()) }.to_string_lossy().into_owned();
+    Err(HipError { code, message })
+}
+
+#[allow(clippy::module_name_repetitions, clippy::empty_enum)]
+pub enum HipAlgorithm {}
+
+impl AlgorithmParamters for HipAlgorithm {
+    type Arguments = HipArguments;
+    type Error = HipError;
+}
+
+impl AlgorithmDefaults for HipAlgorithm {
+    // the deterministic libm core; the device restates the same functions
+    type MathsCore = ReproducibleMathsCore;
+    type Rng<M: MathsCore> = WyHash<M>;
+}
+
+impl<M: MathsCore, O: Scenario<M, WyHash<M>> + HipScenario<M>, R: Reporter>
+    AlgorithmDispatch<M, WyHash<M>, O, R> for HipAlgorithm
+{
+    type Algorithm<'p, P: LocalPartition<'p, R>> = Self;
+
+    fn get_logical_partition_size<P: Partitioning>(
+        _args: &Self::Arguments,
+        _partitioning: &P,
+    ) -> PartitionSize {
+        PartitionSize::MONOLITHIC
+    }
+}
+
+/// The WyHash seed (WyHash is #[repr(C)] { seed: u64, state: u64 },
+/// necsim/impls/no-std/src/cogs/rng/wyhash.rs:15-24; the state is re-primed
+/// before every use, so the seed is all the device needs).
+fn wyhash_seed<M: MathsCore>(rng: &WyHash<M>) -> u64 {
+    unsafe { *(rng as *const WyHash<M>).cast::<u64>() }
+}
+
+struct Device {
+    ctx: *mut ffi::nh_ctx,
+    tables: *mut ffi::nh_tables,
+    job: *mut ffi::nh_job,
+}
+
+impl Drop for Device {
+    fn drop(&mut self) {
+        unsafe {
+            if !self.job.is_null() {
+                ffi::nh_job_destroy(self.job);
+            }
+            if !self.tables.is_null() {
+                ffi::nh_tables_destroy(self.tables);
+            }
+            if !self.ctx.is_null() {
+                ffi::nh_ctx_destroy(self.ctx);
+            }
+        }
+    }
+}
+
+#[allow(clippy::too_many_arguments)]
+fn simulate<'p, M: MathsCore, O: Scenario<M, WyHash<M>> + HipScenario<M>, R: Reporter, P: LocalPartition<'p, R>>(
+    args: &HipArguments,
+    rng: WyHash<M>,
+    scenario: &ScenarioCogs<M, WyHash<M>, O>,
+    lineages: &[Lineage],
+    resume_after: Option<NonNegativeF64>,
+    pause_before: Option<NonNegativeF64>,
+    local_partition: &mut P,
+) -> Result<SimulationOutcome<M, WyHash<M>>, HipError> {
+    let mut dev = Device { ctx: core::ptr::null_mut(), tables: core::ptr::null_mut(), job: core::ptr::null_mut() };
+    check(unsafe { ffi::nh_init(args.device, &mut dev.ctx) })?;
+    let (desc, _keep) = O::hip_scenario(scenario);
+    check(unsafe { ffi::nh_tables_create(dev.ctx, &desc, &mut dev.tables) })?;
+    let params = args.job_params(
+        wyhash_seed(&rng),
+        O::speciation_probability(scenario),
+        O::turnover_rate(scenario),
+        resume_after,
+        pause_before,
+    );
+    check(unsafe { ffi::nh_job_create(dev.ctx, dev.tables, &params, &mut dev.job) })?;
+    // Lineage is #[repr(C)] with the nh_lineage layout (lineage.rs:97-109)
+    check(unsafe { ffi::nh_job_add_lineages(dev.job, lineages.as_ptr().cast(), lineages.len()) })?;
+    let mut buf: Vec<ffi::nh_speciation_event> = Vec::new();
+    loop {
+        let mut finished = 0_i32;
+        check(unsafe { ffi::nh_job_step_round(dev.job, &mut finished) })?;
+        let mut n = 0_usize;
+        check(unsafe { ffi::nh_job_drain_events(dev.job, core::ptr::null_mut(), 0, &mut n) })?;
+        buf.resize(n, ffi::nh_speciation_event::default());
+        check(unsafe { ffi::nh_job_drain_events(dev.job, buf.as_mut_ptr(), n, &mut n) })?;
+        for e in &buf[..n] {
+            // SpeciationEvent is #[repr(C)] with the nh_speciation_event layout (event.rs:58-71)
+            let event: &SpeciationEvent = unsafe { &*(e as *const ffi::nh_speciation_event).cast() };
+            local_partition.get_reporter().report_speciation(event.into());
+        }
+        if finished != 0 {
+            break;
+        }
+    }
+    let mut o = ffi::nh_outcome::default();
+    check(unsafe { ffi::nh_job_outcome(dev.job, &mut o) })?;
+    let time = NonNegativeF64::new(o.time).unwrap_or(NonNegativeF64::zero());
+    if o.paused == 0 {
+        return Ok(SimulationOutcome::Done { time, steps: o.steps });
+    }
+    let mut rem = vec![ffi::nh_lineage::default(); o.n_remaining as usize];
+    let mut n = 0_usize;
+    check(unsafe { ffi::nh_job_take_lineages(dev.job, rem.as_mut_ptr(), rem.len(), &mut n) })?;
+    let lineages = rem[..n]
+        .iter()
+        .map(|l| unsafe { core::ptr::read((l as *const ffi::nh_lineage).cast::<Lineage>()) })
+        .collect();
+    Ok(SimulationOutcome::Paused { time, steps: o.steps, lineages, rng, marker: PhantomData::<M> })
+}
+
+impl<'p, M: MathsCore, O: Scenario<M, WyHash<M>> + HipScenario<M>, R: Reporter, P: LocalPartition<'p, R>>
+    Algorithm<'p, M, WyHash<M>, O, R, P> for HipAlgorithm
+{
+    type LineageStore = IndependentLineageStore<M, O::Habitat>;
+
+    fn get_logical_partition(_args: &Self::Arguments, local_partition: &P) -> Partition {
+        local_partition.get_partition()
+    }
+
+    fn initialise_and_simulate<I: Iterator<Item = u64>>(
+        args: Self::Arguments,
+        rng: WyHash<M>,
+        scenario: ScenarioCogs<M, WyHash<M>, O>,
+        pre_sampler: OriginPreSampler<M, I>,
+        pause_before: Option<NonNegativeF64>,
+        local_partition: &mut P,
+    ) -> Result<SimulationOutcome<M, WyHash<M>>, Self::Error> {
+        // the origin sampler stays on the CPU, in the reference's own order
+        let lineages: Vec<Lineage> = O::sample_lineages(&scenario, pre_sampler);
+        simulate(&args, rng, &scenario, &lineages, None, pause_before, local_partition)
+    }
+
+    fn resume_and_simulate<I: Iterator<Item = u64>, L: ExactSizeIterator<Item = Lineage>>(
+        args: Self::Arguments,
+        rng: WyHash<M>,
+        scenario: ScenarioCogs<M, WyHash<M>, O>,
+        _pre_sampler: OriginPreSampler<M, I>,
+        lineages: L,
+        resume_after: Option<NonNegativeF64>,
+        pause_before: Option<NonNegativeF64>,
+        local_partition: &mut P,
+    ) -> Result<SimulationOutcome<M, WyHash<M>>, ResumeError<Self::Error>> {
+        let lineages: Vec<Lineage> = lineages.collect();
+        Ok(simulate(&args, rng, &scenario, &lineages, resume_after, pause_before, local_partition)?)
+    }
+
+    fn fixup_for_restart<I: Iterator<Item = u64>, L: ExactSizeIterator<Item = Lineage>>(
+        _args: Self::Arguments,
+        _rng: WyHash<M>,
+        _scenario: ScenarioCogs<M, WyHash<M>, O>,
+        _pre_sampler: OriginPreSampler<M, I>,
+        _lineages: L,
+        _restart_at: PositiveF64,
+        _fixup_strategy: RestartFixUpStrategy,
+        _local_partition: &mut P,
+    ) -> Result<SimulationOutcome<M, WyHash<M>>, ResumeError<Self::Error>> {
+        // the CPU Independent algorithm's FixUp path (const event-time sampler,
+        // trespassing dispersal) is not part of the device library
+        Err(ResumeError::Simulate(HipError { code: -4, message: "fixup_for_restart is not offloaded".into() }))
+    }
+}
+
+#[allow(dead_code)]
+fn _reference_type_is_u64(r: GlobalLineageReference) -> u64 {
+    unsafe { r.into_inner() }
+}

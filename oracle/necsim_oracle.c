@@ -1318,14 +1318,15 @@ static void *part_run(void *arg) {
 
 uint64_t orc_simulate_partitioned(const orc_scenario *s, const orc_params *p, double percentage,
                                   uint32_t cx, uint32_t cy, uint32_t r, uint32_t threads,
-                                  double *seconds, uint64_t *n_events) {
+                                  uint32_t partitions, double *seconds, uint64_t *n_events) {
     if (threads < 1) threads = 1;
+    if (partitions < threads) partitions = threads;
     part_job *jobs = (part_job *)calloc(threads, sizeof(part_job));
     pthread_t *tids = (pthread_t *)calloc(threads, sizeof(pthread_t));
     for (uint32_t t = 0; t < threads; t++) {
         jobs[t].s = s; jobs[t].p = p; jobs[t].percentage = percentage;
         jobs[t].cx = cx; jobs[t].cy = cy; jobs[t].r = r;
-        jobs[t].rank = t; jobs[t].size = threads;
+        jobs[t].rank = t; jobs[t].size = partitions;
         pthread_create(&tids[t], NULL, part_run, &jobs[t]);
     }
     uint64_t steps = 0, events = 0;

@@ -159,11 +159,13 @@ int orc_simulate(const orc_scenario *s, const orc_params *p, const orc_lineage *
 uint64_t orc_biodiversity(const orc_speciation_event *events, size_t n);
 
 /* IsolatedIndividuals CPU baseline: T threads, thread r simulates partition r
- * of T of the origin pre-sampler.  Returns total steps; *seconds = wall time
- * of the simulate phase (origin sampling excluded). */
+ * of `partitions` (>= T) of the origin pre-sampler, so T == partitions covers
+ * every lineage and T < partitions a bounded sample.  Returns total steps;
+ * *seconds = wall time of the simulate phase (origin sampling excluded). */
 uint64_t orc_simulate_partitioned(const orc_scenario *s, const orc_params *p, double percentage,
                                   uint32_t circle_cx, uint32_t circle_cy, uint32_t circle_radius,
-                                  uint32_t threads, double *seconds, uint64_t *n_events);
+                                  uint32_t threads, uint32_t partitions, double *seconds,
+                                  uint64_t *n_events);
 
 #ifdef __cplusplus
 }

@@ -108,7 +108,7 @@ def lib():
         L.orc_biodiversity.restype = u64
         L.orc_biodiversity.argtypes = [vp, ctypes.c_size_t]
         L.orc_simulate_partitioned.restype = u64
-        L.orc_simulate_partitioned.argtypes = [vp, ctypes.POINTER(Params), d, u32, u32, u32, u32,
+        L.orc_simulate_partitioned.argtypes = [vp, ctypes.POINTER(Params), d, u32, u32, u32, u32, u32,
                                                ctypes.POINTER(d), ctypes.POINTER(u64)]
         _lib = L
     return _lib
@@ -223,10 +223,12 @@ def biodiversity(events):
     return int(lib().orc_biodiversity(_ptr(events), len(events)))
 
 
-def simulate_partitioned(scenario, params, percentage, threads, circle=(2**31 - 1, 2**31 - 1, 0)):
+def simulate_partitioned(scenario, params, percentage, threads, partitions=None,
+                         circle=(2**31 - 1, 2**31 - 1, 0)):
     secs = ctypes.c_double(0.0)
     nev = ctypes.c_uint64(0)
     cx, cy, r = circle
     steps = lib().orc_simulate_partitioned(scenario.handle, ctypes.byref(params), percentage, cx,
-                                           cy, r, threads, ctypes.byref(secs), ctypes.byref(nev))
+                                           cy, r, threads, partitions or threads, ctypes.byref(secs),
+                                           ctypes.byref(nev))
     return dict(steps=int(steps), seconds=secs.value, events=int(nev.value))
