@@ -102,14 +102,97 @@ def cpu_baseline(workload, pause, budget_s=20.0):
                       f"({r['steps']} steps, {r['seconds']:.1f} s, one thread per partition)"}
 
 
+def config2_run(ctx):
+    """config 2 (sigma 10, r 100, nu 1e-4, Seed(7)) to completion, persistent form"""
+    cogs = nh.AlmostInfiniteNormalDispersalScenario(100, 10.0).new(1e-4)
+    lin = cogs.sample_habitat(nh.OriginPreSampler.all())
+    tables = nh.Tables(ctx, cogs)
+    out = {}
+    for name, dedup in (("dedup_relative_1", nh.DedupCache.Relative(1.0)), ("dedup_none", nh.DedupCache.none())):
+        wall, res = timed_runs(ctx, tables, cogs, lin, nh.IndependentArguments(dedup_cache=dedup), 7, None, 1, 1)
+        o, ms, k = res[0]
+        out[name] = {"steps": int(o.steps), "ms": wall * 1e3, "value": o.steps / wall, "kernel_ms": ms,
+                     "speciations": int(o.n_events), "rounds": int(o.rounds)}
+    return {"lineages": int(len(lin)), **out}
+
+
+GATHER_BYTES_PER_STEP = 48  # cell record 32 B + alias atom 16 B per dispersal step
+
+
+def config3_run(ctx, window):
+    """config 3 (128x128 map, nu 1e-5): the alias-gather workload"""
+    h, rp, c, w = nh.generate_map(128, 128, seed=42)
+    scn = nh.SpatiallyExplicitUniformTurnoverScenario(128, 128, h, rp, c, w)
+    cogs = scn.new(1e-5)
+    lin = cogs.sample_habitat(nh.OriginPreSampler.all())
+    tables = nh.Tables(ctx, cogs)
+    wall, res = timed_runs(ctx, tables, cogs, lin, nh.IndependentArguments(), 42, window, 1, 1)
+    o, ms, k = res[0]
+    achieved = o.steps * GATHER_BYTES_PER_STEP / (ms * 1e-3) / 1e9 if ms else 0.0
+    return {"lineages": int(len(lin)), "window": window, "steps": int(o.steps), "ms": wall * 1e3,
+            "value": o.steps / wall, "kernel_ms": ms, "atoms": int(len(scn.tables()[2])),
+            "gather_roofline": {"bound": "hbm", "bytes_per_step": GATHER_BYTES_PER_STEP,
+                                "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(achieved / HBM_PEAK_GBS, 5),
+                                "roofline_steps_per_s": HBM_PEAK_GBS * 1e9 / GATHER_BYTES_PER_STEP}}
+
+
+def serving_run(ctx, n_requests=150, rate=50.0, cap=1000.0, tick_steps=250, seed=1):
+    """config 5: Poisson stream of simulations on a shared 256x256 map"""
+    h, rp, c, w = nh.generate_map(256, 256, seed=42)
+    scn = nh.SpatiallyExplicitUniformTurnoverScenario(256, 256, h, rp, c, w)
+    base = scn.new(1e-3)
+    tables = nh.Tables(ctx, base)
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    reqs = []
+    for k in range(n_requests):
+        nu = float(10 ** rng.uniform(-7, -3))
+        pct = float(rng.uniform(0.01, 1.0))
+        lin = base.sample_habitat(nh.OriginPreSampler.all().percentage(pct))
+        reqs.append((nu, lin))
+    srv = nh.Server(ctx, tables, max_lineages=1 << 24, hbm_budget_bytes=48 << 30, tick_steps=tick_steps)
+    args = nh.IndependentArguments()
+    t0 = time.perf_counter()
+    submitted = 0
+    lat, steps, done = [], 0, 0
+    submit_t = {}
+    while done < n_requests:
+        now = time.perf_counter() - t0
+        while submitted < n_requests and arrivals[submitted] <= now:
+            nu, lin = reqs[submitted]
+            rid = srv.submit(lin, nh.WyHash.seed_from_u64(submitted), nu, args, pause_before=cap)
+            submit_t[rid] = t0 + arrivals[submitted]
+            submitted += 1
+        active, queued = srv.tick()
+        while True:
+            r = srv.poll()
+            if r is None:
+                break
+            lat.append((time.perf_counter() - submit_t[r.request_id]) * 1e3)
+            steps += int(r.steps)
+            srv.release(r.request_id)
+            done += 1
+        if active == 0 and queued == 0 and submitted < n_requests:
+            time.sleep(max(0.0, min(0.002, arrivals[submitted] - (time.perf_counter() - t0))))
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat = np.array(lat)
+    return {"requests": n_requests, "arrival_rate_per_s": rate, "pause_before_cap": cap,
+            "tick_steps": tick_steps, "wall_s": wall, "steps": steps, "value": steps / wall,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "max_ms": float(lat.max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--window", type=float, default=40.0, help="pause_before of the headline workload")
+    ap.add_argument("--window", type=float, default=2000.0, help="pause_before of the headline workload")
     ap.add_argument("--no-extra", action="store_true", help="headline only")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", type=int, default=1, help="capture persistent rounds in hipGraphs")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for {a.gpus})",
@@ -127,38 +210,48 @@ def main():
     tables = nh.Tables(ctx, cogs)
     setup_s = time.perf_counter() - t
     log(f"config 4: {len(lineages)} lineages, setup {setup_s:.2f} s")
-    args = nh.IndependentArguments(form=nh.PERSISTENT)
+    args = nh.IndependentArguments(form=nh.PERSISTENT, use_graph=bool(a.graph))
     wall, results = timed_runs(ctx, tables, cogs, lineages, args, 42, a.window, a.steps, a.warmup)
     steps_total = sum(int(o.steps) for o, _, _ in results)
     value = steps_total / wall
     kern_ms = sum(ms for _, ms, _ in results)
     launches = sum(k for _, _, k in results)
     o0 = results[0][0]
-    log(f"persistent: {steps_total} steps in {wall:.3f} s -> {value:.4e} steps/s; kernel {kern_ms:.1f} ms over {launches} launches")
+    log(f"persistent: {steps_total} steps in {wall:.3f} s -> {value:.4e} steps/s; "
+        f"device {kern_ms:.1f} ms over {launches} launches")
 
     # roofline of the dominant kernel (k_persistent): algorithmic bytes per
-    # launch = lineage state in + out + events + dedup bucket read/write per slice
-    slices = o0.steps / args.step_slice + len(lineages)
-    alg_bytes = (len(lineages) + o0.n_remaining) * LINEAGE_STATE_BYTES + o0.n_events * EVENT_BYTES \
-        + slices * 2 * CACHE_BUCKET_BYTES
-    per_launch_bytes = alg_bytes / max(results[0][2], 1)
+    # launch = lineage state read + written back once per round, speciation
+    # events, and one dedup bucket read + write per step slice
+    per_launch_steps = o0.steps / max(results[0][2], 1)
+    per_launch_bytes = (len(lineages) * 2 * LINEAGE_STATE_BYTES / max(results[0][2], 1)
+                        + o0.n_events / max(results[0][2], 1) * EVENT_BYTES
+                        + per_launch_steps / args.step_slice * 2 * CACHE_BUCKET_BYTES)
     mean_launch_ms = results[0][1] / max(results[0][2], 1)
     achieved = per_launch_bytes / (mean_launch_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_persistent<almost-infinite>", "mean_launch_ms": round(mean_launch_ms, 3),
-                "algorithmic_bytes_per_launch": int(per_launch_bytes)}
+    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                "kernel": "k_persistent<almost-infinite>" + (" (graph: + event sort)" if a.graph else ""),
+                "mean_launch_ms": round(mean_launch_ms, 3),
+                "algorithmic_bytes_per_launch": int(per_launch_bytes),
+                "steps_per_launch": int(per_launch_steps)}
 
     extra = {}
     if not a.no_extra:
-        # plain form on the same window
         pargs = nh.IndependentArguments(form=nh.PLAIN)
         pwall, pres = timed_runs(ctx, tables, cogs, lineages, pargs, 42, a.window, 1, 0)
         psteps = int(pres[0][0].steps)
         extra["plain_form"] = {"value": psteps / pwall, "ms": pwall * 1e3, "steps": psteps,
-                               "launches": int(pres[0][2])}
+                               "launches": int(pres[0][2]), "kernel_ms": pres[0][1]}
         extra["persistent_over_plain"] = value / (psteps / pwall)
         log(f"plain: {psteps} steps in {pwall:.3f} s -> {psteps / pwall:.4e} steps/s")
+        del tables
+        extra["config2_full_run"] = config2_run(ctx)
+        log(f"config 2: {extra['config2_full_run']}")
+        extra["config3_window"] = config3_run(ctx, 500.0)
+        log(f"config 3: {extra['config3_window']}")
+        extra["serving_config5"] = serving_run(ctx)
+        log(f"serving: {extra['serving_config5']}")
 
     cpu = None
     if not a.no_cpu:
@@ -172,9 +265,10 @@ def main():
         "vs_baseline": None, "dtype": "f64+u64", "data": "synthetic",
         "config": {"workload": "config4-window: almost-infinite normal sigma=50, circle r=2000 "
                                f"(12,566,345 lineages), nu=1e-6, Seed(42), pause_before={a.window}, "
-                               "persistent form, dedup Relative(1.0), step_slice 10",
+                               "persistent form, dedup Relative(1.0), step_slice 10, event_slice 2.0",
                    "lineages": int(len(lineages)), "steps_per_run": int(o0.steps),
-                   "speciations_per_run": int(o0.n_events), "rounds": int(o0.rounds)},
+                   "speciations_per_run": int(o0.n_events), "rounds": int(o0.rounds),
+                   "remaining_lineages": int(o0.n_remaining)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extra": extra,

@@ -219,6 +219,18 @@ int nh_job_destroy(nh_job *job);
  * recorded on the job's stream, and the number of timed launches. */
 int nh_job_kernel_time(const nh_job *job, double *total_ms, uint64_t *launches);
 
+/* Autotuner of the persistent form (no counterpart in the reference, whose
+ * CUDA launch shape is fixed by arguments, cuda/src/arguments.rs:124-138).
+ * Times short pause windows ([0, window)) on `sample` for every candidate
+ * block size (64/128/256), resident blocks per CU (occupancy limit and half of
+ * it) and, when base->step_slice == 0, step slice (5, 10, 20, 50, 150), and
+ * writes the fastest combination (least device time for the window) to
+ * *tuned (a copy of *base with block_size, blocks_per_cu and step_slice set).
+ * *best_ms receives the winner's window time. */
+int nh_autotune(nh_ctx *ctx, const nh_tables *tables, const nh_job_params *base,
+                const nh_lineage *sample, size_t n, double window, nh_job_params *tuned,
+                double *best_ms);
+
 /* ---- serving: continuous batching of simulations on resident tables ------- */
 typedef struct {
     uint64_t max_lineages;      /* lane-state capacity shared by requests in flight */

@@ -73,6 +73,9 @@ struct DevParams {
     const double *ratio;    // ratio[k] = lambda_dt / k, k in 1..RATIO_N-1 (device, global)
 };
 
+constexpr uint8_t FLAG_EXACT = 1; // next_t / next_st hold the exact next event
+constexpr uint8_t FLAG_DEAD = 2;  // the lineage speciated or was dropped as a duplicate
+
 // Lineage state held in HBM between launches (SoA, coalesced per field).
 struct LineageSoA {
     uint64_t *ref;      // global_reference
@@ -80,7 +83,7 @@ struct LineageSoA {
     uint32_t *x, *y, *idx;
     double *next_t;     // next event time (pessimistic lower bound, or exact if flag bit 0)
     uint64_t *next_st;  // RNG state after the final prime of next_t (exact only)
-    uint8_t *flags;     // bit 0: next_t / next_st are exact
+    uint8_t *flags;     // FLAG_EXACT, FLAG_DEAD
 };
 
 // Speciation event, the reference's SpeciationEvent layout (40 B)
@@ -176,7 +179,7 @@ __device__ __forceinline__ double poisson_next(const DevParams &P, const double 
         } else {
             double a, b;
             sample_2d_normal(state, P.lambda_dt, P.lambda_dt_sqrt, a, b);
-            number = as_u32(a > 0.0 || a != a ? (a != a ? 0.0 : a) : 0.0);
+            number = as_u32(a); // f64::max(a, 0.0) as u32: NaN and negatives give 0
         }
         bool found = false;
         for (uint32_t e = 0; e < number; e++) {

@@ -13,7 +13,8 @@ struct Counters {
     unsigned long long steps;         // completed events
     unsigned long long max_time_bits; // max event time (positive f64 bits)
     unsigned long long overflow;      // plain dedup: a bucket list exceeded the device limit
-    unsigned long long pad[2];
+    unsigned long long n_dead;        // persistent form: lineages that died this round
+    unsigned long long pad;
 };
 
 // Per-round inputs read by the kernels from device memory (so that a captured
@@ -21,7 +22,17 @@ struct Counters {
 struct RoundDesc {
     double level;
     unsigned long long n_in;
+    unsigned long long ev_start; // events appended before this round
+    unsigned long long chunk;    // work indices claimed per wave atomic (0: per idle lane)
 };
+
+// Claim granularity of the persistent kernels: exact per-lane claims while
+// there are few lineages per lane, chunks of up to 1024 when there are many.
+inline unsigned long long claim_chunk(unsigned long long n, unsigned long long lanes) {
+    unsigned long long per_wave = lanes ? n / (lanes / 64 * 4 + 1) : 0;
+    if (per_wave < 256) return 0;
+    return per_wave > 1024 ? 1024 : (per_wave & ~63ull);
+}
 
 // plain-form element status (one byte per pass element)
 enum : uint8_t {
@@ -37,7 +48,7 @@ enum : uint8_t {
 struct PersistArgs {
     DevScenario S;
     DevParams P;
-    LineageSoA in, out;
+    LineageSoA lin; // updated in place
     const RoundDesc *round;
     uint64_t *cache;          // 4 u64 per bucket, null if dedup is off
     unsigned long long cache_cap;
@@ -68,11 +79,11 @@ constexpr int SERVE_MAX_REQS = 1024;
 
 // per-request device counters
 struct ReqCounters {
-    unsigned long long n_out;         // lineages written back this tick
-    unsigned long long n_below;       // ... of which still below the water level (tick limit)
+    unsigned long long n_dead;        // lineages that died this tick
+    unsigned long long n_below;       // set if a lineage stopped below the water level (tick limit)
     unsigned long long n_events;      // speciation events appended (all ticks)
-    unsigned long long steps;         // completed events (all ticks)
-    unsigned long long max_time_bits; // max event time
+    unsigned long long steps;         // completed events this tick
+    unsigned long long max_time_bits; // max event time this tick
     unsigned long long pad[3];
 };
 
@@ -80,7 +91,7 @@ struct ServeReq {
     uint64_t seed;
     double nu;
     double level;
-    LineageSoA in, out;
+    LineageSoA lin; // updated in place
     DevEvent *events;
     uint64_t *cache;
     unsigned long long cache_cap;
@@ -97,6 +108,7 @@ struct ServeArgs {
     unsigned long long total_work;
     unsigned long long *head;
     unsigned long long tick_steps;
+    unsigned long long chunk;
 };
 
 void launch_serve(int kind, const ServeArgs &a, int blocks, int threads, hipStream_t stream);
@@ -136,6 +148,17 @@ struct SortBuffers {
 uint32_t sort_blocks(uint64_t n);
 void sort_events(const DevEvent *events, uint64_t n, DevEvent *out, SortBuffers &sb,
                  hipStream_t stream, bool fixed_passes);
+// Same, with the event range [*start, *end) read from device memory when the
+// kernels run (fixed grid for up to `cap` events; capturable in a graph).
+void sort_events_device_sized(const DevEvent *events, const unsigned long long *end,
+                              const unsigned long long *start, uint64_t cap, DevEvent *out, SortBuffers &sb,
+                              hipStream_t stream);
+
+// Ordered compaction of the live (not dead) lineages of in[0, n) into out;
+// the live count is written to blk[soa_compact_blocks(n)].
+uint32_t soa_compact_blocks(uint64_t n);
+void launch_soa_compact(const LineageSoA &in, const LineageSoA &out, uint64_t n, uint32_t *blk,
+                        hipStream_t stream);
 
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream);
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream);

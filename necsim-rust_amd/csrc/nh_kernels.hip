@@ -74,7 +74,74 @@ __device__ __forceinline__ void load_ratio(double *s_ratio, const double *ratio)
 }
 
 // ---------------------------------------------------------------------------
-// Persistent round kernel
+// Wave-level work claiming: each wave owns a private range [next, end) of
+// work indices and refills it with ONE global atomic per `chunk` indices, so
+// lane refill costs a ballot and a popcount, not a contended atomic.  With
+// chunk == 0 a wave claims exactly as many indices as it has idle lanes
+// (best balance when there are few lineages per lane).
+
+struct WaveQueue {
+    unsigned long long next, end;
+    bool drained;   // the global queue is empty
+    bool exhausted; // ... and so is this wave's range
+};
+
+// Hands this lane the next work index if it needs one (need = ballot of
+// lanes without work).  Returns ~0ull when the lane gets nothing.
+__device__ __forceinline__ unsigned long long wave_claim(WaveQueue &q, uint64_t need, bool wants,
+                                                         unsigned long long *head,
+                                                         unsigned long long n_total,
+                                                         unsigned long long chunk) {
+    const unsigned lane = lane_id();
+    if (q.next >= q.end && !q.drained) {
+        uint32_t leader = __ffsll((unsigned long long)need) - 1;
+        unsigned long long take = chunk ? chunk : (unsigned long long)__popcll(need);
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(head, take);
+        base = __shfl(base, leader);
+        if (base >= n_total) {
+            q.drained = true;
+        } else {
+            q.next = base;
+            q.end = base + take < n_total ? base + take : n_total;
+        }
+    }
+    unsigned long long avail = q.end - q.next;
+    unsigned long long mine = ~0ull;
+    if (wants) {
+        unsigned long long r = __popcll(need & lanemask_lt());
+        if (r < avail) mine = q.next + r;
+    }
+    unsigned long long cnt = __popcll(need);
+    q.next += cnt < avail ? cnt : avail;
+    if (q.drained && q.next >= q.end) q.exhausted = true;
+    return mine;
+}
+
+__device__ __forceinline__ void block_flush(unsigned long long steps, double max_t, unsigned long long dead,
+                                            Counters *ctr) {
+    __shared__ unsigned long long s_steps, s_max, s_dead;
+    if (threadIdx.x == 0) {
+        s_steps = 0;
+        s_max = 0;
+        s_dead = 0;
+    }
+    __syncthreads();
+    if (steps) atomicAdd(&s_steps, steps);
+    if (max_t > 0.0) atomicMax(&s_max, (unsigned long long)bits(max_t));
+    if (dead) atomicAdd(&s_dead, dead);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_steps) atomicAdd(&ctr->steps, s_steps);
+        if (s_max) atomicMax(&ctr->max_time_bits, s_max);
+        if (s_dead) atomicAdd(&ctr->n_dead, s_dead);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent round kernel.  Lineages are updated in place: a lineage that
+// reaches the water level is written back to its slot, one that speciates or
+// is dropped as a duplicate only gets its dead flag.
 template <int KIND>
 __global__ void __launch_bounds__(256) k_persistent(PersistArgs A) {
     __shared__ double s_ratio[RATIO_N];
@@ -88,36 +155,32 @@ __global__ void __launch_bounds__(256) k_persistent(PersistArgs A) {
     L.have_next = false;
     L.ms_some = false;
     bool has = false;
-    bool exhausted = false;
-    uint64_t slice_steps = 0;
-    uint64_t tick = 0;
-    unsigned long long my_steps = 0;
+    unsigned long long slot = 0;
+    WaveQueue q{0, 0, false, false};
+    uint64_t slice_steps = 0, tick = 0;
+    unsigned long long my_steps = 0, my_dead = 0;
     double my_max = 0.0;
 
     for (;;) {
-        if (!exhausted) {
+        if (!q.exhausted) {
             uint64_t need = __ballot(!has);
             if (need) {
-                uint32_t cnt = __popcll(need);
-                uint32_t leader = __ffsll((unsigned long long)need) - 1;
-                unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(&A.ctr->head, (unsigned long long)cnt);
-                base = __shfl(base, leader);
-                if (base + cnt >= n_in) exhausted = true;
-                if (!has) {
-                    unsigned long long pos = base + __popcll(need & lanemask_lt());
-                    if (pos < n_in) {
-                        lane_load<KIND>(L, A.S, A.in, pos);
-                        has = true;
-                        slice_steps = 0;
-                        tick = 0;
-                    }
+                unsigned long long w = wave_claim(q, need, !has, &A.ctr->head, n_in, A.round->chunk);
+                if (w != ~0ull && !(A.lin.flags[w] & FLAG_DEAD)) {
+                    slot = w;
+                    lane_load<KIND>(L, A.S, A.lin, slot);
+                    has = true;
+                    slice_steps = 0;
+                    tick = 0;
                 }
             }
         }
-        if (!__any(has)) break;
+        if (!__any(has)) {
+            if (q.exhausted) break;
+            continue;
+        }
 
-        bool emit = false, out = false;
+        bool emit = false;
         DevEvent ev;
         if (has) {
             lane_peek<KIND>(L, A.P, s_ratio);
@@ -129,15 +192,24 @@ __global__ void __launch_bounds__(256) k_persistent(PersistArgs A) {
                 bool dup = false;
                 if (A.cache && L.ms_some)
                     dup = cache_insert_concurrent(A.cache, A.cache_cap, lane_sample<KIND>(L, A.S));
+                if (dup) {
+                    A.lin.flags[slot] = FLAG_DEAD;
+                    my_dead++;
+                } else {
+                    lane_store<KIND>(L, A.S, A.lin, slot);
+                }
                 has = false;
-                out = !dup;
             } else if (slice_steps >= A.P.step_slice) {
                 bool dup = false;
                 if (A.cache && L.ms_some)
                     dup = cache_insert_concurrent(A.cache, A.cache_cap, lane_sample<KIND>(L, A.S));
                 L.ms_some = false;
                 slice_steps = 0;
-                if (dup) has = false;
+                if (dup) {
+                    A.lin.flags[slot] = FLAG_DEAD;
+                    my_dead++;
+                    has = false;
+                }
             } else {
                 int r = lane_execute<KIND>(L, A.S, A.P, ev);
                 my_steps++;
@@ -145,12 +217,14 @@ __global__ void __launch_bounds__(256) k_persistent(PersistArgs A) {
                 tick++;
                 my_max = L.t_last > my_max ? L.t_last : my_max;
                 if (r == STEP_SPECIATED) {
+                    A.lin.flags[slot] = FLAG_DEAD;
+                    my_dead++;
                     has = false;
                     emit = true;
                 }
             }
         }
-        // wave-aggregated appends (events are rare, round-end write-outs are not)
+        // speciation events are rare: one wave-aggregated append
         uint64_t em = __ballot(emit);
         if (em) {
             uint32_t leader = __ffsll((unsigned long long)em) - 1;
@@ -159,31 +233,8 @@ __global__ void __launch_bounds__(256) k_persistent(PersistArgs A) {
             base = __shfl(base, leader);
             if (emit) A.events[base + __popcll(em & lanemask_lt())] = ev;
         }
-        uint64_t om = __ballot(out);
-        if (om) {
-            uint32_t leader = __ffsll((unsigned long long)om) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(&A.ctr->n_out, (unsigned long long)__popcll(om));
-            base = __shfl(base, leader);
-            if (out) lane_store<KIND>(L, A.S, A.out, base + __popcll(om & lanemask_lt()));
-        }
     }
-
-    // block reduction of steps and max time
-    __shared__ unsigned long long s_steps;
-    __shared__ unsigned long long s_max;
-    if (threadIdx.x == 0) {
-        s_steps = 0;
-        s_max = 0;
-    }
-    __syncthreads();
-    if (my_steps) atomicAdd(&s_steps, my_steps);
-    if (my_max > 0.0) atomicMax(&s_max, (unsigned long long)bits(my_max));
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (s_steps) atomicAdd(&A.ctr->steps, s_steps);
-        if (s_max) atomicMax(&A.ctr->max_time_bits, s_max);
-    }
+    block_flush(my_steps, my_max, my_dead, A.ctr);
 }
 
 // ---------------------------------------------------------------------------
@@ -191,7 +242,9 @@ __global__ void __launch_bounds__(256) k_persistent(PersistArgs A) {
 // requests at once.  Global work index w belongs to the request whose
 // [work_base, work_base + n_in) contains it; each lane keeps its request's
 // seed, speciation probability and water level in registers and refills from
-// the shared queue independently of which request the next lineage belongs to.
+// the shared queue whichever request the next lineage belongs to.  Per-request
+// step / time / death tallies are kept per lane and flushed when the lane
+// moves to another request.
 template <int KIND>
 __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
     __shared__ double s_ratio[RATIO_N];
@@ -199,99 +252,162 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
     load_ratio(s_ratio, A.P.ratio);
     for (int r = threadIdx.x; r < A.n_reqs; r += blockDim.x) s_base[r] = A.reqs[r].work_base;
     __syncthreads();
-    const unsigned lane = lane_id();
 
     Lane<KIND> L;
     L.have_next = false;
     L.ms_some = false;
     DevParams P = A.P;
     double level = 0.0;
-    int req = 0;
-    bool has = false, exhausted = false;
+    int req = -1;
+    bool has = false;
+    unsigned long long slot = 0;
+    WaveQueue q{0, 0, false, false};
     uint64_t slice_steps = 0, tick = 0;
-    unsigned long long lin_steps = 0;
-    double lin_max = 0.0;
+    unsigned long long acc_steps = 0, acc_dead = 0;
+    double acc_max = 0.0;
+    auto flush = [&](int r) {
+        if (r < 0) return;
+        ReqCounters *c = A.reqs[r].ctr;
+        if (acc_steps) atomicAdd(&c->steps, acc_steps);
+        if (acc_max > 0.0) atomicMax(&c->max_time_bits, (unsigned long long)bits(acc_max));
+        if (acc_dead) atomicAdd(&c->n_dead, acc_dead);
+        acc_steps = 0;
+        acc_dead = 0;
+        acc_max = 0.0;
+    };
 
     for (;;) {
-        if (!exhausted) {
+        if (!q.exhausted) {
             uint64_t need = __ballot(!has);
             if (need) {
-                uint32_t cnt = __popcll(need);
-                uint32_t leader = __ffsll((unsigned long long)need) - 1;
-                unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(A.head, (unsigned long long)cnt);
-                base = __shfl(base, leader);
-                if (base + cnt >= A.total_work) exhausted = true;
-                if (!has) {
-                    unsigned long long w = base + __popcll(need & lanemask_lt());
-                    if (w < A.total_work) {
-                        int lo = 0, hi = A.n_reqs - 1; // last request with work_base <= w
-                        while (lo < hi) {
-                            int mid = (lo + hi + 1) >> 1;
-                            if (s_base[mid] <= w) lo = mid;
-                            else hi = mid - 1;
+                unsigned long long w = wave_claim(q, need, !has, A.head, A.total_work, A.chunk);
+                if (w != ~0ull) {
+                    int lo = 0, hi = A.n_reqs - 1; // last request with work_base <= w
+                    while (lo < hi) {
+                        int mid = (lo + hi + 1) >> 1;
+                        if (s_base[mid] <= w) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const ServeReq &R = A.reqs[lo];
+                    unsigned long long s = w - s_base[lo];
+                    if (!(R.lin.flags[s] & FLAG_DEAD)) {
+                        if (lo != req) {
+                            flush(req);
+                            req = lo;
+                            P.seed = R.seed;
+                            P.nu = R.nu;
+                            level = R.level;
                         }
-                        req = lo;
-                        const ServeReq &R = A.reqs[req];
-                        P.seed = R.seed;
-                        P.nu = R.nu;
-                        level = R.level;
-                        lane_load<KIND>(L, A.S, R.in, w - s_base[req]);
+                        slot = s;
+                        lane_load<KIND>(L, A.S, R.lin, slot);
                         has = true;
                         slice_steps = 0;
                         tick = 0;
-                        lin_steps = 0;
-                        lin_max = 0.0;
                     }
                 }
             }
         }
-        if (!__any(has)) break;
+        if (!__any(has)) {
+            if (q.exhausted) break;
+            continue;
+        }
         if (has) {
+            const ServeReq &R = A.reqs[req];
             lane_peek<KIND>(L, P, s_ratio);
-            bool leave = false, write = false, speciated = false;
-            DevEvent ev;
             if (L.next_t >= level || (A.tick_steps && tick >= A.tick_steps)) {
-                const ServeReq &R = A.reqs[req];
                 bool dup = false;
                 if (R.cache && L.ms_some)
                     dup = cache_insert_concurrent(R.cache, R.cache_cap, lane_sample<KIND>(L, A.S));
-                leave = true;
-                write = !dup;
+                if (dup) {
+                    R.lin.flags[slot] = FLAG_DEAD;
+                    acc_dead++;
+                } else {
+                    lane_store<KIND>(L, A.S, R.lin, slot);
+                    if (L.next_t < level) R.ctr->n_below = 1; // the round continues next tick
+                }
+                has = false;
             } else if (slice_steps >= P.step_slice) {
-                const ServeReq &R = A.reqs[req];
                 bool dup = false;
                 if (R.cache && L.ms_some)
                     dup = cache_insert_concurrent(R.cache, R.cache_cap, lane_sample<KIND>(L, A.S));
                 L.ms_some = false;
                 slice_steps = 0;
-                if (dup) leave = true;
+                if (dup) {
+                    R.lin.flags[slot] = FLAG_DEAD;
+                    acc_dead++;
+                    has = false;
+                }
             } else {
+                DevEvent ev;
                 int r = lane_execute<KIND>(L, A.S, P, ev);
-                lin_steps++;
+                acc_steps++;
                 slice_steps++;
                 tick++;
-                lin_max = L.t_last;
+                acc_max = L.t_last > acc_max ? L.t_last : acc_max;
                 if (r == STEP_SPECIATED) {
-                    leave = true;
-                    speciated = true;
+                    R.events[atomicAdd(&R.ctr->n_events, 1ull)] = ev;
+                    R.lin.flags[slot] = FLAG_DEAD;
+                    acc_dead++;
+                    has = false;
                 }
-            }
-            if (leave) {
-                const ServeReq &R = A.reqs[req];
-                if (speciated) R.events[atomicAdd(&R.ctr->n_events, 1ull)] = ev;
-                if (write) {
-                    unsigned long long o = atomicAdd(&R.ctr->n_out, 1ull);
-                    lane_store<KIND>(L, A.S, R.out, o);
-                    if (L.next_t < level) atomicAdd(&R.ctr->n_below, 1ull);
-                }
-                if (lin_steps) {
-                    atomicAdd(&R.ctr->steps, lin_steps);
-                    atomicMax(&R.ctr->max_time_bits, (unsigned long long)bits(lin_max));
-                }
-                has = false;
             }
         }
+    }
+    flush(req);
+}
+
+// ---------------------------------------------------------------------------
+// Ordered compaction of a lineage SoA (drops dead slots, keeps slot order)
+constexpr int SC_THREADS = 256;
+constexpr int SC_ITEMS = 8;
+constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
+
+__global__ void __launch_bounds__(SC_THREADS) k_soa_count(const uint8_t *flags, uint64_t n, uint32_t *blk) {
+    __shared__ uint32_t c;
+    if (threadIdx.x == 0) c = 0;
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * SC_TILE;
+    uint32_t a = 0;
+    for (int j = 0; j < SC_ITEMS; j++) {
+        uint64_t q = base + (uint64_t)j * SC_THREADS + threadIdx.x;
+        if (q < n) a += !(flags[q] & FLAG_DEAD);
+    }
+    if (a) atomicAdd(&c, a);
+    __syncthreads();
+    if (threadIdx.x == 0) blk[blockIdx.x] = c;
+}
+
+__global__ void __launch_bounds__(SC_THREADS) k_soa_scatter(LineageSoA in, LineageSoA out, uint64_t n,
+                                                            const uint32_t *off) {
+    __shared__ uint32_t wcnt[SC_THREADS / 64];
+    __shared__ uint32_t run;
+    const unsigned lane = lane_id();
+    const unsigned wave = threadIdx.x / 64;
+    if (threadIdx.x == 0) run = off[blockIdx.x];
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * SC_TILE;
+    for (int j = 0; j < SC_ITEMS; j++) {
+        uint64_t q = base + (uint64_t)j * SC_THREADS + threadIdx.x;
+        bool alive = q < n && !(in.flags[q] & FLAG_DEAD);
+        uint64_t m = __ballot(alive);
+        if (lane == 0) wcnt[wave] = __popcll(m);
+        __syncthreads();
+        uint32_t pos = run + __popcll(m & lanemask_lt());
+        for (unsigned w = 0; w < wave; w++) pos += wcnt[w];
+        if (alive) {
+            out.ref[pos] = in.ref[q];
+            out.t_last[pos] = in.t_last[q];
+            out.x[pos] = in.x[q];
+            out.y[pos] = in.y[q];
+            out.idx[pos] = in.idx[q];
+            out.next_t[pos] = in.next_t[q];
+            out.next_st[pos] = in.next_st[q];
+            out.flags[pos] = in.flags[q];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (unsigned w = 0; w < SC_THREADS / 64; w++) run += wcnt[w];
+        __syncthreads();
     }
 }
 
@@ -548,9 +664,21 @@ constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 
-__global__ void k_event_keys(const DevEvent *ev, uint64_t n, unsigned long long *keys, uint32_t *vals,
+// Sort sizes may come from device memory (graph replays): n = *end - *start
+// and the events start at events + *start.
+struct SortSize {
+    uint64_t n;
+    const unsigned long long *end;
+    const unsigned long long *start;
+    __device__ __forceinline__ uint64_t get() const { return end ? (*end - (start ? *start : 0ull)) : n; }
+    __device__ __forceinline__ uint64_t offset() const { return start ? *start : 0ull; }
+};
+
+__global__ void k_event_keys(const DevEvent *ev0, SortSize sz, unsigned long long *keys, uint32_t *vals,
                              unsigned long long *minmax) {
     uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t n = sz.get();
+    const DevEvent *ev = ev0 + sz.offset();
     if (p >= n) return;
     // positive f64 (PositiveF64 total order) sorts like its bit pattern
     unsigned long long k = bits(ev[p].event_time) & 0x7fffffffffffffffull;
@@ -560,9 +688,10 @@ __global__ void k_event_keys(const DevEvent *ev, uint64_t n, unsigned long long 
     atomicMax(&minmax[1], k);
 }
 
-__global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const unsigned long long *keys, uint64_t n,
+__global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const unsigned long long *keys, SortSize sz,
                                                            int shift, uint32_t *hist, uint32_t nb) {
     __shared__ uint32_t h[256];
+    const uint64_t n = sz.get();
     h[threadIdx.x] = 0;
     __syncthreads();
     uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
@@ -577,9 +706,10 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const unsigned long l
 __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const unsigned long long *kin,
                                                               const uint32_t *vin,
                                                               unsigned long long *kout, uint32_t *vout,
-                                                              uint64_t n, int shift,
+                                                              SortSize sz, int shift,
                                                               const uint32_t *offsets, uint32_t nb) {
     constexpr int W = RS_THREADS / 64;
+    const uint64_t n = sz.get();
     __shared__ uint32_t running[256];
     __shared__ uint32_t wcnt[W][256];
     const unsigned lane = lane_id();
@@ -632,8 +762,10 @@ __device__ __forceinline__ bool event_less(const DevEvent &a, const DevEvent &b)
 
 // Equal event times are rare (only duplicate lineages share them): order
 // each run of equal keys by the full PackedEvent order.
-__global__ void k_fix_ties(const unsigned long long *keys, uint32_t *vals, const DevEvent *ev, uint64_t n) {
+__global__ void k_fix_ties(const unsigned long long *keys, uint32_t *vals, const DevEvent *ev0, SortSize sz) {
     uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t n = sz.get();
+    const DevEvent *ev = ev0 + sz.offset();
     if (p + 1 >= n) return;
     if (keys[p] != keys[p + 1]) return;
     if (p > 0 && keys[p - 1] == keys[p]) return; // not the run start
@@ -650,9 +782,10 @@ __global__ void k_fix_ties(const unsigned long long *keys, uint32_t *vals, const
     }
 }
 
-__global__ void k_gather_events(const DevEvent *ev, const uint32_t *vals, DevEvent *out, uint64_t n) {
+__global__ void k_gather_events(const DevEvent *ev0, const uint32_t *vals, DevEvent *out, SortSize sz) {
     uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) out[p] = ev[vals[p]];
+    const DevEvent *ev = ev0 + sz.offset();
+    if (p < sz.get()) out[p] = ev[vals[p]];
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
@@ -666,6 +799,7 @@ __global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
 __global__ void k_reset_counters(Counters *c) {
     c->head = 0;
     c->n_out = 0;
+    c->n_dead = 0;
 }
 __global__ void k_set_minmax(unsigned long long *mm) {
     mm[0] = ~0ull;
@@ -761,18 +895,36 @@ void launch_compact(const uint8_t *status, const uint32_t *pass, uint64_t m, uin
 
 uint32_t sort_blocks(uint64_t n) { return (uint32_t)((n + RS_TILE - 1) / RS_TILE); }
 
+static void sort_passes(const DevEvent *events, SortSize sz, uint64_t cap, DevEvent *out, SortBuffers &sb,
+                        hipStream_t stream, int last_digit) {
+    uint32_t nb = sort_blocks(cap);
+    unsigned g = grid_for(cap, 256);
+    int cur = 0;
+    for (int d = 0; d <= last_digit; d++) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sz, 8 * d, sb.hist, nb);
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, stream, sb.hist, (uint64_t)256 * nb);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sb.vals[cur],
+                           sb.keys[cur ^ 1], sb.vals[cur ^ 1], sz, 8 * d, sb.hist, nb);
+        cur ^= 1;
+    }
+    hipLaunchKernelGGL(k_fix_ties, dim3(g), dim3(256), 0, stream, sb.keys[cur], sb.vals[cur], events, sz);
+    hipLaunchKernelGGL(k_gather_events, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], out, sz);
+}
+
 void sort_events(const DevEvent *events, uint64_t n, DevEvent *out, SortBuffers &sb, hipStream_t stream,
                  bool fixed_passes) {
     if (n == 0) return;
-    uint32_t nb = sort_blocks(n);
+    SortSize sz{n, nullptr, nullptr};
     hipLaunchKernelGGL(k_set_minmax, dim3(1), dim3(1), 0, stream, sb.minmax);
-    hipLaunchKernelGGL(k_event_keys, dim3(grid_for(n, 256)), dim3(256), 0, stream, events, n, sb.keys[0],
+    hipLaunchKernelGGL(k_event_keys, dim3(grid_for(n, 256)), dim3(256), 0, stream, events, sz, sb.keys[0],
                        sb.vals[0], sb.minmax);
-    int first = 0, last = 7;
+    int last = 7;
     if (!fixed_passes) {
         unsigned long long mm[2];
         (void)hipMemcpyAsync(mm, sb.minmax, sizeof(mm), hipMemcpyDeviceToHost, stream);
         (void)hipStreamSynchronize(stream);
+        // digits above the highest differing bit of min and max are equal
+        // for every key (every key lies between min and max)
         unsigned long long diff = mm[0] ^ mm[1];
         last = -1;
         for (int d = 7; d >= 0; d--)
@@ -780,19 +932,33 @@ void sort_events(const DevEvent *events, uint64_t n, DevEvent *out, SortBuffers 
                 last = d;
                 break;
             }
-        // digits above the highest differing bit are equal for all keys
-        // (every key lies between min and max)
     }
-    int cur = 0;
-    for (int d = first; d <= last; d++) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], n, 8 * d, sb.hist, nb);
-        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, stream, sb.hist, (uint64_t)256 * nb);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sb.vals[cur],
-                           sb.keys[cur ^ 1], sb.vals[cur ^ 1], n, 8 * d, sb.hist, nb);
-        cur ^= 1;
+    sort_passes(events, sz, n, out, sb, stream, last);
+}
+
+void sort_events_device_sized(const DevEvent *events, const unsigned long long *end,
+                              const unsigned long long *start, uint64_t cap, DevEvent *out, SortBuffers &sb,
+                              hipStream_t stream) {
+    if (cap == 0) return;
+    SortSize sz{0, end, start};
+    hipLaunchKernelGGL(k_set_minmax, dim3(1), dim3(1), 0, stream, sb.minmax);
+    hipLaunchKernelGGL(k_event_keys, dim3(grid_for(cap, 256)), dim3(256), 0, stream, events, sz, sb.keys[0],
+                       sb.vals[0], sb.minmax);
+    sort_passes(events, sz, cap, out, sb, stream, 7);
+}
+
+uint32_t soa_compact_blocks(uint64_t n) { return (uint32_t)((n + SC_TILE - 1) / SC_TILE); }
+
+void launch_soa_compact(const LineageSoA &in, const LineageSoA &out, uint64_t n, uint32_t *blk,
+                        hipStream_t stream) {
+    uint32_t nb = soa_compact_blocks(n);
+    if (!nb) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, blk, 0u, (uint64_t)1);
+        return;
     }
-    hipLaunchKernelGGL(k_fix_ties, dim3(grid_for(n, 256)), dim3(256), 0, stream, sb.keys[cur], sb.vals[cur], events, n);
-    hipLaunchKernelGGL(k_gather_events, dim3(grid_for(n, 256)), dim3(256), 0, stream, events, sb.vals[cur], out, n);
+    hipLaunchKernelGGL(k_soa_count, dim3(nb), dim3(SC_THREADS), 0, stream, in.flags, n, blk);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, stream, blk, (uint64_t)nb);
+    hipLaunchKernelGGL(k_soa_scatter, dim3(nb), dim3(SC_THREADS), 0, stream, in, out, n, blk);
 }
 
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream) {

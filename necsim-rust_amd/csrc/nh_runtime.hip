@@ -341,7 +341,9 @@ struct nh_job {
 
     LineageBuffers lin[2];
     int cur = 0;
-    uint64_t cur_n = 0;
+    uint64_t cur_n = 0; // slots of the current buffer (plain: queue length)
+    uint64_t alive = 0; // live lineages (persistent: cur_n minus dead slots)
+    uint32_t *d_blk = nullptr; // SoA compaction block counts
     Counters *d_ctr = nullptr;
     RoundDesc *d_round = nullptr;
     DevEvent *d_events = nullptr, *d_sorted = nullptr;
@@ -365,6 +367,7 @@ struct nh_job {
     int blocks = 0;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr}; // persistent round, input buffer 0 / 1
     double kernel_ms = 0.0;
     uint64_t timed = 0;
     RoundDesc round_host{};
@@ -376,9 +379,11 @@ struct nh_job {
         dfree(sb.hist); dfree(sb.minmax); dfree(d_cache);
         dfree(d_S); dfree(d_F); dfree(d_pass[0]); dfree(d_pass[1]); dfree(d_status);
         dfree(s_u); dfree(s_t); dfree(s_x); dfree(s_y); dfree(s_i); dfree(s_bucket);
-        dfree(d_head); dfree(d_link); dfree(cb.blk_a); dfree(cb.blk_b); dfree(cb.counts);
+        dfree(d_head); dfree(d_link); dfree(cb.blk_a); dfree(cb.blk_b); dfree(cb.counts); dfree(d_blk);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        for (auto &g : graph)
+            if (g) (void)hipGraphExecDestroy(g);
     }
 };
 
@@ -492,9 +497,11 @@ static int job_start(nh_job *j) {
     NH_TRY(upload_lineages(j->pending, j->lin[0], st));
     j->cur = 0;
     j->cur_n = n;
+    j->alive = n;
     j->threads = p.block_size ? (int)p.block_size : 256;
     if (p.form == NH_FORM_PERSISTENT) {
         NH_TRY(j->lin[1].alloc(n));
+        NH_TRY(dalloc(&j->d_blk, soa_compact_blocks(n) + 1));
         int per_cu = persistent_max_blocks_per_cu(j->tables->kind, j->threads);
         if (p.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)p.blocks_per_cu);
         j->blocks = per_cu * (int)j->ctx->cus;
@@ -668,33 +675,78 @@ static int plain_round(nh_job *j) {
     }
     std::swap(j->d_S, j->d_F);
     j->cur_n = f_len;
+    j->alive = f_len;
     return NH_OK;
 }
 
-static int persistent_round(nh_job *j) {
+// ordered compaction of the persistent form's live lineages into the other buffer
+static int compact_lineages(nh_job *j) {
     hipStream_t st = j->ctx->stream;
+    launch_soa_compact(j->lin[j->cur].s, j->lin[j->cur ^ 1].s, j->cur_n, j->d_blk, st);
+    NH_HIP(hipGetLastError());
+    uint32_t live = 0;
+    NH_HIP(hipMemcpyAsync(&live, j->d_blk + soa_compact_blocks(j->cur_n), 4, hipMemcpyDeviceToHost, st));
+    NH_HIP(hipStreamSynchronize(st));
+    if (live != j->alive) return set_error(NH_E_STATE, "compaction found %u live lineages, expected %llu", live, (unsigned long long)j->alive);
+    j->cur ^= 1;
+    j->cur_n = live;
+    return NH_OK;
+}
+
+static PersistArgs persist_args(nh_job *j, int cur) {
     PersistArgs a{};
     a.S = j->tables->dev;
     a.P = j->P;
-    a.in = j->lin[j->cur].s;
-    a.out = j->lin[j->cur ^ 1].s;
+    a.lin = j->lin[cur].s;
     a.round = j->d_round;
     a.cache = j->d_cache;
     a.cache_cap = j->cache_cap;
     a.events = j->d_events;
     a.ctr = j->d_ctr;
     a.tick_steps = 0;
-    launch_reset_counters(j->d_ctr, st);
-    NH_HIP(hipEventRecord(j->ev0, st));
-    launch_persistent(j->tables->kind, a, j->blocks, j->threads, st);
-    NH_HIP(hipGetLastError());
-    NH_HIP(hipEventRecord(j->ev1, st));
+    return a;
+}
+
+// One round of the persistent form.  With use_graph, the round's step ->
+// event-key -> radix-sort -> gather cycle is captured once per input buffer
+// and replayed; the round's level, size and event offset are read from
+// device memory (RoundDesc, Counters).  Sets *sorted when the round's events
+// are already sorted into d_sorted.
+static int persistent_round(nh_job *j, bool *sorted) {
+    hipStream_t st = j->ctx->stream;
+    *sorted = false;
+    if (j->p.use_graph) {
+        int g = j->cur;
+        if (!j->graph[g]) {
+            hipGraph_t graph = nullptr;
+            NH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            launch_reset_counters(j->d_ctr, st);
+            launch_persistent(j->tables->kind, persist_args(j, g), j->blocks, j->threads, st);
+            sort_events_device_sized(j->d_events, &j->d_ctr->n_events, &j->d_round->ev_start, j->n_initial,
+                                     j->d_sorted, j->sb, st);
+            NH_HIP(hipStreamEndCapture(st, &graph));
+            hipError_t e = hipGraphInstantiate(&j->graph[g], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return set_error(NH_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        }
+        NH_HIP(hipEventRecord(j->ev0, st));
+        NH_HIP(hipGraphLaunch(j->graph[g], st));
+        NH_HIP(hipEventRecord(j->ev1, st));
+        *sorted = true;
+    } else {
+        launch_reset_counters(j->d_ctr, st);
+        NH_HIP(hipEventRecord(j->ev0, st));
+        launch_persistent(j->tables->kind, persist_args(j, j->cur), j->blocks, j->threads, st);
+        NH_HIP(hipGetLastError());
+        NH_HIP(hipEventRecord(j->ev1, st));
+    }
     j->launches++;
     Counters c;
     NH_TRY(read_counters(j, c));
     account_kernel_time(j);
-    j->cur ^= 1;
-    j->cur_n = c.n_out;
+    j->alive -= c.n_dead;
+    // drop dead slots once they are an eighth of the buffer
+    if ((j->cur_n - j->alive) * 8 >= j->cur_n && j->cur_n > j->alive) NH_TRY(compact_lineages(j));
     return NH_OK;
 }
 
@@ -710,7 +762,7 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
     NH_HIP(hipSetDevice(j->ctx->device));
     if (!j->started) NH_TRY(job_start(j));
     const nh_job_params &p = j->p;
-    auto done = [&]() { return j->cur_n == 0 || (p.has_pause_before && !(j->level_time < p.pause_before)); };
+    auto done = [&]() { return j->alive == 0 || (p.has_pause_before && !(j->level_time < p.pause_before)); };
     if (!j->finished && done()) j->finished = true;
     if (j->finished) {
         if (finished) *finished = 1;
@@ -719,15 +771,17 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
     hipStream_t st = j->ctx->stream;
     // speciation-only reporting: total rate = sum of lambda * nu over the slow
     // lineages, folded in order (monolithic/mod.rs:150-164)
-    double total_rate = repeated_sum(p.turnover_rate * p.speciation_probability, j->cur_n);
+    double total_rate = repeated_sum(p.turnover_rate * p.speciation_probability, j->alive);
     j->level_time = j->level_time + (double)j->event_slice / total_rate;
     if (p.has_pause_before && p.pause_before < j->level_time) j->level_time = p.pause_before;
-    j->round_host = RoundDesc{j->level_time, (unsigned long long)j->cur_n};
+    j->round_host = RoundDesc{j->level_time, (unsigned long long)j->cur_n, (unsigned long long)j->n_events,
+                              claim_chunk(j->cur_n, (unsigned long long)j->blocks * j->threads)};
     NH_HIP(hipMemcpyAsync(j->d_round, &j->round_host, sizeof(RoundDesc), hipMemcpyHostToDevice, st));
 
     uint64_t ev_start = j->n_events;
+    bool sorted = false;
     if (p.form == NH_FORM_PLAIN) NH_TRY(plain_round(j));
-    else NH_TRY(persistent_round(j));
+    else NH_TRY(persistent_round(j, &sorted));
 
     Counters c;
     NH_TRY(read_counters(j, c));
@@ -743,7 +797,7 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
     // the live water-level reporter sorts each round's events (live.rs:88-115)
     uint64_t ne = j->n_events - ev_start;
     if (ne) {
-        sort_events(j->d_events + ev_start, ne, j->d_sorted, j->sb, st, false);
+        if (!sorted) sort_events(j->d_events + ev_start, ne, j->d_sorted, j->sb, st, false);
         NH_HIP(hipGetLastError());
         size_t old = j->ready.size();
         j->ready.resize(old + ne);
@@ -781,11 +835,11 @@ extern "C" int nh_job_drain_events(nh_job *j, nh_speciation_event *buf, size_t c
 
 extern "C" int nh_job_outcome(const nh_job *j, nh_outcome *o) {
     if (!j || !o) return set_error(NH_E_INVALID, "null argument");
-    o->paused = j->started ? (j->cur_n > 0) : (j->pending.size() > 0);
+    o->paused = j->started ? (j->alive > 0) : (j->pending.size() > 0);
     o->time = j->max_time;
     o->steps = j->steps;
     o->n_events = j->n_events;
-    o->n_remaining = j->started ? j->cur_n : j->pending.size();
+    o->n_remaining = j->started ? j->alive : j->pending.size();
     o->rounds = j->rounds;
     o->launches = j->launches;
     return NH_OK;
@@ -798,10 +852,11 @@ extern "C" int nh_job_take_lineages(nh_job *j, nh_lineage *buf, size_t cap, size
         if (buf) std::memcpy(buf, j->pending.data(), std::min(cap, j->pending.size()) * sizeof(nh_lineage));
         return NH_OK;
     }
-    *n_out = j->cur_n;
+    *n_out = j->alive;
     if (!buf) return NH_OK;
-    if (cap < j->cur_n) return set_error(NH_E_CAPACITY, "lineage buffer holds %zu of %llu", cap, (unsigned long long)j->cur_n);
+    if (cap < j->alive) return set_error(NH_E_CAPACITY, "lineage buffer holds %zu of %llu", cap, (unsigned long long)j->alive);
     NH_HIP(hipSetDevice(j->ctx->device));
+    if (j->p.form == NH_FORM_PERSISTENT && j->cur_n != j->alive) NH_TRY(compact_lineages(j));
     std::vector<nh_lineage> v;
     if (j->p.form == NH_FORM_PLAIN) NH_TRY(download_lineages(j->lin[0], j->d_S, j->cur_n, v, j->ctx->stream));
     else NH_TRY(download_lineages(j->lin[j->cur], nullptr, j->cur_n, v, j->ctx->stream));
@@ -821,5 +876,56 @@ extern "C" int nh_job_kernel_time(const nh_job *j, double *total_ms, uint64_t *l
     if (!j) return set_error(NH_E_INVALID, "null job");
     if (total_ms) *total_ms = j->kernel_ms;
     if (launches) *launches = j->timed;
+    return NH_OK;
+}
+
+// ===========================================================================
+// autotuner
+extern "C" int nh_autotune(nh_ctx *ctx, const nh_tables *tables, const nh_job_params *base,
+                           const nh_lineage *sample, size_t n, double window, nh_job_params *tuned,
+                           double *best_ms) {
+    if (!ctx || !tables || !base || !tuned || (!sample && n) || !(window > 0.0))
+        return set_error(NH_E_INVALID, "nh_autotune: bad argument");
+    std::vector<uint64_t> slices;
+    if (base->step_slice) slices.push_back(base->step_slice);
+    else slices = {5, 10, 20, 50, 150};
+    double best = 1e300;
+    nh_job_params winner = *base;
+    winner.form = NH_FORM_PERSISTENT;
+    for (uint32_t threads : {64u, 128u, 256u}) {
+        int max_per_cu = persistent_max_blocks_per_cu(tables->kind, (int)threads);
+        std::vector<uint32_t> residency = {(uint32_t)max_per_cu};
+        if (max_per_cu >= 2) residency.push_back((uint32_t)(max_per_cu / 2));
+        for (uint32_t per_cu : residency) {
+            for (uint64_t slice : slices) {
+                nh_job_params p = *base;
+                p.form = NH_FORM_PERSISTENT;
+                p.block_size = threads;
+                p.blocks_per_cu = per_cu;
+                p.step_slice = slice;
+                p.has_pause_before = 1;
+                p.pause_before = window;
+                p.use_graph = 0;
+                nh_job *j = nullptr;
+                NH_TRY(nh_job_create(ctx, tables, &p, &j));
+                int rc = nh_job_add_lineages(j, sample, n);
+                if (rc == NH_OK) rc = nh_job_start(j);
+                double t0 = now_s();
+                if (rc == NH_OK) rc = nh_job_simulate(j);
+                double ms = (now_s() - t0) * 1e3;
+                nh_job_destroy(j);
+                if (rc != NH_OK) return rc;
+                if (ms < best) {
+                    best = ms;
+                    winner = p;
+                }
+            }
+        }
+    }
+    winner.has_pause_before = base->has_pause_before;
+    winner.pause_before = base->pause_before;
+    winner.use_graph = base->use_graph;
+    *tuned = winner;
+    if (best_ms) *best_ms = best;
     return NH_OK;
 }

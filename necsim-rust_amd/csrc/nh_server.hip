@@ -86,7 +86,8 @@ struct Req {
     // device state
     Soa soa[2];
     int cur = 0;
-    uint64_t n_in = 0;
+    uint64_t n_in = 0;  // slots of the current buffer
+    uint64_t alive = 0; // live lineages
     DevEvent *d_events = nullptr;
     uint64_t *d_cache = nullptr;
     double level = 0.0;
@@ -130,6 +131,7 @@ struct nh_server {
     ServeReq *d_reqs = nullptr;
     ReqCounters *d_ctr = nullptr; // SERVE_MAX_REQS
     unsigned long long *d_head = nullptr;
+    uint32_t *d_blk = nullptr; // SoA compaction block counts
     DevEvent *d_sorted = nullptr;
     SortBuffers sb{};
     std::vector<ServeReq> h_reqs;
@@ -139,7 +141,7 @@ struct nh_server {
         for (Req *r : active) { r->release(); delete r; }
         for (Req *r : finished) { r->release(); delete r; }
         for (auto &kv : polled) { kv.second->release(); delete kv.second; }
-        dfree(d_ratio); dfree(d_reqs); dfree(d_ctr); dfree(d_head); dfree(d_sorted);
+        dfree(d_ratio); dfree(d_reqs); dfree(d_ctr); dfree(d_head); dfree(d_blk); dfree(d_sorted);
         dfree(sb.keys[0]); dfree(sb.keys[1]); dfree(sb.vals[0]); dfree(sb.vals[1]);
         dfree(sb.hist); dfree(sb.minmax);
     }
@@ -170,6 +172,7 @@ extern "C" int nh_server_create(nh_ctx *ctx, const nh_tables *tables, const nh_s
     if ((rc = dalloc(&s->d_reqs, SERVE_MAX_REQS))) return fail(rc);
     if ((rc = dalloc(&s->d_ctr, SERVE_MAX_REQS))) return fail(rc);
     if ((rc = dalloc(&s->d_head, 1))) return fail(rc);
+    if ((rc = dalloc(&s->d_blk, soa_compact_blocks(cap) + 1))) return fail(rc);
     if ((rc = dalloc(&s->d_sorted, cap))) return fail(rc);
     s->sb.capacity = cap;
     if ((rc = dalloc(&s->sb.keys[0], cap)) || (rc = dalloc(&s->sb.keys[1], cap)) ||
@@ -278,6 +281,7 @@ static int admit(nh_server *s, Req *r) {
     r->lineages.clear();
     r->lineages.shrink_to_fit();
     r->n_in = n;
+    r->alive = n;
     r->cur = 0;
     r->result.admit_s = mono_s();
     s->used_bytes += r->footprint;
@@ -286,12 +290,12 @@ static int admit(nh_server *s, Req *r) {
 }
 
 static bool request_finished(const Req *r) {
-    return r->n_in == 0 || (r->p.has_pause_before && !(r->level < r->p.pause_before));
+    return r->alive == 0 || (r->p.has_pause_before && !(r->level < r->p.pause_before));
 }
 
 // open the next water-level round of a request (monolithic/mod.rs:139-174)
 static void open_round(nh_server *s, Req *r) {
-    double total_rate = repeated_sum(r->p.turnover_rate * r->p.speciation_probability, r->n_in);
+    double total_rate = repeated_sum(r->p.turnover_rate * r->p.speciation_probability, r->alive);
     r->level = r->level + (double)r->event_slice / total_rate;
     if (r->p.has_pause_before && r->p.pause_before < r->level) r->level = r->p.pause_before;
     r->round_open = true;
@@ -301,7 +305,7 @@ static void open_round(nh_server *s, Req *r) {
 static int retire(nh_server *s, Req *r) {
     r->done = true;
     r->result.status = NH_OK;
-    r->result.paused = r->n_in > 0;
+    r->result.paused = r->alive > 0;
     r->result.time = r->max_time;
     r->result.steps = r->steps;
     r->result.n_events = r->events.size();
@@ -342,8 +346,7 @@ extern "C" int nh_server_tick(nh_server *s, uint64_t *n_active, uint64_t *n_queu
         d.seed = r->p.rng_seed;
         d.nu = r->p.speciation_probability;
         d.level = r->level;
-        d.in = r->soa[r->cur].s;
-        d.out = r->soa[r->cur ^ 1].s;
+        d.lin = r->soa[r->cur].s;
         d.events = r->d_events;
         d.cache = r->d_cache;
         d.cache_cap = r->cache_cap;
@@ -352,6 +355,7 @@ extern "C" int nh_server_tick(nh_server *s, uint64_t *n_active, uint64_t *n_queu
         d.ctr = s->d_ctr + k;
         total += r->n_in;
         s->h_ctr[k] = ReqCounters{0, 0, r->n_events, 0, 0, {0, 0, 0}};
+        if (r->alive == 0) d.n_in = 0;
     }
     if (nr > 0 && total > 0) {
         NH_HIP(hipMemcpyAsync(s->d_reqs, s->h_reqs.data(), nr * sizeof(ServeReq), hipMemcpyHostToDevice, st));
@@ -365,6 +369,7 @@ extern "C" int nh_server_tick(nh_server *s, uint64_t *n_active, uint64_t *n_queu
         a.total_work = total;
         a.head = s->d_head;
         a.tick_steps = s->prm.tick_steps;
+        a.chunk = claim_chunk(total, (unsigned long long)s->blocks * s->threads);
         launch_serve(s->kind, a, s->blocks, s->threads, st);
         NH_HIP(hipGetLastError());
         NH_HIP(hipMemcpyAsync(s->h_ctr.data(), s->d_ctr, nr * sizeof(ReqCounters), hipMemcpyDeviceToHost, st));
@@ -373,20 +378,29 @@ extern "C" int nh_server_tick(nh_server *s, uint64_t *n_active, uint64_t *n_queu
     std::vector<Req *> still;
     for (int k = 0; k < nr; k++) {
         Req *r = s->active[k];
-        if (r->n_in > 0) {
-            const ReqCounters &c = s->h_ctr[k];
-            r->cur ^= 1;
-            r->n_in = c.n_out;
-            r->n_events = c.n_events;
-            r->steps += c.steps;
-            if (c.max_time_bits && from_bits(c.max_time_bits) > r->max_time) r->max_time = from_bits(c.max_time_bits);
-            if (c.n_below > 0) { // the round continues next tick
-                still.push_back(r);
-                continue;
-            }
+        const ReqCounters &c = s->h_ctr[k];
+        r->alive -= c.n_dead;
+        r->n_events = c.n_events;
+        r->steps += c.steps;
+        if (c.max_time_bits && from_bits(c.max_time_bits) > r->max_time) r->max_time = from_bits(c.max_time_bits);
+        if (c.n_below && r->alive > 0) { // the round continues next tick
+            still.push_back(r);
+            continue;
         }
-        // the round is complete: sort its events (live water-level reporter)
+        // the round is complete: drop dead slots, sort the round's events
+        // (live water-level reporter)
         r->round_open = false;
+        if (r->n_in > r->alive) {
+            launch_soa_compact(r->soa[r->cur].s, r->soa[r->cur ^ 1].s, r->n_in, s->d_blk, st);
+            NH_HIP(hipGetLastError());
+            uint32_t live = 0;
+            NH_HIP(hipMemcpyAsync(&live, s->d_blk + soa_compact_blocks(r->n_in), 4, hipMemcpyDeviceToHost, st));
+            NH_HIP(hipStreamSynchronize(st));
+            if (live != r->alive) return set_error(NH_E_STATE, "request %llu: %u live lineages, expected %llu",
+                                                  (unsigned long long)r->id, live, (unsigned long long)r->alive);
+            r->cur ^= 1;
+            r->n_in = live;
+        }
         uint64_t ne = r->n_events - r->round_event_start;
         if (ne) {
             sort_events(r->d_events + r->round_event_start, ne, s->d_sorted, s->sb, st, false);

@@ -97,6 +97,8 @@ SIGNATURES = {
     "nh_job_take_lineages": (ctypes.c_int, [c_vp, c_vp, c_sz, ctypes.POINTER(c_sz)]),
     "nh_job_destroy": (ctypes.c_int, [c_vp]),
     "nh_job_kernel_time": (ctypes.c_int, [c_vp, ctypes.POINTER(c_f64), ctypes.POINTER(c_u64)]),
+    "nh_autotune": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(JobParams), c_vp, c_sz, c_f64,
+                                   ctypes.POINTER(JobParams), ctypes.POINTER(c_f64)]),
     "nh_server_create": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(ServerParams), ctypes.POINTER(c_vp)]),
     "nh_server_submit": (ctypes.c_int, [c_vp, ctypes.POINTER(Request), ctypes.POINTER(c_u64)]),
     "nh_server_tick": (ctypes.c_int, [c_vp, ctypes.POINTER(c_u64), ctypes.POINTER(c_u64)]),

@@ -305,18 +305,7 @@ class Job:
 
     def __init__(self, ctx, tables, args: IndependentArguments, rng: WyHash, speciation_probability,
                  turnover_rate, pause_before=None, resume_after=0.0):
-        p = _ffi.JobParams(rng_seed=rng.seed, speciation_probability=speciation_probability,
-                           turnover_rate=turnover_rate, delta_t=args.delta_t,
-                           step_slice=args.step_slice, dedup_mode=args.dedup_cache.mode,
-                           dedup_factor=args.dedup_cache.factor,
-                           dedup_capacity=args.dedup_cache.capacity,
-                           event_slice_mode=args.event_slice.mode,
-                           event_slice_factor=args.event_slice.factor,
-                           event_slice_capacity=args.event_slice.capacity,
-                           has_pause_before=0 if pause_before is None else 1,
-                           pause_before=0.0 if pause_before is None else pause_before,
-                           resume_after=resume_after, form=args.form, block_size=args.block_size,
-                           blocks_per_cu=args.blocks_per_cu, use_graph=1 if args.use_graph else 0)
+        p = _job_params(args, rng, speciation_probability, turnover_rate, pause_before, resume_after)
         h = ctypes.c_void_p()
         check(lib().nh_job_create(ctx.handle, tables.handle, ctypes.byref(p), ctypes.byref(h)))
         self.handle = h
@@ -368,6 +357,38 @@ class Job:
 
     def __del__(self):
         self.close()
+
+
+def _job_params(args, rng, speciation_probability, turnover_rate, pause_before=None, resume_after=0.0):
+    return _ffi.JobParams(rng_seed=rng.seed, speciation_probability=speciation_probability,
+                          turnover_rate=turnover_rate, delta_t=args.delta_t,
+                          step_slice=args.step_slice, dedup_mode=args.dedup_cache.mode,
+                          dedup_factor=args.dedup_cache.factor,
+                          dedup_capacity=args.dedup_cache.capacity,
+                          event_slice_mode=args.event_slice.mode,
+                          event_slice_factor=args.event_slice.factor,
+                          event_slice_capacity=args.event_slice.capacity,
+                          has_pause_before=0 if pause_before is None else 1,
+                          pause_before=0.0 if pause_before is None else pause_before,
+                          resume_after=resume_after, form=args.form, block_size=args.block_size,
+                          blocks_per_cu=args.blocks_per_cu, use_graph=1 if args.use_graph else 0)
+
+
+def autotune(ctx, tables, args: IndependentArguments, rng: WyHash, cogs, sample, window,
+             tune_step_slice=True):
+    """nh_autotune: the fastest persistent launch shape (and step slice) for a
+    window of the workload on `sample`; returns (tuned IndependentArguments, ms)."""
+    import dataclasses
+    base = _job_params(dataclasses.replace(args, step_slice=0 if tune_step_slice else args.step_slice),
+                       rng, cogs.speciation_probability, cogs.turnover_rate)
+    sample = np.ascontiguousarray(sample, dtype=_ffi.LINEAGE_DTYPE)
+    out = _ffi.JobParams()
+    ms = ctypes.c_double(0)
+    check(lib().nh_autotune(ctx.handle, tables.handle, ctypes.byref(base), ptr(sample), len(sample),
+                            window, ctypes.byref(out), ctypes.byref(ms)))
+    return dataclasses.replace(args, form=PERSISTENT, step_slice=int(out.step_slice),
+                               block_size=int(out.block_size),
+                               blocks_per_cu=int(out.blocks_per_cu)), ms.value
 
 
 class HipIndependentAlgorithm:

@@ -138,6 +138,19 @@ def test_config1_dedup_persistent_invariants(config1):
     assert out.steps <= ref["none"]["steps"]
 
 
+def test_config1_graph_replays_match_eager(config1):
+    announce("config1 persistent form, rounds replayed from hipGraphs")
+    cogs, orc, lin, ref = config1
+    # a small event slice gives many water-level rounds (the dedup-None output
+    # does not depend on it), so both captured graphs are replayed repeatedly
+    few = nh.EventSlice.Relative(0.05)
+    out, ev = hip_run(cogs, lin, 42, PERSISTENT, dedup="none", use_graph=True, event_slice=few)
+    assert out.rounds >= 6
+    assert_exact(ref["none"], out, ev, "config1 graph")
+    out, ev = hip_run(cogs, lin, 42, PERSISTENT, dedup=1.0, use_graph=True, event_slice=few)
+    assert_invariants(ev, ref["none"]["events"], ref[1.0]["events"], "config1 graph dedup")
+
+
 # ---------------------------------------------------------------------------
 # config 2: almost-infinite Gaussian sigma=10, circle r=100, nu 1e-4, seed 7
 @pytest.fixture(scope="module")
@@ -277,3 +290,17 @@ def test_edge_general_delta_t_and_step_slice(config1):
     r = oracle_run(orc, sub, 42, 1e-3, dedup=1.0, delta_t=3.0, step_slice=1)
     out, ev = hip_run(cogs, sub, 42, PLAIN, dedup=1.0, delta_t=3.0, step_slice=1)
     assert_exact(r, out, ev, "delta_t 3, step_slice 1")
+
+
+def test_autotuned_persistent_form_keeps_parity(config2):
+    announce("autotune on a config2 sample, then a tuned run")
+    cogs, orc, lin = config2
+    ctx = nh.Context.get(0)
+    tables = nh.Tables(ctx, cogs)
+    args, ms = nh.autotune(ctx, tables, nh.IndependentArguments(dedup_cache=nh.DedupCache.none()),
+                           nh.WyHash.seed_from_u64(7), cogs, lin[::4], 10.0)
+    assert args.block_size in (64, 128, 256) and args.step_slice in (5, 10, 20, 50, 150) and ms > 0
+    ref = oracle_run(orc, lin, 7, 1e-4, dedup="none", pause_before=40.0)
+    out, ev = hip_run(cogs, lin, 7, PERSISTENT, dedup="none", pause_before=40.0,
+                      step_slice=args.step_slice)
+    assert_exact(ref, out, ev, "tuned")
