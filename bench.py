@@ -3,18 +3,21 @@
 
 Headline workload ("config4-window"): the almost-infinite scenario with
 Gaussian dispersal sigma=50, every lineage of a circle of radius 2000
-(12,566,345 lineages), speciation 1e-6, Seed(42), run with the reference's
-CPU-Independent defaults (delta_t 2, step_slice 10, dedup Relative(1.0),
-event_slice Relative(2.0)) up to pause_before = T (default 40, about 20
-events per lineage before dedup).  One bench step simulates the whole
-window from the initial lineages in the persistent form; lineages are
-already resident in HBM when the timed region starts.
+(12,566,345 lineages), speciation 1e-6, Seed(42), delta_t 2, DedupCache::None
+(so the event stream is bit-identical between the forms and the CPU path),
+up to pause_before = T (default 2000: about 1000 events per lineage,
+1.26e10 lineage-steps).  One bench step simulates the whole window from the
+initial lineages with the persistent form (autotuned launch shape, each
+water-level round one hipGraph); lineages are already resident in HBM when
+the timed region starts.
 
 The same JSON line also reports: the plain one-launch-per-slice form on the
-same window (the persistent/plain ratio), config 2 (sigma 10, r 100, nu 1e-4,
-Seed(7), full run), config 3 (128x128 map, nu 1e-5, a pause window, alias
-gathers), the dominant kernel's roofline, and the CPU oracle timed on the
-host's cores on a bounded sample of the headline workload.
+same window (with a bit-identity check of the two event streams), the same
+scenario with the reference's default dedup cache, config 2 (full run),
+config 3 (128x128 map, the alias-gather workload, with its gather roofline),
+serving (config 5: p50/p99 latency), the dominant kernel's roofline, and the
+CPU oracle timed on the host's cores on a bounded sample of the headline
+workload.
 """
 import argparse
 import json
@@ -184,15 +187,26 @@ def serving_run(ctx, n_requests=150, rate=50.0, cap=1000.0, tick_steps=250, seed
             "mean_ms": float(lat.mean()), "max_ms": float(lat.max())}
 
 
+def run_collect(job):
+    fin = False
+    chunks = []
+    while not fin:
+        fin = job.step_round()
+        chunks.append(job.drain_events())
+    return np.concatenate(chunks) if chunks else np.zeros(0, nh.EVENT_DTYPE)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--window", type=float, default=2000.0, help="pause_before of the headline workload")
+    ap.add_argument("--dedup-window", type=float, default=20000.0,
+                    help="pause_before of the dedup-on config 4 comparison")
     ap.add_argument("--no-extra", action="store_true", help="headline only")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graph", type=int, default=1, help="capture persistent rounds in hipGraphs")
+    ap.add_argument("--no-tune", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for {a.gpus})",
@@ -203,14 +217,25 @@ def main():
     cus, arch = ctx.info()
     log(f"device {arch} with {cus} CUs")
 
-    # ---- headline: config 4 window, persistent form ----
+    # ---- headline: config 4, dedup None window, persistent form ----
     t = time.perf_counter()
     cogs = nh.AlmostInfiniteNormalDispersalScenario(2000, 50.0).new(1e-6)
     lineages = cogs.sample_habitat(nh.OriginPreSampler.all())
     tables = nh.Tables(ctx, cogs)
     setup_s = time.perf_counter() - t
     log(f"config 4: {len(lineages)} lineages, setup {setup_s:.2f} s")
-    args = nh.IndependentArguments(form=nh.PERSISTENT, use_graph=bool(a.graph))
+    args = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=nh.PERSISTENT, use_graph=True)
+    tuned = None
+    if not a.no_tune:
+        t = time.perf_counter()
+        targs, tms = nh.autotune(ctx, tables, args, nh.WyHash.seed_from_u64(42), cogs, lineages[::8], 50.0,
+                                 tune_step_slice=False)
+        args = nh.IndependentArguments(dedup_cache=args.dedup_cache, form=nh.PERSISTENT, use_graph=True,
+                                       block_size=targs.block_size, blocks_per_cu=targs.blocks_per_cu,
+                                       inner_steps=targs.inner_steps)
+        tuned = {"block_size": targs.block_size, "blocks_per_cu": targs.blocks_per_cu,
+                 "inner_steps": targs.inner_steps, "trial_ms": tms, "tune_s": time.perf_counter() - t}
+        log(f"autotuned: {tuned}")
     wall, results = timed_runs(ctx, tables, cogs, lineages, args, 42, a.window, a.steps, a.warmup)
     steps_total = sum(int(o.steps) for o, _, _ in results)
     value = steps_total / wall
@@ -220,35 +245,63 @@ def main():
     log(f"persistent: {steps_total} steps in {wall:.3f} s -> {value:.4e} steps/s; "
         f"device {kern_ms:.1f} ms over {launches} launches")
 
-    # roofline of the dominant kernel (k_persistent): algorithmic bytes per
-    # launch = lineage state read + written back once per round, speciation
-    # events, and one dedup bucket read + write per step slice
-    per_launch_steps = o0.steps / max(results[0][2], 1)
-    per_launch_bytes = (len(lineages) * 2 * LINEAGE_STATE_BYTES / max(results[0][2], 1)
-                        + o0.n_events / max(results[0][2], 1) * EVENT_BYTES
-                        + per_launch_steps / args.step_slice * 2 * CACHE_BUCKET_BYTES)
-    mean_launch_ms = results[0][1] / max(results[0][2], 1)
+    # roofline of the dominant kernel (k_persistent, captured with the event
+    # sort in one graph per round): algorithmic bytes per launch = the lineage
+    # state read and written back once per round plus the speciation events
+    nl = max(results[0][2], 1)
+    per_launch_steps = o0.steps / nl
+    per_launch_bytes = len(lineages) * 2 * LINEAGE_STATE_BYTES / nl + o0.n_events / nl * EVENT_BYTES
+    mean_launch_ms = results[0][1] / nl
     achieved = per_launch_bytes / (mean_launch_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                "kernel": "k_persistent<almost-infinite>" + (" (graph: + event sort)" if a.graph else ""),
+                "kernel": "k_persistent<almost-infinite> (+ event sort, one hipGraph per round)",
                 "mean_launch_ms": round(mean_launch_ms, 3),
                 "algorithmic_bytes_per_launch": int(per_launch_bytes),
-                "steps_per_launch": int(per_launch_steps)}
+                "steps_per_launch": int(per_launch_steps),
+                "note": "no table gathers on this workload: the kernel is bound by VALU issue "
+                        "(64-bit integer hashing and f64 maths), not by HBM"}
 
-    extra = {}
+    extra = {"autotuned": tuned}
     if not a.no_extra:
-        pargs = nh.IndependentArguments(form=nh.PLAIN)
-        pwall, pres = timed_runs(ctx, tables, cogs, lineages, pargs, 42, a.window, 1, 0)
-        psteps = int(pres[0][0].steps)
-        extra["plain_form"] = {"value": psteps / pwall, "ms": pwall * 1e3, "steps": psteps,
-                               "launches": int(pres[0][2]), "kernel_ms": pres[0][1]}
-        extra["persistent_over_plain"] = value / (psteps / pwall)
-        log(f"plain: {psteps} steps in {pwall:.3f} s -> {psteps / pwall:.4e} steps/s")
+        # bit-exact comparison with the plain one-launch-per-slice form
+        pj = make_job(ctx, tables, cogs, lineages, nh.IndependentArguments(dedup_cache=nh.DedupCache.none(),
+                                                                          form=nh.PLAIN), 42, a.window)
+        t = time.perf_counter()
+        pev = run_collect(pj)
+        pwall = time.perf_counter() - t
+        po = pj.outcome()
+        pms, pk = pj.kernel_time()
+        pj.close()
+        qj = make_job(ctx, tables, cogs, lineages, args, 42, a.window)
+        qev = run_collect(qj)
+        qo = qj.outcome()
+        qj.close()
+        same = pev.tobytes() == qev.tobytes() and po.steps == qo.steps and po.time == qo.time
+        extra["plain_form"] = {"value": po.steps / pwall, "ms": pwall * 1e3, "steps": int(po.steps),
+                               "launches": int(pk), "kernel_ms": pms,
+                               "events_bit_identical_to_persistent": bool(same)}
+        extra["persistent_over_plain"] = value / (po.steps / pwall)
+        log(f"plain: {po.steps} steps in {pwall:.3f} s -> {po.steps / pwall:.4e} steps/s; identical={same}")
+
+        # config 4 with the reference's default dedup cache (Relative(1.0)):
+        # coalesced lineages are dropped, the window is mostly a coalescence
+        # workload; plain form = the reference's queue order, bit-exact
+        dd = {}
+        for name, form in (("persistent", nh.PERSISTENT), ("plain", nh.PLAIN)):
+            dargs = nh.IndependentArguments(form=form, use_graph=form == nh.PERSISTENT)
+            dwall, dres = timed_runs(ctx, tables, cogs, lineages, dargs, 42, a.dedup_window, 1, 1)
+            do = dres[0][0]
+            dd[name] = {"value": do.steps / dwall, "ms": dwall * 1e3, "steps": int(do.steps),
+                        "kernel_ms": dres[0][1], "launches": int(dres[0][2]),
+                        "remaining_lineages": int(do.n_remaining), "speciations": int(do.n_events)}
+        dd["persistent_over_plain_wall"] = dd["plain"]["ms"] / dd["persistent"]["ms"]
+        extra["config4_dedup_window"] = {"pause_before": a.dedup_window, **dd}
+        log(f"config 4 dedup window: {extra['config4_dedup_window']}")
         del tables
         extra["config2_full_run"] = config2_run(ctx)
         log(f"config 2: {extra['config2_full_run']}")
-        extra["config3_window"] = config3_run(ctx, 500.0)
+        extra["config3_window"] = config3_run(ctx, 2000.0)
         log(f"config 3: {extra['config3_window']}")
         extra["serving_config5"] = serving_run(ctx)
         log(f"serving: {extra['serving_config5']}")
@@ -265,7 +318,8 @@ def main():
         "vs_baseline": None, "dtype": "f64+u64", "data": "synthetic",
         "config": {"workload": "config4-window: almost-infinite normal sigma=50, circle r=2000 "
                                f"(12,566,345 lineages), nu=1e-6, Seed(42), pause_before={a.window}, "
-                               "persistent form, dedup Relative(1.0), step_slice 10, event_slice 2.0",
+                               "DedupCache::None, persistent form (hipGraph rounds, autotuned launch), "
+                               "delta_t 2, event_slice 2.0",
                    "lineages": int(len(lineages)), "steps_per_run": int(o0.steps),
                    "speciations_per_run": int(o0.n_events), "rounds": int(o0.rounds),
                    "remaining_lineages": int(o0.n_remaining)},

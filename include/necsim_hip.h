@@ -152,6 +152,7 @@ typedef struct {
     uint32_t block_size;           /* threads per block, 0 = tuned default */
     uint32_t blocks_per_cu;        /* persistent form: resident blocks per CU, 0 = tuned */
     uint32_t use_graph;            /* persistent form: capture each round in a hipGraph */
+    uint32_t inner_steps;          /* persistent form: steps per lane between refills, 0 = tuned default */
 } nh_job_params;
 
 typedef struct {
@@ -223,9 +224,10 @@ int nh_job_kernel_time(const nh_job *job, double *total_ms, uint64_t *launches);
  * CUDA launch shape is fixed by arguments, cuda/src/arguments.rs:124-138).
  * Times short pause windows ([0, window)) on `sample` for every candidate
  * block size (64/128/256), resident blocks per CU (occupancy limit and half of
- * it) and, when base->step_slice == 0, step slice (5, 10, 20, 50, 150), and
- * writes the fastest combination (least device time for the window) to
- * *tuned (a copy of *base with block_size, blocks_per_cu and step_slice set).
+ * it), steps per lane between refills (1, 4, 16) and, when
+ * base->step_slice == 0, step slice (5, 10, 20, 50, 150), and writes the
+ * fastest combination (least time for the window) to *tuned (a copy of *base
+ * with block_size, blocks_per_cu, inner_steps and step_slice set).
  * *best_ms receives the winner's window time. */
 int nh_autotune(nh_ctx *ctx, const nh_tables *tables, const nh_job_params *base,
                 const nh_lineage *sample, size_t n, double window, nh_job_params *tuned,

@@ -55,6 +55,7 @@ struct PersistArgs {
     DevEvent *events;
     Counters *ctr;
     unsigned long long tick_steps; // 0: unlimited (a lineage runs to the level)
+    unsigned inner_steps;          // steps per lane between wave-level refills (>= 1)
 };
 
 struct PlainArgs {

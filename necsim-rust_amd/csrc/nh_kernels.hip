@@ -143,7 +143,7 @@ __device__ __forceinline__ void block_flush(unsigned long long steps, double max
 // reaches the water level is written back to its slot, one that speciates or
 // is dropped as a duplicate only gets its dead flag.
 template <int KIND>
-__global__ void __launch_bounds__(256) k_persistent(PersistArgs A) {
+__global__ void __launch_bounds__(256, 4) k_persistent(PersistArgs A) {
     __shared__ double s_ratio[RATIO_N];
     load_ratio(s_ratio, A.P.ratio);
 
@@ -180,58 +180,47 @@ __global__ void __launch_bounds__(256) k_persistent(PersistArgs A) {
             continue;
         }
 
-        bool emit = false;
-        DevEvent ev;
+        // up to inner_steps steps per lane between wave-level refills
         if (has) {
-            lane_peek<KIND>(L, A.P, s_ratio);
-            if (L.next_t >= level || (A.tick_steps && tick >= A.tick_steps)) {
-                // The lineage leaves this launch: its (partial) slice ends and
-                // the slice sample is inserted like on reclassification
-                // (monolithic/mod.rs:205-219).  A tick-limited lineage starts
-                // a fresh slice when it is resumed.
-                bool dup = false;
-                if (A.cache && L.ms_some)
-                    dup = cache_insert_concurrent(A.cache, A.cache_cap, lane_sample<KIND>(L, A.S));
-                if (dup) {
-                    A.lin.flags[slot] = FLAG_DEAD;
-                    my_dead++;
-                } else {
-                    lane_store<KIND>(L, A.S, A.lin, slot);
+#pragma unroll 1
+            for (unsigned inner = 0; inner < A.inner_steps; inner++) {
+                lane_peek<KIND>(L, A.P, s_ratio);
+                // Leaving this launch (water level or tick limit) and reaching
+                // the step slice both end the slice: its min speciation sample
+                // is inserted like on reclassification (monolithic/mod.rs:205-219).
+                // A tick-limited lineage starts a fresh slice when resumed.
+                bool leave = L.next_t >= level || (A.tick_steps && tick >= A.tick_steps);
+                if (leave || slice_steps >= A.P.step_slice) {
+                    bool dup = A.cache && L.ms_some &&
+                               cache_insert_concurrent(A.cache, A.cache_cap, lane_sample<KIND>(L, A.S));
+                    L.ms_some = false;
+                    slice_steps = 0;
+                    if (dup) {
+                        A.lin.flags[slot] = FLAG_DEAD;
+                        my_dead++;
+                        has = false;
+                        break;
+                    }
+                    if (leave) {
+                        lane_store<KIND>(L, A.S, A.lin, slot);
+                        has = false;
+                        break;
+                    }
                 }
-                has = false;
-            } else if (slice_steps >= A.P.step_slice) {
-                bool dup = false;
-                if (A.cache && L.ms_some)
-                    dup = cache_insert_concurrent(A.cache, A.cache_cap, lane_sample<KIND>(L, A.S));
-                L.ms_some = false;
-                slice_steps = 0;
-                if (dup) {
-                    A.lin.flags[slot] = FLAG_DEAD;
-                    my_dead++;
-                    has = false;
-                }
-            } else {
+                DevEvent ev;
                 int r = lane_execute<KIND>(L, A.S, A.P, ev);
                 my_steps++;
                 slice_steps++;
                 tick++;
                 my_max = L.t_last > my_max ? L.t_last : my_max;
                 if (r == STEP_SPECIATED) {
+                    A.events[atomicAdd(&A.ctr->n_events, 1ull)] = ev; // rare
                     A.lin.flags[slot] = FLAG_DEAD;
                     my_dead++;
                     has = false;
-                    emit = true;
+                    break;
                 }
             }
-        }
-        // speciation events are rare: one wave-aggregated append
-        uint64_t em = __ballot(emit);
-        if (em) {
-            uint32_t leader = __ffsll((unsigned long long)em) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(&A.ctr->n_events, (unsigned long long)__popcll(em));
-            base = __shfl(base, leader);
-            if (emit) A.events[base + __popcll(em & lanemask_lt())] = ev;
         }
     }
     block_flush(my_steps, my_max, my_dead, A.ctr);
@@ -313,31 +302,28 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
         }
         if (has) {
             const ServeReq &R = A.reqs[req];
-            lane_peek<KIND>(L, P, s_ratio);
-            if (L.next_t >= level || (A.tick_steps && tick >= A.tick_steps)) {
-                bool dup = false;
-                if (R.cache && L.ms_some)
-                    dup = cache_insert_concurrent(R.cache, R.cache_cap, lane_sample<KIND>(L, A.S));
-                if (dup) {
-                    R.lin.flags[slot] = FLAG_DEAD;
-                    acc_dead++;
-                } else {
-                    lane_store<KIND>(L, A.S, R.lin, slot);
-                    if (L.next_t < level) R.ctr->n_below = 1; // the round continues next tick
+#pragma unroll 1
+            for (int inner = 0; inner < 4; inner++) {
+                lane_peek<KIND>(L, P, s_ratio);
+                bool leave = L.next_t >= level || (A.tick_steps && tick >= A.tick_steps);
+                if (leave || slice_steps >= P.step_slice) {
+                    bool dup = R.cache && L.ms_some &&
+                               cache_insert_concurrent(R.cache, R.cache_cap, lane_sample<KIND>(L, A.S));
+                    L.ms_some = false;
+                    slice_steps = 0;
+                    if (dup) {
+                        R.lin.flags[slot] = FLAG_DEAD;
+                        acc_dead++;
+                        has = false;
+                        break;
+                    }
+                    if (leave) {
+                        lane_store<KIND>(L, A.S, R.lin, slot);
+                        if (L.next_t < level) R.ctr->n_below = 1; // the round continues next tick
+                        has = false;
+                        break;
+                    }
                 }
-                has = false;
-            } else if (slice_steps >= P.step_slice) {
-                bool dup = false;
-                if (R.cache && L.ms_some)
-                    dup = cache_insert_concurrent(R.cache, R.cache_cap, lane_sample<KIND>(L, A.S));
-                L.ms_some = false;
-                slice_steps = 0;
-                if (dup) {
-                    R.lin.flags[slot] = FLAG_DEAD;
-                    acc_dead++;
-                    has = false;
-                }
-            } else {
                 DevEvent ev;
                 int r = lane_execute<KIND>(L, A.S, P, ev);
                 acc_steps++;
@@ -349,6 +335,7 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
                     R.lin.flags[slot] = FLAG_DEAD;
                     acc_dead++;
                     has = false;
+                    break;
                 }
             }
         }
@@ -672,6 +659,8 @@ struct SortSize {
     const unsigned long long *start;
     __device__ __forceinline__ uint64_t get() const { return end ? (*end - (start ? *start : 0ull)) : n; }
     __device__ __forceinline__ uint64_t offset() const { return start ? *start : 0ull; }
+    // radix tiles actually used (the launch grid may be sized for capacity)
+    __device__ __forceinline__ uint32_t tiles() const { return (uint32_t)((get() + RS_TILE - 1) / RS_TILE); }
 };
 
 __global__ void k_event_keys(const DevEvent *ev0, SortSize sz, unsigned long long *keys, uint32_t *vals,
@@ -689,9 +678,11 @@ __global__ void k_event_keys(const DevEvent *ev0, SortSize sz, unsigned long lon
 }
 
 __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const unsigned long long *keys, SortSize sz,
-                                                           int shift, uint32_t *hist, uint32_t nb) {
+                                                           int shift, uint32_t *hist) {
     __shared__ uint32_t h[256];
     const uint64_t n = sz.get();
+    const uint32_t nb = sz.tiles();
+    if (blockIdx.x >= nb) return;
     h[threadIdx.x] = 0;
     __syncthreads();
     uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
@@ -707,9 +698,11 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const unsigned lon
                                                               const uint32_t *vin,
                                                               unsigned long long *kout, uint32_t *vout,
                                                               SortSize sz, int shift,
-                                                              const uint32_t *offsets, uint32_t nb) {
+                                                              const uint32_t *offsets) {
     constexpr int W = RS_THREADS / 64;
     const uint64_t n = sz.get();
+    const uint32_t nb = sz.tiles();
+    if (blockIdx.x >= nb) return;
     __shared__ uint32_t running[256];
     __shared__ uint32_t wcnt[W][256];
     const unsigned lane = lane_id();
@@ -895,16 +888,41 @@ void launch_compact(const uint8_t *status, const uint32_t *pass, uint64_t m, uin
 
 uint32_t sort_blocks(uint64_t n) { return (uint32_t)((n + RS_TILE - 1) / RS_TILE); }
 
+// exclusive scan of the 256 x tiles digit histogram of a sort
+__global__ void __launch_bounds__(1024) k_scan_hist(uint32_t *data, SortSize sz) {
+    __shared__ uint32_t partial[1024];
+    const uint64_t n = (uint64_t)256 * sz.tiles();
+    const int T = blockDim.x;
+    uint64_t per = (n + T - 1) / T;
+    uint64_t lo = per * threadIdx.x, hi = lo + per < n ? lo + per : n;
+    uint32_t sum = 0;
+    for (uint64_t j = lo; j < hi; j++) sum += data[j];
+    partial[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < T; off <<= 1) {
+        uint32_t v = threadIdx.x >= (unsigned)off ? partial[threadIdx.x - off] : 0;
+        __syncthreads();
+        partial[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? partial[threadIdx.x - 1] : 0;
+    for (uint64_t j = lo; j < hi; j++) {
+        uint32_t v = data[j];
+        data[j] = run;
+        run += v;
+    }
+}
+
 static void sort_passes(const DevEvent *events, SortSize sz, uint64_t cap, DevEvent *out, SortBuffers &sb,
                         hipStream_t stream, int last_digit) {
     uint32_t nb = sort_blocks(cap);
     unsigned g = grid_for(cap, 256);
     int cur = 0;
     for (int d = 0; d <= last_digit; d++) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sz, 8 * d, sb.hist, nb);
-        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, stream, sb.hist, (uint64_t)256 * nb);
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sz, 8 * d, sb.hist);
+        hipLaunchKernelGGL(k_scan_hist, dim3(1), dim3(1024), 0, stream, sb.hist, sz);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sb.vals[cur],
-                           sb.keys[cur ^ 1], sb.vals[cur ^ 1], sz, 8 * d, sb.hist, nb);
+                           sb.keys[cur ^ 1], sb.vals[cur ^ 1], sz, 8 * d, sb.hist);
         cur ^= 1;
     }
     hipLaunchKernelGGL(k_fix_ties, dim3(g), dim3(256), 0, stream, sb.keys[cur], sb.vals[cur], events, sz);

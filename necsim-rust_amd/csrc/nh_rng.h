@@ -17,12 +17,18 @@ constexpr uint64_t SEAHASH_K = 0x6eed0e9da4d94a4full;
 constexpr uint64_t INV_PHI = 0x9e3779b97f4a7c15ull; // poisson.rs:10
 constexpr double TWO_POW_M53 = 1.1102230246251565e-16; // 0x1.0p-53
 
-// wyhash.rs:85-96: xor-fold of the 128-bit product.  On gfx950 the high half
-// comes from __umul64hi (v_mad_u64_u32 / v_mul_hi_u32 sequences).
+// wyhash.rs:85-96: xor-fold of the 128-bit product.  On gfx950 the product
+// is assembled from four 32x32+64 -> 64 multiply-adds (v_mad_u64_u32); no
+// intermediate sum overflows 64 bits.
 NH_HD uint64_t wymum(uint64_t a, uint64_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    uint64_t lo = a * b;
-    uint64_t hi = __umul64hi(a, b);
+    uint32_t al = (uint32_t)a, ah = (uint32_t)(a >> 32);
+    uint32_t bl = (uint32_t)b, bh = (uint32_t)(b >> 32);
+    uint64_t p0 = (uint64_t)al * bl;
+    uint64_t mid = (uint64_t)al * bh + (p0 >> 32);
+    uint64_t mid2 = (uint64_t)ah * bl + (uint32_t)mid;
+    uint64_t lo = (mid2 << 32) | (uint32_t)p0;
+    uint64_t hi = (uint64_t)ah * bh + (mid >> 32) + (mid2 >> 32);
 #else
     unsigned __int128 r = (unsigned __int128)a * (unsigned __int128)b;
     uint64_t lo = (uint64_t)r;

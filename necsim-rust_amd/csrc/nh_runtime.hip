@@ -704,6 +704,7 @@ static PersistArgs persist_args(nh_job *j, int cur) {
     a.events = j->d_events;
     a.ctr = j->d_ctr;
     a.tick_steps = 0;
+    a.inner_steps = j->p.inner_steps ? j->p.inner_steps : 8;
     return a;
 }
 
@@ -897,12 +898,14 @@ extern "C" int nh_autotune(nh_ctx *ctx, const nh_tables *tables, const nh_job_pa
         std::vector<uint32_t> residency = {(uint32_t)max_per_cu};
         if (max_per_cu >= 2) residency.push_back((uint32_t)(max_per_cu / 2));
         for (uint32_t per_cu : residency) {
+            for (uint32_t inner : {1u, 4u, 16u})
             for (uint64_t slice : slices) {
                 nh_job_params p = *base;
                 p.form = NH_FORM_PERSISTENT;
                 p.block_size = threads;
                 p.blocks_per_cu = per_cu;
                 p.step_slice = slice;
+                p.inner_steps = inner;
                 p.has_pause_before = 1;
                 p.pause_before = window;
                 p.use_graph = 0;

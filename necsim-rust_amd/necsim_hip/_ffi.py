@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "libnecsim_hip.so")
+# NECSIM_HIP_LIBRARY selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("NECSIM_HIP_LIBRARY", os.path.join(PKG_DIR, "libnecsim_hip.so"))
 
 NH_OK = 0
 ERRORS = {-1: "NH_E_INVALID", -2: "NH_E_HIP", -3: "NH_E_NOMEM", -4: "NH_E_STATE",
@@ -49,7 +50,7 @@ class JobParams(ctypes.Structure):
                 ("event_slice_factor", c_f64), ("event_slice_capacity", c_u64),
                 ("has_pause_before", c_i32), ("pause_before", c_f64), ("resume_after", c_f64),
                 ("form", c_i32), ("block_size", c_u32), ("blocks_per_cu", c_u32),
-                ("use_graph", c_u32)]
+                ("use_graph", c_u32), ("inner_steps", c_u32)]
 
 
 class Outcome(ctypes.Structure):

@@ -90,6 +90,7 @@ class IndependentArguments:
     block_size: int = 0
     blocks_per_cu: int = 0
     use_graph: bool = False
+    inner_steps: int = 0
     device: int = 0
 
 
@@ -371,7 +372,8 @@ def _job_params(args, rng, speciation_probability, turnover_rate, pause_before=N
                           has_pause_before=0 if pause_before is None else 1,
                           pause_before=0.0 if pause_before is None else pause_before,
                           resume_after=resume_after, form=args.form, block_size=args.block_size,
-                          blocks_per_cu=args.blocks_per_cu, use_graph=1 if args.use_graph else 0)
+                          blocks_per_cu=args.blocks_per_cu, use_graph=1 if args.use_graph else 0,
+                          inner_steps=args.inner_steps)
 
 
 def autotune(ctx, tables, args: IndependentArguments, rng: WyHash, cogs, sample, window,
@@ -388,7 +390,8 @@ def autotune(ctx, tables, args: IndependentArguments, rng: WyHash, cogs, sample,
                             window, ctypes.byref(out), ctypes.byref(ms)))
     return dataclasses.replace(args, form=PERSISTENT, step_slice=int(out.step_slice),
                                block_size=int(out.block_size),
-                               blocks_per_cu=int(out.blocks_per_cu)), ms.value
+                               blocks_per_cu=int(out.blocks_per_cu),
+                               inner_steps=int(out.inner_steps)), ms.value
 
 
 class HipIndependentAlgorithm:

@@ -110,6 +110,7 @@ pub struct nh_job_params {
     pub block_size: u32,
     pub blocks_per_cu: u32,
     pub use_graph: u32,
+    pub inner_steps: u32,
 }
 
 #[repr(C)]
