@@ -206,7 +206,16 @@ def main():
                     help="pause_before of the dedup-on config 4 comparison")
     ap.add_argument("--no-extra", action="store_true", help="headline only")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shape unless given below")
+    ap.add_argument("--form", choices=("persistent", "plain"), default="persistent",
+                    help="form of the headline runs")
+    ap.add_argument("--block-size", type=int, default=0, help="fixed launch shape (skips the autotuner)")
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--inner-steps", type=int, default=0)
+    ap.add_argument("--exec-fill", type=int, default=0)
+    ap.add_argument("--graph", action="store_true",
+                    help="capture each water-level round (kernel + event sort) in a hipGraph; the "
+                         "kernel timing then covers the whole graph")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for {a.gpus})",
@@ -224,17 +233,23 @@ def main():
     tables = nh.Tables(ctx, cogs)
     setup_s = time.perf_counter() - t
     log(f"config 4: {len(lineages)} lineages, setup {setup_s:.2f} s")
-    args = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=nh.PERSISTENT, use_graph=True)
+    form = nh.PERSISTENT if a.form == "persistent" else nh.PLAIN
+    fixed = a.block_size or a.blocks_per_cu or a.inner_steps or a.exec_fill
+    args = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=form,
+                                   use_graph=a.graph and form == nh.PERSISTENT,
+                                   block_size=a.block_size, blocks_per_cu=a.blocks_per_cu,
+                                   inner_steps=a.inner_steps, exec_fill=a.exec_fill)
     tuned = None
-    if not a.no_tune:
+    if not (a.no_tune or fixed or form == nh.PLAIN):
         t = time.perf_counter()
         targs, tms = nh.autotune(ctx, tables, args, nh.WyHash.seed_from_u64(42), cogs, lineages[::8], 50.0,
                                  tune_step_slice=False)
-        args = nh.IndependentArguments(dedup_cache=args.dedup_cache, form=nh.PERSISTENT, use_graph=True,
+        args = nh.IndependentArguments(dedup_cache=args.dedup_cache, form=nh.PERSISTENT, use_graph=args.use_graph,
                                        block_size=targs.block_size, blocks_per_cu=targs.blocks_per_cu,
-                                       inner_steps=targs.inner_steps)
+                                       inner_steps=targs.inner_steps, exec_fill=targs.exec_fill)
         tuned = {"block_size": targs.block_size, "blocks_per_cu": targs.blocks_per_cu,
-                 "inner_steps": targs.inner_steps, "trial_ms": tms, "tune_s": time.perf_counter() - t}
+                 "inner_steps": targs.inner_steps, "exec_fill": targs.exec_fill, "trial_ms": tms,
+                 "tune_s": time.perf_counter() - t}
         log(f"autotuned: {tuned}")
     wall, results = timed_runs(ctx, tables, cogs, lineages, args, 42, a.window, a.steps, a.warmup)
     steps_total = sum(int(o.steps) for o, _, _ in results)
@@ -262,7 +277,16 @@ def main():
                 "note": "no table gathers on this workload: the kernel is bound by VALU issue "
                         "(64-bit integer hashing and f64 maths), not by HBM"}
 
-    extra = {"autotuned": tuned}
+    extra = {"autotuned": tuned,
+             "launch": {"form": a.form, "block_size": args.block_size, "blocks_per_cu": args.blocks_per_cu,
+                        "inner_steps": args.inner_steps, "exec_fill": args.exec_fill,
+                        "use_graph": bool(args.use_graph)}}
+    # plain copy kernel of known size: achievable HBM bandwidth and the
+    # calibration of the FETCH_SIZE / WRITE_SIZE byte counters
+    cms, cgbs = nh.copy_calibrate(ctx, 1 << 30, 10)
+    extra["copy_kernel"] = {"kernel": "k_copy_calibrate", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
+                            "ms": cms, "GB_per_s": cgbs}
+    log(f"copy kernel: {cgbs:.1f} GB/s")
     if not a.no_extra:
         # bit-exact comparison with the plain one-launch-per-slice form
         pj = make_job(ctx, tables, cogs, lineages, nh.IndependentArguments(dedup_cache=nh.DedupCache.none(),

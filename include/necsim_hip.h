@@ -152,7 +152,10 @@ typedef struct {
     uint32_t block_size;           /* threads per block, 0 = tuned default */
     uint32_t blocks_per_cu;        /* persistent form: resident blocks per CU, 0 = tuned */
     uint32_t use_graph;            /* persistent form: capture each round in a hipGraph */
-    uint32_t inner_steps;          /* persistent form: steps per lane between refills, 0 = tuned default */
+    uint32_t inner_steps;          /* persistent form: iterations per lane between refills, 0 = default 16 */
+    uint32_t exec_fill;            /* persistent form: a wave executes its lanes' next events once at least
+                                      exec_fill eighths of its active lanes have found theirs (1..8);
+                                      0 = default 5 */
 } nh_job_params;
 
 typedef struct {
@@ -223,11 +226,12 @@ int nh_job_kernel_time(const nh_job *job, double *total_ms, uint64_t *launches);
 /* Autotuner of the persistent form (no counterpart in the reference, whose
  * CUDA launch shape is fixed by arguments, cuda/src/arguments.rs:124-138).
  * Times short pause windows ([0, window)) on `sample` for every candidate
- * block size (64/128/256), resident blocks per CU (occupancy limit and half of
- * it), steps per lane between refills (1, 4, 16) and, when
- * base->step_slice == 0, step slice (5, 10, 20, 50, 150), and writes the
- * fastest combination (least time for the window) to *tuned (a copy of *base
- * with block_size, blocks_per_cu, inner_steps and step_slice set).
+ * block size (128/256), resident blocks per CU (occupancy limit and half of
+ * it), iterations per lane between refills (4, 16), execution fill (1, 5, 8
+ * eighths) and, when base->step_slice == 0, step slice (5, 10, 20, 50, 150),
+ * and writes the fastest combination (least time for the window) to *tuned
+ * (a copy of *base with block_size, blocks_per_cu, inner_steps, exec_fill and
+ * step_slice set).
  * *best_ms receives the winner's window time. */
 int nh_autotune(nh_ctx *ctx, const nh_tables *tables, const nh_job_params *base,
                 const nh_lineage *sample, size_t n, double window, nh_job_params *tuned,
@@ -319,6 +323,23 @@ int nh_sample_origins(const nh_scenario *scenario, double percentage, uint32_t r
  * after prime (same inputs, f64 out).  in / out are host buffers of 8-byte
  * words; n is the number of outputs. */
 int nh_debug_eval(nh_ctx *ctx, int32_t op, const uint64_t *in, uint64_t *out, size_t n);
+
+/* Diagnostics, no counterpart in the reference: times `reps` launches of a
+ * plain 16-byte-vector copy kernel (k_copy_calibrate) that reads and writes
+ * `bytes` bytes each, on the context's stream, after a fill and a warm-up
+ * launch; *ms_per_copy = mean duration.  Its known byte count calibrates HBM
+ * byte counters (FETCH_SIZE / WRITE_SIZE) and gives the achievable copy
+ * bandwidth.  Allocates 2 x bytes of device memory for the call. */
+int nh_copy_calibrate(nh_ctx *ctx, size_t bytes, int32_t reps, double *ms_per_copy);
+
+/* Diagnostics: sorts n events (host buffers) on the device by the
+ * PackedEvent order (necsim/core/src/event.rs:256-280), the order in which
+ * LiveWaterLevelReporterProxy::advance_water_level reports each water-level
+ * round's events (monolithic/reporter/live.rs:88-115), with the eager sort
+ * of nh_job_step_round / nh_server_tick (device_sized == 0) or the
+ * graph-capturable variant (device_sized != 0).  Allocates device memory. */
+int nh_debug_sort_events(nh_ctx *ctx, const nh_speciation_event *in, size_t n, int32_t device_sized,
+                         nh_speciation_event *out);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,8 @@
-// nh_debug.hip -- elementwise evaluation of device primitives (diagnostics).
+// nh_debug.hip -- diagnostics: elementwise evaluation of device primitives
+// and a plain copy kernel of known size (the HBM byte-counter calibration).
 #include "../../include/necsim_hip.h"
 #include "nh_internal.h"
+#include "nh_kernels.h"
 #include "nh_rng.h"
 
 using namespace nh;
@@ -32,7 +34,55 @@ __global__ void k_debug_eval(int op, const uint64_t *in, uint64_t *out, size_t n
     }
     out[i] = r;
 }
+// 16-byte vector copy, grid-stride: reads and writes each byte exactly once
+__global__ void __launch_bounds__(256) k_copy_calibrate(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                        size_t n) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+__global__ void __launch_bounds__(256) k_fill_calibrate(uint4 *dst, size_t n) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = make_uint4((unsigned)i, (unsigned)(i >> 32), 0x9e3779b9u, 0x7f4a7c15u);
+}
 } // namespace
+
+extern "C" int nh_copy_calibrate(nh_ctx *ctx, size_t bytes, int32_t reps, double *ms_per_copy) {
+    if (!ctx || !ms_per_copy || reps < 1 || bytes < 16) return set_error(NH_E_INVALID, "nh_copy_calibrate: bad argument");
+    const size_t n = bytes / 16;
+    uint4 *src = nullptr, *dst = nullptr;
+    if (hipSetDevice(ctx->device) != hipSuccess || hipMalloc(&src, n * 16) != hipSuccess ||
+        hipMalloc(&dst, n * 16) != hipSuccess) {
+        if (src) (void)hipFree(src);
+        return set_error(NH_E_NOMEM, "nh_copy_calibrate: allocation failed");
+    }
+    hipStream_t st = ctx->stream;
+    const unsigned blocks = (unsigned)(ctx->cus > 0 ? ctx->cus * 16 : 4096);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_fill_calibrate, dim3(blocks), dim3(256), 0, st, src, n);
+        hipLaunchKernelGGL(k_copy_calibrate, dim3(blocks), dim3(256), 0, st, src, dst, n); // warm-up
+        e = hipEventRecord(e0, st);
+    }
+    for (int r = 0; e == hipSuccess && r < reps; r++) {
+        hipLaunchKernelGGL(k_copy_calibrate, dim3(blocks), dim3(256), 0, st, src, dst, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    if (e != hipSuccess) return set_error(NH_E_HIP, "nh_copy_calibrate: %s", hipGetErrorString(e));
+    *ms_per_copy = (double)ms / reps;
+    return NH_OK;
+}
 
 extern "C" int nh_debug_eval(nh_ctx *ctx, int32_t op, const uint64_t *in, uint64_t *out, size_t n) {
     if (!ctx || !in || !out || op < 0 || op > 11) return set_error(NH_E_INVALID, "nh_debug_eval: bad argument");
@@ -55,4 +105,48 @@ extern "C" int nh_debug_eval(nh_ctx *ctx, int32_t op, const uint64_t *in, uint64
     (void)hipFree(d_out);
     if (e != hipSuccess) return set_error(NH_E_HIP, "nh_debug_eval: %s", hipGetErrorString(e));
     return NH_OK;
+}
+
+// Sorts n host events by the PackedEvent order on the device, with the
+// eager sort of the job and server paths (device_sized == 0) or the
+// capturable one that reads its size from device memory (device_sized != 0).
+extern "C" int nh_debug_sort_events(nh_ctx *ctx, const nh_speciation_event *in, size_t n, int32_t device_sized,
+                                    nh_speciation_event *out) {
+    if (!ctx || (n && (!in || !out))) return set_error(NH_E_INVALID, "nh_debug_sort_events: bad argument");
+    if (n == 0) return NH_OK;
+    if (n > 0xffffffffull) return set_error(NH_E_CAPACITY, "nh_debug_sort_events: at most 2^32 - 1 events");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_error(NH_E_HIP, "nh_debug_sort_events: hipSetDevice");
+    hipStream_t st = ctx->stream;
+    SortBuffers sb{};
+    DevEvent *d_in = nullptr, *d_out = nullptr;
+    unsigned long long *d_range = nullptr;
+    int rc = NH_OK;
+    if ((rc = dalloc(&sb.keys[0], n)) || (rc = dalloc(&sb.keys[1], n)) || (rc = dalloc(&sb.vals[0], n)) ||
+        (rc = dalloc(&sb.vals[1], n)) || (rc = dalloc(&sb.hist, (size_t)256 * sort_blocks(n) + 1)) ||
+        (rc = dalloc(&sb.minmax, SORT_MINMAX_WORDS)) || (rc = dalloc(&d_in, n)) || (rc = dalloc(&d_out, n)) ||
+        (rc = dalloc(&d_range, 2))) {
+    } else {
+        sb.capacity = n;
+        unsigned long long range[2] = {0ull, (unsigned long long)n}; // start, end
+        hipError_t e = hipMemcpyAsync(d_in, in, n * sizeof(DevEvent), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_range, range, sizeof(range), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) {
+            if (device_sized) sort_events_device_sized(d_in, d_range + 1, d_range, n, d_out, sb, st);
+            else sort_events(d_in, n, d_out, sb, st, false);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, n * sizeof(DevEvent), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = set_error(NH_E_HIP, "nh_debug_sort_events: %s", hipGetErrorString(e));
+    }
+    dfree(sb.keys[0]);
+    dfree(sb.keys[1]);
+    dfree(sb.vals[0]);
+    dfree(sb.vals[1]);
+    dfree(sb.hist);
+    dfree(sb.minmax);
+    dfree(d_in);
+    dfree(d_out);
+    dfree(d_range);
+    return rc;
 }

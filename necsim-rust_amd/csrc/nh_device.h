@@ -107,6 +107,10 @@ template <int KIND> struct Lane {
     double next_t;
     uint64_t next_st;
     bool have_next;
+    // an unfinished window-by-window search for the next event
+    bool searching;
+    uint64_t search_ts;
+    double search_before;
     // min speciation sample of the current slice
     bool ms_some;
     double ms_u, ms_t;
@@ -155,46 +159,55 @@ __device__ __forceinline__ void lane_set_location(Lane<KIND> &L, const DevScenar
 
 // ---------------------------------------------------------------------------
 // PoissonEventTimeSampler::next_event_time_at_indexed_location_weakly_after
-// (event_time_sampler/poisson.rs:30-109).  Returns the event time and leaves
-// `state` primed for the event's own draws.
+// (event_time_sampler/poisson.rs:30-109), one time window at a time: the
+// events of window `ts` (its RNG primed with (id, ts)) are drawn, and the
+// earliest one after `time` is the next event.  Returns false if the window
+// holds none; otherwise sets *best and leaves `state` primed for the event's
+// own draws.
+__device__ __forceinline__ bool poisson_window(const DevParams &P, const double *ratio, uint64_t id,
+                                               double time, uint64_t ts, double &best, uint64_t &state) {
+    state = prime(P.seed, id, ts);
+    uint32_t number;
+    if (P.p0 > 0.0) {
+        uint32_t k = 0;
+        double prod = P.p0;
+        double acc = P.p0;
+        double u = uniform_closed_open(state);
+        while (u > acc && prod > 0.0) {
+            k += 1;
+            prod *= (k < RATIO_N) ? ratio[k] : P.lambda_dt / (double)k;
+            acc += prod;
+        }
+        number = k;
+    } else {
+        double a, b;
+        sample_2d_normal(state, P.lambda_dt, P.lambda_dt_sqrt, a, b);
+        number = as_u32(a); // f64::max(a, 0.0) as u32: NaN and negatives give 0
+    }
+    bool found = false;
+    uint32_t best_index = 0;
+    for (uint32_t e = 0; e < number; e++) {
+        double t = ((double)ts + uniform_closed_open(state)) * P.delta_t;
+        if (t > time && (!found || best > t)) {
+            best = t;
+            best_index = e;
+            found = true;
+        }
+    }
+    if (found) state = prime(P.seed, id, ts + INV_PHI * (uint64_t)(best_index + 1));
+    return found;
+}
+
+__device__ __forceinline__ uint64_t poisson_first_window(const DevParams &P, double time) {
+    return as_u64(floor_(P.dt_pow2 ? time * P.inv_delta_t : time / P.delta_t));
+}
+
+// The whole search: windows ts, ts + 1, ... until one holds an event.
 __device__ __forceinline__ double poisson_next(const DevParams &P, const double *ratio, uint64_t id,
                                                double time, uint64_t &state) {
-    uint64_t ts = as_u64(floor_(P.dt_pow2 ? time * P.inv_delta_t : time / P.delta_t));
     double best = 0.0;
-    uint32_t best_index = 0;
-    for (;;) {
-        state = prime(P.seed, id, ts);
-        uint32_t number;
-        if (P.p0 > 0.0) {
-            uint32_t k = 0;
-            double prod = P.p0;
-            double acc = P.p0;
-            double u = uniform_closed_open(state);
-            while (u > acc && prod > 0.0) {
-                k += 1;
-                prod *= (k < RATIO_N) ? ratio[k] : P.lambda_dt / (double)k;
-                acc += prod;
-            }
-            number = k;
-        } else {
-            double a, b;
-            sample_2d_normal(state, P.lambda_dt, P.lambda_dt_sqrt, a, b);
-            number = as_u32(a); // f64::max(a, 0.0) as u32: NaN and negatives give 0
-        }
-        bool found = false;
-        for (uint32_t e = 0; e < number; e++) {
-            double t = ((double)ts + uniform_closed_open(state)) * P.delta_t;
-            if (t > time && (!found || best > t)) {
-                best = t;
-                best_index = e;
-                found = true;
-            }
-        }
-        if (found) break;
-        ts += 1;
-    }
-    state = prime(P.seed, id, ts + INV_PHI * (uint64_t)(best_index + 1));
-    return best;
+    for (uint64_t ts = poisson_first_window(P, time);; ts++)
+        if (poisson_window(P, ratio, id, time, ts, best, state)) return best;
 }
 
 // Make sure L.next_t / L.next_st hold the next event (the `pop` peek).
@@ -210,11 +223,37 @@ __device__ __forceinline__ void lane_peek(Lane<KIND> &L, const DevParams &P, con
     L.have_next = true;
 }
 
+// The same peek advanced by ONE window per call, so that a wave steps all of
+// its lanes' searches in lockstep instead of waiting for its slowest lane's
+// whole search.  After the call that finds the event, L.next_t / L.next_st
+// equal lane_peek's.
+template <int KIND>
+__device__ __forceinline__ void lane_search_window(Lane<KIND> &L, const DevParams &P, const double *ratio) {
+    if (L.have_next) return;
+    if (!L.searching) {
+        L.search_before = P.min_event_time > L.t_last ? P.min_event_time : L.t_last;
+        L.search_ts = poisson_first_window(P, L.search_before);
+        L.searching = true;
+    }
+    double et = 0.0;
+    uint64_t st;
+    if (poisson_window(P, ratio, L.id, L.search_before, L.search_ts, et, st)) {
+        L.next_t = max_after(L.search_before, et);
+        L.next_st = st;
+        L.have_next = true;
+        L.searching = false;
+    } else {
+        L.search_ts++;
+    }
+}
+
 enum : int { STEP_DISPERSED = 0, STEP_SPECIATED = 1 };
 
 // Execute the peeked event (L.have_next must hold and L.next_t < level).
 // On speciation the event is written to *ev.
-template <int KIND>
+// TRACK_MS = false skips the slice's min speciation sample (only the dedup
+// cache reads it).
+template <int KIND, bool TRACK_MS = true>
 __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S, const DevParams &P,
                                             DevEvent &ev) {
     double net = L.next_t;
@@ -223,7 +262,7 @@ __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S,
 
     double u_spec = uniform_closed_open(st);
     // SpeciationSample::update_min keeps the first minimum (tracking.rs:46-55)
-    if (!(L.ms_some && L.ms_u <= u_spec)) {
+    if (TRACK_MS && !(L.ms_some && L.ms_u <= u_spec)) {
         L.ms_some = true;
         L.ms_u = u_spec;
         L.ms_t = net;

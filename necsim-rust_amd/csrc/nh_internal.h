@@ -1,4 +1,5 @@
-// nh_internal.h -- error reporting shared by the library's translation units.
+// nh_internal.h -- error reporting and device allocation helpers shared by
+// the library's translation units.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,6 +8,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+
+#include "../../include/necsim_hip.h"
 
 namespace nh {
 
@@ -21,8 +24,25 @@ inline int set_error(int code, const char *fmt, ...) {
     return code;
 }
 
+// hipMalloc of n elements (at least one); NH_E_NOMEM with a message on failure
+template <class T> int dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess)
+        return set_error(NH_E_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T),
+                         hipGetErrorString(e));
+    return NH_OK;
+}
+template <class T> void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
 } // namespace nh
 
+using nh::dalloc;
+using nh::dfree;
 using nh::set_error;
 
 // a device context: one device, one non-blocking stream (opaque in the C ABI)

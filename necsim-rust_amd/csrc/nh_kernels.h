@@ -26,12 +26,13 @@ struct RoundDesc {
     unsigned long long chunk;    // work indices claimed per wave atomic (0: per idle lane)
 };
 
-// Claim granularity of the persistent kernels: exact per-lane claims while
-// there are few lineages per lane, chunks of up to 1024 when there are many.
+// Largest claim of the persistent kernels' guided work queue: per-lane
+// claims while there are few lineages per lane, up to 1024 indices per claim
+// when there are many.
 inline unsigned long long claim_chunk(unsigned long long n, unsigned long long lanes) {
     unsigned long long per_wave = lanes ? n / (lanes / 64 * 4 + 1) : 0;
     if (per_wave < 256) return 0;
-    return per_wave > 1024 ? 1024 : (per_wave & ~63ull);
+    return 1024;
 }
 
 // plain-form element status (one byte per pass element)
@@ -55,7 +56,8 @@ struct PersistArgs {
     DevEvent *events;
     Counters *ctr;
     unsigned long long tick_steps; // 0: unlimited (a lineage runs to the level)
-    unsigned inner_steps;          // steps per lane between wave-level refills (>= 1)
+    unsigned inner_steps;          // iterations per lane between wave-level refills (>= 1)
+    unsigned exec_fill;            // eighths of the active lanes that must be ready to execute (1..8)
 };
 
 struct PlainArgs {
@@ -117,7 +119,7 @@ int serve_max_blocks_per_cu(int kind, int threads);
 
 // kernel launchers (all asynchronous on `stream`)
 void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream);
-int persistent_max_blocks_per_cu(int kind, int threads);
+int persistent_max_blocks_per_cu(int kind, int threads, bool dedup);
 void launch_plain(int kind, const PlainArgs &a, int threads, hipStream_t stream);
 
 void launch_dedup_push(const uint8_t *status, const unsigned long long *bucket, uint32_t *head,
@@ -139,11 +141,12 @@ void launch_compact(const uint8_t *status, const uint32_t *pass, uint64_t m, uin
                     uint32_t *out_b, uint64_t b_base, CompactBuffers &cb, hipStream_t stream);
 
 // Event sort: sorts events[0..n) by the PackedEvent order into out[0..n).
+constexpr int SORT_MINMAX_WORDS = 16;
 struct SortBuffers {
     unsigned long long *keys[2];
     uint32_t *vals[2];
     uint32_t *hist;         // [256 * nblocks + 1]
-    unsigned long long *minmax; // [2] device
+    unsigned long long *minmax; // [SORT_MINMAX_WORDS] device: key ranges, tie count
     uint64_t capacity;
 };
 uint32_t sort_blocks(uint64_t n);

@@ -19,17 +19,23 @@ __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) -
 
 // ---------------------------------------------------------------------------
 // lineage load / store
+// All fields, flags included, are read with independent loads so that a
+// refill waits for one memory round trip; returns the slot's flags.
 template <int KIND>
-__device__ __forceinline__ void lane_load(Lane<KIND> &L, const DevScenario &S, const LineageSoA &s,
-                                          uint64_t slot) {
+__device__ __forceinline__ uint8_t lane_load(Lane<KIND> &L, const DevScenario &S, const LineageSoA &s,
+                                             uint64_t slot) {
+    uint8_t f = s.flags[slot];
     L.ref = s.ref[slot];
     L.t_last = s.t_last[slot];
-    lane_set_location<KIND>(L, S, s.x[slot], s.y[slot], s.idx[slot]);
-    uint8_t f = s.flags[slot];
-    L.have_next = (f & 1) != 0;
+    uint32_t x = s.x[slot], y = s.y[slot], i = s.idx[slot];
     L.next_t = s.next_t[slot];
-    L.next_st = L.have_next ? s.next_st[slot] : 0;
+    uint64_t nst = s.next_st[slot];
+    L.have_next = (f & FLAG_EXACT) != 0;
+    L.next_st = L.have_next ? nst : 0;
+    L.searching = false;
     L.ms_some = false;
+    if (!(f & FLAG_DEAD)) lane_set_location<KIND>(L, S, x, y, i);
+    return f;
 }
 
 template <int KIND>
@@ -75,16 +81,25 @@ __device__ __forceinline__ void load_ratio(double *s_ratio, const double *ratio)
 
 // ---------------------------------------------------------------------------
 // Wave-level work claiming: each wave owns a private range [next, end) of
-// work indices and refills it with ONE global atomic per `chunk` indices, so
-// lane refill costs a ballot and a popcount, not a contended atomic.  With
-// chunk == 0 a wave claims exactly as many indices as it has idle lanes
-// (best balance when there are few lineages per lane).
+// work indices and refills it with ONE global atomic per claim, so lane
+// refill costs a ballot and a popcount, not a contended atomic.  Claims are
+// guided: a wave takes about remaining / (2 * waves) indices (at most
+// `chunk`, at least its idle lanes), estimating `remaining` from the queue
+// head its last claim returned.  Large claims early keep the atomic rate
+// low; claims shrink to single lanes as the queue drains, so every wave runs
+// out of work at about the same time.  With chunk == 0 a wave always claims
+// exactly as many indices as it has idle lanes.
 
 struct WaveQueue {
     unsigned long long next, end;
+    unsigned long long seen; // queue head returned by this wave's last claim
     bool drained;   // the global queue is empty
     bool exhausted; // ... and so is this wave's range
 };
+
+__device__ __forceinline__ unsigned long long grid_waves() {
+    return (unsigned long long)gridDim.x * ((blockDim.x + 63) / 64);
+}
 
 // Hands this lane the next work index if it needs one (need = ballot of
 // lanes without work).  Returns ~0ull when the lane gets nothing.
@@ -95,10 +110,18 @@ __device__ __forceinline__ unsigned long long wave_claim(WaveQueue &q, uint64_t 
     const unsigned lane = lane_id();
     if (q.next >= q.end && !q.drained) {
         uint32_t leader = __ffsll((unsigned long long)need) - 1;
-        unsigned long long take = chunk ? chunk : (unsigned long long)__popcll(need);
+        unsigned long long idle = (unsigned long long)__popcll(need);
+        unsigned long long take = idle;
+        if (chunk) {
+            unsigned long long rem = n_total > q.seen ? n_total - q.seen : 0;
+            unsigned long long g = rem / (2 * grid_waves());
+            g = g < chunk ? g : chunk;
+            take = g > idle ? g : idle;
+        }
         unsigned long long base = 0;
         if (lane == leader) base = atomicAdd(head, take);
         base = __shfl(base, leader);
+        q.seen = base + take;
         if (base >= n_total) {
             q.drained = true;
         } else {
@@ -141,22 +164,40 @@ __device__ __forceinline__ void block_flush(unsigned long long steps, double max
 // ---------------------------------------------------------------------------
 // Persistent round kernel.  Lineages are updated in place: a lineage that
 // reaches the water level is written back to its slot, one that speciates or
-// is dropped as a duplicate only gets its dead flag.
-template <int KIND>
-__global__ void __launch_bounds__(256, 4) k_persistent(PersistArgs A) {
+// is dropped as a duplicate only gets its dead flag.  Each inner iteration
+// advances every searching lane's next-event search by one Poisson window;
+// once at least exec_fill eighths of the wave's active lanes hold their next
+// event, those lanes handle it (leave at the level, dedup, execute) in the
+// same iteration.  A search takes a variable number of windows, so stepping
+// the searches in lockstep and batching the executions keeps the lanes of
+// the expensive execution path mostly active, where a whole search followed
+// by an execution per lane would wait for the wave's longest search.
+// DEDUP selects the variant with the in-kernel dedup cache (min speciation
+// sample per slice).
+__device__ __forceinline__ bool wave_ready(bool ready, unsigned fill_eighths) {
+    uint32_t act = (uint32_t)__popcll(__ballot(true));
+    uint32_t rdy = (uint32_t)__popcll(__ballot(ready));
+    return rdy != 0 && rdy * 8u >= act * fill_eighths;
+}
+
+// The dedup variant holds the slice's min speciation sample as well: it gets
+// registers for 3 waves per SIMD instead of 4, which measured faster than
+// spilling.
+template <int KIND, bool DEDUP>
+__global__ void __launch_bounds__(256, DEDUP ? 3 : 4) k_persistent(PersistArgs A) {
     __shared__ double s_ratio[RATIO_N];
     load_ratio(s_ratio, A.P.ratio);
 
     const double level = A.round->level;
     const unsigned long long n_in = A.round->n_in;
-    const unsigned lane = lane_id();
 
     Lane<KIND> L;
     L.have_next = false;
+    L.searching = false;
     L.ms_some = false;
     bool has = false;
     unsigned long long slot = 0;
-    WaveQueue q{0, 0, false, false};
+    WaveQueue q{0, 0, 0, false, false};
     uint64_t slice_steps = 0, tick = 0;
     unsigned long long my_steps = 0, my_dead = 0;
     double my_max = 0.0;
@@ -166,9 +207,8 @@ __global__ void __launch_bounds__(256, 4) k_persistent(PersistArgs A) {
             uint64_t need = __ballot(!has);
             if (need) {
                 unsigned long long w = wave_claim(q, need, !has, &A.ctr->head, n_in, A.round->chunk);
-                if (w != ~0ull && !(A.lin.flags[w] & FLAG_DEAD)) {
+                if (w != ~0ull && !(lane_load<KIND>(L, A.S, A.lin, w) & FLAG_DEAD)) {
                     slot = w;
-                    lane_load<KIND>(L, A.S, A.lin, slot);
                     has = true;
                     slice_steps = 0;
                     tick = 0;
@@ -180,35 +220,38 @@ __global__ void __launch_bounds__(256, 4) k_persistent(PersistArgs A) {
             continue;
         }
 
-        // up to inner_steps steps per lane between wave-level refills
+        // up to inner_steps iterations per lane between wave-level refills
         if (has) {
 #pragma unroll 1
             for (unsigned inner = 0; inner < A.inner_steps; inner++) {
-                lane_peek<KIND>(L, A.P, s_ratio);
+                if (!L.have_next) lane_search_window<KIND>(L, A.P, s_ratio);
+                if (!wave_ready(L.have_next, A.exec_fill) || !L.have_next) continue;
                 // Leaving this launch (water level or tick limit) and reaching
                 // the step slice both end the slice: its min speciation sample
                 // is inserted like on reclassification (monolithic/mod.rs:205-219).
                 // A tick-limited lineage starts a fresh slice when resumed.
                 bool leave = L.next_t >= level || (A.tick_steps && tick >= A.tick_steps);
-                if (leave || slice_steps >= A.P.step_slice) {
-                    bool dup = A.cache && L.ms_some &&
-                               cache_insert_concurrent(A.cache, A.cache_cap, lane_sample<KIND>(L, A.S));
-                    L.ms_some = false;
-                    slice_steps = 0;
-                    if (dup) {
-                        A.lin.flags[slot] = FLAG_DEAD;
-                        my_dead++;
-                        has = false;
-                        break;
-                    }
-                    if (leave) {
-                        lane_store<KIND>(L, A.S, A.lin, slot);
-                        has = false;
-                        break;
+                if (DEDUP) {
+                    if (leave || slice_steps >= A.P.step_slice) {
+                        bool dup = L.ms_some &&
+                                   cache_insert_concurrent(A.cache, A.cache_cap, lane_sample<KIND>(L, A.S));
+                        L.ms_some = false;
+                        slice_steps = 0;
+                        if (dup) {
+                            A.lin.flags[slot] = FLAG_DEAD;
+                            my_dead++;
+                            has = false;
+                            break;
+                        }
                     }
                 }
+                if (leave) {
+                    lane_store<KIND>(L, A.S, A.lin, slot);
+                    has = false;
+                    break;
+                }
                 DevEvent ev;
-                int r = lane_execute<KIND>(L, A.S, A.P, ev);
+                int r = lane_execute<KIND, DEDUP>(L, A.S, A.P, ev);
                 my_steps++;
                 slice_steps++;
                 tick++;
@@ -244,13 +287,14 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
 
     Lane<KIND> L;
     L.have_next = false;
+    L.searching = false;
     L.ms_some = false;
     DevParams P = A.P;
     double level = 0.0;
     int req = -1;
     bool has = false;
     unsigned long long slot = 0;
-    WaveQueue q{0, 0, false, false};
+    WaveQueue q{0, 0, 0, false, false};
     uint64_t slice_steps = 0, tick = 0;
     unsigned long long acc_steps = 0, acc_dead = 0;
     double acc_max = 0.0;
@@ -279,7 +323,7 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
                     }
                     const ServeReq &R = A.reqs[lo];
                     unsigned long long s = w - s_base[lo];
-                    if (!(R.lin.flags[s] & FLAG_DEAD)) {
+                    if (!(lane_load<KIND>(L, A.S, R.lin, s) & FLAG_DEAD)) {
                         if (lo != req) {
                             flush(req);
                             req = lo;
@@ -288,7 +332,6 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
                             level = R.level;
                         }
                         slot = s;
-                        lane_load<KIND>(L, A.S, R.lin, slot);
                         has = true;
                         slice_steps = 0;
                         tick = 0;
@@ -304,7 +347,8 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
             const ServeReq &R = A.reqs[req];
 #pragma unroll 1
             for (int inner = 0; inner < 4; inner++) {
-                lane_peek<KIND>(L, P, s_ratio);
+                if (!L.have_next) lane_search_window<KIND>(L, P, s_ratio);
+                if (!wave_ready(L.have_next, 5) || !L.have_next) continue;
                 bool leave = L.next_t >= level || (A.tick_steps && tick >= A.tick_steps);
                 if (leave || slice_steps >= P.step_slice) {
                     bool dup = R.cache && L.ms_some &&
@@ -652,34 +696,117 @@ constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 
 // Sort sizes may come from device memory (graph replays): n = *end - *start
-// and the events start at events + *start.
+// and the events start at events + *start.  A non-null `gate` makes every
+// kernel of a pass a no-op while *gate == 0 (the tie passes of a captured
+// sort, which only run when the round has equal event times).
 struct SortSize {
     uint64_t n;
     const unsigned long long *end;
     const unsigned long long *start;
+    const unsigned long long *gate;
     __device__ __forceinline__ uint64_t get() const { return end ? (*end - (start ? *start : 0ull)) : n; }
     __device__ __forceinline__ uint64_t offset() const { return start ? *start : 0ull; }
     // radix tiles actually used (the launch grid may be sized for capacity)
     __device__ __forceinline__ uint32_t tiles() const { return (uint32_t)((get() + RS_TILE - 1) / RS_TILE); }
+    __device__ __forceinline__ bool skip() const { return gate && *gate == 0ull; }
 };
 
+// The PackedEvent order of speciation events (necsim/core/src/event.rs:256-280)
+// compares event_time, then the origin (x, y, index), the target (equal to
+// the origin), prior_time and the lineage reference.  As LSD radix keys, from
+// the least to the most significant field; positive / non-negative f64 sort
+// like their bit patterns.
+constexpr int SORT_FIELDS = 6;
+constexpr int F_TIME = 5;
+constexpr int FIELD_DIGITS[SORT_FIELDS] = {8, 8, 4, 4, 4, 8}; // ref, prior, index, y, x, time
+static_assert((8 + 8 + 4 + 4 + 4 + 8) % 2 == 0, "the gated tie passes must not change the ping-pong buffer");
+
+__device__ __forceinline__ unsigned long long event_field(const DevEvent &e, int f) {
+    switch (f) {
+    case 0: return e.ref;
+    case 1: return bits(e.prior_time) & 0x7fffffffffffffffull;
+    case 2: return e.index;
+    case 3: return e.y;
+    case 4: return e.x;
+    default: return bits(e.event_time) & 0x7fffffffffffffffull;
+    }
+}
+
+// Word layout of SortBuffers::minmax
+constexpr int MM_TIME = 0;   // [min, max] of the event-time keys
+constexpr int MM_TIES = 2;   // number of adjacent equal event times after the time sort
+constexpr int MM_FIELDS = 4; // [min, max] per field
+constexpr int MM_WORDS = MM_FIELDS + 2 * SORT_FIELDS;
+static_assert(MM_WORDS <= SORT_MINMAX_WORDS, "sort range words");
+
+// min / max over the valid lanes of a wave, one atomic pair per wave
+__device__ __forceinline__ void wave_minmax(unsigned long long k, bool valid, unsigned long long *mm) {
+    unsigned long long lo = valid ? k : ~0ull, hi = valid ? k : 0ull;
+    for (int off = 32; off; off >>= 1) {
+        unsigned long long l2 = __shfl_xor(lo, off), h2 = __shfl_xor(hi, off);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    bool any = __any(valid);
+    if (any && lane_id() == 0) {
+        atomicMin(&mm[0], lo);
+        atomicMax(&mm[1], hi);
+    }
+}
+
+__global__ void k_sort_init(unsigned long long *mm) {
+    int t = threadIdx.x;
+    if (t < MM_WORDS) mm[t] = (t == MM_TIME || (t >= MM_FIELDS && ((t - MM_FIELDS) & 1) == 0)) ? ~0ull : 0ull;
+}
+
+// time keys, identity values and the time key range
 __global__ void k_event_keys(const DevEvent *ev0, SortSize sz, unsigned long long *keys, uint32_t *vals,
-                             unsigned long long *minmax) {
+                             unsigned long long *mm) {
     uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t n = sz.get();
     const DevEvent *ev = ev0 + sz.offset();
-    if (p >= n) return;
-    // positive f64 (PositiveF64 total order) sorts like its bit pattern
-    unsigned long long k = bits(ev[p].event_time) & 0x7fffffffffffffffull;
-    keys[p] = k;
-    vals[p] = (uint32_t)p;
-    atomicMin(&minmax[0], k);
-    atomicMax(&minmax[1], k);
+    bool valid = p < n;
+    unsigned long long k = valid ? event_field(ev[p], F_TIME) : 0ull;
+    if (valid) {
+        keys[p] = k;
+        vals[p] = (uint32_t)p;
+    }
+    wave_minmax(k, valid, mm + MM_TIME);
+}
+
+// number of adjacent equal keys in the sorted time keys
+__global__ void k_tie_count(const unsigned long long *keys, SortSize sz, unsigned long long *count) {
+    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t n = sz.get();
+    bool tie = p + 1 < n && keys[p] == keys[p + 1];
+    uint64_t m = __ballot(tie);
+    if (m && lane_id() == 0) atomicAdd(count, (unsigned long long)__popcll(m));
+}
+
+// per-field key ranges (the digits a field's passes must cover)
+__global__ void k_field_minmax(const DevEvent *ev0, SortSize sz, unsigned long long *mm) {
+    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t n = sz.get();
+    const DevEvent *ev = ev0 + sz.offset();
+    bool valid = p < n;
+    DevEvent e{};
+    if (valid) e = ev[p];
+    for (int f = 0; f < SORT_FIELDS; f++) wave_minmax(event_field(e, f), valid, mm + MM_FIELDS + 2 * f);
+}
+
+// keys[p] = field f of the event at the current permutation's position p
+__global__ void k_field_keys(const DevEvent *ev0, const uint32_t *vals, unsigned long long *keys, int f,
+                             SortSize sz) {
+    if (sz.skip()) return;
+    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const DevEvent *ev = ev0 + sz.offset();
+    if (p < sz.get()) keys[p] = event_field(ev[vals[p]], f);
 }
 
 __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const unsigned long long *keys, SortSize sz,
                                                            int shift, uint32_t *hist) {
     __shared__ uint32_t h[256];
+    if (sz.skip()) return;
     const uint64_t n = sz.get();
     const uint32_t nb = sz.tiles();
     if (blockIdx.x >= nb) return;
@@ -700,6 +827,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const unsigned lon
                                                               SortSize sz, int shift,
                                                               const uint32_t *offsets) {
     constexpr int W = RS_THREADS / 64;
+    if (sz.skip()) return;
     const uint64_t n = sz.get();
     const uint32_t nb = sz.tiles();
     if (blockIdx.x >= nb) return;
@@ -740,41 +868,6 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const unsigned lon
     }
 }
 
-// PackedEvent order for speciation events (necsim/core/src/event.rs:256-280):
-// event_time, origin (x, y, index), target (= origin), prior_time, reference
-__device__ __forceinline__ bool event_less(const DevEvent &a, const DevEvent &b) {
-    uint64_t ta = bits(a.event_time) & 0x7fffffffffffffffull, tb = bits(b.event_time) & 0x7fffffffffffffffull;
-    if (ta != tb) return ta < tb;
-    if (a.x != b.x) return a.x < b.x;
-    if (a.y != b.y) return a.y < b.y;
-    if (a.index != b.index) return a.index < b.index;
-    uint64_t pa = bits(a.prior_time) & 0x7fffffffffffffffull, pb = bits(b.prior_time) & 0x7fffffffffffffffull;
-    if (pa != pb) return pa < pb;
-    return a.ref < b.ref;
-}
-
-// Equal event times are rare (only duplicate lineages share them): order
-// each run of equal keys by the full PackedEvent order.
-__global__ void k_fix_ties(const unsigned long long *keys, uint32_t *vals, const DevEvent *ev0, SortSize sz) {
-    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t n = sz.get();
-    const DevEvent *ev = ev0 + sz.offset();
-    if (p + 1 >= n) return;
-    if (keys[p] != keys[p + 1]) return;
-    if (p > 0 && keys[p - 1] == keys[p]) return; // not the run start
-    uint64_t e = p + 1;
-    while (e < n && keys[e] == keys[p]) e++;
-    for (uint64_t a = p + 1; a < e; a++) {
-        uint32_t v = vals[a];
-        uint64_t c = a;
-        while (c > p && event_less(ev[v], ev[vals[c - 1]])) {
-            vals[c] = vals[c - 1];
-            c--;
-        }
-        vals[c] = v;
-    }
-}
-
 __global__ void k_gather_events(const DevEvent *ev0, const uint32_t *vals, DevEvent *out, SortSize sz) {
     uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const DevEvent *ev = ev0 + sz.offset();
@@ -794,33 +887,41 @@ __global__ void k_reset_counters(Counters *c) {
     c->n_out = 0;
     c->n_dead = 0;
 }
-__global__ void k_set_minmax(unsigned long long *mm) {
-    mm[0] = ~0ull;
-    mm[1] = 0;
-}
 
 // ---------------------------------------------------------------------------
 // launchers
 static inline unsigned grid_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream) {
+template <bool DEDUP>
+static void launch_persistent_t(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream) {
     switch (kind) {
-    case K_NON_SPATIAL: hipLaunchKernelGGL(k_persistent<K_NON_SPATIAL>, dim3(blocks), dim3(threads), 0, stream, a); break;
-    case K_ALMOST_INFINITE: hipLaunchKernelGGL(k_persistent<K_ALMOST_INFINITE>, dim3(blocks), dim3(threads), 0, stream, a); break;
-    case K_SEPARABLE: hipLaunchKernelGGL(k_persistent<K_SEPARABLE>, dim3(blocks), dim3(threads), 0, stream, a); break;
-    default: hipLaunchKernelGGL(k_persistent<K_PACKED>, dim3(blocks), dim3(threads), 0, stream, a); break;
+    case K_NON_SPATIAL: hipLaunchKernelGGL((k_persistent<K_NON_SPATIAL, DEDUP>), dim3(blocks), dim3(threads), 0, stream, a); break;
+    case K_ALMOST_INFINITE: hipLaunchKernelGGL((k_persistent<K_ALMOST_INFINITE, DEDUP>), dim3(blocks), dim3(threads), 0, stream, a); break;
+    case K_SEPARABLE: hipLaunchKernelGGL((k_persistent<K_SEPARABLE, DEDUP>), dim3(blocks), dim3(threads), 0, stream, a); break;
+    default: hipLaunchKernelGGL((k_persistent<K_PACKED, DEDUP>), dim3(blocks), dim3(threads), 0, stream, a); break;
     }
 }
 
-int persistent_max_blocks_per_cu(int kind, int threads) {
+void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream) {
+    if (a.cache) launch_persistent_t<true>(kind, a, blocks, threads, stream);
+    else launch_persistent_t<false>(kind, a, blocks, threads, stream);
+}
+
+template <bool DEDUP> static int persistent_max_blocks_per_cu_t(int kind, int threads) {
     int n = 0;
     switch (kind) {
-    case K_NON_SPATIAL: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_NON_SPATIAL>, threads, 0); break;
-    case K_ALMOST_INFINITE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_ALMOST_INFINITE>, threads, 0); break;
-    case K_SEPARABLE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_SEPARABLE>, threads, 0); break;
-    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_PACKED>, threads, 0); break;
+    case K_NON_SPATIAL: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_NON_SPATIAL, DEDUP>, threads, 0); break;
+    case K_ALMOST_INFINITE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_ALMOST_INFINITE, DEDUP>, threads, 0); break;
+    case K_SEPARABLE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_SEPARABLE, DEDUP>, threads, 0); break;
+    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_PACKED, DEDUP>, threads, 0); break;
     }
     return n > 0 ? n : 1;
+}
+
+// resident blocks per CU of the persistent kernel variant with / without dedup
+int persistent_max_blocks_per_cu(int kind, int threads, bool dedup) {
+    return dedup ? persistent_max_blocks_per_cu_t<true>(kind, threads)
+                 : persistent_max_blocks_per_cu_t<false>(kind, threads);
 }
 
 void launch_serve(int kind, const ServeArgs &a, int blocks, int threads, hipStream_t stream) {
@@ -891,6 +992,7 @@ uint32_t sort_blocks(uint64_t n) { return (uint32_t)((n + RS_TILE - 1) / RS_TILE
 // exclusive scan of the 256 x tiles digit histogram of a sort
 __global__ void __launch_bounds__(1024) k_scan_hist(uint32_t *data, SortSize sz) {
     __shared__ uint32_t partial[1024];
+    if (sz.skip()) return;
     const uint64_t n = (uint64_t)256 * sz.tiles();
     const int T = blockDim.x;
     uint64_t per = (n + T - 1) / T;
@@ -913,56 +1015,86 @@ __global__ void __launch_bounds__(1024) k_scan_hist(uint32_t *data, SortSize sz)
     }
 }
 
-static void sort_passes(const DevEvent *events, SortSize sz, uint64_t cap, DevEvent *out, SortBuffers &sb,
-                        hipStream_t stream, int last_digit) {
+// digit passes first..last over (keys, vals)[cur]; returns the buffer that
+// holds the result
+static int radix_passes(SortSize sz, uint64_t cap, SortBuffers &sb, hipStream_t stream, int first, int last,
+                        int cur) {
     uint32_t nb = sort_blocks(cap);
-    unsigned g = grid_for(cap, 256);
-    int cur = 0;
-    for (int d = 0; d <= last_digit; d++) {
+    for (int d = first; d <= last; d++) {
         hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sz, 8 * d, sb.hist);
         hipLaunchKernelGGL(k_scan_hist, dim3(1), dim3(1024), 0, stream, sb.hist, sz);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sb.vals[cur],
                            sb.keys[cur ^ 1], sb.vals[cur ^ 1], sz, 8 * d, sb.hist);
         cur ^= 1;
     }
-    hipLaunchKernelGGL(k_fix_ties, dim3(g), dim3(256), 0, stream, sb.keys[cur], sb.vals[cur], events, sz);
-    hipLaunchKernelGGL(k_gather_events, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], out, sz);
+    return cur;
 }
 
+static int highest_digit(unsigned long long diff) {
+    for (int d = 7; d >= 0; d--)
+        if ((diff >> (8 * d)) & 255ull) return d;
+    return -1;
+}
+
+// Sort of one round's speciation events by the PackedEvent order: an LSD
+// radix sort of the event times; if two events share a time (duplicate
+// lineages when dedup is off), a full LSD radix sort over every field of the
+// order.  Digits above the highest one in which a key range differs are
+// equal for every key and are skipped.
 void sort_events(const DevEvent *events, uint64_t n, DevEvent *out, SortBuffers &sb, hipStream_t stream,
                  bool fixed_passes) {
     if (n == 0) return;
-    SortSize sz{n, nullptr, nullptr};
-    hipLaunchKernelGGL(k_set_minmax, dim3(1), dim3(1), 0, stream, sb.minmax);
-    hipLaunchKernelGGL(k_event_keys, dim3(grid_for(n, 256)), dim3(256), 0, stream, events, sz, sb.keys[0],
-                       sb.vals[0], sb.minmax);
+    SortSize sz{n, nullptr, nullptr, nullptr};
+    unsigned g = grid_for(n, 256);
+    hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, stream, sb.minmax);
+    hipLaunchKernelGGL(k_event_keys, dim3(g), dim3(256), 0, stream, events, sz, sb.keys[0], sb.vals[0], sb.minmax);
+    unsigned long long mm[MM_WORDS];
     int last = 7;
     if (!fixed_passes) {
-        unsigned long long mm[2];
+        (void)hipMemcpyAsync(mm, sb.minmax, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, stream);
+        (void)hipStreamSynchronize(stream);
+        last = highest_digit(mm[MM_TIME] ^ mm[MM_TIME + 1]);
+    }
+    int cur = radix_passes(sz, n, sb, stream, 0, last, 0);
+    if (n > 1) {
+        hipLaunchKernelGGL(k_tie_count, dim3(g), dim3(256), 0, stream, sb.keys[cur], sz, sb.minmax + MM_TIES);
+        hipLaunchKernelGGL(k_field_minmax, dim3(g), dim3(256), 0, stream, events, sz, sb.minmax);
         (void)hipMemcpyAsync(mm, sb.minmax, sizeof(mm), hipMemcpyDeviceToHost, stream);
         (void)hipStreamSynchronize(stream);
-        // digits above the highest differing bit of min and max are equal
-        // for every key (every key lies between min and max)
-        unsigned long long diff = mm[0] ^ mm[1];
-        last = -1;
-        for (int d = 7; d >= 0; d--)
-            if ((diff >> (8 * d)) & 255ull) {
-                last = d;
-                break;
+        if (mm[MM_TIES]) {
+            for (int f = 0; f < SORT_FIELDS; f++) {
+                int lf = fixed_passes ? FIELD_DIGITS[f] - 1
+                                      : highest_digit(mm[MM_FIELDS + 2 * f] ^ mm[MM_FIELDS + 2 * f + 1]);
+                if (lf < 0) continue;
+                hipLaunchKernelGGL(k_field_keys, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], sb.keys[cur], f, sz);
+                cur = radix_passes(sz, n, sb, stream, 0, lf, cur);
             }
+        }
     }
-    sort_passes(events, sz, n, out, sb, stream, last);
+    hipLaunchKernelGGL(k_gather_events, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], out, sz);
 }
 
+// Capturable variant: the event range is read from device memory, grids are
+// sized for `cap` events, every digit pass is launched, and the tie passes
+// are gated on the device-side tie count (an even number of passes, so the
+// result lies in the same buffer whether they run or not).
 void sort_events_device_sized(const DevEvent *events, const unsigned long long *end,
                               const unsigned long long *start, uint64_t cap, DevEvent *out, SortBuffers &sb,
                               hipStream_t stream) {
     if (cap == 0) return;
-    SortSize sz{0, end, start};
-    hipLaunchKernelGGL(k_set_minmax, dim3(1), dim3(1), 0, stream, sb.minmax);
-    hipLaunchKernelGGL(k_event_keys, dim3(grid_for(cap, 256)), dim3(256), 0, stream, events, sz, sb.keys[0],
-                       sb.vals[0], sb.minmax);
-    sort_passes(events, sz, cap, out, sb, stream, 7);
+    SortSize sz{0, end, start, nullptr};
+    unsigned g = grid_for(cap, 256);
+    hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, stream, sb.minmax);
+    hipLaunchKernelGGL(k_event_keys, dim3(g), dim3(256), 0, stream, events, sz, sb.keys[0], sb.vals[0], sb.minmax);
+    int cur = radix_passes(sz, cap, sb, stream, 0, 7, 0);
+    hipLaunchKernelGGL(k_tie_count, dim3(g), dim3(256), 0, stream, sb.keys[cur], sz, sb.minmax + MM_TIES);
+    SortSize gz = sz;
+    gz.gate = sb.minmax + MM_TIES;
+    for (int f = 0; f < SORT_FIELDS; f++) {
+        hipLaunchKernelGGL(k_field_keys, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], sb.keys[cur], f, gz);
+        cur = radix_passes(gz, cap, sb, stream, 0, FIELD_DIGITS[f] - 1, cur);
+    }
+    hipLaunchKernelGGL(k_gather_events, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], out, sz);
 }
 
 uint32_t soa_compact_blocks(uint64_t n) { return (uint32_t)((n + SC_TILE - 1) / SC_TILE); }

@@ -43,19 +43,6 @@ struct nh_tables {
 
 namespace {
 
-template <class T> int dalloc(T **p, size_t n) {
-    *p = nullptr;
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
-    if (e != hipSuccess)
-        return set_error(NH_E_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
-    return NH_OK;
-}
-template <class T> void dfree(T *&p) {
-    if (p) (void)hipFree((void *)p);
-    p = nullptr;
-}
-
 int kind_of(int32_t scenario_kind) {
     switch (scenario_kind) {
     case NH_SCENARIO_NON_SPATIAL: return K_NON_SPATIAL;
@@ -396,6 +383,7 @@ extern "C" int nh_job_create(nh_ctx *ctx, const nh_tables *tables, const nh_job_
         return set_error(NH_E_INVALID, "turnover rate must be positive");
     if (!(p->delta_t > 0.0) || std::isinf(p->delta_t)) return set_error(NH_E_INVALID, "delta_t must be positive");
     if (p->step_slice == 0) return set_error(NH_E_INVALID, "step_slice must be positive");
+    if (p->exec_fill > 8) return set_error(NH_E_INVALID, "exec_fill is a number of eighths (0..8)");
     if (p->form != NH_FORM_PLAIN && p->form != NH_FORM_PERSISTENT) return set_error(NH_E_INVALID, "unknown form");
     if (p->dedup_mode < NH_DEDUP_NONE || p->dedup_mode > NH_DEDUP_ABSOLUTE) return set_error(NH_E_INVALID, "dedup mode");
     if (p->dedup_mode == NH_DEDUP_RELATIVE && !(p->dedup_factor > 0.0)) return set_error(NH_E_INVALID, "dedup factor");
@@ -485,7 +473,7 @@ static int job_start(nh_job *j) {
     NH_TRY(dalloc(&j->sb.vals[0], n));
     NH_TRY(dalloc(&j->sb.vals[1], n));
     NH_TRY(dalloc(&j->sb.hist, (size_t)256 * sort_blocks(n) + 1));
-    NH_TRY(dalloc(&j->sb.minmax, 2));
+    NH_TRY(dalloc(&j->sb.minmax, SORT_MINMAX_WORDS));
     if (j->cache_cap) {
         NH_TRY(dalloc(&j->d_cache, 4 * j->cache_cap));
         launch_fill_u64(j->d_cache, 0, 4 * j->cache_cap, st);
@@ -502,7 +490,7 @@ static int job_start(nh_job *j) {
     if (p.form == NH_FORM_PERSISTENT) {
         NH_TRY(j->lin[1].alloc(n));
         NH_TRY(dalloc(&j->d_blk, soa_compact_blocks(n) + 1));
-        int per_cu = persistent_max_blocks_per_cu(j->tables->kind, j->threads);
+        int per_cu = persistent_max_blocks_per_cu(j->tables->kind, j->threads, j->cache_cap != 0);
         if (p.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)p.blocks_per_cu);
         j->blocks = per_cu * (int)j->ctx->cus;
     } else {
@@ -704,7 +692,8 @@ static PersistArgs persist_args(nh_job *j, int cur) {
     a.events = j->d_events;
     a.ctr = j->d_ctr;
     a.tick_steps = 0;
-    a.inner_steps = j->p.inner_steps ? j->p.inner_steps : 8;
+    a.inner_steps = j->p.inner_steps ? j->p.inner_steps : 16;
+    a.exec_fill = j->p.exec_fill ? j->p.exec_fill : 5;
     return a;
 }
 
@@ -893,12 +882,14 @@ extern "C" int nh_autotune(nh_ctx *ctx, const nh_tables *tables, const nh_job_pa
     double best = 1e300;
     nh_job_params winner = *base;
     winner.form = NH_FORM_PERSISTENT;
-    for (uint32_t threads : {64u, 128u, 256u}) {
-        int max_per_cu = persistent_max_blocks_per_cu(tables->kind, (int)threads);
+    for (uint32_t threads : {128u, 256u}) {
+        int max_per_cu = persistent_max_blocks_per_cu(tables->kind, (int)threads,
+                                                      base->dedup_mode != NH_DEDUP_NONE);
         std::vector<uint32_t> residency = {(uint32_t)max_per_cu};
         if (max_per_cu >= 2) residency.push_back((uint32_t)(max_per_cu / 2));
         for (uint32_t per_cu : residency) {
-            for (uint32_t inner : {1u, 4u, 16u})
+            for (uint32_t inner : {4u, 16u})
+            for (uint32_t fill : {1u, 5u, 8u})
             for (uint64_t slice : slices) {
                 nh_job_params p = *base;
                 p.form = NH_FORM_PERSISTENT;
@@ -906,6 +897,7 @@ extern "C" int nh_autotune(nh_ctx *ctx, const nh_tables *tables, const nh_job_pa
                 p.blocks_per_cu = per_cu;
                 p.step_slice = slice;
                 p.inner_steps = inner;
+                p.exec_fill = fill;
                 p.has_pause_before = 1;
                 p.pause_before = window;
                 p.use_graph = 0;

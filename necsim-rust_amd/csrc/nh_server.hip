@@ -46,18 +46,6 @@ double mono_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-template <class T> int dalloc(T **p, size_t n) {
-    *p = nullptr;
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
-    if (e != hipSuccess) return set_error(NH_E_NOMEM, "hipMalloc of %zu bytes: %s", n * sizeof(T), hipGetErrorString(e));
-    return NH_OK;
-}
-template <class T> void dfree(T *&p) {
-    if (p) (void)hipFree((void *)p);
-    p = nullptr;
-}
-
 struct Soa {
     LineageSoA s{};
     int alloc(uint64_t n) {
@@ -177,7 +165,7 @@ extern "C" int nh_server_create(nh_ctx *ctx, const nh_tables *tables, const nh_s
     s->sb.capacity = cap;
     if ((rc = dalloc(&s->sb.keys[0], cap)) || (rc = dalloc(&s->sb.keys[1], cap)) ||
         (rc = dalloc(&s->sb.vals[0], cap)) || (rc = dalloc(&s->sb.vals[1], cap)) ||
-        (rc = dalloc(&s->sb.hist, (size_t)256 * sort_blocks(cap) + 1)) || (rc = dalloc(&s->sb.minmax, 2)))
+        (rc = dalloc(&s->sb.hist, (size_t)256 * sort_blocks(cap) + 1)) || (rc = dalloc(&s->sb.minmax, SORT_MINMAX_WORDS)))
         return fail(rc);
     s->h_reqs.resize(SERVE_MAX_REQS);
     s->h_ctr.resize(SERVE_MAX_REQS);

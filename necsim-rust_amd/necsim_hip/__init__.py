@@ -12,7 +12,7 @@ from .algorithm import (PERSISTENT, PLAIN, AlmostInfiniteNormalDispersalScenario
                         HipIndependentAlgorithm, IndependentArguments, Job, NonSpatialScenario,
                         OriginPreSampler, ScenarioCogs, Server, SimulationOutcome,
                         SpatiallyExplicitUniformTurnoverScenario, Tables, WyHash,
-                        autotune, build_alias_tables)
+                        autotune, build_alias_tables, copy_calibrate)
 from .maps import generate_map
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -50,7 +50,7 @@ class JobParams(ctypes.Structure):
                 ("event_slice_factor", c_f64), ("event_slice_capacity", c_u64),
                 ("has_pause_before", c_i32), ("pause_before", c_f64), ("resume_after", c_f64),
                 ("form", c_i32), ("block_size", c_u32), ("blocks_per_cu", c_u32),
-                ("use_graph", c_u32), ("inner_steps", c_u32)]
+                ("use_graph", c_u32), ("inner_steps", c_u32), ("exec_fill", c_u32)]
 
 
 class Outcome(ctypes.Structure):
@@ -111,6 +111,8 @@ SIGNATURES = {
     "nh_build_alias_tables": (ctypes.c_int, [c_u32, c_u32, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp,
                                              c_vp, c_sz, ctypes.POINTER(c_sz)]),
     "nh_debug_eval": (ctypes.c_int, [c_vp, c_i32, c_vp, c_vp, c_sz]),
+    "nh_copy_calibrate": (ctypes.c_int, [c_vp, c_sz, c_i32, ctypes.POINTER(ctypes.c_double)]),
+    "nh_debug_sort_events": (ctypes.c_int, [c_vp, c_vp, c_sz, c_i32, c_vp]),
     "nh_sample_origins": (ctypes.c_int, [ctypes.POINTER(Scenario), c_f64, c_u32, c_u32, c_u32, c_u32,
                                          c_u32, c_vp, c_sz, ctypes.POINTER(c_sz)]),
 }

@@ -91,6 +91,7 @@ class IndependentArguments:
     blocks_per_cu: int = 0
     use_graph: bool = False
     inner_steps: int = 0
+    exec_fill: int = 0
     device: int = 0
 
 
@@ -373,7 +374,7 @@ def _job_params(args, rng, speciation_probability, turnover_rate, pause_before=N
                           pause_before=0.0 if pause_before is None else pause_before,
                           resume_after=resume_after, form=args.form, block_size=args.block_size,
                           blocks_per_cu=args.blocks_per_cu, use_graph=1 if args.use_graph else 0,
-                          inner_steps=args.inner_steps)
+                          inner_steps=args.inner_steps, exec_fill=args.exec_fill)
 
 
 def autotune(ctx, tables, args: IndependentArguments, rng: WyHash, cogs, sample, window,
@@ -391,7 +392,16 @@ def autotune(ctx, tables, args: IndependentArguments, rng: WyHash, cogs, sample,
     return dataclasses.replace(args, form=PERSISTENT, step_slice=int(out.step_slice),
                                block_size=int(out.block_size),
                                blocks_per_cu=int(out.blocks_per_cu),
-                               inner_steps=int(out.inner_steps)), ms.value
+                               inner_steps=int(out.inner_steps),
+                               exec_fill=int(out.exec_fill)), ms.value
+
+
+def copy_calibrate(ctx, nbytes, reps=10):
+    """nh_copy_calibrate: mean duration (ms) of a plain copy kernel over
+    `nbytes` and the copy bandwidth in GB/s (bytes read + written)."""
+    ms = ctypes.c_double(0)
+    check(lib().nh_copy_calibrate(ctx.handle, int(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9
 
 
 class HipIndependentAlgorithm:

@@ -111,6 +111,7 @@ pub struct nh_job_params {
     pub blocks_per_cu: u32,
     pub use_graph: u32,
     pub inner_steps: u32,
+    pub exec_fill: u32,
 }
 
 #[repr(C)]
