@@ -33,6 +33,13 @@ import numpy as np  # noqa: E402
 import necsim_hip as nh  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# MI355X VALU issue: 4 SIMDs per CU, each retiring one full-rate wave64
+# vector instruction per 4 cycles (16 lanes per cycle) = one wave
+# instruction per CU per cycle at the 2.4 GHz peak engine clock (the same
+# rate as the 78.6 TFLOP/s f64 vector FMA peak: 64 lanes x 2 flops per CU-cycle)
+ENGINE_CLOCK_HZ = 2.4e9
+PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "k_persistent_config4_window.txt")
+GATHER_PROFILE = os.path.join(ROOT, "profiles", "k_persistent_alias_gather_map256.txt")
 # bytes of lineage state one persistent round reads and writes per lineage
 # (SoA: ref 8, t_last 8, x/y/index 12, next_t 8, next_state 8, flags 1)
 LINEAGE_STATE_BYTES = 45
@@ -122,22 +129,47 @@ def config2_run(ctx):
 GATHER_BYTES_PER_STEP = 48  # cell record 32 B + alias atom 16 B per dispersal step
 
 
-def config3_run(ctx, window):
-    """config 3 (128x128 map, nu 1e-5): the alias-gather workload"""
-    h, rp, c, w = nh.generate_map(128, 128, seed=42)
-    scn = nh.SpatiallyExplicitUniformTurnoverScenario(128, 128, h, rp, c, w)
-    cogs = scn.new(1e-5)
+def map_window_run(ctx, size, nu, window, dedup, seed=42):
+    """A window of the spatially explicit map scenario (the generator of
+    configs 3 and 5): every dispersal gathers the target cell's record and
+    one alias atom.  Returns timings and the gather-bound roofline."""
+    h, rp, c, w = nh.generate_map(size, size, seed=42)
+    scn = nh.SpatiallyExplicitUniformTurnoverScenario(size, size, h, rp, c, w)
+    cogs = scn.new(nu)
     lin = cogs.sample_habitat(nh.OriginPreSampler.all())
     tables = nh.Tables(ctx, cogs)
-    wall, res = timed_runs(ctx, tables, cogs, lin, nh.IndependentArguments(), 42, window, 1, 1)
+    atoms = int(len(scn.tables()[2]))
+    wall, res = timed_runs(ctx, tables, cogs, lin, nh.IndependentArguments(dedup_cache=dedup), seed, window, 1, 1)
     o, ms, k = res[0]
     achieved = o.steps * GATHER_BYTES_PER_STEP / (ms * 1e-3) / 1e9 if ms else 0.0
-    return {"lineages": int(len(lin)), "window": window, "steps": int(o.steps), "ms": wall * 1e3,
-            "value": o.steps / wall, "kernel_ms": ms, "atoms": int(len(scn.tables()[2])),
+    bound = HBM_PEAK_GBS * 1e9 / GATHER_BYTES_PER_STEP
+    return {"map": f"{size}x{size}", "lineages": int(len(lin)), "window": window, "speciation": nu,
+            "dedup": "none" if dedup.mode == 0 else "relative 1.0", "steps": int(o.steps), "ms": wall * 1e3,
+            "value": o.steps / wall, "kernel_ms": ms, "launches": int(k), "atoms": atoms,
+            "table_bytes": atoms * 16 + size * size * 32,
             "gather_roofline": {"bound": "hbm", "bytes_per_step": GATHER_BYTES_PER_STEP,
                                 "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                "frac": round(achieved / HBM_PEAK_GBS, 5),
-                                "roofline_steps_per_s": HBM_PEAK_GBS * 1e9 / GATHER_BYTES_PER_STEP}}
+                                "frac": round(achieved / HBM_PEAK_GBS, 5), "roofline_steps_per_s": bound,
+                                "steps_per_s_frac_of_roofline": round(o.steps / (ms * 1e-3) / bound, 5)}}
+
+
+def config3_run(ctx, window):
+    """config 3 (128x128 map, nu 1e-5, the reference's default dedup)"""
+    return map_window_run(ctx, 128, 1e-5, window, nh.DedupCache.Relative(1.0))
+
+
+def gather_run(ctx, window):
+    """the alias-gather workload: 256x256 map (config 5's generator; its
+    tables fill the 256 MB last-level cache), no dedup.  Adds the HBM
+    figures of the committed rocprofv3 summary of the same window."""
+    r = map_window_run(ctx, 256, 1e-5, window, nh.DedupCache.none())
+    prof = read_profile_summary(GATHER_PROFILE)
+    if prof.get("window") == window:
+        r["rocprof"] = {k: prof.get(k) for k in ("kernel_mean_ms", "hbm_bytes_per_launch", "hbm_rate_GBps",
+                                                  "hbm_frac_of_peak", "fetch_over_gather_bytes",
+                                                  "valu_issue_fraction", "valu_lane_utilization")}
+        r["rocprof"]["source"] = os.path.relpath(GATHER_PROFILE, ROOT)
+    return r
 
 
 def serving_run(ctx, n_requests=150, rate=50.0, cap=1000.0, tick_steps=250, seed=1):
@@ -196,6 +228,57 @@ def run_collect(job):
     return np.concatenate(chunks) if chunks else np.zeros(0, nh.EVENT_DTYPE)
 
 
+def read_profile_summary(path):
+    """key: value lines of a committed rocprofv3 summary (numbers where they parse)"""
+    out = {}
+    if not os.path.exists(path):
+        return out
+    for line in open(path):
+        if ":" not in line or line.startswith((" ", "#", "|", "-")):
+            continue
+        k, v = line.split(":", 1)
+        v = v.strip()
+        try:
+            out[k.strip()] = float(v)
+        except ValueError:
+            out[k.strip()] = v
+    return out
+
+
+def summarize(dirs):
+    """Summary of rocprofv3 output directories (kernel trace --stats and
+    --pmc passes): per kernel, dispatches, mean duration and counters per
+    dispatch.  Prints text for profiles/."""
+    import csv
+    import glob
+    from collections import defaultdict
+    for d in dirs:
+        for f in sorted(glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True)):
+            print(f"## kernel stats ({os.path.relpath(f, d)})")
+            print("| kernel | calls | mean ms | total ms |")
+            print("|---|---|---|---|")
+            for r in csv.DictReader(open(f)):
+                print(f"| {r['Name'][:90]} | {r['Calls']} | {float(r['AverageNs']) / 1e6:.4f} | "
+                      f"{float(r['TotalDurationNs']) / 1e6:.3f} |")
+        for f in sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)):
+            per = defaultdict(lambda: defaultdict(float))
+            names = {}
+            for r in csv.DictReader(open(f)):
+                key = int(r["Dispatch_Id"])
+                names[key] = r["Kernel_Name"]
+                per[key][r["Counter_Name"]] += float(r["Counter_Value"])
+            agg = defaultdict(lambda: defaultdict(float))
+            calls = defaultdict(int)
+            for key, cs in per.items():
+                calls[names[key]] += 1
+                for c, v in cs.items():
+                    agg[names[key]][c] += v
+            print(f"## counters per dispatch ({os.path.relpath(f, d)})")
+            for kname in sorted(agg, key=lambda k: -max(agg[k].values())):
+                cs = ", ".join(f"{c}={v / calls[kname]:.6g}" for c, v in sorted(agg[kname].items()))
+                print(f"- {kname[:90]} (x{calls[kname]}): {cs}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,7 +287,9 @@ def main():
     ap.add_argument("--window", type=float, default=2000.0, help="pause_before of the headline workload")
     ap.add_argument("--dedup-window", type=float, default=20000.0,
                     help="pause_before of the dedup-on config 4 comparison")
-    ap.add_argument("--no-extra", action="store_true", help="headline only")
+    ap.add_argument("--part", choices=("all", "headline", "gather"), default="all",
+                    help="headline: the headline runs only; gather: the alias-gather window only")
+    ap.add_argument("--no-extra", action="store_true", help="same as --part headline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape unless given below")
     ap.add_argument("--form", choices=("persistent", "plain"), default="persistent",
@@ -216,23 +301,42 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="capture each water-level round (kernel + event sort) in a hipGraph; the "
                          "kernel timing then covers the whole graph")
+    ap.add_argument("--summarize", nargs="+", metavar="DIR",
+                    help="print a summary of rocprofv3 output directories and exit (no GPU)")
     a = ap.parse_args()
+    if a.summarize:
+        summarize(a.summarize)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for {a.gpus})",
               file=sys.stderr)
         sys.exit(2)
+    if a.no_extra:
+        a.part = "headline"
 
     ctx = nh.Context.get(0)
     cus, arch = ctx.info()
     log(f"device {arch} with {cus} CUs")
 
-    # ---- headline: config 4, dedup None window, persistent form ----
+    # plain copy kernel of known size: achievable HBM bandwidth and the
+    # calibration of the FETCH_SIZE / WRITE_SIZE byte counters
+    cms, cgbs = nh.copy_calibrate(ctx, 1 << 30, 10)
+    copy = {"kernel": "k_copy_calibrate", "bytes_read": 1 << 30, "bytes_written": 1 << 30, "ms": cms,
+            "GB_per_s": cgbs}
+    log(f"copy kernel: {cgbs:.1f} GB/s")
+
+    if a.part == "gather":
+        r = gather_run(ctx, 2000.0)
+        log(f"gather: {r}")
+        print(json.dumps({"part": "gather", "gather_window": r, "copy_kernel": copy}), flush=True)
+        return
+
+    # ---- headline: config 4, dedup None window ----
     t = time.perf_counter()
     cogs = nh.AlmostInfiniteNormalDispersalScenario(2000, 50.0).new(1e-6)
     lineages = cogs.sample_habitat(nh.OriginPreSampler.all())
     tables = nh.Tables(ctx, cogs)
-    setup_s = time.perf_counter() - t
-    log(f"config 4: {len(lineages)} lineages, setup {setup_s:.2f} s")
+    log(f"config 4: {len(lineages)} lineages, setup {time.perf_counter() - t:.2f} s")
     form = nh.PERSISTENT if a.form == "persistent" else nh.PLAIN
     fixed = a.block_size or a.blocks_per_cu or a.inner_steps or a.exec_fill
     args = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=form,
@@ -251,43 +355,61 @@ def main():
                  "inner_steps": targs.inner_steps, "exec_fill": targs.exec_fill, "trial_ms": tms,
                  "tune_s": time.perf_counter() - t}
         log(f"autotuned: {tuned}")
+    t = time.perf_counter()
+    make_job(ctx, tables, cogs, lineages, args, 42, a.window).close()
+    upload_s = time.perf_counter() - t  # host -> HBM upload of the lineages (+ allocation)
     wall, results = timed_runs(ctx, tables, cogs, lineages, args, 42, a.window, a.steps, a.warmup)
     steps_total = sum(int(o.steps) for o, _, _ in results)
     value = steps_total / wall
     kern_ms = sum(ms for _, ms, _ in results)
     launches = sum(k for _, _, k in results)
     o0 = results[0][0]
-    log(f"persistent: {steps_total} steps in {wall:.3f} s -> {value:.4e} steps/s; "
+    log(f"{a.form}: {steps_total} steps in {wall:.3f} s -> {value:.4e} steps/s; "
         f"device {kern_ms:.1f} ms over {launches} launches")
 
-    # roofline of the dominant kernel (k_persistent, captured with the event
-    # sort in one graph per round): algorithmic bytes per launch = the lineage
-    # state read and written back once per round plus the speciation events
-    nl = max(results[0][2], 1)
-    per_launch_steps = o0.steps / nl
-    per_launch_bytes = len(lineages) * 2 * LINEAGE_STATE_BYTES / nl + o0.n_events / nl * EVENT_BYTES
-    mean_launch_ms = results[0][1] / nl
+    # Roofline of the dominant kernel.  Algorithmic HBM bytes of one round
+    # launch: every lineage's state read once (45 B), the survivors' state
+    # written back (45 B), a dead flag per speciated lineage, 40 B per
+    # speciation event; the almost-infinite scenario reads no tables.  The
+    # plain form reads and writes the state of its pass list once per launch.
+    nl = max(launches, 1)
+    n_in, n_left, n_ev = len(lineages), int(o0.n_remaining), int(o0.n_events)
+    if form == nh.PERSISTENT:
+        round_bytes = n_in * LINEAGE_STATE_BYTES + n_left * LINEAGE_STATE_BYTES + n_ev * (1 + EVENT_BYTES)
+        per_launch_bytes = round_bytes * a.steps / nl
+    else:
+        per_launch_bytes = n_in * 2 * LINEAGE_STATE_BYTES
+    mean_launch_ms = kern_ms / nl
     achieved = per_launch_bytes / (mean_launch_ms * 1e-3) / 1e9
+    prof = read_profile_summary(PROFILE_SUMMARY)
+    same_workload = (prof.get("window") == a.window and prof.get("form") == a.form)
+    traffic = prof.get("hbm_bytes_per_launch") if same_workload else None
+    kname = "k_persistent<almost-infinite, no dedup>" if form == nh.PERSISTENT else "k_plain<almost-infinite>"
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                "kernel": "k_persistent<almost-infinite> (+ event sort, one hipGraph per round)",
-                "mean_launch_ms": round(mean_launch_ms, 3),
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "kernel": kname,
+                "mean_launch_ms": round(mean_launch_ms, 3), "launches": nl,
                 "algorithmic_bytes_per_launch": int(per_launch_bytes),
-                "steps_per_launch": int(per_launch_steps),
-                "note": "no table gathers on this workload: the kernel is bound by VALU issue "
-                        "(64-bit integer hashing and f64 maths), not by HBM"}
+                "lineage_steps_per_launch": int(steps_total / nl)}
+    ipl = prof.get("valu_wave_instructions_per_lineage_step") if same_workload else None
+    if ipl:
+        inst_rate = ipl * steps_total / (kern_ms * 1e-3)
+        peak = cus * ENGINE_CLOCK_HZ
+        roofline["valu"] = {"wave_instructions_per_lineage_step": ipl, "achieved": inst_rate,
+                            "peak": peak, "unit": "wave-instructions/s", "frac": round(inst_rate / peak, 4),
+                            "lane_utilization": prof.get("valu_lane_utilization"),
+                            "source": os.path.relpath(PROFILE_SUMMARY, ROOT)}
+    roofline["note"] = ("the almost-infinite scenario gathers no tables: this kernel is bound by VALU issue "
+                        "(64-bit integer hashing and f64 libm), see roofline.valu; HBM traffic is the "
+                        "lineage state once per round")
 
     extra = {"autotuned": tuned,
              "launch": {"form": a.form, "block_size": args.block_size, "blocks_per_cu": args.blocks_per_cu,
                         "inner_steps": args.inner_steps, "exec_fill": args.exec_fill,
-                        "use_graph": bool(args.use_graph)}}
-    # plain copy kernel of known size: achievable HBM bandwidth and the
-    # calibration of the FETCH_SIZE / WRITE_SIZE byte counters
-    cms, cgbs = nh.copy_calibrate(ctx, 1 << 30, 10)
-    extra["copy_kernel"] = {"kernel": "k_copy_calibrate", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
-                            "ms": cms, "GB_per_s": cgbs}
-    log(f"copy kernel: {cgbs:.1f} GB/s")
-    if not a.no_extra:
+                        "use_graph": bool(args.use_graph)},
+             "copy_kernel": copy,
+             "upload_s": upload_s,
+             "pcie_inclusive_value": steps_total / (wall + a.steps * upload_s)}
+    if a.part == "all":
         # bit-exact comparison with the plain one-launch-per-slice form
         pj = make_job(ctx, tables, cogs, lineages, nh.IndependentArguments(dedup_cache=nh.DedupCache.none(),
                                                                           form=nh.PLAIN), 42, a.window)
@@ -312,13 +434,31 @@ def main():
         # coalesced lineages are dropped, the window is mostly a coalescence
         # workload; plain form = the reference's queue order, bit-exact
         dd = {}
-        for name, form in (("persistent", nh.PERSISTENT), ("plain", nh.PLAIN)):
-            dargs = nh.IndependentArguments(form=form, use_graph=form == nh.PERSISTENT)
-            dwall, dres = timed_runs(ctx, tables, cogs, lineages, dargs, 42, a.dedup_window, 1, 1)
-            do = dres[0][0]
+        t = time.perf_counter()
+        dtuned, dtms = nh.autotune(ctx, tables, nh.IndependentArguments(), nh.WyHash.seed_from_u64(42), cogs,
+                                   lineages[::8], 200.0, tune_step_slice=True)
+        dd["autotuned"] = {"step_slice": dtuned.step_slice, "block_size": dtuned.block_size,
+                           "blocks_per_cu": dtuned.blocks_per_cu, "inner_steps": dtuned.inner_steps,
+                           "exec_fill": dtuned.exec_fill, "tune_s": time.perf_counter() - t}
+        species = {}
+        for name, dargs in (("persistent", dtuned), ("plain", nh.IndependentArguments(form=nh.PLAIN))):
+            dj = make_job(ctx, tables, cogs, lineages, dargs, 42, a.dedup_window)
+            t = time.perf_counter()
+            dev = run_collect(dj)
+            dwall = time.perf_counter() - t
+            do = dj.outcome()
+            dms, dk = dj.kernel_time()
+            dj.close()
+            # a species is a distinct (time, origin) speciation event
+            # (SpeciationEvent equality, necsim/core/src/event.rs:301-307, as
+            # counted by plugins/common/src/biodiversity.rs:32-41): duplicate
+            # lineages that a lossy cache lets through repeat one
+            species[name] = set(zip(dev["event_time"].tolist(), dev["x"].tolist(), dev["y"].tolist(),
+                                    dev["index"].tolist()))
             dd[name] = {"value": do.steps / dwall, "ms": dwall * 1e3, "steps": int(do.steps),
-                        "kernel_ms": dres[0][1], "launches": int(dres[0][2]),
-                        "remaining_lineages": int(do.n_remaining), "speciations": int(do.n_events)}
+                        "kernel_ms": dms, "launches": int(dk), "remaining_lineages": int(do.n_remaining),
+                        "speciation_events": int(do.n_events), "species": len(species[name])}
+        dd["species_sets_identical"] = species["persistent"] == species["plain"]
         dd["persistent_over_plain_wall"] = dd["plain"]["ms"] / dd["persistent"]["ms"]
         extra["config4_dedup_window"] = {"pause_before": a.dedup_window, **dd}
         log(f"config 4 dedup window: {extra['config4_dedup_window']}")
@@ -327,6 +467,8 @@ def main():
         log(f"config 2: {extra['config2_full_run']}")
         extra["config3_window"] = config3_run(ctx, 2000.0)
         log(f"config 3: {extra['config3_window']}")
+        extra["gather_window"] = gather_run(ctx, 2000.0)
+        log(f"gather: {extra['gather_window']}")
         extra["serving_config5"] = serving_run(ctx)
         log(f"serving: {extra['serving_config5']}")
 
@@ -342,11 +484,9 @@ def main():
         "vs_baseline": None, "dtype": "f64+u64", "data": "synthetic",
         "config": {"workload": "config4-window: almost-infinite normal sigma=50, circle r=2000 "
                                f"(12,566,345 lineages), nu=1e-6, Seed(42), pause_before={a.window}, "
-                               "DedupCache::None, persistent form (hipGraph rounds, autotuned launch), "
-                               "delta_t 2, event_slice 2.0",
+                               f"DedupCache::None, {a.form} form, delta_t 2, event_slice Relative(2.0)",
                    "lineages": int(len(lineages)), "steps_per_run": int(o0.steps),
-                   "speciations_per_run": int(o0.n_events), "rounds": int(o0.rounds),
-                   "remaining_lineages": int(o0.n_remaining)},
+                   "speciations_per_run": n_ev, "rounds": int(o0.rounds), "remaining_lineages": n_left},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extra": extra,
