@@ -107,7 +107,7 @@ def cpu_baseline(workload, pause, budget_s=20.0):
     parts = int(max(threads, min(parts, total * threads / max(want_lineages, 1))))
     r = O.simulate_partitioned(scn, params, 1.0, threads, parts, circle=circle)
     return {"value": r["steps"] / r["seconds"], "unit": "lineage-steps/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "per_core_value": r["steps"] / r["seconds"] / threads,
             "sample": f"{threads} of {parts} IsolatedIndividuals partitions of the same workload "
                       f"({r['steps']} steps, {r['seconds']:.1f} s, one thread per partition)"}
 

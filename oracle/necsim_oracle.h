@@ -9,6 +9,24 @@
  *
  * Every function cites the reference file:line that it restates.  Paths are
  * relative to the root of juntyr/necsim-rust.
+ *
+ * What pins it (tests/test_oracle_pins.py, tests/test_golden.py): published
+ * seahash, rand_core seed_from_u64 and FNV-1a vectors; wymum against exact
+ * 128-bit products; ln/exp/sin/cos within 1 ulp of high-precision values and
+ * at the special values; alias tables against their weights; the Poisson
+ * sampler's contract and rate; origin order and partitions; and the
+ * whole-path invariants (dedup keeps the species and the unique events,
+ * sorted output, pause/resume, slicing independence, partition union).
+ *
+ * Parity unpinned (no reference-held vector or closed form fixes the bits):
+ *  - the exact output bits of crate libm 0.2.8 log/exp/sin/cos as compiled
+ *    by rustc: orc_ln/orc_exp/orc_sin/orc_cos restate the same msun
+ *    algorithms and are pinned to within 1 ulp, not bit for bit;
+ *  - the composition inside WyHash::prime_with and sample_u64 (which
+ *    constant meets which word): its parts (seahash_diffuse, wymum) are
+ *    pinned, the composition follows wyhash.rs:53-80 only;
+ *  - IntrinsicsMathsCore (glibc log/exp), the CPU Independent algorithm's
+ *    default, is not restated at all.
  */
 #ifndef NECSIM_ORACLE_H
 #define NECSIM_ORACLE_H
