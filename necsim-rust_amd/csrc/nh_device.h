@@ -57,6 +57,11 @@ struct DevScenario {
 };
 
 constexpr int RATIO_N = 256; // Poisson inversion ratios lambda*dt/k cached for k < RATIO_N
+// The inversion's running (acc_k, prod_k) for k < PTBL, computed on the host
+// by the same sequence of IEEE operations as the loop: the count is found by
+// comparisons against the table and the loop only runs past it.
+constexpr int PTBL = 8;
+constexpr int POISSON_TABLE_N = RATIO_N + 2 * PTBL; // ratio[RATIO_N], acc[PTBL], prod[PTBL]
 
 // Per-simulation constants
 struct DevParams {
@@ -70,8 +75,30 @@ struct DevParams {
     int dt_pow2;
     double min_event_time;  // resume_after
     uint64_t step_slice;
-    const double *ratio;    // ratio[k] = lambda_dt / k, k in 1..RATIO_N-1 (device, global)
+    const double *ratio;    // POISSON_TABLE_N doubles: ratio[k] = lambda_dt / k (k in 1..RATIO_N-1),
+                            // then acc[k] and prod[k] for k < PTBL (device, global)
+    uint32_t prod_zero_at;  // first k < PTBL with prod[k] == 0, PTBL if none
 };
+
+// Host side: the table of DevParams::ratio (the loop of poisson.rs:53-63
+// unrolled for k < PTBL); returns prod_zero_at.
+inline uint32_t poisson_table(double lambda_dt, double p0, double *t) {
+    t[0] = 0.0;
+    for (int k = 1; k < RATIO_N; k++) t[k] = lambda_dt / (double)k;
+    double *acc = t + RATIO_N, *prod = acc + PTBL;
+    double pr = p0, ac = p0;
+    uint32_t zero_at = PTBL;
+    for (int k = 0; k < PTBL; k++) {
+        if (k > 0) {
+            pr *= t[k];
+            ac += pr;
+        }
+        acc[k] = ac;
+        prod[k] = pr;
+        if (!(pr > 0.0) && zero_at == PTBL) zero_at = (uint32_t)k;
+    }
+    return zero_at;
+}
 
 constexpr uint8_t FLAG_EXACT = 1; // next_t / next_st hold the exact next event
 constexpr uint8_t FLAG_DEAD = 2;  // the lineage speciated or was dropped as a duplicate
@@ -82,7 +109,7 @@ struct LineageSoA {
     double *t_last;     // last_event_time
     uint32_t *x, *y, *idx;
     double *next_t;     // next event time (pessimistic lower bound, or exact if flag bit 0)
-    uint64_t *next_st;  // RNG state after the final prime of next_t (exact only)
+    uint64_t *next_st;  // prime key of next_t's event: its draws use prime(seed, id, key) (exact only)
     uint8_t *flags;     // FLAG_EXACT, FLAG_DEAD
 };
 
@@ -105,7 +132,7 @@ template <int KIND> struct Lane {
     uint64_t id;
     CellRec rec;        // finite in-memory kinds: record of the current cell
     double next_t;
-    uint64_t next_st;
+    uint64_t next_st; // prime key of the next event
     bool have_next;
     // an unfinished window-by-window search for the next event
     bool searching;
@@ -165,18 +192,31 @@ __device__ __forceinline__ void lane_set_location(Lane<KIND> &L, const DevScenar
 // holds none; otherwise sets *best and leaves `state` primed for the event's
 // own draws.
 __device__ __forceinline__ bool poisson_window(const DevParams &P, const double *ratio, uint64_t id,
-                                               double time, uint64_t ts, double &best, uint64_t &state) {
-    state = prime(P.seed, id, ts);
+                                               double time, uint64_t ts, double &best, uint64_t &key) {
+    uint64_t state = prime(P.seed, id, ts);
     uint32_t number;
     if (P.p0 > 0.0) {
-        uint32_t k = 0;
-        double prod = P.p0;
-        double acc = P.p0;
+        // while u > acc_k && prod_k > 0: k += 1 (acc is non-decreasing, so
+        // the loop ends at min(first k with u <= acc_k, first k with prod_k == 0))
+        const double *acc_t = ratio + RATIO_N, *prod_t = acc_t + PTBL;
         double u = uniform_closed_open(state);
-        while (u > acc && prod > 0.0) {
-            k += 1;
-            prod *= (k < RATIO_N) ? ratio[k] : P.lambda_dt / (double)k;
-            acc += prod;
+        // m = #{k < PTBL : u > acc_k}, by binary search over the sorted table
+        static_assert(PTBL == 8, "three search steps");
+        uint32_t m = 0;
+        m += u > acc_t[m + 3] ? 4u : 0u;
+        m += u > acc_t[m + 1] ? 2u : 0u;
+        m += u > acc_t[m] ? 1u : 0u;
+        m += (m == PTBL - 1 && u > acc_t[PTBL - 1]) ? 1u : 0u;
+        uint32_t k = m < P.prod_zero_at ? m : P.prod_zero_at;
+        if (k == PTBL) { // both past the table: continue the loop from its last entry
+            k = PTBL - 1;
+            double prod = prod_t[k];
+            double acc = acc_t[k];
+            while (u > acc && prod > 0.0) {
+                k += 1;
+                prod *= (k < RATIO_N) ? ratio[k] : P.lambda_dt / (double)k;
+                acc += prod;
+            }
         }
         number = k;
     } else {
@@ -194,7 +234,9 @@ __device__ __forceinline__ bool poisson_window(const DevParams &P, const double 
             found = true;
         }
     }
-    if (found) state = prime(P.seed, id, ts + INV_PHI * (uint64_t)(best_index + 1));
+    // the event's own draws use prime(id, key) (poisson.rs:102-106); the
+    // prime itself is left to the execution, where the lanes run together
+    if (found) key = ts + INV_PHI * (uint64_t)(best_index + 1);
     return found;
 }
 
@@ -204,10 +246,10 @@ __device__ __forceinline__ uint64_t poisson_first_window(const DevParams &P, dou
 
 // The whole search: windows ts, ts + 1, ... until one holds an event.
 __device__ __forceinline__ double poisson_next(const DevParams &P, const double *ratio, uint64_t id,
-                                               double time, uint64_t &state) {
+                                               double time, uint64_t &key) {
     double best = 0.0;
     for (uint64_t ts = poisson_first_window(P, time);; ts++)
-        if (poisson_window(P, ratio, id, time, ts, best, state)) return best;
+        if (poisson_window(P, ratio, id, time, ts, best, key)) return best;
 }
 
 // Make sure L.next_t / L.next_st hold the next event (the `pop` peek).
@@ -257,7 +299,7 @@ template <int KIND, bool TRACK_MS = true>
 __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S, const DevParams &P,
                                             DevEvent &ev) {
     double net = L.next_t;
-    uint64_t st = L.next_st;
+    uint64_t st = prime(P.seed, L.id, L.next_st); // next_st holds the event's prime key
     L.have_next = false;
 
     double u_spec = uniform_closed_open(st);

@@ -55,7 +55,6 @@ struct PersistArgs {
     unsigned long long cache_cap;
     DevEvent *events;
     Counters *ctr;
-    unsigned long long tick_steps; // 0: unlimited (a lineage runs to the level)
     unsigned inner_steps;          // iterations per lane between wave-level refills (>= 1)
     unsigned exec_fill;            // eighths of the active lanes that must be ready to execute (1..8)
 };

@@ -75,7 +75,7 @@ __device__ __forceinline__ bool cache_insert_concurrent(uint64_t *cache, unsigne
 }
 
 __device__ __forceinline__ void load_ratio(double *s_ratio, const double *ratio) {
-    for (int k = threadIdx.x; k < RATIO_N; k += blockDim.x) s_ratio[k] = ratio[k];
+    for (int k = threadIdx.x; k < POISSON_TABLE_N; k += blockDim.x) s_ratio[k] = ratio[k];
     __syncthreads();
 }
 
@@ -185,7 +185,7 @@ __device__ __forceinline__ bool wave_ready(bool ready, unsigned fill_eighths) {
 // spilling.
 template <int KIND, bool DEDUP>
 __global__ void __launch_bounds__(256, DEDUP ? 3 : 4) k_persistent(PersistArgs A) {
-    __shared__ double s_ratio[RATIO_N];
+    __shared__ double s_ratio[POISSON_TABLE_N];
     load_ratio(s_ratio, A.P.ratio);
 
     const double level = A.round->level;
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(256, DEDUP ? 3 : 4) k_persistent(PersistArgs A
     bool has = false;
     unsigned long long slot = 0;
     WaveQueue q{0, 0, 0, false, false};
-    uint64_t slice_steps = 0, tick = 0;
+    uint64_t slice_steps = 0;
     unsigned long long my_steps = 0, my_dead = 0;
     double my_max = 0.0;
 
@@ -211,7 +211,6 @@ __global__ void __launch_bounds__(256, DEDUP ? 3 : 4) k_persistent(PersistArgs A
                     slot = w;
                     has = true;
                     slice_steps = 0;
-                    tick = 0;
                 }
             }
         }
@@ -226,11 +225,10 @@ __global__ void __launch_bounds__(256, DEDUP ? 3 : 4) k_persistent(PersistArgs A
             for (unsigned inner = 0; inner < A.inner_steps; inner++) {
                 if (!L.have_next) lane_search_window<KIND>(L, A.P, s_ratio);
                 if (!wave_ready(L.have_next, A.exec_fill) || !L.have_next) continue;
-                // Leaving this launch (water level or tick limit) and reaching
-                // the step slice both end the slice: its min speciation sample
-                // is inserted like on reclassification (monolithic/mod.rs:205-219).
-                // A tick-limited lineage starts a fresh slice when resumed.
-                bool leave = L.next_t >= level || (A.tick_steps && tick >= A.tick_steps);
+                // Leaving this round (water level) and reaching the step slice
+                // both end the slice: its min speciation sample is inserted like
+                // on reclassification (monolithic/mod.rs:205-219).
+                bool leave = L.next_t >= level;
                 if (DEDUP) {
                     if (leave || slice_steps >= A.P.step_slice) {
                         bool dup = L.ms_some &&
@@ -254,7 +252,6 @@ __global__ void __launch_bounds__(256, DEDUP ? 3 : 4) k_persistent(PersistArgs A
                 int r = lane_execute<KIND, DEDUP>(L, A.S, A.P, ev);
                 my_steps++;
                 slice_steps++;
-                tick++;
                 my_max = L.t_last > my_max ? L.t_last : my_max;
                 if (r == STEP_SPECIATED) {
                     A.events[atomicAdd(&A.ctr->n_events, 1ull)] = ev; // rare
@@ -279,7 +276,7 @@ __global__ void __launch_bounds__(256, DEDUP ? 3 : 4) k_persistent(PersistArgs A
 // moves to another request.
 template <int KIND>
 __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
-    __shared__ double s_ratio[RATIO_N];
+    __shared__ double s_ratio[POISSON_TABLE_N];
     __shared__ unsigned long long s_base[SERVE_MAX_REQS];
     load_ratio(s_ratio, A.P.ratio);
     for (int r = threadIdx.x; r < A.n_reqs; r += blockDim.x) s_base[r] = A.reqs[r].work_base;
@@ -446,7 +443,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_soa_scatter(LineageSoA in, Linea
 // Plain slice kernel: element q of the pass runs one step slice
 template <int KIND>
 __global__ void __launch_bounds__(256) k_plain(PlainArgs A) {
-    __shared__ double s_ratio[RATIO_N];
+    __shared__ double s_ratio[POISSON_TABLE_N];
     load_ratio(s_ratio, A.P.ratio);
     const unsigned long long q = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
     const double level = A.round->level;

@@ -438,10 +438,10 @@ static int job_start(nh_job *j) {
     P.inv_delta_t = P.dt_pow2 ? 1.0 / p.delta_t : 0.0; // exact for powers of two
     P.min_event_time = p.resume_after;
     P.step_slice = p.step_slice;
-    std::vector<double> ratio(RATIO_N, 0.0);
-    for (int k = 1; k < RATIO_N; k++) ratio[k] = P.lambda_dt / (double)k;
-    NH_TRY(dalloc(&j->d_ratio, RATIO_N));
-    NH_HIP(hipMemcpy(j->d_ratio, ratio.data(), RATIO_N * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> ratio(POISSON_TABLE_N, 0.0);
+    P.prod_zero_at = poisson_table(P.lambda_dt, P.p0, ratio.data());
+    NH_TRY(dalloc(&j->d_ratio, POISSON_TABLE_N));
+    NH_HIP(hipMemcpy(j->d_ratio, ratio.data(), POISSON_TABLE_N * sizeof(double), hipMemcpyHostToDevice));
     P.ratio = j->d_ratio;
 
     // EventSlice::capacity and DedupCache::construct (parallelisation/independent/mod.rs:34-82)
@@ -691,7 +691,6 @@ static PersistArgs persist_args(nh_job *j, int cur) {
     a.cache_cap = j->cache_cap;
     a.events = j->d_events;
     a.ctr = j->d_ctr;
-    a.tick_steps = 0;
     a.inner_steps = j->p.inner_steps ? j->p.inner_steps : 16;
     a.exec_fill = j->p.exec_fill ? j->p.exec_fill : 5;
     return a;

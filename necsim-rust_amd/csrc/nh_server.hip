@@ -156,7 +156,7 @@ extern "C" int nh_server_create(nh_ctx *ctx, const nh_tables *tables, const nh_s
     uint64_t cap = prm->max_lineages;
     auto fail = [&](int rc) { delete s; return rc; };
     int rc;
-    if ((rc = dalloc(&s->d_ratio, RATIO_N))) return fail(rc);
+    if ((rc = dalloc(&s->d_ratio, POISSON_TABLE_N))) return fail(rc);
     if ((rc = dalloc(&s->d_reqs, SERVE_MAX_REQS))) return fail(rc);
     if ((rc = dalloc(&s->d_ctr, SERVE_MAX_REQS))) return fail(rc);
     if ((rc = dalloc(&s->d_head, 1))) return fail(rc);
@@ -195,9 +195,9 @@ extern "C" int nh_server_submit(nh_server *s, const nh_request *rq, uint64_t *id
         P.inv_delta_t = P.dt_pow2 ? 1.0 / rq->delta_t : 0.0;
         P.min_event_time = 0.0;
         P.step_slice = rq->step_slice;
-        std::vector<double> ratio(RATIO_N, 0.0);
-        for (int k = 1; k < RATIO_N; k++) ratio[k] = P.lambda_dt / (double)k;
-        NH_HIP(hipMemcpy(s->d_ratio, ratio.data(), RATIO_N * sizeof(double), hipMemcpyHostToDevice));
+        std::vector<double> ratio(POISSON_TABLE_N, 0.0);
+        P.prod_zero_at = poisson_table(P.lambda_dt, P.p0, ratio.data());
+        NH_HIP(hipMemcpy(s->d_ratio, ratio.data(), POISSON_TABLE_N * sizeof(double), hipMemcpyHostToDevice));
         P.ratio = s->d_ratio;
     } else if (rq->turnover_rate != s->turnover || rq->delta_t != s->delta_t || rq->step_slice != s->step_slice) {
         return set_error(NH_E_INVALID, "turnover rate, delta_t and step_slice must match the server's first request");
