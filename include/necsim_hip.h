@@ -190,6 +190,37 @@ int nh_device_info(nh_ctx *ctx, uint32_t *compute_units, char *name, size_t name
  * into resident device tables. */
 int nh_tables_create(nh_ctx *ctx, const nh_scenario *scenario, nh_tables **out);
 int nh_tables_destroy(nh_tables *tables);
+
+/* An in-memory map scenario as the reference's SpatiallyExplicit map scenario
+ * reads it (rustcoalescence/scenarios/src/spatially_explicit/maps/mod.rs): a
+ * habitat map and a dispersal map, the latter row-major per origin cell with
+ * the row's entries in ascending column order (a dense Array2D row filtered
+ * to its non-zero weights yields exactly these entries). */
+typedef struct {
+    int32_t kind;                 /* NH_SCENARIO_SPATIALLY_EXPLICIT (separable) or _PACKED */
+    uint32_t width, height;
+    uint32_t origin_x, origin_y;
+    const uint32_t *habitat;      /* width*height */
+    const uint64_t *row_ptr;      /* width*height + 1, row_ptr[0] = 0, non-decreasing */
+    const uint32_t *col;          /* row_ptr[width*height] target cells */
+    const double *weight;         /* row_ptr[width*height] dispersal weights */
+} nh_map_scenario;
+
+/* InMemoryPackedSeparableAliasDispersalSampler::new
+ * (.../in_memory/packed_separable_alias/mod.rs:64-203) or
+ * InMemoryPackedAliasDispersalSampler::new (.../in_memory/packed_alias/mod.rs:58-118)
+ * followed by nh_tables_create, built on the device (one origin row per
+ * thread): the same resident tables as nh_build_alias_tables +
+ * nh_tables_create, bit for bit, with the same contract errors
+ * (.../in_memory/contract.rs:13-69). */
+int nh_tables_create_from_map(nh_ctx *ctx, const nh_map_scenario *map, nh_tables **out);
+
+/* Diagnostics, no counterpart in the reference: copy resident in-memory
+ * tables back, 32-byte cell records {prefix u64, habitat u32, atom_start u32,
+ * atom_len u32, non_self u32, self_p f64} and 16-byte atoms {u f64, e u32,
+ * k u32}.  With null buffers only the counts are returned. */
+int nh_tables_export(const nh_tables *tables, void *cells, size_t cells_cap, void *atoms, size_t atoms_cap,
+                     size_t *n_cells, size_t *n_atoms);
 /* device bytes held by the tables */
 int nh_tables_footprint(const nh_tables *tables, uint64_t *bytes);
 

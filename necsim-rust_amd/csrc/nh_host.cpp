@@ -74,6 +74,8 @@ extern "C" int nh_build_alias_tables(uint32_t width, uint32_t height, const uint
             uint32_t c = col[j];
             double p = weight[j];
             if (c >= cells) return set_error(NH_E_INVALID, "dispersal column outside the habitat");
+            if (j > row_ptr[row] && c <= col[j - 1])
+                return set_error(NH_E_INVALID, "dispersal columns must ascend within a row");
             if (p < 0.0 || (p > 0.0 && (habitat[row] == 0 || habitat[c] == 0)))
                 return set_error(NH_E_CONTRACT, "dispersal from/to non-habitat");
             double wt = p * (double)habitat[c];

@@ -24,6 +24,22 @@ inline int set_error(int code, const char *fmt, ...) {
     return code;
 }
 
+// return NH_E_HIP with the failing call from a C-ABI entry point
+#define NH_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return nh::set_error(NH_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                 __FILE__, __LINE__);                                         \
+    } while (0)
+
+// propagate a non-OK return code
+#define NH_TRY(call)                \
+    do {                            \
+        int r_ = (call);            \
+        if (r_ != NH_OK) return r_; \
+    } while (0)
+
 // hipMalloc of n elements (at least one); NH_E_NOMEM with a message on failure
 template <class T> int dalloc(T **p, size_t n) {
     *p = nullptr;

@@ -1,7 +1,11 @@
 // nh_kernels.h -- launch interface of the gfx950 kernels (internal to the library).
 #pragma once
 
+#include <vector>
+
 #include "nh_device.h"
+
+struct nh_ctx;
 
 namespace nh {
 
@@ -168,3 +172,17 @@ void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream);
 void launch_reset_counters(Counters *c, hipStream_t stream);
 
 } // namespace nh
+
+// Resident scenario tables (opaque nh_tables of the C ABI).
+struct nh_tables {
+    nh_ctx *ctx = nullptr;
+    int kind = 0;
+    uint32_t width = 0, height = 0, ox = 0, oy = 0, deme = 1;
+    double sigma = 0.0;
+    std::vector<uint32_t> habitat; // host copy for lineage validation
+    nh::CellRec *d_cells = nullptr;
+    nh::Atom *d_atoms = nullptr;
+    uint64_t n_atoms = 0;
+    uint64_t bytes = 0;
+    nh::DevScenario dev{};
+};

@@ -15,32 +15,6 @@
 
 using namespace nh;
 
-#define NH_HIP(call)                                                                          \
-    do {                                                                                      \
-        hipError_t e_ = (call);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return set_error(NH_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                             __FILE__, __LINE__);                                             \
-    } while (0)
-
-#define NH_TRY(call)             \
-    do {                         \
-        int r_ = (call);         \
-        if (r_ != NH_OK) return r_; \
-    } while (0)
-
-struct nh_tables {
-    nh_ctx *ctx = nullptr;
-    int kind = 0;
-    uint32_t width = 0, height = 0, ox = 0, oy = 0, deme = 1;
-    double sigma = 0.0;
-    std::vector<uint32_t> habitat; // host copy for lineage validation
-    CellRec *d_cells = nullptr;
-    Atom *d_atoms = nullptr;
-    uint64_t bytes = 0;
-    DevScenario dev{};
-};
-
 namespace {
 
 int kind_of(int32_t scenario_kind) {
@@ -271,6 +245,7 @@ extern "C" int nh_tables_create(nh_ctx *ctx, const nh_scenario *s, nh_tables **o
         if (e == hipSuccess && !at.empty())
             e = hipMemcpy(t->d_atoms, at.data(), at.size() * sizeof(Atom), hipMemcpyHostToDevice);
         if (e != hipSuccess) return fail(set_error(NH_E_HIP, "table upload: %s", hipGetErrorString(e)));
+        t->n_atoms = at.size();
         t->bytes = cells * sizeof(CellRec) + at.size() * sizeof(Atom);
         d.cells = t->d_cells;
         d.atoms = t->d_atoms;

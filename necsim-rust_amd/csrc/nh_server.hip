@@ -24,18 +24,6 @@
 
 using namespace nh;
 
-#define NH_HIP(call)                                                                          \
-    do {                                                                                      \
-        hipError_t e_ = (call);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return set_error(NH_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                             __FILE__, __LINE__);                                             \
-    } while (0)
-#define NH_TRY(call)                \
-    do {                            \
-        int r_ = (call);            \
-        if (r_ != NH_OK) return r_; \
-    } while (0)
 
 // the table's device view (defined in nh_runtime.hip)
 const DevScenario *nh_tables_device_view(const nh_tables *t, int *kind);

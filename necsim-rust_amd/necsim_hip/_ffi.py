@@ -31,6 +31,10 @@ EVENT_DTYPE = np.dtype([("global_reference", "<u8"), ("prior_time", "<f8"), ("ev
 ATOM_DTYPE = np.dtype([("u", "<f8"), ("e", "<u8"), ("k", "<u8")])
 RANGE_DTYPE = np.dtype([("start", "<u8"), ("end", "<u8")])
 SELF_DTYPE = np.dtype([("self_dispersal", "<f8"), ("non_self_dispersal_event", "<u8")])
+# resident device tables (nh_tables_export)
+CELL_RECORD_DTYPE = np.dtype([("prefix", "<u8"), ("habitat", "<u4"), ("atom_start", "<u4"),
+                              ("atom_len", "<u4"), ("non_self", "<u4"), ("self_p", "<f8")])
+DEVICE_ATOM_DTYPE = np.dtype([("u", "<f8"), ("e", "<u4"), ("k", "<u4")])
 
 c_u64, c_u32, c_i32, c_f64, c_vp, c_sz = (ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int32,
                                           ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t)
@@ -41,6 +45,11 @@ class Scenario(ctypes.Structure):
                 ("origin_y", c_u32), ("deme", c_u32), ("sigma", c_f64), ("habitat", c_vp),
                 ("alias_ranges", c_vp), ("self_dispersal", c_vp), ("atoms", c_vp),
                 ("n_atoms", c_sz)]
+
+
+class MapScenario(ctypes.Structure):
+    _fields_ = [("kind", c_i32), ("width", c_u32), ("height", c_u32), ("origin_x", c_u32),
+                ("origin_y", c_u32), ("habitat", c_vp), ("row_ptr", c_vp), ("col", c_vp), ("weight", c_vp)]
 
 
 class JobParams(ctypes.Structure):
@@ -88,6 +97,9 @@ SIGNATURES = {
     "nh_tables_create": (ctypes.c_int, [c_vp, ctypes.POINTER(Scenario), ctypes.POINTER(c_vp)]),
     "nh_tables_destroy": (ctypes.c_int, [c_vp]),
     "nh_tables_footprint": (ctypes.c_int, [c_vp, ctypes.POINTER(c_u64)]),
+    "nh_tables_create_from_map": (ctypes.c_int, [c_vp, ctypes.POINTER(MapScenario), ctypes.POINTER(c_vp)]),
+    "nh_tables_export": (ctypes.c_int, [c_vp, c_vp, c_sz, c_vp, c_sz, ctypes.POINTER(c_sz),
+                                        ctypes.POINTER(c_sz)]),
     "nh_job_create": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(JobParams), ctypes.POINTER(c_vp)]),
     "nh_job_add_lineages": (ctypes.c_int, [c_vp, c_vp, c_sz]),
     "nh_job_start": (ctypes.c_int, [c_vp]),

@@ -205,16 +205,38 @@ class SpatiallyExplicitUniformTurnoverScenario:
                                               self.col, self.weight, separable=not self.packed)
         return self._tables
 
-    def new(self, speciation_probability):
-        ranges, selfd, atoms = self.tables()
-        habitat = np.ascontiguousarray(self.habitat, dtype=np.uint32).ravel()
-        kind = (_ffi.SCENARIO_SPATIALLY_EXPLICIT_PACKED if self.packed
+    def kind(self):
+        return (_ffi.SCENARIO_SPATIALLY_EXPLICIT_PACKED if self.packed
                 else _ffi.SCENARIO_SPATIALLY_EXPLICIT)
-        d = _ffi.Scenario(kind=kind, width=self.width, height=self.height, habitat=ptr(habitat),
-                          alias_ranges=ptr(ranges), self_dispersal=ptr(selfd), atoms=ptr(atoms),
-                          n_atoms=len(atoms))
-        return ScenarioCogs(d, speciation_probability, self.turnover_rate,
-                            keep=(habitat, ranges, selfd, atoms))
+
+    def new(self, speciation_probability, host_tables=True):
+        """Scenario cogs.  With host_tables=False the alias tables are not
+        built on the host: Tables(ctx, cogs) then builds them on the GPU
+        (nh_tables_create_from_map)."""
+        habitat = np.ascontiguousarray(self.habitat, dtype=np.uint32).ravel()
+        if host_tables:
+            ranges, selfd, atoms = self.tables()
+            d = _ffi.Scenario(kind=self.kind(), width=self.width, height=self.height, habitat=ptr(habitat),
+                              alias_ranges=ptr(ranges), self_dispersal=ptr(selfd), atoms=ptr(atoms),
+                              n_atoms=len(atoms))
+            keep = (habitat, ranges, selfd, atoms)
+        else:
+            d = _ffi.Scenario(kind=self.kind(), width=self.width, height=self.height, habitat=ptr(habitat))
+            keep = (habitat,)
+        cogs = ScenarioCogs(d, speciation_probability, self.turnover_rate, keep=keep)
+        cogs.map_scenario = self
+        cogs.host_tables = host_tables
+        return cogs
+
+    def map_desc(self):
+        """nh_map_scenario view of the maps (arrays kept alive by the caller)"""
+        arrs = (np.ascontiguousarray(self.habitat, dtype=np.uint32).ravel(),
+                np.ascontiguousarray(self.row_ptr, dtype=np.uint64),
+                np.ascontiguousarray(self.col, dtype=np.uint32),
+                np.ascontiguousarray(self.weight, dtype=np.float64))
+        d = _ffi.MapScenario(kind=self.kind(), width=self.width, height=self.height, habitat=ptr(arrs[0]),
+                             row_ptr=ptr(arrs[1]), col=ptr(arrs[2]), weight=ptr(arrs[3]))
+        return d, arrs
 
 
 # ---------------------------------------------------------------------------
@@ -288,13 +310,31 @@ class Context:
 
 
 class Tables:
-    """Resident device tables of one scenario (nh_tables_create)."""
+    """Resident device tables of one scenario: nh_tables_create from host-built
+    tables, or nh_tables_create_from_map (alias tables built on the GPU) for
+    map scenarios whose cogs were made with host_tables=False."""
 
-    def __init__(self, ctx, cogs):
+    def __init__(self, ctx, cogs, on_device=None):
         h = ctypes.c_void_p()
-        check(lib().nh_tables_create(ctx.handle, ctypes.byref(cogs.desc), ctypes.byref(h)))
+        if on_device is None:
+            on_device = not getattr(cogs, "host_tables", True)
+        if on_device:
+            d, keep = cogs.map_scenario.map_desc()
+            check(lib().nh_tables_create_from_map(ctx.handle, ctypes.byref(d), ctypes.byref(h)))
+        else:
+            check(lib().nh_tables_create(ctx.handle, ctypes.byref(cogs.desc), ctypes.byref(h)))
         self.handle = h
         self.cogs = cogs
+
+    def export(self):
+        """(cell records, atoms) of in-memory tables, as resident on the device"""
+        nc, na = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        check(lib().nh_tables_export(self.handle, None, 0, None, 0, ctypes.byref(nc), ctypes.byref(na)))
+        cells = np.zeros(nc.value, _ffi.CELL_RECORD_DTYPE)
+        atoms = np.zeros(na.value, _ffi.DEVICE_ATOM_DTYPE)
+        check(lib().nh_tables_export(self.handle, ptr(cells), len(cells), ptr(atoms), len(atoms),
+                                     ctypes.byref(nc), ctypes.byref(na)))
+        return cells, atoms
 
     def __del__(self):
         if getattr(self, "handle", None):
