@@ -345,6 +345,17 @@ int nh_sample_origins(const nh_scenario *scenario, double percentage, uint32_t r
                       uint32_t size, uint32_t circle_centre_x, uint32_t circle_centre_y,
                       uint32_t circle_radius, nh_lineage *out, size_t cap, size_t *n_out);
 
+/* ---- event log ------------------------------------------------------------ */
+/* EventLogRecorder::sort_and_write_segment
+ * (necsim/impls/std/src/event_log/recorder.rs:174-208) for speciation events:
+ * writes one event-log segment file, the bincode 1.3 (default options) image
+ * of EventLogHeader (event_log/mod.rs:9-19, with_speciation = true,
+ * with_dispersal = false) followed by n 56-byte PackedEvents
+ * (necsim/core/src/event.rs:16-29).  The events must be non-empty and in
+ * PackedEvent order, as nh_job_drain_events returns them; the reference's
+ * replay (event_log/replay) reads such segments. */
+int nh_event_log_write_segment(const char *path, const nh_speciation_event *events, size_t n);
+
 /* ---- diagnostics ---------------------------------------------------------
  * Evaluate one device primitive elementwise on the GPU (no counterpart in the
  * reference: used to pin the device MathsCore and RNG against the CPU).

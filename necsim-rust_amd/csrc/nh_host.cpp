@@ -3,6 +3,7 @@
 // the inputs of the hot path; the Rust glue obtains them from the reference's
 // own scenario code instead, standalone C/C++/Python users from here.
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -252,5 +253,73 @@ extern "C" int nh_sample_origins(const nh_scenario *s, double percentage, uint32
     }
     *n_out = n;
     if (out && n > cap) return set_error(NH_E_CAPACITY, "origin buffer too small");
+    return NH_OK;
+}
+
+namespace {
+
+void put_u64(std::vector<uint8_t> &b, uint64_t v) {
+    for (int i = 0; i < 8; i++) b.push_back((uint8_t)(v >> (8 * i)));
+}
+void put_u32(std::vector<uint8_t> &b, uint32_t v) {
+    for (int i = 0; i < 4; i++) b.push_back((uint8_t)(v >> (8 * i)));
+}
+
+// PackedEvent order of two speciation events (necsim/core/src/event.rs:256-280):
+// event_time, origin, target (= origin), prior_time, reference
+bool speciation_less(const nh_speciation_event &a, const nh_speciation_event &b) {
+    const uint64_t m = 0x7fffffffffffffffull;
+    uint64_t ta = bits(a.event_time) & m, tb = bits(b.event_time) & m;
+    if (ta != tb) return ta < tb;
+    if (a.x != b.x) return a.x < b.x;
+    if (a.y != b.y) return a.y < b.y;
+    if (a.index != b.index) return a.index < b.index;
+    uint64_t pa = bits(a.prior_time) & m, pb = bits(b.prior_time) & m;
+    if (pa != pb) return pa < pb;
+    return a.global_reference < b.global_reference;
+}
+
+} // namespace
+
+// EventLogRecorder::sort_and_write_segment (necsim/impls/std/src/event_log/recorder.rs:174-208)
+// for speciation events.  bincode 1.3 with its default options (little-endian,
+// fixed-width integers, usize as u64, bool as one byte, newtypes as their
+// inner value) of EventLogHeader {min_time, max_time, length,
+// with_speciation, with_dispersal} (event_log/mod.rs:9-19) followed by each
+// PackedEvent (necsim/core/src/event.rs:16-29): reference u64, prior_time
+// f64 with the sign bit set (a speciation, event.rs:103-115), event_time f64,
+// origin and target (x, y, index u32 each; target = origin), coalescence u64
+// (= the reference).
+extern "C" int nh_event_log_write_segment(const char *path, const nh_speciation_event *events, size_t n) {
+    static_assert(sizeof(double) == 8, "f64");
+    if (!path || !events || n == 0)
+        return set_error(NH_E_INVALID, "nh_event_log_write_segment: a segment holds at least one event");
+    for (size_t i = 1; i < n; i++)
+        if (speciation_less(events[i], events[i - 1]))
+            return set_error(NH_E_INVALID, "nh_event_log_write_segment: events are not in PackedEvent order");
+    std::vector<uint8_t> b;
+    b.reserve(26 + 56 * n);
+    put_u64(b, bits(events[0].event_time) & 0x7fffffffffffffffull);
+    put_u64(b, bits(events[n - 1].event_time) & 0x7fffffffffffffffull);
+    put_u64(b, (uint64_t)n);
+    b.push_back(1); // with_speciation
+    b.push_back(0); // with_dispersal
+    for (size_t i = 0; i < n; i++) {
+        const nh_speciation_event &e = events[i];
+        put_u64(b, e.global_reference);
+        put_u64(b, bits(e.prior_time) | 0x8000000000000000ull);
+        put_u64(b, bits(e.event_time));
+        for (int rep = 0; rep < 2; rep++) {
+            put_u32(b, e.x);
+            put_u32(b, e.y);
+            put_u32(b, e.index);
+        }
+        put_u64(b, e.global_reference);
+    }
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return set_error(NH_E_INVALID, "nh_event_log_write_segment: cannot open %s", path);
+    size_t w = std::fwrite(b.data(), 1, b.size(), f);
+    int c = std::fclose(f);
+    if (w != b.size() || c != 0) return set_error(NH_E_INVALID, "nh_event_log_write_segment: write to %s failed", path);
     return NH_OK;
 }

@@ -1,15 +1,19 @@
 // nh_kernels.hip -- gfx950 kernels of the Independent algorithm.
 //
-//  * k_persistent<KIND>: one water-level round.  Wave64 lanes each hold one
-//    lineage in registers and refill from a device work queue (one atomic per
-//    wave, ballot/mbcnt distribution) whenever their lineage speciates, is
-//    dropped as a duplicate, or reaches the water level.  Dedup inserts happen
-//    in-kernel at every step-slice boundary.
+//  * k_persistent<KIND, DEDUP>: one water-level round.  Wave64 lanes each
+//    hold one lineage in registers and refill from a guided device work queue
+//    (one atomic per wave claim, ballot/mbcnt distribution) whenever their
+//    lineage speciates, is dropped as a duplicate, or reaches the water level.
+//    Next-event searches advance one Poisson window per iteration and found
+//    events execute in batches.  The DEDUP variant inserts into the dedup
+//    cache in-kernel at every step-slice boundary.
+//  * k_serve<KIND>: the persistent kernel over the lineages of many requests.
 //  * k_plain<KIND>: one launch per step slice over the reference's queue
 //    order (parallelisation/independent/monolithic/mod.rs:183-251); dedup and
 //    reclassification follow in separate order-preserving kernels.
 //  * ordered dedup, ordered compaction and an LSD radix sort (wave ballot
-//    ranking) for the water-level event sort (monolithic/reporter/live.rs:88-115).
+//    ranking; a full per-field pass set orders equal event times) for the
+//    water-level event sort (monolithic/reporter/live.rs:88-115).
 #include "nh_kernels.h"
 
 namespace nh {

@@ -8,7 +8,8 @@ generators used by the tests and the benchmark.
 from . import _ffi
 from ._ffi import EVENT_DTYPE, LINEAGE_DTYPE, NecsimHipError, lib
 from .algorithm import (PERSISTENT, PLAIN, AlmostInfiniteNormalDispersalScenario,
-                        BiodiversityReporter, Context, DedupCache, EventCollector, EventSlice,
+                        BiodiversityReporter, Context, DedupCache, EventCollector, EventLogReporter,
+                        EventSlice,
                         HipIndependentAlgorithm, IndependentArguments, Job, NonSpatialScenario,
                         OriginPreSampler, ScenarioCogs, Server, SimulationOutcome,
                         SpatiallyExplicitUniformTurnoverScenario, Tables, WyHash,

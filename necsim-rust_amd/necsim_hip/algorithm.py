@@ -15,6 +15,7 @@ rustcoalescence/algorithms/src/result.rs:11-26 (SimulationOutcome).
 All simulation work runs in libnecsim_hip.so on the GPU.
 """
 import ctypes
+import os
 from dataclasses import dataclass, field
 from typing import Optional
 
@@ -285,6 +286,53 @@ class EventCollector:
         if not self.chunks:
             return np.zeros(0, _ffi.EVENT_DTYPE)
         return np.concatenate(self.chunks)
+
+
+class EventLogReporter:
+    """EventLogRecorder (necsim/impls/std/src/event_log/recorder.rs:20-210) as a
+    speciation reporter: events are buffered in reporting order and written as
+    numbered segment files of `segment_capacity` events (the last one shorter),
+    each sorted by the PackedEvent order; a log without events leaves no
+    directory behind (recorder.rs:46-55).  The directory must not exist yet."""
+
+    def __init__(self, directory, segment_capacity=1 << 20):
+        if segment_capacity < 1:
+            raise ValueError("segment_capacity must be positive")
+        directory = os.path.abspath(directory)
+        os.makedirs(os.path.dirname(directory), exist_ok=True)
+        os.mkdir(directory)
+        self.directory = directory
+        self.capacity = int(segment_capacity)
+        self.segment_index = 0
+        self._buffer = []
+        self._buffered = 0
+
+    def report_speciation(self, events):
+        if len(events) == 0:
+            return
+        self._buffer.append(np.ascontiguousarray(events, dtype=_ffi.EVENT_DTYPE))
+        self._buffered += len(events)
+        while self._buffered >= self.capacity:
+            ev = np.concatenate(self._buffer)
+            self._write(ev[:self.capacity])
+            rest = ev[self.capacity:]
+            self._buffer = [rest] if len(rest) else []
+            self._buffered = len(rest)
+
+    def _write(self, ev):
+        order = np.lexsort((ev["global_reference"], ev["prior_time"], ev["index"], ev["y"], ev["x"],
+                            ev["event_time"]))
+        ev = np.ascontiguousarray(ev[order])
+        path = os.path.join(self.directory, str(self.segment_index)).encode()
+        check(lib().nh_event_log_write_segment(path, ptr(ev), len(ev)))
+        self.segment_index += 1
+
+    def close(self):
+        if self._buffered:
+            self._write(np.concatenate(self._buffer))
+            self._buffer, self._buffered = [], 0
+        if self.segment_index == 0 and os.path.isdir(self.directory) and not os.listdir(self.directory):
+            os.rmdir(self.directory)
 
 
 class Context:
