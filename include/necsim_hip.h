@@ -356,6 +356,16 @@ int nh_sample_origins(const nh_scenario *scenario, double percentage, uint32_t r
  * replay (event_log/replay) reads such segments. */
 int nh_event_log_write_segment(const char *path, const nh_speciation_event *events, size_t n);
 
+/* ---- lineage files (pause / resume) -------------------------------------- */
+/* LineageFileSaver (necsim/impls/std/src/lineage_file/saver.rs:54-75): writes
+ * lineages as the bincode 1.3 `bincode::options()` (varint) image of a
+ * Vec<Lineage>; the file must not exist yet.  Paused lineages from
+ * nh_job_take_lineages round-trip through it into nh_job_add_lineages. */
+int nh_lineage_file_write(const char *path, const nh_lineage *lineages, size_t n);
+/* LineageFileLoader (lineage_file/loader.rs:33-45).  With buf == NULL only the
+ * count is returned in *n_out. */
+int nh_lineage_file_read(const char *path, nh_lineage *buf, size_t cap, size_t *n_out);
+
 /* ---- diagnostics ---------------------------------------------------------
  * Evaluate one device primitive elementwise on the GPU (no counterpart in the
  * reference: used to pin the device MathsCore and RNG against the CPU).

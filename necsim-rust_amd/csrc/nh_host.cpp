@@ -323,3 +323,114 @@ extern "C" int nh_event_log_write_segment(const char *path, const nh_speciation_
     if (w != b.size() || c != 0) return set_error(NH_E_INVALID, "nh_event_log_write_segment: write to %s failed", path);
     return NH_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Lineage files (necsim/impls/std/src/lineage_file/{saver,loader}.rs): the
+// bincode 1.3 `bincode::options()` image of a Vec<Lineage>, i.e. varint
+// integers (v < 251: one byte; else 251 / 252 / 253 followed by the value as
+// little-endian u16 / u32 / u64), f64 as 8 little-endian bytes, a sequence as
+// its varint length followed by its elements, and Lineage as its fields in
+// order: global_reference u64, last_event_time f64, indexed_location
+// {x u32, y u32, index u32} (necsim/core/src/lineage.rs:88-108,
+// necsim/core/src/landscape/location.rs:70-78).
+namespace {
+
+void put_varint(std::vector<uint8_t> &b, uint64_t v) {
+    if (v < 251) {
+        b.push_back((uint8_t)v);
+    } else if (v <= 0xffffull) {
+        b.push_back(251);
+        for (int i = 0; i < 2; i++) b.push_back((uint8_t)(v >> (8 * i)));
+    } else if (v <= 0xffffffffull) {
+        b.push_back(252);
+        for (int i = 0; i < 4; i++) b.push_back((uint8_t)(v >> (8 * i)));
+    } else {
+        b.push_back(253);
+        for (int i = 0; i < 8; i++) b.push_back((uint8_t)(v >> (8 * i)));
+    }
+}
+
+struct Reader {
+    const std::vector<uint8_t> &b;
+    size_t pos = 0;
+    bool ok = true;
+    uint64_t fixed(int bytes) {
+        if (pos + (size_t)bytes > b.size()) {
+            ok = false;
+            return 0;
+        }
+        uint64_t v = 0;
+        for (int i = 0; i < bytes; i++) v |= (uint64_t)b[pos + i] << (8 * i);
+        pos += bytes;
+        return v;
+    }
+    uint64_t varint(uint64_t max) {
+        uint64_t tag = fixed(1), v;
+        if (tag < 251) v = tag;
+        else if (tag == 251) v = fixed(2);
+        else if (tag == 252) v = fixed(4);
+        else if (tag == 253) v = fixed(8);
+        else {
+            ok = false;
+            return 0;
+        }
+        if (v > max) ok = false;
+        return v;
+    }
+};
+
+} // namespace
+
+// LineageFileSaver::try_new + write (saver.rs:54-75): the file must not exist yet.
+extern "C" int nh_lineage_file_write(const char *path, const nh_lineage *lineages, size_t n) {
+    if (!path || (n && !lineages)) return set_error(NH_E_INVALID, "nh_lineage_file_write: null argument");
+    std::vector<uint8_t> b;
+    b.reserve(9 + 32 * n);
+    put_varint(b, (uint64_t)n);
+    for (size_t i = 0; i < n; i++) {
+        const nh_lineage &l = lineages[i];
+        put_varint(b, l.global_reference);
+        put_u64(b, bits(l.last_event_time));
+        put_varint(b, l.x);
+        put_varint(b, l.y);
+        put_varint(b, l.index);
+    }
+    FILE *f = std::fopen(path, "wbx"); // create_new
+    if (!f) return set_error(NH_E_INVALID, "nh_lineage_file_write: cannot create %s (it must not exist)", path);
+    size_t w = std::fwrite(b.data(), 1, b.size(), f);
+    int c = std::fclose(f);
+    if (w != b.size() || c != 0) return set_error(NH_E_INVALID, "nh_lineage_file_write: write to %s failed", path);
+    return NH_OK;
+}
+
+// LineageFileLoader::try_new (loader.rs:33-45).  With buf == NULL only the
+// lineage count is returned.
+extern "C" int nh_lineage_file_read(const char *path, nh_lineage *buf, size_t cap, size_t *n_out) {
+    if (!path || !n_out) return set_error(NH_E_INVALID, "nh_lineage_file_read: null argument");
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return set_error(NH_E_INVALID, "nh_lineage_file_read: cannot open %s", path);
+    std::vector<uint8_t> b;
+    uint8_t chunk[1 << 16];
+    size_t got;
+    while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) b.insert(b.end(), chunk, chunk + got);
+    std::fclose(f);
+    Reader r{b};
+    uint64_t n = r.varint(~0ull);
+    if (!r.ok) return set_error(NH_E_INVALID, "nh_lineage_file_read: %s is not a lineage file", path);
+    *n_out = (size_t)n;
+    if (!buf) return NH_OK;
+    if (cap < n) return set_error(NH_E_CAPACITY, "nh_lineage_file_read: buffer holds %zu of %llu lineages", cap,
+                                  (unsigned long long)n);
+    for (uint64_t i = 0; i < n && r.ok; i++) {
+        nh_lineage l{};
+        l.global_reference = r.varint(~0ull);
+        l.last_event_time = from_bits(r.fixed(8));
+        l.x = (uint32_t)r.varint(0xffffffffull);
+        l.y = (uint32_t)r.varint(0xffffffffull);
+        l.index = (uint32_t)r.varint(0xffffffffull);
+        buf[i] = l;
+    }
+    if (!r.ok || r.pos != b.size())
+        return set_error(NH_E_INVALID, "nh_lineage_file_read: %s is truncated or has trailing bytes", path);
+    return NH_OK;
+}

@@ -13,7 +13,7 @@ from .algorithm import (PERSISTENT, PLAIN, AlmostInfiniteNormalDispersalScenario
                         HipIndependentAlgorithm, IndependentArguments, Job, NonSpatialScenario,
                         OriginPreSampler, ScenarioCogs, Server, SimulationOutcome,
                         SpatiallyExplicitUniformTurnoverScenario, Tables, WyHash,
-                        autotune, build_alias_tables, copy_calibrate)
+                        autotune, build_alias_tables, copy_calibrate, load_lineages, save_lineages)
 from .maps import generate_map
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -288,6 +288,21 @@ class EventCollector:
         return np.concatenate(self.chunks)
 
 
+def save_lineages(path, lineages):
+    """LineageFileSaver: paused lineages to a new lineage file (bincode varint)"""
+    lin = np.ascontiguousarray(lineages, dtype=_ffi.LINEAGE_DTYPE)
+    check(lib().nh_lineage_file_write(os.fsencode(path), ptr(lin), len(lin)))
+
+
+def load_lineages(path):
+    """LineageFileLoader: the lineages of a lineage file"""
+    n = ctypes.c_size_t(0)
+    check(lib().nh_lineage_file_read(os.fsencode(path), None, 0, ctypes.byref(n)))
+    out = np.zeros(n.value, _ffi.LINEAGE_DTYPE)
+    check(lib().nh_lineage_file_read(os.fsencode(path), ptr(out), len(out), ctypes.byref(n)))
+    return out
+
+
 class EventLogReporter:
     """EventLogRecorder (necsim/impls/std/src/event_log/recorder.rs:20-210) as a
     speciation reporter: events are buffered in reporting order and written as
