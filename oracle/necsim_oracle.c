@@ -1037,6 +1037,41 @@ static int cmp_event(const void *pa, const void *pb) {
     return 0;
 }
 
+/* PackedEvent::cmp (event.rs:256-280) */
+static int cmp_packed(const void *pa, const void *pb) {
+    const orc_packed_event *a = (const orc_packed_event *)pa;
+    const orc_packed_event *b = (const orc_packed_event *)pb;
+    int c = cmp_f64_total(fabs(a->event_time), fabs(b->event_time));
+    if (c) return c;
+    if (a->ox != b->ox) return a->ox < b->ox ? -1 : 1;
+    if (a->oy != b->oy) return a->oy < b->oy ? -1 : 1;
+    if (a->oindex != b->oindex) return a->oindex < b->oindex ? -1 : 1;
+    if (a->tx != b->tx) return a->tx < b->tx ? -1 : 1;
+    if (a->ty != b->ty) return a->ty < b->ty ? -1 : 1;
+    if (a->tindex != b->tindex) return a->tindex < b->tindex ? -1 : 1;
+    c = cmp_f64_total(fabs(a->prior_time), fabs(b->prior_time));
+    if (c) return c;
+    if (a->global_reference != b->global_reference)
+        return a->global_reference < b->global_reference ? -1 : 1;
+    return 0;
+}
+
+static double negated(double x) { return of_bits(bits_of(x) | 0x8000000000000000ull); }
+
+/* growing PackedEvent buffer of orc_simulate_packed */
+typedef struct {
+    orc_packed_event *b;
+    size_t n, cap;
+} packed_sink;
+
+static void sink_push(packed_sink *k, orc_packed_event e) {
+    if (k->n == k->cap) {
+        k->cap = k->cap ? 2 * k->cap : 1024;
+        k->b = (orc_packed_event *)realloc(k->b, k->cap * sizeof(orc_packed_event));
+    }
+    k->b[k->n++] = e;
+}
+
 uint64_t orc_biodiversity(const orc_speciation_event *ev, size_t n) {
     uint64_t count = 0;
     for (size_t j = 0; j < n; j++) {
@@ -1103,9 +1138,12 @@ static int cache_insert(dedup_cache *c, const spec_sample *v) {
     return insert;
 }
 
-int orc_simulate(const orc_scenario *s, const orc_params *p, const orc_lineage *lineages,
-                 size_t n, orc_speciation_event *events_out, orc_lineage *remaining_out,
-                 orc_outcome *outcome) {
+/* The loop of both entry points: speciation events go to events_out (40 B),
+ * or, with a sink, every reported event goes there as a PackedEvent. */
+static int simulate_impl(const orc_scenario *s, const orc_params *p, const orc_lineage *lineages,
+                         size_t n, orc_speciation_event *events_out, packed_sink *sink,
+                         orc_lineage *remaining_out, orc_outcome *outcome) {
+    const int report_dispersal = sink && p->report_dispersal;
     const uint64_t seed = orc_seed_from_u64(p->seed);
     const double lambda = p->turnover_rate;
     const double nu = p->speciation_probability;
@@ -1146,10 +1184,12 @@ int orc_simulate(const orc_scenario *s, const orc_params *p, const orc_lineage *
     size_t n_events = 0;
 
     while (slow.len > 0 && (!p->has_pause || level_time < p->pause_before)) {
-        /* speciation-only reporting: rate = lambda * nu per lineage, summed
-         * sequentially over the slow deque (mod.rs:150-164) */
+        /* the reported event rate per slow lineage, summed sequentially over
+         * the slow deque: lambda with dispersal reporting (mod.rs:139-148),
+         * lambda * nu for speciation only (mod.rs:150-164) */
         double total_event_rate = 0.0;
-        for (size_t j = 0; j < slow.len; j++) total_event_rate = total_event_rate + lambda * nu;
+        const double rate = report_dispersal ? lambda : lambda * nu;
+        for (size_t j = 0; j < slow.len; j++) total_event_rate = total_event_rate + rate;
         level_time = level_time + (double)event_slice / total_event_rate;
         if (p->has_pause && p->pause_before < level_time) level_time = p->pause_before;
 
@@ -1220,14 +1260,22 @@ int orc_simulate(const orc_scenario *s, const orc_params *p, const orc_lineage *
                     min_sample.i = chosen.index;
                 }
                 if (u_spec < nu) {
-                    orc_speciation_event *e = &events_out[n_events++];
-                    e->global_reference = chosen.global_reference;
-                    e->prior_time = chosen.last_event_time;
-                    e->event_time = next_event_time;
-                    e->x = chosen.x;
-                    e->y = chosen.y;
-                    e->index = chosen.index;
-                    e->_pad = 0;
+                    if (sink) {
+                        orc_packed_event e = {chosen.global_reference, negated(chosen.last_event_time),
+                                              next_event_time, chosen.x, chosen.y, chosen.index,
+                                              chosen.x, chosen.y, chosen.index, chosen.global_reference};
+                        sink_push(sink, e);
+                    } else {
+                        orc_speciation_event *e = &events_out[n_events];
+                        e->global_reference = chosen.global_reference;
+                        e->prior_time = chosen.last_event_time;
+                        e->event_time = next_event_time;
+                        e->x = chosen.x;
+                        e->y = chosen.y;
+                        e->index = chosen.index;
+                        e->_pad = 0;
+                    }
+                    n_events++;
                 } else {
                     uint32_t tx, ty;
                     orc_sample_dispersal(s, chosen.x, chosen.y, &state, &tx, &ty);
@@ -1238,6 +1286,15 @@ int orc_simulate(const orc_scenario *s, const orc_params *p, const orc_lineage *
                     uint32_t idx;
                     if (s->kind == ORC_ALMOST_INFINITE_NORMAL) idx = 0; /* SingletonDemesHabitat */
                     else idx = f64_as_u32(orc_floor(u_c * (double)deme));
+                    if (report_dispersal) {
+                        /* DispersalEvent with LineageInteraction::Maybe
+                         * (event_sampler/independent.rs:174-222, event.rs:140-146) */
+                        orc_packed_event e = {chosen.global_reference, chosen.last_event_time,
+                                              negated(next_event_time), chosen.x, chosen.y, chosen.index,
+                                              tx, ty, idx, chosen.global_reference};
+                        sink_push(sink, e);
+                        n_events++;
+                    }
                     active = chosen;
                     active.x = tx;
                     active.y = ty;
@@ -1258,7 +1315,8 @@ int orc_simulate(const orc_scenario *s, const orc_params *p, const orc_lineage *
         fast = t;
     }
 
-    qsort(events_out, n_events, sizeof(orc_speciation_event), cmp_event);
+    if (sink) qsort(sink->b, sink->n, sizeof(orc_packed_event), cmp_packed);
+    else qsort(events_out, n_events, sizeof(orc_speciation_event), cmp_event);
 
     size_t n_rem = slow.len;
     if (remaining_out) {
@@ -1277,6 +1335,24 @@ int orc_simulate(const orc_scenario *s, const orc_params *p, const orc_lineage *
     free(cache.b);
     return 0;
 }
+
+int orc_simulate(const orc_scenario *s, const orc_params *p, const orc_lineage *lineages,
+                 size_t n, orc_speciation_event *events_out, orc_lineage *remaining_out,
+                 orc_outcome *outcome) {
+    return simulate_impl(s, p, lineages, n, events_out, NULL, remaining_out, outcome);
+}
+
+int orc_simulate_packed(const orc_scenario *s, const orc_params *p, const orc_lineage *lineages,
+                        size_t n, orc_packed_event **events_out, size_t *n_events, orc_lineage *remaining_out,
+                        orc_outcome *outcome) {
+    packed_sink sink = {NULL, 0, 0};
+    int rc = simulate_impl(s, p, lineages, n, NULL, &sink, remaining_out, outcome);
+    *events_out = sink.b;
+    *n_events = sink.n;
+    return rc;
+}
+
+void orc_free(void *ptr) { free(ptr); }
 
 /* ======================================================================== */
 /* IsolatedIndividuals CPU baseline (multi-threaded)                         */

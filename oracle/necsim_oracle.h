@@ -98,7 +98,21 @@ typedef struct {
     int has_pause;
     double pause_before;
     double min_event_time;          /* resume_after; 0.0 for a fresh simulation */
+    int report_dispersal;           /* R::ReportDispersal: every event is reported */
 } orc_params;
+
+/* necsim/core/src/event.rs:16-29 (PackedEvent, 56 B repr(C)): speciation is
+ * prior(-) event(+), target = origin; the Independent algorithm's dispersal
+ * (LineageInteraction::Maybe) is prior(+) event(-); coalescence = the
+ * lineage's own reference for both (event.rs:103-155). */
+typedef struct {
+    uint64_t global_reference;
+    double prior_time;
+    double event_time;
+    uint32_t ox, oy, oindex;
+    uint32_t tx, ty, tindex;
+    uint64_t coalescence;
+} orc_packed_event;
 
 typedef struct {
     int paused;        /* Status::Paused */
@@ -172,6 +186,16 @@ size_t orc_sample_origins(const orc_scenario *s, double percentage, uint32_t ran
 int orc_simulate(const orc_scenario *s, const orc_params *p, const orc_lineage *lineages,
                  size_t n, orc_speciation_event *events_out, orc_lineage *remaining_out,
                  orc_outcome *outcome);
+
+/* The same run with every reported event as a PackedEvent, sorted by the
+ * PackedEvent order.  With p->report_dispersal the water level advances by
+ * event_slice / sum(lambda) (monolithic/mod.rs:139-148) and every dispersal
+ * is reported; otherwise only speciations.  *events_out is malloc'd
+ * (release with orc_free). */
+int orc_simulate_packed(const orc_scenario *s, const orc_params *p, const orc_lineage *lineages,
+                        size_t n, orc_packed_event **events_out, size_t *n_events, orc_lineage *remaining_out,
+                        orc_outcome *outcome);
+void orc_free(void *ptr);
 
 /* Biodiversity reporter over a sorted stream (necsim/plugins/common/src/biodiversity.rs:33-41) */
 uint64_t orc_biodiversity(const orc_speciation_event *events, size_t n);

@@ -17,6 +17,10 @@ LINEAGE_DTYPE = np.dtype([("global_reference", "<u8"), ("last_event_time", "<f8"
                           ("x", "<u4"), ("y", "<u4"), ("index", "<u4"), ("_pad", "<u4")])
 EVENT_DTYPE = np.dtype([("global_reference", "<u8"), ("prior_time", "<f8"), ("event_time", "<f8"),
                         ("x", "<u4"), ("y", "<u4"), ("index", "<u4"), ("_pad", "<u4")])
+# PackedEvent (necsim/core/src/event.rs:16-29): sign bits of the times encode the event type
+PACKED_DTYPE = np.dtype([("global_reference", "<u8"), ("prior_time", "<f8"), ("event_time", "<f8"),
+                         ("ox", "<u4"), ("oy", "<u4"), ("oindex", "<u4"),
+                         ("tx", "<u4"), ("ty", "<u4"), ("tindex", "<u4"), ("coalescence", "<u8")])
 
 NON_SPATIAL, ALMOST_INFINITE_NORMAL, SPATIALLY_EXPLICIT, SPATIALLY_EXPLICIT_PACKED = 0, 1, 2, 3
 DEDUP_NONE, DEDUP_RELATIVE, DEDUP_ABSOLUTE = 0, 1, 2
@@ -37,7 +41,8 @@ class Params(ctypes.Structure):
                 ("dedup_factor", ctypes.c_double), ("dedup_capacity", ctypes.c_uint64),
                 ("event_slice_mode", ctypes.c_int), ("event_slice_factor", ctypes.c_double),
                 ("event_slice_capacity", ctypes.c_uint64), ("has_pause", ctypes.c_int),
-                ("pause_before", ctypes.c_double), ("min_event_time", ctypes.c_double)]
+                ("pause_before", ctypes.c_double), ("min_event_time", ctypes.c_double),
+                ("report_dispersal", ctypes.c_int)]
 
 
 class Outcome(ctypes.Structure):
@@ -102,6 +107,12 @@ def lib():
         L.orc_sample_origins.restype = ctypes.c_size_t
         L.orc_sample_origins.argtypes = [vp, d, u32, u32, u32, u32, u32, vp, ctypes.c_size_t,
                                          ctypes.POINTER(ctypes.c_size_t)]
+        L.orc_simulate_packed.restype = ctypes.c_int
+        L.orc_simulate_packed.argtypes = [vp, ctypes.POINTER(Params), vp, ctypes.c_size_t,
+                                          ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t), vp,
+                                          ctypes.POINTER(Outcome)]
+        L.orc_free.restype = None
+        L.orc_free.argtypes = [vp]
         L.orc_simulate.restype = ctypes.c_int
         L.orc_simulate.argtypes = [vp, ctypes.POINTER(Params), vp, ctypes.c_size_t, vp, vp,
                                    ctypes.POINTER(Outcome)]
@@ -185,10 +196,10 @@ class Scenario:
 
 def make_params(seed, speciation_probability, delta_t=2.0, step_slice=10, dedup=("relative", 1.0),
                 event_slice=("relative", 2.0), pause_before=None, min_event_time=0.0,
-                turnover_rate=0.5):
+                turnover_rate=0.5, report_dispersal=False):
     p = Params(seed=seed, speciation_probability=speciation_probability,
                turnover_rate=turnover_rate, delta_t=delta_t, step_slice=step_slice,
-               min_event_time=min_event_time)
+               min_event_time=min_event_time, report_dispersal=1 if report_dispersal else 0)
     if dedup is None or dedup == "none":
         p.dedup_mode = DEDUP_NONE
     elif dedup[0] == "relative":
@@ -216,6 +227,26 @@ def simulate(scenario, params, lineages):
         raise RuntimeError(f"orc_simulate failed: {rc}")
     return dict(paused=bool(out.paused), time=out.time, steps=out.steps,
                 events=events[:out.n_events].copy(), remaining=remaining[:out.n_remaining].copy())
+
+
+def simulate_packed(scenario, params, lineages):
+    """orc_simulate_packed: every reported event as a 56-byte PackedEvent"""
+    lineages = np.ascontiguousarray(lineages, dtype=LINEAGE_DTYPE)
+    n = len(lineages)
+    remaining = np.zeros(max(n, 1), LINEAGE_DTYPE)
+    out = Outcome()
+    buf = ctypes.c_void_p()
+    nev = ctypes.c_size_t(0)
+    rc = lib().orc_simulate_packed(scenario.handle, ctypes.byref(params), _ptr(lineages), n,
+                                   ctypes.byref(buf), ctypes.byref(nev), _ptr(remaining), ctypes.byref(out))
+    if rc != 0:
+        raise RuntimeError(f"orc_simulate_packed failed: {rc}")
+    events = np.zeros(nev.value, PACKED_DTYPE)
+    if nev.value:
+        ctypes.memmove(events.ctypes.data, buf, nev.value * PACKED_DTYPE.itemsize)
+    lib().orc_free(buf)
+    return dict(paused=bool(out.paused), time=out.time, steps=out.steps, events=events,
+                remaining=remaining[:out.n_remaining].copy())
 
 
 def biodiversity(events):

@@ -326,3 +326,85 @@ def test_partitioned_baseline_union_matches_monolithic(small_runs):
     s, lin, runs = small_runs
     r = O.simulate_partitioned(s, O.make_params(42, 1e-2, dedup=None), 1.0, 4)
     assert r["steps"] == runs[None]["steps"] and r["events"] == len(lin)
+
+
+# ---------------------------------------------------------------------------
+# dispersal reporting (R::ReportDispersal): every event reported as a PackedEvent
+def _sign(x):
+    return (np.ascontiguousarray(x, dtype=np.float64).view(np.uint64) >> np.uint64(63)).astype(bool)
+
+
+def _speciations_of(packed):
+    """the speciation PackedEvents (prior(-), event(+), event.rs:103-115) as SpeciationEvents"""
+    sp = packed[_sign(packed["prior_time"]) & ~_sign(packed["event_time"])]
+    out = np.zeros(len(sp), O.EVENT_DTYPE)
+    out["global_reference"] = sp["global_reference"]
+    out["prior_time"] = np.abs(sp["prior_time"])
+    out["event_time"] = sp["event_time"]
+    out["x"], out["y"], out["index"] = sp["ox"], sp["oy"], sp["oindex"]
+    return out
+
+
+@pytest.fixture(scope="module")
+def dispersal_runs():
+    s = O.Scenario.spatially_explicit(*_small_map(12, 12, seed=7))
+    lin = s.origins(1.0)
+    p = dict(dedup=None)
+    full = O.simulate_packed(s, O.make_params(3, 2e-2, report_dispersal=True, **p), lin)
+    spec = O.simulate(s, O.make_params(3, 2e-2, **p), lin)
+    return s, lin, full, spec
+
+
+def _small_map(w, h, seed):
+    import necsim_hip as nh
+    return (w, h) + tuple(nh.generate_map(w, h, seed=seed, sigma=1.5, cutoff=3.0))
+
+
+def test_dispersal_reporting_keeps_the_speciation_stream(dispersal_runs):
+    """the reported events below each water level do not depend on how the
+    level advances (slicing independence), so the speciations of a
+    dispersal-reporting run are the speciation-only stream"""
+    s, lin, full, spec = dispersal_runs
+    assert _speciations_of(full["events"]).tobytes() == spec["events"].tobytes()
+    assert full["steps"] == spec["steps"] and full["time"] == spec["time"]
+    assert len(full["events"]) == full["steps"]  # one reported event per step
+
+
+def test_dispersal_events_are_maybe_encoded_and_chain_per_lineage(dispersal_runs):
+    s, lin, full, spec = dispersal_runs
+    ev = full["events"]
+    disp = ~_sign(ev["prior_time"]) & _sign(ev["event_time"])  # LineageInteraction::Maybe
+    spec_mask = _sign(ev["prior_time"]) & ~_sign(ev["event_time"])
+    assert np.all(disp | spec_mask) and np.all(ev["coalescence"] == ev["global_reference"])
+    assert np.all(ev["ox"][spec_mask] == ev["tx"][spec_mask])
+    start = {int(l["global_reference"]): (float(l["last_event_time"]), int(l["x"]), int(l["y"]),
+                                          int(l["index"])) for l in lin}
+    by_ref = {}
+    for e in ev:  # the stream is time-sorted, so each lineage's events come in order
+        by_ref.setdefault(int(e["global_reference"]), []).append(e)
+    for ref, chain in by_ref.items():
+        t, x, y, i = start[ref]
+        for k, e in enumerate(chain):
+            assert abs(float(e["prior_time"])) == t and (int(e["ox"]), int(e["oy"]), int(e["oindex"])) == (x, y, i)
+            t, x, y, i = abs(float(e["event_time"])), int(e["tx"]), int(e["ty"]), int(e["tindex"])
+            # a full run ends each chain with its only speciation
+            assert (k == len(chain) - 1) == bool(_sign(e["prior_time"]))
+    assert len(by_ref) == len(lin)
+
+
+def test_dispersal_stream_is_sorted_by_packed_event_order(dispersal_runs):
+    s, lin, full, spec = dispersal_runs
+    ev = full["events"]
+    keys = list(zip(np.abs(ev["event_time"]), ev["ox"], ev["oy"], ev["oindex"], ev["tx"], ev["ty"], ev["tindex"],
+                    np.abs(ev["prior_time"]), ev["global_reference"]))
+    assert keys == sorted(keys)
+
+
+def test_dispersal_reporting_pause_resume(dispersal_runs):
+    s, lin, full, spec = dispersal_runs
+    p1 = O.simulate_packed(s, O.make_params(3, 2e-2, dedup=None, report_dispersal=True, pause_before=40.0), lin)
+    p2 = O.simulate_packed(s, O.make_params(3, 2e-2, dedup=None, report_dispersal=True, min_event_time=40.0),
+                           p1["remaining"])
+    both = np.concatenate([p1["events"], p2["events"]])
+    assert both.tobytes() == full["events"].tobytes()
+    assert p1["steps"] + p2["steps"] == full["steps"]
