@@ -17,8 +17,8 @@
  *    thread-safe: use each context (and the objects created from it) from one
  *    thread at a time.  Concurrency comes from batching jobs inside one
  *    persistent kernel (the server API), not from concurrent calls.
- *  - Struct layouts of nh_lineage, nh_speciation_event, nh_alias_range,
- *    nh_self_dispersal and nh_alias_atom equal the reference's #[repr(C)]
+ *  - Struct layouts of nh_lineage, nh_speciation_event, nh_packed_event,
+ *    nh_alias_range, nh_self_dispersal and nh_alias_atom equal the reference's #[repr(C)]
  *    types on 64-bit targets, so the glue passes its own buffers unchanged.
  */
 #ifndef NECSIM_HIP_H
@@ -63,6 +63,21 @@ typedef struct {
     uint32_t x, y, index;
     uint32_t _pad;
 } nh_speciation_event;
+
+/* necsim/core/src/event.rs:16-29 `PackedEvent` (56 B): origin (x, y, index),
+ * target (x, y, index) and the interaction's coalescence reference.  A
+ * speciation has the sign bit of prior_time set and target = origin; a
+ * dispersal of the Independent algorithm (LineageInteraction::Maybe) has the
+ * sign bit of event_time set; coalescence = the lineage's own reference for
+ * both (event.rs:103-155). */
+typedef struct {
+    uint64_t global_reference;
+    double prior_time;
+    double event_time;
+    uint32_t origin_x, origin_y, origin_index;
+    uint32_t target_x, target_y, target_index;
+    uint64_t coalescence;
+} nh_packed_event;
 
 /* necsim/impls/no-std/src/alias/packed.rs:11-15 `AliasMethodSamplerAtom<usize>` (24 B) */
 typedef struct {
@@ -156,13 +171,19 @@ typedef struct {
     uint32_t exec_fill;            /* persistent form: a wave executes its lanes' next events once at least
                                       exec_fill eighths of its active lanes have found theirs (1..8);
                                       0 = default 5 */
+    int32_t report_dispersal;      /* the reporter's ReportDispersal: every event is reported as a
+                                      PackedEvent (nh_job_drain_packed_events) and the water level
+                                      advances by event_slice / sum(lambda) (monolithic/mod.rs:139-148);
+                                      0 = speciation only (nh_job_drain_events) */
+    uint64_t event_buffer;         /* report_dispersal: PackedEvents one round may report;
+                                      0 = default 2 * event_slice + lineages + 65536 */
 } nh_job_params;
 
 typedef struct {
     int32_t paused;      /* Status::Paused (lineages remain) */
     double time;         /* SimulationOutcome time: max event time */
     uint64_t steps;      /* SimulationOutcome steps: completed events */
-    uint64_t n_events;   /* speciation events produced so far */
+    uint64_t n_events;   /* events reported so far (speciations, plus dispersals with report_dispersal) */
     uint64_t n_remaining;/* lineages left (paused) */
     uint64_t rounds;     /* water-level rounds run */
     uint64_t launches;   /* step-kernel launches */
@@ -246,6 +267,14 @@ int nh_job_simulate(nh_job *job);
  * reporter (monolithic/reporter/live.rs:88-115) hands to the reporter.  Call
  * with buf == NULL to query the count in *n_out. */
 int nh_job_drain_events(nh_job *job, nh_speciation_event *buf, size_t cap, size_t *n_out);
+/* The same for a job with report_dispersal: every speciation and dispersal
+ * event since the last drain as PackedEvents, sorted by the PackedEvent
+ * order.  Each round's events are kept in a device buffer of event_buffer
+ * entries (by default 2 * event_slice + lineages + 65536, about twice what a
+ * round reports on average); a round that reports more fails with
+ * NH_E_CAPACITY and leaves the job failed (later calls: NH_E_STATE).  Either drain call on a job of the other kind
+ * fails with NH_E_STATE. */
+int nh_job_drain_packed_events(nh_job *job, nh_packed_event *buf, size_t cap, size_t *n_out);
 int nh_job_outcome(const nh_job *job, nh_outcome *out);
 /* Lineages remaining after a paused run (SimulationOutcome::Paused.lineages) */
 int nh_job_take_lineages(nh_job *job, nh_lineage *buf, size_t cap, size_t *n_out);
@@ -355,6 +384,11 @@ int nh_sample_origins(const nh_scenario *scenario, double percentage, uint32_t r
  * PackedEvent order, as nh_job_drain_events returns them; the reference's
  * replay (event_log/replay) reads such segments. */
 int nh_event_log_write_segment(const char *path, const nh_speciation_event *events, size_t n);
+/* The same for PackedEvents (speciations and dispersals as they are), with
+ * the header's with_speciation / with_dispersal flags given by the caller
+ * (the reporter's ReportSpeciation / ReportDispersal, recorder.rs:146-152). */
+int nh_event_log_write_packed_segment(const char *path, const nh_packed_event *events, size_t n,
+                                      int32_t with_speciation, int32_t with_dispersal);
 
 /* ---- lineage files (pause / resume) -------------------------------------- */
 /* LineageFileSaver (necsim/impls/std/src/lineage_file/saver.rs:54-75): writes
@@ -392,6 +426,9 @@ int nh_copy_calibrate(nh_ctx *ctx, size_t bytes, int32_t reps, double *ms_per_co
  * graph-capturable variant (device_sized != 0).  Allocates device memory. */
 int nh_debug_sort_events(nh_ctx *ctx, const nh_speciation_event *in, size_t n, int32_t device_sized,
                          nh_speciation_event *out);
+/* The same sort over PackedEvents (dispersal reporting's records). */
+int nh_debug_sort_packed_events(nh_ctx *ctx, const nh_packed_event *in, size_t n, int32_t device_sized,
+                                nh_packed_event *out);
 
 #ifdef __cplusplus
 }

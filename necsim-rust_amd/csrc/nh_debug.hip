@@ -110,15 +110,15 @@ extern "C" int nh_debug_eval(nh_ctx *ctx, int32_t op, const uint64_t *in, uint64
 // Sorts n host events by the PackedEvent order on the device, with the
 // eager sort of the job and server paths (device_sized == 0) or the
 // capturable one that reads its size from device memory (device_sized != 0).
-extern "C" int nh_debug_sort_events(nh_ctx *ctx, const nh_speciation_event *in, size_t n, int32_t device_sized,
-                                    nh_speciation_event *out) {
-    if (!ctx || (n && (!in || !out))) return set_error(NH_E_INVALID, "nh_debug_sort_events: bad argument");
+template <class E>
+static int debug_sort(nh_ctx *ctx, const E *in, size_t n, int32_t device_sized, E *out, const char *what) {
+    if (!ctx || (n && (!in || !out))) return set_error(NH_E_INVALID, "%s: bad argument", what);
     if (n == 0) return NH_OK;
-    if (n > 0xffffffffull) return set_error(NH_E_CAPACITY, "nh_debug_sort_events: at most 2^32 - 1 events");
-    if (hipSetDevice(ctx->device) != hipSuccess) return set_error(NH_E_HIP, "nh_debug_sort_events: hipSetDevice");
+    if (n > 0xffffffffull) return set_error(NH_E_CAPACITY, "%s: at most 2^32 - 1 events", what);
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_error(NH_E_HIP, "%s: hipSetDevice", what);
     hipStream_t st = ctx->stream;
     SortBuffers sb{};
-    DevEvent *d_in = nullptr, *d_out = nullptr;
+    E *d_in = nullptr, *d_out = nullptr;
     unsigned long long *d_range = nullptr;
     int rc = NH_OK;
     if ((rc = dalloc(&sb.keys[0], n)) || (rc = dalloc(&sb.keys[1], n)) || (rc = dalloc(&sb.vals[0], n)) ||
@@ -128,16 +128,16 @@ extern "C" int nh_debug_sort_events(nh_ctx *ctx, const nh_speciation_event *in, 
     } else {
         sb.capacity = n;
         unsigned long long range[2] = {0ull, (unsigned long long)n}; // start, end
-        hipError_t e = hipMemcpyAsync(d_in, in, n * sizeof(DevEvent), hipMemcpyHostToDevice, st);
+        hipError_t e = hipMemcpyAsync(d_in, in, n * sizeof(E), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(d_range, range, sizeof(range), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) {
             if (device_sized) sort_events_device_sized(d_in, d_range + 1, d_range, n, d_out, sb, st);
             else sort_events(d_in, n, d_out, sb, st, false);
             e = hipGetLastError();
         }
-        if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, n * sizeof(DevEvent), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, n * sizeof(E), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) rc = set_error(NH_E_HIP, "nh_debug_sort_events: %s", hipGetErrorString(e));
+        if (e != hipSuccess) rc = set_error(NH_E_HIP, "%s: %s", what, hipGetErrorString(e));
     }
     dfree(sb.keys[0]);
     dfree(sb.keys[1]);
@@ -149,4 +149,18 @@ extern "C" int nh_debug_sort_events(nh_ctx *ctx, const nh_speciation_event *in, 
     dfree(d_out);
     dfree(d_range);
     return rc;
+}
+
+extern "C" int nh_debug_sort_events(nh_ctx *ctx, const nh_speciation_event *in, size_t n, int32_t device_sized,
+                                    nh_speciation_event *out) {
+    static_assert(sizeof(nh_speciation_event) == sizeof(DevEvent), "event record");
+    return debug_sort(ctx, reinterpret_cast<const DevEvent *>(in), n, device_sized,
+                      reinterpret_cast<DevEvent *>(out), "nh_debug_sort_events");
+}
+
+extern "C" int nh_debug_sort_packed_events(nh_ctx *ctx, const nh_packed_event *in, size_t n, int32_t device_sized,
+                                           nh_packed_event *out) {
+    static_assert(sizeof(nh_packed_event) == sizeof(DevPacked), "packed event record");
+    return debug_sort(ctx, reinterpret_cast<const DevPacked *>(in), n, device_sized,
+                      reinterpret_cast<DevPacked *>(out), "nh_debug_sort_packed_events");
 }

@@ -122,6 +122,21 @@ struct DevEvent {
 };
 static_assert(sizeof(DevEvent) == 40, "event is 40 B");
 
+// PackedEvent (necsim/core/src/event.rs:16-29), 56 B: the reported events when
+// dispersal is reported too.  Speciation: prior_time sign bit set, target =
+// origin; the Independent algorithm's dispersal (LineageInteraction::Maybe):
+// event_time sign bit set; coalescence = the lineage's own reference for
+// both (event.rs:103-155).
+struct DevPacked {
+    uint64_t ref;
+    double prior_time;
+    double event_time;
+    uint32_t ox, oy, oindex;
+    uint32_t tx, ty, tindex;
+    uint64_t coalescence;
+};
+static_assert(sizeof(DevPacked) == 56, "packed event is 56 B");
+
 // Lane-resident lineage.  a/b hold (x, y) for the almost-infinite habitat and
 // (cell index, unused) for finite extents; `id` is the injective location id
 // used to prime the RNG.

@@ -279,6 +279,32 @@ bool speciation_less(const nh_speciation_event &a, const nh_speciation_event &b)
     return a.global_reference < b.global_reference;
 }
 
+// PackedEvent::cmp (necsim/core/src/event.rs:256-280): event_time, origin,
+// target, prior_time, reference; times by magnitude
+bool packed_less(const nh_packed_event &a, const nh_packed_event &b) {
+    const uint64_t m = 0x7fffffffffffffffull;
+    uint64_t ta = bits(a.event_time) & m, tb = bits(b.event_time) & m;
+    if (ta != tb) return ta < tb;
+    if (a.origin_x != b.origin_x) return a.origin_x < b.origin_x;
+    if (a.origin_y != b.origin_y) return a.origin_y < b.origin_y;
+    if (a.origin_index != b.origin_index) return a.origin_index < b.origin_index;
+    if (a.target_x != b.target_x) return a.target_x < b.target_x;
+    if (a.target_y != b.target_y) return a.target_y < b.target_y;
+    if (a.target_index != b.target_index) return a.target_index < b.target_index;
+    uint64_t pa = bits(a.prior_time) & m, pb = bits(b.prior_time) & m;
+    if (pa != pb) return pa < pb;
+    return a.global_reference < b.global_reference;
+}
+
+int write_file(const char *path, const std::vector<uint8_t> &b, const char *what) {
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return set_error(NH_E_INVALID, "%s: cannot open %s", what, path);
+    size_t w = std::fwrite(b.data(), 1, b.size(), f);
+    int c = std::fclose(f);
+    if (w != b.size() || c != 0) return set_error(NH_E_INVALID, "%s: write to %s failed", what, path);
+    return NH_OK;
+}
+
 } // namespace
 
 // EventLogRecorder::sort_and_write_segment (necsim/impls/std/src/event_log/recorder.rs:174-208)
@@ -316,12 +342,41 @@ extern "C" int nh_event_log_write_segment(const char *path, const nh_speciation_
         }
         put_u64(b, e.global_reference);
     }
-    FILE *f = std::fopen(path, "wb");
-    if (!f) return set_error(NH_E_INVALID, "nh_event_log_write_segment: cannot open %s", path);
-    size_t w = std::fwrite(b.data(), 1, b.size(), f);
-    int c = std::fclose(f);
-    if (w != b.size() || c != 0) return set_error(NH_E_INVALID, "nh_event_log_write_segment: write to %s failed", path);
-    return NH_OK;
+    return write_file(path, b, "nh_event_log_write_segment");
+}
+
+// The same for PackedEvents as they are (speciations and dispersals), with
+// the header's filter flags given by the caller (recorder.rs:146-152: the
+// reporter's ReportSpeciation / ReportDispersal); the header times are the
+// event times' magnitudes (PackedEvent::event_time, event.rs:33-36).
+extern "C" int nh_event_log_write_packed_segment(const char *path, const nh_packed_event *events, size_t n,
+                                                 int32_t with_speciation, int32_t with_dispersal) {
+    if (!path || !events || n == 0)
+        return set_error(NH_E_INVALID, "nh_event_log_write_packed_segment: a segment holds at least one event");
+    for (size_t i = 1; i < n; i++)
+        if (packed_less(events[i], events[i - 1]))
+            return set_error(NH_E_INVALID, "nh_event_log_write_packed_segment: events are not in PackedEvent order");
+    std::vector<uint8_t> b;
+    b.reserve(26 + 56 * n);
+    put_u64(b, bits(events[0].event_time) & 0x7fffffffffffffffull);
+    put_u64(b, bits(events[n - 1].event_time) & 0x7fffffffffffffffull);
+    put_u64(b, (uint64_t)n);
+    b.push_back(with_speciation ? 1 : 0);
+    b.push_back(with_dispersal ? 1 : 0);
+    for (size_t i = 0; i < n; i++) {
+        const nh_packed_event &e = events[i];
+        put_u64(b, e.global_reference);
+        put_u64(b, bits(e.prior_time));
+        put_u64(b, bits(e.event_time));
+        put_u32(b, e.origin_x);
+        put_u32(b, e.origin_y);
+        put_u32(b, e.origin_index);
+        put_u32(b, e.target_x);
+        put_u32(b, e.target_y);
+        put_u32(b, e.target_index);
+        put_u64(b, e.coalescence);
+    }
+    return write_file(path, b, "nh_event_log_write_packed_segment");
 }
 
 // ---------------------------------------------------------------------------

@@ -13,12 +13,12 @@ namespace nh {
 struct Counters {
     unsigned long long head;          // persistent work-queue head
     unsigned long long n_out;         // lineages written to the next round
-    unsigned long long n_events;      // speciation events appended
+    unsigned long long n_events;      // events appended (dispersal reporting: this round only)
     unsigned long long steps;         // completed events
     unsigned long long max_time_bits; // max event time (positive f64 bits)
     unsigned long long overflow;      // plain dedup: a bucket list exceeded the device limit
     unsigned long long n_dead;        // persistent form: lineages that died this round
-    unsigned long long pad;
+    unsigned long long ev_overflow;   // dispersal reporting: the round's event buffer was full
 };
 
 // Per-round inputs read by the kernels from device memory (so that a captured
@@ -61,6 +61,8 @@ struct PersistArgs {
     Counters *ctr;
     unsigned inner_steps;          // iterations per lane between wave-level refills (>= 1)
     unsigned exec_fill;            // eighths of the active lanes that must be ready to execute (1..8)
+    DevPacked *packed;             // dispersal reporting: the round's PackedEvents, else null
+    unsigned long long packed_cap;
 };
 
 struct PlainArgs {
@@ -78,6 +80,8 @@ struct PlainArgs {
     unsigned long long cache_cap; // 0: dedup off
     DevEvent *events;
     Counters *ctr;
+    DevPacked *packed;             // dispersal reporting: the round's PackedEvents, else null
+    unsigned long long packed_cap;
 };
 
 // ---- serving: one persistent kernel over the lineages of many requests ----
@@ -122,7 +126,7 @@ int serve_max_blocks_per_cu(int kind, int threads);
 
 // kernel launchers (all asynchronous on `stream`)
 void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream);
-int persistent_max_blocks_per_cu(int kind, int threads, bool dedup);
+int persistent_max_blocks_per_cu(int kind, int threads, bool dedup, bool report_dispersal);
 void launch_plain(int kind, const PlainArgs &a, int threads, hipStream_t stream);
 
 void launch_dedup_push(const uint8_t *status, const unsigned long long *bucket, uint32_t *head,
@@ -144,7 +148,7 @@ void launch_compact(const uint8_t *status, const uint32_t *pass, uint64_t m, uin
                     uint32_t *out_b, uint64_t b_base, CompactBuffers &cb, hipStream_t stream);
 
 // Event sort: sorts events[0..n) by the PackedEvent order into out[0..n).
-constexpr int SORT_MINMAX_WORDS = 16;
+constexpr int SORT_MINMAX_WORDS = 32;
 struct SortBuffers {
     unsigned long long *keys[2];
     uint32_t *vals[2];
@@ -153,13 +157,13 @@ struct SortBuffers {
     uint64_t capacity;
 };
 uint32_t sort_blocks(uint64_t n);
-void sort_events(const DevEvent *events, uint64_t n, DevEvent *out, SortBuffers &sb,
-                 hipStream_t stream, bool fixed_passes);
+template <class E>
+void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream_t stream, bool fixed_passes);
 // Same, with the event range [*start, *end) read from device memory when the
 // kernels run (fixed grid for up to `cap` events; capturable in a graph).
-void sort_events_device_sized(const DevEvent *events, const unsigned long long *end,
-                              const unsigned long long *start, uint64_t cap, DevEvent *out, SortBuffers &sb,
-                              hipStream_t stream);
+template <class E>
+void sort_events_device_sized(const E *events, const unsigned long long *end, const unsigned long long *start,
+                              uint64_t cap, E *out, SortBuffers &sb, hipStream_t stream);
 
 // Ordered compaction of the live (not dead) lineages of in[0, n) into out;
 // the live count is written to blk[soa_compact_blocks(n)].
@@ -169,7 +173,9 @@ void launch_soa_compact(const LineageSoA &in, const LineageSoA &out, uint64_t n,
 
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream);
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream);
-void launch_reset_counters(Counters *c, hipStream_t stream);
+// reset_events: also the event counter and its overflow flag (dispersal
+// reporting keeps one round's events at a time)
+void launch_reset_counters(Counters *c, bool reset_events, hipStream_t stream);
 
 } // namespace nh
 

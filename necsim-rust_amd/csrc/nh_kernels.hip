@@ -84,6 +84,65 @@ __device__ __forceinline__ void load_ratio(double *s_ratio, const double *ratio)
 }
 
 // ---------------------------------------------------------------------------
+// Dispersal reporting: every executed step is reported as a PackedEvent.
+// Each step first reserves its slot in the round's event buffer (one atomic
+// per wave for all lanes executing together); a lane whose slot lies beyond
+// the buffer does not execute, parks its lineage and flags the round, so
+// nothing is ever written past the buffer.
+__device__ __forceinline__ unsigned long long wave_reserve(unsigned long long *ctr) {
+    const uint64_t act = __ballot(1);
+    const uint32_t leader = __ffsll((unsigned long long)act) - 1;
+    unsigned long long base = 0;
+    if (lane_id() == leader) base = atomicAdd(ctr, (unsigned long long)__popcll(act));
+    base = __shfl(base, leader);
+    return base + (unsigned long long)__popcll(act & lanemask_lt());
+}
+
+constexpr uint64_t SIGN_BIT = 0x8000000000000000ull;
+
+struct Origin {
+    uint32_t x, y, i;
+    double prior;
+};
+
+template <int KIND>
+__device__ __forceinline__ Origin lane_origin(const Lane<KIND> &L, const DevScenario &S) {
+    Origin o;
+    location_xy<KIND>(S, L.a, L.b, o.x, o.y);
+    o.i = L.i;
+    o.prior = L.t_last;
+    return o;
+}
+
+// the event the step that left the lineage in L reported: a speciation
+// (prior(-), event(+), target = origin) or a dispersal with
+// LineageInteraction::Maybe (prior(+), event(-)); coalescence = the
+// lineage's own reference (necsim/core/src/event.rs:103-155)
+template <int KIND>
+__device__ __forceinline__ void emit_packed(DevPacked *out, const Origin &o, const Lane<KIND> &L,
+                                            const DevScenario &S, bool speciated) {
+    DevPacked e;
+    e.ref = L.ref;
+    e.coalescence = L.ref;
+    e.ox = o.x;
+    e.oy = o.y;
+    e.oindex = o.i;
+    if (speciated) {
+        e.prior_time = from_bits(bits(o.prior) | SIGN_BIT);
+        e.event_time = L.t_last;
+        e.tx = o.x;
+        e.ty = o.y;
+        e.tindex = o.i;
+    } else {
+        e.prior_time = o.prior;
+        e.event_time = from_bits(bits(L.t_last) | SIGN_BIT);
+        location_xy<KIND>(S, L.a, L.b, e.tx, e.ty);
+        e.tindex = L.i;
+    }
+    *out = e;
+}
+
+// ---------------------------------------------------------------------------
 // Wave-level work claiming: each wave owns a private range [next, end) of
 // work indices and refills it with ONE global atomic per claim, so lane
 // refill costs a ballot and a popcount, not a contended atomic.  Claims are
@@ -186,9 +245,10 @@ __device__ __forceinline__ bool wave_ready(bool ready, unsigned fill_eighths) {
 
 // The dedup variant holds the slice's min speciation sample as well: it gets
 // registers for 3 waves per SIMD instead of 4, which measured faster than
-// spilling.
-template <int KIND, bool DEDUP>
-__global__ void __launch_bounds__(256, DEDUP ? 3 : 4) k_persistent(PersistArgs A) {
+// spilling.  The DISP variant reports every step as a PackedEvent into the
+// round's buffer (speciations included) instead of speciations only.
+template <int KIND, bool DEDUP, bool DISP>
+__global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(PersistArgs A) {
     __shared__ double s_ratio[POISSON_TABLE_N];
     load_ratio(s_ratio, A.P.ratio);
 
@@ -252,13 +312,26 @@ __global__ void __launch_bounds__(256, DEDUP ? 3 : 4) k_persistent(PersistArgs A
                     has = false;
                     break;
                 }
+                Origin o;
+                unsigned long long eslot = 0;
+                if (DISP) {
+                    eslot = wave_reserve(&A.ctr->n_events);
+                    if (eslot >= A.packed_cap) {
+                        A.ctr->ev_overflow = 1;
+                        lane_store<KIND>(L, A.S, A.lin, slot);
+                        has = false;
+                        break;
+                    }
+                    o = lane_origin<KIND>(L, A.S);
+                }
                 DevEvent ev;
                 int r = lane_execute<KIND, DEDUP>(L, A.S, A.P, ev);
                 my_steps++;
                 slice_steps++;
                 my_max = L.t_last > my_max ? L.t_last : my_max;
+                if (DISP) emit_packed<KIND>(A.packed + eslot, o, L, A.S, r == STEP_SPECIATED);
                 if (r == STEP_SPECIATED) {
-                    A.events[atomicAdd(&A.ctr->n_events, 1ull)] = ev; // rare
+                    if (!DISP) A.events[atomicAdd(&A.ctr->n_events, 1ull)] = ev; // rare
                     A.lin.flags[slot] = FLAG_DEAD;
                     my_dead++;
                     has = false;
@@ -445,7 +518,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_soa_scatter(LineageSoA in, Linea
 
 // ---------------------------------------------------------------------------
 // Plain slice kernel: element q of the pass runs one step slice
-template <int KIND>
+template <int KIND, bool DISP>
 __global__ void __launch_bounds__(256) k_plain(PlainArgs A) {
     __shared__ double s_ratio[POISSON_TABLE_N];
     load_ratio(s_ratio, A.P.ratio);
@@ -476,11 +549,23 @@ __global__ void __launch_bounds__(256) k_plain(PlainArgs A) {
                     st = ST_SLOW;
                     break;
                 }
+                Origin o;
+                unsigned long long eslot = 0;
+                if (DISP) {
+                    eslot = wave_reserve(&A.ctr->n_events);
+                    if (eslot >= A.packed_cap) {
+                        A.ctr->ev_overflow = 1;
+                        st = ST_SLOW;
+                        break;
+                    }
+                    o = lane_origin<KIND>(L, A.S);
+                }
                 int r = lane_execute<KIND>(L, A.S, A.P, ev);
                 my_steps++;
+                if (DISP) emit_packed<KIND>(A.packed + eslot, o, L, A.S, r == STEP_SPECIATED);
                 if (r == STEP_SPECIATED) {
                     st = ST_SPEC;
-                    emit = true;
+                    emit = !DISP;
                     break;
                 }
             }
@@ -701,43 +786,72 @@ constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 // kernel of a pass a no-op while *gate == 0 (the tie passes of a captured
 // sort, which only run when the round has equal event times).
 struct SortSize {
-    uint64_t n;
+    uint64_t n; // the event count, or with `end` the capacity of the range
     const unsigned long long *end;
     const unsigned long long *start;
     const unsigned long long *gate;
-    __device__ __forceinline__ uint64_t get() const { return end ? (*end - (start ? *start : 0ull)) : n; }
+    __device__ __forceinline__ uint64_t get() const {
+        if (!end) return n;
+        uint64_t c = *end - (start ? *start : 0ull);
+        return c < n ? c : n; // an overflowed round reports its overflow; its sort stays in bounds
+    }
     __device__ __forceinline__ uint64_t offset() const { return start ? *start : 0ull; }
     // radix tiles actually used (the launch grid may be sized for capacity)
     __device__ __forceinline__ uint32_t tiles() const { return (uint32_t)((get() + RS_TILE - 1) / RS_TILE); }
     __device__ __forceinline__ bool skip() const { return gate && *gate == 0ull; }
 };
 
-// The PackedEvent order of speciation events (necsim/core/src/event.rs:256-280)
-// compares event_time, then the origin (x, y, index), the target (equal to
-// the origin), prior_time and the lineage reference.  As LSD radix keys, from
-// the least to the most significant field; positive / non-negative f64 sort
-// like their bit patterns.
-constexpr int SORT_FIELDS = 6;
-constexpr int F_TIME = 5;
-constexpr int FIELD_DIGITS[SORT_FIELDS] = {8, 8, 4, 4, 4, 8}; // ref, prior, index, y, x, time
-static_assert((8 + 8 + 4 + 4 + 4 + 8) % 2 == 0, "the gated tie passes must not change the ping-pong buffer");
-
-__device__ __forceinline__ unsigned long long event_field(const DevEvent &e, int f) {
-    switch (f) {
-    case 0: return e.ref;
-    case 1: return bits(e.prior_time) & 0x7fffffffffffffffull;
-    case 2: return e.index;
-    case 3: return e.y;
-    case 4: return e.x;
-    default: return bits(e.event_time) & 0x7fffffffffffffffull;
+// The PackedEvent order (necsim/core/src/event.rs:256-280) compares
+// event_time, then the origin (x, y, index), the target, prior_time and the
+// lineage reference, times by magnitude.  As LSD radix keys, from the least
+// to the most significant field; positive f64 sort like their bit patterns.
+// DevEvent (speciation only) has target = origin, so its key skips the
+// target.  Each record's total digit count is even, so the gated tie passes
+// of a captured sort leave the result in the same buffer whether they run.
+constexpr unsigned long long ABS_BITS = 0x7fffffffffffffffull;
+template <class E> struct EventOrder;
+template <> struct EventOrder<DevEvent> {
+    static constexpr int N = 6; // ref, prior, index, y, x, time
+    static constexpr int TIME = 5;
+    __host__ __device__ static constexpr int digits(int f) { return f <= 1 || f == 5 ? 8 : 4; }
+    __device__ static unsigned long long field(const DevEvent &e, int f) {
+        switch (f) {
+        case 0: return e.ref;
+        case 1: return bits(e.prior_time) & ABS_BITS;
+        case 2: return e.index;
+        case 3: return e.y;
+        case 4: return e.x;
+        default: return bits(e.event_time) & ABS_BITS;
+        }
     }
-}
+};
+template <> struct EventOrder<DevPacked> {
+    static constexpr int N = 9; // ref, prior, target index/y/x, origin index/y/x, time
+    static constexpr int TIME = 8;
+    __host__ __device__ static constexpr int digits(int f) { return f <= 1 || f == 8 ? 8 : 4; }
+    __device__ static unsigned long long field(const DevPacked &e, int f) {
+        switch (f) {
+        case 0: return e.ref;
+        case 1: return bits(e.prior_time) & ABS_BITS;
+        case 2: return e.tindex;
+        case 3: return e.ty;
+        case 4: return e.tx;
+        case 5: return e.oindex;
+        case 6: return e.oy;
+        case 7: return e.ox;
+        default: return bits(e.event_time) & ABS_BITS;
+        }
+    }
+};
+static_assert((8 + 8 + 4 + 4 + 4 + 8) % 2 == 0 && (8 + 8 + 6 * 4 + 8) % 2 == 0,
+              "the gated tie passes must not change the ping-pong buffer");
+constexpr int SORT_FIELDS_MAX = 9;
 
 // Word layout of SortBuffers::minmax
 constexpr int MM_TIME = 0;   // [min, max] of the event-time keys
 constexpr int MM_TIES = 2;   // number of adjacent equal event times after the time sort
 constexpr int MM_FIELDS = 4; // [min, max] per field
-constexpr int MM_WORDS = MM_FIELDS + 2 * SORT_FIELDS;
+constexpr int MM_WORDS = MM_FIELDS + 2 * SORT_FIELDS_MAX;
 static_assert(MM_WORDS <= SORT_MINMAX_WORDS, "sort range words");
 
 // min / max over the valid lanes of a wave, one atomic pair per wave
@@ -761,13 +875,14 @@ __global__ void k_sort_init(unsigned long long *mm) {
 }
 
 // time keys, identity values and the time key range
-__global__ void k_event_keys(const DevEvent *ev0, SortSize sz, unsigned long long *keys, uint32_t *vals,
+template <class E>
+__global__ void k_event_keys(const E *ev0, SortSize sz, unsigned long long *keys, uint32_t *vals,
                              unsigned long long *mm) {
     uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t n = sz.get();
-    const DevEvent *ev = ev0 + sz.offset();
+    const E *ev = ev0 + sz.offset();
     bool valid = p < n;
-    unsigned long long k = valid ? event_field(ev[p], F_TIME) : 0ull;
+    unsigned long long k = valid ? EventOrder<E>::field(ev[p], EventOrder<E>::TIME) : 0ull;
     if (valid) {
         keys[p] = k;
         vals[p] = (uint32_t)p;
@@ -785,23 +900,24 @@ __global__ void k_tie_count(const unsigned long long *keys, SortSize sz, unsigne
 }
 
 // per-field key ranges (the digits a field's passes must cover)
-__global__ void k_field_minmax(const DevEvent *ev0, SortSize sz, unsigned long long *mm) {
+template <class E> __global__ void k_field_minmax(const E *ev0, SortSize sz, unsigned long long *mm) {
     uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t n = sz.get();
-    const DevEvent *ev = ev0 + sz.offset();
+    const E *ev = ev0 + sz.offset();
     bool valid = p < n;
-    DevEvent e{};
+    E e{};
     if (valid) e = ev[p];
-    for (int f = 0; f < SORT_FIELDS; f++) wave_minmax(event_field(e, f), valid, mm + MM_FIELDS + 2 * f);
+    for (int f = 0; f < EventOrder<E>::N; f++)
+        wave_minmax(EventOrder<E>::field(e, f), valid, mm + MM_FIELDS + 2 * f);
 }
 
 // keys[p] = field f of the event at the current permutation's position p
-__global__ void k_field_keys(const DevEvent *ev0, const uint32_t *vals, unsigned long long *keys, int f,
-                             SortSize sz) {
+template <class E>
+__global__ void k_field_keys(const E *ev0, const uint32_t *vals, unsigned long long *keys, int f, SortSize sz) {
     if (sz.skip()) return;
     uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const DevEvent *ev = ev0 + sz.offset();
-    if (p < sz.get()) keys[p] = event_field(ev[vals[p]], f);
+    const E *ev = ev0 + sz.offset();
+    if (p < sz.get()) keys[p] = EventOrder<E>::field(ev[vals[p]], f);
 }
 
 __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const unsigned long long *keys, SortSize sz,
@@ -869,9 +985,9 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const unsigned lon
     }
 }
 
-__global__ void k_gather_events(const DevEvent *ev0, const uint32_t *vals, DevEvent *out, SortSize sz) {
+template <class E> __global__ void k_gather_events(const E *ev0, const uint32_t *vals, E *out, SortSize sz) {
     uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const DevEvent *ev = ev0 + sz.offset();
+    const E *ev = ev0 + sz.offset();
     if (p < sz.get()) out[p] = ev[vals[p]];
 }
 
@@ -883,46 +999,59 @@ __global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
-__global__ void k_reset_counters(Counters *c) {
+__global__ void k_reset_counters(Counters *c, bool reset_events) {
     c->head = 0;
     c->n_out = 0;
     c->n_dead = 0;
+    if (reset_events) {
+        c->n_events = 0;
+        c->ev_overflow = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 static inline unsigned grid_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-template <bool DEDUP>
+template <bool DEDUP, bool DISP>
 static void launch_persistent_t(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream) {
     switch (kind) {
-    case K_NON_SPATIAL: hipLaunchKernelGGL((k_persistent<K_NON_SPATIAL, DEDUP>), dim3(blocks), dim3(threads), 0, stream, a); break;
-    case K_ALMOST_INFINITE: hipLaunchKernelGGL((k_persistent<K_ALMOST_INFINITE, DEDUP>), dim3(blocks), dim3(threads), 0, stream, a); break;
-    case K_SEPARABLE: hipLaunchKernelGGL((k_persistent<K_SEPARABLE, DEDUP>), dim3(blocks), dim3(threads), 0, stream, a); break;
-    default: hipLaunchKernelGGL((k_persistent<K_PACKED, DEDUP>), dim3(blocks), dim3(threads), 0, stream, a); break;
+    case K_NON_SPATIAL: hipLaunchKernelGGL((k_persistent<K_NON_SPATIAL, DEDUP, DISP>), dim3(blocks), dim3(threads), 0, stream, a); break;
+    case K_ALMOST_INFINITE: hipLaunchKernelGGL((k_persistent<K_ALMOST_INFINITE, DEDUP, DISP>), dim3(blocks), dim3(threads), 0, stream, a); break;
+    case K_SEPARABLE: hipLaunchKernelGGL((k_persistent<K_SEPARABLE, DEDUP, DISP>), dim3(blocks), dim3(threads), 0, stream, a); break;
+    default: hipLaunchKernelGGL((k_persistent<K_PACKED, DEDUP, DISP>), dim3(blocks), dim3(threads), 0, stream, a); break;
     }
 }
 
 void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream) {
-    if (a.cache) launch_persistent_t<true>(kind, a, blocks, threads, stream);
-    else launch_persistent_t<false>(kind, a, blocks, threads, stream);
+    if (a.packed) {
+        if (a.cache) launch_persistent_t<true, true>(kind, a, blocks, threads, stream);
+        else launch_persistent_t<false, true>(kind, a, blocks, threads, stream);
+    } else {
+        if (a.cache) launch_persistent_t<true, false>(kind, a, blocks, threads, stream);
+        else launch_persistent_t<false, false>(kind, a, blocks, threads, stream);
+    }
 }
 
-template <bool DEDUP> static int persistent_max_blocks_per_cu_t(int kind, int threads) {
+template <bool DEDUP, bool DISP> static int persistent_max_blocks_per_cu_t(int kind, int threads) {
     int n = 0;
     switch (kind) {
-    case K_NON_SPATIAL: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_NON_SPATIAL, DEDUP>, threads, 0); break;
-    case K_ALMOST_INFINITE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_ALMOST_INFINITE, DEDUP>, threads, 0); break;
-    case K_SEPARABLE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_SEPARABLE, DEDUP>, threads, 0); break;
-    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_PACKED, DEDUP>, threads, 0); break;
+    case K_NON_SPATIAL: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_NON_SPATIAL, DEDUP, DISP>, threads, 0); break;
+    case K_ALMOST_INFINITE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_ALMOST_INFINITE, DEDUP, DISP>, threads, 0); break;
+    case K_SEPARABLE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_SEPARABLE, DEDUP, DISP>, threads, 0); break;
+    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_PACKED, DEDUP, DISP>, threads, 0); break;
     }
     return n > 0 ? n : 1;
 }
 
-// resident blocks per CU of the persistent kernel variant with / without dedup
-int persistent_max_blocks_per_cu(int kind, int threads, bool dedup) {
-    return dedup ? persistent_max_blocks_per_cu_t<true>(kind, threads)
-                 : persistent_max_blocks_per_cu_t<false>(kind, threads);
+// resident blocks per CU of the persistent kernel variant with / without
+// dedup and dispersal reporting
+int persistent_max_blocks_per_cu(int kind, int threads, bool dedup, bool report_dispersal) {
+    if (report_dispersal)
+        return dedup ? persistent_max_blocks_per_cu_t<true, true>(kind, threads)
+                     : persistent_max_blocks_per_cu_t<false, true>(kind, threads);
+    return dedup ? persistent_max_blocks_per_cu_t<true, false>(kind, threads)
+                 : persistent_max_blocks_per_cu_t<false, false>(kind, threads);
 }
 
 void launch_serve(int kind, const ServeArgs &a, int blocks, int threads, hipStream_t stream) {
@@ -945,15 +1074,21 @@ int serve_max_blocks_per_cu(int kind, int threads) {
     return n > 0 ? n : 1;
 }
 
+template <bool DISP>
+static void launch_plain_t(int kind, const PlainArgs &a, unsigned g, int threads, hipStream_t stream) {
+    switch (kind) {
+    case K_NON_SPATIAL: hipLaunchKernelGGL((k_plain<K_NON_SPATIAL, DISP>), dim3(g), dim3(threads), 0, stream, a); break;
+    case K_ALMOST_INFINITE: hipLaunchKernelGGL((k_plain<K_ALMOST_INFINITE, DISP>), dim3(g), dim3(threads), 0, stream, a); break;
+    case K_SEPARABLE: hipLaunchKernelGGL((k_plain<K_SEPARABLE, DISP>), dim3(g), dim3(threads), 0, stream, a); break;
+    default: hipLaunchKernelGGL((k_plain<K_PACKED, DISP>), dim3(g), dim3(threads), 0, stream, a); break;
+    }
+}
+
 void launch_plain(int kind, const PlainArgs &a, int threads, hipStream_t stream) {
     unsigned g = grid_for(a.m, threads);
     if (!g) return;
-    switch (kind) {
-    case K_NON_SPATIAL: hipLaunchKernelGGL(k_plain<K_NON_SPATIAL>, dim3(g), dim3(threads), 0, stream, a); break;
-    case K_ALMOST_INFINITE: hipLaunchKernelGGL(k_plain<K_ALMOST_INFINITE>, dim3(g), dim3(threads), 0, stream, a); break;
-    case K_SEPARABLE: hipLaunchKernelGGL(k_plain<K_SEPARABLE>, dim3(g), dim3(threads), 0, stream, a); break;
-    default: hipLaunchKernelGGL(k_plain<K_PACKED>, dim3(g), dim3(threads), 0, stream, a); break;
-    }
+    if (a.packed) launch_plain_t<true>(kind, a, g, threads, stream);
+    else launch_plain_t<false>(kind, a, g, threads, stream);
 }
 
 void launch_dedup_push(const uint8_t *status, const unsigned long long *bucket, uint32_t *head,
@@ -1037,18 +1172,20 @@ static int highest_digit(unsigned long long diff) {
     return -1;
 }
 
-// Sort of one round's speciation events by the PackedEvent order: an LSD
-// radix sort of the event times; if two events share a time (duplicate
-// lineages when dedup is off), a full LSD radix sort over every field of the
-// order.  Digits above the highest one in which a key range differs are
-// equal for every key and are skipped.
-void sort_events(const DevEvent *events, uint64_t n, DevEvent *out, SortBuffers &sb, hipStream_t stream,
-                 bool fixed_passes) {
+// Sort of one round's events by the PackedEvent order: an LSD radix sort of
+// the event times; if two events share a time (duplicate lineages when dedup
+// is off), a full LSD radix sort over every field of the order.  Digits
+// above the highest one in which a key range differs are equal for every key
+// and are skipped.
+template <class E>
+void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream_t stream, bool fixed_passes) {
+    using O = EventOrder<E>;
     if (n == 0) return;
     SortSize sz{n, nullptr, nullptr, nullptr};
     unsigned g = grid_for(n, 256);
     hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, stream, sb.minmax);
-    hipLaunchKernelGGL(k_event_keys, dim3(g), dim3(256), 0, stream, events, sz, sb.keys[0], sb.vals[0], sb.minmax);
+    hipLaunchKernelGGL(k_event_keys<E>, dim3(g), dim3(256), 0, stream, events, sz, sb.keys[0], sb.vals[0],
+                       sb.minmax);
     unsigned long long mm[MM_WORDS];
     int last = 7;
     if (!fixed_passes) {
@@ -1059,44 +1196,57 @@ void sort_events(const DevEvent *events, uint64_t n, DevEvent *out, SortBuffers 
     int cur = radix_passes(sz, n, sb, stream, 0, last, 0);
     if (n > 1) {
         hipLaunchKernelGGL(k_tie_count, dim3(g), dim3(256), 0, stream, sb.keys[cur], sz, sb.minmax + MM_TIES);
-        hipLaunchKernelGGL(k_field_minmax, dim3(g), dim3(256), 0, stream, events, sz, sb.minmax);
+        hipLaunchKernelGGL(k_field_minmax<E>, dim3(g), dim3(256), 0, stream, events, sz, sb.minmax);
         (void)hipMemcpyAsync(mm, sb.minmax, sizeof(mm), hipMemcpyDeviceToHost, stream);
         (void)hipStreamSynchronize(stream);
         if (mm[MM_TIES]) {
-            for (int f = 0; f < SORT_FIELDS; f++) {
-                int lf = fixed_passes ? FIELD_DIGITS[f] - 1
+            for (int f = 0; f < O::N; f++) {
+                int lf = fixed_passes ? O::digits(f) - 1
                                       : highest_digit(mm[MM_FIELDS + 2 * f] ^ mm[MM_FIELDS + 2 * f + 1]);
                 if (lf < 0) continue;
-                hipLaunchKernelGGL(k_field_keys, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], sb.keys[cur], f, sz);
+                hipLaunchKernelGGL(k_field_keys<E>, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], sb.keys[cur],
+                                   f, sz);
                 cur = radix_passes(sz, n, sb, stream, 0, lf, cur);
             }
         }
     }
-    hipLaunchKernelGGL(k_gather_events, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], out, sz);
+    hipLaunchKernelGGL(k_gather_events<E>, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], out, sz);
 }
 
 // Capturable variant: the event range is read from device memory, grids are
 // sized for `cap` events, every digit pass is launched, and the tie passes
 // are gated on the device-side tie count (an even number of passes, so the
 // result lies in the same buffer whether they run or not).
-void sort_events_device_sized(const DevEvent *events, const unsigned long long *end,
-                              const unsigned long long *start, uint64_t cap, DevEvent *out, SortBuffers &sb,
-                              hipStream_t stream) {
+template <class E>
+void sort_events_device_sized(const E *events, const unsigned long long *end, const unsigned long long *start,
+                              uint64_t cap, E *out, SortBuffers &sb, hipStream_t stream) {
+    using O = EventOrder<E>;
     if (cap == 0) return;
-    SortSize sz{0, end, start, nullptr};
+    SortSize sz{cap, end, start, nullptr};
     unsigned g = grid_for(cap, 256);
     hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, stream, sb.minmax);
-    hipLaunchKernelGGL(k_event_keys, dim3(g), dim3(256), 0, stream, events, sz, sb.keys[0], sb.vals[0], sb.minmax);
+    hipLaunchKernelGGL(k_event_keys<E>, dim3(g), dim3(256), 0, stream, events, sz, sb.keys[0], sb.vals[0],
+                       sb.minmax);
     int cur = radix_passes(sz, cap, sb, stream, 0, 7, 0);
     hipLaunchKernelGGL(k_tie_count, dim3(g), dim3(256), 0, stream, sb.keys[cur], sz, sb.minmax + MM_TIES);
     SortSize gz = sz;
     gz.gate = sb.minmax + MM_TIES;
-    for (int f = 0; f < SORT_FIELDS; f++) {
-        hipLaunchKernelGGL(k_field_keys, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], sb.keys[cur], f, gz);
-        cur = radix_passes(gz, cap, sb, stream, 0, FIELD_DIGITS[f] - 1, cur);
+    for (int f = 0; f < O::N; f++) {
+        hipLaunchKernelGGL(k_field_keys<E>, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], sb.keys[cur], f,
+                           gz);
+        cur = radix_passes(gz, cap, sb, stream, 0, O::digits(f) - 1, cur);
     }
-    hipLaunchKernelGGL(k_gather_events, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], out, sz);
+    hipLaunchKernelGGL(k_gather_events<E>, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], out, sz);
 }
+
+template void sort_events<DevEvent>(const DevEvent *, uint64_t, DevEvent *, SortBuffers &, hipStream_t, bool);
+template void sort_events<DevPacked>(const DevPacked *, uint64_t, DevPacked *, SortBuffers &, hipStream_t, bool);
+template void sort_events_device_sized<DevEvent>(const DevEvent *, const unsigned long long *,
+                                                 const unsigned long long *, uint64_t, DevEvent *, SortBuffers &,
+                                                 hipStream_t);
+template void sort_events_device_sized<DevPacked>(const DevPacked *, const unsigned long long *,
+                                                  const unsigned long long *, uint64_t, DevPacked *, SortBuffers &,
+                                                  hipStream_t);
 
 uint32_t soa_compact_blocks(uint64_t n) { return (uint32_t)((n + SC_TILE - 1) / SC_TILE); }
 
@@ -1118,8 +1268,8 @@ void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream) {
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream) {
     if (n) hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, stream, p, v, n);
 }
-void launch_reset_counters(Counters *c, hipStream_t stream) {
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, stream, c);
+void launch_reset_counters(Counters *c, bool reset_events, hipStream_t stream) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, stream, c, reset_events);
 }
 
 } // namespace nh

@@ -285,6 +285,11 @@ static bool lineage_valid(const nh_tables *t, const nh_lineage &l) {
 
 // ===========================================================================
 // jobs
+static_assert(sizeof(nh_packed_event) == sizeof(DevPacked) &&
+                  offsetof(nh_packed_event, coalescence) == offsetof(DevPacked, coalescence) &&
+                  offsetof(nh_packed_event, target_x) == offsetof(DevPacked, tx),
+              "nh_packed_event is the device PackedEvent record");
+
 struct nh_job {
     nh_ctx *ctx = nullptr;
     const nh_tables *tables = nullptr;
@@ -312,6 +317,13 @@ struct nh_job {
     uint64_t n_events = 0;
     std::vector<nh_speciation_event> ready;
     size_t ready_pos = 0;
+    // dispersal reporting: one round's PackedEvents at a time
+    bool disp = false;
+    bool failed = false;
+    DevPacked *d_packed = nullptr, *d_packed_sorted = nullptr;
+    uint64_t packed_cap = 0;
+    std::vector<nh_packed_event> ready_packed;
+    size_t packed_pos = 0;
     SortBuffers sb{};
     uint64_t *d_cache = nullptr;
 
@@ -337,6 +349,7 @@ struct nh_job {
     ~nh_job() {
         for (auto &l : lin) l.release();
         dfree(d_ratio); dfree(d_ctr); dfree(d_round); dfree(d_events); dfree(d_sorted);
+        dfree(d_packed); dfree(d_packed_sorted);
         dfree(sb.keys[0]); dfree(sb.keys[1]); dfree(sb.vals[0]); dfree(sb.vals[1]);
         dfree(sb.hist); dfree(sb.minmax); dfree(d_cache);
         dfree(d_S); dfree(d_F); dfree(d_pass[0]); dfree(d_pass[1]); dfree(d_status);
@@ -376,6 +389,7 @@ extern "C" int nh_job_create(nh_ctx *ctx, const nh_tables *tables, const nh_job_
     j->ctx = ctx;
     j->tables = tables;
     j->p = *p;
+    j->disp = p->report_dispersal != 0;
     *out = j;
     return NH_OK;
 }
@@ -440,14 +454,26 @@ static int job_start(nh_job *j) {
     NH_TRY(dalloc(&j->d_ctr, 1));
     NH_TRY(dalloc(&j->d_round, 1));
     NH_HIP(hipMemsetAsync(j->d_ctr, 0, sizeof(Counters), st));
-    NH_TRY(dalloc(&j->d_events, n));
-    NH_TRY(dalloc(&j->d_sorted, n));
-    j->sb.capacity = n;
-    NH_TRY(dalloc(&j->sb.keys[0], n));
-    NH_TRY(dalloc(&j->sb.keys[1], n));
-    NH_TRY(dalloc(&j->sb.vals[0], n));
-    NH_TRY(dalloc(&j->sb.vals[1], n));
-    NH_TRY(dalloc(&j->sb.hist, (size_t)256 * sort_blocks(n) + 1));
+    // speciation-only reporting appends to one buffer for the whole run (each
+    // lineage speciates at most once); dispersal reporting keeps one round's
+    // events, about event_slice of them on average (the level advances by
+    // event_slice / sum(lambda))
+    uint64_t sort_cap = n;
+    if (j->disp) {
+        j->packed_cap = p.event_buffer ? p.event_buffer : 2 * j->event_slice + n + 65536;
+        NH_TRY(dalloc(&j->d_packed, j->packed_cap));
+        NH_TRY(dalloc(&j->d_packed_sorted, j->packed_cap));
+        sort_cap = j->packed_cap;
+    } else {
+        NH_TRY(dalloc(&j->d_events, n));
+        NH_TRY(dalloc(&j->d_sorted, n));
+    }
+    j->sb.capacity = sort_cap;
+    NH_TRY(dalloc(&j->sb.keys[0], sort_cap));
+    NH_TRY(dalloc(&j->sb.keys[1], sort_cap));
+    NH_TRY(dalloc(&j->sb.vals[0], sort_cap));
+    NH_TRY(dalloc(&j->sb.vals[1], sort_cap));
+    NH_TRY(dalloc(&j->sb.hist, (size_t)256 * sort_blocks(sort_cap) + 1));
     NH_TRY(dalloc(&j->sb.minmax, SORT_MINMAX_WORDS));
     if (j->cache_cap) {
         NH_TRY(dalloc(&j->d_cache, 4 * j->cache_cap));
@@ -465,7 +491,7 @@ static int job_start(nh_job *j) {
     if (p.form == NH_FORM_PERSISTENT) {
         NH_TRY(j->lin[1].alloc(n));
         NH_TRY(dalloc(&j->d_blk, soa_compact_blocks(n) + 1));
-        int per_cu = persistent_max_blocks_per_cu(j->tables->kind, j->threads, j->cache_cap != 0);
+        int per_cu = persistent_max_blocks_per_cu(j->tables->kind, j->threads, j->cache_cap != 0, j->disp);
         if (p.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)p.blocks_per_cu);
         j->blocks = per_cu * (int)j->ctx->cus;
     } else {
@@ -577,11 +603,19 @@ static int plain_round(nh_job *j) {
         a.cache_cap = j->cache_cap;
         a.events = j->d_events;
         a.ctr = j->d_ctr;
+        a.packed = j->d_packed;
+        a.packed_cap = j->packed_cap;
         NH_HIP(hipEventRecord(j->ev0, st));
         launch_plain(j->tables->kind, a, j->threads, st);
         NH_HIP(hipGetLastError());
         NH_HIP(hipEventRecord(j->ev1, st));
         j->launches++;
+        if (j->disp) {
+            // a lineage that found the buffer full made no progress: stop here
+            Counters c;
+            NH_TRY(read_counters(j, c));
+            if (c.ev_overflow) return NH_OK;
+        }
         if (j->cache_cap) {
             launch_dedup_push(j->d_status, j->s_bucket, j->d_head, j->d_link, m, st);
             launch_dedup_resolve(j->d_status, j->s_bucket, j->s_u, j->s_t, j->s_x, j->s_y, j->s_i, j->d_head,
@@ -666,6 +700,8 @@ static PersistArgs persist_args(nh_job *j, int cur) {
     a.cache_cap = j->cache_cap;
     a.events = j->d_events;
     a.ctr = j->d_ctr;
+    a.packed = j->d_packed;
+    a.packed_cap = j->packed_cap;
     a.inner_steps = j->p.inner_steps ? j->p.inner_steps : 16;
     a.exec_fill = j->p.exec_fill ? j->p.exec_fill : 5;
     return a;
@@ -684,10 +720,14 @@ static int persistent_round(nh_job *j, bool *sorted) {
         if (!j->graph[g]) {
             hipGraph_t graph = nullptr;
             NH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            launch_reset_counters(j->d_ctr, st);
+            launch_reset_counters(j->d_ctr, j->disp, st);
             launch_persistent(j->tables->kind, persist_args(j, g), j->blocks, j->threads, st);
-            sort_events_device_sized(j->d_events, &j->d_ctr->n_events, &j->d_round->ev_start, j->n_initial,
-                                     j->d_sorted, j->sb, st);
+            if (j->disp)
+                sort_events_device_sized(j->d_packed, &j->d_ctr->n_events, nullptr, j->packed_cap,
+                                         j->d_packed_sorted, j->sb, st);
+            else
+                sort_events_device_sized(j->d_events, &j->d_ctr->n_events, &j->d_round->ev_start, j->n_initial,
+                                         j->d_sorted, j->sb, st);
             NH_HIP(hipStreamEndCapture(st, &graph));
             hipError_t e = hipGraphInstantiate(&j->graph[g], graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
@@ -698,7 +738,7 @@ static int persistent_round(nh_job *j, bool *sorted) {
         NH_HIP(hipEventRecord(j->ev1, st));
         *sorted = true;
     } else {
-        launch_reset_counters(j->d_ctr, st);
+        launch_reset_counters(j->d_ctr, j->disp, st);
         NH_HIP(hipEventRecord(j->ev0, st));
         launch_persistent(j->tables->kind, persist_args(j, j->cur), j->blocks, j->threads, st);
         NH_HIP(hipGetLastError());
@@ -727,29 +767,44 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
     if (!j->started) NH_TRY(job_start(j));
     const nh_job_params &p = j->p;
     auto done = [&]() { return j->alive == 0 || (p.has_pause_before && !(j->level_time < p.pause_before)); };
+    if (j->failed) return set_error(NH_E_STATE, "the job failed in an earlier round");
     if (!j->finished && done()) j->finished = true;
     if (j->finished) {
         if (finished) *finished = 1;
         return NH_OK;
     }
     hipStream_t st = j->ctx->stream;
-    // speciation-only reporting: total rate = sum of lambda * nu over the slow
-    // lineages, folded in order (monolithic/mod.rs:150-164)
-    double total_rate = repeated_sum(p.turnover_rate * p.speciation_probability, j->alive);
+    // total rate = sum of the reported event rate over the slow lineages,
+    // folded in order: lambda with dispersal reporting (monolithic/mod.rs:139-148),
+    // lambda * nu for speciation only (monolithic/mod.rs:150-164)
+    double rate = j->disp ? p.turnover_rate : p.turnover_rate * p.speciation_probability;
+    double total_rate = repeated_sum(rate, j->alive);
     j->level_time = j->level_time + (double)j->event_slice / total_rate;
     if (p.has_pause_before && p.pause_before < j->level_time) j->level_time = p.pause_before;
-    j->round_host = RoundDesc{j->level_time, (unsigned long long)j->cur_n, (unsigned long long)j->n_events,
+    uint64_t ev_start = j->disp ? 0 : j->n_events;
+    j->round_host = RoundDesc{j->level_time, (unsigned long long)j->cur_n, (unsigned long long)ev_start,
                               claim_chunk(j->cur_n, (unsigned long long)j->blocks * j->threads)};
     NH_HIP(hipMemcpyAsync(j->d_round, &j->round_host, sizeof(RoundDesc), hipMemcpyHostToDevice, st));
+    if (j->disp && p.form == NH_FORM_PLAIN) {
+        NH_HIP(hipMemsetAsync(&j->d_ctr->n_events, 0, sizeof(unsigned long long), st));
+        NH_HIP(hipMemsetAsync(&j->d_ctr->ev_overflow, 0, sizeof(unsigned long long), st));
+    }
 
-    uint64_t ev_start = j->n_events;
     bool sorted = false;
     if (p.form == NH_FORM_PLAIN) NH_TRY(plain_round(j));
     else NH_TRY(persistent_round(j, &sorted));
 
     Counters c;
     NH_TRY(read_counters(j, c));
-    j->n_events = c.n_events;
+    if (c.ev_overflow) {
+        j->failed = true;
+        return set_error(NH_E_CAPACITY,
+                         "a round reported more than %llu events (the dispersal event buffer); "
+                         "use a smaller event slice",
+                         (unsigned long long)j->packed_cap);
+    }
+    uint64_t ne = c.n_events - ev_start;
+    j->n_events = j->disp ? j->n_events + ne : c.n_events;
     j->steps = c.steps;
     double mt = from_bits(c.max_time_bits);
     if (c.max_time_bits && mt > j->max_time) j->max_time = mt;
@@ -759,8 +814,15 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
     j->rounds++;
 
     // the live water-level reporter sorts each round's events (live.rs:88-115)
-    uint64_t ne = j->n_events - ev_start;
-    if (ne) {
+    if (ne && j->disp) {
+        if (!sorted) sort_events(j->d_packed, ne, j->d_packed_sorted, j->sb, st, false);
+        NH_HIP(hipGetLastError());
+        size_t old = j->ready_packed.size();
+        j->ready_packed.resize(old + ne);
+        NH_HIP(hipMemcpyAsync(j->ready_packed.data() + old, j->d_packed_sorted, ne * sizeof(DevPacked),
+                              hipMemcpyDeviceToHost, st));
+        NH_HIP(hipStreamSynchronize(st));
+    } else if (ne) {
         if (!sorted) sort_events(j->d_events + ev_start, ne, j->d_sorted, j->sb, st, false);
         NH_HIP(hipGetLastError());
         size_t old = j->ready.size();
@@ -779,21 +841,34 @@ extern "C" int nh_job_simulate(nh_job *j) {
     return NH_OK;
 }
 
-extern "C" int nh_job_drain_events(nh_job *j, nh_speciation_event *buf, size_t cap, size_t *n_out) {
-    if (!j || !n_out) return set_error(NH_E_INVALID, "null argument");
-    size_t avail = j->ready.size() - j->ready_pos;
+// hands out the front of a ready list, in order; the list is emptied once read
+template <class T> static void drain_ready(std::vector<T> &ready, size_t &pos, T *buf, size_t cap, size_t *n_out) {
+    size_t avail = ready.size() - pos;
     if (!buf) {
         *n_out = avail;
-        return NH_OK;
+        return;
     }
     size_t n = std::min(avail, cap);
-    std::memcpy(buf, j->ready.data() + j->ready_pos, n * sizeof(nh_speciation_event));
-    j->ready_pos += n;
-    if (j->ready_pos == j->ready.size()) {
-        j->ready.clear();
-        j->ready_pos = 0;
+    std::memcpy(buf, ready.data() + pos, n * sizeof(T));
+    pos += n;
+    if (pos == ready.size()) {
+        ready.clear();
+        pos = 0;
     }
     *n_out = n;
+}
+
+extern "C" int nh_job_drain_packed_events(nh_job *j, nh_packed_event *buf, size_t cap, size_t *n_out) {
+    if (!j || !n_out) return set_error(NH_E_INVALID, "null argument");
+    if (!j->disp) return set_error(NH_E_STATE, "the job reports speciation events only (nh_job_drain_events)");
+    drain_ready(j->ready_packed, j->packed_pos, buf, cap, n_out);
+    return NH_OK;
+}
+
+extern "C" int nh_job_drain_events(nh_job *j, nh_speciation_event *buf, size_t cap, size_t *n_out) {
+    if (!j || !n_out) return set_error(NH_E_INVALID, "null argument");
+    if (j->disp) return set_error(NH_E_STATE, "the job reports dispersal: drain PackedEvents (nh_job_drain_packed_events)");
+    drain_ready(j->ready, j->ready_pos, buf, cap, n_out);
     return NH_OK;
 }
 
@@ -858,7 +933,8 @@ extern "C" int nh_autotune(nh_ctx *ctx, const nh_tables *tables, const nh_job_pa
     winner.form = NH_FORM_PERSISTENT;
     for (uint32_t threads : {128u, 256u}) {
         int max_per_cu = persistent_max_blocks_per_cu(tables->kind, (int)threads,
-                                                      base->dedup_mode != NH_DEDUP_NONE);
+                                                      base->dedup_mode != NH_DEDUP_NONE,
+                                                      base->report_dispersal != 0);
         std::vector<uint32_t> residency = {(uint32_t)max_per_cu};
         if (max_per_cu >= 2) residency.push_back((uint32_t)(max_per_cu / 2));
         for (uint32_t per_cu : residency) {

@@ -6,14 +6,15 @@ rustcoalescence's algorithm plugin interface and the synthetic workload
 generators used by the tests and the benchmark.
 """
 from . import _ffi
-from ._ffi import EVENT_DTYPE, LINEAGE_DTYPE, NecsimHipError, lib
+from ._ffi import EVENT_DTYPE, LINEAGE_DTYPE, PACKED_EVENT_DTYPE, NecsimHipError, lib
 from .algorithm import (PERSISTENT, PLAIN, AlmostInfiniteNormalDispersalScenario,
                         BiodiversityReporter, Context, DedupCache, EventCollector, EventLogReporter,
                         EventSlice,
                         HipIndependentAlgorithm, IndependentArguments, Job, NonSpatialScenario,
-                        OriginPreSampler, ScenarioCogs, Server, SimulationOutcome,
+                        OriginPreSampler, PackedEventCollector, ScenarioCogs, Server, SimulationOutcome,
                         SpatiallyExplicitUniformTurnoverScenario, Tables, WyHash,
-                        autotune, build_alias_tables, copy_calibrate, load_lineages, save_lineages)
+                        autotune, build_alias_tables, copy_calibrate, is_speciation, load_lineages,
+                        packed_order, save_lineages, speciation_events)
 from .maps import generate_map
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -28,6 +28,11 @@ LINEAGE_DTYPE = np.dtype([("global_reference", "<u8"), ("last_event_time", "<f8"
                           ("x", "<u4"), ("y", "<u4"), ("index", "<u4"), ("_pad", "<u4")])
 EVENT_DTYPE = np.dtype([("global_reference", "<u8"), ("prior_time", "<f8"), ("event_time", "<f8"),
                         ("x", "<u4"), ("y", "<u4"), ("index", "<u4"), ("_pad", "<u4")])
+# necsim/core/src/event.rs:16-29 PackedEvent (nh_packed_event)
+PACKED_EVENT_DTYPE = np.dtype([("global_reference", "<u8"), ("prior_time", "<f8"), ("event_time", "<f8"),
+                               ("origin_x", "<u4"), ("origin_y", "<u4"), ("origin_index", "<u4"),
+                               ("target_x", "<u4"), ("target_y", "<u4"), ("target_index", "<u4"),
+                               ("coalescence", "<u8")])
 ATOM_DTYPE = np.dtype([("u", "<f8"), ("e", "<u8"), ("k", "<u8")])
 RANGE_DTYPE = np.dtype([("start", "<u8"), ("end", "<u8")])
 SELF_DTYPE = np.dtype([("self_dispersal", "<f8"), ("non_self_dispersal_event", "<u8")])
@@ -59,7 +64,8 @@ class JobParams(ctypes.Structure):
                 ("event_slice_factor", c_f64), ("event_slice_capacity", c_u64),
                 ("has_pause_before", c_i32), ("pause_before", c_f64), ("resume_after", c_f64),
                 ("form", c_i32), ("block_size", c_u32), ("blocks_per_cu", c_u32),
-                ("use_graph", c_u32), ("inner_steps", c_u32), ("exec_fill", c_u32)]
+                ("use_graph", c_u32), ("inner_steps", c_u32), ("exec_fill", c_u32),
+                ("report_dispersal", c_i32), ("event_buffer", c_u64)]
 
 
 class Outcome(ctypes.Structure):
@@ -106,6 +112,7 @@ SIGNATURES = {
     "nh_job_step_round": (ctypes.c_int, [c_vp, ctypes.POINTER(c_i32)]),
     "nh_job_simulate": (ctypes.c_int, [c_vp]),
     "nh_job_drain_events": (ctypes.c_int, [c_vp, c_vp, c_sz, ctypes.POINTER(c_sz)]),
+    "nh_job_drain_packed_events": (ctypes.c_int, [c_vp, c_vp, c_sz, ctypes.POINTER(c_sz)]),
     "nh_job_outcome": (ctypes.c_int, [c_vp, ctypes.POINTER(Outcome)]),
     "nh_job_take_lineages": (ctypes.c_int, [c_vp, c_vp, c_sz, ctypes.POINTER(c_sz)]),
     "nh_job_destroy": (ctypes.c_int, [c_vp]),
@@ -124,10 +131,12 @@ SIGNATURES = {
                                              c_vp, c_sz, ctypes.POINTER(c_sz)]),
     "nh_debug_eval": (ctypes.c_int, [c_vp, c_i32, c_vp, c_vp, c_sz]),
     "nh_event_log_write_segment": (ctypes.c_int, [ctypes.c_char_p, c_vp, c_sz]),
+    "nh_event_log_write_packed_segment": (ctypes.c_int, [ctypes.c_char_p, c_vp, c_sz, c_i32, c_i32]),
     "nh_lineage_file_write": (ctypes.c_int, [ctypes.c_char_p, c_vp, c_sz]),
     "nh_lineage_file_read": (ctypes.c_int, [ctypes.c_char_p, c_vp, c_sz, ctypes.POINTER(c_sz)]),
     "nh_copy_calibrate": (ctypes.c_int, [c_vp, c_sz, c_i32, ctypes.POINTER(ctypes.c_double)]),
     "nh_debug_sort_events": (ctypes.c_int, [c_vp, c_vp, c_sz, c_i32, c_vp]),
+    "nh_debug_sort_packed_events": (ctypes.c_int, [c_vp, c_vp, c_sz, c_i32, c_vp]),
     "nh_sample_origins": (ctypes.c_int, [ctypes.POINTER(Scenario), c_f64, c_u32, c_u32, c_u32, c_u32,
                                          c_u32, c_vp, c_sz, ctypes.POINTER(c_sz)]),
 }

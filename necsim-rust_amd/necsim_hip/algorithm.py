@@ -288,6 +288,58 @@ class EventCollector:
         return np.concatenate(self.chunks)
 
 
+SIGN_BIT = np.uint64(1 << 63)
+
+
+def _magnitude_bits(f64):
+    return np.ascontiguousarray(f64, dtype=np.float64).view(np.uint64) & ~SIGN_BIT
+
+
+def is_speciation(packed):
+    """PackedEvent -> SpeciationEvent classification: the sign bit of
+    prior_time (necsim/core/src/event.rs:103-115, 191-204)"""
+    return (np.ascontiguousarray(packed["prior_time"]).view(np.uint64) & SIGN_BIT) != 0
+
+
+def speciation_events(packed):
+    """the speciation events of a PackedEvent stream, as SpeciationEvents"""
+    sp = packed[is_speciation(packed)]
+    out = np.zeros(len(sp), _ffi.EVENT_DTYPE)
+    out["global_reference"] = sp["global_reference"]
+    out["prior_time"] = _magnitude_bits(sp["prior_time"]).view(np.float64)
+    out["event_time"] = sp["event_time"]
+    out["x"], out["y"], out["index"] = sp["origin_x"], sp["origin_y"], sp["origin_index"]
+    return out
+
+
+def packed_order(packed):
+    """indices that sort PackedEvents by PackedEvent::cmp (event.rs:256-280)"""
+    return np.lexsort((packed["global_reference"], _magnitude_bits(packed["prior_time"]),
+                       packed["target_index"], packed["target_y"], packed["target_x"],
+                       packed["origin_index"], packed["origin_y"], packed["origin_x"],
+                       _magnitude_bits(packed["event_time"])))
+
+
+class PackedEventCollector:
+    """A reporter with ReportDispersal (the reference's
+    Reporter::ReportDispersal = True): keeps every reported event, speciations
+    and dispersals, as the sorted PackedEvent stream."""
+
+    REPORT_DISPERSAL = True
+
+    def __init__(self):
+        self.chunks = []
+
+    def report_packed(self, events):
+        self.chunks.append(np.array(events, copy=True))
+
+    @property
+    def events(self):
+        if not self.chunks:
+            return np.zeros(0, _ffi.PACKED_EVENT_DTYPE)
+        return np.concatenate(self.chunks)
+
+
 def save_lineages(path, lineages):
     """LineageFileSaver: paused lineages to a new lineage file (bincode varint)"""
     lin = np.ascontiguousarray(lineages, dtype=_ffi.LINEAGE_DTYPE)
@@ -305,12 +357,14 @@ def load_lineages(path):
 
 class EventLogReporter:
     """EventLogRecorder (necsim/impls/std/src/event_log/recorder.rs:20-210) as a
-    speciation reporter: events are buffered in reporting order and written as
-    numbered segment files of `segment_capacity` events (the last one shorter),
-    each sorted by the PackedEvent order; a log without events leaves no
-    directory behind (recorder.rs:46-55).  The directory must not exist yet."""
+    reporter: events are buffered in reporting order and written as numbered
+    segment files of `segment_capacity` events (the last one shorter), each
+    sorted by the PackedEvent order; a log without events leaves no directory
+    behind (recorder.rs:46-55).  The directory must not exist yet.  With
+    report_dispersal the log records every event (speciations and
+    dispersals, header with_dispersal = true) instead of speciations only."""
 
-    def __init__(self, directory, segment_capacity=1 << 20):
+    def __init__(self, directory, segment_capacity=1 << 20, report_dispersal=False):
         if segment_capacity < 1:
             raise ValueError("segment_capacity must be positive")
         directory = os.path.abspath(directory)
@@ -319,13 +373,14 @@ class EventLogReporter:
         self.directory = directory
         self.capacity = int(segment_capacity)
         self.segment_index = 0
+        self.REPORT_DISPERSAL = bool(report_dispersal)
         self._buffer = []
         self._buffered = 0
 
-    def report_speciation(self, events):
+    def _append(self, events, dtype):
         if len(events) == 0:
             return
-        self._buffer.append(np.ascontiguousarray(events, dtype=_ffi.EVENT_DTYPE))
+        self._buffer.append(np.ascontiguousarray(events, dtype=dtype))
         self._buffered += len(events)
         while self._buffered >= self.capacity:
             ev = np.concatenate(self._buffer)
@@ -334,12 +389,26 @@ class EventLogReporter:
             self._buffer = [rest] if len(rest) else []
             self._buffered = len(rest)
 
+    def report_speciation(self, events):
+        if self.REPORT_DISPERSAL:
+            raise TypeError("this log records PackedEvents (report_packed)")
+        self._append(events, _ffi.EVENT_DTYPE)
+
+    def report_packed(self, events):
+        if not self.REPORT_DISPERSAL:
+            raise TypeError("this log records speciation events only (report_speciation)")
+        self._append(events, _ffi.PACKED_EVENT_DTYPE)
+
     def _write(self, ev):
-        order = np.lexsort((ev["global_reference"], ev["prior_time"], ev["index"], ev["y"], ev["x"],
-                            ev["event_time"]))
-        ev = np.ascontiguousarray(ev[order])
         path = os.path.join(self.directory, str(self.segment_index)).encode()
-        check(lib().nh_event_log_write_segment(path, ptr(ev), len(ev)))
+        if self.REPORT_DISPERSAL:
+            ev = np.ascontiguousarray(ev[packed_order(ev)])
+            check(lib().nh_event_log_write_packed_segment(path, ptr(ev), len(ev), 1, 1))
+        else:
+            order = np.lexsort((ev["global_reference"], ev["prior_time"], ev["index"], ev["y"], ev["x"],
+                                ev["event_time"]))
+            ev = np.ascontiguousarray(ev[order])
+            check(lib().nh_event_log_write_segment(path, ptr(ev), len(ev)))
         self.segment_index += 1
 
     def close(self):
@@ -409,8 +478,10 @@ class Job:
     """One simulation on the device (nh_job_*)."""
 
     def __init__(self, ctx, tables, args: IndependentArguments, rng: WyHash, speciation_probability,
-                 turnover_rate, pause_before=None, resume_after=0.0):
+                 turnover_rate, pause_before=None, resume_after=0.0, report_dispersal=False, event_buffer=0):
         p = _job_params(args, rng, speciation_probability, turnover_rate, pause_before, resume_after)
+        p.report_dispersal = 1 if report_dispersal else 0
+        p.event_buffer = int(event_buffer)
         h = ctypes.c_void_p()
         check(lib().nh_job_create(ctx.handle, tables.handle, ctypes.byref(p), ctypes.byref(h)))
         self.handle = h
@@ -434,6 +505,15 @@ class Job:
         buf = np.zeros(n.value, _ffi.EVENT_DTYPE)
         if n.value:
             check(lib().nh_job_drain_events(self.handle, ptr(buf), n.value, ctypes.byref(n)))
+        return buf[:n.value]
+
+    def drain_packed_events(self):
+        """every event since the last drain (report_dispersal jobs), sorted"""
+        n = ctypes.c_size_t(0)
+        check(lib().nh_job_drain_packed_events(self.handle, None, 0, ctypes.byref(n)))
+        buf = np.zeros(n.value, _ffi.PACKED_EVENT_DTYPE)
+        if n.value:
+            check(lib().nh_job_drain_packed_events(self.handle, ptr(buf), n.value, ctypes.byref(n)))
         return buf[:n.value]
 
     def outcome(self):
@@ -509,22 +589,34 @@ def copy_calibrate(ctx, nbytes, reps=10):
 
 class HipIndependentAlgorithm:
     """Algorithm::initialise_and_simulate / resume_and_simulate of the HIP
-    Independent algorithm (Monolithic parallelism mode)."""
+    Independent algorithm (Monolithic parallelism mode).
+
+    A reporter whose REPORT_DISPERSAL is true (the reference's
+    Reporter::ReportDispersal) receives every event of each water-level round,
+    speciations and dispersals, as sorted PackedEvents through report_packed;
+    any other reporter receives the speciation events through
+    report_speciation."""
 
     @staticmethod
     def _run(args, rng, cogs, lineages, pause_before, resume_after, reporter):
         ctx = Context.get(args.device)
         tables = Tables(ctx, cogs)
+        disp = bool(getattr(reporter, "REPORT_DISPERSAL", False))
         job = Job(ctx, tables, args, rng, cogs.speciation_probability, cogs.turnover_rate,
-                  pause_before, resume_after)
+                  pause_before, resume_after, report_dispersal=disp)
         try:
             job.add_lineages(lineages)
             finished = False
             while not finished:
                 finished = job.step_round()
-                ev = job.drain_events()
-                if len(ev) and reporter is not None:
-                    reporter.report_speciation(ev)
+                if disp:
+                    ev = job.drain_packed_events()
+                    if len(ev):
+                        reporter.report_packed(ev)
+                else:
+                    ev = job.drain_events()
+                    if len(ev) and reporter is not None:
+                        reporter.report_speciation(ev)
             o = job.outcome()
             ms, _ = job.kernel_time()
             rem = job.take_lineages() if o.paused else None

@@ -43,6 +43,22 @@ pub struct nh_speciation_event {
     pub _pad: u32,
 }
 
+/// Same layout as necsim_core::event::PackedEvent (#[repr(C)], 56 B).
+#[repr(C)]
+#[derive(Clone, Copy, Default)]
+pub struct nh_packed_event {
+    pub global_reference: u64,
+    pub prior_time: f64,
+    pub event_time: f64,
+    pub origin_x: u32,
+    pub origin_y: u32,
+    pub origin_index: u32,
+    pub target_x: u32,
+    pub target_y: u32,
+    pub target_index: u32,
+    pub coalescence: u64,
+}
+
 #[repr(C)]
 pub struct nh_alias_atom {
     pub u: f64,
@@ -112,6 +128,8 @@ pub struct nh_job_params {
     pub use_graph: u32,
     pub inner_steps: u32,
     pub exec_fill: u32,
+    pub report_dispersal: i32,
+    pub event_buffer: u64,
 }
 
 #[repr(C)]
@@ -136,6 +154,7 @@ extern "C" {
     pub fn nh_job_add_lineages(job: *mut nh_job, l: *const nh_lineage, n: usize) -> c_int;
     pub fn nh_job_step_round(job: *mut nh_job, finished: *mut i32) -> c_int;
     pub fn nh_job_drain_events(job: *mut nh_job, buf: *mut nh_speciation_event, cap: usize, n_out: *mut usize) -> c_int;
+    pub fn nh_job_drain_packed_events(job: *mut nh_job, buf: *mut nh_packed_event, cap: usize, n_out: *mut usize) -> c_int;
     pub fn nh_job_outcome(job: *const nh_job, out: *mut nh_outcome) -> c_int;
     pub fn nh_job_take_lineages(job: *mut nh_job, buf: *mut nh_lineage, cap: usize, n_out: *mut usize) -> c_int;
     pub fn nh_job_destroy(job: *mut nh_job) -> c_int;

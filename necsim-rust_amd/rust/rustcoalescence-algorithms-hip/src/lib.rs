@@ -13,9 +13,9 @@ use std::{ffi::CStr, marker::PhantomData};
 
 use necsim_core::{
     cogs::{MathsCore, RngCore},
-    event::SpeciationEvent,
+    event::{PackedEvent, SpeciationEvent, TypedEvent},
     lineage::{GlobalLineageReference, Lineage},
-    reporter::Reporter,
+    reporter::{boolean::Boolean, Reporter},
 };
 use necsim_core_bond::{NonNegativeF64, PositiveF64};
 use necsim_impls_no_std::cogs::{
@@ -128,28 +128,49 @@ fn simulate<'p, M: MathsCore, O: Scenario<M, WyHash<M>> + HipScenario<M>, R: Rep
     check(unsafe { ffi::nh_init(args.device, &mut dev.ctx) })?;
     let (desc, _keep) = O::hip_scenario(scenario);
     check(unsafe { ffi::nh_tables_create(dev.ctx, &desc, &mut dev.tables) })?;
-    let params = args.job_params(
+    let mut params = args.job_params(
         wyhash_seed(&rng),
         O::speciation_probability(scenario),
         O::turnover_rate(scenario),
         resume_after,
         pause_before,
     );
+    // the reporter's ReportDispersal selects the reported event rate and
+    // stream (monolithic/mod.rs:139-164)
+    let report_dispersal = R::ReportDispersal::VALUE;
+    params.report_dispersal = i32::from(report_dispersal);
     check(unsafe { ffi::nh_job_create(dev.ctx, dev.tables, &params, &mut dev.job) })?;
     // Lineage is #[repr(C)] with the nh_lineage layout (lineage.rs:97-109)
     check(unsafe { ffi::nh_job_add_lineages(dev.job, lineages.as_ptr().cast(), lineages.len()) })?;
     let mut buf: Vec<ffi::nh_speciation_event> = Vec::new();
+    let mut packed: Vec<ffi::nh_packed_event> = Vec::new();
     loop {
         let mut finished = 0_i32;
         check(unsafe { ffi::nh_job_step_round(dev.job, &mut finished) })?;
         let mut n = 0_usize;
-        check(unsafe { ffi::nh_job_drain_events(dev.job, core::ptr::null_mut(), 0, &mut n) })?;
-        buf.resize(n, ffi::nh_speciation_event::default());
-        check(unsafe { ffi::nh_job_drain_events(dev.job, buf.as_mut_ptr(), n, &mut n) })?;
-        for e in &buf[..n] {
-            // SpeciationEvent is #[repr(C)] with the nh_speciation_event layout (event.rs:58-71)
-            let event: &SpeciationEvent = unsafe { &*(e as *const ffi::nh_speciation_event).cast() };
-            local_partition.get_reporter().report_speciation(event.into());
+        if report_dispersal {
+            // every event of the round, sorted; dispatched like the live
+            // water-level reporter (monolithic/reporter/live.rs:88-106)
+            check(unsafe { ffi::nh_job_drain_packed_events(dev.job, core::ptr::null_mut(), 0, &mut n) })?;
+            packed.resize(n, ffi::nh_packed_event::default());
+            check(unsafe { ffi::nh_job_drain_packed_events(dev.job, packed.as_mut_ptr(), n, &mut n) })?;
+            for e in &packed[..n] {
+                // PackedEvent is #[repr(C)] with the nh_packed_event layout (event.rs:16-29)
+                let event: PackedEvent = unsafe { core::ptr::read((e as *const ffi::nh_packed_event).cast()) };
+                match event.into() {
+                    TypedEvent::Speciation(event) => local_partition.get_reporter().report_speciation(&event.into()),
+                    TypedEvent::Dispersal(event) => local_partition.get_reporter().report_dispersal(&event.into()),
+                }
+            }
+        } else {
+            check(unsafe { ffi::nh_job_drain_events(dev.job, core::ptr::null_mut(), 0, &mut n) })?;
+            buf.resize(n, ffi::nh_speciation_event::default());
+            check(unsafe { ffi::nh_job_drain_events(dev.job, buf.as_mut_ptr(), n, &mut n) })?;
+            for e in &buf[..n] {
+                // SpeciationEvent is #[repr(C)] with the nh_speciation_event layout (event.rs:58-71)
+                let event: &SpeciationEvent = unsafe { &*(e as *const ffi::nh_speciation_event).cast() };
+                local_partition.get_reporter().report_speciation(event.into());
+            }
         }
         if finished != 0 {
             break;

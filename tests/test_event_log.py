@@ -126,3 +126,59 @@ def test_simulation_event_log_matches_reported_stream(tmp_path):
     want = np.ascontiguousarray(col.events, dtype=nh.EVENT_DTYPE)
     assert len(names) == -(-len(want) // 500)
     assert got.tobytes() == want.tobytes()
+
+
+PACKED_ROWS = [  # (ref, prior bits, event bits, origin, target)
+    (7, 0x8000000000000000, 0x3FF4000000000000, (3, 4, 0), (3, 4, 0)),       # speciation, prior -0.0, t 1.25
+    (2, 0x3FE0000000000000, 0xC000000000000000, (1, 1, 2), (5, 0, 1)),       # dispersal 0.5 -> -2.0
+    (9, 0x3FFC000000000000, 0xC000000000000000, (1, 1, 2), (5, 0, 3)),       # dispersal 1.75 -> -2.0
+]
+
+
+def packed(rows):
+    ev = np.zeros(len(rows), nh.PACKED_EVENT_DTYPE)
+    raw = ev.view(np.uint64).reshape(len(rows), 7)
+    for i, (ref, pb, tb, o, t) in enumerate(rows):
+        ev[i]["global_reference"], ev[i]["coalescence"] = ref, ref
+        raw[i, 1], raw[i, 2] = pb, tb
+        ev[i]["origin_x"], ev[i]["origin_y"], ev[i]["origin_index"] = o
+        ev[i]["target_x"], ev[i]["target_y"], ev[i]["target_index"] = t
+    return ev
+
+
+def test_packed_segment_bytes_follow_the_bincode_layout(tmp_path):
+    ev = packed(PACKED_ROWS)
+    path = str(tmp_path / "0")
+    _ffi.check(_ffi.lib().nh_event_log_write_packed_segment(path.encode(), _ffi.ptr(ev), len(ev), 1, 1))
+    raw = open(path, "rb").read()
+    want = HEADER.pack(1.25, 2.0, 3, True, True)  # header times are magnitudes (event.rs:33-36)
+    for ref, pb, tb, o, t in PACKED_ROWS:
+        want += struct.pack("<QQQIIIIIIQ", ref, pb, tb, *o, *t, ref)
+    assert raw == want
+    # the dispersal events tie on time and origin: the target orders them
+    bad = ev[[0, 2, 1]]
+    with pytest.raises(_ffi.NecsimHipError):
+        _ffi.check(_ffi.lib().nh_event_log_write_packed_segment(path.encode(), _ffi.ptr(bad), 3, 1, 1))
+
+
+def test_dispersal_reporter_segments(tmp_path):
+    ev = packed(PACKED_ROWS)
+    log = nh.EventLogReporter(str(tmp_path / "log"), segment_capacity=2, report_dispersal=True)
+    with pytest.raises(TypeError):
+        log.report_speciation(SAMPLE)
+    log.report_packed(ev[[1, 0]])  # one segment, sorted on write
+    log.report_packed(ev[2:])
+    log.close()
+    raw0 = open(tmp_path / "log" / "0", "rb").read()
+    raw1 = open(tmp_path / "log" / "1", "rb").read()
+    assert HEADER.unpack_from(raw0, 0) == (1.25, 2.0, 2, True, True)
+    assert raw0[26:] == ev[:2].tobytes() and raw1[26:] == ev[2:].tobytes()
+
+
+def test_packed_helpers():
+    ev = packed(PACKED_ROWS)
+    assert nh.is_speciation(ev).tolist() == [True, False, False]
+    sp = nh.speciation_events(ev)
+    assert len(sp) == 1 and sp[0]["prior_time"] == 0.0 and sp[0]["event_time"] == 1.25
+    assert np.signbit(sp["prior_time"]).tolist() == [False]
+    assert nh.packed_order(ev[[2, 1, 0]]).tolist() == [2, 1, 0]

@@ -1,0 +1,262 @@
+"""GPU parity of dispersal reporting (a reporter with ReportDispersal).
+
+Every step of the HIP path is reported as a 56-byte PackedEvent
+(necsim/core/src/event.rs:16-29): speciations with the sign bit of prior_time
+set, dispersals (LineageInteraction::Maybe) with the sign bit of event_time
+set, and the water level advances by event_slice / sum(lambda)
+(parallelisation/independent/monolithic/mod.rs:139-148).  The oracle's
+orc_simulate_packed is the checker.
+
+Parity bar: the sorted PackedEvent stream (all 56 bytes of every event), the
+step count, the outcome time and the remaining lineages are identical for
+DedupCache::None in both forms and with dedup on in the plain form; the
+persistent form with dedup on keeps the species and reports only events of
+the dedup-None stream.
+"""
+import numpy as np
+import pytest
+
+import necsim_hip as nh
+import oracle as O
+from necsim_hip import _ffi
+from parity_utils import map_scenarios
+
+pytestmark = pytest.mark.gpu
+PLAIN, PERSISTENT = nh.PLAIN, nh.PERSISTENT
+PK = nh.PACKED_EVENT_DTYPE
+
+
+def announce(msg):
+    print(f"[gpu case] {msg}", flush=True)
+
+
+def _dedup(dedup):
+    if dedup == "none":
+        return nh.DedupCache.none(), None
+    return nh.DedupCache.Relative(dedup), ("relative", dedup)
+
+
+def hip_packed(cogs, lineages, seed, form, dedup="none", pause_before=None, step_slice=10,
+               event_slice=None, use_graph=False, resume_after=None):
+    dc, _ = _dedup(dedup)
+    es = nh.EventSlice.Relative(2.0) if event_slice is None else event_slice
+    args = nh.IndependentArguments(step_slice=step_slice, dedup_cache=dc, event_slice=es, form=form,
+                                   use_graph=use_graph)
+    rep = nh.PackedEventCollector()
+    out = nh.HipIndependentAlgorithm.resume_and_simulate(args, nh.WyHash.seed_from_u64(seed), cogs, lineages,
+                                                         resume_after=resume_after, pause_before=pause_before,
+                                                         reporter=rep)
+    return out, rep.events
+
+
+def oracle_packed(orc, lineages, seed, nu, dedup="none", pause_before=None, step_slice=10):
+    _, d = _dedup(dedup)
+    p = O.make_params(seed, nu, step_slice=step_slice, dedup=d, pause_before=pause_before,
+                      report_dispersal=True)
+    return O.simulate_packed(orc, p, lineages)
+
+
+def packed_bytes(ev):
+    return np.ascontiguousarray(ev).view(np.uint8).tobytes()
+
+
+def assert_packed_exact(ref, out, ev, what):
+    assert PK.itemsize == O.PACKED_DTYPE.itemsize == 56
+    assert len(ev) == len(ref["events"]), f"{what}: {len(ev)} vs {len(ref['events'])} events"
+    if packed_bytes(ev) != packed_bytes(ref["events"]):
+        a = np.ascontiguousarray(ev).view(np.uint8).reshape(-1, 56)
+        b = np.ascontiguousarray(ref["events"]).view(np.uint8).reshape(-1, 56)
+        bad = np.nonzero(np.any(a != b, axis=1))[0]
+        raise AssertionError(f"{what}: PackedEvent streams differ at {bad[:5].tolist()}: "
+                             f"{ev[bad[:2]]} vs {ref['events'][bad[:2]]}")
+    assert out.steps == ref["steps"], f"{what}: steps {out.steps} vs {ref['steps']}"
+    assert np.float64(out.time).tobytes() == np.float64(ref["time"]).tobytes(), f"{what}: time"
+    assert out.paused == ref["paused"], what
+    if out.paused:
+        rem = np.sort(out.lineages, order=["global_reference"])
+        orem = np.sort(ref["remaining"], order=["global_reference"])
+        assert rem.tobytes() == orem.tobytes(), f"{what}: remaining lineages differ"
+
+
+def _species(packed):
+    return O.biodiversity(nh.speciation_events(packed))
+
+
+# ---------------------------------------------------------------------------
+# one small case per scenario kind: (name, hip cogs, oracle scenario, lineages, seed, nu, pause_before)
+def _non_spatial():
+    return ("non-spatial 30x30 deme 2", nh.NonSpatialScenario((30, 30), 2).new(1e-2),
+            O.Scenario.non_spatial(30, 30, 2), None, 5, 1e-2, None)
+
+
+def _almost_infinite():
+    orc = O.Scenario.almost_infinite_normal(3.0)
+    lin = orc.origins(1.0, circle=(2**31 - 1, 2**31 - 1, 20))
+    return ("almost-infinite sigma 3 r 20 pause 60", nh.AlmostInfiniteNormalDispersalScenario(20, 3.0).new(1e-3),
+            orc, lin, 7, 1e-3, 60.0)
+
+
+def _map(packed):
+    hip, orc = map_scenarios(24, 24, seed=11, packed=packed, sigma=1.5, cutoff=3.0)
+    return (f"24x24 map packed={packed}", hip.new(2e-2), orc, None, 3, 2e-2, None)
+
+
+CASES = {"non_spatial": _non_spatial, "almost_infinite": _almost_infinite,
+         "separable_map": lambda: _map(False), "packed_map": lambda: _map(True)}
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def case(request):
+    name, cogs, orc, lin, seed, nu, pause = CASES[request.param]()
+    if lin is None:
+        lin = orc.origins(1.0)
+    ref = oracle_packed(orc, lin, seed, nu, pause_before=pause)
+    assert len(ref["events"]) == ref["steps"] > 1000
+    return name, cogs, orc, lin, seed, nu, pause, ref
+
+
+@pytest.mark.parametrize("form", [PLAIN, PERSISTENT])
+def test_dedup_none_bit_exact(case, form):
+    name, cogs, orc, lin, seed, nu, pause, ref = case
+    announce(f"dispersal reporting {name} dedup None form {form}")
+    out, ev = hip_packed(cogs, lin, seed, form, pause_before=pause)
+    assert_packed_exact(ref, out, ev, f"{name} form {form}")
+
+
+def test_speciations_equal_the_speciation_only_run(case):
+    name, cogs, orc, lin, seed, nu, pause, ref = case
+    announce(f"dispersal reporting {name}: speciation subset vs speciation-only run")
+    args = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=PERSISTENT)
+    col = nh.EventCollector()
+    out = nh.HipIndependentAlgorithm.resume_and_simulate(args, nh.WyHash.seed_from_u64(seed), cogs, lin,
+                                                         pause_before=pause, reporter=col)
+    _, ev = hip_packed(cogs, lin, seed, PERSISTENT, pause_before=pause)
+    want = np.ascontiguousarray(col.events, dtype=nh.EVENT_DTYPE)
+    assert nh.speciation_events(ev).tobytes() == want.tobytes()
+    assert out.steps == len(ev)
+
+
+def test_dedup_plain_bit_exact_and_persistent_invariants(case):
+    name, cogs, orc, lin, seed, nu, pause, ref = case
+    announce(f"dispersal reporting {name} dedup Relative(1.0)")
+    refd = oracle_packed(orc, lin, seed, nu, dedup=1.0, pause_before=pause)
+    out, ev = hip_packed(cogs, lin, seed, PLAIN, dedup=1.0, pause_before=pause)
+    assert_packed_exact(refd, out, ev, f"{name} dedup plain")
+    out, ev = hip_packed(cogs, lin, seed, PERSISTENT, dedup=1.0, pause_before=pause)
+    assert _species(ev) == _species(refd["events"]) == _species(ref["events"])
+    full = set(np.ascontiguousarray(ref["events"]).view(np.uint8).reshape(-1, 56).view(f"V56").ravel().tolist())
+    mine = np.ascontiguousarray(ev).view(np.uint8).reshape(-1, 56).view("V56").ravel().tolist()
+    assert all(e in full for e in mine), f"{name}: an event outside the dedup-None stream"
+    assert out.steps <= ref["steps"]
+
+
+def test_graph_replays_match_eager():
+    announce("dispersal reporting non-spatial 30x30, rounds replayed from hipGraphs")
+    name, cogs, orc, _, seed, nu, pause = _non_spatial()
+    lin = orc.origins(1.0)
+    ref = oracle_packed(orc, lin, seed, nu)
+    few = nh.EventSlice.Relative(0.25)  # many rounds; the sorted stream does not depend on them
+    out, ev = hip_packed(cogs, lin, seed, PERSISTENT, event_slice=few, use_graph=True)
+    assert out.rounds >= 20
+    assert_packed_exact(ref, out, ev, "graph")
+
+
+@pytest.mark.parametrize("form", [PLAIN, PERSISTENT])
+def test_pause_resume_continuity(form):
+    announce(f"dispersal reporting pause at 150 and resume, form {form}")
+    name, cogs, orc, _, seed, nu, _p = _non_spatial()
+    lin = orc.origins(1.0)
+    ref = oracle_packed(orc, lin, seed, nu)
+    out1, ev1 = hip_packed(cogs, lin, seed, form, pause_before=150.0)
+    assert out1.paused and len(out1.lineages)
+    out2, ev2 = hip_packed(cogs, out1.lineages, seed, form, resume_after=150.0)
+    assert not out2.paused
+    both = np.concatenate([ev1, ev2])
+    assert packed_bytes(both) == packed_bytes(ref["events"])
+    assert out1.steps + out2.steps == ref["steps"]
+
+
+@pytest.mark.parametrize("form", [PLAIN, PERSISTENT])
+def test_event_buffer_overflow_is_an_error(ctx, form):
+    """a round that reports more events than the buffer holds fails with
+    NH_E_CAPACITY, writes nothing past the buffer and leaves the job failed"""
+    announce(f"dispersal event buffer of 64 entries, form {form}")
+    cogs = nh.NonSpatialScenario((30, 30), 2).new(1e-2)
+    lin = cogs.sample_habitat(nh.OriginPreSampler.all())
+    tables = nh.Tables(ctx, cogs)
+    job = nh.Job(ctx, tables, nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=form),
+                 nh.WyHash.seed_from_u64(5), 1e-2, 0.5, report_dispersal=True, event_buffer=64)
+    job.add_lineages(lin)
+    with pytest.raises(_ffi.NecsimHipError) as e:
+        job.step_round()
+    assert e.value.code == -7  # NH_E_CAPACITY
+    with pytest.raises(_ffi.NecsimHipError) as e:
+        job.step_round()
+    assert e.value.code == -4  # NH_E_STATE
+    job.close()
+
+
+def test_drain_calls_match_the_job_kind(ctx):
+    announce("drain calls on jobs of the other reporting kind")
+    cogs = nh.NonSpatialScenario((10, 10), 1).new(1e-1)
+    lin = cogs.sample_habitat(nh.OriginPreSampler.all())
+    tables = nh.Tables(ctx, cogs)
+    args = nh.IndependentArguments(dedup_cache=nh.DedupCache.none())
+    for disp in (False, True):
+        job = nh.Job(ctx, tables, args, nh.WyHash.seed_from_u64(1), 1e-1, 0.5, report_dispersal=disp)
+        job.add_lineages(lin)
+        job.step_round()
+        with pytest.raises(_ffi.NecsimHipError) as e:
+            job.drain_events() if disp else job.drain_packed_events()
+        assert e.value.code == -4
+        job.close()
+
+
+@pytest.mark.parametrize("device_sized", [0, 1])
+def test_device_sort_of_packed_events(ctx, device_sized):
+    """the PackedEvent order on the device against a lexicographic sort, on
+    records with many equal times, origins and targets and signed times"""
+    announce(f"device PackedEvent sort, device_sized={device_sized}")
+    rng = np.random.default_rng(17)
+    n = 200_000
+    ev = np.zeros(n, PK)
+    ev["global_reference"] = rng.permutation(n).astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    times = rng.uniform(1, 50, 3000)
+    t = rng.choice(times, n)
+    sign = rng.random(n) < 0.5
+    ev["event_time"] = np.where(sign, -t, t)
+    prior = t - rng.choice([0.0, 0.5, 1.0], n)
+    ev["prior_time"] = np.where(sign, prior, -prior)
+    for f, hi in (("origin_x", 4), ("origin_y", 3), ("origin_index", 2), ("target_x", 1 << 32),
+                  ("target_y", 3), ("target_index", 70000)):
+        ev[f] = rng.integers(0, hi, n, dtype=np.uint64).astype(np.uint32)
+    ev["coalescence"] = ev["global_reference"]
+    want = ev[nh.packed_order(ev)]
+    got = np.zeros(n, PK)
+    _ffi.check(_ffi.lib().nh_debug_sort_packed_events(ctx.handle, _ffi.ptr(ev), n, device_sized, _ffi.ptr(got)))
+    assert packed_bytes(got) == packed_bytes(want)
+
+
+def test_event_log_records_dispersal(tmp_path):
+    announce("dispersal-reporting run recorded to an event log")
+    import struct
+    name, cogs, orc, _, seed, nu, _p = _non_spatial()
+    lin = orc.origins(1.0)
+    col = nh.PackedEventCollector()
+    args = nh.IndependentArguments(dedup_cache=nh.DedupCache.none())
+    nh.HipIndependentAlgorithm.resume_and_simulate(args, nh.WyHash.seed_from_u64(seed), cogs, lin, reporter=col)
+    log = nh.EventLogReporter(str(tmp_path / "log"), segment_capacity=50_000, report_dispersal=True)
+    nh.HipIndependentAlgorithm.resume_and_simulate(args, nh.WyHash.seed_from_u64(seed), cogs, lin, reporter=log)
+    log.close()
+    import os
+    names = sorted(os.listdir(tmp_path / "log"), key=int)
+    assert len(names) == -(-len(col.events) // 50_000)
+    got = []
+    for nme in names:
+        raw = open(tmp_path / "log" / nme, "rb").read()
+        lo, hi, length, with_s, with_d = struct.unpack_from("<ddQ??", raw, 0)
+        assert with_s and with_d and len(raw) == 26 + 56 * length
+        seg = np.frombuffer(raw[26:], PK)
+        assert lo == abs(seg["event_time"][0]) and hi == abs(seg["event_time"][-1])
+        got.append(seg)
+    assert packed_bytes(np.concatenate(got)) == packed_bytes(col.events)

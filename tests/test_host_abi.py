@@ -35,8 +35,48 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_reference_repr_c():
     assert nh.LINEAGE_DTYPE.itemsize == 32    # necsim/core/src/lineage.rs:97-109
     assert nh.EVENT_DTYPE.itemsize == 40      # necsim/core/src/event.rs:58-71 (size assert :68-71)
+    assert nh.PACKED_EVENT_DTYPE.itemsize == 56  # necsim/core/src/event.rs:16-29 (size assert :38-44)
     assert _ffi.ATOM_DTYPE.itemsize == 24     # AliasMethodSamplerAtom<usize>
     assert _ffi.RANGE_DTYPE.itemsize == 16 and _ffi.SELF_DTYPE.itemsize == 16
+
+
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """sizes and field offsets of the ctypes mirrors against the C compiler's
+    view of include/necsim_hip.h"""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    structs = {"nh_job_params": _ffi.JobParams, "nh_outcome": _ffi.Outcome, "nh_scenario": _ffi.Scenario,
+               "nh_map_scenario": _ffi.MapScenario, "nh_server_params": _ffi.ServerParams,
+               "nh_request": _ffi.Request, "nh_request_result": _ffi.RequestResult}
+    dtypes = {"nh_lineage": nh.LINEAGE_DTYPE, "nh_speciation_event": nh.EVENT_DTYPE,
+              "nh_packed_event": nh.PACKED_EVENT_DTYPE}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "necsim_hip.h"', "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append(f'printf("{name} %zu\\n", sizeof({name}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{name}.{f} %zu\\n", offsetof({name}, {f}));')
+    for name, dt in dtypes.items():
+        lines.append(f'printf("{name} %zu\\n", sizeof({name}));')
+        for f in dt.names:
+            lines.append(f'printf("{name}.{f} %zu\\n", offsetof({name}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                         check=True).stdout.splitlines())
+    for name, cls in structs.items():
+        assert int(got[name]) == ctypes.sizeof(cls), name
+        for f, _ in cls._fields_:
+            assert int(got[f"{name}.{f}"]) == getattr(cls, f).offset, f"{name}.{f}"
+    for name, dt in dtypes.items():
+        assert int(got[name]) == dt.itemsize, name
+        for f in dt.names:
+            assert int(got[f"{name}.{f}"]) == dt.fields[f][1], f"{name}.{f}"
 
 
 def test_seed_from_u64_matches_oracle():

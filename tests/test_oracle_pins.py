@@ -392,6 +392,16 @@ def test_dispersal_events_are_maybe_encoded_and_chain_per_lineage(dispersal_runs
     assert len(by_ref) == len(lin)
 
 
+def test_packed_order_helper_matches_the_oracle_sort(dispersal_runs):
+    """nh.packed_order (the reference of the device PackedEvent sort tests)
+    reproduces the oracle's PackedEvent::cmp sort of a shuffled stream"""
+    import necsim_hip as nh
+    s, lin, full, spec = dispersal_runs
+    ev = np.ascontiguousarray(full["events"]).view(nh.PACKED_EVENT_DTYPE)
+    shuffled = ev[np.random.default_rng(0).permutation(len(ev))]
+    assert shuffled[nh.packed_order(shuffled)].tobytes() == ev.tobytes()
+
+
 def test_dispersal_stream_is_sorted_by_packed_event_order(dispersal_runs):
     s, lin, full, spec = dispersal_runs
     ev = full["events"]
