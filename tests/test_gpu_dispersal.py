@@ -79,7 +79,7 @@ def assert_packed_exact(ref, out, ev, what):
 
 
 def _species(packed):
-    return O.biodiversity(nh.speciation_events(packed))
+    return O.biodiversity(nh.speciation_events(np.ascontiguousarray(packed).view(PK)))
 
 
 # ---------------------------------------------------------------------------
