@@ -177,6 +177,9 @@ typedef struct {
                                       0 = speciation only (nh_job_drain_events) */
     uint64_t event_buffer;         /* report_dispersal: PackedEvents one round may report;
                                       0 = default 2 * event_slice + lineages + 65536 */
+    int32_t device_event_counter;  /* report_dispersal: the events stay on the device, where an
+                                      EventCounterReporter counts them (nh_job_event_counts);
+                                      nothing is left to drain */
 } nh_job_params;
 
 typedef struct {
@@ -188,6 +191,20 @@ typedef struct {
     uint64_t rounds;     /* water-level rounds run */
     uint64_t launches;   /* step-kernel launches */
 } nh_outcome;
+
+/* The counters of the EventCounterReporter
+ * (necsim/plugins/common/src/event_counter.rs:12-26, report rules :58-109),
+ * in field order. */
+typedef struct {
+    uint64_t raw_total;
+    uint64_t speciation;
+    uint64_t out_dispersal;
+    uint64_t self_dispersal;
+    uint64_t out_coalescence;
+    uint64_t self_coalescence;
+    uint64_t late_dispersal_coalescence;
+    uint64_t late_coalescence;
+} nh_event_counts;
 
 typedef struct nh_ctx nh_ctx;
 typedef struct nh_tables nh_tables;
@@ -275,6 +292,11 @@ int nh_job_drain_events(nh_job *job, nh_speciation_event *buf, size_t cap, size_
  * NH_E_CAPACITY and leaves the job failed (later calls: NH_E_STATE).  Either drain call on a job of the other kind
  * fails with NH_E_STATE. */
 int nh_job_drain_packed_events(nh_job *job, nh_packed_event *buf, size_t cap, size_t *n_out);
+/* A device_event_counter job: the EventCounterReporter's counters over every
+ * event reported so far, counted on the device from each round's sorted
+ * events (the reporter's report_speciation / report_dispersal,
+ * event_counter.rs:58-109).  Other jobs fail with NH_E_STATE. */
+int nh_job_event_counts(nh_job *job, nh_event_counts *out);
 int nh_job_outcome(const nh_job *job, nh_outcome *out);
 /* Lineages remaining after a paused run (SimulationOutcome::Paused.lineages) */
 int nh_job_take_lineages(nh_job *job, nh_lineage *buf, size_t cap, size_t *n_out);
@@ -429,6 +451,9 @@ int nh_debug_sort_events(nh_ctx *ctx, const nh_speciation_event *in, size_t n, i
 /* The same sort over PackedEvents (dispersal reporting's records). */
 int nh_debug_sort_packed_events(nh_ctx *ctx, const nh_packed_event *in, size_t n, int32_t device_sized,
                                 nh_packed_event *out);
+/* The device EventCounterReporter of device_event_counter jobs over n
+ * PackedEvents in PackedEvent order, from a fresh reporter. */
+int nh_debug_count_packed_events(nh_ctx *ctx, const nh_packed_event *in, size_t n, nh_event_counts *out);
 
 #ifdef __cplusplus
 }

@@ -1,5 +1,7 @@
 // nh_debug.hip -- diagnostics: elementwise evaluation of device primitives
 // and a plain copy kernel of known size (the HBM byte-counter calibration).
+#include <cstring>
+
 #include "../../include/necsim_hip.h"
 #include "nh_internal.h"
 #include "nh_kernels.h"
@@ -163,4 +165,34 @@ extern "C" int nh_debug_sort_packed_events(nh_ctx *ctx, const nh_packed_event *i
     static_assert(sizeof(nh_packed_event) == sizeof(DevPacked), "packed event record");
     return debug_sort(ctx, reinterpret_cast<const DevPacked *>(in), n, device_sized,
                       reinterpret_cast<DevPacked *>(out), "nh_debug_sort_packed_events");
+}
+
+// Counts n host PackedEvents (in PackedEvent order) with the device
+// EventCounterReporter of device_event_counter jobs, from a fresh reporter.
+extern "C" int nh_debug_count_packed_events(nh_ctx *ctx, const nh_packed_event *in, size_t n, nh_event_counts *out) {
+    if (!ctx || !out || (n && !in)) return set_error(NH_E_INVALID, "nh_debug_count_packed_events: bad argument");
+    std::memset(out, 0, sizeof(*out));
+    if (n == 0) return NH_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_error(NH_E_HIP, "nh_debug_count_packed_events: hipSetDevice");
+    hipStream_t st = ctx->stream;
+    DevPacked *d_in = nullptr;
+    EventCounts *d_c = nullptr;
+    int rc = NH_OK;
+    if ((rc = dalloc(&d_in, n)) || (rc = dalloc(&d_c, 1))) {
+    } else {
+        EventCounts c{};
+        hipError_t e = hipMemcpyAsync(d_in, in, n * sizeof(DevPacked), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_c, &c, sizeof(c), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) {
+            launch_count_events(d_in, nullptr, n, d_c, st);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(&c, d_c, sizeof(c), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = set_error(NH_E_HIP, "nh_debug_count_packed_events: %s", hipGetErrorString(e));
+        else std::memcpy(out, c.count, sizeof(*out));
+    }
+    dfree(d_in);
+    dfree(d_c);
+    return rc;
 }

@@ -171,6 +171,24 @@ uint32_t soa_compact_blocks(uint64_t n);
 void launch_soa_compact(const LineageSoA &in, const LineageSoA &out, uint64_t n, uint32_t *blk,
                         hipStream_t stream);
 
+// EventCounterReporter (necsim/plugins/common/src/event_counter.rs:12-26)
+// kept on the device: its eight counters, and the prior time of the last
+// counted event (the reporter's last_parent_prior_time) carried from one
+// water-level round to the next.
+enum {
+    EC_RAW_TOTAL, EC_SPECIATION, EC_OUT_DISPERSAL, EC_SELF_DISPERSAL, EC_OUT_COALESCENCE,
+    EC_SELF_COALESCENCE, EC_LATE_DISPERSAL_COALESCENCE, EC_LATE_COALESCENCE, EC_COUNTERS
+};
+struct EventCounts {
+    unsigned long long count[EC_COUNTERS];
+    unsigned long long carry_has;   // an event was counted before this round
+    unsigned long long carry_prior; // its prior time (magnitude bits)
+};
+// Counts one round's sorted PackedEvents ev[0, n): with `end`, n is read from
+// *end when the kernels run (clamped to `cap`; capturable in a graph).
+void launch_count_events(const DevPacked *ev, const unsigned long long *end, uint64_t n_or_cap, EventCounts *c,
+                         hipStream_t stream);
+
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream);
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream);
 // reset_events: also the event counter and its overflow flag (dispersal

@@ -1248,6 +1248,92 @@ template void sort_events_device_sized<DevPacked>(const DevPacked *, const unsig
                                                   const unsigned long long *, uint64_t, DevPacked *, SortBuffers &,
                                                   hipStream_t);
 
+// ---------------------------------------------------------------------------
+// EventCounterReporter::report_{speciation,dispersal} (necsim/plugins/common/
+// src/event_counter.rs:58-109) over one round's events in PackedEvent order.
+// The reporter compares each event with the last one of its own kind
+// (SpeciationEvent: equal origin and event time; DispersalEvent: equal
+// origin, event time and target, event.rs:301-320) and with the prior time of
+// the event just before it.  In sorted order, events with equal origin and
+// time are contiguous, so the last event of the same kind is equal exactly
+// when one lies in the contiguous run of equal keys that ends at this event:
+// each event scans back through that run (one step unless coalesced copies
+// share the key).  Equal times never straddle two rounds (all events below
+// the water level are in one round), so only the last prior time is carried.
+__device__ __forceinline__ bool packed_is_speciation(const DevPacked &e) {
+    return (bits(e.prior_time) >> 63) != 0 && (bits(e.event_time) >> 63) == 0; // p(-), e(+)
+}
+
+__device__ __forceinline__ bool same_event_key(const DevPacked &a, const DevPacked &b, bool speciation) {
+    return (bits(a.event_time) & ABS_BITS) == (bits(b.event_time) & ABS_BITS) && a.ox == b.ox && a.oy == b.oy &&
+           a.oindex == b.oindex && (speciation || (a.tx == b.tx && a.ty == b.ty && a.tindex == b.tindex));
+}
+
+__global__ void __launch_bounds__(256) k_count_events(const DevPacked *ev, SortSize sz, EventCounts *c) {
+    __shared__ unsigned long long s_cnt[EC_COUNTERS];
+    if (threadIdx.x < EC_COUNTERS) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t n = sz.get();
+    int which = -1; // the counter this event adds to, besides the raw total
+    if (i < n) {
+        const DevPacked e = ev[i];
+        const bool sp = packed_is_speciation(e);
+        const bool has_prev = i > 0 || c->carry_has != 0;
+        const uint64_t prev_prior = i > 0 ? (bits(ev[i - 1].prior_time) & ABS_BITS) : c->carry_prior;
+        bool dup = false;
+        for (uint64_t j = i; j-- > 0;) {
+            const DevPacked p = ev[j];
+            if (!same_event_key(p, e, sp)) break;
+            if (packed_is_speciation(p) == sp) {
+                dup = true;
+                break;
+            }
+        }
+        if (dup) {
+            if (!has_prev || prev_prior != (bits(e.prior_time) & ABS_BITS)) which = EC_LATE_COALESCENCE;
+        } else if (sp) {
+            which = EC_SPECIATION;
+        } else {
+            // LineageInteraction from the signs (event.rs:186-229)
+            const bool prior_pos = (bits(e.prior_time) >> 63) == 0, event_pos = (bits(e.event_time) >> 63) == 0;
+            if (prior_pos && !event_pos) {
+                which = EC_LATE_DISPERSAL_COALESCENCE; // Maybe
+            } else {
+                const bool self = e.ox == e.tx && e.oy == e.ty && e.oindex == e.tindex;
+                const bool coal = prior_pos && event_pos;
+                which = self ? (coal ? EC_SELF_COALESCENCE : EC_SELF_DISPERSAL)
+                             : (coal ? EC_OUT_COALESCENCE : EC_OUT_DISPERSAL);
+            }
+        }
+    }
+    const uint64_t valid = __ballot(i < n);
+    if (lane_id() == 0 && valid) atomicAdd(&s_cnt[EC_RAW_TOTAL], (unsigned long long)__popcll(valid));
+    for (int k = EC_SPECIATION; k < EC_COUNTERS; k++) {
+        const uint64_t m = __ballot(which == k);
+        if (lane_id() == 0 && m) atomicAdd(&s_cnt[k], (unsigned long long)__popcll(m));
+    }
+    __syncthreads();
+    if (threadIdx.x < EC_COUNTERS && s_cnt[threadIdx.x]) atomicAdd(&c->count[threadIdx.x], s_cnt[threadIdx.x]);
+}
+
+// after the round's count: the last prior time for the next round
+__global__ void k_count_carry(const DevPacked *ev, SortSize sz, EventCounts *c) {
+    const uint64_t n = sz.get();
+    if (n) {
+        c->carry_has = 1;
+        c->carry_prior = bits(ev[n - 1].prior_time) & ABS_BITS;
+    }
+}
+
+void launch_count_events(const DevPacked *ev, const unsigned long long *end, uint64_t n_or_cap, EventCounts *c,
+                         hipStream_t stream) {
+    if (n_or_cap == 0) return;
+    SortSize sz{n_or_cap, end, nullptr, nullptr};
+    hipLaunchKernelGGL(k_count_events, dim3(grid_for(n_or_cap, 256)), dim3(256), 0, stream, ev, sz, c);
+    hipLaunchKernelGGL(k_count_carry, dim3(1), dim3(1), 0, stream, ev, sz, c);
+}
+
 uint32_t soa_compact_blocks(uint64_t n) { return (uint32_t)((n + SC_TILE - 1) / SC_TILE); }
 
 void launch_soa_compact(const LineageSoA &in, const LineageSoA &out, uint64_t n, uint32_t *blk,

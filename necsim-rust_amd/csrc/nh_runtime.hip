@@ -324,6 +324,7 @@ struct nh_job {
     uint64_t packed_cap = 0;
     std::vector<nh_packed_event> ready_packed;
     size_t packed_pos = 0;
+    EventCounts *d_counts = nullptr; // device_event_counter jobs
     SortBuffers sb{};
     uint64_t *d_cache = nullptr;
 
@@ -349,7 +350,7 @@ struct nh_job {
     ~nh_job() {
         for (auto &l : lin) l.release();
         dfree(d_ratio); dfree(d_ctr); dfree(d_round); dfree(d_events); dfree(d_sorted);
-        dfree(d_packed); dfree(d_packed_sorted);
+        dfree(d_packed); dfree(d_packed_sorted); dfree(d_counts);
         dfree(sb.keys[0]); dfree(sb.keys[1]); dfree(sb.vals[0]); dfree(sb.vals[1]);
         dfree(sb.hist); dfree(sb.minmax); dfree(d_cache);
         dfree(d_S); dfree(d_F); dfree(d_pass[0]); dfree(d_pass[1]); dfree(d_status);
@@ -372,6 +373,8 @@ extern "C" int nh_job_create(nh_ctx *ctx, const nh_tables *tables, const nh_job_
     if (!(p->delta_t > 0.0) || std::isinf(p->delta_t)) return set_error(NH_E_INVALID, "delta_t must be positive");
     if (p->step_slice == 0) return set_error(NH_E_INVALID, "step_slice must be positive");
     if (p->exec_fill > 8) return set_error(NH_E_INVALID, "exec_fill is a number of eighths (0..8)");
+    if (p->device_event_counter && !p->report_dispersal)
+        return set_error(NH_E_INVALID, "device_event_counter counts the events of a report_dispersal job");
     if (p->form != NH_FORM_PLAIN && p->form != NH_FORM_PERSISTENT) return set_error(NH_E_INVALID, "unknown form");
     if (p->dedup_mode < NH_DEDUP_NONE || p->dedup_mode > NH_DEDUP_ABSOLUTE) return set_error(NH_E_INVALID, "dedup mode");
     if (p->dedup_mode == NH_DEDUP_RELATIVE && !(p->dedup_factor > 0.0)) return set_error(NH_E_INVALID, "dedup factor");
@@ -464,6 +467,10 @@ static int job_start(nh_job *j) {
         NH_TRY(dalloc(&j->d_packed, j->packed_cap));
         NH_TRY(dalloc(&j->d_packed_sorted, j->packed_cap));
         sort_cap = j->packed_cap;
+        if (p.device_event_counter) {
+            NH_TRY(dalloc(&j->d_counts, 1));
+            NH_HIP(hipMemsetAsync(j->d_counts, 0, sizeof(EventCounts), st));
+        }
     } else {
         NH_TRY(dalloc(&j->d_events, n));
         NH_TRY(dalloc(&j->d_sorted, n));
@@ -722,9 +729,12 @@ static int persistent_round(nh_job *j, bool *sorted) {
             NH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             launch_reset_counters(j->d_ctr, j->disp, st);
             launch_persistent(j->tables->kind, persist_args(j, g), j->blocks, j->threads, st);
-            if (j->disp)
+            if (j->disp) {
                 sort_events_device_sized(j->d_packed, &j->d_ctr->n_events, nullptr, j->packed_cap,
                                          j->d_packed_sorted, j->sb, st);
+                if (j->d_counts)
+                    launch_count_events(j->d_packed_sorted, &j->d_ctr->n_events, j->packed_cap, j->d_counts, st);
+            }
             else
                 sort_events_device_sized(j->d_events, &j->d_ctr->n_events, &j->d_round->ev_start, j->n_initial,
                                          j->d_sorted, j->sb, st);
@@ -814,7 +824,13 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
     j->rounds++;
 
     // the live water-level reporter sorts each round's events (live.rs:88-115)
-    if (ne && j->disp) {
+    if (ne && j->disp && j->d_counts) {
+        if (!sorted) {
+            sort_events(j->d_packed, ne, j->d_packed_sorted, j->sb, st, false);
+            launch_count_events(j->d_packed_sorted, nullptr, ne, j->d_counts, st);
+        }
+        NH_HIP(hipGetLastError());
+    } else if (ne && j->disp) {
         if (!sorted) sort_events(j->d_packed, ne, j->d_packed_sorted, j->sb, st, false);
         NH_HIP(hipGetLastError());
         size_t old = j->ready_packed.size();
@@ -862,6 +878,23 @@ extern "C" int nh_job_drain_packed_events(nh_job *j, nh_packed_event *buf, size_
     if (!j || !n_out) return set_error(NH_E_INVALID, "null argument");
     if (!j->disp) return set_error(NH_E_STATE, "the job reports speciation events only (nh_job_drain_events)");
     drain_ready(j->ready_packed, j->packed_pos, buf, cap, n_out);
+    return NH_OK;
+}
+
+extern "C" int nh_job_event_counts(nh_job *j, nh_event_counts *out) {
+    static_assert(sizeof(nh_event_counts) == sizeof(unsigned long long) * EC_COUNTERS, "event counters");
+    if (!j || !out) return set_error(NH_E_INVALID, "null argument");
+    if (!j->disp || !j->p.device_event_counter)
+        return set_error(NH_E_STATE, "the job does not count its events on the device");
+    if (!j->d_counts) { // not started: nothing counted yet
+        std::memset(out, 0, sizeof(*out));
+        return NH_OK;
+    }
+    NH_HIP(hipSetDevice(j->ctx->device));
+    EventCounts c;
+    NH_HIP(hipMemcpyAsync(&c, j->d_counts, sizeof(c), hipMemcpyDeviceToHost, j->ctx->stream));
+    NH_HIP(hipStreamSynchronize(j->ctx->stream));
+    std::memcpy(out, c.count, sizeof(*out));
     return NH_OK;
 }
 

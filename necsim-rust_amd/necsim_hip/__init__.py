@@ -16,5 +16,6 @@ from .algorithm import (PERSISTENT, PLAIN, AlmostInfiniteNormalDispersalScenario
                         autotune, build_alias_tables, copy_calibrate, is_speciation, load_lineages,
                         packed_order, save_lineages, speciation_events)
 from .maps import generate_map
+from .reporters import CsvReporter, EventCounterReporter, rust_f64_display
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -65,7 +65,13 @@ class JobParams(ctypes.Structure):
                 ("has_pause_before", c_i32), ("pause_before", c_f64), ("resume_after", c_f64),
                 ("form", c_i32), ("block_size", c_u32), ("blocks_per_cu", c_u32),
                 ("use_graph", c_u32), ("inner_steps", c_u32), ("exec_fill", c_u32),
-                ("report_dispersal", c_i32), ("event_buffer", c_u64)]
+                ("report_dispersal", c_i32), ("event_buffer", c_u64), ("device_event_counter", c_i32)]
+
+
+class EventCounts(ctypes.Structure):
+    _fields_ = [("raw_total", c_u64), ("speciation", c_u64), ("out_dispersal", c_u64),
+                ("self_dispersal", c_u64), ("out_coalescence", c_u64), ("self_coalescence", c_u64),
+                ("late_dispersal_coalescence", c_u64), ("late_coalescence", c_u64)]
 
 
 class Outcome(ctypes.Structure):
@@ -113,6 +119,7 @@ SIGNATURES = {
     "nh_job_simulate": (ctypes.c_int, [c_vp]),
     "nh_job_drain_events": (ctypes.c_int, [c_vp, c_vp, c_sz, ctypes.POINTER(c_sz)]),
     "nh_job_drain_packed_events": (ctypes.c_int, [c_vp, c_vp, c_sz, ctypes.POINTER(c_sz)]),
+    "nh_job_event_counts": (ctypes.c_int, [c_vp, ctypes.POINTER(EventCounts)]),
     "nh_job_outcome": (ctypes.c_int, [c_vp, ctypes.POINTER(Outcome)]),
     "nh_job_take_lineages": (ctypes.c_int, [c_vp, c_vp, c_sz, ctypes.POINTER(c_sz)]),
     "nh_job_destroy": (ctypes.c_int, [c_vp]),
@@ -137,6 +144,7 @@ SIGNATURES = {
     "nh_copy_calibrate": (ctypes.c_int, [c_vp, c_sz, c_i32, ctypes.POINTER(ctypes.c_double)]),
     "nh_debug_sort_events": (ctypes.c_int, [c_vp, c_vp, c_sz, c_i32, c_vp]),
     "nh_debug_sort_packed_events": (ctypes.c_int, [c_vp, c_vp, c_sz, c_i32, c_vp]),
+    "nh_debug_count_packed_events": (ctypes.c_int, [c_vp, c_vp, c_sz, ctypes.POINTER(EventCounts)]),
     "nh_sample_origins": (ctypes.c_int, [ctypes.POINTER(Scenario), c_f64, c_u32, c_u32, c_u32, c_u32,
                                          c_u32, c_vp, c_sz, ctypes.POINTER(c_sz)]),
 }

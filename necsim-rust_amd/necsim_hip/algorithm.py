@@ -296,9 +296,11 @@ def _magnitude_bits(f64):
 
 
 def is_speciation(packed):
-    """PackedEvent -> SpeciationEvent classification: the sign bit of
-    prior_time (necsim/core/src/event.rs:103-115, 191-204)"""
-    return (np.ascontiguousarray(packed["prior_time"]).view(np.uint64) & SIGN_BIT) != 0
+    """PackedEvent -> TypedEvent::Speciation: prior_time negative and
+    event_time positive (necsim/core/src/event.rs:103-115, 186-229)"""
+    prior = np.ascontiguousarray(packed["prior_time"]).view(np.uint64)
+    event = np.ascontiguousarray(packed["event_time"]).view(np.uint64)
+    return ((prior & SIGN_BIT) != 0) & ((event & SIGN_BIT) == 0)
 
 
 def speciation_events(packed):
@@ -478,10 +480,12 @@ class Job:
     """One simulation on the device (nh_job_*)."""
 
     def __init__(self, ctx, tables, args: IndependentArguments, rng: WyHash, speciation_probability,
-                 turnover_rate, pause_before=None, resume_after=0.0, report_dispersal=False, event_buffer=0):
+                 turnover_rate, pause_before=None, resume_after=0.0, report_dispersal=False, event_buffer=0,
+                 device_event_counter=False):
         p = _job_params(args, rng, speciation_probability, turnover_rate, pause_before, resume_after)
         p.report_dispersal = 1 if report_dispersal else 0
         p.event_buffer = int(event_buffer)
+        p.device_event_counter = 1 if device_event_counter else 0
         h = ctypes.c_void_p()
         check(lib().nh_job_create(ctx.handle, tables.handle, ctypes.byref(p), ctypes.byref(h)))
         self.handle = h
@@ -515,6 +519,12 @@ class Job:
         if n.value:
             check(lib().nh_job_drain_packed_events(self.handle, ptr(buf), n.value, ctypes.byref(n)))
         return buf[:n.value]
+
+    def event_counts(self):
+        """the device EventCounterReporter's counters (device_event_counter jobs)"""
+        c = _ffi.EventCounts()
+        check(lib().nh_job_event_counts(self.handle, ctypes.byref(c)))
+        return {f: int(getattr(c, f)) for f, _ in _ffi.EventCounts._fields_}
 
     def outcome(self):
         o = _ffi.Outcome()
@@ -593,22 +603,26 @@ class HipIndependentAlgorithm:
 
     A reporter whose REPORT_DISPERSAL is true (the reference's
     Reporter::ReportDispersal) receives every event of each water-level round,
-    speciations and dispersals, as sorted PackedEvents through report_packed;
-    any other reporter receives the speciation events through
-    report_speciation."""
+    speciations and dispersals, as sorted PackedEvents through report_packed
+    (or, with DEVICE_EVENT_COUNTER, counts them on the device and hands the
+    counters over once through add_counts); any other reporter receives the
+    speciation events through report_speciation."""
 
     @staticmethod
     def _run(args, rng, cogs, lineages, pause_before, resume_after, reporter):
         ctx = Context.get(args.device)
         tables = Tables(ctx, cogs)
         disp = bool(getattr(reporter, "REPORT_DISPERSAL", False))
+        on_device = disp and bool(getattr(reporter, "DEVICE_EVENT_COUNTER", False))
         job = Job(ctx, tables, args, rng, cogs.speciation_probability, cogs.turnover_rate,
-                  pause_before, resume_after, report_dispersal=disp)
+                  pause_before, resume_after, report_dispersal=disp, device_event_counter=on_device)
         try:
             job.add_lineages(lineages)
             finished = False
             while not finished:
                 finished = job.step_round()
+                if on_device:
+                    continue
                 if disp:
                     ev = job.drain_packed_events()
                     if len(ev):
@@ -617,6 +631,8 @@ class HipIndependentAlgorithm:
                     ev = job.drain_events()
                     if len(ev) and reporter is not None:
                         reporter.report_speciation(ev)
+            if on_device:
+                reporter.add_counts(job.event_counts())
             o = job.outcome()
             ms, _ = job.kernel_time()
             rem = job.take_lineages() if o.paused else None

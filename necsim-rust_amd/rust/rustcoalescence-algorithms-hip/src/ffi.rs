@@ -130,6 +130,7 @@ pub struct nh_job_params {
     pub exec_fill: u32,
     pub report_dispersal: i32,
     pub event_buffer: u64,
+    pub device_event_counter: i32,
 }
 
 #[repr(C)]

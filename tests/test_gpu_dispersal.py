@@ -13,6 +13,8 @@ DedupCache::None in both forms and with dedup on in the plain form; the
 persistent form with dedup on keeps the species and reports only events of
 the dedup-None stream.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -260,3 +262,44 @@ def test_event_log_records_dispersal(tmp_path):
         assert lo == abs(seg["event_time"][0]) and hi == abs(seg["event_time"][-1])
         got.append(seg)
     assert packed_bytes(np.concatenate(got)) == packed_bytes(col.events)
+
+
+# ---------------------------------------------------------------------------
+# EventCounterReporter on the device
+def _device_counts(ctx, ev):
+    ev = np.ascontiguousarray(ev, dtype=PK)
+    c = _ffi.EventCounts()
+    _ffi.check(_ffi.lib().nh_debug_count_packed_events(ctx.handle, _ffi.ptr(ev), len(ev), ctypes.byref(c)))
+    return {f: int(getattr(c, f)) for f, _ in _ffi.EventCounts._fields_}
+
+
+def test_device_counter_on_hand_and_coalescing_streams(ctx):
+    """the device counter against the sequential restatement of
+    event_counter.rs:58-109 (tests/test_reporters.py)"""
+    announce("device EventCounterReporter on the hand-made and a coalescing stream")
+    from test_reporters import hand_stream, reference_counts
+    ev = hand_stream()
+    assert _device_counts(ctx, ev) == reference_counts([ev])
+    s = O.Scenario.non_spatial(6, 6, 2)
+    full = O.simulate_packed(s, O.make_params(11, 0.02, dedup=None, report_dispersal=True), s.origins(1.0))
+    ev = np.ascontiguousarray(full["events"]).view(PK)
+    assert _device_counts(ctx, ev) == reference_counts([ev])
+
+
+@pytest.mark.parametrize("form,graph", [(PLAIN, False), (PERSISTENT, False), (PERSISTENT, True)])
+def test_device_counter_job_matches_host_counter(form, graph):
+    announce(f"device_event_counter job vs host EventCounterReporter, form {form} graph {graph}")
+    cogs = nh.NonSpatialScenario((12, 12), 2).new(5e-3)
+    lin = cogs.sample_habitat(nh.OriginPreSampler.all())
+    args = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=form, use_graph=graph,
+                                   event_slice=nh.EventSlice.Relative(0.5))
+    host, dev = nh.EventCounterReporter(), nh.EventCounterReporter(on_device=True)
+    col = nh.PackedEventCollector()
+    outs = [nh.HipIndependentAlgorithm.resume_and_simulate(args, nh.WyHash.seed_from_u64(2), cogs, lin, reporter=r)
+            for r in (host, dev, col)]
+    assert outs[0].steps == outs[1].steps == outs[2].steps
+    assert outs[1].rounds > 5
+    assert dev.counts == host.counts
+    from test_reporters import reference_counts
+    assert dev.counts == reference_counts([col.events])
+    assert dev.counts["late_coalescence"] > 0 and dev.counts["raw_total"] == outs[1].steps
