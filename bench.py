@@ -227,6 +227,56 @@ def serving_run(ctx, n_requests=150, rate=50.0, cap=1000.0, tick_steps=250, seed
             "mean_ms": float(lat.mean()), "max_ms": float(lat.max())}
 
 
+PACKED_EVENT_BYTES = 56
+
+
+def dispersal_run(ctx, tables, cogs, lineages, args, window, drain_window, seed=42):
+    """Dispersal reporting (a reporter with ReportDispersal) on the headline
+    workload: every step is a 56-byte PackedEvent, the water level advances by
+    event_slice / sum(lambda), and each round's events are sorted on the
+    device.  Three ways to consume them: the speciation-only run as the
+    yardstick, the device EventCounterReporter (events stay in HBM; each
+    round is one hipGraph of step kernel, sort and count), and draining every
+    sorted event to the host (bounded by the device-to-host copy)."""
+    import dataclasses
+    gargs = dataclasses.replace(args, use_graph=True)
+    out = {"window": window, "host_drain_window": drain_window}
+    modes = (("speciation_only", window, {}),
+             ("device_counter", window, dict(report_dispersal=True, device_event_counter=True)),
+             ("host_drain", drain_window, dict(report_dispersal=True)))
+    for name, win, kw in modes:
+        job = nh.Job(ctx, tables, gargs, nh.WyHash.seed_from_u64(seed), cogs.speciation_probability,
+                     cogs.turnover_rate, pause_before=win, **kw)
+        job.add_lineages(lineages)
+        job.start()
+        t0 = time.perf_counter()
+        fin, drained = False, 0
+        while not fin:
+            fin = job.step_round()
+            if name == "host_drain":
+                drained += len(job.drain_packed_events())
+            elif name == "speciation_only":
+                job.drain_events()
+        wall = time.perf_counter() - t0
+        o = job.outcome()
+        ms, k = job.kernel_time()
+        r = {"steps": int(o.steps), "events": int(o.n_events), "rounds": int(o.rounds), "wall_ms": wall * 1e3,
+             "device_ms": ms, "steps_per_s": o.steps / wall, "device_steps_per_s": o.steps / (ms * 1e-3)}
+        if name == "device_counter":
+            r["counts"] = job.event_counts()
+            r["counts_raw_total_equals_steps"] = r["counts"]["raw_total"] == o.steps
+        if name == "host_drain":
+            r["drained_events"] = drained
+            r["drained_GB_per_s"] = drained * PACKED_EVENT_BYTES / wall / 1e9
+        job.close()
+        out[name] = r
+        log(f"dispersal {name}: {r}")
+    sp, dc = out["speciation_only"], out["device_counter"]
+    out["device_counter_species_equal_speciation_only"] = dc["counts"]["speciation"] == sp["events"]
+    out["device_counter_over_speciation_only"] = dc["steps_per_s"] / sp["steps_per_s"]
+    return out
+
+
 def run_collect(job):
     fin = False
     chunks = []
@@ -295,8 +345,13 @@ def main():
     ap.add_argument("--window", type=float, default=2000.0, help="pause_before of the headline workload")
     ap.add_argument("--dedup-window", type=float, default=20000.0,
                     help="pause_before of the dedup-on config 4 comparison")
-    ap.add_argument("--part", choices=("all", "headline", "gather"), default="all",
-                    help="headline: the headline runs only; gather: the alias-gather window only")
+    ap.add_argument("--part", choices=("all", "headline", "gather", "dispersal"), default="all",
+                    help="headline: the headline runs only; gather: the alias-gather window only; "
+                         "dispersal: the dispersal-reporting windows only")
+    ap.add_argument("--dispersal-window", type=float, default=200.0,
+                    help="pause_before of the dispersal-reporting runs on the device")
+    ap.add_argument("--drain-window", type=float, default=20.0,
+                    help="pause_before of the dispersal-reporting run drained to the host")
     ap.add_argument("--no-extra", action="store_true", help="same as --part headline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape unless given below")
@@ -337,6 +392,16 @@ def main():
         r = gather_run(ctx, 2000.0)
         log(f"gather: {r}")
         print(json.dumps({"part": "gather", "gather_window": r, "copy_kernel": copy}), flush=True)
+        return
+    if a.part == "dispersal":
+        cogs = nh.AlmostInfiniteNormalDispersalScenario(2000, 50.0).new(1e-6)
+        lineages = cogs.sample_habitat(nh.OriginPreSampler.all())
+        tables = nh.Tables(ctx, cogs)
+        dargs = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), block_size=a.block_size,
+                                        blocks_per_cu=a.blocks_per_cu, inner_steps=a.inner_steps,
+                                        exec_fill=a.exec_fill)
+        r = dispersal_run(ctx, tables, cogs, lineages, dargs, a.dispersal_window, a.drain_window)
+        print(json.dumps({"part": "dispersal", "dispersal_windows": r, "copy_kernel": copy}), flush=True)
         return
 
     # ---- headline: config 4, dedup None window ----
@@ -470,6 +535,8 @@ def main():
         dd["persistent_over_plain_wall"] = dd["plain"]["ms"] / dd["persistent"]["ms"]
         extra["config4_dedup_window"] = {"pause_before": a.dedup_window, **dd}
         log(f"config 4 dedup window: {extra['config4_dedup_window']}")
+        extra["dispersal_reporting"] = dispersal_run(ctx, tables, cogs, lineages, args, a.dispersal_window,
+                                                     a.drain_window)
         del tables
         extra["config2_full_run"] = config2_run(ctx)
         log(f"config 2: {extra['config2_full_run']}")

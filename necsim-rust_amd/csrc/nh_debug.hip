@@ -124,7 +124,7 @@ static int debug_sort(nh_ctx *ctx, const E *in, size_t n, int32_t device_sized, 
     unsigned long long *d_range = nullptr;
     int rc = NH_OK;
     if ((rc = dalloc(&sb.keys[0], n)) || (rc = dalloc(&sb.keys[1], n)) || (rc = dalloc(&sb.vals[0], n)) ||
-        (rc = dalloc(&sb.vals[1], n)) || (rc = dalloc(&sb.hist, (size_t)256 * sort_blocks(n) + 1)) ||
+        (rc = dalloc(&sb.vals[1], n)) || (rc = dalloc(&sb.hist, sort_hist_words(n))) ||
         (rc = dalloc(&sb.minmax, SORT_MINMAX_WORDS)) || (rc = dalloc(&d_in, n)) || (rc = dalloc(&d_out, n)) ||
         (rc = dalloc(&d_range, 2))) {
     } else {
@@ -184,7 +184,7 @@ extern "C" int nh_debug_count_packed_events(nh_ctx *ctx, const nh_packed_event *
         hipError_t e = hipMemcpyAsync(d_in, in, n * sizeof(DevPacked), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(d_c, &c, sizeof(c), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) {
-            launch_count_events(d_in, nullptr, n, d_c, st);
+            launch_count_events(d_in, nullptr, n, nullptr, d_c, st);
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipMemcpyAsync(&c, d_c, sizeof(c), hipMemcpyDeviceToHost, st);

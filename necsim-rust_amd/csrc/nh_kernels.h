@@ -19,7 +19,13 @@ struct Counters {
     unsigned long long overflow;      // plain dedup: a bucket list exceeded the device limit
     unsigned long long n_dead;        // persistent form: lineages that died this round
     unsigned long long ev_overflow;   // dispersal reporting: the round's event buffer was full
+    unsigned long long n_ghost;       // dispersal reporting: reserved event slots left unused
 };
+
+// Persistent dispersal reporting reserves event slots EVENT_CHUNK at a time
+// per wave; the unused rest of a wave's last chunk holds ghost records
+// (event time +inf, sorted after every event) counted in Counters::n_ghost.
+constexpr unsigned long long EVENT_CHUNK = 256;
 
 // Per-round inputs read by the kernels from device memory (so that a captured
 // graph can be replayed with new values).
@@ -152,11 +158,12 @@ constexpr int SORT_MINMAX_WORDS = 32;
 struct SortBuffers {
     unsigned long long *keys[2];
     uint32_t *vals[2];
-    uint32_t *hist;         // [256 * nblocks + 1]
+    uint32_t *hist;         // [sort_hist_words(capacity)]: digit counts, then scan chunk sums
     unsigned long long *minmax; // [SORT_MINMAX_WORDS] device: key ranges, tie count
     uint64_t capacity;
 };
 uint32_t sort_blocks(uint64_t n);
+size_t sort_hist_words(uint64_t n); // words of SortBuffers::hist for n events
 template <class E>
 void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream_t stream, bool fixed_passes);
 // Same, with the event range [*start, *end) read from device memory when the
@@ -186,8 +193,9 @@ struct EventCounts {
 };
 // Counts one round's sorted PackedEvents ev[0, n): with `end`, n is read from
 // *end when the kernels run (clamped to `cap`; capturable in a graph).
-void launch_count_events(const DevPacked *ev, const unsigned long long *end, uint64_t n_or_cap, EventCounts *c,
-                         hipStream_t stream);
+// `ghosts` (or null): the number of trailing ghost records to leave out.
+void launch_count_events(const DevPacked *ev, const unsigned long long *end, uint64_t n_or_cap,
+                         const unsigned long long *ghosts, EventCounts *c, hipStream_t stream);
 
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream);
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream);

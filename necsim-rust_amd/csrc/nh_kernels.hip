@@ -85,10 +85,12 @@ __device__ __forceinline__ void load_ratio(double *s_ratio, const double *ratio)
 
 // ---------------------------------------------------------------------------
 // Dispersal reporting: every executed step is reported as a PackedEvent.
-// Each step first reserves its slot in the round's event buffer (one atomic
-// per wave for all lanes executing together); a lane whose slot lies beyond
-// the buffer does not execute, parks its lineage and flags the round, so
-// nothing is ever written past the buffer.
+// Each step first reserves its slot in the round's event buffer; a lane whose
+// slot lies beyond the buffer does not execute, parks its lineage and flags
+// the round, so nothing is ever written past the buffer.
+//
+// wave_reserve: one atomic per wave for all lanes executing together (the
+// plain kernel: no unused slots).
 __device__ __forceinline__ unsigned long long wave_reserve(unsigned long long *ctr) {
     const uint64_t act = __ballot(1);
     const uint32_t leader = __ffsll((unsigned long long)act) - 1;
@@ -99,6 +101,47 @@ __device__ __forceinline__ unsigned long long wave_reserve(unsigned long long *c
 }
 
 constexpr uint64_t SIGN_BIT = 0x8000000000000000ull;
+
+// wave_take_slot: slots come from a wave-private chunk [ws[0], ws[1]) that is
+// refilled EVENT_CHUNK at a time with one atomic (same-address atomics from
+// every executing wave would serialise at the L2).  ws lives in LDS and is
+// only touched by the wave's lowest active lane.
+__device__ __forceinline__ unsigned long long wave_take_slot(volatile unsigned long long *ws,
+                                                             unsigned long long *ctr) {
+    const uint64_t act = __ballot(1);
+    const uint32_t leader = __ffsll((unsigned long long)act) - 1;
+    const unsigned long long need = (unsigned long long)__popcll(act);
+    const unsigned long long rank = (unsigned long long)__popcll(act & lanemask_lt());
+    unsigned long long old_base = 0, rem = 0, new_base = 0;
+    if (lane_id() == leader) {
+        const unsigned long long nx = ws[0], en = ws[1];
+        rem = en - nx < need ? en - nx : need;
+        old_base = nx;
+        if (rem < need) {
+            new_base = atomicAdd(ctr, EVENT_CHUNK);
+            ws[0] = new_base + (need - rem);
+            ws[1] = new_base + EVENT_CHUNK;
+        } else {
+            ws[0] = nx + need;
+        }
+    }
+    old_base = __shfl(old_base, leader);
+    rem = __shfl(rem, leader);
+    new_base = __shfl(new_base, leader);
+    return rank < rem ? old_base + rank : new_base + (rank - rem);
+}
+
+// at wave exit (all lanes): ghost records into the unused slots below cap
+__device__ __forceinline__ void wave_fill_ghosts(volatile unsigned long long *ws, DevPacked *out,
+                                                 unsigned long long cap, unsigned long long *n_ghost) {
+    const unsigned long long nx = ws[0], en0 = ws[1];
+    const unsigned long long en = en0 < cap ? en0 : cap;
+    if (en <= nx) return;
+    DevPacked g{};
+    g.event_time = from_bits(0x7ff0000000000000ull);
+    for (unsigned long long s = nx + lane_id(); s < en; s += 64) out[s] = g;
+    if (lane_id() == 0) atomicAdd(n_ghost, en - nx);
+}
 
 struct Origin {
     uint32_t x, y, i;
@@ -250,6 +293,12 @@ __device__ __forceinline__ bool wave_ready(bool ready, unsigned fill_eighths) {
 template <int KIND, bool DEDUP, bool DISP>
 __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(PersistArgs A) {
     __shared__ double s_ratio[POISSON_TABLE_N];
+    __shared__ unsigned long long s_slots[DISP ? 2 * (256 / 64) : 1]; // per wave: event chunk [next, end)
+    volatile unsigned long long *ws = &s_slots[DISP ? 2 * (threadIdx.x / 64) : 0];
+    if (DISP && lane_id() == 0) {
+        ws[0] = 0;
+        ws[1] = 0;
+    }
     load_ratio(s_ratio, A.P.ratio);
 
     const double level = A.round->level;
@@ -315,7 +364,7 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
                 Origin o;
                 unsigned long long eslot = 0;
                 if (DISP) {
-                    eslot = wave_reserve(&A.ctr->n_events);
+                    eslot = wave_take_slot(ws, &A.ctr->n_events);
                     if (eslot >= A.packed_cap) {
                         A.ctr->ev_overflow = 1;
                         lane_store<KIND>(L, A.S, A.lin, slot);
@@ -340,6 +389,7 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
             }
         }
     }
+    if (DISP) wave_fill_ghosts(ws, A.packed, A.packed_cap, &A.ctr->n_ghost);
     block_flush(my_steps, my_max, my_dead, A.ctr);
 }
 
@@ -854,19 +904,53 @@ constexpr int MM_FIELDS = 4; // [min, max] per field
 constexpr int MM_WORDS = MM_FIELDS + 2 * SORT_FIELDS_MAX;
 static_assert(MM_WORDS <= SORT_MINMAX_WORDS, "sort range words");
 
-// min / max over the valid lanes of a wave, one atomic pair per wave
-__device__ __forceinline__ void wave_minmax(unsigned long long k, bool valid, unsigned long long *mm) {
-    unsigned long long lo = valid ? k : ~0ull, hi = valid ? k : 0ull;
+// Per-element sort kernels process KS_ITEMS elements per thread and combine
+// their results per block, so that each block issues one atomic per result
+// (a same-address atomic from every wave serialises at the L2).
+constexpr int KS_THREADS = 256;
+constexpr int KS_ITEMS = 16;
+constexpr int KS_TILE = KS_THREADS * KS_ITEMS;
+// time key of a reserved but unused event slot: +inf sorts after every event
+constexpr unsigned long long GHOST_TIME_BITS = 0x7ff0000000000000ull;
+
+// [min, max] over a block (lo > hi when the block saw no key): one atomic pair
+__device__ __forceinline__ void block_minmax(unsigned long long lo, unsigned long long hi, unsigned long long *mm) {
+    __shared__ unsigned long long s_lo[KS_THREADS / 64], s_hi[KS_THREADS / 64];
     for (int off = 32; off; off >>= 1) {
         unsigned long long l2 = __shfl_xor(lo, off), h2 = __shfl_xor(hi, off);
         lo = l2 < lo ? l2 : lo;
         hi = h2 > hi ? h2 : hi;
     }
-    bool any = __any(valid);
-    if (any && lane_id() == 0) {
-        atomicMin(&mm[0], lo);
-        atomicMax(&mm[1], hi);
+    const unsigned wave = threadIdx.x / 64;
+    if (lane_id() == 0) {
+        s_lo[wave] = lo;
+        s_hi[wave] = hi;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (unsigned w = 1; w < blockDim.x / 64; w++) {
+            lo = s_lo[w] < lo ? s_lo[w] : lo;
+            hi = s_hi[w] > hi ? s_hi[w] : hi;
+        }
+        if (lo <= hi) {
+            atomicMin(&mm[0], lo);
+            atomicMax(&mm[1], hi);
+        }
+    }
+    __syncthreads();
+}
+
+// sum over a block, added to *out with one atomic
+__device__ __forceinline__ void block_add(unsigned long long v, unsigned long long *out) {
+    __shared__ unsigned long long s_v[KS_THREADS / 64];
+    for (int off = 32; off; off >>= 1) v += __shfl_xor(v, off);
+    if (lane_id() == 0) s_v[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (unsigned w = 1; w < blockDim.x / 64; w++) v += s_v[w];
+        if (v) atomicAdd(out, v);
+    }
+    __syncthreads();
 }
 
 __global__ void k_sort_init(unsigned long long *mm) {
@@ -876,42 +960,69 @@ __global__ void k_sort_init(unsigned long long *mm) {
 
 // time keys, identity values and the time key range
 template <class E>
-__global__ void k_event_keys(const E *ev0, SortSize sz, unsigned long long *keys, uint32_t *vals,
-                             unsigned long long *mm) {
-    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t n = sz.get();
+__global__ void __launch_bounds__(KS_THREADS) k_event_keys(const E *ev0, SortSize sz, unsigned long long *keys,
+                                                           uint32_t *vals, unsigned long long *mm) {
+    const uint64_t n = sz.get();
+    const uint64_t base = (uint64_t)blockIdx.x * KS_TILE;
+    if (base >= n) return;
     const E *ev = ev0 + sz.offset();
-    bool valid = p < n;
-    unsigned long long k = valid ? EventOrder<E>::field(ev[p], EventOrder<E>::TIME) : 0ull;
-    if (valid) {
-        keys[p] = k;
-        vals[p] = (uint32_t)p;
+    unsigned long long lo = ~0ull, hi = 0ull;
+    for (int j = 0; j < KS_ITEMS; j++) {
+        const uint64_t p = base + (uint64_t)j * KS_THREADS + threadIdx.x;
+        if (p < n) {
+            const unsigned long long k = EventOrder<E>::field(ev[p], EventOrder<E>::TIME);
+            keys[p] = k;
+            vals[p] = (uint32_t)p;
+            lo = k < lo ? k : lo;
+            hi = k > hi ? k : hi;
+        }
     }
-    wave_minmax(k, valid, mm + MM_TIME);
+    block_minmax(lo, hi, mm + MM_TIME);
 }
 
-// number of adjacent equal keys in the sorted time keys
-__global__ void k_tie_count(const unsigned long long *keys, SortSize sz, unsigned long long *count) {
-    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t n = sz.get();
-    bool tie = p + 1 < n && keys[p] == keys[p + 1];
-    uint64_t m = __ballot(tie);
-    if (m && lane_id() == 0) atomicAdd(count, (unsigned long long)__popcll(m));
+// number of adjacent equal keys in the sorted time keys (unused slots excluded)
+__global__ void __launch_bounds__(KS_THREADS) k_tie_count(const unsigned long long *keys, SortSize sz,
+                                                          unsigned long long *count) {
+    const uint64_t n = sz.get();
+    const uint64_t base = (uint64_t)blockIdx.x * KS_TILE;
+    if (base >= n) return;
+    unsigned long long ties = 0;
+    for (int j = 0; j < KS_ITEMS; j++) {
+        const uint64_t p = base + (uint64_t)j * KS_THREADS + threadIdx.x;
+        if (p + 1 < n) {
+            const unsigned long long k = keys[p];
+            ties += (k == keys[p + 1] && k != GHOST_TIME_BITS) ? 1 : 0;
+        }
+    }
+    block_add(ties, count);
 }
 
-// per-field key ranges (the digits a field's passes must cover)
-template <class E> __global__ void k_field_minmax(const E *ev0, SortSize sz, unsigned long long *mm) {
-    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t n = sz.get();
+template <class E>
+__global__ void __launch_bounds__(KS_THREADS) k_field_minmax(const E *ev0, SortSize sz, unsigned long long *mm) {
+    using O = EventOrder<E>;
+    const uint64_t n = sz.get();
+    const uint64_t base = (uint64_t)blockIdx.x * KS_TILE;
+    if (base >= n) return;
     const E *ev = ev0 + sz.offset();
-    bool valid = p < n;
-    E e{};
-    if (valid) e = ev[p];
-    for (int f = 0; f < EventOrder<E>::N; f++)
-        wave_minmax(EventOrder<E>::field(e, f), valid, mm + MM_FIELDS + 2 * f);
+    unsigned long long lo[O::N], hi[O::N];
+    for (int f = 0; f < O::N; f++) {
+        lo[f] = ~0ull;
+        hi[f] = 0ull;
+    }
+    for (int j = 0; j < KS_ITEMS; j++) {
+        const uint64_t p = base + (uint64_t)j * KS_THREADS + threadIdx.x;
+        if (p < n) {
+            const E e = ev[p];
+            for (int f = 0; f < O::N; f++) {
+                const unsigned long long k = O::field(e, f);
+                lo[f] = k < lo[f] ? k : lo[f];
+                hi[f] = k > hi[f] ? k : hi[f];
+            }
+        }
+    }
+    for (int f = 0; f < O::N; f++) block_minmax(lo[f], hi[f], mm + MM_FIELDS + 2 * f);
 }
 
-// keys[p] = field f of the event at the current permutation's position p
 template <class E>
 __global__ void k_field_keys(const E *ev0, const uint32_t *vals, unsigned long long *keys, int f, SortSize sz) {
     if (sz.skip()) return;
@@ -1006,6 +1117,7 @@ __global__ void k_reset_counters(Counters *c, bool reset_events) {
     if (reset_events) {
         c->n_events = 0;
         c->ev_overflow = 0;
+        c->n_ghost = 0;
     }
 }
 
@@ -1126,18 +1238,45 @@ void launch_compact(const uint8_t *status, const uint32_t *pass, uint64_t m, uin
 uint32_t sort_blocks(uint64_t n) { return (uint32_t)((n + RS_TILE - 1) / RS_TILE); }
 
 // exclusive scan of the 256 x tiles digit histogram of a sort
-__global__ void __launch_bounds__(1024) k_scan_hist(uint32_t *data, SortSize sz) {
+// Exclusive scan of the digit-major histogram (256 x tiles counts) in three
+// phases: per-chunk sums, a scan of the chunk sums, and each chunk's scan
+// offset by its sum prefix.  The chunk sums live after the histogram.
+constexpr int HS_THREADS = 256;
+constexpr int HS_ITEMS = 16;
+constexpr int HS_CHUNK = HS_THREADS * HS_ITEMS;
+
+__device__ __forceinline__ uint64_t hist_words(const SortSize &sz) { return (uint64_t)256 * sz.tiles(); }
+
+__global__ void __launch_bounds__(HS_THREADS) k_hist_chunk_sums(const uint32_t *h, SortSize sz, uint32_t *sums) {
+    if (sz.skip()) return;
+    const uint64_t n = hist_words(sz);
+    const uint64_t base = (uint64_t)blockIdx.x * HS_CHUNK;
+    if (base >= n) return;
+    unsigned long long s = 0;
+    for (int j = 0; j < HS_ITEMS; j++) {
+        const uint64_t p = base + (uint64_t)j * HS_THREADS + threadIdx.x;
+        if (p < n) s += h[p];
+    }
+    __shared__ unsigned long long total;
+    if (threadIdx.x == 0) total = 0;
+    __syncthreads();
+    block_add(s, &total);
+    __syncthreads();
+    if (threadIdx.x == 0) sums[blockIdx.x] = (uint32_t)total;
+}
+
+// one block: exclusive scan of the chunk sums in place
+__global__ void __launch_bounds__(1024) k_scan_chunk_sums(uint32_t *sums, SortSize sz) {
     __shared__ uint32_t partial[1024];
     if (sz.skip()) return;
-    const uint64_t n = (uint64_t)256 * sz.tiles();
-    const int T = blockDim.x;
-    uint64_t per = (n + T - 1) / T;
-    uint64_t lo = per * threadIdx.x, hi = lo + per < n ? lo + per : n;
+    const uint64_t n = (hist_words(sz) + HS_CHUNK - 1) / HS_CHUNK;
+    const uint64_t per = (n + 1023) / 1024;
+    const uint64_t lo = per * threadIdx.x, hi = lo + per < n ? lo + per : n;
     uint32_t sum = 0;
-    for (uint64_t j = lo; j < hi; j++) sum += data[j];
+    for (uint64_t j = lo; j < hi; j++) sum += sums[j];
     partial[threadIdx.x] = sum;
     __syncthreads();
-    for (int off = 1; off < T; off <<= 1) {
+    for (int off = 1; off < 1024; off <<= 1) {
         uint32_t v = threadIdx.x >= (unsigned)off ? partial[threadIdx.x - off] : 0;
         __syncthreads();
         partial[threadIdx.x] += v;
@@ -1145,20 +1284,58 @@ __global__ void __launch_bounds__(1024) k_scan_hist(uint32_t *data, SortSize sz)
     }
     uint32_t run = threadIdx.x ? partial[threadIdx.x - 1] : 0;
     for (uint64_t j = lo; j < hi; j++) {
-        uint32_t v = data[j];
-        data[j] = run;
+        uint32_t v = sums[j];
+        sums[j] = run;
         run += v;
     }
 }
 
-// digit passes first..last over (keys, vals)[cur]; returns the buffer that
-// holds the result
+// each thread scans HS_ITEMS consecutive counts; the block scans the thread
+// totals (wave shuffles, then the wave totals)
+__global__ void __launch_bounds__(HS_THREADS) k_hist_scan_apply(uint32_t *h, SortSize sz, const uint32_t *sums) {
+    __shared__ uint32_t s_wave[HS_THREADS / 64];
+    if (sz.skip()) return;
+    const uint64_t n = hist_words(sz);
+    const uint64_t base = (uint64_t)blockIdx.x * HS_CHUNK;
+    if (base >= n) return;
+    const uint64_t first = base + (uint64_t)threadIdx.x * HS_ITEMS;
+    uint32_t v[HS_ITEMS];
+    uint32_t total = 0;
+    for (int j = 0; j < HS_ITEMS; j++) {
+        v[j] = first + j < n ? h[first + j] : 0u;
+        total += v[j];
+    }
+    const unsigned lane = lane_id(), wave = threadIdx.x / 64;
+    uint32_t incl = total;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(incl, off);
+        if (lane >= (unsigned)off) incl += y;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t run = sums[blockIdx.x] + incl - total;
+    for (unsigned w = 0; w < wave; w++) run += s_wave[w];
+    for (int j = 0; j < HS_ITEMS; j++) {
+        if (first + j < n) h[first + j] = run;
+        run += v[j];
+    }
+}
+
+size_t sort_hist_words(uint64_t n) {
+    uint64_t h = (uint64_t)256 * sort_blocks(n);
+    return (size_t)(h + 1 + (h + HS_CHUNK - 1) / HS_CHUNK + 1);
+}
+
 static int radix_passes(SortSize sz, uint64_t cap, SortBuffers &sb, hipStream_t stream, int first, int last,
                         int cur) {
     uint32_t nb = sort_blocks(cap);
+    uint32_t nc = (uint32_t)(((uint64_t)256 * nb + HS_CHUNK - 1) / HS_CHUNK);
+    uint32_t *chunk_sums = sb.hist + (size_t)256 * nb + 1;
     for (int d = first; d <= last; d++) {
         hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sz, 8 * d, sb.hist);
-        hipLaunchKernelGGL(k_scan_hist, dim3(1), dim3(1024), 0, stream, sb.hist, sz);
+        hipLaunchKernelGGL(k_hist_chunk_sums, dim3(nc), dim3(HS_THREADS), 0, stream, sb.hist, sz, chunk_sums);
+        hipLaunchKernelGGL(k_scan_chunk_sums, dim3(1), dim3(1024), 0, stream, chunk_sums, sz);
+        hipLaunchKernelGGL(k_hist_scan_apply, dim3(nc), dim3(HS_THREADS), 0, stream, sb.hist, sz, chunk_sums);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sb.vals[cur],
                            sb.keys[cur ^ 1], sb.vals[cur ^ 1], sz, 8 * d, sb.hist);
         cur ^= 1;
@@ -1182,9 +1359,9 @@ void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream
     using O = EventOrder<E>;
     if (n == 0) return;
     SortSize sz{n, nullptr, nullptr, nullptr};
-    unsigned g = grid_for(n, 256);
+    unsigned g = grid_for(n, 256), gk = grid_for(n, KS_TILE);
     hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, stream, sb.minmax);
-    hipLaunchKernelGGL(k_event_keys<E>, dim3(g), dim3(256), 0, stream, events, sz, sb.keys[0], sb.vals[0],
+    hipLaunchKernelGGL(k_event_keys<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sz, sb.keys[0], sb.vals[0],
                        sb.minmax);
     unsigned long long mm[MM_WORDS];
     int last = 7;
@@ -1195,8 +1372,8 @@ void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream
     }
     int cur = radix_passes(sz, n, sb, stream, 0, last, 0);
     if (n > 1) {
-        hipLaunchKernelGGL(k_tie_count, dim3(g), dim3(256), 0, stream, sb.keys[cur], sz, sb.minmax + MM_TIES);
-        hipLaunchKernelGGL(k_field_minmax<E>, dim3(g), dim3(256), 0, stream, events, sz, sb.minmax);
+        hipLaunchKernelGGL(k_tie_count, dim3(gk), dim3(KS_THREADS), 0, stream, sb.keys[cur], sz, sb.minmax + MM_TIES);
+        hipLaunchKernelGGL(k_field_minmax<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sz, sb.minmax);
         (void)hipMemcpyAsync(mm, sb.minmax, sizeof(mm), hipMemcpyDeviceToHost, stream);
         (void)hipStreamSynchronize(stream);
         if (mm[MM_TIES]) {
@@ -1223,12 +1400,12 @@ void sort_events_device_sized(const E *events, const unsigned long long *end, co
     using O = EventOrder<E>;
     if (cap == 0) return;
     SortSize sz{cap, end, start, nullptr};
-    unsigned g = grid_for(cap, 256);
+    unsigned g = grid_for(cap, 256), gk = grid_for(cap, KS_TILE);
     hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, stream, sb.minmax);
-    hipLaunchKernelGGL(k_event_keys<E>, dim3(g), dim3(256), 0, stream, events, sz, sb.keys[0], sb.vals[0],
+    hipLaunchKernelGGL(k_event_keys<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sz, sb.keys[0], sb.vals[0],
                        sb.minmax);
     int cur = radix_passes(sz, cap, sb, stream, 0, 7, 0);
-    hipLaunchKernelGGL(k_tie_count, dim3(g), dim3(256), 0, stream, sb.keys[cur], sz, sb.minmax + MM_TIES);
+    hipLaunchKernelGGL(k_tie_count, dim3(gk), dim3(KS_THREADS), 0, stream, sb.keys[cur], sz, sb.minmax + MM_TIES);
     SortSize gz = sz;
     gz.gate = sb.minmax + MM_TIES;
     for (int f = 0; f < O::N; f++) {
@@ -1269,12 +1446,13 @@ __device__ __forceinline__ bool same_event_key(const DevPacked &a, const DevPack
            a.oindex == b.oindex && (speciation || (a.tx == b.tx && a.ty == b.ty && a.tindex == b.tindex));
 }
 
-__global__ void __launch_bounds__(256) k_count_events(const DevPacked *ev, SortSize sz, EventCounts *c) {
+__global__ void __launch_bounds__(256) k_count_events(const DevPacked *ev, SortSize sz,
+                                                     const unsigned long long *ghosts, EventCounts *c) {
     __shared__ unsigned long long s_cnt[EC_COUNTERS];
     if (threadIdx.x < EC_COUNTERS) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t n = sz.get();
+    const uint64_t n = sz.get() - (ghosts ? *ghosts : 0ull);
     int which = -1; // the counter this event adds to, besides the raw total
     if (i < n) {
         const DevPacked e = ev[i];
@@ -1318,20 +1496,21 @@ __global__ void __launch_bounds__(256) k_count_events(const DevPacked *ev, SortS
 }
 
 // after the round's count: the last prior time for the next round
-__global__ void k_count_carry(const DevPacked *ev, SortSize sz, EventCounts *c) {
-    const uint64_t n = sz.get();
+__global__ void k_count_carry(const DevPacked *ev, SortSize sz, const unsigned long long *ghosts,
+                              EventCounts *c) {
+    const uint64_t n = sz.get() - (ghosts ? *ghosts : 0ull);
     if (n) {
         c->carry_has = 1;
         c->carry_prior = bits(ev[n - 1].prior_time) & ABS_BITS;
     }
 }
 
-void launch_count_events(const DevPacked *ev, const unsigned long long *end, uint64_t n_or_cap, EventCounts *c,
-                         hipStream_t stream) {
+void launch_count_events(const DevPacked *ev, const unsigned long long *end, uint64_t n_or_cap,
+                         const unsigned long long *ghosts, EventCounts *c, hipStream_t stream) {
     if (n_or_cap == 0) return;
     SortSize sz{n_or_cap, end, nullptr, nullptr};
-    hipLaunchKernelGGL(k_count_events, dim3(grid_for(n_or_cap, 256)), dim3(256), 0, stream, ev, sz, c);
-    hipLaunchKernelGGL(k_count_carry, dim3(1), dim3(1), 0, stream, ev, sz, c);
+    hipLaunchKernelGGL(k_count_events, dim3(grid_for(n_or_cap, 256)), dim3(256), 0, stream, ev, sz, ghosts, c);
+    hipLaunchKernelGGL(k_count_carry, dim3(1), dim3(1), 0, stream, ev, sz, ghosts, c);
 }
 
 uint32_t soa_compact_blocks(uint64_t n) { return (uint32_t)((n + SC_TILE - 1) / SC_TILE); }

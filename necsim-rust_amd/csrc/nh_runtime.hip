@@ -457,31 +457,6 @@ static int job_start(nh_job *j) {
     NH_TRY(dalloc(&j->d_ctr, 1));
     NH_TRY(dalloc(&j->d_round, 1));
     NH_HIP(hipMemsetAsync(j->d_ctr, 0, sizeof(Counters), st));
-    // speciation-only reporting appends to one buffer for the whole run (each
-    // lineage speciates at most once); dispersal reporting keeps one round's
-    // events, about event_slice of them on average (the level advances by
-    // event_slice / sum(lambda))
-    uint64_t sort_cap = n;
-    if (j->disp) {
-        j->packed_cap = p.event_buffer ? p.event_buffer : 2 * j->event_slice + n + 65536;
-        NH_TRY(dalloc(&j->d_packed, j->packed_cap));
-        NH_TRY(dalloc(&j->d_packed_sorted, j->packed_cap));
-        sort_cap = j->packed_cap;
-        if (p.device_event_counter) {
-            NH_TRY(dalloc(&j->d_counts, 1));
-            NH_HIP(hipMemsetAsync(j->d_counts, 0, sizeof(EventCounts), st));
-        }
-    } else {
-        NH_TRY(dalloc(&j->d_events, n));
-        NH_TRY(dalloc(&j->d_sorted, n));
-    }
-    j->sb.capacity = sort_cap;
-    NH_TRY(dalloc(&j->sb.keys[0], sort_cap));
-    NH_TRY(dalloc(&j->sb.keys[1], sort_cap));
-    NH_TRY(dalloc(&j->sb.vals[0], sort_cap));
-    NH_TRY(dalloc(&j->sb.vals[1], sort_cap));
-    NH_TRY(dalloc(&j->sb.hist, (size_t)256 * sort_blocks(sort_cap) + 1));
-    NH_TRY(dalloc(&j->sb.minmax, SORT_MINMAX_WORDS));
     if (j->cache_cap) {
         NH_TRY(dalloc(&j->d_cache, 4 * j->cache_cap));
         launch_fill_u64(j->d_cache, 0, 4 * j->cache_cap, st);
@@ -526,6 +501,34 @@ static int job_start(nh_job *j) {
             launch_fill_u32(j->d_head, 0xffffffffu, j->cache_cap, st);
         }
     }
+    // speciation-only reporting appends to one buffer for the whole run (each
+    // lineage speciates at most once); dispersal reporting keeps one round's
+    // events, about event_slice of them on average (the level advances by
+    // event_slice / sum(lambda))
+    uint64_t sort_cap = n;
+    if (j->disp) {
+        // persistent waves reserve slots EVENT_CHUNK at a time: each may leave
+        // up to a chunk unused per round
+        uint64_t waves = p.form == NH_FORM_PERSISTENT ? (uint64_t)j->blocks * (uint64_t)(j->threads / 64) : 0;
+        j->packed_cap = p.event_buffer ? p.event_buffer : 2 * j->event_slice + n + 65536 + waves * EVENT_CHUNK;
+        NH_TRY(dalloc(&j->d_packed, j->packed_cap));
+        NH_TRY(dalloc(&j->d_packed_sorted, j->packed_cap));
+        sort_cap = j->packed_cap;
+        if (p.device_event_counter) {
+            NH_TRY(dalloc(&j->d_counts, 1));
+            NH_HIP(hipMemsetAsync(j->d_counts, 0, sizeof(EventCounts), st));
+        }
+    } else {
+        NH_TRY(dalloc(&j->d_events, n));
+        NH_TRY(dalloc(&j->d_sorted, n));
+    }
+    j->sb.capacity = sort_cap;
+    NH_TRY(dalloc(&j->sb.keys[0], sort_cap));
+    NH_TRY(dalloc(&j->sb.keys[1], sort_cap));
+    NH_TRY(dalloc(&j->sb.vals[0], sort_cap));
+    NH_TRY(dalloc(&j->sb.vals[1], sort_cap));
+    NH_TRY(dalloc(&j->sb.hist, sort_hist_words(sort_cap)));
+    NH_TRY(dalloc(&j->sb.minmax, SORT_MINMAX_WORDS));
     j->pending.clear();
     j->pending.shrink_to_fit();
     NH_HIP(hipStreamSynchronize(st));
@@ -733,7 +736,8 @@ static int persistent_round(nh_job *j, bool *sorted) {
                 sort_events_device_sized(j->d_packed, &j->d_ctr->n_events, nullptr, j->packed_cap,
                                          j->d_packed_sorted, j->sb, st);
                 if (j->d_counts)
-                    launch_count_events(j->d_packed_sorted, &j->d_ctr->n_events, j->packed_cap, j->d_counts, st);
+                    launch_count_events(j->d_packed_sorted, &j->d_ctr->n_events, j->packed_cap, &j->d_ctr->n_ghost,
+                                        j->d_counts, st);
             }
             else
                 sort_events_device_sized(j->d_events, &j->d_ctr->n_events, &j->d_round->ev_start, j->n_initial,
@@ -798,6 +802,7 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
     if (j->disp && p.form == NH_FORM_PLAIN) {
         NH_HIP(hipMemsetAsync(&j->d_ctr->n_events, 0, sizeof(unsigned long long), st));
         NH_HIP(hipMemsetAsync(&j->d_ctr->ev_overflow, 0, sizeof(unsigned long long), st));
+        NH_HIP(hipMemsetAsync(&j->d_ctr->n_ghost, 0, sizeof(unsigned long long), st));
     }
 
     bool sorted = false;
@@ -813,7 +818,10 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
                          "use a smaller event slice",
                          (unsigned long long)j->packed_cap);
     }
-    uint64_t ne = c.n_events - ev_start;
+    // dispersal reporting: slots [0, min(reserved, cap)) hold the round's
+    // events and the ghost records of unused reserved slots
+    uint64_t slots = j->disp ? std::min<uint64_t>(c.n_events, j->packed_cap) : 0;
+    uint64_t ne = j->disp ? slots - c.n_ghost : c.n_events - ev_start;
     j->n_events = j->disp ? j->n_events + ne : c.n_events;
     j->steps = c.steps;
     double mt = from_bits(c.max_time_bits);
@@ -824,14 +832,15 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
     j->rounds++;
 
     // the live water-level reporter sorts each round's events (live.rs:88-115)
+    // (ghost records sort after every event)
     if (ne && j->disp && j->d_counts) {
         if (!sorted) {
-            sort_events(j->d_packed, ne, j->d_packed_sorted, j->sb, st, false);
-            launch_count_events(j->d_packed_sorted, nullptr, ne, j->d_counts, st);
+            sort_events(j->d_packed, slots, j->d_packed_sorted, j->sb, st, false);
+            launch_count_events(j->d_packed_sorted, nullptr, ne, nullptr, j->d_counts, st);
         }
         NH_HIP(hipGetLastError());
     } else if (ne && j->disp) {
-        if (!sorted) sort_events(j->d_packed, ne, j->d_packed_sorted, j->sb, st, false);
+        if (!sorted) sort_events(j->d_packed, slots, j->d_packed_sorted, j->sb, st, false);
         NH_HIP(hipGetLastError());
         size_t old = j->ready_packed.size();
         j->ready_packed.resize(old + ne);

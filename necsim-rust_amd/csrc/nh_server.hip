@@ -153,7 +153,7 @@ extern "C" int nh_server_create(nh_ctx *ctx, const nh_tables *tables, const nh_s
     s->sb.capacity = cap;
     if ((rc = dalloc(&s->sb.keys[0], cap)) || (rc = dalloc(&s->sb.keys[1], cap)) ||
         (rc = dalloc(&s->sb.vals[0], cap)) || (rc = dalloc(&s->sb.vals[1], cap)) ||
-        (rc = dalloc(&s->sb.hist, (size_t)256 * sort_blocks(cap) + 1)) || (rc = dalloc(&s->sb.minmax, SORT_MINMAX_WORDS)))
+        (rc = dalloc(&s->sb.hist, sort_hist_words(cap))) || (rc = dalloc(&s->sb.minmax, SORT_MINMAX_WORDS)))
         return fail(rc);
     s->h_reqs.resize(SERVE_MAX_REQS);
     s->h_ctr.resize(SERVE_MAX_REQS);
