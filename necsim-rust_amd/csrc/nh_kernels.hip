@@ -901,7 +901,9 @@ constexpr int SORT_FIELDS_MAX = 9;
 constexpr int MM_TIME = 0;   // [min, max] of the event-time keys
 constexpr int MM_TIES = 2;   // number of adjacent equal event times after the time sort
 constexpr int MM_FIELDS = 4; // [min, max] per field
-constexpr int MM_WORDS = MM_FIELDS + 2 * SORT_FIELDS_MAX;
+constexpr int MM_TIE_MIXED = MM_FIELDS + 2 * SORT_FIELDS_MAX; // a run of equal times has several origins / targets
+constexpr int MM_TIE_LONG = MM_TIE_MIXED + 1;                  // a run is longer than TIE_RUN_MAX
+constexpr int MM_WORDS = MM_TIE_LONG + 1;
 static_assert(MM_WORDS <= SORT_MINMAX_WORDS, "sort range words");
 
 // Per-element sort kernels process KS_ITEMS elements per thread and combine
@@ -955,7 +957,9 @@ __device__ __forceinline__ void block_add(unsigned long long v, unsigned long lo
 
 __global__ void k_sort_init(unsigned long long *mm) {
     int t = threadIdx.x;
-    if (t < MM_WORDS) mm[t] = (t == MM_TIME || (t >= MM_FIELDS && ((t - MM_FIELDS) & 1) == 0)) ? ~0ull : 0ull;
+    // minima start at ~0, maxima, counts and flags at 0
+    const bool field_min = t >= MM_FIELDS && t < MM_FIELDS + 2 * SORT_FIELDS_MAX && ((t - MM_FIELDS) & 1) == 0;
+    if (t < MM_WORDS) mm[t] = (t == MM_TIME || field_min) ? ~0ull : 0ull;
 }
 
 // time keys, identity values and the time key range
@@ -1021,6 +1025,121 @@ __global__ void __launch_bounds__(KS_THREADS) k_field_minmax(const E *ev0, SortS
         }
     }
     for (int f = 0; f < O::N; f++) block_minmax(lo[f], hi[f], mm + MM_FIELDS + 2 * f);
+}
+
+// ---------------------------------------------------------------------------
+// Runs of equal event times after the time sort.  Events of one run come
+// from lineages that share a location and a Poisson event (coalesced
+// duplicates), so they share origin and target, and the PackedEvent order
+// within the run reduces to (prior time, reference).  Runs of up to
+// TIE_RUN_MAX events are sorted in LDS: the workgroup of window w sorts the
+// runs that start in positions [w * TIE_WINDOW, (w + 1) * TIE_WINDOW), which
+// with their tails span fewer than TIE_WINDOW + TIE_RUN_MAX positions.
+// Longer runs are ordered by radix passes over the reference, prior and
+// time fields only; runs with several origins or targets by passes over
+// every field.
+constexpr uint64_t TIE_RUN_MAX = 1024;
+constexpr uint64_t TIE_WINDOW = 1024;
+constexpr int TIE_SORT_N = 2048; // >= TIE_WINDOW + TIE_RUN_MAX - 1, a power of two
+constexpr int TIE_THREADS = 256;
+
+// (prior, reference) of every event in time order, into scratch; flags runs
+// whose members differ in origin or target (flags[0]) and runs longer than
+// TIE_RUN_MAX (flags[1], checked at each run's head)
+template <class E>
+__global__ void __launch_bounds__(KS_THREADS) k_tie_gather(const E *ev, const unsigned long long *keys,
+                                                           const uint32_t *vals, uint64_t n,
+                                                           unsigned long long *prior_ws, unsigned long long *ref_ws,
+                                                           unsigned long long *flags) {
+    using O = EventOrder<E>;
+    const uint64_t base = (uint64_t)blockIdx.x * KS_TILE;
+    if (base >= n) return;
+    unsigned long long mixed = 0, longer = 0;
+    for (int j = 0; j < KS_ITEMS; j++) {
+        const uint64_t p = base + (uint64_t)j * KS_THREADS + threadIdx.x;
+        if (p >= n) continue;
+        const E e = ev[vals[p]];
+        prior_ws[p] = O::field(e, 1);
+        ref_ws[p] = O::field(e, 0);
+        const unsigned long long k = keys[p];
+        const bool ghost = k == GHOST_TIME_BITS; // unused event slots: runs of one
+        const bool head = ghost || p == 0 || keys[p - 1] != k;
+        if (!head) {
+            const E q = ev[vals[p - 1]];
+            for (int f = 2; f < O::N - 1; f++) mixed |= O::field(e, f) != O::field(q, f) ? 1ull : 0ull;
+        } else if (!ghost && p + TIE_RUN_MAX < n && keys[p + TIE_RUN_MAX] == k) {
+            longer = 1;
+        }
+    }
+    block_add(mixed, flags);
+    block_add(longer, flags + 1);
+}
+
+// first run head in [lo, hi) (hi if none), found by the whole block
+__device__ __forceinline__ uint64_t block_first_head(const unsigned long long *keys, uint64_t lo, uint64_t hi) {
+    __shared__ unsigned long long s_first;
+    if (threadIdx.x == 0) s_first = hi;
+    __syncthreads();
+    for (uint64_t p = lo + threadIdx.x; p < hi; p += blockDim.x)
+        if (p == 0 || keys[p] != keys[p - 1] || keys[p] == GHOST_TIME_BITS)
+            atomicMin(&s_first, (unsigned long long)p);
+    __syncthreads();
+    const uint64_t r = s_first;
+    __syncthreads();
+    return r;
+}
+
+// Sorts the window's runs by (time key, prior, reference, time-sorted
+// position) with a bitonic sort in LDS and writes their permutation.
+// Unused event slots (ghost keys) sort after every event and count as runs
+// of one.
+__global__ void __launch_bounds__(TIE_THREADS) k_tie_sort_windows(const unsigned long long *keys, const uint32_t *vals,
+                                                                  uint64_t n, const unsigned long long *prior_ws,
+                                                                  const unsigned long long *ref_ws,
+                                                                  uint32_t *out_vals) {
+    __shared__ unsigned long long s_t[TIE_SORT_N], s_p[TIE_SORT_N], s_r[TIE_SORT_N];
+    __shared__ uint16_t s_i[TIE_SORT_N];
+    const uint64_t lo = (uint64_t)blockIdx.x * TIE_WINDOW;
+    if (lo >= n) return;
+    const uint64_t hi = lo + TIE_WINDOW < n ? lo + TIE_WINDOW : n;
+    const uint64_t s = block_first_head(keys, lo, hi);
+    if (s >= hi) return; // every position of the window continues an earlier run
+    const uint64_t tail_hi = hi + TIE_RUN_MAX < n ? hi + TIE_RUN_MAX : n;
+    const uint64_t e = hi < n ? block_first_head(keys, hi, tail_hi) : n;
+    const int len = (int)(e - s);
+    for (int i = threadIdx.x; i < TIE_SORT_N; i += TIE_THREADS) {
+        if (i < len) {
+            s_t[i] = keys[s + i];
+            s_p[i] = prior_ws[s + i];
+            s_r[i] = ref_ws[s + i];
+        } else {
+            s_t[i] = ~0ull;
+            s_p[i] = ~0ull;
+            s_r[i] = ~0ull;
+        }
+        s_i[i] = (uint16_t)i;
+    }
+    __syncthreads();
+    for (int k = 2; k <= TIE_SORT_N; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < TIE_SORT_N; i += TIE_THREADS) {
+                const int l = i ^ j;
+                if (l <= i) continue;
+                const bool ascending = (i & k) == 0;
+                const unsigned long long ti = s_t[i], tl = s_t[l], pi = s_p[i], pl = s_p[l], ri = s_r[i], rl = s_r[l];
+                const uint16_t ii = s_i[i], il = s_i[l];
+                const bool l_less = tl < ti || (tl == ti && (pl < pi || (pl == pi && (rl < ri || (rl == ri && il < ii)))));
+                if (l_less == ascending) {
+                    s_t[i] = tl; s_t[l] = ti;
+                    s_p[i] = pl; s_p[l] = pi;
+                    s_r[i] = rl; s_r[l] = ri;
+                    s_i[i] = il; s_i[l] = ii;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < len; i += TIE_THREADS) out_vals[s + i] = vals[s + s_i[i]];
 }
 
 template <class E>
@@ -1373,11 +1492,38 @@ void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream
     int cur = radix_passes(sz, n, sb, stream, 0, last, 0);
     if (n > 1) {
         hipLaunchKernelGGL(k_tie_count, dim3(gk), dim3(KS_THREADS), 0, stream, sb.keys[cur], sz, sb.minmax + MM_TIES);
-        hipLaunchKernelGGL(k_field_minmax<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sz, sb.minmax);
-        (void)hipMemcpyAsync(mm, sb.minmax, sizeof(mm), hipMemcpyDeviceToHost, stream);
+        (void)hipMemcpyAsync(&mm[MM_TIES], sb.minmax + MM_TIES, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             stream);
         (void)hipStreamSynchronize(stream);
+        if (mm[MM_TIES] && !fixed_passes) {
+            // rank each run of equal times in place (prior / reference scratch in `out`)
+            unsigned long long *prior_ws = reinterpret_cast<unsigned long long *>(out);
+            unsigned long long *ref_ws = prior_ws + n;
+            static_assert(sizeof(E) >= 16, "scratch of two words per event");
+            static_assert(MM_TIE_LONG == MM_TIE_MIXED + 1, "tie flags are adjacent");
+            hipLaunchKernelGGL(k_tie_gather<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sb.keys[cur],
+                               sb.vals[cur], n, prior_ws, ref_ws, sb.minmax + MM_TIE_MIXED);
+            (void)hipMemcpyAsync(&mm[MM_TIE_MIXED], sb.minmax + MM_TIE_MIXED, 2 * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, stream);
+            (void)hipStreamSynchronize(stream);
+            if (!mm[MM_TIE_MIXED] && !mm[MM_TIE_LONG]) {
+                hipLaunchKernelGGL(k_tie_sort_windows, dim3(grid_for(n, TIE_WINDOW)), dim3(TIE_THREADS), 0, stream,
+                                   sb.keys[cur], sb.vals[cur], n, prior_ws, ref_ws, sb.vals[cur ^ 1]);
+                hipLaunchKernelGGL(k_gather_events<E>, dim3(g), dim3(256), 0, stream, events, sb.vals[cur ^ 1], out,
+                                   sz);
+                return;
+            }
+        }
         if (mm[MM_TIES]) {
+            if (!fixed_passes) {
+                hipLaunchKernelGGL(k_field_minmax<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sz, sb.minmax);
+                (void)hipMemcpyAsync(mm, sb.minmax, sizeof(mm), hipMemcpyDeviceToHost, stream);
+                (void)hipStreamSynchronize(stream);
+            }
+            // runs sharing origin and target need the reference, prior and time passes only
+            const bool uniform_runs = !fixed_passes && !mm[MM_TIE_MIXED];
             for (int f = 0; f < O::N; f++) {
+                if (uniform_runs && f >= 2 && f < O::N - 1) continue;
                 int lf = fixed_passes ? O::digits(f) - 1
                                       : highest_digit(mm[MM_FIELDS + 2 * f] ^ mm[MM_FIELDS + 2 * f + 1]);
                 if (lf < 0) continue;

@@ -324,6 +324,10 @@ struct nh_job {
     uint64_t packed_cap = 0;
     std::vector<nh_packed_event> ready_packed;
     size_t packed_pos = 0;
+    // the last round's sorted events not drained yet, still in d_packed_sorted
+    // [dev_pos, dev_n): drained straight into the caller's buffer, moved to
+    // ready_packed before the next round overwrites them
+    uint64_t dev_pos = 0, dev_n = 0;
     EventCounts *d_counts = nullptr; // device_event_counter jobs
     SortBuffers sb{};
     uint64_t *d_cache = nullptr;
@@ -721,7 +725,10 @@ static PersistArgs persist_args(nh_job *j, int cur) {
 // event-key -> radix-sort -> gather cycle is captured once per input buffer
 // and replayed; the round's level, size and event offset are read from
 // device memory (RoundDesc, Counters).  Sets *sorted when the round's events
-// are already sorted into d_sorted.
+// are already sorted into d_sorted.  Dispersal reporting captures the step
+// kernel only: its rounds are large (about event_slice events), so the eager
+// sort, which reads the key ranges back and skips constant digits, costs one
+// synchronisation per round and saves many digit passes.
 static int persistent_round(nh_job *j, bool *sorted) {
     hipStream_t st = j->ctx->stream;
     *sorted = false;
@@ -732,14 +739,7 @@ static int persistent_round(nh_job *j, bool *sorted) {
             NH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             launch_reset_counters(j->d_ctr, j->disp, st);
             launch_persistent(j->tables->kind, persist_args(j, g), j->blocks, j->threads, st);
-            if (j->disp) {
-                sort_events_device_sized(j->d_packed, &j->d_ctr->n_events, nullptr, j->packed_cap,
-                                         j->d_packed_sorted, j->sb, st);
-                if (j->d_counts)
-                    launch_count_events(j->d_packed_sorted, &j->d_ctr->n_events, j->packed_cap, &j->d_ctr->n_ghost,
-                                        j->d_counts, st);
-            }
-            else
+            if (!j->disp)
                 sort_events_device_sized(j->d_events, &j->d_ctr->n_events, &j->d_round->ev_start, j->n_initial,
                                          j->d_sorted, j->sb, st);
             NH_HIP(hipStreamEndCapture(st, &graph));
@@ -750,7 +750,7 @@ static int persistent_round(nh_job *j, bool *sorted) {
         NH_HIP(hipEventRecord(j->ev0, st));
         NH_HIP(hipGraphLaunch(j->graph[g], st));
         NH_HIP(hipEventRecord(j->ev1, st));
-        *sorted = true;
+        *sorted = !j->disp;
     } else {
         launch_reset_counters(j->d_ctr, j->disp, st);
         NH_HIP(hipEventRecord(j->ev0, st));
@@ -775,6 +775,17 @@ extern "C" int nh_job_start(nh_job *j) {
     return job_start(j);
 }
 
+// the undrained rest of the last round's sorted events, copied to the host
+static int stash_device_events(nh_job *j) {
+    uint64_t k = j->dev_n - j->dev_pos;
+    size_t old = j->ready_packed.size();
+    j->ready_packed.resize(old + k);
+    NH_HIP(hipMemcpy(j->ready_packed.data() + old, j->d_packed_sorted + j->dev_pos, k * sizeof(DevPacked),
+                     hipMemcpyDeviceToHost));
+    j->dev_pos = j->dev_n = 0;
+    return NH_OK;
+}
+
 extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
     if (!j) return set_error(NH_E_INVALID, "null job");
     NH_HIP(hipSetDevice(j->ctx->device));
@@ -788,6 +799,7 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
         return NH_OK;
     }
     hipStream_t st = j->ctx->stream;
+    if (j->dev_pos < j->dev_n) NH_TRY(stash_device_events(j));
     // total rate = sum of the reported event rate over the slow lineages,
     // folded in order: lambda with dispersal reporting (monolithic/mod.rs:139-148),
     // lambda * nu for speciation only (monolithic/mod.rs:150-164)
@@ -842,11 +854,9 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
     } else if (ne && j->disp) {
         if (!sorted) sort_events(j->d_packed, slots, j->d_packed_sorted, j->sb, st, false);
         NH_HIP(hipGetLastError());
-        size_t old = j->ready_packed.size();
-        j->ready_packed.resize(old + ne);
-        NH_HIP(hipMemcpyAsync(j->ready_packed.data() + old, j->d_packed_sorted, ne * sizeof(DevPacked),
-                              hipMemcpyDeviceToHost, st));
         NH_HIP(hipStreamSynchronize(st));
+        j->dev_pos = 0;
+        j->dev_n = ne;
     } else if (ne) {
         if (!sorted) sort_events(j->d_events + ev_start, ne, j->d_sorted, j->sb, st, false);
         NH_HIP(hipGetLastError());
@@ -874,7 +884,7 @@ template <class T> static void drain_ready(std::vector<T> &ready, size_t &pos, T
         return;
     }
     size_t n = std::min(avail, cap);
-    std::memcpy(buf, ready.data() + pos, n * sizeof(T));
+    if (n) std::memcpy(buf, ready.data() + pos, n * sizeof(T));
     pos += n;
     if (pos == ready.size()) {
         ready.clear();
@@ -886,7 +896,20 @@ template <class T> static void drain_ready(std::vector<T> &ready, size_t &pos, T
 extern "C" int nh_job_drain_packed_events(nh_job *j, nh_packed_event *buf, size_t cap, size_t *n_out) {
     if (!j || !n_out) return set_error(NH_E_INVALID, "null argument");
     if (!j->disp) return set_error(NH_E_STATE, "the job reports speciation events only (nh_job_drain_events)");
-    drain_ready(j->ready_packed, j->packed_pos, buf, cap, n_out);
+    const uint64_t on_device = j->dev_n - j->dev_pos;
+    if (!buf) {
+        *n_out = j->ready_packed.size() - j->packed_pos + on_device;
+        return NH_OK;
+    }
+    size_t host = 0;
+    drain_ready(j->ready_packed, j->packed_pos, buf, cap, &host); // older events first
+    size_t k = std::min<uint64_t>(cap - host, on_device);
+    if (k) {
+        NH_HIP(hipSetDevice(j->ctx->device));
+        NH_HIP(hipMemcpy(buf + host, j->d_packed_sorted + j->dev_pos, k * sizeof(DevPacked), hipMemcpyDeviceToHost));
+        j->dev_pos += k;
+    }
+    *n_out = host + k;
     return NH_OK;
 }
 

@@ -515,7 +515,7 @@ class Job:
         """every event since the last drain (report_dispersal jobs), sorted"""
         n = ctypes.c_size_t(0)
         check(lib().nh_job_drain_packed_events(self.handle, None, 0, ctypes.byref(n)))
-        buf = np.zeros(n.value, _ffi.PACKED_EVENT_DTYPE)
+        buf = np.empty(n.value, _ffi.PACKED_EVENT_DTYPE)  # every byte is written
         if n.value:
             check(lib().nh_job_drain_packed_events(self.handle, ptr(buf), n.value, ctypes.byref(n)))
         return buf[:n.value]

@@ -215,24 +215,39 @@ def test_drain_calls_match_the_job_kind(ctx):
 
 
 @pytest.mark.parametrize("device_sized", [0, 1])
-def test_device_sort_of_packed_events(ctx, device_sized):
+@pytest.mark.parametrize("shape", ["mixed", "runs", "long_runs", "runs_and_ghosts"])
+def test_device_sort_of_packed_events(ctx, device_sized, shape):
     """the PackedEvent order on the device against a lexicographic sort, on
-    records with many equal times, origins and targets and signed times"""
-    announce(f"device PackedEvent sort, device_sized={device_sized}")
+    records with many equal times and signed times: runs of equal times with
+    several origins and targets (per-field passes), runs sharing one origin
+    and target as coalesced lineages produce them (ranked in place), and runs
+    longer than the in-place limit (per-field passes again)"""
+    announce(f"device PackedEvent sort, {shape}, device_sized={device_sized}")
     rng = np.random.default_rng(17)
     n = 200_000
     ev = np.zeros(n, PK)
     ev["global_reference"] = rng.permutation(n).astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
-    times = rng.uniform(1, 50, 3000)
-    t = rng.choice(times, n)
+    n_times = {"mixed": 3000, "runs": 60000, "long_runs": 12, "runs_and_ghosts": 60000}[shape]
+    times = rng.uniform(1, 50, n_times)
+    pick = rng.integers(0, n_times, n)
+    t = times[pick]
     sign = rng.random(n) < 0.5
     ev["event_time"] = np.where(sign, -t, t)
-    prior = t - rng.choice([0.0, 0.5, 1.0], n)
+    prior = t - rng.choice([0.0, 0.5, 1.0, 1.5], n)
     ev["prior_time"] = np.where(sign, prior, -prior)
-    for f, hi in (("origin_x", 4), ("origin_y", 3), ("origin_index", 2), ("target_x", 1 << 32),
-                  ("target_y", 3), ("target_index", 70000)):
-        ev[f] = rng.integers(0, hi, n, dtype=np.uint64).astype(np.uint32)
+    fields = (("origin_x", 4), ("origin_y", 3), ("origin_index", 2), ("target_x", 1 << 32), ("target_y", 3),
+              ("target_index", 70000))
+    for f, hi in fields:
+        if shape == "mixed":
+            ev[f] = rng.integers(0, hi, n, dtype=np.uint64).astype(np.uint32)
+        else:  # one origin and target per time
+            ev[f] = rng.integers(0, hi, n_times, dtype=np.uint64).astype(np.uint32)[pick]
     ev["coalescence"] = ev["global_reference"]
+    if shape == "runs_and_ghosts":  # unused event slots of a persistent round: time +inf, zeros
+        ghosts = np.zeros(5000, PK)
+        ghosts["event_time"] = np.inf
+        ev = np.concatenate([ev, ghosts])[rng.permutation(n + 5000)]
+        n = len(ev)
     want = ev[nh.packed_order(ev)]
     got = np.zeros(n, PK)
     _ffi.check(_ffi.lib().nh_debug_sort_packed_events(ctx.handle, _ffi.ptr(ev), n, device_sized, _ffi.ptr(got)))
@@ -303,3 +318,34 @@ def test_device_counter_job_matches_host_counter(form, graph):
     from test_reporters import reference_counts
     assert dev.counts == reference_counts([col.events])
     assert dev.counts["late_coalescence"] > 0 and dev.counts["raw_total"] == outs[1].steps
+
+
+def test_partial_drains_keep_the_order(ctx):
+    """events left undrained on the device move to the host before the next
+    round; drains hand out host-held events first, then device-held ones"""
+    announce("partial drains of PackedEvents across rounds")
+    name, cogs, orc, _, seed, nu, _p = _non_spatial()
+    lin = orc.origins(1.0)
+    ref = oracle_packed(orc, lin, seed, nu)
+    tables = nh.Tables(ctx, cogs)
+    args = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=PERSISTENT, use_graph=True,
+                                   event_slice=nh.EventSlice.Relative(0.25))
+    job = nh.Job(ctx, tables, args, nh.WyHash.seed_from_u64(seed), nu, 0.5, report_dispersal=True)
+    job.add_lineages(lin)
+    got, fin, k = [], False, 0
+    lib = _ffi.lib()
+    while not fin:
+        fin = job.step_round()
+        n = ctypes.c_size_t(0)
+        _ffi.check(lib.nh_job_drain_packed_events(job.handle, None, 0, ctypes.byref(n)))
+        take = n.value // 3 if k % 3 else (0 if k % 2 else n.value)
+        buf = np.empty(take, PK)
+        m = ctypes.c_size_t(0)
+        _ffi.check(lib.nh_job_drain_packed_events(job.handle, _ffi.ptr(buf) if take else None, take,
+                                                  ctypes.byref(m)))
+        got.append(buf[:m.value] if take else buf[:0])
+        k += 1
+    got.append(job.drain_packed_events())
+    job.close()
+    assert k >= 10
+    assert packed_bytes(np.concatenate(got)) == packed_bytes(ref["events"])

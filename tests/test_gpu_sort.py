@@ -70,7 +70,17 @@ CASES = {
     "zero_prior_times": lambda rng: random_events(rng, 3000, n_times=500, n_prior=1),
     "large_with_ties": lambda rng: np.concatenate([random_events(rng, 1 << 19),
                                                    duplicate_runs(rng, 1000, 64)]),
+    # runs that share time and origin but not prior time (ranked in place)
+    "coalesced_runs": lambda rng: coalesced_runs(rng, 3000, 40),
+    # runs longer than the in-place ranking handles (per-field passes)
+    "long_duplicate_runs": lambda rng: duplicate_runs(rng, 3, 5000),
 }
+
+
+def coalesced_runs(rng, n_groups, size):
+    ev = duplicate_runs(rng, n_groups, size)
+    ev["prior_time"] = ev["event_time"] * rng.choice([0.0, 0.25, 0.5, 0.75], len(ev))
+    return ev
 
 
 @pytest.mark.parametrize("device_sized", [0, 1], ids=["eager", "capturable"])
