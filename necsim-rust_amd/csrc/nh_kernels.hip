@@ -1041,7 +1041,7 @@ __global__ void __launch_bounds__(KS_THREADS) k_field_minmax(const E *ev0, SortS
 constexpr uint64_t TIE_RUN_MAX = 1024;
 constexpr uint64_t TIE_WINDOW = 1024;
 constexpr int TIE_SORT_N = 2048; // >= TIE_WINDOW + TIE_RUN_MAX - 1, a power of two
-constexpr int TIE_THREADS = 256;
+constexpr int TIE_THREADS = 512;
 
 // (prior, reference) of every event in time order, into scratch; flags runs
 // whose members differ in origin or target (flags[0]) and runs longer than
@@ -1120,11 +1120,12 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_sort_windows(const unsigned
         s_i[i] = (uint16_t)i;
     }
     __syncthreads();
+    // bitonic network: in stage (k, j), pair q compares i and i + j
     for (int k = 2; k <= TIE_SORT_N; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < TIE_SORT_N; i += TIE_THREADS) {
-                const int l = i ^ j;
-                if (l <= i) continue;
+            for (int q = threadIdx.x; q < TIE_SORT_N / 2; q += TIE_THREADS) {
+                const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                const int l = i + j;
                 const bool ascending = (i & k) == 0;
                 const unsigned long long ti = s_t[i], tl = s_t[l], pi = s_p[i], pl = s_p[l], ri = s_r[i], rl = s_r[l];
                 const uint16_t ii = s_i[i], il = s_i[l];
