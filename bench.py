@@ -485,7 +485,8 @@ def main():
     if a.part == "all":
         # bit-exact comparison with the plain one-launch-per-slice form
         pj = make_job(ctx, tables, cogs, lineages, nh.IndependentArguments(dedup_cache=nh.DedupCache.none(),
-                                                                          form=nh.PLAIN), 42, a.window)
+                                                                          form=nh.PLAIN, plain_batch=1),
+                      42, a.window)
         t = time.perf_counter()
         pev = run_collect(pj)
         pwall = time.perf_counter() - t
@@ -513,8 +514,11 @@ def main():
         dd["autotuned"] = {"step_slice": dtuned.step_slice, "block_size": dtuned.block_size,
                            "blocks_per_cu": dtuned.blocks_per_cu, "inner_steps": dtuned.inner_steps,
                            "exec_fill": dtuned.exec_fill, "tune_s": time.perf_counter() - t}
-        species = {}
-        for name, dargs in (("persistent", dtuned), ("plain", nh.IndependentArguments(form=nh.PLAIN))):
+        species, streams = {}, {}
+        # plain: one host round trip per pass (the reference's launch structure);
+        # plain_batched: the same passes, 16 per host synchronisation
+        for name, dargs in (("persistent", dtuned), ("plain", nh.IndependentArguments(form=nh.PLAIN, plain_batch=1)),
+                            ("plain_batched", nh.IndependentArguments(form=nh.PLAIN))):
             dj = make_job(ctx, tables, cogs, lineages, dargs, 42, a.dedup_window)
             t = time.perf_counter()
             dev = run_collect(dj)
@@ -528,11 +532,15 @@ def main():
             # lineages that a lossy cache lets through repeat one
             species[name] = set(zip(dev["event_time"].tolist(), dev["x"].tolist(), dev["y"].tolist(),
                                     dev["index"].tolist()))
+            streams[name] = (dev.tobytes(), int(do.steps), do.time)
             dd[name] = {"value": do.steps / dwall, "ms": dwall * 1e3, "steps": int(do.steps),
                         "kernel_ms": dms, "launches": int(dk), "remaining_lineages": int(do.n_remaining),
                         "speciation_events": int(do.n_events), "species": len(species[name])}
-        dd["species_sets_identical"] = species["persistent"] == species["plain"]
+        dd["species_sets_identical"] = species["persistent"] == species["plain"] == species["plain_batched"]
         dd["persistent_over_plain_wall"] = dd["plain"]["ms"] / dd["persistent"]["ms"]
+        # bit-exact with dedup on: the batched plain form keeps the reference's queue order
+        dd["plain_batched_events_bit_identical_to_plain"] = streams["plain_batched"] == streams["plain"]
+        dd["plain_batched_over_plain_wall"] = dd["plain"]["ms"] / dd["plain_batched"]["ms"]
         extra["config4_dedup_window"] = {"pause_before": a.dedup_window, **dd}
         log(f"config 4 dedup window: {extra['config4_dedup_window']}")
         extra["dispersal_reporting"] = dispersal_run(ctx, tables, cogs, lineages, args, a.dispersal_window,

@@ -180,6 +180,10 @@ typedef struct {
     int32_t device_event_counter;  /* report_dispersal: the events stay on the device, where an
                                       EventCounterReporter counts them (nh_job_event_counts);
                                       nothing is left to drain */
+    uint32_t plain_batch;          /* plain form: step-slice passes run back to back on the device per
+                                      host synchronisation (pass sizes stay on the device); 1 = one
+                                      host round trip per pass, the reference's launch structure;
+                                      0 = default 16.  Results do not depend on it. */
 } nh_job_params;
 
 typedef struct {

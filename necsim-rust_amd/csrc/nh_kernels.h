@@ -16,7 +16,6 @@ struct Counters {
     unsigned long long n_events;      // events appended (dispersal reporting: this round only)
     unsigned long long steps;         // completed events
     unsigned long long max_time_bits; // max event time (positive f64 bits)
-    unsigned long long overflow;      // plain dedup: a bucket list exceeded the device limit
     unsigned long long n_dead;        // persistent form: lineages that died this round
     unsigned long long ev_overflow;   // dispersal reporting: the round's event buffer was full
     unsigned long long n_ghost;       // dispersal reporting: reserved event slots left unused
@@ -76,7 +75,8 @@ struct PlainArgs {
     DevParams P;
     LineageSoA L;
     const uint32_t *pass;     // lineage slots in processing order
-    unsigned long long m;
+    unsigned long long m;     // pass size, or with m_dev the launch bound
+    const unsigned long long *m_dev; // device pass size (batched passes), else null
     const RoundDesc *round;
     uint8_t *status;          // [m]
     // min speciation samples of the slice, per pass element
@@ -135,12 +135,13 @@ void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, 
 int persistent_max_blocks_per_cu(int kind, int threads, bool dedup, bool report_dispersal);
 void launch_plain(int kind, const PlainArgs &a, int threads, hipStream_t stream);
 
+// m: pass size, or with m_dev the launch bound of a pass whose size is *m_dev
 void launch_dedup_push(const uint8_t *status, const unsigned long long *bucket, uint32_t *head,
-                       uint32_t *link, uint64_t m, hipStream_t stream);
+                       uint32_t *link, uint64_t m, const unsigned long long *m_dev, hipStream_t stream);
 void launch_dedup_resolve(uint8_t *status, const unsigned long long *bucket, const double *s_u,
                           const double *s_t, const uint32_t *s_x, const uint32_t *s_y,
                           const uint32_t *s_i, uint32_t *head, const uint32_t *link,
-                          uint64_t *cache, uint64_t m, Counters *ctr, hipStream_t stream);
+                          uint64_t *cache, uint64_t m, const unsigned long long *m_dev, hipStream_t stream);
 
 // Ordered two-way compaction of pass elements: slots with status ST_SLOW go to
 // out_a (next pass), status ST_FAST to out_b + *b_base (appended).  Counts are
@@ -152,6 +153,18 @@ struct CompactBuffers {
 uint32_t compact_blocks(uint64_t m);
 void launch_compact(const uint8_t *status, const uint32_t *pass, uint64_t m, uint32_t *out_a,
                     uint32_t *out_b, uint64_t b_base, CompactBuffers &cb, hipStream_t stream);
+
+// Batched passes of the plain form: the pass size and the fast-list length
+// live on the device, so passes run back to back without a host round trip.
+struct PlainLoop {
+    unsigned long long m;     // size of the current pass
+    unsigned long long f_len; // lineages already in the fast list
+};
+// one compaction of a pass whose size is loop->m (launch bound m_bound):
+// slow lineages to out_a, fast ones appended to out_b at loop->f_len; then
+// loop->m and loop->f_len advance
+void launch_compact_loop(const uint8_t *status, const uint32_t *pass, uint64_t m_bound, PlainLoop *loop,
+                         uint32_t *out_a, uint32_t *out_b, CompactBuffers &cb, hipStream_t stream);
 
 // Event sort: sorts events[0..n) by the PackedEvent order into out[0..n).
 constexpr int SORT_MINMAX_WORDS = 32;

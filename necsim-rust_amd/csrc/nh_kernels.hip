@@ -573,12 +573,13 @@ __global__ void __launch_bounds__(256) k_plain(PlainArgs A) {
     __shared__ double s_ratio[POISSON_TABLE_N];
     load_ratio(s_ratio, A.P.ratio);
     const unsigned long long q = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long m = A.m_dev ? *A.m_dev : A.m;
     const double level = A.round->level;
     unsigned long long my_steps = 0;
     double my_max = 0.0;
     bool emit = false;
     DevEvent ev;
-    if (q < A.m) {
+    if (q < m) {
         uint32_t slot = A.pass[q];
         // the pop loop's check on the stored (pessimistic) next event time
         // (monolithic/mod.rs:186-196)
@@ -669,33 +670,64 @@ constexpr uint32_t NO_LINK = 0xffffffffu;
 constexpr int DEDUP_LIST_MAX = 64;
 
 __global__ void k_dedup_push(const uint8_t *status, const unsigned long long *bucket, uint32_t *head,
-                             uint32_t *link, uint64_t m) {
+                             uint32_t *link, uint64_t m, const unsigned long long *m_dev) {
     uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= m) return;
+    if (q >= (m_dev ? *m_dev : m)) return;
     if (!(status[q] & ST_HAS_SAMPLE)) return;
     uint32_t old = atomicExch(&head[bucket[q]], (uint32_t)q);
     link[q] = old;
 }
 
+// One insert into a cache bucket held in registers: marks the element a
+// duplicate if the bucket holds an equal sample (cache.rs:39-54).
+__device__ __forceinline__ void bucket_replay(uint64_t &w0, uint64_t &w1, uint64_t &w2, uint64_t &w3, uint32_t e,
+                                              uint8_t *status, const double *s_u, const double *s_t,
+                                              const uint32_t *s_x, const uint32_t *s_y, const uint32_t *s_i) {
+    Sample s{s_u[e], s_t[e], s_x[e], s_y[e], s_i[e]};
+    uint64_t n0 = bits(s.t), n1 = bits(s.u), n2 = ((uint64_t)s.y << 32) | s.x;
+    uint64_t n3 = ((uint64_t)sample_check(s) << 32) | s.i;
+    if (w0 == n0 && w1 == n1 && w2 == n2 && w3 == n3) status[e] = (uint8_t)((status[e] & ~ST_KIND_MASK) | ST_DUP);
+    w0 = n0; w1 = n1; w2 = n2; w3 = n3;
+}
+
 // The last pusher of a bucket owns it: it orders the bucket's inserts by
 // pass position and replays them sequentially against the cache bucket.
+// Lists longer than DEDUP_LIST_MAX (tiny caches) are replayed by repeated
+// minimum searches over the list, O(length^2) but still in pass order.
 __global__ void k_dedup_resolve(uint8_t *status, const unsigned long long *bucket, const double *s_u,
                                 const double *s_t, const uint32_t *s_x, const uint32_t *s_y,
                                 const uint32_t *s_i, uint32_t *head, const uint32_t *link,
-                                uint64_t *cache, uint64_t m, Counters *ctr) {
+                                uint64_t *cache, uint64_t m, const unsigned long long *m_dev) {
     uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= m) return;
+    if (q >= (m_dev ? *m_dev : m)) return;
     if (!(status[q] & ST_HAS_SAMPLE)) return;
     unsigned long long b = bucket[q];
     if (head[b] != (uint32_t)q) return;
     uint32_t list[DEDUP_LIST_MAX];
     int n = 0;
+    bool longer = false;
     for (uint32_t e = (uint32_t)q; e != NO_LINK; e = link[e]) {
         if (n == DEDUP_LIST_MAX) {
-            atomicOr(&ctr->overflow, 1ull);
-            return; // the host replays this pass
+            longer = true;
+            break;
         }
         list[n++] = e;
+    }
+    if (longer) {
+        uint64_t *cb = cache + 4 * b;
+        uint64_t w0 = cb[0], w1 = cb[1], w2 = cb[2], w3 = cb[3];
+        uint32_t prev = 0;
+        for (bool first = true;; first = false) {
+            uint32_t next = NO_LINK;
+            for (uint32_t e = (uint32_t)q; e != NO_LINK; e = link[e])
+                if ((first || e > prev) && e < next) next = e;
+            if (next == NO_LINK) break;
+            bucket_replay(w0, w1, w2, w3, next, status, s_u, s_t, s_x, s_y, s_i);
+            prev = next;
+        }
+        cb[0] = w0; cb[1] = w1; cb[2] = w2; cb[3] = w3;
+        head[b] = NO_LINK;
+        return;
     }
     for (int a = 1; a < n; a++) { // insertion sort by pass position
         uint32_t v = list[a];
@@ -708,16 +740,7 @@ __global__ void k_dedup_resolve(uint8_t *status, const unsigned long long *bucke
     }
     uint64_t *cb = cache + 4 * b;
     uint64_t w0 = cb[0], w1 = cb[1], w2 = cb[2], w3 = cb[3];
-    for (int a = 0; a < n; a++) {
-        uint32_t e = list[a];
-        Sample s{s_u[e], s_t[e], s_x[e], s_y[e], s_i[e]};
-        uint64_t n0 = bits(s.t), n1 = bits(s.u), n2 = ((uint64_t)s.y << 32) | s.x;
-        uint64_t n3 = ((uint64_t)sample_check(s) << 32) | s.i;
-        if (w0 == n0 && w1 == n1 && w2 == n2 && w3 == n3) {
-            status[e] = (uint8_t)((status[e] & ~ST_KIND_MASK) | ST_DUP);
-        }
-        w0 = n0; w1 = n1; w2 = n2; w3 = n3;
-    }
+    for (int a = 0; a < n; a++) bucket_replay(w0, w1, w2, w3, list[a], status, s_u, s_t, s_x, s_y, s_i);
     cb[0] = w0; cb[1] = w1; cb[2] = w2; cb[3] = w3;
     head[b] = NO_LINK;
 }
@@ -729,8 +752,10 @@ constexpr int CT_ITEMS = 8;
 constexpr int CT_TILE = CT_THREADS * CT_ITEMS;
 
 __global__ void __launch_bounds__(CT_THREADS) k_compact_count(const uint8_t *status, uint64_t m,
+                                                              const unsigned long long *m_dev,
                                                               uint32_t *blk_a, uint32_t *blk_b) {
     __shared__ uint32_t ca, cbn;
+    if (m_dev) m = *m_dev;
     if (threadIdx.x == 0) { ca = 0; cbn = 0; }
     __syncthreads();
     uint64_t base = (uint64_t)blockIdx.x * CT_TILE;
@@ -778,11 +803,15 @@ __global__ void __launch_bounds__(1024) k_scan_exclusive(uint32_t *data, uint64_
 }
 
 __global__ void __launch_bounds__(CT_THREADS) k_compact_scatter(const uint8_t *status, const uint32_t *pass,
-                                                                uint64_t m, const uint32_t *off_a,
-                                                                const uint32_t *off_b, uint32_t *out_a,
-                                                                uint32_t *out_b, uint64_t b_base) {
+                                                                uint64_t m, const PlainLoop *loop,
+                                                                const uint32_t *off_a, const uint32_t *off_b,
+                                                                uint32_t *out_a, uint32_t *out_b, uint64_t b_base) {
     __shared__ uint32_t wa[CT_THREADS / 64], wb[CT_THREADS / 64];
     __shared__ uint32_t run_a, run_b;
+    if (loop) {
+        m = loop->m;
+        b_base = loop->f_len;
+    }
     const unsigned lane = lane_id();
     const unsigned wave = threadIdx.x / 64;
     if (threadIdx.x == 0) {
@@ -823,6 +852,12 @@ __global__ void k_compact_totals(const uint32_t *off_a, const uint32_t *off_b, u
                                  unsigned int *counts) {
     counts[0] = off_a[nb];
     counts[1] = off_b[nb];
+}
+
+// the next pass: its size, and the fast lineages appended by this one
+__global__ void k_plain_loop_advance(const uint32_t *off_a, const uint32_t *off_b, uint32_t nb, PlainLoop *loop) {
+    loop->m = off_a[nb];
+    loop->f_len += off_b[nb];
 }
 
 // ---------------------------------------------------------------------------
@@ -1324,18 +1359,19 @@ void launch_plain(int kind, const PlainArgs &a, int threads, hipStream_t stream)
 }
 
 void launch_dedup_push(const uint8_t *status, const unsigned long long *bucket, uint32_t *head,
-                       uint32_t *link, uint64_t m, hipStream_t stream) {
+                       uint32_t *link, uint64_t m, const unsigned long long *m_dev, hipStream_t stream) {
     if (!m) return;
-    hipLaunchKernelGGL(k_dedup_push, dim3(grid_for(m, 256)), dim3(256), 0, stream, status, bucket, head, link, m);
+    hipLaunchKernelGGL(k_dedup_push, dim3(grid_for(m, 256)), dim3(256), 0, stream, status, bucket, head, link, m,
+                       m_dev);
 }
 
 void launch_dedup_resolve(uint8_t *status, const unsigned long long *bucket, const double *s_u,
                           const double *s_t, const uint32_t *s_x, const uint32_t *s_y,
                           const uint32_t *s_i, uint32_t *head, const uint32_t *link,
-                          uint64_t *cache, uint64_t m, Counters *ctr, hipStream_t stream) {
+                          uint64_t *cache, uint64_t m, const unsigned long long *m_dev, hipStream_t stream) {
     if (!m) return;
     hipLaunchKernelGGL(k_dedup_resolve, dim3(grid_for(m, 256)), dim3(256), 0, stream, status, bucket, s_u,
-                       s_t, s_x, s_y, s_i, head, link, cache, m, ctr);
+                       s_t, s_x, s_y, s_i, head, link, cache, m, m_dev);
 }
 
 uint32_t compact_blocks(uint64_t m) { return (uint32_t)((m + CT_TILE - 1) / CT_TILE); }
@@ -1347,12 +1383,28 @@ void launch_compact(const uint8_t *status, const uint32_t *pass, uint64_t m, uin
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, (uint32_t *)cb.counts, 0u, (uint64_t)2);
         return;
     }
-    hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(CT_THREADS), 0, stream, status, m, cb.blk_a, cb.blk_b);
+    hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(CT_THREADS), 0, stream, status, m,
+                       (const unsigned long long *)nullptr, cb.blk_a, cb.blk_b);
     hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, stream, cb.blk_a, (uint64_t)nb);
     hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, stream, cb.blk_b, (uint64_t)nb);
-    hipLaunchKernelGGL(k_compact_scatter, dim3(nb), dim3(CT_THREADS), 0, stream, status, pass, m, cb.blk_a,
-                       cb.blk_b, out_a, out_b, b_base);
+    hipLaunchKernelGGL(k_compact_scatter, dim3(nb), dim3(CT_THREADS), 0, stream, status, pass, m,
+                       (const PlainLoop *)nullptr, cb.blk_a, cb.blk_b, out_a, out_b, b_base);
     hipLaunchKernelGGL(k_compact_totals, dim3(1), dim3(1), 0, stream, cb.blk_a, cb.blk_b, nb, cb.counts);
+}
+
+// Blocks of the launch bound past the device pass size count zero, so the
+// scans over the bound's blocks give the pass's totals.
+void launch_compact_loop(const uint8_t *status, const uint32_t *pass, uint64_t m_bound, PlainLoop *loop,
+                         uint32_t *out_a, uint32_t *out_b, CompactBuffers &cb, hipStream_t stream) {
+    uint32_t nb = compact_blocks(m_bound);
+    if (nb == 0) return;
+    hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(CT_THREADS), 0, stream, status, m_bound, &loop->m, cb.blk_a,
+                       cb.blk_b);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, stream, cb.blk_a, (uint64_t)nb);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, stream, cb.blk_b, (uint64_t)nb);
+    hipLaunchKernelGGL(k_compact_scatter, dim3(nb), dim3(CT_THREADS), 0, stream, status, pass, m_bound,
+                       (const PlainLoop *)loop, cb.blk_a, cb.blk_b, out_a, out_b, (uint64_t)0);
+    hipLaunchKernelGGL(k_plain_loop_advance, dim3(1), dim3(1), 0, stream, cb.blk_a, cb.blk_b, nb, loop);
 }
 
 uint32_t sort_blocks(uint64_t n) { return (uint32_t)((n + RS_TILE - 1) / RS_TILE); }

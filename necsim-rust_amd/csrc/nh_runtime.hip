@@ -340,6 +340,7 @@ struct nh_job {
     unsigned long long *s_bucket = nullptr;
     uint32_t *d_head = nullptr, *d_link = nullptr;
     CompactBuffers cb{};
+    PlainLoop *d_loop = nullptr; // batched passes
 
     // persistent form
     int threads = 256;
@@ -360,6 +361,7 @@ struct nh_job {
         dfree(d_S); dfree(d_F); dfree(d_pass[0]); dfree(d_pass[1]); dfree(d_status);
         dfree(s_u); dfree(s_t); dfree(s_x); dfree(s_y); dfree(s_i); dfree(s_bucket);
         dfree(d_head); dfree(d_link); dfree(cb.blk_a); dfree(cb.blk_b); dfree(cb.counts); dfree(d_blk);
+        dfree(d_loop);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         for (auto &g : graph)
@@ -493,6 +495,7 @@ static int job_start(nh_job *j) {
         NH_TRY(dalloc(&j->cb.blk_a, nb + 1));
         NH_TRY(dalloc(&j->cb.blk_b, nb + 1));
         NH_TRY(dalloc(&j->cb.counts, 2));
+        NH_TRY(dalloc(&j->d_loop, 1));
         if (j->cache_cap) {
             NH_TRY(dalloc(&j->s_u, n));
             NH_TRY(dalloc(&j->s_t, n));
@@ -553,63 +556,26 @@ static void account_kernel_time(nh_job *j) {
     }
 }
 
-// Sequential replay of one pass's dedup inserts on the host, for buckets whose
-// insert list exceeded the device limit (tiny caches only).
-static int host_dedup_replay(nh_job *j, uint64_t m) {
-    hipStream_t st = j->ctx->stream;
-    std::vector<uint8_t> status(m);
-    std::vector<double> u(m), t(m);
-    std::vector<uint32_t> x(m), y(m), i(m);
-    std::vector<unsigned long long> b(m);
-    std::vector<uint64_t> cache(4 * j->cache_cap);
-    NH_HIP(hipMemcpyAsync(status.data(), j->d_status, m, hipMemcpyDeviceToHost, st));
-    NH_HIP(hipMemcpyAsync(u.data(), j->s_u, m * 8, hipMemcpyDeviceToHost, st));
-    NH_HIP(hipMemcpyAsync(t.data(), j->s_t, m * 8, hipMemcpyDeviceToHost, st));
-    NH_HIP(hipMemcpyAsync(x.data(), j->s_x, m * 4, hipMemcpyDeviceToHost, st));
-    NH_HIP(hipMemcpyAsync(y.data(), j->s_y, m * 4, hipMemcpyDeviceToHost, st));
-    NH_HIP(hipMemcpyAsync(i.data(), j->s_i, m * 4, hipMemcpyDeviceToHost, st));
-    NH_HIP(hipMemcpyAsync(b.data(), j->s_bucket, m * 8, hipMemcpyDeviceToHost, st));
-    NH_HIP(hipMemcpyAsync(cache.data(), j->d_cache, cache.size() * 8, hipMemcpyDeviceToHost, st));
-    NH_HIP(hipStreamSynchronize(st));
-    // the device resolved some buckets already; their dup flags are final, but
-    // overflowed buckets were left untouched: replay only those (head != NONE)
-    std::vector<uint32_t> head(j->cache_cap);
-    NH_HIP(hipMemcpy(head.data(), j->d_head, j->cache_cap * 4, hipMemcpyDeviceToHost));
-    for (uint64_t q = 0; q < m; q++) {
-        if (!(status[q] & ST_HAS_SAMPLE)) continue;
-        unsigned long long bk = b[q];
-        if (head[bk] == 0xffffffffu) continue; // resolved on the device
-        Sample s{u[q], t[q], x[q], y[q], i[q]};
-        uint64_t *cb = &cache[4 * bk];
-        uint64_t n0 = bits(s.t), n1 = bits(s.u), n2 = ((uint64_t)s.y << 32) | s.x;
-        uint64_t n3 = ((uint64_t)sample_check(s) << 32) | s.i;
-        if (cb[0] == n0 && cb[1] == n1 && cb[2] == n2 && cb[3] == n3)
-            status[q] = (uint8_t)((status[q] & ~ST_KIND_MASK) | ST_DUP);
-        cb[0] = n0; cb[1] = n1; cb[2] = n2; cb[3] = n3;
-    }
-    NH_HIP(hipMemcpyAsync(j->d_status, status.data(), m, hipMemcpyHostToDevice, st));
-    NH_HIP(hipMemcpyAsync(j->d_cache, cache.data(), cache.size() * 8, hipMemcpyHostToDevice, st));
-    launch_fill_u32(j->d_head, 0xffffffffu, j->cache_cap, st);
-    NH_HIP(hipMemsetAsync(&j->d_ctr->overflow, 0, 8, st));
-    NH_HIP(hipStreamSynchronize(st));
-    return NH_OK;
-}
-
 // one water-level round of the plain form: one launch per step slice over the
-// reference's FIFO queue order (monolithic/mod.rs:183-251)
+// reference's FIFO queue order (monolithic/mod.rs:183-251).  The first pass
+// is ordered on the host; with plain_batch > 1 the later passes run in
+// batches whose pass size and fast-list length stay on the device (one host
+// synchronisation per batch instead of two per pass).
 static int plain_round(nh_job *j) {
     hipStream_t st = j->ctx->stream;
     uint64_t m = j->cur_n;
     const uint32_t *plist = j->d_S;
     uint64_t f_len = 0;
     int pb = 0;
-    for (uint64_t pass = 1; m > 0; pass++) {
+    const uint32_t batch = j->p.plain_batch ? j->p.plain_batch : 16;
+    auto pass_args = [&](uint64_t m_or_bound, const unsigned long long *m_dev) {
         PlainArgs a{};
         a.S = j->tables->dev;
         a.P = j->P;
         a.L = j->lin[0].s;
         a.pass = plist;
-        a.m = m;
+        a.m = m_or_bound;
+        a.m_dev = m_dev;
         a.round = j->d_round;
         a.status = j->d_status;
         a.s_u = j->s_u; a.s_t = j->s_t; a.s_x = j->s_x; a.s_y = j->s_y; a.s_i = j->s_i;
@@ -619,27 +585,62 @@ static int plain_round(nh_job *j) {
         a.ctr = j->d_ctr;
         a.packed = j->d_packed;
         a.packed_cap = j->packed_cap;
+        return a;
+    };
+    auto dedup = [&](uint64_t m_or_bound, const unsigned long long *m_dev) {
+        if (!j->cache_cap) return;
+        launch_dedup_push(j->d_status, j->s_bucket, j->d_head, j->d_link, m_or_bound, m_dev, st);
+        launch_dedup_resolve(j->d_status, j->s_bucket, j->s_u, j->s_t, j->s_x, j->s_y, j->s_i, j->d_head,
+                             j->d_link, j->d_cache, m_or_bound, m_dev, st);
+    };
+    auto overflowed = [&](bool *out) -> int {
+        // a dispersal-reporting lineage that found the event buffer full made no progress
+        *out = false;
+        if (!j->disp) return NH_OK;
+        Counters c;
+        NH_TRY(read_counters(j, c));
+        *out = c.ev_overflow != 0;
+        return NH_OK;
+    };
+    for (uint64_t pass = 1; m > 0; pass++) {
+        uint32_t *next = j->d_pass[pb];
+        if (pass > 1 && batch > 1) {
+            // batched passes: launch bounds from the last known pass size
+            const uint64_t bound = m;
+            PlainLoop loop{m, f_len};
+            NH_HIP(hipMemcpyAsync(j->d_loop, &loop, sizeof(loop), hipMemcpyHostToDevice, st));
+            NH_HIP(hipEventRecord(j->ev0, st));
+            for (uint32_t b = 0; b < batch; b++) {
+                launch_plain(j->tables->kind, pass_args(bound, &j->d_loop->m), j->threads, st);
+                dedup(bound, &j->d_loop->m);
+                launch_compact_loop(j->d_status, plist, bound, j->d_loop, j->d_pass[pb], j->d_F, j->cb, st);
+                NH_HIP(hipGetLastError());
+                plist = j->d_pass[pb];
+                pb ^= 1;
+                j->launches++;
+            }
+            NH_HIP(hipEventRecord(j->ev1, st));
+            NH_HIP(hipMemcpyAsync(&loop, j->d_loop, sizeof(loop), hipMemcpyDeviceToHost, st));
+            NH_HIP(hipStreamSynchronize(st));
+            account_kernel_time(j);
+            m = loop.m;
+            f_len = loop.f_len;
+            pass += batch - 1;
+            bool ov;
+            NH_TRY(overflowed(&ov));
+            if (ov) return NH_OK;
+            continue;
+        }
         NH_HIP(hipEventRecord(j->ev0, st));
-        launch_plain(j->tables->kind, a, j->threads, st);
+        launch_plain(j->tables->kind, pass_args(m, nullptr), j->threads, st);
         NH_HIP(hipGetLastError());
         NH_HIP(hipEventRecord(j->ev1, st));
         j->launches++;
-        if (j->disp) {
-            // a lineage that found the buffer full made no progress: stop here
-            Counters c;
-            NH_TRY(read_counters(j, c));
-            if (c.ev_overflow) return NH_OK;
-        }
-        if (j->cache_cap) {
-            launch_dedup_push(j->d_status, j->s_bucket, j->d_head, j->d_link, m, st);
-            launch_dedup_resolve(j->d_status, j->s_bucket, j->s_u, j->s_t, j->s_x, j->s_y, j->s_i, j->d_head,
-                                 j->d_link, j->d_cache, m, j->d_ctr, st);
-            NH_HIP(hipGetLastError());
-            Counters c;
-            NH_TRY(read_counters(j, c));
-            if (c.overflow) NH_TRY(host_dedup_replay(j, m));
-        }
-        uint32_t *next = j->d_pass[pb];
+        bool ov;
+        NH_TRY(overflowed(&ov));
+        if (ov) return NH_OK;
+        dedup(m, nullptr);
+        NH_HIP(hipGetLastError());
         uint64_t next_m;
         if (pass == 1) {
             // The first pass pops every lineage of the round: skipped ones go to

@@ -93,6 +93,7 @@ class IndependentArguments:
     use_graph: bool = False
     inner_steps: int = 0
     exec_fill: int = 0
+    plain_batch: int = 0  # plain form: passes per host synchronisation (1: one per pass; 0: 16)
     device: int = 0
 
 
@@ -567,7 +568,8 @@ def _job_params(args, rng, speciation_probability, turnover_rate, pause_before=N
                           pause_before=0.0 if pause_before is None else pause_before,
                           resume_after=resume_after, form=args.form, block_size=args.block_size,
                           blocks_per_cu=args.blocks_per_cu, use_graph=1 if args.use_graph else 0,
-                          inner_steps=args.inner_steps, exec_fill=args.exec_fill)
+                          inner_steps=args.inner_steps, exec_fill=args.exec_fill,
+                          plain_batch=args.plain_batch)
 
 
 def autotune(ctx, tables, args: IndependentArguments, rng: WyHash, cogs, sample, window,

@@ -131,6 +131,7 @@ pub struct nh_job_params {
     pub report_dispersal: i32,
     pub event_buffer: u64,
     pub device_event_counter: i32,
+    pub plain_batch: u32,
 }
 
 #[repr(C)]

@@ -123,10 +123,12 @@ def test_config1_dedup_none_bit_exact(config1, form):
     assert O.biodiversity(ev) == 67
 
 
-def test_config1_dedup_plain_bit_exact(config1):
-    announce("config1 dedup Relative(1.0) plain form")
+@pytest.mark.parametrize("plain_batch", [1, 3, 0])
+def test_config1_dedup_plain_bit_exact(config1, plain_batch):
+    """one host round trip per pass (1), and passes batched on the device"""
+    announce(f"config1 dedup Relative(1.0) plain form, plain_batch {plain_batch}")
     cogs, orc, lin, ref = config1
-    out, ev = hip_run(cogs, lin, 42, PLAIN, dedup=1.0)
+    out, ev = hip_run(cogs, lin, 42, PLAIN, dedup=1.0, plain_batch=plain_batch)
     assert_exact(ref[1.0], out, ev, "config1 dedup plain")
 
 
@@ -174,11 +176,12 @@ def test_config2_window_dedup_none(config2, form):
     assert rem.tobytes() == orem.tobytes()
 
 
-def test_config2_window_dedup_plain_exact(config2):
-    announce("config2 pause 40 dedup plain")
+@pytest.mark.parametrize("plain_batch", [1, 0])
+def test_config2_window_dedup_plain_exact(config2, plain_batch):
+    announce(f"config2 pause 40 dedup plain, plain_batch {plain_batch}")
     cogs, orc, lin = config2
     ref = oracle_run(orc, lin, 7, 1e-4, dedup=1.0, pause_before=40.0)
-    out, ev = hip_run(cogs, lin, 7, PLAIN, dedup=1.0, pause_before=40.0)
+    out, ev = hip_run(cogs, lin, 7, PLAIN, dedup=1.0, pause_before=40.0, plain_batch=plain_batch)
     assert_exact(ref, out, ev, "config2 dedup plain")
     # the plain form keeps the reference's queue order of remaining lineages
     assert out.lineages.tobytes() == ref["remaining"].tobytes()
@@ -263,13 +266,16 @@ def test_edge_certain_speciation():
         assert_exact(ref, out, ev, "nu=1")
 
 
-def test_edge_tiny_absolute_cache_host_replay():
-    announce("edge: Absolute(3) dedup cache (bucket lists beyond the device limit)")
+@pytest.mark.parametrize("plain_batch", [1, 0])
+def test_edge_tiny_absolute_cache_long_bucket_lists(plain_batch):
+    """an Absolute(3) cache puts hundreds of inserts into one bucket per pass:
+    the long lists are replayed in pass order by repeated minimum searches"""
+    announce(f"edge: Absolute(3) dedup cache (bucket lists beyond 64 inserts), plain_batch {plain_batch}")
     cogs = nh.NonSpatialScenario((30, 30), 2).new(1e-2)
     orc = O.Scenario.non_spatial(30, 30, 2)
     lin = orc.origins(1.0)
     ref = oracle_run(orc, lin, 5, 1e-2, dedup=("absolute", 3))
-    out, ev = hip_run(cogs, lin, 5, PLAIN, dedup=("absolute", 3))
+    out, ev = hip_run(cogs, lin, 5, PLAIN, dedup=("absolute", 3), plain_batch=plain_batch)
     assert_exact(ref, out, ev, "absolute(3)")
 
 
