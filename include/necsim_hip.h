@@ -184,6 +184,11 @@ typedef struct {
                                       host synchronisation (pass sizes stay on the device); 1 = one
                                       host round trip per pass, the reference's launch structure;
                                       0 = default 16.  Results do not depend on it. */
+    uint32_t coop_search;          /* persistent form: lanes that are not searching evaluate the next
+                                      Poisson windows of those that are (shortens the serial chains of
+                                      sparse waves); 0 = automatic (with dedup, or fewer than 4
+                                      lineages per lane, and not with report_dispersal), 1 = off,
+                                      2 = on.  Results do not depend on it. */
 } nh_job_params;
 
 typedef struct {

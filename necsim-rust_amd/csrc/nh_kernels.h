@@ -68,6 +68,7 @@ struct PersistArgs {
     unsigned exec_fill;            // eighths of the active lanes that must be ready to execute (1..8)
     DevPacked *packed;             // dispersal reporting: the round's PackedEvents, else null
     unsigned long long packed_cap;
+    unsigned coop;                 // lanes that are not searching search ahead for those that are
 };
 
 struct PlainArgs {
@@ -132,7 +133,7 @@ int serve_max_blocks_per_cu(int kind, int threads);
 
 // kernel launchers (all asynchronous on `stream`)
 void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream);
-int persistent_max_blocks_per_cu(int kind, int threads, bool dedup, bool report_dispersal);
+int persistent_max_blocks_per_cu(int kind, int threads, bool dedup, bool report_dispersal, bool coop);
 void launch_plain(int kind, const PlainArgs &a, int threads, hipStream_t stream);
 
 // m: pass size, or with m_dev the launch bound of a pass whose size is *m_dev

@@ -280,6 +280,86 @@ __device__ __forceinline__ void block_flush(unsigned long long steps, double max
 // by an execution per lane would wait for the wave's longest search.
 // DEDUP selects the variant with the in-kernel dedup cache (min speciation
 // sample per slice).
+// ---------------------------------------------------------------------------
+// Cooperative next-event search.  A window's events depend only on the
+// location and the window index (the RNG is primed with both), so any lane
+// can evaluate any lineage's next window.  In each iteration the lanes that
+// are not searching (waiting for their wave to execute, or without a
+// lineage) are spread over the searching ones: helper r takes searcher
+// r mod s at window offset 1 + r / s, where s is the number of searchers.
+// Every lane then evaluates one window, and each searcher takes the first of
+// its group's windows that holds an event, which is the window the
+// sequential search would have stopped at.  Without helpers this is one
+// window per searcher, as before.
+struct CoopLds {
+    uint8_t tab[256 / 64][64];     // per wave: searcher lanes by rank
+    unsigned long long win[256];   // per searcher lane: (offset << 6) | lane of its first window with an event
+    double et[256];                // a helper's event time
+    uint64_t key[256];             // a helper's event key
+};
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int KIND>
+__device__ __forceinline__ void coop_search_window(Lane<KIND> &L, bool srch, const DevParams &P,
+                                                   const double *ratio, CoopLds &c) {
+    const uint64_t ms = __ballot(srch);
+    if (!ms) return;
+    const uint64_t mh = __ballot(!srch);
+    const unsigned lane = lane_id(), wave = threadIdx.x / 64, base = wave * 64;
+    const unsigned ns = (unsigned)__popcll(ms), nh = (unsigned)__popcll(mh);
+    unsigned tgt = lane;
+    uint64_t off = 0;
+    if (nh) {
+        if (srch) {
+            c.tab[wave][__popcll(ms & lanemask_lt())] = (uint8_t)lane;
+            c.win[threadIdx.x] = ~0ull;
+        }
+        wave_sync_lds();
+        if (!srch) {
+            const unsigned r = (unsigned)__popcll(mh & lanemask_lt());
+            tgt = c.tab[wave][r % ns];
+            off = 1 + r / ns;
+        }
+    }
+    const uint64_t id = __shfl(L.id, tgt);
+    const double before = __shfl(L.search_before, tgt);
+    const uint64_t ts = __shfl(L.search_ts, tgt);
+    double et = 0.0;
+    uint64_t key = 0;
+    bool found = poisson_window(P, ratio, id, before, ts + off, et, key);
+    if (nh) {
+        if (!srch && found) {
+            c.et[threadIdx.x] = et;
+            c.key[threadIdx.x] = key;
+            atomicMin(&c.win[base + tgt], (unsigned long long)((off << 6) | lane));
+        }
+        wave_sync_lds();
+    }
+    if (!srch) return;
+    if (!found && nh) {
+        const unsigned long long w = c.win[threadIdx.x];
+        if (w != ~0ull) {
+            const unsigned src = base + (unsigned)(w & 63u);
+            et = c.et[src];
+            key = c.key[src];
+            found = true;
+        }
+    }
+    if (found) {
+        L.next_t = max_after(L.search_before, et);
+        L.next_st = key;
+        L.have_next = true;
+        L.searching = false;
+    } else {
+        const unsigned k = (unsigned)__popcll(ms & lanemask_lt());
+        L.search_ts += 1 + (nh ? nh / ns + (k < nh % ns ? 1u : 0u) : 0u);
+    }
+}
+
 __device__ __forceinline__ bool wave_ready(bool ready, unsigned fill_eighths) {
     uint32_t act = (uint32_t)__popcll(__ballot(true));
     uint32_t rdy = (uint32_t)__popcll(__ballot(ready));
@@ -289,10 +369,14 @@ __device__ __forceinline__ bool wave_ready(bool ready, unsigned fill_eighths) {
 // The dedup variant holds the slice's min speciation sample as well: it gets
 // registers for 3 waves per SIMD instead of 4, which measured faster than
 // spilling.  The DISP variant reports every step as a PackedEvent into the
-// round's buffer (speciations included) instead of speciations only.
-template <int KIND, bool DEDUP, bool DISP>
+// round's buffer (speciations included) instead of speciations only.  The
+// COOP variant searches cooperatively (coop_search_window): it pays off where
+// waves hold few live lineages (coalescing populations, small samples) and
+// costs the dense kernel registers, so it is a variant of its own.
+template <int KIND, bool DEDUP, bool DISP, bool COOP>
 __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(PersistArgs A) {
     __shared__ double s_ratio[POISSON_TABLE_N];
+    __shared__ typename std::conditional<COOP, CoopLds, char>::type s_coop;
     __shared__ unsigned long long s_slots[DISP ? 2 * (256 / 64) : 1]; // per wave: event chunk [next, end)
     volatile unsigned long long *ws = &s_slots[DISP ? 2 * (threadIdx.x / 64) : 0];
     if (DISP && lane_id() == 0) {
@@ -332,11 +416,25 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
             continue;
         }
 
-        // up to inner_steps iterations per lane between wave-level refills
-        if (has) {
+        // up to inner_steps iterations per lane between wave-level refills;
+        // with COOP, lanes whose lineage ended stay in the loop to help the
+        // searches of the others
+        if (has || COOP) {
 #pragma unroll 1
             for (unsigned inner = 0; inner < A.inner_steps; inner++) {
-                if (!L.have_next) lane_search_window<KIND>(L, A.P, s_ratio);
+                if constexpr (COOP) {
+                    if (!__any(has)) break;
+                    const bool srch = has && !L.have_next;
+                    if (srch && !L.searching) {
+                        L.search_before = A.P.min_event_time > L.t_last ? A.P.min_event_time : L.t_last;
+                        L.search_ts = poisson_first_window(A.P, L.search_before);
+                        L.searching = true;
+                    }
+                    coop_search_window<KIND>(L, srch, A.P, s_ratio, s_coop);
+                    if (!has) continue;
+                } else if (!L.have_next) {
+                    lane_search_window<KIND>(L, A.P, s_ratio);
+                }
                 if (!wave_ready(L.have_next, A.exec_fill) || !L.have_next) continue;
                 // Leaving this round (water level) and reaching the step slice
                 // both end the slice: its min speciation sample is inserted like
@@ -352,6 +450,7 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
                             A.lin.flags[slot] = FLAG_DEAD;
                             my_dead++;
                             has = false;
+                            if (COOP) continue;
                             break;
                         }
                     }
@@ -359,6 +458,7 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
                 if (leave) {
                     lane_store<KIND>(L, A.S, A.lin, slot);
                     has = false;
+                    if (COOP) continue;
                     break;
                 }
                 Origin o;
@@ -369,6 +469,7 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
                         A.ctr->ev_overflow = 1;
                         lane_store<KIND>(L, A.S, A.lin, slot);
                         has = false;
+                        if (COOP) continue;
                         break;
                     }
                     o = lane_origin<KIND>(L, A.S);
@@ -384,6 +485,7 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
                     A.lin.flags[slot] = FLAG_DEAD;
                     my_dead++;
                     has = false;
+                    if (COOP) continue;
                     break;
                 }
             }
@@ -1280,45 +1382,52 @@ __global__ void k_reset_counters(Counters *c, bool reset_events) {
 // launchers
 static inline unsigned grid_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-template <bool DEDUP, bool DISP>
+template <bool DEDUP, bool DISP, bool COOP>
 static void launch_persistent_t(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream) {
     switch (kind) {
-    case K_NON_SPATIAL: hipLaunchKernelGGL((k_persistent<K_NON_SPATIAL, DEDUP, DISP>), dim3(blocks), dim3(threads), 0, stream, a); break;
-    case K_ALMOST_INFINITE: hipLaunchKernelGGL((k_persistent<K_ALMOST_INFINITE, DEDUP, DISP>), dim3(blocks), dim3(threads), 0, stream, a); break;
-    case K_SEPARABLE: hipLaunchKernelGGL((k_persistent<K_SEPARABLE, DEDUP, DISP>), dim3(blocks), dim3(threads), 0, stream, a); break;
-    default: hipLaunchKernelGGL((k_persistent<K_PACKED, DEDUP, DISP>), dim3(blocks), dim3(threads), 0, stream, a); break;
+    case K_NON_SPATIAL: hipLaunchKernelGGL((k_persistent<K_NON_SPATIAL, DEDUP, DISP, COOP>), dim3(blocks), dim3(threads), 0, stream, a); break;
+    case K_ALMOST_INFINITE: hipLaunchKernelGGL((k_persistent<K_ALMOST_INFINITE, DEDUP, DISP, COOP>), dim3(blocks), dim3(threads), 0, stream, a); break;
+    case K_SEPARABLE: hipLaunchKernelGGL((k_persistent<K_SEPARABLE, DEDUP, DISP, COOP>), dim3(blocks), dim3(threads), 0, stream, a); break;
+    default: hipLaunchKernelGGL((k_persistent<K_PACKED, DEDUP, DISP, COOP>), dim3(blocks), dim3(threads), 0, stream, a); break;
     }
 }
 
+// dispersal reporting has no cooperative variant (its rounds are dense)
 void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream) {
     if (a.packed) {
-        if (a.cache) launch_persistent_t<true, true>(kind, a, blocks, threads, stream);
-        else launch_persistent_t<false, true>(kind, a, blocks, threads, stream);
+        if (a.cache) launch_persistent_t<true, true, false>(kind, a, blocks, threads, stream);
+        else launch_persistent_t<false, true, false>(kind, a, blocks, threads, stream);
+    } else if (a.coop) {
+        if (a.cache) launch_persistent_t<true, false, true>(kind, a, blocks, threads, stream);
+        else launch_persistent_t<false, false, true>(kind, a, blocks, threads, stream);
     } else {
-        if (a.cache) launch_persistent_t<true, false>(kind, a, blocks, threads, stream);
-        else launch_persistent_t<false, false>(kind, a, blocks, threads, stream);
+        if (a.cache) launch_persistent_t<true, false, false>(kind, a, blocks, threads, stream);
+        else launch_persistent_t<false, false, false>(kind, a, blocks, threads, stream);
     }
 }
 
-template <bool DEDUP, bool DISP> static int persistent_max_blocks_per_cu_t(int kind, int threads) {
+template <bool DEDUP, bool DISP, bool COOP> static int persistent_max_blocks_per_cu_t(int kind, int threads) {
     int n = 0;
     switch (kind) {
-    case K_NON_SPATIAL: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_NON_SPATIAL, DEDUP, DISP>, threads, 0); break;
-    case K_ALMOST_INFINITE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_ALMOST_INFINITE, DEDUP, DISP>, threads, 0); break;
-    case K_SEPARABLE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_SEPARABLE, DEDUP, DISP>, threads, 0); break;
-    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_PACKED, DEDUP, DISP>, threads, 0); break;
+    case K_NON_SPATIAL: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_NON_SPATIAL, DEDUP, DISP, COOP>, threads, 0); break;
+    case K_ALMOST_INFINITE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_ALMOST_INFINITE, DEDUP, DISP, COOP>, threads, 0); break;
+    case K_SEPARABLE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_SEPARABLE, DEDUP, DISP, COOP>, threads, 0); break;
+    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<K_PACKED, DEDUP, DISP, COOP>, threads, 0); break;
     }
     return n > 0 ? n : 1;
 }
 
 // resident blocks per CU of the persistent kernel variant with / without
-// dedup and dispersal reporting
-int persistent_max_blocks_per_cu(int kind, int threads, bool dedup, bool report_dispersal) {
+// dedup, dispersal reporting and the cooperative search
+int persistent_max_blocks_per_cu(int kind, int threads, bool dedup, bool report_dispersal, bool coop) {
     if (report_dispersal)
-        return dedup ? persistent_max_blocks_per_cu_t<true, true>(kind, threads)
-                     : persistent_max_blocks_per_cu_t<false, true>(kind, threads);
-    return dedup ? persistent_max_blocks_per_cu_t<true, false>(kind, threads)
-                 : persistent_max_blocks_per_cu_t<false, false>(kind, threads);
+        return dedup ? persistent_max_blocks_per_cu_t<true, true, false>(kind, threads)
+                     : persistent_max_blocks_per_cu_t<false, true, false>(kind, threads);
+    if (coop)
+        return dedup ? persistent_max_blocks_per_cu_t<true, false, true>(kind, threads)
+                     : persistent_max_blocks_per_cu_t<false, false, true>(kind, threads);
+    return dedup ? persistent_max_blocks_per_cu_t<true, false, false>(kind, threads)
+                 : persistent_max_blocks_per_cu_t<false, false, false>(kind, threads);
 }
 
 void launch_serve(int kind, const ServeArgs &a, int blocks, int threads, hipStream_t stream) {

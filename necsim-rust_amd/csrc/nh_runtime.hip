@@ -345,6 +345,7 @@ struct nh_job {
     // persistent form
     int threads = 256;
     int blocks = 0;
+    bool coop = false; // cooperative next-event search
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr}; // persistent round, input buffer 0 / 1
@@ -379,6 +380,7 @@ extern "C" int nh_job_create(nh_ctx *ctx, const nh_tables *tables, const nh_job_
     if (!(p->delta_t > 0.0) || std::isinf(p->delta_t)) return set_error(NH_E_INVALID, "delta_t must be positive");
     if (p->step_slice == 0) return set_error(NH_E_INVALID, "step_slice must be positive");
     if (p->exec_fill > 8) return set_error(NH_E_INVALID, "exec_fill is a number of eighths (0..8)");
+    if (p->coop_search > 2) return set_error(NH_E_INVALID, "coop_search is 0 (automatic), 1 (off) or 2 (on)");
     if (p->device_event_counter && !p->report_dispersal)
         return set_error(NH_E_INVALID, "device_event_counter counts the events of a report_dispersal job");
     if (p->form != NH_FORM_PLAIN && p->form != NH_FORM_PERSISTENT) return set_error(NH_E_INVALID, "unknown form");
@@ -479,8 +481,17 @@ static int job_start(nh_job *j) {
     if (p.form == NH_FORM_PERSISTENT) {
         NH_TRY(j->lin[1].alloc(n));
         NH_TRY(dalloc(&j->d_blk, soa_compact_blocks(n) + 1));
-        int per_cu = persistent_max_blocks_per_cu(j->tables->kind, j->threads, j->cache_cap != 0, j->disp);
+        int per_cu = persistent_max_blocks_per_cu(j->tables->kind, j->threads, j->cache_cap != 0, j->disp, false);
         if (p.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)p.blocks_per_cu);
+        // the cooperative search pays off where waves will hold few live
+        // lineages: coalescing populations (dedup) and small samples
+        const uint64_t lanes = (uint64_t)per_cu * j->ctx->cus * (uint64_t)j->threads;
+        if (p.coop_search == 2) j->coop = true;
+        else if (p.coop_search == 0) j->coop = !j->disp && (j->cache_cap != 0 || n < 4 * lanes);
+        if (j->coop) {
+            per_cu = persistent_max_blocks_per_cu(j->tables->kind, j->threads, j->cache_cap != 0, false, true);
+            if (p.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)p.blocks_per_cu);
+        }
         j->blocks = per_cu * (int)j->ctx->cus;
     } else {
         NH_TRY(dalloc(&j->d_S, n));
@@ -718,6 +729,7 @@ static PersistArgs persist_args(nh_job *j, int cur) {
     a.packed = j->d_packed;
     a.packed_cap = j->packed_cap;
     a.inner_steps = j->p.inner_steps ? j->p.inner_steps : 16;
+    a.coop = j->coop ? 1u : 0u;
     a.exec_fill = j->p.exec_fill ? j->p.exec_fill : 5;
     return a;
 }
@@ -1000,7 +1012,7 @@ extern "C" int nh_autotune(nh_ctx *ctx, const nh_tables *tables, const nh_job_pa
     for (uint32_t threads : {128u, 256u}) {
         int max_per_cu = persistent_max_blocks_per_cu(tables->kind, (int)threads,
                                                       base->dedup_mode != NH_DEDUP_NONE,
-                                                      base->report_dispersal != 0);
+                                                      base->report_dispersal != 0, false);
         std::vector<uint32_t> residency = {(uint32_t)max_per_cu};
         if (max_per_cu >= 2) residency.push_back((uint32_t)(max_per_cu / 2));
         for (uint32_t per_cu : residency) {

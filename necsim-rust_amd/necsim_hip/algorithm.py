@@ -94,6 +94,7 @@ class IndependentArguments:
     inner_steps: int = 0
     exec_fill: int = 0
     plain_batch: int = 0  # plain form: passes per host synchronisation (1: one per pass; 0: 16)
+    coop_search: int = 0  # persistent form: cooperative next-event search (0: automatic, 1: off, 2: on)
     device: int = 0
 
 
@@ -569,7 +570,7 @@ def _job_params(args, rng, speciation_probability, turnover_rate, pause_before=N
                           resume_after=resume_after, form=args.form, block_size=args.block_size,
                           blocks_per_cu=args.blocks_per_cu, use_graph=1 if args.use_graph else 0,
                           inner_steps=args.inner_steps, exec_fill=args.exec_fill,
-                          plain_batch=args.plain_batch)
+                          plain_batch=args.plain_batch, coop_search=args.coop_search)
 
 
 def autotune(ctx, tables, args: IndependentArguments, rng: WyHash, cogs, sample, window,

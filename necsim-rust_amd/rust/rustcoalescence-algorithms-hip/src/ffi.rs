@@ -132,6 +132,7 @@ pub struct nh_job_params {
     pub event_buffer: u64,
     pub device_event_counter: i32,
     pub plain_batch: u32,
+    pub coop_search: u32,
 }
 
 #[repr(C)]

@@ -114,11 +114,12 @@ def config1():
     return cogs, orc, lin, {d: oracle_run(orc, lin, 42, 1e-3, dedup=d) for d in ("none", 1.0)}
 
 
-@pytest.mark.parametrize("form", [PLAIN, PERSISTENT])
-def test_config1_dedup_none_bit_exact(config1, form):
-    announce(f"config1 dedup None form {form}")
+# persistent form: automatic, plain (1) and cooperative (2) next-event search
+@pytest.mark.parametrize("form,coop", [(PLAIN, 0), (PERSISTENT, 0), (PERSISTENT, 1), (PERSISTENT, 2)])
+def test_config1_dedup_none_bit_exact(config1, form, coop):
+    announce(f"config1 dedup None form {form} coop_search {coop}")
     cogs, orc, lin, ref = config1
-    out, ev = hip_run(cogs, lin, 42, form, dedup="none")
+    out, ev = hip_run(cogs, lin, 42, form, dedup="none", coop_search=coop)
     assert_exact(ref["none"], out, ev, "config1 none")
     assert O.biodiversity(ev) == 67
 
@@ -132,10 +133,11 @@ def test_config1_dedup_plain_bit_exact(config1, plain_batch):
     assert_exact(ref[1.0], out, ev, "config1 dedup plain")
 
 
-def test_config1_dedup_persistent_invariants(config1):
-    announce("config1 dedup Relative(1.0) persistent form")
+@pytest.mark.parametrize("coop", [1, 2])
+def test_config1_dedup_persistent_invariants(config1, coop):
+    announce(f"config1 dedup Relative(1.0) persistent form coop_search {coop}")
     cogs, orc, lin, ref = config1
-    out, ev = hip_run(cogs, lin, 42, PERSISTENT, dedup=1.0)
+    out, ev = hip_run(cogs, lin, 42, PERSISTENT, dedup=1.0, coop_search=coop)
     assert_invariants(ev, ref["none"]["events"], ref[1.0]["events"], "config1 dedup persistent")
     assert out.steps <= ref["none"]["steps"]
 
@@ -164,12 +166,12 @@ def config2():
     return cogs, orc, lin
 
 
-@pytest.mark.parametrize("form", [PLAIN, PERSISTENT])
-def test_config2_window_dedup_none(config2, form):
-    announce(f"config2 pause 40 dedup None form {form}")
+@pytest.mark.parametrize("form,coop", [(PLAIN, 0), (PERSISTENT, 1), (PERSISTENT, 2)])
+def test_config2_window_dedup_none(config2, form, coop):
+    announce(f"config2 pause 40 dedup None form {form} coop_search {coop}")
     cogs, orc, lin = config2
     ref = oracle_run(orc, lin, 7, 1e-4, dedup="none", pause_before=40.0)
-    out, ev = hip_run(cogs, lin, 7, form, dedup="none", pause_before=40.0)
+    out, ev = hip_run(cogs, lin, 7, form, dedup="none", pause_before=40.0, coop_search=coop)
     assert_exact(ref, out, ev, "config2 window")
     rem = np.sort(out.lineages, order=["global_reference"])
     orem = np.sort(ref["remaining"], order=["global_reference"])
@@ -210,15 +212,15 @@ def map64():
     return hip.new(1e-3), orc, orc.origins(1.0)
 
 
-@pytest.mark.parametrize("form", [PLAIN, PERSISTENT])
-def test_map64_window(map64, form):
-    announce(f"64x64 map pause 30 form {form}")
+@pytest.mark.parametrize("form,coop", [(PLAIN, 0), (PERSISTENT, 1), (PERSISTENT, 2)])
+def test_map64_window(map64, form, coop):
+    announce(f"64x64 map pause 30 form {form} coop_search {coop}")
     cogs, orc, lin = map64
     ref = oracle_run(orc, lin, 9, 1e-3, dedup="none", pause_before=30.0)
-    out, ev = hip_run(cogs, lin, 9, form, dedup="none", pause_before=30.0)
+    out, ev = hip_run(cogs, lin, 9, form, dedup="none", pause_before=30.0, coop_search=coop)
     assert_exact(ref, out, ev, "map64 window")
     refd = oracle_run(orc, lin, 9, 1e-3, dedup=1.0, pause_before=30.0)
-    outd, evd = hip_run(cogs, lin, 9, form, dedup=1.0, pause_before=30.0)
+    outd, evd = hip_run(cogs, lin, 9, form, dedup=1.0, pause_before=30.0, coop_search=coop)
     if form == PLAIN:
         assert_exact(refd, outd, evd, "map64 dedup plain")
     else:
