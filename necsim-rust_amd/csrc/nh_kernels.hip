@@ -303,7 +303,9 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int KIND>
+// LANE_SEED: lanes may hold lineages of different simulations (serving), so
+// a helper also takes its searcher's seed.
+template <int KIND, bool LANE_SEED = false>
 __device__ __forceinline__ void coop_search_window(Lane<KIND> &L, bool srch, const DevParams &P,
                                                    const double *ratio, CoopLds &c) {
     const uint64_t ms = __ballot(srch);
@@ -330,7 +332,14 @@ __device__ __forceinline__ void coop_search_window(Lane<KIND> &L, bool srch, con
     const uint64_t ts = __shfl(L.search_ts, tgt);
     double et = 0.0;
     uint64_t key = 0;
-    bool found = poisson_window(P, ratio, id, before, ts + off, et, key);
+    bool found;
+    if constexpr (LANE_SEED) {
+        DevParams Pt = P;
+        Pt.seed = __shfl(P.seed, tgt);
+        found = poisson_window(Pt, ratio, id, before, ts + off, et, key);
+    } else {
+        found = poisson_window(P, ratio, id, before, ts + off, et, key);
+    }
     if (nh) {
         if (!srch && found) {
             c.et[threadIdx.x] = et;
@@ -507,6 +516,7 @@ template <int KIND>
 __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
     __shared__ double s_ratio[POISSON_TABLE_N];
     __shared__ unsigned long long s_base[SERVE_MAX_REQS];
+    __shared__ CoopLds s_coop; // cooperative next-event search: requests are often small
     load_ratio(s_ratio, A.P.ratio);
     for (int r = threadIdx.x; r < A.n_reqs; r += blockDim.x) s_base[r] = A.reqs[r].work_base;
     __syncthreads();
@@ -569,11 +579,20 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
             if (q.exhausted) break;
             continue;
         }
-        if (has) {
-            const ServeReq &R = A.reqs[req];
+        {
+            // every lane iterates: those without a lineage help the searches
+            const ServeReq &R = A.reqs[req < 0 ? 0 : req];
 #pragma unroll 1
             for (int inner = 0; inner < 4; inner++) {
-                if (!L.have_next) lane_search_window<KIND>(L, P, s_ratio);
+                if (!__any(has)) break;
+                const bool srch = has && !L.have_next;
+                if (srch && !L.searching) {
+                    L.search_before = P.min_event_time > L.t_last ? P.min_event_time : L.t_last;
+                    L.search_ts = poisson_first_window(P, L.search_before);
+                    L.searching = true;
+                }
+                coop_search_window<KIND, true>(L, srch, P, s_ratio, s_coop);
+                if (!has) continue;
                 if (!wave_ready(L.have_next, 5) || !L.have_next) continue;
                 bool leave = L.next_t >= level || (A.tick_steps && tick >= A.tick_steps);
                 if (leave || slice_steps >= P.step_slice) {
@@ -585,13 +604,13 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
                         R.lin.flags[slot] = FLAG_DEAD;
                         acc_dead++;
                         has = false;
-                        break;
+                        continue;
                     }
                     if (leave) {
                         lane_store<KIND>(L, A.S, R.lin, slot);
                         if (L.next_t < level) R.ctr->n_below = 1; // the round continues next tick
                         has = false;
-                        break;
+                        continue;
                     }
                 }
                 DevEvent ev;
@@ -605,7 +624,7 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
                     R.lin.flags[slot] = FLAG_DEAD;
                     acc_dead++;
                     has = false;
-                    break;
+                    continue;
                 }
             }
         }
