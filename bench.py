@@ -277,6 +277,41 @@ def dispersal_run(ctx, tables, cogs, lineages, args, window, drain_window, seed=
     return out
 
 
+def task_list_run(ctx, tables, cogs, lineages, pargs, window, seed=42):
+    """The persistent form against the reference CUDA algorithm's launch
+    structure (cuda/src/parallelisation/monolithic.rs:222-318, rebuilt on
+    this library's kernels as the task-list form): a task list of grid x block
+    lineages per launch, uploaded and downloaded around it, dedup cache and
+    slow/fast queues on the host; its defaults are 64 x 64 tasks and a step
+    slice of 150 (cuda/src/arguments.rs:124-138).  DedupCache::None, where
+    every form's event stream is bit-exact, on a window short enough for the
+    task-list form to finish in seconds."""
+    out = {"window": window}
+    streams = {}
+    shapes = (("persistent", pargs),
+              ("task_list_64x64", nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=nh.TASK_LIST,
+                                                          step_slice=150, block_size=64, task_list_size=4096)),
+              ("task_list_gpu_filling", nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=nh.TASK_LIST,
+                                                                step_slice=150, block_size=256,
+                                                                task_list_size=256 * 1024)))
+    for name, args in shapes:
+        job = make_job(ctx, tables, cogs, lineages, args, seed, window)
+        t = time.perf_counter()
+        ev = run_collect(job)
+        wall = time.perf_counter() - t
+        o = job.outcome()
+        ms, k = job.kernel_time()
+        job.close()
+        streams[name] = (ev.tobytes(), int(o.steps), o.time)
+        out[name] = {"value": o.steps / wall, "ms": wall * 1e3, "steps": int(o.steps), "launches": int(k),
+                     "kernel_ms": ms}
+        log(f"task-list comparison {name}: {out[name]}")
+    for name in ("task_list_64x64", "task_list_gpu_filling"):
+        out[f"persistent_over_{name}"] = out["persistent"]["value"] / out[name]["value"]
+        out[f"{name}_events_bit_identical_to_persistent"] = streams[name] == streams["persistent"]
+    return out
+
+
 def run_collect(job):
     fin = False
     chunks = []
@@ -350,6 +385,8 @@ def main():
                          "dispersal: the dispersal-reporting windows only")
     ap.add_argument("--dispersal-window", type=float, default=200.0,
                     help="pause_before of the dispersal-reporting runs on the device")
+    ap.add_argument("--task-window", type=float, default=200.0,
+                    help="pause_before of the persistent vs task-list (reference CUDA structure) comparison")
     ap.add_argument("--drain-window", type=float, default=20.0,
                     help="pause_before of the dispersal-reporting run drained to the host")
     ap.add_argument("--no-extra", action="store_true", help="same as --part headline")
@@ -503,6 +540,7 @@ def main():
                                "events_bit_identical_to_persistent": bool(same)}
         extra["persistent_over_plain"] = value / (po.steps / pwall)
         log(f"plain: {po.steps} steps in {pwall:.3f} s -> {po.steps / pwall:.4e} steps/s; identical={same}")
+        extra["task_list_window"] = task_list_run(ctx, tables, cogs, lineages, args, a.task_window)
 
         # config 4 with the reference's default dedup cache (Relative(1.0)):
         # coalesced lineages are dropped, the window is mostly a coalescence

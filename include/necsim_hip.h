@@ -143,6 +143,13 @@ enum {
      * DedupCache::None; with dedup on, which of two coalesced duplicates
      * survives may differ, while species and unique events do not) */
     NH_FORM_PERSISTENT = 1,
+    /* the reference CUDA algorithm's launch structure
+     * (rustcoalescence/algorithms/cuda/src/parallelisation/monolithic.rs:222-318):
+     * a task list of task_list_size lineages per launch, uploaded before and
+     * downloaded after it, with the dedup cache and the slow/fast queues on the
+     * host (bit-exact with DedupCache::None; with dedup on it keeps that loop's
+     * queue order, which is not the CPU algorithm's) */
+    NH_FORM_TASK_LIST = 2,
 };
 
 /* IndependentArguments (rustcoalescence/algorithms/independent/src/arguments.rs:79-146)
@@ -189,6 +196,8 @@ typedef struct {
                                       sparse waves); 0 = automatic (with dedup, or fewer than 4
                                       lineages per lane, and not with report_dispersal), 1 = off,
                                       2 = on.  Results do not depend on it. */
+    uint32_t task_list_size;       /* NH_FORM_TASK_LIST: lineages per launch; 0 = 4096, the reference
+                                      CUDA algorithm's default grid x block of 64 x 64 */
 } nh_job_params;
 
 typedef struct {

@@ -746,14 +746,15 @@ __global__ void __launch_bounds__(256) k_plain(PlainArgs A) {
                 lane_store<KIND>(L, A.S, A.L, slot);
                 if (L.ms_some) {
                     st |= ST_HAS_SAMPLE;
-                    if (A.cache_cap) {
+                    if (A.s_u) { // the slice's sample for the dedup cache (device or host)
                         Sample s = lane_sample<KIND>(L, A.S);
                         A.s_u[q] = s.u;
                         A.s_t[q] = s.t;
                         A.s_x[q] = s.x;
                         A.s_y[q] = s.y;
                         A.s_i[q] = s.i;
-                        A.s_bucket[q] = speciation_sample_hash(s.u, s.t, s.x, s.y, s.i) % A.cache_cap;
+                        if (A.cache_cap)
+                            A.s_bucket[q] = speciation_sample_hash(s.u, s.t, s.x, s.y, s.i) % A.cache_cap;
                     }
                 }
             }

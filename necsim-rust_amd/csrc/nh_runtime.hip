@@ -3,6 +3,7 @@
 // necsim/impls/no-std/src/parallelisation/independent/monolithic/mod.rs:43-270
 // driven on the host, every per-lineage step on the device).
 #include <algorithm>
+#include <deque>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -342,6 +343,20 @@ struct nh_job {
     CompactBuffers cb{};
     PlainLoop *d_loop = nullptr; // batched passes
 
+    // task-list form: the slow / fast queues and the dedup cache on the host
+    struct Queued {
+        nh_lineage l;
+        double next; // pessimistic next event time (monolithic/mod.rs:94-103)
+    };
+    std::deque<Queued> slow_q, fast_q;
+    struct HostSample {
+        bool some;
+        double u, t;
+        uint32_t x, y, i;
+    };
+    std::vector<HostSample> host_cache;
+    uint64_t task_cap = 0;
+
     // persistent form
     int threads = 256;
     int blocks = 0;
@@ -383,7 +398,10 @@ extern "C" int nh_job_create(nh_ctx *ctx, const nh_tables *tables, const nh_job_
     if (p->coop_search > 2) return set_error(NH_E_INVALID, "coop_search is 0 (automatic), 1 (off) or 2 (on)");
     if (p->device_event_counter && !p->report_dispersal)
         return set_error(NH_E_INVALID, "device_event_counter counts the events of a report_dispersal job");
-    if (p->form != NH_FORM_PLAIN && p->form != NH_FORM_PERSISTENT) return set_error(NH_E_INVALID, "unknown form");
+    if (p->form != NH_FORM_PLAIN && p->form != NH_FORM_PERSISTENT && p->form != NH_FORM_TASK_LIST)
+        return set_error(NH_E_INVALID, "unknown form");
+    if (p->form == NH_FORM_TASK_LIST && p->report_dispersal)
+        return set_error(NH_E_INVALID, "the task-list form reports speciation events only");
     if (p->dedup_mode < NH_DEDUP_NONE || p->dedup_mode > NH_DEDUP_ABSOLUTE) return set_error(NH_E_INVALID, "dedup mode");
     if (p->dedup_mode == NH_DEDUP_RELATIVE && !(p->dedup_factor > 0.0)) return set_error(NH_E_INVALID, "dedup factor");
     if (p->event_slice_mode == NH_SLICE_RELATIVE && !(p->event_slice_factor > 0.0))
@@ -472,13 +490,36 @@ static int job_start(nh_job *j) {
     NH_HIP(hipEventCreate(&j->ev0));
     NH_HIP(hipEventCreate(&j->ev1));
 
-    NH_TRY(j->lin[0].alloc(n));
-    NH_TRY(upload_lineages(j->pending, j->lin[0], st));
+    if (p.form == NH_FORM_TASK_LIST) {
+        // the lineages stay on the host, in the slow queue with their
+        // last_event_time as the pessimistic next event time
+        j->task_cap = p.task_list_size ? p.task_list_size : 4096;
+        NH_TRY(j->lin[0].alloc(j->task_cap));
+        for (auto &l : j->pending) j->slow_q.push_back({l, l.last_event_time});
+    } else {
+        NH_TRY(j->lin[0].alloc(n));
+        NH_TRY(upload_lineages(j->pending, j->lin[0], st));
+    }
     j->cur = 0;
     j->cur_n = n;
     j->alive = n;
-    j->threads = p.block_size ? (int)p.block_size : 256;
-    if (p.form == NH_FORM_PERSISTENT) {
+    j->threads = p.block_size ? (int)p.block_size : (p.form == NH_FORM_TASK_LIST ? 64 : 256);
+    if (p.form == NH_FORM_TASK_LIST) {
+        const uint64_t T = j->task_cap;
+        NH_TRY(dalloc(&j->d_status, T));
+        NH_TRY(dalloc(&j->d_S, T));
+        std::vector<uint32_t> iota(T);
+        for (uint64_t k = 0; k < T; k++) iota[k] = (uint32_t)k;
+        NH_HIP(hipMemcpy(j->d_S, iota.data(), T * 4, hipMemcpyHostToDevice));
+        if (j->cache_cap) {
+            NH_TRY(dalloc(&j->s_u, T));
+            NH_TRY(dalloc(&j->s_t, T));
+            NH_TRY(dalloc(&j->s_x, T));
+            NH_TRY(dalloc(&j->s_y, T));
+            NH_TRY(dalloc(&j->s_i, T));
+            j->host_cache.assign(j->cache_cap, nh_job::HostSample{false, 0.0, 0.0, 0, 0, 0});
+        }
+    } else if (p.form == NH_FORM_PERSISTENT) {
         NH_TRY(j->lin[1].alloc(n));
         NH_TRY(dalloc(&j->d_blk, soa_compact_blocks(n) + 1));
         int per_cu = persistent_max_blocks_per_cu(j->tables->kind, j->threads, j->cache_cap != 0, j->disp, false);
@@ -702,6 +743,109 @@ static int plain_round(nh_job *j) {
     return NH_OK;
 }
 
+// One water-level round of the task-list form, the reference CUDA algorithm's
+// host loop (cuda/src/parallelisation/monolithic.rs:222-318): fill a task
+// list from the front of the slow queue (lineages already at the level go to
+// the fast queue), upload it, run one step slice per task, download the
+// tasks' state, next event times and slice samples, then insert the samples
+// into the dedup cache and requeue the surviving tasks, in task order.
+static int task_list_round(nh_job *j) {
+    hipStream_t st = j->ctx->stream;
+    const double level = j->level_time;
+    const uint64_t T = j->task_cap;
+    std::vector<nh_lineage> tasks;
+    std::vector<uint64_t> ref(T), nst(T, 0);
+    std::vector<double> t(T), nt(T), su, stt;
+    std::vector<uint32_t> x(T), y(T), idx(T), sx, sy, si;
+    std::vector<uint8_t> fl(T, 0), status(T);
+    if (j->cache_cap) {
+        su.resize(T); stt.resize(T); sx.resize(T); sy.resize(T); si.resize(T);
+    }
+    while (!j->slow_q.empty()) {
+        tasks.clear();
+        while (tasks.size() < T && !j->slow_q.empty()) {
+            nh_job::Queued q = j->slow_q.front();
+            j->slow_q.pop_front();
+            if (q.next < level) {
+                tasks.push_back(q.l);
+                nt[tasks.size() - 1] = q.next;
+            } else {
+                j->fast_q.push_back(q);
+            }
+        }
+        const uint64_t m = tasks.size();
+        if (!m) break;
+        for (uint64_t k = 0; k < m; k++) {
+            ref[k] = tasks[k].global_reference;
+            t[k] = tasks[k].last_event_time;
+            x[k] = tasks[k].x;
+            y[k] = tasks[k].y;
+            idx[k] = tasks[k].index;
+        }
+        const LineageSoA &s = j->lin[0].s;
+        NH_HIP(hipMemcpyAsync(s.ref, ref.data(), m * 8, hipMemcpyHostToDevice, st));
+        NH_HIP(hipMemcpyAsync(s.t_last, t.data(), m * 8, hipMemcpyHostToDevice, st));
+        NH_HIP(hipMemcpyAsync(s.next_t, nt.data(), m * 8, hipMemcpyHostToDevice, st));
+        NH_HIP(hipMemcpyAsync(s.next_st, nst.data(), m * 8, hipMemcpyHostToDevice, st));
+        NH_HIP(hipMemcpyAsync(s.x, x.data(), m * 4, hipMemcpyHostToDevice, st));
+        NH_HIP(hipMemcpyAsync(s.y, y.data(), m * 4, hipMemcpyHostToDevice, st));
+        NH_HIP(hipMemcpyAsync(s.idx, idx.data(), m * 4, hipMemcpyHostToDevice, st));
+        NH_HIP(hipMemcpyAsync(s.flags, fl.data(), m, hipMemcpyHostToDevice, st));
+        PlainArgs a{};
+        a.S = j->tables->dev;
+        a.P = j->P;
+        a.L = s;
+        a.pass = j->d_S;
+        a.m = m;
+        a.round = j->d_round;
+        a.status = j->d_status;
+        a.s_u = j->s_u; a.s_t = j->s_t; a.s_x = j->s_x; a.s_y = j->s_y; a.s_i = j->s_i;
+        a.cache_cap = 0; // the cache is the host's
+        a.events = j->d_events;
+        a.ctr = j->d_ctr;
+        NH_HIP(hipEventRecord(j->ev0, st));
+        launch_plain(j->tables->kind, a, j->threads, st);
+        NH_HIP(hipGetLastError());
+        NH_HIP(hipEventRecord(j->ev1, st));
+        j->launches++;
+        NH_HIP(hipMemcpyAsync(status.data(), j->d_status, m, hipMemcpyDeviceToHost, st));
+        NH_HIP(hipMemcpyAsync(t.data(), s.t_last, m * 8, hipMemcpyDeviceToHost, st));
+        NH_HIP(hipMemcpyAsync(nt.data(), s.next_t, m * 8, hipMemcpyDeviceToHost, st));
+        NH_HIP(hipMemcpyAsync(x.data(), s.x, m * 4, hipMemcpyDeviceToHost, st));
+        NH_HIP(hipMemcpyAsync(y.data(), s.y, m * 4, hipMemcpyDeviceToHost, st));
+        NH_HIP(hipMemcpyAsync(idx.data(), s.idx, m * 4, hipMemcpyDeviceToHost, st));
+        if (j->cache_cap) {
+            NH_HIP(hipMemcpyAsync(su.data(), j->s_u, m * 8, hipMemcpyDeviceToHost, st));
+            NH_HIP(hipMemcpyAsync(stt.data(), j->s_t, m * 8, hipMemcpyDeviceToHost, st));
+            NH_HIP(hipMemcpyAsync(sx.data(), j->s_x, m * 4, hipMemcpyDeviceToHost, st));
+            NH_HIP(hipMemcpyAsync(sy.data(), j->s_y, m * 4, hipMemcpyDeviceToHost, st));
+            NH_HIP(hipMemcpyAsync(si.data(), j->s_i, m * 4, hipMemcpyDeviceToHost, st));
+        }
+        NH_HIP(hipStreamSynchronize(st));
+        account_kernel_time(j);
+        for (uint64_t k = 0; k < m; k++) {
+            // DirectMappedCache::insert (cache.rs:39-54): false if the bucket
+            // already holds an equal sample
+            bool dup = false;
+            if (j->cache_cap && (status[k] & ST_HAS_SAMPLE)) {
+                nh_job::HostSample &b =
+                    j->host_cache[speciation_sample_hash(su[k], stt[k], sx[k], sy[k], si[k]) % j->cache_cap];
+                dup = b.some && b.u == su[k] && b.t == stt[k] && b.x == sx[k] && b.y == sy[k] && b.i == si[k];
+                if (!dup) b = nh_job::HostSample{true, su[k], stt[k], sx[k], sy[k], si[k]};
+            }
+            const uint8_t kind = status[k] & ST_KIND_MASK;
+            if (kind == ST_SPEC || dup) continue;
+            nh_job::Queued q{nh_lineage{ref[k], t[k], x[k], y[k], idx[k], 0}, nt[k]};
+            if (q.next < level) j->slow_q.push_back(q);
+            else j->fast_q.push_back(q);
+        }
+    }
+    std::swap(j->slow_q, j->fast_q);
+    j->cur_n = j->slow_q.size();
+    j->alive = j->cur_n;
+    return NH_OK;
+}
+
 // ordered compaction of the persistent form's live lineages into the other buffer
 static int compact_lineages(nh_job *j) {
     hipStream_t st = j->ctx->stream;
@@ -832,6 +976,7 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
 
     bool sorted = false;
     if (p.form == NH_FORM_PLAIN) NH_TRY(plain_round(j));
+    else if (p.form == NH_FORM_TASK_LIST) NH_TRY(task_list_round(j));
     else NH_TRY(persistent_round(j, &sorted));
 
     Counters c;
@@ -975,7 +1120,9 @@ extern "C" int nh_job_take_lineages(nh_job *j, nh_lineage *buf, size_t cap, size
     NH_HIP(hipSetDevice(j->ctx->device));
     if (j->p.form == NH_FORM_PERSISTENT && j->cur_n != j->alive) NH_TRY(compact_lineages(j));
     std::vector<nh_lineage> v;
-    if (j->p.form == NH_FORM_PLAIN) NH_TRY(download_lineages(j->lin[0], j->d_S, j->cur_n, v, j->ctx->stream));
+    if (j->p.form == NH_FORM_TASK_LIST)
+        for (auto &q : j->slow_q) v.push_back(q.l);
+    else if (j->p.form == NH_FORM_PLAIN) NH_TRY(download_lineages(j->lin[0], j->d_S, j->cur_n, v, j->ctx->stream));
     else NH_TRY(download_lineages(j->lin[j->cur], nullptr, j->cur_n, v, j->ctx->stream));
     std::memcpy(buf, v.data(), v.size() * sizeof(nh_lineage));
     return NH_OK;

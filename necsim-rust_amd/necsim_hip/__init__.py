@@ -7,7 +7,7 @@ generators used by the tests and the benchmark.
 """
 from . import _ffi
 from ._ffi import EVENT_DTYPE, LINEAGE_DTYPE, PACKED_EVENT_DTYPE, NecsimHipError, lib
-from .algorithm import (PERSISTENT, PLAIN, AlmostInfiniteNormalDispersalScenario,
+from .algorithm import (PERSISTENT, PLAIN, TASK_LIST, AlmostInfiniteNormalDispersalScenario,
                         BiodiversityReporter, Context, DedupCache, EventCollector, EventLogReporter,
                         EventSlice,
                         HipIndependentAlgorithm, IndependentArguments, Job, NonSpatialScenario,

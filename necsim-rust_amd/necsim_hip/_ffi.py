@@ -22,7 +22,7 @@ SCENARIO_SPATIALLY_EXPLICIT = 2
 SCENARIO_SPATIALLY_EXPLICIT_PACKED = 3
 DEDUP_NONE, DEDUP_RELATIVE, DEDUP_ABSOLUTE = 0, 1, 2
 SLICE_RELATIVE, SLICE_ABSOLUTE = 1, 2
-FORM_PLAIN, FORM_PERSISTENT = 0, 1
+FORM_PLAIN, FORM_PERSISTENT, FORM_TASK_LIST = 0, 1, 2
 
 LINEAGE_DTYPE = np.dtype([("global_reference", "<u8"), ("last_event_time", "<f8"),
                           ("x", "<u4"), ("y", "<u4"), ("index", "<u4"), ("_pad", "<u4")])
@@ -66,7 +66,7 @@ class JobParams(ctypes.Structure):
                 ("form", c_i32), ("block_size", c_u32), ("blocks_per_cu", c_u32),
                 ("use_graph", c_u32), ("inner_steps", c_u32), ("exec_fill", c_u32),
                 ("report_dispersal", c_i32), ("event_buffer", c_u64), ("device_event_counter", c_i32),
-                ("plain_batch", c_u32), ("coop_search", c_u32)]
+                ("plain_batch", c_u32), ("coop_search", c_u32), ("task_list_size", c_u32)]
 
 
 class EventCounts(ctypes.Structure):

@@ -77,6 +77,7 @@ class EventSlice:
 
 PLAIN = _ffi.FORM_PLAIN
 PERSISTENT = _ffi.FORM_PERSISTENT
+TASK_LIST = _ffi.FORM_TASK_LIST
 
 
 @dataclass(frozen=True)
@@ -95,6 +96,7 @@ class IndependentArguments:
     exec_fill: int = 0
     plain_batch: int = 0  # plain form: passes per host synchronisation (1: one per pass; 0: 16)
     coop_search: int = 0  # persistent form: cooperative next-event search (0: automatic, 1: off, 2: on)
+    task_list_size: int = 0  # task-list form: lineages per launch (0: 4096)
     device: int = 0
 
 
@@ -570,7 +572,8 @@ def _job_params(args, rng, speciation_probability, turnover_rate, pause_before=N
                           resume_after=resume_after, form=args.form, block_size=args.block_size,
                           blocks_per_cu=args.blocks_per_cu, use_graph=1 if args.use_graph else 0,
                           inner_steps=args.inner_steps, exec_fill=args.exec_fill,
-                          plain_batch=args.plain_batch, coop_search=args.coop_search)
+                          plain_batch=args.plain_batch, coop_search=args.coop_search,
+                          task_list_size=args.task_list_size)
 
 
 def autotune(ctx, tables, args: IndependentArguments, rng: WyHash, cogs, sample, window,

@@ -133,6 +133,7 @@ pub struct nh_job_params {
     pub device_event_counter: i32,
     pub plain_batch: u32,
     pub coop_search: u32,
+    pub task_list_size: u32,
 }
 
 #[repr(C)]

@@ -12,11 +12,11 @@ import pytest
 import necsim_hip as nh
 import oracle as O
 from necsim_hip import _ffi
-from parity_utils import (assert_exact, assert_invariants, hip_run, map_scenarios, oracle_run,
+from parity_utils import (as_bytes, assert_exact, assert_invariants, hip_run, map_scenarios, oracle_run,
                           unique_keys)
 
 pytestmark = pytest.mark.gpu
-PLAIN, PERSISTENT = nh.PLAIN, nh.PERSISTENT
+PLAIN, PERSISTENT, TASK_LIST = nh.PLAIN, nh.PERSISTENT, nh.TASK_LIST
 
 
 def announce(msg):
@@ -115,7 +115,8 @@ def config1():
 
 
 # persistent form: automatic, plain (1) and cooperative (2) next-event search
-@pytest.mark.parametrize("form,coop", [(PLAIN, 0), (PERSISTENT, 0), (PERSISTENT, 1), (PERSISTENT, 2)])
+@pytest.mark.parametrize("form,coop", [(PLAIN, 0), (PERSISTENT, 0), (PERSISTENT, 1), (PERSISTENT, 2),
+                                       (TASK_LIST, 0)])
 def test_config1_dedup_none_bit_exact(config1, form, coop):
     announce(f"config1 dedup None form {form} coop_search {coop}")
     cogs, orc, lin, ref = config1
@@ -166,7 +167,7 @@ def config2():
     return cogs, orc, lin
 
 
-@pytest.mark.parametrize("form,coop", [(PLAIN, 0), (PERSISTENT, 1), (PERSISTENT, 2)])
+@pytest.mark.parametrize("form,coop", [(PLAIN, 0), (PERSISTENT, 1), (PERSISTENT, 2), (TASK_LIST, 0)])
 def test_config2_window_dedup_none(config2, form, coop):
     announce(f"config2 pause 40 dedup None form {form} coop_search {coop}")
     cogs, orc, lin = config2
@@ -212,7 +213,7 @@ def map64():
     return hip.new(1e-3), orc, orc.origins(1.0)
 
 
-@pytest.mark.parametrize("form,coop", [(PLAIN, 0), (PERSISTENT, 1), (PERSISTENT, 2)])
+@pytest.mark.parametrize("form,coop", [(PLAIN, 0), (PERSISTENT, 1), (PERSISTENT, 2), (TASK_LIST, 0)])
 def test_map64_window(map64, form, coop):
     announce(f"64x64 map pause 30 form {form} coop_search {coop}")
     cogs, orc, lin = map64
@@ -223,6 +224,8 @@ def test_map64_window(map64, form, coop):
     outd, evd = hip_run(cogs, lin, 9, form, dedup=1.0, pause_before=30.0, coop_search=coop)
     if form == PLAIN:
         assert_exact(refd, outd, evd, "map64 dedup plain")
+    elif form == TASK_LIST:  # the CUDA loop's queue order, not the CPU algorithm's
+        assert_invariants(evd, ref["events"], refd["events"], "map64 dedup task list")
     else:
         assert_invariants(evd, ref["events"], refd["events"], "map64 dedup persistent")
 
@@ -312,3 +315,30 @@ def test_autotuned_persistent_form_keeps_parity(config2):
     out, ev = hip_run(cogs, lin, 7, PERSISTENT, dedup="none", pause_before=40.0,
                       step_slice=args.step_slice)
     assert_exact(ref, out, ev, "tuned")
+
+
+# ---------------------------------------------------------------------------
+# the reference CUDA algorithm's launch structure (task lists, host dedup)
+@pytest.mark.parametrize("task_list_size", [64, 1000, 0])
+def test_task_list_form_config1(config1, task_list_size):
+    """dedup None: bit-exact whatever the task list size; dedup on: the CUDA
+    loop's queue order (tasks requeued after the skipped lineages of their
+    fill), so the dedup-on invariants"""
+    announce(f"task-list form config1, task_list_size {task_list_size}")
+    cogs, orc, lin, ref = config1
+    out, ev = hip_run(cogs, lin, 42, TASK_LIST, dedup="none", task_list_size=task_list_size)
+    assert_exact(ref["none"], out, ev, "task list none")
+    out, ev = hip_run(cogs, lin, 42, TASK_LIST, dedup=1.0, task_list_size=task_list_size)
+    assert_invariants(ev, ref["none"]["events"], ref[1.0]["events"], "task list dedup")
+
+
+def test_task_list_form_pause_resume(config1):
+    announce("task-list form pause at 500 and resume")
+    cogs, orc, lin, ref = config1
+    out1, ev1 = hip_run(cogs, lin, 42, TASK_LIST, pause_before=500.0, task_list_size=700)
+    assert out1.paused and len(out1.lineages)
+    out2, ev2 = hip_run(cogs, out1.lineages, 42, TASK_LIST, resume_after=500.0, task_list_size=700)
+    both = np.concatenate([ev1, ev2])
+    assert as_bytes(np.sort(both, order=["event_time", "x", "y", "index", "prior_time", "global_reference"])) == \
+        as_bytes(ref["none"]["events"])
+    assert out1.steps + out2.steps == ref["none"]["steps"]
