@@ -153,6 +153,12 @@ template <int KIND> struct Lane {
     bool searching;
     uint64_t search_ts;
     double search_before;
+    // lane_search_step: the open window's remaining event draws
+    uint64_t win_st;   // RNG state before the next draw
+    uint32_t win_left; // draws left (0: no window open)
+    uint32_t win_e;    // index of the next draw
+    uint32_t win_bi;   // index of the earliest valid event so far, ~0u if none
+    double win_best;   // its time
     // min speciation sample of the current slice
     bool ms_some;
     double ms_u, ms_t;
@@ -203,12 +209,13 @@ __device__ __forceinline__ void lane_set_location(Lane<KIND> &L, const DevScenar
 // PoissonEventTimeSampler::next_event_time_at_indexed_location_weakly_after
 // (event_time_sampler/poisson.rs:30-109), one time window at a time: the
 // events of window `ts` (its RNG primed with (id, ts)) are drawn, and the
-// earliest one after `time` is the next event.  Returns false if the window
-// holds none; otherwise sets *best and leaves `state` primed for the event's
-// own draws.
-__device__ __forceinline__ bool poisson_window(const DevParams &P, const double *ratio, uint64_t id,
-                                               double time, uint64_t ts, double &best, uint64_t &key) {
-    uint64_t state = prime(P.seed, id, ts);
+// earliest one after `time` is the next event (poisson_window = prime,
+// poisson_count, poisson_events).  The event's own draws use prime(id, key)
+// with the key set by poisson_events.
+//
+// poisson_count: the number of events of the window whose primed RNG is
+// `state`; leaves `state` before the first event's time draw.
+__device__ __forceinline__ uint32_t poisson_count(const DevParams &P, const double *ratio, uint64_t &state) {
     uint32_t number;
     if (P.p0 > 0.0) {
         // while u > acc_k && prod_k > 0: k += 1 (acc is non-decreasing, so
@@ -239,6 +246,16 @@ __device__ __forceinline__ bool poisson_window(const DevParams &P, const double 
         sample_2d_normal(state, P.lambda_dt, P.lambda_dt_sqrt, a, b);
         number = as_u32(a); // f64::max(a, 0.0) as u32: NaN and negatives give 0
     }
+    return number;
+}
+
+// The window's `number` event times (ts + U[0,1)) * delta_t, drawn in order
+// from `state`; the first strictly smallest one after `time` is the event.
+// Event e's draw is the (e + 1)-th sample after `state`, i.e. the one made
+// from the WyHash state state + (e + 1) * P0 (sample_u64 adds P0, then
+// hashes), so the draws can also be evaluated out of order.
+__device__ __forceinline__ bool poisson_events(const DevParams &P, uint64_t state, uint32_t number, uint64_t ts,
+                                               double time, double &best, uint64_t &key) {
     bool found = false;
     uint32_t best_index = 0;
     for (uint32_t e = 0; e < number; e++) {
@@ -253,6 +270,14 @@ __device__ __forceinline__ bool poisson_window(const DevParams &P, const double 
     // prime itself is left to the execution, where the lanes run together
     if (found) key = ts + INV_PHI * (uint64_t)(best_index + 1);
     return found;
+}
+
+// One window: false if it holds no event after `time`, else sets best / key.
+__device__ __forceinline__ bool poisson_window(const DevParams &P, const double *ratio, uint64_t id,
+                                               double time, uint64_t ts, double &best, uint64_t &key) {
+    uint64_t state = prime(P.seed, id, ts);
+    const uint32_t number = poisson_count(P, ratio, state);
+    return poisson_events(P, state, number, ts, time, best, key);
 }
 
 __device__ __forceinline__ uint64_t poisson_first_window(const DevParams &P, double time) {
@@ -280,23 +305,52 @@ __device__ __forceinline__ void lane_peek(Lane<KIND> &L, const DevParams &P, con
     L.have_next = true;
 }
 
-// The same peek advanced by ONE window per call, so that a wave steps all of
-// its lanes' searches in lockstep instead of waiting for its slowest lane's
-// whole search.  After the call that finds the event, L.next_t / L.next_st
-// equal lane_peek's.
+// The same peek advanced a little per call, so that a wave steps all of its
+// lanes' searches in lockstep instead of waiting for its slowest lane's
+// whole search.  A call opens the next window if none is open (prime and
+// count draw) and makes at most STEP_DRAWS of its event draws.  A window holds a Poisson number of
+// events, so drawing all of them in one call makes the wave wait for its
+// lane with the most; here a lane with many events finishes its window in
+// later calls while the other lanes go on.  The draws, their order and the
+// pick (first strictly smallest time after search_before) are those of
+// poisson_window, so L.next_t / L.next_st end up equal to lane_peek's.
+constexpr uint32_t STEP_DRAWS = 2;
 template <int KIND>
-__device__ __forceinline__ void lane_search_window(Lane<KIND> &L, const DevParams &P, const double *ratio) {
+__device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams &P, const double *ratio) {
     if (L.have_next) return;
     if (!L.searching) {
         L.search_before = P.min_event_time > L.t_last ? P.min_event_time : L.t_last;
         L.search_ts = poisson_first_window(P, L.search_before);
         L.searching = true;
+        L.win_left = 0;
     }
-    double et = 0.0;
-    uint64_t st;
-    if (poisson_window(P, ratio, L.id, L.search_before, L.search_ts, et, st)) {
-        L.next_t = max_after(L.search_before, et);
-        L.next_st = st;
+    if (L.win_left == 0) {
+        uint64_t st = prime(P.seed, L.id, L.search_ts);
+        L.win_left = poisson_count(P, ratio, st);
+        L.win_st = st;
+        L.win_e = 0;
+        L.win_bi = ~0u;
+        if (L.win_left == 0) {
+            L.search_ts++;
+            return;
+        }
+    }
+    const double tsd = (double)L.search_ts;
+#pragma unroll
+    for (uint32_t k = 0; k < STEP_DRAWS; k++) {
+        if (L.win_left == 0) break;
+        const double t = (tsd + uniform_closed_open(L.win_st)) * P.delta_t;
+        if (t > L.search_before && (L.win_bi == ~0u || L.win_best > t)) {
+            L.win_best = t;
+            L.win_bi = L.win_e;
+        }
+        L.win_e++;
+        L.win_left--;
+    }
+    if (L.win_left != 0) return;
+    if (L.win_bi != ~0u) {
+        L.next_t = max_after(L.search_before, L.win_best);
+        L.next_st = L.search_ts + INV_PHI * (uint64_t)(L.win_bi + 1);
         L.have_next = true;
         L.searching = false;
     } else {

@@ -4,7 +4,7 @@
 //    hold one lineage in registers and refill from a guided device work queue
 //    (one atomic per wave claim, ballot/mbcnt distribution) whenever their
 //    lineage speciates, is dropped as a duplicate, or reaches the water level.
-//    Next-event searches advance one Poisson window per iteration and found
+//    Next-event searches advance a bounded amount per iteration and found
 //    events execute in batches.  The DEDUP variant inserts into the dedup
 //    cache in-kernel at every step-slice boundary.
 //  * k_serve<KIND>: the persistent kernel over the lineages of many requests.
@@ -271,7 +271,8 @@ __device__ __forceinline__ void block_flush(unsigned long long steps, double max
 // Persistent round kernel.  Lineages are updated in place: a lineage that
 // reaches the water level is written back to its slot, one that speciates or
 // is dropped as a duplicate only gets its dead flag.  Each inner iteration
-// advances every searching lane's next-event search by one Poisson window;
+// advances every searching lane's next-event search a little (at most one
+// Poisson window opened and two of its event draws, lane_search_step);
 // once at least exec_fill eighths of the wave's active lanes hold their next
 // event, those lanes handle it (leave at the level, dedup, execute) in the
 // same iteration.  A search takes a variable number of windows, so stepping
@@ -442,7 +443,7 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
                     coop_search_window<KIND>(L, srch, A.P, s_ratio, s_coop);
                     if (!has) continue;
                 } else if (!L.have_next) {
-                    lane_search_window<KIND>(L, A.P, s_ratio);
+                    lane_search_step<KIND>(L, A.P, s_ratio);
                 }
                 if (!wave_ready(L.have_next, A.exec_fill) || !L.have_next) continue;
                 // Leaving this round (water level) and reaching the step slice
