@@ -138,13 +138,13 @@ struct DevPacked {
 static_assert(sizeof(DevPacked) == 56, "packed event is 56 B");
 
 // Lane-resident lineage.  a/b hold (x, y) for the almost-infinite habitat and
-// (cell index, unused) for finite extents; `id` is the injective location id
-// used to prime the RNG.
+// (cell index, unused) for finite extents; `idh` is the injective location
+// id hashed for priming the RNG (prime_location).
 template <int KIND> struct Lane {
     uint64_t ref;
     double t_last;
     uint32_t a, b, i;
-    uint64_t id;
+    uint64_t idh;
     CellRec rec;        // finite in-memory kinds: record of the current cell
     double next_t;
     uint64_t next_st; // prime key of the next event
@@ -190,17 +190,17 @@ __device__ __forceinline__ void lane_set_location(Lane<KIND> &L, const DevScenar
     if (KIND == K_ALMOST_INFINITE) {
         L.a = x;
         L.b = y;
-        L.id = ((uint64_t)y << 32) | (uint64_t)x; // habitat/almost_infinite.rs:67-71
+        L.idh = prime_location(((uint64_t)y << 32) | (uint64_t)x); // habitat/almost_infinite.rs:67-71
     } else {
         uint32_t cell = (y - S.oy) * S.width + (x - S.ox);
         L.a = cell;
         L.b = 0;
         if (KIND == K_NON_SPATIAL) {
             // habitat/non_spatial.rs:93-109
-            L.id = (uint64_t)cell * (uint64_t)S.deme + (uint64_t)i;
+            L.idh = prime_location((uint64_t)cell * (uint64_t)S.deme + (uint64_t)i);
         } else {
             L.rec = S.cells[cell];
-            L.id = L.rec.prefix + (uint64_t)i; // habitat/in_memory.rs:71-81
+            L.idh = prime_location(L.rec.prefix + (uint64_t)i); // habitat/in_memory.rs:71-81
         }
     }
 }
@@ -208,7 +208,7 @@ __device__ __forceinline__ void lane_set_location(Lane<KIND> &L, const DevScenar
 // ---------------------------------------------------------------------------
 // PoissonEventTimeSampler::next_event_time_at_indexed_location_weakly_after
 // (event_time_sampler/poisson.rs:30-109), one time window at a time: the
-// events of window `ts` (its RNG primed with (id, ts)) are drawn, and the
+// events of window `ts` (its RNG primed with (location, ts)) are drawn, and the
 // earliest one after `time` is the next event (poisson_window = prime,
 // poisson_count, poisson_events).  The event's own draws use prime(id, key)
 // with the key set by poisson_events.
@@ -272,10 +272,11 @@ __device__ __forceinline__ bool poisson_events(const DevParams &P, uint64_t stat
     return found;
 }
 
-// One window: false if it holds no event after `time`, else sets best / key.
-__device__ __forceinline__ bool poisson_window(const DevParams &P, const double *ratio, uint64_t id,
+// One window at the location hashed as idh (prime_location): false if it
+// holds no event after `time`, else sets best / key.
+__device__ __forceinline__ bool poisson_window(const DevParams &P, const double *ratio, uint64_t idh,
                                                double time, uint64_t ts, double &best, uint64_t &key) {
-    uint64_t state = prime(P.seed, id, ts);
+    uint64_t state = prime_at(P.seed, idh, ts);
     const uint32_t number = poisson_count(P, ratio, state);
     return poisson_events(P, state, number, ts, time, best, key);
 }
@@ -285,11 +286,11 @@ __device__ __forceinline__ uint64_t poisson_first_window(const DevParams &P, dou
 }
 
 // The whole search: windows ts, ts + 1, ... until one holds an event.
-__device__ __forceinline__ double poisson_next(const DevParams &P, const double *ratio, uint64_t id,
+__device__ __forceinline__ double poisson_next(const DevParams &P, const double *ratio, uint64_t idh,
                                                double time, uint64_t &key) {
     double best = 0.0;
     for (uint64_t ts = poisson_first_window(P, time);; ts++)
-        if (poisson_window(P, ratio, id, time, ts, best, key)) return best;
+        if (poisson_window(P, ratio, idh, time, ts, best, key)) return best;
 }
 
 // Make sure L.next_t / L.next_st hold the next event (the `pop` peek).
@@ -299,7 +300,7 @@ __device__ __forceinline__ void lane_peek(Lane<KIND> &L, const DevParams &P, con
     // NonNegativeF64::max of last_event_time and min_event_time (sampler.rs:87)
     double before = P.min_event_time > L.t_last ? P.min_event_time : L.t_last;
     uint64_t st;
-    double et = poisson_next(P, ratio, L.id, before, st);
+    double et = poisson_next(P, ratio, L.idh, before, st);
     L.next_t = max_after(before, et);
     L.next_st = st;
     L.have_next = true;
@@ -325,7 +326,7 @@ __device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams 
         L.win_left = 0;
     }
     if (L.win_left == 0) {
-        uint64_t st = prime(P.seed, L.id, L.search_ts);
+        uint64_t st = prime_at(P.seed, L.idh, L.search_ts);
         L.win_left = poisson_count(P, ratio, st);
         L.win_st = st;
         L.win_e = 0;
@@ -368,7 +369,7 @@ template <int KIND, bool TRACK_MS = true>
 __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S, const DevParams &P,
                                             DevEvent &ev) {
     double net = L.next_t;
-    uint64_t st = prime(P.seed, L.id, L.next_st); // next_st holds the event's prime key
+    uint64_t st = prime_at(P.seed, L.idh, L.next_st); // next_st holds the event's prime key
     L.have_next = false;
 
     double u_spec = uniform_closed_open(st);
@@ -406,7 +407,7 @@ __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S,
         uint32_t ci = as_u32(floor_(uc * S.deme_f));
         L.a = (uint32_t)cell;
         L.i = ci;
-        L.id = cell * (uint64_t)S.deme + (uint64_t)ci;
+        L.idh = prime_location(cell * (uint64_t)S.deme + (uint64_t)ci);
     } else if (KIND == K_ALMOST_INFINITE) {
         // AlmostInfiniteNormalDispersalSampler + clamp_round_dispersal
         // (almost_infinite_normal.rs:52-69, habitat/almost_infinite.rs:98-117).
@@ -419,7 +420,7 @@ __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S,
         L.a = (uint32_t)((uint64_t)L.a + (uint64_t)ix);
         L.b = (uint32_t)((uint64_t)L.b + (uint64_t)iy);
         L.i = 0;
-        L.id = ((uint64_t)L.b << 32) | (uint64_t)L.a;
+        L.idh = prime_location(((uint64_t)L.b << 32) | (uint64_t)L.a);
     } else {
         // InMemoryPackedSeparableAliasDispersalSampler (packed_separable_alias/dispersal.rs:16-77)
         // and InMemoryPackedAliasDispersalSampler (packed_alias/dispersal.rs:16-52)
@@ -453,7 +454,7 @@ __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S,
         L.a = target;
         L.i = ci;
         L.rec = t;
-        L.id = t.prefix + (uint64_t)ci;
+        L.idh = prime_location(t.prefix + (uint64_t)ci);
     }
     L.t_last = net;
     return STEP_DISPERSED;

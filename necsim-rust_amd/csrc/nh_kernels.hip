@@ -328,7 +328,7 @@ __device__ __forceinline__ void coop_search_window(Lane<KIND> &L, bool srch, con
             off = 1 + r / ns;
         }
     }
-    const uint64_t id = __shfl(L.id, tgt);
+    const uint64_t idh = __shfl(L.idh, tgt);
     const double before = __shfl(L.search_before, tgt);
     const uint64_t ts = __shfl(L.search_ts, tgt);
     double et = 0.0;
@@ -337,9 +337,9 @@ __device__ __forceinline__ void coop_search_window(Lane<KIND> &L, bool srch, con
     if constexpr (LANE_SEED) {
         DevParams Pt = P;
         Pt.seed = __shfl(P.seed, tgt);
-        found = poisson_window(Pt, ratio, id, before, ts + off, et, key);
+        found = poisson_window(Pt, ratio, idh, before, ts + off, et, key);
     } else {
-        found = poisson_window(P, ratio, id, before, ts + off, et, key);
+        found = poisson_window(P, ratio, idh, before, ts + off, et, key);
     }
     if (nh) {
         if (!srch && found) {

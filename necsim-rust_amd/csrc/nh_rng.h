@@ -49,11 +49,16 @@ NH_HD uint64_t rot32(uint64_t x) { return (x << 32) | (x >> 32); }
 
 // PrimeableRng::prime_with, wyhash.rs:67-80.  Returns the new `state`; the
 // `seed` never changes after seeding.
-NH_HD uint64_t prime(uint64_t seed, uint64_t location_index, uint64_t time_index) {
-    uint64_t l = seahash_diffuse(location_index);
+// prime_location is its location half, which every prime at one location
+// shares: the kernels hash a lineage's location once per move.
+NH_HD uint64_t prime_location(uint64_t location_index) { return rot32(seahash_diffuse(location_index)); }
+NH_HD uint64_t prime_at(uint64_t seed, uint64_t location_hash, uint64_t time_index) {
     uint64_t t = seahash_diffuse(time_index);
-    uint64_t h = wymum(rot32(l) ^ (seed ^ WY_P0), rot32(t) ^ WY_P2);
+    uint64_t h = wymum(location_hash ^ (seed ^ WY_P0), rot32(t) ^ WY_P2);
     return wymum(h, 16ull ^ WY_P5);
+}
+NH_HD uint64_t prime(uint64_t seed, uint64_t location_index, uint64_t time_index) {
+    return prime_at(seed, prime_location(location_index), time_index);
 }
 
 // RngCore::sample_u64, wyhash.rs:53-63
