@@ -39,6 +39,10 @@ def _inputs(rng, kind, n):
         return -6.283185307179586 * (rng.integers(0, 2**53, n, dtype=np.uint64).astype(np.float64) * 2.0**-53)
     if kind == "wide":
         return np.exp(rng.uniform(-700, 700, n))
+    if kind == "angle_wide":  # every case of the pi/2 reduction, and the ulps around k pi/4
+        k = np.arange(1, 41)
+        near = ((k * (np.pi / 4)).view(np.int64)[:, None] + np.arange(-300, 301)).ravel().view(np.float64)
+        return np.concatenate([rng.uniform(-1e5, 1e5, n // 4), rng.uniform(-40, 40, n // 2), near, -near])
     if kind == "exp":
         return rng.uniform(-740, 705, n)
     if kind == "sqrt":
@@ -54,7 +58,8 @@ SPECIALS = [1.0, 2.0, 0.5, 2.0**-53, 1.0 - 2.0**-53, np.pi / 2, -np.pi / 2, -np.
 
 @pytest.mark.parametrize("op,name,kind", [(0, "orc_ln", "unit_open"), (0, "orc_ln", "wide"),
                                           (1, "orc_exp", "exp"), (2, "orc_sin", "angle"),
-                                          (3, "orc_cos", "angle"), (4, "orc_sqrt", "sqrt"),
+                                          (3, "orc_cos", "angle"), (2, "orc_sin", "angle_wide"),
+                                          (3, "orc_cos", "angle_wide"), (4, "orc_sqrt", "sqrt"),
                                           (5, "orc_floor", "round"), (6, "orc_round", "round")])
 def test_device_maths_core(ctx, op, name, kind):
     announce(f"device maths op {op} {name} {kind}")
