@@ -153,12 +153,13 @@ template <int KIND> struct Lane {
     bool searching;
     uint64_t search_ts;
     double search_before;
-    // lane_search_step: the open window's remaining event draws
-    uint64_t win_st;   // RNG state before the next draw
+    // lane_search_step: the open window's remaining event draws.  While the
+    // search runs, next_st holds the RNG state before the window's next draw
+    // and next_t the earliest valid event time so far (both only mean the
+    // next event once have_next is set), which saves registers.
     uint32_t win_left; // draws left (0: no window open)
     uint32_t win_e;    // index of the next draw
     uint32_t win_bi;   // index of the earliest valid event so far, ~0u if none
-    double win_best;   // its time
     // min speciation sample of the current slice
     bool ms_some;
     double ms_u, ms_t;
@@ -309,26 +310,29 @@ __device__ __forceinline__ void lane_peek(Lane<KIND> &L, const DevParams &P, con
 // The same peek advanced a little per call, so that a wave steps all of its
 // lanes' searches in lockstep instead of waiting for its slowest lane's
 // whole search.  A call opens the next window if none is open (prime and
-// count draw) and makes at most STEP_DRAWS of its event draws.  A window holds a Poisson number of
-// events, so drawing all of them in one call makes the wave wait for its
-// lane with the most; here a lane with many events finishes its window in
-// later calls while the other lanes go on.  The draws, their order and the
-// pick (first strictly smallest time after search_before) are those of
-// poisson_window, so L.next_t / L.next_st end up equal to lane_peek's.
+// count draw) and makes at most STEP_DRAWS of its event draws.  A window
+// holds a Poisson number of events, so drawing all of them in one call makes
+// the wave wait for its lane with the most; here a lane with many events
+// finishes its window in later calls while the other lanes go on.  The
+// draws, their order and the pick (first strictly smallest time after the
+// search's lower bound) are those of poisson_window, so L.next_t / L.next_st
+// end up equal to lane_peek's.
 constexpr uint32_t STEP_DRAWS = 2;
 template <int KIND>
 __device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams &P, const double *ratio) {
     if (L.have_next) return;
+    // NonNegativeF64::max of last_event_time and min_event_time (sampler.rs:87),
+    // recomputed per call instead of held in a register
+    const double before = P.min_event_time > L.t_last ? P.min_event_time : L.t_last;
     if (!L.searching) {
-        L.search_before = P.min_event_time > L.t_last ? P.min_event_time : L.t_last;
-        L.search_ts = poisson_first_window(P, L.search_before);
+        L.search_ts = poisson_first_window(P, before);
         L.searching = true;
         L.win_left = 0;
     }
     if (L.win_left == 0) {
         uint64_t st = prime_at(P.seed, L.idh, L.search_ts);
         L.win_left = poisson_count(P, ratio, st);
-        L.win_st = st;
+        L.next_st = st;
         L.win_e = 0;
         L.win_bi = ~0u;
         if (L.win_left == 0) {
@@ -340,9 +344,9 @@ __device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams 
 #pragma unroll
     for (uint32_t k = 0; k < STEP_DRAWS; k++) {
         if (L.win_left == 0) break;
-        const double t = (tsd + uniform_closed_open(L.win_st)) * P.delta_t;
-        if (t > L.search_before && (L.win_bi == ~0u || L.win_best > t)) {
-            L.win_best = t;
+        const double t = (tsd + uniform_closed_open(L.next_st)) * P.delta_t;
+        if (t > before && (L.win_bi == ~0u || L.next_t > t)) {
+            L.next_t = t;
             L.win_bi = L.win_e;
         }
         L.win_e++;
@@ -350,7 +354,7 @@ __device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams 
     }
     if (L.win_left != 0) return;
     if (L.win_bi != ~0u) {
-        L.next_t = max_after(L.search_before, L.win_best);
+        L.next_t = max_after(before, L.next_t);
         L.next_st = L.search_ts + INV_PHI * (uint64_t)(L.win_bi + 1);
         L.have_next = true;
         L.searching = false;
