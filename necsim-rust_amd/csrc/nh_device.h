@@ -366,7 +366,9 @@ __device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams 
 enum : int { STEP_DISPERSED = 0, STEP_SPECIATED = 1 };
 
 // Execute the peeked event (L.have_next must hold and L.next_t < level).
-// On speciation the event is written to *ev.
+// On speciation the event is written to *ev, all but its reference (the
+// lineage's global reference never changes, so callers that do not keep it
+// in a register read it from the lineage store).
 // TRACK_MS = false skips the slice's min speciation sample (only the dedup
 // cache reads it).
 template <int KIND, bool TRACK_MS = true>
@@ -389,8 +391,7 @@ __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S,
     if (u_spec < P.nu) {
         uint32_t x, y;
         location_xy<KIND>(S, L.a, L.b, x, y);
-        ev.ref = L.ref;
-        ev.prior_time = L.t_last;
+        ev.prior_time = L.t_last; // ev.ref is the caller's: the lineage's reference
         ev.event_time = net;
         ev.x = x;
         ev.y = y;

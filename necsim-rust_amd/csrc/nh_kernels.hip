@@ -47,8 +47,7 @@ __device__ __forceinline__ void lane_store(const Lane<KIND> &L, const DevScenari
                                            const LineageSoA &s, uint64_t slot) {
     uint32_t x, y;
     location_xy<KIND>(S, L.a, L.b, x, y);
-    s.ref[slot] = L.ref;
-    s.t_last[slot] = L.t_last;
+    s.t_last[slot] = L.t_last; // the reference never changes: not written back
     s.x[slot] = x;
     s.y[slot] = y;
     s.idx[slot] = L.i;
@@ -491,7 +490,10 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
                 my_max = L.t_last > my_max ? L.t_last : my_max;
                 if (DISP) emit_packed<KIND>(A.packed + eslot, o, L, A.S, r == STEP_SPECIATED);
                 if (r == STEP_SPECIATED) {
-                    if (!DISP) A.events[atomicAdd(&A.ctr->n_events, 1ull)] = ev; // rare
+                    if (!DISP) { // rare
+                        ev.ref = A.lin.ref[slot];
+                        A.events[atomicAdd(&A.ctr->n_events, 1ull)] = ev;
+                    }
                     A.lin.flags[slot] = FLAG_DEAD;
                     my_dead++;
                     has = false;
@@ -621,6 +623,7 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
                 tick++;
                 acc_max = L.t_last > acc_max ? L.t_last : acc_max;
                 if (r == STEP_SPECIATED) {
+                    ev.ref = R.lin.ref[slot];
                     R.events[atomicAdd(&R.ctr->n_events, 1ull)] = ev;
                     R.lin.flags[slot] = FLAG_DEAD;
                     acc_dead++;
@@ -737,6 +740,7 @@ __global__ void __launch_bounds__(256) k_plain(PlainArgs A) {
                 my_steps++;
                 if (DISP) emit_packed<KIND>(A.packed + eslot, o, L, A.S, r == STEP_SPECIATED);
                 if (r == STEP_SPECIATED) {
+                    ev.ref = L.ref;
                     st = ST_SPEC;
                     emit = !DISP;
                     break;
