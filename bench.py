@@ -43,6 +43,7 @@ GATHER_PROFILE = os.path.join(ROOT, "profiles", "k_persistent_alias_gather_map25
 # bytes of lineage state one persistent round reads and writes per lineage
 # (SoA: ref 8, t_last 8, x/y/index 12, next_t 8, next_state 8, flags 1)
 LINEAGE_STATE_BYTES = 45
+MOVING_STATE_BYTES = LINEAGE_STATE_BYTES - 8  # all but the reference, which never changes
 EVENT_BYTES = 40
 CACHE_BUCKET_BYTES = 32
 
@@ -478,17 +479,19 @@ def main():
         f"device {kern_ms:.1f} ms over {launches} launches")
 
     # Roofline of the dominant kernel.  Algorithmic HBM bytes of one round
-    # launch: every lineage's state read once (45 B), the survivors' state
-    # written back (45 B), a dead flag per speciated lineage, 40 B per
-    # speciation event; the almost-infinite scenario reads no tables.  The
-    # plain form reads and writes the state of its pass list once per launch.
+    # launch: every lineage's state but its reference read once (37 B), the
+    # survivors' state written back (37 B), per speciation a dead flag, the
+    # reference and the 40 B event; the almost-infinite scenario reads no
+    # tables.  The plain form reads the whole state (45 B) and writes the
+    # moving part (37 B) of its pass list once per launch.
     nl = max(launches, 1)
     n_in, n_left, n_ev = len(lineages), int(o0.n_remaining), int(o0.n_events)
     if form == nh.PERSISTENT:
-        round_bytes = n_in * LINEAGE_STATE_BYTES + n_left * LINEAGE_STATE_BYTES + n_ev * (1 + EVENT_BYTES)
+        round_bytes = (n_in * MOVING_STATE_BYTES + n_left * MOVING_STATE_BYTES
+                       + n_ev * (1 + 8 + EVENT_BYTES))
         per_launch_bytes = round_bytes * a.steps / nl
     else:
-        per_launch_bytes = n_in * 2 * LINEAGE_STATE_BYTES
+        per_launch_bytes = n_in * (LINEAGE_STATE_BYTES + MOVING_STATE_BYTES)
     mean_launch_ms = kern_ms / nl
     achieved = per_launch_bytes / (mean_launch_ms * 1e-3) / 1e9
     prof = read_profile_summary(PROFILE_SUMMARY)
