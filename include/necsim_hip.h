@@ -342,8 +342,10 @@ typedef struct {
     uint64_t max_lineages;      /* lane-state capacity shared by requests in flight */
     uint64_t hbm_budget_bytes;  /* admission limit for per-request device memory */
     uint64_t tick_steps;        /* max steps per lineage per kernel tick (latency knob) */
-    uint32_t block_size;        /* 0 = default */
-    uint32_t blocks_per_cu;     /* 0 = default */
+    uint32_t block_size;        /* 0 = default (256) */
+    uint32_t blocks_per_cu;     /* 0 = default (1): a tick's latency is set by the
+                                   slowest requests' serial lineage chains, which
+                                   run faster with few resident waves per CU */
 } nh_server_params;
 
 typedef struct {

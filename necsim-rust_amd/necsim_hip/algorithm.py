@@ -665,7 +665,9 @@ class Server:
     """Continuous-batching server on resident tables (nh_server_*).
 
     Requests are admitted by device footprint as memory frees up; every
-    tick() runs one shared persistent kernel over all admitted requests."""
+    tick() runs one shared persistent kernel over all admitted requests.
+    block_size / blocks_per_cu of 0 launch one block of 256 threads per CU,
+    the latency-optimal shape of the serving stream."""
 
     def __init__(self, ctx, tables, max_lineages, hbm_budget_bytes, tick_steps=0, block_size=0,
                  blocks_per_cu=0):
