@@ -308,6 +308,22 @@ def test_edge_general_delta_t_and_step_slice(config1):
     assert_exact(r, out, ev, "delta_t 3, step_slice 1")
 
 
+# Windows of many events: lambda * delta_t = 1.5 (delta_t not a power of two),
+# 20 (the Poisson count past its inversion table; a window's draws spread over
+# many calls of the stepped search) and 1000 (exp(-1000) underflows: the count
+# comes from the normal approximation, poisson.rs:66-74)
+@pytest.mark.parametrize("delta_t,pause", [(3.0, None), (40.0, 400.0), (2000.0, 200.0)])
+@pytest.mark.parametrize("form,coop", [(PLAIN, 0), (PERSISTENT, 1), (PERSISTENT, 2), (TASK_LIST, 0)])
+def test_edge_windows_of_many_events(config1, delta_t, pause, form, coop):
+    announce(f"edge: delta_t {delta_t} form {form} coop_search {coop}")
+    cogs, orc, lin, _ = config1
+    sub = lin[::50]
+    r = oracle_run(orc, sub, 42, 1e-3, dedup="none", delta_t=delta_t, pause_before=pause)
+    out, ev = hip_run(cogs, sub, 42, form, dedup="none", delta_t=delta_t, pause_before=pause, coop_search=coop)
+    assert r["steps"] > 10 * len(sub)
+    assert_exact(r, out, ev, f"delta_t {delta_t}")
+
+
 def test_autotuned_persistent_form_keeps_parity(config2):
     announce("autotune on a config2 sample, then a tuned run")
     cogs, orc, lin = config2
