@@ -310,14 +310,16 @@ __device__ __forceinline__ void lane_peek(Lane<KIND> &L, const DevParams &P, con
 // The same peek advanced a little per call, so that a wave steps all of its
 // lanes' searches in lockstep instead of waiting for its slowest lane's
 // whole search.  A call opens the next window if none is open (prime and
-// count draw) and makes at most STEP_DRAWS of its event draws.  A window
-// holds a Poisson number of events, so drawing all of them in one call makes
-// the wave wait for its lane with the most; here a lane with many events
-// finishes its window in later calls while the other lanes go on.  The
-// draws, their order and the pick (first strictly smallest time after the
-// search's lower bound) are those of poisson_window, so L.next_t / L.next_st
-// end up equal to lane_peek's.
+// count draw) and makes at most STEP_DRAWS of its event draws; a window that
+// holds no events is followed by the next one in the same call, up to
+// STEP_OPENS windows.  A window holds a Poisson number of events, so drawing
+// all of them in one call makes the wave wait for its lane with the most;
+// here a lane with many events finishes its window in later calls while the
+// other lanes go on.  The draws, their order and the pick (first strictly
+// smallest time after the search's lower bound) are those of poisson_window,
+// so L.next_t / L.next_st end up equal to lane_peek's.
 constexpr uint32_t STEP_DRAWS = 2;
+constexpr uint32_t STEP_OPENS = 2;
 template <int KIND>
 __device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams &P, const double *ratio) {
     if (L.have_next) return;
@@ -329,17 +331,17 @@ __device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams 
         L.searching = true;
         L.win_left = 0;
     }
-    if (L.win_left == 0) {
+    // open windows until one holds events (at most STEP_OPENS per call)
+#pragma unroll
+    for (uint32_t o = 0; o < STEP_OPENS && L.win_left == 0; o++) {
         uint64_t st = prime_at(P.seed, L.idh, L.search_ts);
         L.win_left = poisson_count(P, ratio, st);
         L.next_st = st;
         L.win_e = 0;
         L.win_bi = ~0u;
-        if (L.win_left == 0) {
-            L.search_ts++;
-            return;
-        }
+        if (L.win_left == 0) L.search_ts++;
     }
+    if (L.win_left == 0) return;
     const double tsd = (double)L.search_ts;
 #pragma unroll
     for (uint32_t k = 0; k < STEP_DRAWS; k++) {
