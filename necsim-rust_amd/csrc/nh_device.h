@@ -372,8 +372,8 @@ enum : int { STEP_DISPERSED = 0, STEP_SPECIATED = 1 };
 // lineage's global reference never changes, so callers that do not keep it
 // in a register read it from the lineage store).
 // TRACK_MS = false skips the slice's min speciation sample (only the dedup
-// cache reads it).
-template <int KIND, bool TRACK_MS = true>
+// cache reads it).  FLAT takes the straight-line sin / cos (sincos_flat).
+template <int KIND, bool TRACK_MS = true, bool FLAT = false>
 __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S, const DevParams &P,
                                             DevEvent &ev) {
     double net = L.next_t;
@@ -420,7 +420,7 @@ __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S,
         // (almost_infinite_normal.rs:52-69, habitat/almost_infinite.rs:98-117).
         // ((x + dx) % 2^32 + 2^32) % 2^32 is the low word of x + dx.
         double dx, dy;
-        sample_2d_normal(st, 0.0, S.sigma, dx, dy);
+        sample_2d_normal<FLAT>(st, 0.0, S.sigma, dx, dy);
         int64_t ix = as_i64(round_(dx));
         int64_t iy = as_i64(round_(dy));
         (void)uniform_closed_open(st); // CoalescenceRngSample::new, unused by singleton demes

@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
                     o = lane_origin<KIND>(L, A.S);
                 }
                 DevEvent ev;
-                int r = lane_execute<KIND, DEDUP>(L, A.S, A.P, ev);
+                int r = lane_execute<KIND, DEDUP, COOP>(L, A.S, A.P, ev);
                 my_steps++;
                 slice_steps++;
                 my_max = L.t_last > my_max ? L.t_last : my_max;
@@ -617,7 +617,7 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
                     }
                 }
                 DevEvent ev;
-                int r = lane_execute<KIND>(L, A.S, P, ev);
+                int r = lane_execute<KIND, true, true>(L, A.S, P, ev);
                 acc_steps++;
                 slice_steps++;
                 tick++;

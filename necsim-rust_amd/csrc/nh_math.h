@@ -286,6 +286,67 @@ NH_HD void sincos(double x, double &s, double &c) {
     }
 }
 
+// The same sin / cos as one straight-line sequence for the arguments where
+// the case-by-case code has several paths: __rem_pio2's cases for
+// |x| <= 9 pi / 4 become one arithmetic sequence with the multiple k of pi/2
+// and its constants picked by selects (x + k pio2_1 is computed as
+// x - (-k pio2_1), the same IEEE operation), |x| <= pi/4 takes y0 = x,
+// y1 = 0, n = 0, and the two kernels run once; only the medium reduction
+// (near multiples of pi/2, |x| > 9 pi / 4) is a branch.  The bits equal
+// sincos's.  A lone lane (the serial tail of a coalescing simulation) then
+// runs one basic block that the compiler interleaves with the logarithm of
+// the same Box-Muller draw; a full wave of angles runs the case-by-case code
+// about as fast.
+NH_HD void sincos_flat(double x, double &s, double &c) {
+    const double pio2_1 = 1.57079632673412561417e+00;
+    const double pio2_1t = 6.07710050650619224932e-11;
+    const uint64_t u = bits(x);
+    const bool neg = (u >> 63) != 0;
+    const uint32_t ix = (uint32_t)(u >> 32) & 0x7fffffffu;
+    const bool small = ix <= 0x3fe921fbu;
+    const bool medium = ix > 0x401c463bu || (ix <= 0x400f6a7au && (ix & 0xfffffu) == 0x921fbu) ||
+                        ix == 0x4012d97cu || ix == 0x401921fbu;
+    const int32_t k = ix <= 0x4002d97cu ? 1 : ix <= 0x400f6a7au ? 2 : ix <= 0x4015fdbcu ? 3 : 4;
+    const double ck = k == 1 ? pio2_1 : k == 2 ? 2 * pio2_1 : k == 3 ? 3 * pio2_1 : 4 * pio2_1;
+    const double tk = k == 1 ? pio2_1t : k == 2 ? 2 * pio2_1t : k == 3 ? 3 * pio2_1t : 4 * pio2_1t;
+    const double cs = neg ? -ck : ck, ts = neg ? -tk : tk;
+    const double z = x - cs;
+    double y0 = z - ts;
+    double y1 = (z - y0) - ts;
+    int32_t n = neg ? -k : k;
+    if (small) {
+        y0 = x;
+        y1 = 0.0;
+        n = 0;
+    } else if (medium && ix < 0x7ff00000u) {
+        n = rem_pio2(x, y0, y1);
+    }
+    // k_sin (iy = 0 for the unreduced small argument) and k_cos
+    const double S1 = -1.66666666666666324348e-01;
+    const double S2 = 8.33333333332248946124e-03;
+    const double S3 = -1.98412698298579493134e-04;
+    const double S4 = 2.75573137070700676789e-06;
+    const double S5 = -2.50507602534068634195e-08;
+    const double S6 = 1.58969099521155010221e-10;
+    const double zz = y0 * y0;
+    const double w = zz * zz;
+    const double r = S2 + zz * (S3 + zz * S4) + zz * w * (S5 + zz * S6);
+    const double v = zz * y0;
+    double ks = small ? y0 + v * (S1 + zz * r) : y0 - ((zz * (0.5 * y1 - v * r) - y1) - v * S1);
+    double kc = k_cos(y0, y1);
+    if (small) {
+        if (ix < 0x3e500000u) ks = x;
+        if (ix < 0x3e46a09eu && as_i32(x) == 0) kc = 1.0;
+    }
+    // quadrant n & 3: (s, c) = (ks, kc), (kc, -ks), (-ks, -kc), (-kc, ks)
+    const bool odd = (n & 1) != 0;
+    s = odd ? kc : ks;
+    c = odd ? ks : kc;
+    if (n & 2) s = -s;
+    if ((n + 1) & 2) c = -c;
+    if (ix >= 0x7ff00000u) s = c = x - x;
+}
+
 NH_HD double sin(double x) { double s, c; sincos(x, s, c); return s; }
 NH_HD double cos(double x) { double s, c; sincos(x, s, c); return c; }
 

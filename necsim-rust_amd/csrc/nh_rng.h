@@ -77,13 +77,16 @@ NH_HD double uniform_open_closed(uint64_t &state) {
 }
 
 // rng.rs:180-197 (Box-Muller with sigma and mu)
+// FLAT: the straight-line sin / cos (same bits), for lone lanes
+template <bool FLAT = false>
 NH_HD void sample_2d_normal(uint64_t &state, double mu, double sigma, double &a, double &b) {
     double u0 = uniform_open_closed(state);
     double u1 = uniform_closed_open(state);
     double r = sqrt_(-2.0 * ln(u0));
     double theta = -6.283185307179586 * u1; // -TAU
     double s, c;
-    sincos(theta, s, c);
+    if (FLAT) sincos_flat(theta, s, c);
+    else sincos(theta, s, c);
     a = (r * s) * sigma + mu;
     b = (r * c) * sigma + mu;
 }
