@@ -61,7 +61,10 @@ constexpr int RATIO_N = 256; // Poisson inversion ratios lambda*dt/k cached for 
 // by the same sequence of IEEE operations as the loop: the count is found by
 // comparisons against the table and the loop only runs past it.
 constexpr int PTBL = 8;
-constexpr int POISSON_TABLE_N = RATIO_N + 2 * PTBL; // ratio[RATIO_N], acc[PTBL], prod[PTBL]
+// ratio[RATIO_N], acc[PTBL], prod[PTBL], then floor(acc[k] * 2^53) as u64 bit
+// patterns: a U[0,1) draw is m * 2^-53 for an integer m < 2^53, so
+// U > acc[k] exactly when m > floor(acc[k] * 2^53)
+constexpr int POISSON_TABLE_N = RATIO_N + 3 * PTBL;
 
 // Per-simulation constants
 struct DevParams {
@@ -74,6 +77,7 @@ struct DevParams {
     double inv_delta_t;     // 1/delta_t when delta_t is a power of two
     int dt_pow2;
     double min_event_time;  // resume_after
+    uint64_t nu_m;          // ceil(nu * 2^53): a U[0,1) draw m * 2^-53 is below nu exactly when m < nu_m
     uint64_t step_slice;
     const double *ratio;    // POISSON_TABLE_N doubles: ratio[k] = lambda_dt / k (k in 1..RATIO_N-1),
                             // then acc[k] and prod[k] for k < PTBL (device, global)
@@ -85,7 +89,7 @@ struct DevParams {
 inline uint32_t poisson_table(double lambda_dt, double p0, double *t) {
     t[0] = 0.0;
     for (int k = 1; k < RATIO_N; k++) t[k] = lambda_dt / (double)k;
-    double *acc = t + RATIO_N, *prod = acc + PTBL;
+    double *acc = t + RATIO_N, *prod = acc + PTBL, *acc_m = prod + PTBL;
     double pr = p0, ac = p0;
     uint32_t zero_at = PTBL;
     for (int k = 0; k < PTBL; k++) {
@@ -96,8 +100,17 @@ inline uint32_t poisson_table(double lambda_dt, double p0, double *t) {
         acc[k] = ac;
         prod[k] = pr;
         if (!(pr > 0.0) && zero_at == PTBL) zero_at = (uint32_t)k;
+        // ac * 2^53 is exact; past 2^53 no draw exceeds it
+        const double a = ac * 9007199254740992.0;
+        acc_m[k] = from_bits(a >= 9007199254740992.0 ? (1ull << 53) : (uint64_t)__builtin_floor(a));
     }
     return zero_at;
+}
+
+// ceil(nu * 2^53) for DevParams::nu_m (nu in [0, 1])
+inline uint64_t draw_threshold_below(double nu) {
+    const double a = nu * 9007199254740992.0;
+    return a >= 9007199254740992.0 ? (1ull << 53) : (a > 0.0 ? (uint64_t)__builtin_ceil(a) : 0ull);
 }
 
 constexpr uint8_t FLAG_EXACT = 1; // next_t / next_st hold the exact next event
@@ -221,17 +234,19 @@ __device__ __forceinline__ uint32_t poisson_count(const DevParams &P, const doub
     if (P.p0 > 0.0) {
         // while u > acc_k && prod_k > 0: k += 1 (acc is non-decreasing, so
         // the loop ends at min(first k with u <= acc_k, first k with prod_k == 0))
-        const double *acc_t = ratio + RATIO_N, *prod_t = acc_t + PTBL;
-        double u = uniform_closed_open(state);
-        // m = #{k < PTBL : u > acc_k}, by binary search over the sorted table
+        const double *acc_t = ratio + RATIO_N, *prod_t = acc_t + PTBL, *acc_m = prod_t + PTBL;
+        // the draw U = mu * 2^-53 (uniform_closed_open), compared as an integer
+        const uint64_t mu = sample_u64(state) >> 11;
+        // m = #{k < PTBL : U > acc_k}, by binary search over the sorted table
         static_assert(PTBL == 8, "three search steps");
         uint32_t m = 0;
-        m += u > acc_t[m + 3] ? 4u : 0u;
-        m += u > acc_t[m + 1] ? 2u : 0u;
-        m += u > acc_t[m] ? 1u : 0u;
-        m += (m == PTBL - 1 && u > acc_t[PTBL - 1]) ? 1u : 0u;
+        m += mu > bits(acc_m[m + 3]) ? 4u : 0u;
+        m += mu > bits(acc_m[m + 1]) ? 2u : 0u;
+        m += mu > bits(acc_m[m]) ? 1u : 0u;
+        m += (m == PTBL - 1 && mu > bits(acc_m[PTBL - 1])) ? 1u : 0u;
         uint32_t k = m < P.prod_zero_at ? m : P.prod_zero_at;
         if (k == PTBL) { // both past the table: continue the loop from its last entry
+            const double u = (double)mu * TWO_POW_M53;
             k = PTBL - 1;
             double prod = prod_t[k];
             double acc = acc_t[k];
@@ -380,17 +395,22 @@ __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S,
     uint64_t st = prime_at(P.seed, L.idh, L.next_st); // next_st holds the event's prime key
     L.have_next = false;
 
-    double u_spec = uniform_closed_open(st);
-    // SpeciationSample::update_min keeps the first minimum (tracking.rs:46-55)
-    if (TRACK_MS && !(L.ms_some && L.ms_u <= u_spec)) {
-        L.ms_some = true;
-        L.ms_u = u_spec;
-        L.ms_t = net;
-        L.ms_a = L.a;
-        L.ms_b = L.b;
-        L.ms_i = L.i;
+    // the speciation draw U = m * 2^-53 (uniform_closed_open); U < nu is
+    // m < nu_m, and only the dedup cache's min sample needs U itself
+    const uint64_t m_spec = sample_u64(st) >> 11;
+    if (TRACK_MS) {
+        const double u_spec = (double)m_spec * TWO_POW_M53;
+        // SpeciationSample::update_min keeps the first minimum (tracking.rs:46-55)
+        if (!(L.ms_some && L.ms_u <= u_spec)) {
+            L.ms_some = true;
+            L.ms_u = u_spec;
+            L.ms_t = net;
+            L.ms_a = L.a;
+            L.ms_b = L.b;
+            L.ms_i = L.i;
+        }
     }
-    if (u_spec < P.nu) {
+    if (m_spec < P.nu_m) {
         uint32_t x, y;
         location_xy<KIND>(S, L.a, L.b, x, y);
         ev.prior_time = L.t_last; // ev.ref is the caller's: the lineage's reference

@@ -107,6 +107,7 @@ struct ReqCounters {
 struct ServeReq {
     uint64_t seed;
     double nu;
+    uint64_t nu_m; // ceil(nu * 2^53) (DevParams::nu_m)
     double level;
     LineageSoA lin; // updated in place
     DevEvent *events;

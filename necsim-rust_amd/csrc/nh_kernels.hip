@@ -568,6 +568,7 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
                             req = lo;
                             P.seed = R.seed;
                             P.nu = R.nu;
+                            P.nu_m = R.nu_m;
                             level = R.level;
                         }
                         slot = s;

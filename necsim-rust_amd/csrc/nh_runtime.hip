@@ -448,6 +448,7 @@ static int job_start(nh_job *j) {
     DevParams &P = j->P;
     P.seed = p.rng_seed;
     P.nu = p.speciation_probability;
+    P.nu_m = draw_threshold_below(P.nu);
     P.lambda_dt = p.turnover_rate * p.delta_t;
     P.p0 = nh::exp(-P.lambda_dt);
     P.lambda_dt_sqrt = sqrt_(P.lambda_dt);

@@ -325,6 +325,7 @@ extern "C" int nh_server_tick(nh_server *s, uint64_t *n_active, uint64_t *n_queu
         ServeReq &d = s->h_reqs[k];
         d.seed = r->p.rng_seed;
         d.nu = r->p.speciation_probability;
+        d.nu_m = draw_threshold_below(d.nu);
         d.level = r->level;
         d.lin = r->soa[r->cur].s;
         d.events = r->d_events;
