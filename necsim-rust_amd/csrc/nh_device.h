@@ -441,8 +441,17 @@ __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S,
         // ((x + dx) % 2^32 + 2^32) % 2^32 is the low word of x + dx.
         double dx, dy;
         sample_2d_normal<FLAT>(st, 0.0, S.sigma, dx, dy);
-        int64_t ix = as_i64(round_(dx));
-        int64_t iy = as_i64(round_(dy));
+        // `as i64` of the rounded offsets; below 2^31 in magnitude (every
+        // practical sigma) that is the 32-bit conversion, one instruction
+        const double rx = round_(dx), ry = round_(dy);
+        int64_t ix, iy;
+        if (__builtin_fabs(rx) < 2147483648.0 && __builtin_fabs(ry) < 2147483648.0) {
+            ix = (int32_t)rx;
+            iy = (int32_t)ry;
+        } else {
+            ix = as_i64(rx);
+            iy = as_i64(ry);
+        }
         (void)uniform_closed_open(st); // CoalescenceRngSample::new, unused by singleton demes
         L.a = (uint32_t)((uint64_t)L.a + (uint64_t)ix);
         L.b = (uint32_t)((uint64_t)L.b + (uint64_t)iy);
