@@ -298,7 +298,17 @@ __device__ __forceinline__ bool poisson_window(const DevParams &P, const double 
 }
 
 __device__ __forceinline__ uint64_t poisson_first_window(const DevParams &P, double time) {
-    return as_u64(floor_(P.dt_pow2 ? time * P.inv_delta_t : time / P.delta_t));
+    const double w = floor_(P.dt_pow2 ? time * P.inv_delta_t : time / P.delta_t);
+    // below 2^32 (every practical simulation) one 32-bit conversion gives
+    // the saturating u64 cast's value
+    if (w >= 0.0 && w < 4294967296.0) return (uint64_t)(uint32_t)w;
+    return as_u64(w);
+}
+
+// (double)ts, with one 32-bit conversion below 2^32
+__device__ __forceinline__ double window_f64(uint64_t ts) {
+    if (ts < (1ull << 32)) return (double)(uint32_t)ts;
+    return (double)ts;
 }
 
 // The whole search: windows ts, ts + 1, ... until one holds an event.
@@ -357,7 +367,7 @@ __device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams 
         if (L.win_left == 0) L.search_ts++;
     }
     if (L.win_left == 0) return;
-    const double tsd = (double)L.search_ts;
+    const double tsd = window_f64(L.search_ts);
 #pragma unroll
     for (uint32_t k = 0; k < STEP_DRAWS; k++) {
         if (L.win_left == 0) break;
