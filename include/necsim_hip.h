@@ -193,8 +193,8 @@ typedef struct {
                                       0 = default 16.  Results do not depend on it. */
     uint32_t coop_search;          /* persistent form: lanes that are not searching evaluate the next
                                       Poisson windows of those that are (shortens the serial chains of
-                                      sparse waves); 0 = automatic (with dedup, or fewer than 4
-                                      lineages per lane, and not with report_dispersal), 1 = off,
+                                      sparse waves); 0 = automatic (with dedup, or fewer lineages
+                                      than lanes, and not with report_dispersal), 1 = off,
                                       2 = on.  Results do not depend on it. */
     uint32_t task_list_size;       /* NH_FORM_TASK_LIST: lineages per launch; 0 = 4096, the reference
                                       CUDA algorithm's default grid x block of 64 x 64 */

@@ -529,7 +529,7 @@ static int job_start(nh_job *j) {
         // lineages: coalescing populations (dedup) and small samples
         const uint64_t lanes = (uint64_t)per_cu * j->ctx->cus * (uint64_t)j->threads;
         if (p.coop_search == 2) j->coop = true;
-        else if (p.coop_search == 0) j->coop = !j->disp && (j->cache_cap != 0 || n < 4 * lanes);
+        else if (p.coop_search == 0) j->coop = !j->disp && (j->cache_cap != 0 || n < lanes);
         if (j->coop) {
             per_cu = persistent_max_blocks_per_cu(j->tables->kind, j->threads, j->cache_cap != 0, false, true);
             if (p.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)p.blocks_per_cu);
