@@ -1253,9 +1253,10 @@ __device__ __forceinline__ uint64_t block_first_head(const unsigned long long *k
 }
 
 // Sorts the window's runs by (time key, prior, reference, time-sorted
-// position) with a bitonic sort in LDS and writes their permutation.
-// Unused event slots (ghost keys) sort after every event and count as runs
-// of one.
+// position) with a bitonic sort in LDS over the next power of two >= the
+// window's span and writes their permutation; a window without equal times
+// is copied.  Unused event slots (ghost keys) sort after every event and
+// count as runs of one.
 __global__ void __launch_bounds__(TIE_THREADS) k_tie_sort_windows(const unsigned long long *keys, const uint32_t *vals,
                                                                   uint64_t n, const unsigned long long *prior_ws,
                                                                   const unsigned long long *ref_ws,
@@ -1270,7 +1271,10 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_sort_windows(const unsigned
     const uint64_t tail_hi = hi + TIE_RUN_MAX < n ? hi + TIE_RUN_MAX : n;
     const uint64_t e = hi < n ? block_first_head(keys, hi, tail_hi) : n;
     const int len = (int)(e - s);
-    for (int i = threadIdx.x; i < TIE_SORT_N; i += TIE_THREADS) {
+    // the network spans the next power of two >= len (padded with ~0 keys)
+    int nsort = 64;
+    while (nsort < len) nsort <<= 1;
+    for (int i = threadIdx.x; i < nsort; i += TIE_THREADS) {
         if (i < len) {
             s_t[i] = keys[s + i];
             s_p[i] = prior_ws[s + i];
@@ -1283,10 +1287,18 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_sort_windows(const unsigned
         s_i[i] = (uint16_t)i;
     }
     __syncthreads();
+    // a window without equal times keeps its order
+    int tie = 0;
+    for (int i = threadIdx.x + 1; i < len; i += TIE_THREADS)
+        tie |= (s_t[i] == s_t[i - 1] && s_t[i] != GHOST_TIME_BITS) ? 1 : 0;
+    if (!__syncthreads_or(tie)) {
+        for (int i = threadIdx.x; i < len; i += TIE_THREADS) out_vals[s + i] = vals[s + i];
+        return;
+    }
     // bitonic network: in stage (k, j), pair q compares i and i + j
-    for (int k = 2; k <= TIE_SORT_N; k <<= 1) {
+    for (int k = 2; k <= nsort; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int q = threadIdx.x; q < TIE_SORT_N / 2; q += TIE_THREADS) {
+            for (int q = threadIdx.x; q < nsort / 2; q += TIE_THREADS) {
                 const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
                 const int l = i + j;
                 const bool ascending = (i & k) == 0;
