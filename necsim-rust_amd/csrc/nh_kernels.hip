@@ -1215,21 +1215,46 @@ __global__ void __launch_bounds__(KS_THREADS) k_tie_gather(const E *ev, const un
                                                            unsigned long long *prior_ws, unsigned long long *ref_ws,
                                                            unsigned long long *flags) {
     using O = EventOrder<E>;
+    constexpr int NF = O::N - 3; // the origin / target fields 2 .. N - 2 (32-bit values)
+    // each event's origin / target fields, for the thread that compares the
+    // next position with it (a ring of three iterations: a slot is rewritten
+    // only after every thread has passed the barrier that follows its reads)
+    __shared__ uint32_t s_f[3][KS_THREADS][NF];
     const uint64_t base = (uint64_t)blockIdx.x * KS_TILE;
     if (base >= n) return;
     unsigned long long mixed = 0, longer = 0;
     for (int j = 0; j < KS_ITEMS; j++) {
         const uint64_t p = base + (uint64_t)j * KS_THREADS + threadIdx.x;
-        if (p >= n) continue;
-        const E e = ev[vals[p]];
-        prior_ws[p] = O::field(e, 1);
-        ref_ws[p] = O::field(e, 0);
+        const bool have = p < n;
+        uint32_t f[NF];
+        if (have) {
+            const E e = ev[vals[p]];
+            prior_ws[p] = O::field(e, 1);
+            ref_ws[p] = O::field(e, 0);
+#pragma unroll
+            for (int k = 0; k < NF; k++) f[k] = s_f[j % 3][threadIdx.x][k] = (uint32_t)O::field(e, 2 + k);
+        }
+        __syncthreads();
+        if (!have) continue;
         const unsigned long long k = keys[p];
         const bool ghost = k == GHOST_TIME_BITS; // unused event slots: runs of one
         const bool head = ghost || p == 0 || keys[p - 1] != k;
         if (!head) {
-            const E q = ev[vals[p - 1]];
-            for (int f = 2; f < O::N - 1; f++) mixed |= O::field(e, f) != O::field(q, f) ? 1ull : 0ull;
+            // position p - 1: the previous thread, the last thread of the
+            // previous iteration, or (the block's first position) memory
+            uint32_t q[NF];
+            if (threadIdx.x > 0 || j > 0) {
+                const int qt = threadIdx.x > 0 ? (int)threadIdx.x - 1 : KS_THREADS - 1;
+                const int qj = threadIdx.x > 0 ? j : j - 1;
+#pragma unroll
+                for (int kk = 0; kk < NF; kk++) q[kk] = s_f[qj % 3][qt][kk];
+            } else {
+                const E qe = ev[vals[p - 1]];
+#pragma unroll
+                for (int kk = 0; kk < NF; kk++) q[kk] = (uint32_t)O::field(qe, 2 + kk);
+            }
+#pragma unroll
+            for (int kk = 0; kk < NF; kk++) mixed |= f[kk] != q[kk] ? 1ull : 0ull;
         } else if (!ghost && p + TIE_RUN_MAX < n && keys[p + TIE_RUN_MAX] == k) {
             longer = 1;
         }
