@@ -21,6 +21,7 @@ workload.
 """
 import argparse
 import dataclasses
+import gc
 import json
 import os
 import sys
@@ -610,6 +611,7 @@ def main():
         log(f"config 3: {extra['config3_window']}")
         extra["gather_window"] = gather_run(ctx, 2000.0)
         log(f"gather: {extra['gather_window']}")
+        gc.collect()  # the earlier parts' host arrays: the serving loop polls from the host
         extra["serving_config5"] = serving_run(ctx)
         log(f"serving: {extra['serving_config5']}")
 
