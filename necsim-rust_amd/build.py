@@ -22,6 +22,9 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
+# per-file extras: the round kernels are VALU-issue bound, and the GCN
+# max-ILP machine scheduler measured 1 % faster on them than the default
+EXTRA = {"nh_kernels.hip": ["-Xarch_device", "-mllvm=-misched=gcn-max-ilp"]}
 
 
 def build(verbose=False):
@@ -31,7 +34,7 @@ def build(verbose=False):
     for src in SOURCES:
         obj = os.path.join(HERE, "_build", src.rsplit(".", 1)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmd = [HIPCC] + FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + FLAGS + EXTRA.get(src, []) + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
         objs.append(obj)
     for src, p in procs:
