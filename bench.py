@@ -183,7 +183,7 @@ def gather_run(ctx, window):
     return r
 
 
-def serving_run(ctx, n_requests=150, rate=50.0, cap=1000.0, tick_steps=250, seed=1):
+def serving_run(ctx, n_requests=150, rate=50.0, cap=1000.0, tick_steps=1000, seed=1):
     """config 5: Poisson stream of simulations on a shared 256x256 map"""
     h, rp, c, w = nh.generate_map(256, 256, seed=42)
     scn = nh.SpatiallyExplicitUniformTurnoverScenario(256, 256, h, rp, c, w)
