@@ -448,8 +448,11 @@ int nh_lineage_file_read(const char *path, nh_lineage *buf, size_t cap, size_t *
  * op: 0 ln, 1 exp, 2 sin, 3 cos, 4 sqrt, 5 floor, 6 round, 7 divide
  * (in[2i] / in[2i+1]), 8 wymum (u64 pairs), 9 seahash_diffuse (u64),
  * 10 prime (in[3i..3i+2] = seed, location, time; u64), 11 uniform_closed_open
- * after prime (same inputs, f64 out).  in / out are host buffers of 8-byte
- * words; n is the number of outputs. */
+ * after prime (same inputs, f64 out), 12 the Poisson event-time sampler's
+ * next event (in[3i..3i+2] = seed, location, time bits for i < n / 2, then
+ * bits of lambda and delta_t; out[2i] = event time bits, out[2i + 1] = the
+ * WyHash state primed for that event's draws).  in / out are host buffers
+ * of 8-byte words; n is the number of outputs. */
 int nh_debug_eval(nh_ctx *ctx, int32_t op, const uint64_t *in, uint64_t *out, size_t n);
 
 /* Diagnostics, no counterpart in the reference: times `reps` launches of a

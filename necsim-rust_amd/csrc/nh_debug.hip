@@ -1,6 +1,7 @@
 // nh_debug.hip -- diagnostics: elementwise evaluation of device primitives
 // and a plain copy kernel of known size (the HBM byte-counter calibration).
 #include <cstring>
+#include <vector>
 
 #include "../../include/necsim_hip.h"
 #include "nh_internal.h"
@@ -35,6 +36,20 @@ __global__ void k_debug_eval(int op, const uint64_t *in, uint64_t *out, size_t n
     default: r = 0;
     }
     out[i] = r;
+}
+
+// op 12: the Poisson event-time sampler's next event (poisson_next) of
+// (seed, location, time) triples under one DevParams
+__global__ void k_debug_poisson(DevParams P, const uint64_t *in, uint64_t *out, size_t m) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    DevParams Q = P;
+    Q.seed = in[3 * i];
+    const uint64_t idh = prime_location(in[3 * i + 1]);
+    uint64_t key = 0;
+    const double et = poisson_next(Q, P.ratio, idh, from_bits(in[3 * i + 2]), key);
+    out[2 * i] = bits(et);
+    out[2 * i + 1] = prime_at(Q.seed, idh, key);
 }
 // 16-byte vector copy, grid-stride: reads and writes each byte exactly once
 __global__ void __launch_bounds__(256) k_copy_calibrate(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
@@ -87,24 +102,49 @@ extern "C" int nh_copy_calibrate(nh_ctx *ctx, size_t bytes, int32_t reps, double
 }
 
 extern "C" int nh_debug_eval(nh_ctx *ctx, int32_t op, const uint64_t *in, uint64_t *out, size_t n) {
-    if (!ctx || !in || !out || op < 0 || op > 11) return set_error(NH_E_INVALID, "nh_debug_eval: bad argument");
+    if (!ctx || !in || !out || op < 0 || op > 12 || (op == 12 && n % 2))
+        return set_error(NH_E_INVALID, "nh_debug_eval: bad argument");
     if (n == 0) return NH_OK;
     const nh_ctx *c = ctx;
-    size_t n_in = op == 7 || op == 8 ? 2 * n : (op >= 10 ? 3 * n : n);
+    const size_t m = n / 2; // op 12: triples
+    size_t n_in = op == 12 ? 3 * m + 2 : (op == 7 || op == 8 ? 2 * n : (op >= 10 ? 3 * n : n));
     uint64_t *d_in = nullptr, *d_out = nullptr;
+    double *d_ratio = nullptr;
     if (hipSetDevice(c->device) != hipSuccess || hipMalloc(&d_in, n_in * 8) != hipSuccess ||
-        hipMalloc(&d_out, n * 8) != hipSuccess) {
+        hipMalloc(&d_out, n * 8) != hipSuccess ||
+        (op == 12 && hipMalloc(&d_ratio, POISSON_TABLE_N * sizeof(double)) != hipSuccess)) {
         if (d_in) (void)hipFree(d_in);
+        if (d_out) (void)hipFree(d_out);
         return set_error(NH_E_NOMEM, "nh_debug_eval: allocation failed");
     }
     hipError_t e = hipMemcpy(d_in, in, n_in * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
+    if (e == hipSuccess && op == 12) {
+        // the sampler's constants as a job evaluates them (poisson.rs:39-42)
+        DevParams P{};
+        const double lambda = from_bits(in[3 * m]), delta_t = from_bits(in[3 * m + 1]);
+        P.lambda_dt = lambda * delta_t;
+        P.p0 = nh::exp(-P.lambda_dt);
+        P.lambda_dt_sqrt = sqrt_(P.lambda_dt);
+        P.delta_t = delta_t;
+        const uint64_t b = bits(delta_t);
+        P.dt_pow2 = ((b >> 52) & 0x7ff) != 0 && (b & 0xfffffffffffffull) == 0;
+        P.inv_delta_t = P.dt_pow2 ? 1.0 / delta_t : 0.0;
+        std::vector<double> ratio(POISSON_TABLE_N, 0.0);
+        P.prod_zero_at = poisson_table(P.lambda_dt, P.p0, ratio.data());
+        e = hipMemcpy(d_ratio, ratio.data(), POISSON_TABLE_N * sizeof(double), hipMemcpyHostToDevice);
+        P.ratio = d_ratio;
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_debug_poisson, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, 0, P, d_in, d_out, m);
+            e = hipGetLastError();
+        }
+    } else if (e == hipSuccess) {
         hipLaunchKernelGGL(k_debug_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, op, d_in, d_out, n);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(out, d_out, n * 8, hipMemcpyDeviceToHost);
     (void)hipFree(d_in);
     (void)hipFree(d_out);
+    if (d_ratio) (void)hipFree(d_ratio);
     if (e != hipSuccess) return set_error(NH_E_HIP, "nh_debug_eval: %s", hipGetErrorString(e));
     return NH_OK;
 }

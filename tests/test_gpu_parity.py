@@ -109,6 +109,31 @@ def test_device_rng(ctx):
     assert np.array_equal(got, np.array(want).view(np.uint64))
 
 
+@pytest.mark.parametrize("delta_t", [2.0, 3.0, 40.0, 2000.0])
+def test_device_poisson_sampler(ctx, delta_t):
+    """PoissonEventTimeSampler::next_event_time_at_indexed_location_weakly_after
+    (poisson.rs:30-109) on the device against the oracle: lambda * delta_t of 1
+    (table), 1.5 (division path), 20 (counts past the table) and 1000 (the
+    normal approximation); the event time and the state primed for its draws"""
+    announce(f"device Poisson sampler, delta_t {delta_t}")
+    rng = np.random.default_rng(int(delta_t))
+    m = 20_000 if delta_t < 100 else 4_000
+    seeds = rng.integers(0, 2**63, m, dtype=np.uint64)
+    locs = rng.integers(0, 2**63, m, dtype=np.uint64)
+    times = rng.uniform(0.0, 1e4, m)
+    times[:16] = [0.0, 2.0, 4.0, 1.9999999999999998, 3.0, 6.0, 1e-300, 5e-324, 2000.0, 1999.9999999999998,
+                  4000.0, 40.0, 39.99999999999999, 80.0, 1.0, 0.5]
+    words = np.concatenate([np.stack([seeds, locs, times.view(np.uint64)], 1).ravel(),
+                            np.array([0.5, delta_t]).view(np.uint64)])
+    got = device_eval(ctx, 12, words, 2 * m).reshape(m, 2)
+    st = ctypes.c_uint64()
+    for k in range(m):
+        t = O.lib().orc_poisson_next_event_time(int(seeds[k]), int(locs[k]), float(times[k]), 0.5, delta_t,
+                                                ctypes.byref(st))
+        assert got[k, 0] == np.float64(t).view(np.uint64) and got[k, 1] == st.value, \
+            f"item {k}: time {times[k]!r}, device {got[k, 0].view(np.float64)!r}, oracle {t!r}"
+
+
 # ---------------------------------------------------------------------------
 # config 1: NonSpatial 100x100, deme 1, nu 1e-3, seed 42, 100% sampled
 @pytest.fixture(scope="module")
