@@ -267,9 +267,6 @@ __device__ __forceinline__ uint32_t poisson_count(const DevParams &P, const doub
 
 // The window's `number` event times (ts + U[0,1)) * delta_t, drawn in order
 // from `state`; the first strictly smallest one after `time` is the event.
-// Event e's draw is the (e + 1)-th sample after `state`, i.e. the one made
-// from the WyHash state state + (e + 1) * P0 (sample_u64 adds P0, then
-// hashes), so the draws can also be evaluated out of order.
 __device__ __forceinline__ bool poisson_events(const DevParams &P, uint64_t state, uint32_t number, uint64_t ts,
                                                double time, double &best, uint64_t &key) {
     bool found = false;
