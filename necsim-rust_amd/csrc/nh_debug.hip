@@ -45,11 +45,11 @@ __global__ void k_debug_poisson(DevParams P, const uint64_t *in, uint64_t *out, 
     if (i >= m) return;
     DevParams Q = P;
     Q.seed = in[3 * i];
-    const uint64_t idh = prime_location(in[3 * i + 1]);
+    const uint64_t lkey = prime_key(Q.seed, in[3 * i + 1]);
     uint64_t key = 0;
-    const double et = poisson_next(Q, P.ratio, idh, from_bits(in[3 * i + 2]), key);
+    const double et = poisson_next(Q, P.ratio, lkey, from_bits(in[3 * i + 2]), key);
     out[2 * i] = bits(et);
-    out[2 * i + 1] = prime_at(Q.seed, idh, key);
+    out[2 * i + 1] = prime_keyed(lkey, key);
 }
 // 16-byte vector copy, grid-stride: reads and writes each byte exactly once
 __global__ void __launch_bounds__(256) k_copy_calibrate(const uint4 *__restrict__ src, uint4 *__restrict__ dst,

@@ -151,13 +151,13 @@ struct DevPacked {
 static_assert(sizeof(DevPacked) == 56, "packed event is 56 B");
 
 // Lane-resident lineage.  a/b hold (x, y) for the almost-infinite habitat and
-// (cell index, unused) for finite extents; `idh` is the injective location
-// id hashed for priming the RNG (prime_location).
+// (cell index, unused) for finite extents; `lkey` is the injective location
+// id hashed with the seed for priming the RNG (prime_key).
 template <int KIND> struct Lane {
     uint64_t ref;
     double t_last;
     uint32_t a, b, i;
-    uint64_t idh;
+    uint64_t lkey;
     CellRec rec;        // finite in-memory kinds: record of the current cell
     double next_t;
     uint64_t next_st; // prime key of the next event
@@ -199,22 +199,22 @@ __device__ __forceinline__ void location_xy(const DevScenario &S, uint32_t a, ui
 // outside the habitat extent.
 template <int KIND>
 __device__ __forceinline__ void lane_set_location(Lane<KIND> &L, const DevScenario &S, uint32_t x,
-                                                  uint32_t y, uint32_t i) {
+                                                  uint32_t y, uint32_t i, uint64_t seed) {
     L.i = i;
     if (KIND == K_ALMOST_INFINITE) {
         L.a = x;
         L.b = y;
-        L.idh = prime_location(((uint64_t)y << 32) | (uint64_t)x); // habitat/almost_infinite.rs:67-71
+        L.lkey = prime_key(seed, ((uint64_t)y << 32) | (uint64_t)x); // habitat/almost_infinite.rs:67-71
     } else {
         uint32_t cell = (y - S.oy) * S.width + (x - S.ox);
         L.a = cell;
         L.b = 0;
         if (KIND == K_NON_SPATIAL) {
             // habitat/non_spatial.rs:93-109
-            L.idh = prime_location((uint64_t)cell * (uint64_t)S.deme + (uint64_t)i);
+            L.lkey = prime_key(seed, (uint64_t)cell * (uint64_t)S.deme + (uint64_t)i);
         } else {
             L.rec = S.cells[cell];
-            L.idh = prime_location(L.rec.prefix + (uint64_t)i); // habitat/in_memory.rs:71-81
+            L.lkey = prime_key(seed, L.rec.prefix + (uint64_t)i); // habitat/in_memory.rs:71-81
         }
     }
 }
@@ -285,11 +285,11 @@ __device__ __forceinline__ bool poisson_events(const DevParams &P, uint64_t stat
     return found;
 }
 
-// One window at the location hashed as idh (prime_location): false if it
+// One window at the location keyed as lkey (prime_key): false if it
 // holds no event after `time`, else sets best / key.
-__device__ __forceinline__ bool poisson_window(const DevParams &P, const double *ratio, uint64_t idh,
+__device__ __forceinline__ bool poisson_window(const DevParams &P, const double *ratio, uint64_t lkey,
                                                double time, uint64_t ts, double &best, uint64_t &key) {
-    uint64_t state = prime_at(P.seed, idh, ts);
+    uint64_t state = prime_keyed(lkey, ts);
     const uint32_t number = poisson_count(P, ratio, state);
     return poisson_events(P, state, number, ts, time, best, key);
 }
@@ -309,11 +309,11 @@ __device__ __forceinline__ double window_f64(uint64_t ts) {
 }
 
 // The whole search: windows ts, ts + 1, ... until one holds an event.
-__device__ __forceinline__ double poisson_next(const DevParams &P, const double *ratio, uint64_t idh,
+__device__ __forceinline__ double poisson_next(const DevParams &P, const double *ratio, uint64_t lkey,
                                                double time, uint64_t &key) {
     double best = 0.0;
     for (uint64_t ts = poisson_first_window(P, time);; ts++)
-        if (poisson_window(P, ratio, idh, time, ts, best, key)) return best;
+        if (poisson_window(P, ratio, lkey, time, ts, best, key)) return best;
 }
 
 // Make sure L.next_t / L.next_st hold the next event (the `pop` peek).
@@ -323,7 +323,7 @@ __device__ __forceinline__ void lane_peek(Lane<KIND> &L, const DevParams &P, con
     // NonNegativeF64::max of last_event_time and min_event_time (sampler.rs:87)
     double before = P.min_event_time > L.t_last ? P.min_event_time : L.t_last;
     uint64_t st;
-    double et = poisson_next(P, ratio, L.idh, before, st);
+    double et = poisson_next(P, ratio, L.lkey, before, st);
     L.next_t = max_after(before, et);
     L.next_st = st;
     L.have_next = true;
@@ -356,7 +356,7 @@ __device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams 
     // open windows until one holds events (at most STEP_OPENS per call)
 #pragma unroll
     for (uint32_t o = 0; o < STEP_OPENS && L.win_left == 0; o++) {
-        uint64_t st = prime_at(P.seed, L.idh, L.search_ts);
+        uint64_t st = prime_keyed(L.lkey, L.search_ts);
         L.win_left = poisson_count(P, ratio, st);
         L.next_st = st;
         L.win_e = 0;
@@ -399,7 +399,7 @@ template <int KIND, bool TRACK_MS = true, bool FLAT = false>
 __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S, const DevParams &P,
                                             DevEvent &ev) {
     double net = L.next_t;
-    uint64_t st = prime_at(P.seed, L.idh, L.next_st); // next_st holds the event's prime key
+    uint64_t st = prime_keyed(L.lkey, L.next_st); // next_st holds the event's prime key
     L.have_next = false;
 
     // the speciation draw U = m * 2^-53 (uniform_closed_open); U < nu is
@@ -441,7 +441,7 @@ __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S,
         uint32_t ci = as_u32(floor_(uc * S.deme_f));
         L.a = (uint32_t)cell;
         L.i = ci;
-        L.idh = prime_location(cell * (uint64_t)S.deme + (uint64_t)ci);
+        L.lkey = prime_key(P.seed, cell * (uint64_t)S.deme + (uint64_t)ci);
     } else if (KIND == K_ALMOST_INFINITE) {
         // AlmostInfiniteNormalDispersalSampler + clamp_round_dispersal
         // (almost_infinite_normal.rs:52-69, habitat/almost_infinite.rs:98-117).
@@ -463,7 +463,7 @@ __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S,
         L.a = (uint32_t)((uint64_t)L.a + (uint64_t)ix);
         L.b = (uint32_t)((uint64_t)L.b + (uint64_t)iy);
         L.i = 0;
-        L.idh = prime_location(((uint64_t)L.b << 32) | (uint64_t)L.a);
+        L.lkey = prime_key(P.seed, ((uint64_t)L.b << 32) | (uint64_t)L.a);
     } else {
         // InMemoryPackedSeparableAliasDispersalSampler (packed_separable_alias/dispersal.rs:16-77)
         // and InMemoryPackedAliasDispersalSampler (packed_alias/dispersal.rs:16-52)
@@ -497,7 +497,7 @@ __device__ __forceinline__ int lane_execute(Lane<KIND> &L, const DevScenario &S,
         L.a = target;
         L.i = ci;
         L.rec = t;
-        L.idh = prime_location(t.prefix + (uint64_t)ci);
+        L.lkey = prime_key(P.seed, t.prefix + (uint64_t)ci);
     }
     L.t_last = net;
     return STEP_DISPERSED;

@@ -27,7 +27,7 @@ __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) -
 // refill waits for one memory round trip; returns the slot's flags.
 template <int KIND>
 __device__ __forceinline__ uint8_t lane_load(Lane<KIND> &L, const DevScenario &S, const LineageSoA &s,
-                                             uint64_t slot) {
+                                             uint64_t slot, uint64_t seed) {
     uint8_t f = s.flags[slot];
     L.ref = s.ref[slot];
     L.t_last = s.t_last[slot];
@@ -38,7 +38,7 @@ __device__ __forceinline__ uint8_t lane_load(Lane<KIND> &L, const DevScenario &S
     L.next_st = L.have_next ? nst : 0;
     L.searching = false;
     L.ms_some = false;
-    if (!(f & FLAG_DEAD)) lane_set_location<KIND>(L, S, x, y, i);
+    if (!(f & FLAG_DEAD)) lane_set_location<KIND>(L, S, x, y, i, seed);
     return f;
 }
 
@@ -303,9 +303,9 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// LANE_SEED: lanes may hold lineages of different simulations (serving), so
-// a helper also takes its searcher's seed.
-template <int KIND, bool LANE_SEED = false>
+// Lanes may hold lineages of different simulations (serving): the location
+// key a helper takes from its searcher carries the searcher's seed.
+template <int KIND>
 __device__ __forceinline__ void coop_search_window(Lane<KIND> &L, bool srch, const DevParams &P,
                                                    const double *ratio, CoopLds &c) {
     const uint64_t ms = __ballot(srch);
@@ -327,19 +327,12 @@ __device__ __forceinline__ void coop_search_window(Lane<KIND> &L, bool srch, con
             off = 1 + r / ns;
         }
     }
-    const uint64_t idh = __shfl(L.idh, tgt);
+    const uint64_t lkey = __shfl(L.lkey, tgt);
     const double before = __shfl(L.search_before, tgt);
     const uint64_t ts = __shfl(L.search_ts, tgt);
     double et = 0.0;
     uint64_t key = 0;
-    bool found;
-    if constexpr (LANE_SEED) {
-        DevParams Pt = P;
-        Pt.seed = __shfl(P.seed, tgt);
-        found = poisson_window(Pt, ratio, idh, before, ts + off, et, key);
-    } else {
-        found = poisson_window(P, ratio, idh, before, ts + off, et, key);
-    }
+    bool found = poisson_window(P, ratio, lkey, before, ts + off, et, key);
     if (nh) {
         if (!srch && found) {
             c.et[threadIdx.x] = et;
@@ -413,7 +406,7 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
             uint64_t need = __ballot(!has);
             if (need) {
                 unsigned long long w = wave_claim(q, need, !has, &A.ctr->head, n_in, A.round->chunk);
-                if (w != ~0ull && !(lane_load<KIND>(L, A.S, A.lin, w) & FLAG_DEAD)) {
+                if (w != ~0ull && !(lane_load<KIND>(L, A.S, A.lin, w, A.P.seed) & FLAG_DEAD)) {
                     slot = w;
                     has = true;
                     slice_steps = 0;
@@ -562,7 +555,7 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
                     }
                     const ServeReq &R = A.reqs[lo];
                     unsigned long long s = w - s_base[lo];
-                    if (!(lane_load<KIND>(L, A.S, R.lin, s) & FLAG_DEAD)) {
+                    if (!(lane_load<KIND>(L, A.S, R.lin, s, R.seed) & FLAG_DEAD)) {
                         if (lo != req) {
                             flush(req);
                             req = lo;
@@ -595,7 +588,7 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
                     L.search_ts = poisson_first_window(P, L.search_before);
                     L.searching = true;
                 }
-                coop_search_window<KIND, true>(L, srch, P, s_ratio, s_coop);
+                coop_search_window<KIND>(L, srch, P, s_ratio, s_coop);
                 if (!has) continue;
                 if (!wave_ready(L.have_next, 5) || !L.have_next) continue;
                 bool leave = L.next_t >= level || (A.tick_steps && tick >= A.tick_steps);
@@ -714,7 +707,7 @@ __global__ void __launch_bounds__(256) k_plain(PlainArgs A) {
             A.status[q] = ST_SKIP;
         } else {
             Lane<KIND> L;
-            lane_load<KIND>(L, A.S, A.L, slot);
+            lane_load<KIND>(L, A.S, A.L, slot, A.P.seed);
             uint8_t st;
             for (;;) {
                 lane_peek<KIND>(L, A.P, s_ratio);

@@ -49,16 +49,19 @@ NH_HD uint64_t rot32(uint64_t x) { return (x << 32) | (x >> 32); }
 
 // PrimeableRng::prime_with, wyhash.rs:67-80.  Returns the new `state`; the
 // `seed` never changes after seeding.
-// prime_location is its location half, which every prime at one location
-// shares: the kernels hash a lineage's location once per move.
-NH_HD uint64_t prime_location(uint64_t location_index) { return rot32(seahash_diffuse(location_index)); }
-NH_HD uint64_t prime_at(uint64_t seed, uint64_t location_hash, uint64_t time_index) {
+// prime_key is the part that depends only on the seed and the location,
+// which every prime of one simulation at one location shares: the kernels
+// compute it once per move, prime_keyed adds the time.
+NH_HD uint64_t prime_key(uint64_t seed, uint64_t location_index) {
+    return rot32(seahash_diffuse(location_index)) ^ (seed ^ WY_P0);
+}
+NH_HD uint64_t prime_keyed(uint64_t location_key, uint64_t time_index) {
     uint64_t t = seahash_diffuse(time_index);
-    uint64_t h = wymum(location_hash ^ (seed ^ WY_P0), rot32(t) ^ WY_P2);
+    uint64_t h = wymum(location_key, rot32(t) ^ WY_P2);
     return wymum(h, 16ull ^ WY_P5);
 }
 NH_HD uint64_t prime(uint64_t seed, uint64_t location_index, uint64_t time_index) {
-    return prime_at(seed, prime_location(location_index), time_index);
+    return prime_keyed(prime_key(seed, location_index), time_index);
 }
 
 // RngCore::sample_u64, wyhash.rs:53-63
