@@ -370,6 +370,138 @@ double orc_cos(double x) {
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Test-only twins in fdlibm 5.3's evaluation order (e_log.c, k_sin.c,       */
+/* k_cos.c), built from the SAME constants and the same pi/2 reduction as    */
+/* orc_ln / orc_sin / orc_cos above.  They are not on the simulated path:    */
+/* an independent fdlibm port (V8's Math.log / Math.sin / Math.cos) must     */
+/* reproduce them bit for bit, which pins every constant and the reduction   */
+/* of the restatement; only the musl evaluation order of the final           */
+/* assembly (which the libm crate follows) remains pinned by the <= 1 ulp    */
+/* checks alone.                                                             */
+/* ------------------------------------------------------------------------ */
+double orc_fdlibm_log(double x) {
+    const double two54 = of_bits(0x4350000000000000ull);
+    uint64_t ui = bits_of(x);
+    int32_t hx = (int32_t)(ui >> 32);
+    uint32_t lx = (uint32_t)ui;
+    int32_t k = 0;
+    if (hx < 0x00100000) {
+        if (((hx & 0x7fffffff) | lx) == 0) return -1.0 / (x * x);
+        if (hx < 0) return (x - x) / 0.0;
+        k -= 54;
+        x *= two54;
+        hx = (int32_t)(bits_of(x) >> 32);
+    }
+    if (hx >= 0x7ff00000) return x + x;
+    k += (hx >> 20) - 1023;
+    hx &= 0x000fffff;
+    int32_t i = (hx + 0x95f64) & 0x100000;
+    x = of_bits(((uint64_t)(uint32_t)(hx | (i ^ 0x3ff00000)) << 32) | (bits_of(x) & 0xffffffffull));
+    k += (i >> 20);
+    double f = x - 1.0, dk, R, hfsq;
+    if ((0x000fffff & (2 + hx)) < 3) { /* -2^-20 <= f < 2^-20 */
+        if (f == 0.0) {
+            if (k == 0) return 0.0;
+            dk = (double)k;
+            return dk * ln2_hi + dk * ln2_lo;
+        }
+        R = f * f * (0.5 - 0.33333333333333333 * f);
+        if (k == 0) return f - R;
+        dk = (double)k;
+        return dk * ln2_hi - ((R - dk * ln2_lo) - f);
+    }
+    double s = f / (2.0 + f);
+    dk = (double)k;
+    double z = s * s;
+    i = hx - 0x6147a;
+    double w = z * z;
+    int32_t j = 0x6b851 - hx;
+    double t1 = w * (Lg2 + w * (Lg4 + w * Lg6));
+    double t2 = z * (Lg1 + w * (Lg3 + w * (Lg5 + w * Lg7)));
+    i |= j;
+    R = t2 + t1;
+    if (i > 0) {
+        hfsq = 0.5 * f * f;
+        if (k == 0) return f - (hfsq - s * (hfsq + R));
+        return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+    }
+    if (k == 0) return f - s * (f - R);
+    return dk * ln2_hi - ((s * (f - R) - dk * ln2_lo) - f);
+}
+
+static double fdlibm_k_sin(double x, double y, int iy) {
+    double z = x * x;
+    double v = z * x;
+    double r = S2 + z * (S3 + z * (S4 + z * (S5 + z * S6)));
+    if (iy == 0) return x + v * (S1 + z * r);
+    return x - ((z * (0.5 * y - v * r) - y) - v * S1);
+}
+
+static double fdlibm_k_cos(double x, double y) {
+    uint32_t ix = (uint32_t)(bits_of(x) >> 32) & 0x7fffffffu;
+    double z = x * x;
+    double r = z * (C1 + z * (C2 + z * (C3 + z * (C4 + z * (C5 + z * C6)))));
+    if (ix < 0x3fd33333u) return 1.0 - (0.5 * z - (z * r - x * y)); /* |x| < 0.3 */
+    double qx = ix > 0x3fe90000u ? 0.28125 : of_bits((uint64_t)(ix - 0x00200000u) << 32); /* x/4 */
+    double iz = 0.5 * z - qx;
+    double a = 1.0 - qx;
+    return a - (iz - (z * r - x * y));
+}
+
+double orc_fdlibm_sin(double x) {
+    uint32_t ix = (uint32_t)(bits_of(x) >> 32) & 0x7fffffffu;
+    double y0, y1;
+    if (ix <= 0x3fe921fbu) {
+        if (ix < 0x3e400000u && (int)x == 0) return x;
+        return fdlibm_k_sin(x, 0.0, 0);
+    }
+    if (ix >= 0x7ff00000u) return x - x;
+    int32_t n = orc_rem_pio2(x, &y0, &y1);
+    switch (n & 3) {
+    case 0: return fdlibm_k_sin(y0, y1, 1);
+    case 1: return fdlibm_k_cos(y0, y1);
+    case 2: return -fdlibm_k_sin(y0, y1, 1);
+    default: return -fdlibm_k_cos(y0, y1);
+    }
+}
+
+double orc_fdlibm_cos(double x) {
+    uint32_t ix = (uint32_t)(bits_of(x) >> 32) & 0x7fffffffu;
+    double y0, y1;
+    if (ix <= 0x3fe921fbu) {
+        if (ix < 0x3e400000u && (int)x == 0) return 1.0;
+        return fdlibm_k_cos(x, 0.0);
+    }
+    if (ix >= 0x7ff00000u) return x - x;
+    int32_t n = orc_rem_pio2(x, &y0, &y1);
+    switch (n & 3) {
+    case 0: return fdlibm_k_cos(y0, y1);
+    case 1: return -fdlibm_k_sin(y0, y1, 1);
+    case 2: return -fdlibm_k_cos(y0, y1);
+    default: return fdlibm_k_sin(y0, y1, 1);
+    }
+}
+
+/* Vector evaluation for the pins: op 0 ln, 1 exp, 2 sin, 3 cos, 4 the fdlibm
+ * log twin, 5 the fdlibm sin twin, 6 the fdlibm cos twin. */
+int orc_maths_vec(int op, const double *in, double *out, size_t n) {
+    for (size_t i = 0; i < n; i++) {
+        double x = in[i];
+        switch (op) {
+        case 0: out[i] = orc_ln(x); break;
+        case 1: out[i] = orc_exp(x); break;
+        case 2: out[i] = orc_sin(x); break;
+        case 3: out[i] = orc_cos(x); break;
+        case 4: out[i] = orc_fdlibm_log(x); break;
+        case 5: out[i] = orc_fdlibm_sin(x); break;
+        case 6: out[i] = orc_fdlibm_cos(x); break;
+        default: return -1;
+        }
+    }
+    return 0;
+}
+
 /* ======================================================================== */
 /* RNG: WyHash PrimeableRng and RngSampler                                    */
 /* ======================================================================== */

@@ -12,19 +12,28 @@
  *
  * What pins it (tests/test_oracle_pins.py, tests/test_golden.py): published
  * seahash, rand_core seed_from_u64 and FNV-1a vectors; wymum against exact
- * 128-bit products; ln/exp/sin/cos within 1 ulp of high-precision values and
- * at the special values; alias tables against their weights; the Poisson
- * sampler's contract and rate; origin order and partitions; and the
+ * 128-bit products; closed forms of the WyHash composition (operands zeroed
+ * through the inverse of seahash_diffuse give prime_with = 16 ^ P5, a
+ * location-independent state, seed/location translation, and sample_u64 of
+ * a zeroed first operand); exp bit for bit against V8's fdlibm port (node),
+ * and fdlibm-order twins of log/sin/cos built from this file's constants and
+ * pi/2 reduction bit for bit against V8 on 2e5 inputs per range
+ * (tests/golden/v8_maths.json), with log/sin/cos within 1 ulp of their twins
+ * and of high-precision values; alias tables against their weights; the
+ * Poisson sampler's contract and rate; origin order and partitions; and the
  * whole-path invariants (dedup keeps the species and the unique events,
  * sorted output, pause/resume, slicing independence, partition union).
  *
- * Parity unpinned (no reference-held vector or closed form fixes the bits):
- *  - the exact output bits of crate libm 0.2.8 log/exp/sin/cos as compiled
- *    by rustc: orc_ln/orc_exp/orc_sin/orc_cos restate the same msun
- *    algorithms and are pinned to within 1 ulp, not bit for bit;
- *  - the composition inside WyHash::prime_with and sample_u64 (which
- *    constant meets which word): its parts (seahash_diffuse, wymum) are
- *    pinned, the composition follows wyhash.rs:53-80 only;
+ * Parity unpinned (no reference-held vector or independent implementation
+ * fixes these bits):
+ *  - the musl evaluation order of the final assembly of log (and its
+ *    reduction threshold) and of the sin/cos kernel polynomials, which crate
+ *    libm 0.2.8 follows (reproducible.rs:18-46): V8 evaluates the same
+ *    constants in fdlibm's order, so these orders are pinned to within 1 ulp
+ *    of the V8-pinned twins, not bit for bit;
+ *  - the absolute values of WyHash::prime_with / sample_u64 (wyhash.rs:53-80):
+ *    the closed forms fix which constant meets which word and where the
+ *    rotations sit, but no vector of the crate's own outputs exists here;
  *  - IntrinsicsMathsCore (glibc log/exp), the CPU Independent algorithm's
  *    default, is not restated at all.
  */
@@ -127,6 +136,12 @@ double orc_ln(double x);
 double orc_exp(double x);
 double orc_sin(double x);
 double orc_cos(double x);
+/* test-only fdlibm-order twins sharing the constants above, and a vector
+ * evaluator of ln/exp/sin/cos and the twins (op 0..6) */
+double orc_fdlibm_log(double x);
+double orc_fdlibm_sin(double x);
+double orc_fdlibm_cos(double x);
+int orc_maths_vec(int op, const double *in, double *out, size_t n);
 double orc_sqrt(double x);
 double orc_floor(double x);
 double orc_round(double x);

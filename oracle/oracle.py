@@ -67,6 +67,11 @@ def lib():
         for name in ("orc_ln", "orc_exp", "orc_sin", "orc_cos", "orc_sqrt", "orc_floor", "orc_round"):
             getattr(L, name).restype = d
             getattr(L, name).argtypes = [d]
+        for name in ("orc_fdlibm_log", "orc_fdlibm_sin", "orc_fdlibm_cos"):
+            getattr(L, name).restype = d
+            getattr(L, name).argtypes = [d]
+        L.orc_maths_vec.restype = ctypes.c_int
+        L.orc_maths_vec.argtypes = [ctypes.c_int, vp, vp, ctypes.c_size_t]
         L.orc_seed_from_u64.restype = u64
         L.orc_seed_from_u64.argtypes = [u64]
         L.orc_seahash_diffuse.restype = u64
@@ -263,3 +268,14 @@ def simulate_partitioned(scenario, params, percentage, threads, partitions=None,
                                            cy, r, threads, partitions or threads, ctypes.byref(secs),
                                            ctypes.byref(nev))
     return dict(steps=int(steps), seconds=secs.value, events=int(nev.value))
+
+
+def maths_vec(op, x):
+    """orc_maths_vec: op 0 ln, 1 exp, 2 sin, 3 cos, 4/5/6 the fdlibm-order
+    twins of ln / sin / cos; x is a float64 array."""
+    import numpy as np
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    if lib().orc_maths_vec(op, x.ctypes.data, y.ctypes.data, len(x)) != 0:
+        raise ValueError(f"unknown maths op {op}")
+    return y
