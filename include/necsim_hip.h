@@ -282,10 +282,28 @@ int nh_tables_footprint(const nh_tables *tables, uint64_t *bytes);
 
 /* ---- one simulation (Algorithm::initialise_and_simulate / resume_and_simulate) ---- */
 int nh_job_create(nh_ctx *ctx, const nh_tables *tables, const nh_job_params *params, nh_job **out);
+
+/* Lineage classes of IndependentActiveLineageSampler::resume_with_store_and_lineages
+ * (necsim/impls/no-std/src/cogs/active_lineage_sampler/independent/mod.rs:98-117),
+ * i.e. the variants of ExceptionalLineage it produces
+ * (.../active_lineage_sampler/resuming/lineage.rs:7-14). */
+enum {
+    NH_LINEAGE_OK = 0,
+    NH_LINEAGE_OUT_OF_DEME = 1,     /* index >= the habitat at its location */
+    NH_LINEAGE_OUT_OF_HABITAT = 2,  /* location outside the extent or of habitat 0 */
+};
+/* Classify resumed lineages against the scenario (kinds[n] receives
+ * NH_LINEAGE_*; *n_exceptional, if not NULL, the number not OK).  The resume
+ * path of the Independent algorithm (rustcoalescence/algorithms/independent/
+ * src/initialiser/resume.rs:71-85) fails with ResumeError::Sample(every
+ * exceptional lineage) when any is not OK; the glue builds that list from
+ * kinds before it adds the lineages to a job. */
+int nh_classify_lineages(const nh_tables *tables, const nh_lineage *lineages, size_t n, int32_t *kinds,
+                         size_t *n_exceptional);
 /* Append lineages in origin-sampler order (the order defines the water-level
  * queue, parallelisation/independent/monolithic/mod.rs:94-103).  Lineages
- * outside the habitat or their deme fail with NH_E_OUT_OF_DEME
- * (active_lineage_sampler/independent/mod.rs:98-117). */
+ * that nh_classify_lineages does not class OK fail with NH_E_OUT_OF_DEME
+ * (active_lineage_sampler/independent/mod.rs:98-117) and nothing is added. */
 int nh_job_add_lineages(nh_job *job, const nh_lineage *lineages, size_t n);
 /* Allocate the job's device state and upload its lineages (done implicitly by
  * the first nh_job_step_round; callable earlier so that the upload stays out
@@ -338,14 +356,23 @@ int nh_autotune(nh_ctx *ctx, const nh_tables *tables, const nh_job_params *base,
                 double *best_ms);
 
 /* ---- serving: continuous batching of simulations on resident tables ------- */
+/* No counterpart in the reference (which runs one simulation per process).
+ * Request memory comes from one device allocation of hbm_budget_bytes made by
+ * nh_server_create: admission and retirement never allocate or free device
+ * memory.  A request's block is nh_server_request_footprint() bytes. */
 typedef struct {
     uint64_t max_lineages;      /* lane-state capacity shared by requests in flight */
-    uint64_t hbm_budget_bytes;  /* admission limit for per-request device memory */
-    uint64_t tick_steps;        /* max steps per lineage per kernel tick (latency knob) */
+    uint64_t hbm_budget_bytes;  /* device memory of the request pool (allocated at creation) */
+    uint64_t tick_steps;        /* max steps per lineage per kernel tick (latency knob); a lane
+                                   stops only at a step-slice boundary, so a tick may run up to
+                                   step_slice - 1 steps more */
     uint32_t block_size;        /* 0 = default (256) */
     uint32_t blocks_per_cu;     /* 0 = default (1): a tick's latency is set by the
                                    slowest requests' serial lineage chains, which
                                    run faster with few resident waves per CU */
+    uint64_t max_bypass_ticks;  /* admission runs in arrival order, but a request that does not
+                                   fit yet may be passed by later ones for this many ticks; then
+                                   nothing is admitted past it until it fits (0 = default 16) */
 } nh_server_params;
 
 typedef struct {
@@ -378,11 +405,25 @@ typedef struct {
 
 int nh_server_create(nh_ctx *ctx, const nh_tables *tables, const nh_server_params *params,
                      nh_server **out);
-/* Queue a simulation; returns immediately with its id. */
+/* Device bytes a request of n_lineages lineages and cache_buckets dedup-cache
+ * buckets takes from the pool: 130 B per lineage (two lineage-state copies
+ * and one event slot), 32 B per bucket and 4608 B of alignment, rounded up to
+ * a multiple of 256 B.  Requests whose footprints sum to at most the pool
+ * size fit in it together. */
+int nh_server_request_footprint(uint64_t n_lineages, uint64_t cache_buckets, uint64_t *bytes);
+/* Queue a simulation; returns immediately with its id.  The request is
+ * checked like nh_job_create + nh_job_add_lineages (invalid arguments:
+ * NH_E_INVALID; a lineage that nh_classify_lineages does not class OK:
+ * NH_E_OUT_OF_DEME; a footprint above the pool or more lineages than
+ * max_lineages: NH_E_CAPACITY), so no kernel ever runs on a bad request. */
 int nh_server_submit(nh_server *srv, const nh_request *req, uint64_t *request_id);
 /* Advance the server by one kernel tick: admit queued requests that fit,
  * run the shared persistent kernel over every admitted request, retire
- * finished ones.  *n_active / *n_queued report the load afterwards. */
+ * finished ones.  *n_active / *n_queued report the load afterwards.  A
+ * request whose own step fails (upload, compaction, event sort) is finished
+ * with that error in its result's status and the others go on; the call then
+ * returns the first such error.  If the shared kernel tick itself fails,
+ * every active request is finished with the error. */
 int nh_server_tick(nh_server *srv, uint64_t *n_active, uint64_t *n_queued);
 /* Pop one finished request (status NH_E_STATE if none is ready). */
 int nh_server_poll(nh_server *srv, nh_request_result *result);

@@ -3,6 +3,7 @@
 
 #include <vector>
 
+#include "../../include/necsim_hip.h"
 #include "nh_device.h"
 
 struct nh_ctx;
@@ -233,3 +234,7 @@ struct nh_tables {
     uint64_t bytes = 0;
     nh::DevScenario dev{};
 };
+
+// NH_LINEAGE_* class of a lineage against the tables' scenario (defined in
+// nh_runtime.hip)
+int nh_lineage_class(const nh_tables *t, const nh_lineage &l);

@@ -591,8 +591,15 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
                 coop_search_window<KIND>(L, srch, P, s_ratio, s_coop);
                 if (!has) continue;
                 if (!wave_ready(L.have_next, 5) || !L.have_next) continue;
-                bool leave = L.next_t >= level || (A.tick_steps && tick >= A.tick_steps);
-                if (leave || slice_steps >= P.step_slice) {
+                // The tick limit ends a lane's work only at a step-slice
+                // boundary (or before the slice's first step), where the
+                // slice's min sample has just been inserted: the next tick
+                // starts a fresh slice, so slices (and the dedup inserts) fall
+                // where they do in an uninterrupted round.
+                const bool slice_end = slice_steps >= P.step_slice;
+                bool leave = L.next_t >= level ||
+                             (A.tick_steps && tick >= A.tick_steps && (slice_end || slice_steps == 0));
+                if (leave || slice_end) {
                     bool dup = R.cache && L.ms_some &&
                                cache_insert_concurrent(R.cache, R.cache_cap, lane_sample<KIND>(L, A.S));
                     L.ms_some = false;

@@ -274,14 +274,30 @@ extern "C" int nh_tables_footprint(const nh_tables *t, uint64_t *bytes) {
     return NH_OK;
 }
 
-// check a lineage against IndependentActiveLineageSampler's resume filters
-// (active_lineage_sampler/independent/mod.rs:98-117)
-static bool lineage_valid(const nh_tables *t, const nh_lineage &l) {
-    if (t->kind == K_ALMOST_INFINITE) return l.index == 0;
+// IndependentActiveLineageSampler's resume filters
+// (active_lineage_sampler/independent/mod.rs:98-117): a lineage whose
+// location is not habitable (outside the extent, or a cell of habitat 0,
+// necsim/core/src/cogs/habitat.rs:25-27) is OutOfHabitat, one whose index is
+// not below the location's habitat is OutOfDeme.
+int nh_lineage_class(const nh_tables *t, const nh_lineage &l) {
+    if (t->kind == K_ALMOST_INFINITE) return l.index == 0 ? NH_LINEAGE_OK : NH_LINEAGE_OUT_OF_DEME;
     uint32_t cx = l.x - t->ox, cy = l.y - t->oy;
-    if (cx >= t->width || cy >= t->height) return false;
+    if (cx >= t->width || cy >= t->height) return NH_LINEAGE_OUT_OF_HABITAT;
     uint32_t deme = t->kind == K_NON_SPATIAL ? t->deme : t->habitat[(uint64_t)cy * t->width + cx];
-    return l.index < deme;
+    if (deme == 0) return NH_LINEAGE_OUT_OF_HABITAT;
+    return l.index < deme ? NH_LINEAGE_OK : NH_LINEAGE_OUT_OF_DEME;
+}
+
+extern "C" int nh_classify_lineages(const nh_tables *t, const nh_lineage *l, size_t n, int32_t *kinds,
+                                    size_t *n_exceptional) {
+    if (!t || (n && (!l || !kinds))) return set_error(NH_E_INVALID, "nh_classify_lineages: null argument");
+    size_t bad = 0;
+    for (size_t k = 0; k < n; k++) {
+        kinds[k] = nh_lineage_class(t, l[k]);
+        bad += kinds[k] != NH_LINEAGE_OK;
+    }
+    if (n_exceptional) *n_exceptional = bad;
+    return NH_OK;
 }
 
 // ===========================================================================
@@ -427,7 +443,7 @@ extern "C" int nh_job_add_lineages(nh_job *j, const nh_lineage *l, size_t n) {
     if (!j || (!l && n)) return set_error(NH_E_INVALID, "nh_job_add_lineages: null argument");
     if (j->started) return set_error(NH_E_STATE, "lineages must be added before the first round");
     for (size_t k = 0; k < n; k++) {
-        if (!lineage_valid(j->tables, l[k]))
+        if (nh_lineage_class(j->tables, l[k]) != NH_LINEAGE_OK)
             return set_error(NH_E_OUT_OF_DEME, "lineage %zu at (%u, %u, %u) is outside the habitat or its deme", k,
                              l[k].x, l[k].y, l[k].index);
         if (!(l[k].last_event_time >= 0.0)) return set_error(NH_E_INVALID, "negative last_event_time");

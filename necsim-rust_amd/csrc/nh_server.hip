@@ -34,22 +34,72 @@ double mono_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-struct Soa {
-    LineageSoA s{};
-    int alloc(uint64_t n) {
-        NH_TRY(dalloc(&s.ref, n)); NH_TRY(dalloc(&s.t_last, n)); NH_TRY(dalloc(&s.x, n));
-        NH_TRY(dalloc(&s.y, n)); NH_TRY(dalloc(&s.idx, n)); NH_TRY(dalloc(&s.next_t, n));
-        NH_TRY(dalloc(&s.next_st, n)); NH_TRY(dalloc(&s.flags, n));
+// ---------------------------------------------------------------------------
+// Request memory comes from ONE device allocation of hbm_budget_bytes made
+// when the server is created: each admitted request takes one contiguous
+// block (first fit over a free list ordered by offset; freed blocks merge
+// with their neighbours), so admission and retirement never call hipMalloc
+// or hipFree (hipFree synchronises the device, which would stall the tick
+// loop).
+constexpr uint64_t ARENA_ALIGN = 256;
+constexpr int REQ_ARRAYS = 2 * 8 + 2; // two SoA copies, the event slots, the dedup cache
+constexpr uint64_t BYTES_PER_LINEAGE = 2 * 45 + 40; // two SoA copies + one event slot
+constexpr uint64_t BYTES_PER_BUCKET = 32;
+constexpr uint64_t REQ_SLACK = REQ_ARRAYS * ARENA_ALIGN; // alignment of the sub-arrays
+
+uint64_t align_up(uint64_t v) { return (v + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN; }
+
+struct Arena {
+    char *base = nullptr;
+    uint64_t size = 0;
+    std::map<uint64_t, uint64_t> free_; // offset -> length
+    int create(uint64_t bytes) {
+        size = align_up(bytes);
+        if (size == 0) return NH_OK;
+        NH_TRY(dalloc(&base, size));
+        free_[0] = size;
         return NH_OK;
     }
-    void release() {
-        dfree(s.ref); dfree(s.t_last); dfree(s.x); dfree(s.y); dfree(s.idx);
-        dfree(s.next_t); dfree(s.next_st); dfree(s.flags);
+    void destroy() { dfree(base); }
+    bool alloc(uint64_t len, uint64_t *off) {
+        len = align_up(len ? len : 1);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < len) continue;
+            *off = it->first;
+            uint64_t rest = it->second - len, at = it->first + len;
+            free_.erase(it);
+            if (rest) free_[at] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t len) {
+        len = align_up(len ? len : 1);
+        auto nx = free_.lower_bound(off);
+        if (nx != free_.end() && off + len == nx->first) { // merge with the next block
+            len += nx->second;
+            nx = free_.erase(nx);
+        }
+        if (nx != free_.begin()) {
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == off) { // ... and the previous one
+                pv->second += len;
+                return;
+            }
+        }
+        free_[off] = len;
     }
 };
 
-constexpr uint64_t BYTES_PER_LINEAGE = 2 * 45 + 40; // two SoA copies + event slot
-constexpr uint64_t BYTES_PER_BUCKET = 32;
+// Carves a request's arrays out of its block (every sub-array 256-aligned).
+struct Carver {
+    char *p;
+    template <class T> T *take(uint64_t n) {
+        T *r = reinterpret_cast<T *>(p);
+        p += align_up(n * sizeof(T));
+        return r;
+    }
+};
 
 struct Req {
     uint64_t id = 0;
@@ -59,8 +109,11 @@ struct Req {
     uint64_t event_slice = 1;
     uint64_t cache_cap = 0;
     uint64_t footprint = 0;
-    // device state
-    Soa soa[2];
+    uint64_t blocked_ticks = 0; // ticks it waited for memory or lanes
+    // device state (inside the arena block [arena_off, arena_off + footprint))
+    bool admitted = false;
+    uint64_t arena_off = 0;
+    LineageSoA soa[2]{};
     int cur = 0;
     uint64_t n_in = 0;  // slots of the current buffer
     uint64_t alive = 0; // live lineages
@@ -74,14 +127,31 @@ struct Req {
     // results
     std::vector<nh_speciation_event> events;
     nh_request_result result{};
-    bool done = false;
-    void release() {
-        soa[0].release();
-        soa[1].release();
-        dfree(d_events);
-        dfree(d_cache);
-    }
 };
+
+// the device footprint of a request: its arena block, a multiple of the
+// block alignment (so that requests whose footprints sum to the pool size
+// fit it together)
+uint64_t request_footprint(uint64_t n, uint64_t cache_cap) {
+    return align_up(n * BYTES_PER_LINEAGE + cache_cap * BYTES_PER_BUCKET + REQ_SLACK);
+}
+
+void carve(Req *r, char *block) {
+    Carver c{block};
+    uint64_t n = r->n_initial;
+    for (LineageSoA &s : r->soa) {
+        s.ref = c.take<uint64_t>(n);
+        s.t_last = c.take<double>(n);
+        s.next_t = c.take<double>(n);
+        s.next_st = c.take<uint64_t>(n);
+        s.x = c.take<uint32_t>(n);
+        s.y = c.take<uint32_t>(n);
+        s.idx = c.take<uint32_t>(n);
+        s.flags = c.take<uint8_t>(n);
+    }
+    r->d_events = c.take<DevEvent>(n);
+    r->d_cache = r->cache_cap ? c.take<uint64_t>(4 * r->cache_cap) : nullptr;
+}
 
 } // namespace
 
@@ -91,6 +161,7 @@ struct nh_server {
     DevScenario S{};
     int kind = 0;
     nh_server_params prm{};
+    uint64_t max_bypass_ticks = 16;
     DevParams P{};
     double *d_ratio = nullptr;
     double lambda_dt = 0.0, delta_t = 0.0, turnover = 0.0;
@@ -103,6 +174,7 @@ struct nh_server {
     std::deque<Req *> finished; // not yet polled
     std::map<uint64_t, Req *> polled;
     uint64_t used_bytes = 0, used_lineages = 0;
+    Arena arena;
     // shared device buffers
     ServeReq *d_reqs = nullptr;
     ReqCounters *d_ctr = nullptr; // SERVE_MAX_REQS
@@ -114,9 +186,10 @@ struct nh_server {
     std::vector<ReqCounters> h_ctr;
     ~nh_server() {
         for (Req *r : queued) delete r;
-        for (Req *r : active) { r->release(); delete r; }
-        for (Req *r : finished) { r->release(); delete r; }
-        for (auto &kv : polled) { kv.second->release(); delete kv.second; }
+        for (Req *r : active) delete r;
+        for (Req *r : finished) delete r;
+        for (auto &kv : polled) delete kv.second;
+        arena.destroy();
         dfree(d_ratio); dfree(d_reqs); dfree(d_ctr); dfree(d_head); dfree(d_blk); dfree(d_sorted);
         dfree(sb.keys[0]); dfree(sb.keys[1]); dfree(sb.vals[0]); dfree(sb.vals[1]);
         dfree(sb.hist); dfree(sb.minmax);
@@ -131,12 +204,14 @@ extern "C" int nh_server_create(nh_ctx *ctx, const nh_tables *tables, const nh_s
         return set_error(NH_E_INVALID, "max_lineages must lie in 1..2^32-1");
     if (prm->block_size && (prm->block_size % 64 || prm->block_size > 256))
         return set_error(NH_E_INVALID, "block size must be 64, 128, 192 or 256");
+    if (prm->hbm_budget_bytes == 0) return set_error(NH_E_INVALID, "hbm_budget_bytes must be positive");
     NH_HIP(hipSetDevice(ctx->device));
     nh_server *s = new nh_server();
     s->ctx = ctx;
     s->tables = tables;
     s->S = *nh_tables_device_view(tables, &s->kind);
     s->prm = *prm;
+    if (prm->max_bypass_ticks) s->max_bypass_ticks = prm->max_bypass_ticks;
     s->threads = prm->block_size ? (int)prm->block_size : 256;
     // one block per CU unless asked otherwise: the serving stream is latency
     // bound (the last lineages of each request run serially, searched
@@ -148,6 +223,7 @@ extern "C" int nh_server_create(nh_ctx *ctx, const nh_tables *tables, const nh_s
     uint64_t cap = prm->max_lineages;
     auto fail = [&](int rc) { delete s; return rc; };
     int rc;
+    if ((rc = s->arena.create(prm->hbm_budget_bytes))) return fail(rc);
     if ((rc = dalloc(&s->d_ratio, POISSON_TABLE_N))) return fail(rc);
     if ((rc = dalloc(&s->d_reqs, SERVE_MAX_REQS))) return fail(rc);
     if ((rc = dalloc(&s->d_ctr, SERVE_MAX_REQS))) return fail(rc);
@@ -165,18 +241,57 @@ extern "C" int nh_server_create(nh_ctx *ctx, const nh_tables *tables, const nh_s
     return NH_OK;
 }
 
+extern "C" int nh_server_request_footprint(uint64_t n_lineages, uint64_t cache_buckets, uint64_t *bytes) {
+    if (!bytes) return set_error(NH_E_INVALID, "nh_server_request_footprint: null argument");
+    *bytes = request_footprint(n_lineages, cache_buckets);
+    return NH_OK;
+}
+
 extern "C" int nh_server_submit(nh_server *s, const nh_request *rq, uint64_t *id) {
     if (!s || !rq || !id || (!rq->lineages && rq->n_lineages)) return set_error(NH_E_INVALID, "nh_server_submit: null argument");
+    // the argument checks of nh_job_create
     if (!(rq->speciation_probability > 0.0 && rq->speciation_probability <= 1.0))
         return set_error(NH_E_INVALID, "speciation probability must lie in (0, 1]");
-    if (!(rq->turnover_rate > 0.0) || !(rq->delta_t > 0.0) || rq->step_slice == 0)
+    if (!(rq->turnover_rate > 0.0) || std::isinf(rq->turnover_rate) || !(rq->delta_t > 0.0) ||
+        std::isinf(rq->delta_t) || rq->step_slice == 0)
         return set_error(NH_E_INVALID, "turnover rate, delta_t and step_slice must be positive");
+    if (rq->dedup_mode < NH_DEDUP_NONE || rq->dedup_mode > NH_DEDUP_ABSOLUTE) return set_error(NH_E_INVALID, "dedup mode");
+    if (rq->dedup_mode == NH_DEDUP_RELATIVE && !(rq->dedup_factor > 0.0)) return set_error(NH_E_INVALID, "dedup factor");
+    if (rq->event_slice_mode != NH_SLICE_RELATIVE && rq->event_slice_mode != NH_SLICE_ABSOLUTE)
+        return set_error(NH_E_INVALID, "event slice mode");
+    if (rq->event_slice_mode == NH_SLICE_RELATIVE && !(rq->event_slice_factor > 0.0))
+        return set_error(NH_E_INVALID, "event slice factor");
+    if (rq->event_slice_mode == NH_SLICE_ABSOLUTE && rq->event_slice_capacity == 0)
+        return set_error(NH_E_INVALID, "event slice capacity");
+    if (rq->has_pause_before && std::isnan(rq->pause_before)) return set_error(NH_E_INVALID, "pause_before is NaN");
     // lambda, delta_t and step_slice are shared by every request of a server
+    if (s->shared_set &&
+        (rq->turnover_rate != s->turnover || rq->delta_t != s->delta_t || rq->step_slice != s->step_slice))
+        return set_error(NH_E_INVALID, "turnover rate, delta_t and step_slice must match the server's first request");
+    if (rq->n_lineages > s->prm.max_lineages)
+        return set_error(NH_E_CAPACITY, "request of %zu lineages exceeds the server capacity", rq->n_lineages);
+    // the job path's lineage filter (nh_job_add_lineages): no kernel ever
+    // sees a lineage outside the scenario's habitat or its deme
+    for (size_t k = 0; k < rq->n_lineages; k++) {
+        const nh_lineage &l = rq->lineages[k];
+        if (nh_lineage_class(s->tables, l) != NH_LINEAGE_OK)
+            return set_error(NH_E_OUT_OF_DEME, "lineage %zu at (%u, %u, %u) is outside the habitat or its deme", k,
+                             l.x, l.y, l.index);
+        if (!(l.last_event_time >= 0.0) || std::isinf(l.last_event_time))
+            return set_error(NH_E_INVALID, "lineage %zu: last_event_time must be finite and non-negative", k);
+    }
+    uint64_t n = rq->n_lineages, event_slice, cache_cap = 0;
+    if (rq->event_slice_mode == NH_SLICE_ABSOLUTE) event_slice = std::max<uint64_t>(rq->event_slice_capacity, 1);
+    else event_slice = std::max<uint64_t>(as_u64((double)n * rq->event_slice_factor), 1);
+    if (rq->dedup_mode == NH_DEDUP_RELATIVE) cache_cap = as_u64((double)n * rq->dedup_factor);
+    else if (rq->dedup_mode == NH_DEDUP_ABSOLUTE) cache_cap = rq->dedup_capacity;
+    if (cache_cap > (UINT64_MAX - REQ_SLACK - ARENA_ALIGN - n * BYTES_PER_LINEAGE) / BYTES_PER_BUCKET ||
+        request_footprint(n, cache_cap) > s->arena.size)
+        return set_error(NH_E_CAPACITY, "request footprint exceeds the HBM budget");
     if (!s->shared_set) {
         s->turnover = rq->turnover_rate;
         s->delta_t = rq->delta_t;
         s->step_slice = rq->step_slice;
-        s->shared_set = true;
         DevParams &P = s->P;
         P.lambda_dt = rq->turnover_rate * rq->delta_t;
         P.p0 = nh::exp(-P.lambda_dt);
@@ -191,27 +306,17 @@ extern "C" int nh_server_submit(nh_server *s, const nh_request *rq, uint64_t *id
         P.prod_zero_at = poisson_table(P.lambda_dt, P.p0, ratio.data());
         NH_HIP(hipMemcpy(s->d_ratio, ratio.data(), POISSON_TABLE_N * sizeof(double), hipMemcpyHostToDevice));
         P.ratio = s->d_ratio;
-    } else if (rq->turnover_rate != s->turnover || rq->delta_t != s->delta_t || rq->step_slice != s->step_slice) {
-        return set_error(NH_E_INVALID, "turnover rate, delta_t and step_slice must match the server's first request");
+        s->shared_set = true;
     }
-    if (rq->n_lineages > s->prm.max_lineages)
-        return set_error(NH_E_CAPACITY, "request of %zu lineages exceeds the server capacity", rq->n_lineages);
     Req *r = new Req();
     r->id = s->next_id++;
     r->p = *rq;
     r->p.lineages = nullptr;
     r->lineages.assign(rq->lineages, rq->lineages + rq->n_lineages);
-    r->n_initial = rq->n_lineages;
-    uint64_t n = r->n_initial;
-    if (rq->event_slice_mode == NH_SLICE_ABSOLUTE) r->event_slice = std::max<uint64_t>(rq->event_slice_capacity, 1);
-    else r->event_slice = std::max<uint64_t>(as_u64((double)n * rq->event_slice_factor), 1);
-    if (rq->dedup_mode == NH_DEDUP_RELATIVE) r->cache_cap = as_u64((double)n * rq->dedup_factor);
-    else if (rq->dedup_mode == NH_DEDUP_ABSOLUTE) r->cache_cap = rq->dedup_capacity;
-    r->footprint = n * BYTES_PER_LINEAGE + r->cache_cap * BYTES_PER_BUCKET;
-    if (r->footprint > s->prm.hbm_budget_bytes) {
-        delete r;
-        return set_error(NH_E_CAPACITY, "request footprint exceeds the HBM budget");
-    }
+    r->n_initial = n;
+    r->event_slice = event_slice;
+    r->cache_cap = cache_cap;
+    r->footprint = request_footprint(n, cache_cap);
     if (n) {
         double mx = r->lineages[0].last_event_time, mn = mx;
         for (auto &l : r->lineages) {
@@ -228,16 +333,42 @@ extern "C" int nh_server_submit(nh_server *s, const nh_request *rq, uint64_t *id
     return NH_OK;
 }
 
-static int admit(nh_server *s, Req *r) {
+// gives a request's arena block back
+static void release_block(nh_server *s, Req *r) {
+    if (!r->admitted) return;
+    s->arena.release(r->arena_off, r->footprint);
+    s->used_bytes -= r->footprint;
+    s->used_lineages -= r->n_initial;
+    r->admitted = false;
+    r->soa[0] = r->soa[1] = LineageSoA{};
+    r->d_events = nullptr;
+    r->d_cache = nullptr;
+}
+
+// ends a request (status NH_OK, or the error that ended it) and queues its
+// result for nh_server_poll
+static void finish(nh_server *s, Req *r, int status) {
+    r->result.status = status;
+    r->result.paused = r->alive > 0;
+    r->result.time = r->max_time;
+    r->result.steps = r->steps;
+    r->result.n_events = r->events.size();
+    r->result.done_s = mono_s();
+    release_block(s, r);
+    s->finished.push_back(r);
+}
+
+// uploads an admitted request's lineages into its arena block
+static int admit(nh_server *s, Req *r, uint64_t off) {
     hipStream_t st = s->ctx->stream;
     uint64_t n = r->n_initial;
-    NH_TRY(r->soa[0].alloc(n));
-    NH_TRY(r->soa[1].alloc(n));
-    NH_TRY(dalloc(&r->d_events, n));
-    if (r->cache_cap) {
-        NH_TRY(dalloc(&r->d_cache, 4 * r->cache_cap));
-        launch_fill_u64(r->d_cache, 0, 4 * r->cache_cap, st);
-    }
+    r->admitted = true;
+    r->arena_off = off;
+    s->used_bytes += r->footprint;
+    s->used_lineages += n;
+    carve(r, s->arena.base + off);
+    r->result.admit_s = mono_s();
+    if (r->cache_cap) launch_fill_u64(r->d_cache, 0, 4 * r->cache_cap, st);
     if (n) {
         std::vector<uint64_t> ref(n), zero(n, 0);
         std::vector<double> t(n);
@@ -247,7 +378,7 @@ static int admit(nh_server *s, Req *r) {
             const nh_lineage &l = r->lineages[k];
             ref[k] = l.global_reference; t[k] = l.last_event_time; x[k] = l.x; y[k] = l.y; idx[k] = l.index;
         }
-        LineageSoA &d = r->soa[0].s;
+        LineageSoA &d = r->soa[0];
         NH_HIP(hipMemcpyAsync(d.ref, ref.data(), n * 8, hipMemcpyHostToDevice, st));
         NH_HIP(hipMemcpyAsync(d.t_last, t.data(), n * 8, hipMemcpyHostToDevice, st));
         NH_HIP(hipMemcpyAsync(d.next_t, t.data(), n * 8, hipMemcpyHostToDevice, st));
@@ -257,15 +388,14 @@ static int admit(nh_server *s, Req *r) {
         NH_HIP(hipMemcpyAsync(d.idx, idx.data(), n * 4, hipMemcpyHostToDevice, st));
         NH_HIP(hipMemcpyAsync(d.flags, fl.data(), n, hipMemcpyHostToDevice, st));
         NH_HIP(hipStreamSynchronize(st));
+    } else {
+        NH_HIP(hipGetLastError());
     }
     r->lineages.clear();
     r->lineages.shrink_to_fit();
     r->n_in = n;
     r->alive = n;
     r->cur = 0;
-    r->result.admit_s = mono_s();
-    s->used_bytes += r->footprint;
-    s->used_lineages += n;
     return NH_OK;
 }
 
@@ -282,18 +412,61 @@ static void open_round(nh_server *s, Req *r) {
     r->round_event_start = r->n_events;
 }
 
-static int retire(nh_server *s, Req *r) {
-    r->done = true;
-    r->result.status = NH_OK;
-    r->result.paused = r->alive > 0;
-    r->result.time = r->max_time;
-    r->result.steps = r->steps;
-    r->result.n_events = r->events.size();
-    r->result.done_s = mono_s();
-    s->used_bytes -= r->footprint;
-    s->used_lineages -= r->n_initial;
-    r->release();
-    s->finished.push_back(r);
+// Admission: queued requests in arrival order, each as soon as its block and
+// its lanes fit.  A request that does not fit yet is passed by later ones
+// until it has waited max_bypass_ticks ticks; from then on nothing is
+// admitted past it, so the memory it needs drains.
+static void admit_queued(nh_server *s) {
+    size_t k = 0;
+    bool blocked_head = false;
+    while (k < s->queued.size() && s->active.size() < (size_t)SERVE_MAX_REQS) {
+        Req *r = s->queued[k];
+        uint64_t off = 0;
+        if (s->used_lineages + r->n_initial > s->prm.max_lineages || !s->arena.alloc(r->footprint, &off)) {
+            if (!blocked_head) {
+                blocked_head = true;
+                if (++r->blocked_ticks > s->max_bypass_ticks) break; // waited long enough: let it drain
+            }
+            k++;
+            continue;
+        }
+        s->queued.erase(s->queued.begin() + k);
+        int rc = admit(s, r, off);
+        if (rc != NH_OK) {
+            finish(s, r, rc);
+            continue;
+        }
+        if (request_finished(r)) finish(s, r, NH_OK);
+        else s->active.push_back(r);
+    }
+}
+
+// the end of a request's water-level round: drop dead slots and sort the
+// round's events like the live water-level reporter
+static int close_round(nh_server *s, Req *r) {
+    hipStream_t st = s->ctx->stream;
+    r->round_open = false;
+    if (r->n_in > r->alive) {
+        launch_soa_compact(r->soa[r->cur], r->soa[r->cur ^ 1], r->n_in, s->d_blk, st);
+        NH_HIP(hipGetLastError());
+        uint32_t live = 0;
+        NH_HIP(hipMemcpyAsync(&live, s->d_blk + soa_compact_blocks(r->n_in), 4, hipMemcpyDeviceToHost, st));
+        NH_HIP(hipStreamSynchronize(st));
+        if (live != r->alive)
+            return set_error(NH_E_STATE, "request %llu: %u live lineages, expected %llu", (unsigned long long)r->id,
+                             live, (unsigned long long)r->alive);
+        r->cur ^= 1;
+        r->n_in = live;
+    }
+    uint64_t ne = r->n_events - r->round_event_start;
+    if (ne) {
+        sort_events(r->d_events + r->round_event_start, ne, s->d_sorted, s->sb, st, false);
+        NH_HIP(hipGetLastError());
+        size_t old = r->events.size();
+        r->events.resize(old + ne);
+        NH_HIP(hipMemcpyAsync(r->events.data() + old, s->d_sorted, ne * sizeof(DevEvent), hipMemcpyDeviceToHost, st));
+        NH_HIP(hipStreamSynchronize(st));
+    }
     return NH_OK;
 }
 
@@ -301,20 +474,7 @@ extern "C" int nh_server_tick(nh_server *s, uint64_t *n_active, uint64_t *n_queu
     if (!s) return set_error(NH_E_INVALID, "null server");
     NH_HIP(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
-    // admission in arrival order while memory and lane state allow
-    while (!s->queued.empty() && s->active.size() < (size_t)SERVE_MAX_REQS) {
-        Req *r = s->queued.front();
-        if (s->used_bytes + r->footprint > s->prm.hbm_budget_bytes ||
-            s->used_lineages + r->n_initial > s->prm.max_lineages)
-            break;
-        s->queued.pop_front();
-        NH_TRY(admit(s, r));
-        if (request_finished(r)) {
-            NH_TRY(retire(s, r));
-            continue;
-        }
-        s->active.push_back(r);
-    }
+    admit_queued(s);
     // every active request without an open round starts its next one
     for (Req *r : s->active)
         if (!r->round_open) open_round(s, r);
@@ -327,7 +487,7 @@ extern "C" int nh_server_tick(nh_server *s, uint64_t *n_active, uint64_t *n_queu
         d.nu = r->p.speciation_probability;
         d.nu_m = draw_threshold_below(d.nu);
         d.level = r->level;
-        d.lin = r->soa[r->cur].s;
+        d.lin = r->soa[r->cur];
         d.events = r->d_events;
         d.cache = r->d_cache;
         d.cache_cap = r->cache_cap;
@@ -338,23 +498,37 @@ extern "C" int nh_server_tick(nh_server *s, uint64_t *n_active, uint64_t *n_queu
         s->h_ctr[k] = ReqCounters{0, 0, r->n_events, 0, 0, {0, 0, 0}};
         if (r->alive == 0) d.n_in = 0;
     }
+    int rc = NH_OK;
     if (nr > 0 && total > 0) {
-        NH_HIP(hipMemcpyAsync(s->d_reqs, s->h_reqs.data(), nr * sizeof(ServeReq), hipMemcpyHostToDevice, st));
-        NH_HIP(hipMemcpyAsync(s->d_ctr, s->h_ctr.data(), nr * sizeof(ReqCounters), hipMemcpyHostToDevice, st));
-        NH_HIP(hipMemsetAsync(s->d_head, 0, 8, st));
-        ServeArgs a{};
-        a.S = s->S;
-        a.P = s->P;
-        a.reqs = s->d_reqs;
-        a.n_reqs = nr;
-        a.total_work = total;
-        a.head = s->d_head;
-        a.tick_steps = s->prm.tick_steps;
-        a.chunk = claim_chunk(total, (unsigned long long)s->blocks * s->threads);
-        launch_serve(s->kind, a, s->blocks, s->threads, st);
-        NH_HIP(hipGetLastError());
-        NH_HIP(hipMemcpyAsync(s->h_ctr.data(), s->d_ctr, nr * sizeof(ReqCounters), hipMemcpyDeviceToHost, st));
-        NH_HIP(hipStreamSynchronize(st));
+        auto shared_tick = [&]() -> int {
+            NH_HIP(hipMemcpyAsync(s->d_reqs, s->h_reqs.data(), nr * sizeof(ServeReq), hipMemcpyHostToDevice, st));
+            NH_HIP(hipMemcpyAsync(s->d_ctr, s->h_ctr.data(), nr * sizeof(ReqCounters), hipMemcpyHostToDevice, st));
+            NH_HIP(hipMemsetAsync(s->d_head, 0, 8, st));
+            ServeArgs a{};
+            a.S = s->S;
+            a.P = s->P;
+            a.reqs = s->d_reqs;
+            a.n_reqs = nr;
+            a.total_work = total;
+            a.head = s->d_head;
+            a.tick_steps = s->prm.tick_steps;
+            a.chunk = claim_chunk(total, (unsigned long long)s->blocks * s->threads);
+            launch_serve(s->kind, a, s->blocks, s->threads, st);
+            NH_HIP(hipGetLastError());
+            NH_HIP(hipMemcpyAsync(s->h_ctr.data(), s->d_ctr, nr * sizeof(ReqCounters), hipMemcpyDeviceToHost, st));
+            NH_HIP(hipStreamSynchronize(st));
+            return NH_OK;
+        };
+        rc = shared_tick();
+        if (rc != NH_OK) {
+            // the shared tick failed: no request's state can be trusted, so
+            // every active request ends with the error
+            for (Req *r : s->active) finish(s, r, rc);
+            s->active.clear();
+            if (n_active) *n_active = 0;
+            if (n_queued) *n_queued = s->queued.size();
+            return rc;
+        }
     }
     std::vector<Req *> still;
     for (int k = 0; k < nr; k++) {
@@ -368,36 +542,19 @@ extern "C" int nh_server_tick(nh_server *s, uint64_t *n_active, uint64_t *n_queu
             still.push_back(r);
             continue;
         }
-        // the round is complete: drop dead slots, sort the round's events
-        // (live water-level reporter)
-        r->round_open = false;
-        if (r->n_in > r->alive) {
-            launch_soa_compact(r->soa[r->cur].s, r->soa[r->cur ^ 1].s, r->n_in, s->d_blk, st);
-            NH_HIP(hipGetLastError());
-            uint32_t live = 0;
-            NH_HIP(hipMemcpyAsync(&live, s->d_blk + soa_compact_blocks(r->n_in), 4, hipMemcpyDeviceToHost, st));
-            NH_HIP(hipStreamSynchronize(st));
-            if (live != r->alive) return set_error(NH_E_STATE, "request %llu: %u live lineages, expected %llu",
-                                                  (unsigned long long)r->id, live, (unsigned long long)r->alive);
-            r->cur ^= 1;
-            r->n_in = live;
+        int rr = close_round(s, r);
+        if (rr != NH_OK) {
+            finish(s, r, rr); // this request fails; the others go on
+            if (rc == NH_OK) rc = rr;
+            continue;
         }
-        uint64_t ne = r->n_events - r->round_event_start;
-        if (ne) {
-            sort_events(r->d_events + r->round_event_start, ne, s->d_sorted, s->sb, st, false);
-            NH_HIP(hipGetLastError());
-            size_t old = r->events.size();
-            r->events.resize(old + ne);
-            NH_HIP(hipMemcpyAsync(r->events.data() + old, s->d_sorted, ne * sizeof(DevEvent), hipMemcpyDeviceToHost, st));
-            NH_HIP(hipStreamSynchronize(st));
-        }
-        if (request_finished(r)) NH_TRY(retire(s, r));
+        if (request_finished(r)) finish(s, r, NH_OK);
         else still.push_back(r);
     }
     s->active.swap(still);
     if (n_active) *n_active = s->active.size();
     if (n_queued) *n_queued = s->queued.size();
-    return NH_OK;
+    return rc;
 }
 
 extern "C" int nh_server_poll(nh_server *s, nh_request_result *res) {

@@ -11,7 +11,8 @@ from .algorithm import (PERSISTENT, PLAIN, TASK_LIST, AlmostInfiniteNormalDisper
                         BiodiversityReporter, Context, DedupCache, EventCollector, EventLogReporter,
                         EventSlice,
                         HipIndependentAlgorithm, IndependentArguments, Job, NonSpatialScenario,
-                        OriginPreSampler, PackedEventCollector, ScenarioCogs, Server, SimulationOutcome,
+                        OriginPreSampler, PackedEventCollector, ResumeSampleError, ScenarioCogs, Server,
+                        SimulationOutcome,
                         SpatiallyExplicitUniformTurnoverScenario, Tables, WyHash,
                         autotune, build_alias_tables, copy_calibrate, is_speciation, load_lineages,
                         packed_order, save_lineages, speciation_events)

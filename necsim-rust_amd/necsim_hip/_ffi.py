@@ -82,7 +82,7 @@ class Outcome(ctypes.Structure):
 
 class ServerParams(ctypes.Structure):
     _fields_ = [("max_lineages", c_u64), ("hbm_budget_bytes", c_u64), ("tick_steps", c_u64),
-                ("block_size", c_u32), ("blocks_per_cu", c_u32)]
+                ("block_size", c_u32), ("blocks_per_cu", c_u32), ("max_bypass_ticks", c_u64)]
 
 
 class Request(ctypes.Structure):
@@ -114,6 +114,7 @@ SIGNATURES = {
     "nh_tables_export": (ctypes.c_int, [c_vp, c_vp, c_sz, c_vp, c_sz, ctypes.POINTER(c_sz),
                                         ctypes.POINTER(c_sz)]),
     "nh_job_create": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(JobParams), ctypes.POINTER(c_vp)]),
+    "nh_classify_lineages": (ctypes.c_int, [c_vp, c_vp, c_sz, c_vp, ctypes.POINTER(c_sz)]),
     "nh_job_add_lineages": (ctypes.c_int, [c_vp, c_vp, c_sz]),
     "nh_job_start": (ctypes.c_int, [c_vp]),
     "nh_job_step_round": (ctypes.c_int, [c_vp, ctypes.POINTER(c_i32)]),
@@ -128,6 +129,7 @@ SIGNATURES = {
     "nh_autotune": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(JobParams), c_vp, c_sz, c_f64,
                                    ctypes.POINTER(JobParams), ctypes.POINTER(c_f64)]),
     "nh_server_create": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(ServerParams), ctypes.POINTER(c_vp)]),
+    "nh_server_request_footprint": (ctypes.c_int, [c_u64, c_u64, ctypes.POINTER(c_u64)]),
     "nh_server_submit": (ctypes.c_int, [c_vp, ctypes.POINTER(Request), ctypes.POINTER(c_u64)]),
     "nh_server_tick": (ctypes.c_int, [c_vp, ctypes.POINTER(c_u64), ctypes.POINTER(c_u64)]),
     "nh_server_poll": (ctypes.c_int, [c_vp, ctypes.POINTER(RequestResult)]),

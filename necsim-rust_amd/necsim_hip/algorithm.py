@@ -464,6 +464,16 @@ class Tables:
         self.handle = h
         self.cogs = cogs
 
+    def classify(self, lineages):
+        """NH_LINEAGE_* class of every lineage (nh_classify_lineages): 0 OK,
+        1 OutOfDeme, 2 OutOfHabitat."""
+        lineages = np.ascontiguousarray(lineages, dtype=_ffi.LINEAGE_DTYPE)
+        kinds = np.zeros(len(lineages), np.int32)
+        bad = ctypes.c_size_t(0)
+        check(lib().nh_classify_lineages(self.handle, ptr(lineages), len(lineages), ptr(kinds),
+                                         ctypes.byref(bad)))
+        return kinds
+
     def export(self):
         """(cell records, atoms) of in-memory tables, as resident on the device"""
         nc, na = ctypes.c_size_t(0), ctypes.c_size_t(0)
@@ -494,6 +504,8 @@ class Job:
         check(lib().nh_job_create(ctx.handle, tables.handle, ctypes.byref(p), ctypes.byref(h)))
         self.handle = h
         self.rng = rng
+        # the job reads the tables until it is destroyed: keep them alive
+        self.tables = tables
 
     def add_lineages(self, lineages):
         lineages = np.ascontiguousarray(lineages, dtype=_ffi.LINEAGE_DTYPE)
@@ -603,6 +615,18 @@ def copy_calibrate(ctx, nbytes, reps=10):
     return ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9
 
 
+class ResumeSampleError(Exception):
+    """ResumeError::Sample (rustcoalescence/algorithms/src/result.rs:29-32):
+    the resumed lineages that IndependentActiveLineageSampler rejects
+    (active_lineage_sampler/independent/mod.rs:98-117).  `exceptional` is a
+    list of (kind, lineage) with kind "OutOfDeme" or "OutOfHabitat", in input
+    order."""
+
+    def __init__(self, exceptional):
+        super().__init__(f"{len(exceptional)} exceptional lineage(s) on resume")
+        self.exceptional = exceptional
+
+
 class HipIndependentAlgorithm:
     """Algorithm::initialise_and_simulate / resume_and_simulate of the HIP
     Independent algorithm (Monolithic parallelism mode).
@@ -615,9 +639,17 @@ class HipIndependentAlgorithm:
     speciation events through report_speciation."""
 
     @staticmethod
-    def _run(args, rng, cogs, lineages, pause_before, resume_after, reporter):
+    def _run(args, rng, cogs, lineages, pause_before, resume_after, reporter, check_resume=False):
         ctx = Context.get(args.device)
         tables = Tables(ctx, cogs)
+        if check_resume:
+            # IndependentLineageStoreSampleInitialiser's resume
+            # (rustcoalescence/algorithms/independent/src/initialiser/resume.rs:71-85)
+            kinds = tables.classify(lineages)
+            bad = np.nonzero(kinds)[0]
+            if len(bad):
+                names = {1: "OutOfDeme", 2: "OutOfHabitat"}
+                raise ResumeSampleError([(names[int(kinds[k])], lineages[k]) for k in bad])
         disp = bool(getattr(reporter, "REPORT_DISPERSAL", False))
         on_device = disp and bool(getattr(reporter, "DEVICE_EVENT_COUNTER", False))
         job = Job(ctx, tables, args, rng, cogs.speciation_probability, cogs.turnover_rate,
@@ -658,7 +690,7 @@ class HipIndependentAlgorithm:
     def resume_and_simulate(cls, args: IndependentArguments, rng: WyHash, cogs: ScenarioCogs,
                             lineages, resume_after=None, pause_before=None, reporter=None):
         return cls._run(args, rng, cogs, lineages, pause_before,
-                        0.0 if resume_after is None else resume_after, reporter)
+                        0.0 if resume_after is None else resume_after, reporter, check_resume=True)
 
 
 class Server:
@@ -670,10 +702,10 @@ class Server:
     the latency-optimal shape of the serving stream."""
 
     def __init__(self, ctx, tables, max_lineages, hbm_budget_bytes, tick_steps=0, block_size=0,
-                 blocks_per_cu=0):
+                 blocks_per_cu=0, max_bypass_ticks=0):
         p = _ffi.ServerParams(max_lineages=max_lineages, hbm_budget_bytes=hbm_budget_bytes,
                               tick_steps=tick_steps, block_size=block_size,
-                              blocks_per_cu=blocks_per_cu)
+                              blocks_per_cu=blocks_per_cu, max_bypass_ticks=max_bypass_ticks)
         h = ctypes.c_void_p()
         check(lib().nh_server_create(ctx.handle, tables.handle, ctypes.byref(p), ctypes.byref(h)))
         self.handle = h
@@ -696,6 +728,13 @@ class Server:
         rid = ctypes.c_uint64(0)
         check(lib().nh_server_submit(self.handle, ctypes.byref(r), ctypes.byref(rid)))
         return rid.value
+
+    @staticmethod
+    def request_footprint(n_lineages, cache_buckets=0):
+        """Device bytes a request takes from the server's pool."""
+        b = ctypes.c_uint64(0)
+        check(lib().nh_server_request_footprint(int(n_lineages), int(cache_buckets), ctypes.byref(b)))
+        return b.value
 
     def tick(self):
         a = ctypes.c_uint64(0)

@@ -319,9 +319,23 @@ def test_edge_out_of_deme_rejected(config1):
     cogs, orc, lin, ref = config1
     bad = lin[:4].copy()
     bad[2]["index"] = 5
-    with pytest.raises(nh.NecsimHipError) as e:
+    bad[3]["x"] = 100  # outside the 100x100 extent
+    # resume: ResumeError::Sample with every exceptional lineage, in order
+    # (initialiser/resume.rs:71-85)
+    with pytest.raises(nh.ResumeSampleError) as e:
         hip_run(cogs, bad, 42, PLAIN)
-    assert e.value.code == -6
+    kinds = [k for k, _ in e.value.exceptional]
+    assert kinds == ["OutOfDeme", "OutOfHabitat"]
+    assert e.value.exceptional[0][1].tobytes() == bad[2].tobytes()
+    assert e.value.exceptional[1][1].tobytes() == bad[3].tobytes()
+    # the C ABI refuses them as well, before any kernel runs
+    ctx = nh.Context.get(0)
+    job = nh.Job(ctx, nh.Tables(ctx, cogs), nh.IndependentArguments(), nh.WyHash.seed_from_u64(42),
+                 1e-3, 0.5)
+    with pytest.raises(nh.NecsimHipError) as e2:
+        job.add_lineages(bad)
+    assert e2.value.code == -6
+    job.close()
 
 
 def test_edge_general_delta_t_and_step_slice(config1):
