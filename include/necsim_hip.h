@@ -512,6 +512,18 @@ int nh_copy_calibrate(nh_ctx *ctx, size_t bytes, int32_t reps, double *ms_per_co
  * graph-capturable variant (device_sized != 0).  Allocates device memory. */
 int nh_debug_sort_events(nh_ctx *ctx, const nh_speciation_event *in, size_t n, int32_t device_sized,
                          nh_speciation_event *out);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The sort of nh_debug_sort_events over DEVICE buffers:
+ * d_in holds n speciation events, d_out (distinct from d_in) receives them in
+ * PackedEvent order.  Its sort buffers live in one file-scope device array of
+ * 25 MiB per GPU, held until the process ends; nothing is allocated or freed,
+ * and the call returns after the sort has finished on the context's stream.
+ * n above 1,048,576 events fails with NH_E_CAPACITY.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (from several
+ * threads or contexts) race and must be serialised by the caller. */
+int nh_sort_events_noalloc(nh_ctx *ctx, const nh_speciation_event *d_in, size_t n, int32_t device_sized,
+                           nh_speciation_event *d_out);
 /* The same sort over PackedEvents (dispersal reporting's records). */
 int nh_debug_sort_packed_events(nh_ctx *ctx, const nh_packed_event *in, size_t n, int32_t device_sized,
                                 nh_packed_event *out);

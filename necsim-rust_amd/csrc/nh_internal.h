@@ -67,4 +67,7 @@ struct nh_ctx {
     hipStream_t stream = nullptr;
     uint32_t cus = 0;
     std::string name;
+    // device address of the library's file-scope scratch array on this
+    // context's device (nh_sort_events_noalloc), resolved by nh_init
+    void *scratch = nullptr;
 };

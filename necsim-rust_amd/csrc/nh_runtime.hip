@@ -4,6 +4,7 @@
 // driven on the host, every per-lineage step on the device).
 #include <algorithm>
 #include <deque>
+#include <mutex>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -116,6 +117,91 @@ int download_lineages(const LineageBuffers &b, const uint32_t *d_slots, uint64_t
 } // namespace
 
 // ===========================================================================
+// Scratch of the allocation-free event sort (nh_sort_events_noalloc): one
+// file-scope device array, present on every device the library runs on and
+// held until the process ends.  Host code never passes `scratch` itself to a
+// kernel (on the host it is only a stand-in symbol): nh_init resolves its
+// address on each device once, with that device current, and every use goes
+// through that address.  The sort's kernels initialise every word they read,
+// so nothing carries over from the previous call.
+constexpr uint64_t NOALLOC_MAX_EVENTS = 1ull << 20;
+constexpr uint64_t NOALLOC_SCRATCH_BYTES = 25ull << 20; // 25 MiB
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+
+uint64_t align256(uint64_t v) { return (v + 255) / 256 * 256; }
+
+// carves the sort buffers for n events (and the two range words of the
+// device-sized variant) out of the scratch array at `base`; returns the bytes used
+uint64_t carve_sort_scratch(uintptr_t base, uint64_t n, SortBuffers &sb, unsigned long long **range) {
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) {
+        uintptr_t p = base + off;
+        off += align256(bytes);
+        return p;
+    };
+    sb.keys[0] = reinterpret_cast<unsigned long long *>(take(8 * n));
+    sb.keys[1] = reinterpret_cast<unsigned long long *>(take(8 * n));
+    sb.vals[0] = reinterpret_cast<uint32_t *>(take(4 * n));
+    sb.vals[1] = reinterpret_cast<uint32_t *>(take(4 * n));
+    sb.hist = reinterpret_cast<uint32_t *>(take(4 * (uint64_t)sort_hist_words(n)));
+    sb.minmax = reinterpret_cast<unsigned long long *>(take(8 * (uint64_t)SORT_MINMAX_WORDS));
+    *range = reinterpret_cast<unsigned long long *>(take(16));
+    sb.capacity = n;
+    return off;
+}
+
+// the scratch array's address on `device` (current), resolved once
+int resolve_scratch(int device, void **out) {
+    if (device >= MAX_DEVICES) return set_error(NH_E_INVALID, "device %d above the scratch table", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(NH_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return NH_OK;
+}
+} // namespace
+
+extern "C" int nh_sort_events_noalloc(nh_ctx *ctx, const nh_speciation_event *d_in, size_t n,
+                                      int32_t device_sized, nh_speciation_event *d_out) {
+    static_assert(sizeof(nh_speciation_event) == sizeof(DevEvent), "event record");
+    if (!ctx || (n && (!d_in || !d_out))) return set_error(NH_E_INVALID, "nh_sort_events_noalloc: null argument");
+    if (n == 0) return NH_OK;
+    if (n > NOALLOC_MAX_EVENTS)
+        return set_error(NH_E_CAPACITY, "nh_sort_events_noalloc: %zu events, at most %llu", n,
+                         (unsigned long long)NOALLOC_MAX_EVENTS);
+    if (d_in == d_out) return set_error(NH_E_INVALID, "nh_sort_events_noalloc: input and output must differ");
+    if (!ctx->scratch) return set_error(NH_E_STATE, "nh_sort_events_noalloc: scratch not resolved");
+    NH_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    SortBuffers sb{};
+    unsigned long long *range = nullptr;
+    carve_sort_scratch(reinterpret_cast<uintptr_t>(ctx->scratch), n, sb, &range);
+    const DevEvent *in = reinterpret_cast<const DevEvent *>(d_in);
+    DevEvent *out = reinterpret_cast<DevEvent *>(d_out);
+    if (device_sized) {
+        // the range words are written by fill kernels, never by a memset
+        launch_fill_u64(reinterpret_cast<uint64_t *>(range), 0, 1, st);
+        launch_fill_u64(reinterpret_cast<uint64_t *>(range) + 1, n, 1, st);
+        sort_events_device_sized(in, range + 1, range, n, out, sb, st);
+    } else {
+        sort_events(in, n, out, sb, st, false);
+    }
+    NH_HIP(hipGetLastError());
+    NH_HIP(hipStreamSynchronize(st));
+    return NH_OK;
+}
+
+// ===========================================================================
 // context
 extern "C" int nh_abi_version(void) { return NH_ABI_VERSION; }
 
@@ -142,6 +228,23 @@ extern "C" int nh_init(int device, nh_ctx **out) {
     if (e != hipSuccess) {
         delete c;
         return set_error(NH_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    {
+        // the scratch of nh_sort_events_noalloc must hold the sort buffers of
+        // its largest input
+        SortBuffers sb{};
+        unsigned long long *range = nullptr;
+        if (carve_sort_scratch(0, NOALLOC_MAX_EVENTS, sb, &range) > NOALLOC_SCRATCH_BYTES) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return set_error(NH_E_STATE, "scratch array smaller than the sort buffers");
+        }
+    }
+    int rs = resolve_scratch(device, &c->scratch);
+    if (rs != NH_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rs;
     }
     *out = c;
     return NH_OK;

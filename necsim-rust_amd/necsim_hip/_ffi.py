@@ -146,6 +146,7 @@ SIGNATURES = {
     "nh_lineage_file_read": (ctypes.c_int, [ctypes.c_char_p, c_vp, c_sz, ctypes.POINTER(c_sz)]),
     "nh_copy_calibrate": (ctypes.c_int, [c_vp, c_sz, c_i32, ctypes.POINTER(ctypes.c_double)]),
     "nh_debug_sort_events": (ctypes.c_int, [c_vp, c_vp, c_sz, c_i32, c_vp]),
+    "nh_sort_events_noalloc": (ctypes.c_int, [c_vp, c_vp, c_sz, c_i32, c_vp]),
     "nh_debug_sort_packed_events": (ctypes.c_int, [c_vp, c_vp, c_sz, c_i32, c_vp]),
     "nh_debug_count_packed_events": (ctypes.c_int, [c_vp, c_vp, c_sz, ctypes.POINTER(EventCounts)]),
     "nh_sample_origins": (ctypes.c_int, [ctypes.POINTER(Scenario), c_f64, c_u32, c_u32, c_u32, c_u32,
