@@ -86,13 +86,24 @@ def timed_runs(ctx, tables, cogs, lineages, args, seed, pause, steps, warmup):
     return wall, results
 
 
-def cpu_baseline(workload, pause, budget_s=20.0):
-    """The oracle (C restatement of the reference CPU Independent algorithm)
-    on the host's cores, IsolatedIndividuals partitions over a bounded sample
-    of the same workload."""
+def host_threads():
+    """Threads of the CPU baseline: the CPUs this process may run on, capped
+    at the host's CPU share per GPU where OMP_NUM_THREADS states it (16 per
+    GPU on the MI355X hosts, whatever os.cpu_count() shows)."""
+    n = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(n, share) if share > 0 else n), n
+
+
+def cpu_baseline(workload, pause, budget_s=20.0, single_budget_s=8.0):
+    """The oracle (C restatement of the reference CPU Independent algorithm,
+    gcc -O3 -ffp-contract=off) on the host's cores: IsolatedIndividuals
+    partitions over a bounded sample of the same workload, one thread per
+    partition, on every thread of the host's share; and one thread alone on
+    a smaller sample (the measured per-core rate)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, visible = host_threads()
     if workload == "config4-window":
         scn = O.Scenario.almost_infinite_normal(50.0)
         circle = (2**31 - 1, 2**31 - 1, 2000)
@@ -100,19 +111,28 @@ def cpu_baseline(workload, pause, budget_s=20.0):
     else:
         raise ValueError(workload)
     params = O.make_params(seed, nu, pause_before=pause)
-    # calibrate: a small sample first, then size the sample to the budget
-    parts = 4096
-    r = O.simulate_partitioned(scn, params, 1.0, threads, parts, circle=circle)
-    rate = r["steps"] / max(r["seconds"], 1e-6)
-    sample_lineages = total * threads / parts
-    steps_per_lineage = r["steps"] / max(sample_lineages, 1)
-    want_lineages = budget_s * rate / max(steps_per_lineage, 1e-9)
-    parts = int(max(threads, min(parts, total * threads / max(want_lineages, 1))))
-    r = O.simulate_partitioned(scn, params, 1.0, threads, parts, circle=circle)
-    return {"value": r["steps"] / r["seconds"], "unit": "lineage-steps/s", "cores": threads,
-            "kind": "port", "per_core_value": r["steps"] / r["seconds"] / threads,
-            "sample": f"{threads} of {parts} IsolatedIndividuals partitions of the same workload "
-                      f"({r['steps']} steps, {r['seconds']:.1f} s, one thread per partition)"}
+
+    def sized_run(nthreads, budget):
+        # calibrate on a small sample, then size the sample to the budget
+        parts = 4096
+        r = O.simulate_partitioned(scn, params, 1.0, nthreads, parts, circle=circle)
+        rate = r["steps"] / max(r["seconds"], 1e-6)
+        steps_per_lineage = r["steps"] / max(total * nthreads / parts, 1)
+        want_lineages = budget * rate / max(steps_per_lineage, 1e-9)
+        parts = int(max(nthreads, min(1 << 20, total * nthreads / max(want_lineages, 1))))
+        r = O.simulate_partitioned(scn, params, 1.0, nthreads, parts, circle=circle)
+        return r, parts
+
+    r, parts = sized_run(threads, budget_s)
+    r1, parts1 = sized_run(1, single_budget_s)
+    value, single = r["steps"] / r["seconds"], r1["steps"] / r1["seconds"]
+    return {"value": value, "unit": "lineage-steps/s", "cores": threads, "kind": "port",
+            "single_thread_value": single, "all_threads_over_single": value / single,
+            "host_cpus": {"os_cpu_count": os.cpu_count(), "affinity": visible,
+                          "omp_num_threads": os.environ.get("OMP_NUM_THREADS")},
+            "sample": f"{threads} of {parts} IsolatedIndividuals partitions of the same workload on "
+                      f"{threads} threads ({r['steps']} steps, {r['seconds']:.1f} s); single thread: 1 of "
+                      f"{parts1} partitions ({r1['steps']} steps, {r1['seconds']:.1f} s)"}
 
 
 def config2_run(ctx):
@@ -197,7 +217,7 @@ def serving_run(ctx, n_requests=150, rate=50.0, cap=1000.0, tick_steps=1000, see
         pct = float(rng.uniform(0.01, 1.0))
         lin = base.sample_habitat(nh.OriginPreSampler.all().percentage(pct))
         reqs.append((nu, lin))
-    srv = nh.Server(ctx, tables, max_lineages=1 << 24, hbm_budget_bytes=48 << 30, tick_steps=tick_steps)
+    srv = nh.Server(ctx, tables, max_lineages=1 << 24, hbm_budget_bytes=8 << 30, tick_steps=tick_steps)
     args = nh.IndependentArguments()
     t0 = time.perf_counter()
     submitted = 0
@@ -480,12 +500,17 @@ def main():
     log(f"{a.form}: {steps_total} steps in {wall:.3f} s -> {value:.4e} steps/s; "
         f"device {kern_ms:.1f} ms over {launches} launches")
 
-    # Roofline of the dominant kernel.  Algorithmic HBM bytes of one round
-    # launch: every lineage's state but its reference read once (37 B), the
-    # survivors' state written back (37 B), per speciation a dead flag, the
-    # reference and the 40 B event; the almost-infinite scenario reads no
-    # tables.  The plain form reads the whole state (45 B) and writes the
-    # moving part (37 B) of its pass list once per launch.
+    # Roofline of the dominant kernel.  The almost-infinite scenario gathers
+    # no tables, so the binding limit is VALU issue (64-bit integer hashing
+    # and f64 libm): achieved wave instructions per second, from the
+    # committed rocprofv3 count of VALU instructions per lineage step at a
+    # launch shape times the steps per second that shape runs at here (HIP
+    # events on the kernel's stream), against one wave64 instruction per CU
+    # per cycle.  HBM is reported beside it: algorithmic bytes of one round
+    # launch are every lineage's state but its reference read once (37 B),
+    # the survivors' state written back (37 B), per speciation a dead flag,
+    # the reference and the 40 B event.  The plain form reads the whole state
+    # (45 B) and writes the moving part (37 B) of its pass list per launch.
     nl = max(launches, 1)
     n_in, n_left, n_ev = len(lineages), int(o0.n_remaining), int(o0.n_events)
     if form == nh.PERSISTENT:
@@ -495,27 +520,45 @@ def main():
     else:
         per_launch_bytes = n_in * (LINEAGE_STATE_BYTES + MOVING_STATE_BYTES)
     mean_launch_ms = kern_ms / nl
-    achieved = per_launch_bytes / (mean_launch_ms * 1e-3) / 1e9
+    achieved_gbs = per_launch_bytes / (mean_launch_ms * 1e-3) / 1e9
     prof = read_profile_summary(PROFILE_SUMMARY)
-    same_workload = (prof.get("window") == a.window and prof.get("form") == a.form)
-    traffic = prof.get("hbm_bytes_per_launch") if same_workload else None
+    shape = (args.block_size, args.blocks_per_cu, args.inner_steps, args.exec_fill)
+    prof_shape = tuple(int(prof.get(k, -1)) for k in ("block_size", "blocks_per_cu", "inner_steps", "exec_fill"))
+    same_workload = prof.get("window") == a.window and prof.get("form") == a.form
     kname = "k_persistent<almost-infinite, no dedup>" if form == nh.PERSISTENT else "k_plain<almost-infinite>"
-    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "kernel": kname,
-                "mean_launch_ms": round(mean_launch_ms, 3), "launches": nl,
-                "algorithmic_bytes_per_launch": int(per_launch_bytes),
-                "lineage_steps_per_launch": int(steps_total / nl)}
+    roofline = {"bound": "valu", "achieved": None, "peak": cus * ENGINE_CLOCK_HZ, "unit": "wave-instructions/s",
+                "frac": None, "traffic": None, "kernel": kname}
     ipl = prof.get("valu_wave_instructions_per_lineage_step") if same_workload else None
     if ipl:
-        inst_rate = ipl * steps_total / (kern_ms * 1e-3)
-        peak = cus * ENGINE_CLOCK_HZ
-        roofline["valu"] = {"wave_instructions_per_lineage_step": ipl, "achieved": inst_rate,
-                            "peak": peak, "unit": "wave-instructions/s", "frac": round(inst_rate / peak, 4),
-                            "lane_utilization": prof.get("valu_lane_utilization"),
-                            "source": os.path.relpath(PROFILE_SUMMARY, ROOT)}
-    roofline["note"] = ("the almost-infinite scenario gathers no tables: this kernel is bound by VALU issue "
-                        "(64-bit integer hashing and f64 libm), see roofline.valu; HBM traffic is the "
-                        "lineage state once per round")
+        if prof_shape == shape:
+            v_steps, v_ms, v_launches = steps_total, kern_ms, nl
+        else:
+            # the instruction count was profiled at another launch shape: time
+            # that shape here, so that count and time belong to the same launch
+            vargs = dataclasses.replace(args, block_size=prof_shape[0], blocks_per_cu=prof_shape[1],
+                                        inner_steps=prof_shape[2], exec_fill=prof_shape[3])
+            _, vres = timed_runs(ctx, tables, cogs, lineages, vargs, 42, a.window, 1, 0)
+            v_steps, v_ms, v_launches = int(vres[0][0].steps), vres[0][1], vres[0][2]
+        inst_rate = ipl * v_steps / (v_ms * 1e-3)
+        roofline.update({"achieved": inst_rate, "frac": round(inst_rate / roofline["peak"], 4),
+                         "lane_utilization": prof.get("valu_lane_utilization"),
+                         "lane_weighted_frac": round(inst_rate / roofline["peak"]
+                                                     * (prof.get("valu_lane_utilization") or 0.0), 4),
+                         "wave_instructions_per_lineage_step": ipl,
+                         "profiled_launch_shape": {"block_size": prof_shape[0], "blocks_per_cu": prof_shape[1],
+                                                   "inner_steps": prof_shape[2], "exec_fill": prof_shape[3]},
+                         "mean_launch_ms_at_profiled_shape": round(v_ms / max(v_launches, 1), 3),
+                         "source": os.path.relpath(PROFILE_SUMMARY, ROOT)})
+    roofline["traffic"] = prof.get("hbm_bytes_per_launch") if same_workload and prof_shape == shape else None
+    roofline.update({"mean_launch_ms": round(mean_launch_ms, 3), "launches": nl,
+                     "lineage_steps_per_launch": int(steps_total / nl),
+                     "hbm": {"achieved": round(achieved_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(achieved_gbs / HBM_PEAK_GBS, 6),
+                             "algorithmic_bytes_per_launch": int(per_launch_bytes),
+                             "traffic": prof.get("hbm_bytes_per_launch") if same_workload else None},
+                     "note": "the almost-infinite scenario gathers no tables: the kernel is bound by VALU issue "
+                             "(frac); lane_weighted_frac scales it by the share of active lanes; HBM carries "
+                             "only the lineage state once per round"})
 
     extra = {"autotuned": tuned,
              "launch": {"form": a.form, "block_size": args.block_size, "blocks_per_cu": args.blocks_per_cu,
