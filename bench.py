@@ -335,12 +335,127 @@ def task_list_run(ctx, tables, cogs, lineages, pargs, window, seed=42):
     return out
 
 
-def run_collect(job):
+def species_of(ev):
+    """a species is a distinct (time, origin) speciation event (SpeciationEvent
+    equality, necsim/core/src/event.rs:301-307, as counted by
+    plugins/common/src/biodiversity.rs:32-41)"""
+    return set(zip(ev["event_time"].tolist(), ev["x"].tolist(), ev["y"].tolist(), ev["index"].tolist()))
+
+
+def dedup_window_run(ctx, tables, cogs, lineages, dedup_window):
+    """config 4 with the reference's default dedup cache (Relative(1.0)) up
+    to pause_before = dedup_window: the persistent form (autotuned, then the
+    residency chosen on the window) against the plain form, one host round
+    trip per pass and batched.  Coalesced lineages are dropped, so the window
+    is mostly a coalescence workload; the plain form keeps the reference's
+    queue order, bit-exact.  Returns the results and the tuned arguments."""
+    dd = {}
+    t = time.perf_counter()
+    dtuned, dtms = nh.autotune(ctx, tables, nh.IndependentArguments(), nh.WyHash.seed_from_u64(42), cogs,
+                               lineages[::8], 200.0, tune_step_slice=True)
+    # Second tuning stage on the window itself: the sample trial is
+    # throughput bound and favours full residency, but this window ends
+    # in long serial chains (the population collapses from 12.6M to 15k
+    # lineages), which run faster with fewer resident waves per SIMD.
+    # One untimed run per residency of 256-thread blocks picks it.
+    residency = {}
+    for bpc in (1, 2, 3):
+        targs = dataclasses.replace(dtuned, block_size=256, blocks_per_cu=bpc)
+        tj = make_job(ctx, tables, cogs, lineages, targs, 42, dedup_window)
+        tt = time.perf_counter()
+        run_collect(tj)
+        residency[bpc] = round((time.perf_counter() - tt) * 1e3, 2)
+        tj.close()
+    best_bpc = min(residency, key=residency.get)
+    dtuned = dataclasses.replace(dtuned, block_size=256, blocks_per_cu=best_bpc)
+    dd["autotuned"] = {"step_slice": dtuned.step_slice, "block_size": dtuned.block_size,
+                       "blocks_per_cu": dtuned.blocks_per_cu, "inner_steps": dtuned.inner_steps,
+                       "exec_fill": dtuned.exec_fill, "residency_trial_ms": residency,
+                       "tune_s": time.perf_counter() - t}
+    species, streams = {}, {}
+    # plain: one host round trip per pass (the reference's launch structure);
+    # plain_batched: the same passes, 16 per host synchronisation
+    for name, dargs in (("persistent", dtuned), ("plain", nh.IndependentArguments(form=nh.PLAIN, plain_batch=1)),
+                        ("plain_batched", nh.IndependentArguments(form=nh.PLAIN))):
+        dj = make_job(ctx, tables, cogs, lineages, dargs, 42, dedup_window)
+        t = time.perf_counter()
+        dev = run_collect(dj)
+        dwall = time.perf_counter() - t
+        do = dj.outcome()
+        dms, dk = dj.kernel_time()
+        dj.close()
+        # duplicate lineages that a lossy cache lets through repeat a species
+        species[name] = species_of(dev)
+        streams[name] = (dev.tobytes(), int(do.steps), do.time)
+        dd[name] = {"value": do.steps / dwall, "ms": dwall * 1e3, "steps": int(do.steps),
+                    "kernel_ms": dms, "launches": int(dk), "remaining_lineages": int(do.n_remaining),
+                    "speciation_events": int(do.n_events), "species": len(species[name])}
+    dd["species_sets_identical"] = species["persistent"] == species["plain"] == species["plain_batched"]
+    dd["persistent_over_plain_wall"] = dd["plain"]["ms"] / dd["persistent"]["ms"]
+    # bit-exact with dedup on: the batched plain form keeps the reference's queue order
+    dd["plain_batched_events_bit_identical_to_plain"] = streams["plain_batched"] == streams["plain"]
+    dd["plain_batched_over_plain_wall"] = dd["plain"]["ms"] / dd["plain_batched"]["ms"]
+    return {"pause_before": dedup_window, **dd}, dtuned
+
+
+def config4_tail_run(ctx, tables, cogs, lineages, dtuned, dedup_window, window=2e5, completion=True, seed=42):
+    """config 4's heavy-tailed lineage lifetimes (mean 1/(lambda nu) = 2e6
+    time units) under the reference's default dedup cache: a window 10x the
+    dedup window, and the whole simulation to completion.  The persistent
+    form (tuned on the dedup window; each water-level round's step / sort /
+    compaction cycle replayed from a hipGraph) against the plain form with one
+    host round trip per pass and batched; the phase between the dedup window
+    and this one is the difference of the two runs of each form."""
+    out = {"window": window}
+    gtuned = dataclasses.replace(dtuned, use_graph=True)
+    runs = [("window", window, (("persistent", gtuned),
+                                ("plain", nh.IndependentArguments(form=nh.PLAIN, plain_batch=1)),
+                                ("plain_batched", nh.IndependentArguments(form=nh.PLAIN))))]
+    if completion:
+        runs.append(("completion", None, (("persistent", gtuned),)))
+    for label, pause, forms in runs:
+        r, species = {"pause_before": pause}, {}
+        for name, fargs in forms:
+            job = make_job(ctx, tables, cogs, lineages, fargs, seed, pause)
+            t = time.perf_counter()
+            ev = run_collect(job, progress=f"config 4 tail {label} {name}")
+            wall = time.perf_counter() - t
+            o = job.outcome()
+            ms, k = job.kernel_time()
+            job.close()
+            species[name] = species_of(ev)
+            r[name] = {"value": o.steps / wall, "ms": wall * 1e3, "steps": int(o.steps), "kernel_ms": ms,
+                       "launches": int(k), "rounds": int(o.rounds), "time": o.time,
+                       "remaining_lineages": int(o.n_remaining), "speciation_events": int(o.n_events),
+                       "species": len(species[name])}
+            log(f"config 4 tail {label} {name}: {r[name]}")
+        if len(forms) > 1:
+            r["species_sets_identical"] = len({frozenset(v) for v in species.values()}) == 1
+            for name in ("plain", "plain_batched"):
+                r[f"persistent_over_{name}_wall"] = r[name]["ms"] / r["persistent"]["ms"]
+        out[label] = r
+    # the tail phase (dedup window .. window) of each form
+    w = out["window"]
+    phase = {}
+    for name in ("persistent", "plain", "plain_batched"):
+        a, b = dedup_window[name], w[name]
+        dsteps, dms = b["steps"] - a["steps"], b["ms"] - a["ms"]
+        phase[name] = {"steps": dsteps, "ms": dms, "value": dsteps / (dms * 1e-3) if dms > 0 else None}
+    for name in ("plain", "plain_batched"):
+        phase[f"persistent_over_{name}_wall"] = phase[name]["ms"] / phase["persistent"]["ms"]
+    out[f"phase_{dedup_window['pause_before']:g}_to_{window:g}"] = phase
+    return out
+
+
+def run_collect(job, progress=None):
     fin = False
     chunks = []
     while not fin:
         fin = job.step_round()
         chunks.append(job.drain_events())
+        if progress:
+            o = job.outcome()
+            log(f"{progress}: round {o.rounds}, {o.n_remaining} lineages left, time {o.time:.4g}")
     return np.concatenate(chunks) if chunks else np.zeros(0, nh.EVENT_DTYPE)
 
 
@@ -403,7 +518,11 @@ def main():
     ap.add_argument("--window", type=float, default=2000.0, help="pause_before of the headline workload")
     ap.add_argument("--dedup-window", type=float, default=20000.0,
                     help="pause_before of the dedup-on config 4 comparison")
-    ap.add_argument("--part", choices=("all", "headline", "gather", "dispersal"), default="all",
+    ap.add_argument("--tail-window", type=float, default=200000.0,
+                    help="pause_before of the config 4 heavy-tail comparison (0: skip it)")
+    ap.add_argument("--no-completion", action="store_true",
+                    help="skip the config 4 run to completion (persistent form)")
+    ap.add_argument("--part", choices=("all", "headline", "gather", "dispersal", "tail"), default="all",
                     help="headline: the headline runs only; gather: the alias-gather window only; "
                          "dispersal: the dispersal-reporting windows only")
     ap.add_argument("--dispersal-window", type=float, default=200.0,
@@ -470,6 +589,13 @@ def main():
     lineages = cogs.sample_habitat(nh.OriginPreSampler.all())
     tables = nh.Tables(ctx, cogs)
     log(f"config 4: {len(lineages)} lineages, setup {time.perf_counter() - t:.2f} s")
+    if a.part == "tail":
+        dd, dtuned = dedup_window_run(ctx, tables, cogs, lineages, a.dedup_window)
+        log(f"config 4 dedup window: {dd}")
+        tail = config4_tail_run(ctx, tables, cogs, lineages, dtuned, dd, a.tail_window,
+                                completion=not a.no_completion)
+        print(json.dumps({"part": "tail", "config4_dedup_window": dd, "config4_tail": tail}), flush=True)
+        return
     form = nh.PERSISTENT if a.form == "persistent" else nh.PLAIN
     fixed = a.block_size or a.blocks_per_cu or a.inner_steps or a.exec_fill
     args = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=form,
@@ -590,61 +716,13 @@ def main():
         log(f"plain: {po.steps} steps in {pwall:.3f} s -> {po.steps / pwall:.4e} steps/s; identical={same}")
         extra["task_list_window"] = task_list_run(ctx, tables, cogs, lineages, args, a.task_window)
 
-        # config 4 with the reference's default dedup cache (Relative(1.0)):
-        # coalesced lineages are dropped, the window is mostly a coalescence
-        # workload; plain form = the reference's queue order, bit-exact
-        dd = {}
-        t = time.perf_counter()
-        dtuned, dtms = nh.autotune(ctx, tables, nh.IndependentArguments(), nh.WyHash.seed_from_u64(42), cogs,
-                                   lineages[::8], 200.0, tune_step_slice=True)
-        # Second tuning stage on the window itself: the sample trial is
-        # throughput bound and favours full residency, but this window ends
-        # in long serial chains (the population collapses from 12.6M to 15k
-        # lineages), which run faster with fewer resident waves per SIMD.
-        # One untimed run per residency of 256-thread blocks picks it.
-        residency = {}
-        for bpc in (1, 2, 3):
-            targs = dataclasses.replace(dtuned, block_size=256, blocks_per_cu=bpc)
-            tj = make_job(ctx, tables, cogs, lineages, targs, 42, a.dedup_window)
-            tt = time.perf_counter()
-            run_collect(tj)
-            residency[bpc] = round((time.perf_counter() - tt) * 1e3, 2)
-            tj.close()
-        best_bpc = min(residency, key=residency.get)
-        dtuned = dataclasses.replace(dtuned, block_size=256, blocks_per_cu=best_bpc)
-        dd["autotuned"] = {"step_slice": dtuned.step_slice, "block_size": dtuned.block_size,
-                           "blocks_per_cu": dtuned.blocks_per_cu, "inner_steps": dtuned.inner_steps,
-                           "exec_fill": dtuned.exec_fill, "residency_trial_ms": residency,
-                           "tune_s": time.perf_counter() - t}
-        species, streams = {}, {}
-        # plain: one host round trip per pass (the reference's launch structure);
-        # plain_batched: the same passes, 16 per host synchronisation
-        for name, dargs in (("persistent", dtuned), ("plain", nh.IndependentArguments(form=nh.PLAIN, plain_batch=1)),
-                            ("plain_batched", nh.IndependentArguments(form=nh.PLAIN))):
-            dj = make_job(ctx, tables, cogs, lineages, dargs, 42, a.dedup_window)
-            t = time.perf_counter()
-            dev = run_collect(dj)
-            dwall = time.perf_counter() - t
-            do = dj.outcome()
-            dms, dk = dj.kernel_time()
-            dj.close()
-            # a species is a distinct (time, origin) speciation event
-            # (SpeciationEvent equality, necsim/core/src/event.rs:301-307, as
-            # counted by plugins/common/src/biodiversity.rs:32-41): duplicate
-            # lineages that a lossy cache lets through repeat one
-            species[name] = set(zip(dev["event_time"].tolist(), dev["x"].tolist(), dev["y"].tolist(),
-                                    dev["index"].tolist()))
-            streams[name] = (dev.tobytes(), int(do.steps), do.time)
-            dd[name] = {"value": do.steps / dwall, "ms": dwall * 1e3, "steps": int(do.steps),
-                        "kernel_ms": dms, "launches": int(dk), "remaining_lineages": int(do.n_remaining),
-                        "speciation_events": int(do.n_events), "species": len(species[name])}
-        dd["species_sets_identical"] = species["persistent"] == species["plain"] == species["plain_batched"]
-        dd["persistent_over_plain_wall"] = dd["plain"]["ms"] / dd["persistent"]["ms"]
-        # bit-exact with dedup on: the batched plain form keeps the reference's queue order
-        dd["plain_batched_events_bit_identical_to_plain"] = streams["plain_batched"] == streams["plain"]
-        dd["plain_batched_over_plain_wall"] = dd["plain"]["ms"] / dd["plain_batched"]["ms"]
-        extra["config4_dedup_window"] = {"pause_before": a.dedup_window, **dd}
+        dd, dtuned = dedup_window_run(ctx, tables, cogs, lineages, a.dedup_window)
+        extra["config4_dedup_window"] = dd
         log(f"config 4 dedup window: {extra['config4_dedup_window']}")
+        if a.tail_window > 0:
+            extra["config4_tail"] = config4_tail_run(ctx, tables, cogs, lineages, dtuned,
+                                                     extra["config4_dedup_window"], a.tail_window,
+                                                     completion=not a.no_completion)
         extra["dispersal_reporting"] = dispersal_run(ctx, tables, cogs, lineages, args, a.dispersal_window,
                                                      a.drain_window)
         del tables

@@ -20,6 +20,8 @@ struct Counters {
     unsigned long long n_dead;        // persistent form: lineages that died this round
     unsigned long long ev_overflow;   // dispersal reporting: the round's event buffer was full
     unsigned long long n_ghost;       // dispersal reporting: reserved event slots left unused
+    unsigned long long compact_live;  // gated lineage compaction: live lineages it counted
+    unsigned long long compact_done;  // ... and whether it compacted them into the other buffer
 };
 
 // Persistent dispersal reporting reserves event slots EVENT_CHUNK at a time
@@ -193,6 +195,14 @@ void sort_events_device_sized(const E *events, const unsigned long long *end, co
 uint32_t soa_compact_blocks(uint64_t n);
 void launch_soa_compact(const LineageSoA &in, const LineageSoA &out, uint64_t n, uint32_t *blk,
                         hipStream_t stream);
+// The same, decided on the device (capturable in a graph): the slot count is
+// *n_dev (at most cap), the live lineages are counted, and only if dead slots
+// make up at least an eighth of the buffer are the live ones compacted into
+// `out`; ctr->compact_live / compact_done report the count and the decision.
+// blk holds soa_compact_blocks(cap) + 1 words.
+void launch_soa_compact_gated(const LineageSoA &in, const LineageSoA &out, uint64_t cap,
+                              const unsigned long long *n_dev, uint32_t *blk, Counters *ctr,
+                              hipStream_t stream);
 
 // EventCounterReporter (necsim/plugins/common/src/event_counter.rs:12-26)
 // kept on the device: its eight counters, and the prior time of the last

@@ -643,8 +643,10 @@ constexpr int SC_THREADS = 256;
 constexpr int SC_ITEMS = 8;
 constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
 
-__global__ void __launch_bounds__(SC_THREADS) k_soa_count(const uint8_t *flags, uint64_t n, uint32_t *blk) {
+__global__ void __launch_bounds__(SC_THREADS) k_soa_count(const uint8_t *flags, uint64_t n,
+                                                           const unsigned long long *n_dev, uint32_t *blk) {
     __shared__ uint32_t c;
+    if (n_dev) n = *n_dev < n ? *n_dev : n;
     if (threadIdx.x == 0) c = 0;
     __syncthreads();
     uint64_t base = (uint64_t)blockIdx.x * SC_TILE;
@@ -659,9 +661,13 @@ __global__ void __launch_bounds__(SC_THREADS) k_soa_count(const uint8_t *flags, 
 }
 
 __global__ void __launch_bounds__(SC_THREADS) k_soa_scatter(LineageSoA in, LineageSoA out, uint64_t n,
+                                                            const unsigned long long *n_dev, const Counters *gate,
                                                             const uint32_t *off) {
     __shared__ uint32_t wcnt[SC_THREADS / 64];
     __shared__ uint32_t run;
+    if (gate && !gate->compact_done) return;
+    if (n_dev) n = *n_dev < n ? *n_dev : n;
+    if ((uint64_t)blockIdx.x * SC_TILE >= n) return;
     const unsigned lane = lane_id();
     const unsigned wave = threadIdx.x / 64;
     if (threadIdx.x == 0) run = off[blockIdx.x];
@@ -690,6 +696,16 @@ __global__ void __launch_bounds__(SC_THREADS) k_soa_scatter(LineageSoA in, Linea
             for (unsigned w = 0; w < SC_THREADS / 64; w++) run += wcnt[w];
         __syncthreads();
     }
+}
+
+// the compaction decision of the gated compaction: dead slots are at least an
+// eighth of the n slots (blk[nb] holds the live count after the scan)
+__global__ void k_soa_gate(const uint32_t *blk, uint32_t nb, uint64_t cap, const unsigned long long *n_dev,
+                           Counters *ctr) {
+    const uint64_t n = *n_dev < cap ? *n_dev : cap;
+    const uint64_t live = blk[nb];
+    ctr->compact_live = live;
+    ctr->compact_done = (n > live && (n - live) * 8 >= n) ? 1ull : 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -1434,6 +1450,7 @@ __global__ void k_reset_counters(Counters *c, bool reset_events) {
     c->head = 0;
     c->n_out = 0;
     c->n_dead = 0;
+    c->compact_done = 0;
     if (reset_events) {
         c->n_events = 0;
         c->ev_overflow = 0;
@@ -1893,9 +1910,24 @@ void launch_soa_compact(const LineageSoA &in, const LineageSoA &out, uint64_t n,
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, blk, 0u, (uint64_t)1);
         return;
     }
-    hipLaunchKernelGGL(k_soa_count, dim3(nb), dim3(SC_THREADS), 0, stream, in.flags, n, blk);
+    hipLaunchKernelGGL(k_soa_count, dim3(nb), dim3(SC_THREADS), 0, stream, in.flags, n,
+                       (const unsigned long long *)nullptr, blk);
     hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, stream, blk, (uint64_t)nb);
-    hipLaunchKernelGGL(k_soa_scatter, dim3(nb), dim3(SC_THREADS), 0, stream, in, out, n, blk);
+    hipLaunchKernelGGL(k_soa_scatter, dim3(nb), dim3(SC_THREADS), 0, stream, in, out, n,
+                       (const unsigned long long *)nullptr, (const Counters *)nullptr, blk);
+}
+
+void launch_soa_compact_gated(const LineageSoA &in, const LineageSoA &out, uint64_t cap,
+                              const unsigned long long *n_dev, uint32_t *blk, Counters *ctr,
+                              hipStream_t stream) {
+    uint32_t nb = soa_compact_blocks(cap);
+    if (!nb) return;
+    // blocks past *n_dev count zero, so the scan total is the live count
+    hipLaunchKernelGGL(k_soa_count, dim3(nb), dim3(SC_THREADS), 0, stream, in.flags, cap, n_dev, blk);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, stream, blk, (uint64_t)nb);
+    hipLaunchKernelGGL(k_soa_gate, dim3(1), dim3(1), 0, stream, blk, nb, cap, n_dev, ctr);
+    hipLaunchKernelGGL(k_soa_scatter, dim3(nb), dim3(SC_THREADS), 0, stream, in, out, cap, n_dev,
+                       (const Counters *)ctr, blk);
 }
 
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream) {

@@ -1019,6 +1019,9 @@ static int persistent_round(nh_job *j, bool *sorted) {
             if (!j->disp)
                 sort_events_device_sized(j->d_events, &j->d_ctr->n_events, &j->d_round->ev_start, j->n_initial,
                                          j->d_sorted, j->sb, st);
+            // step -> sort -> compaction: the whole cycle of a round in one graph
+            launch_soa_compact_gated(j->lin[g].s, j->lin[g ^ 1].s, j->lin[g].cap, &j->d_round->n_in, j->d_blk,
+                                     j->d_ctr, st);
             NH_HIP(hipStreamEndCapture(st, &graph));
             hipError_t e = hipGraphInstantiate(&j->graph[g], graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
@@ -1034,14 +1037,24 @@ static int persistent_round(nh_job *j, bool *sorted) {
         launch_persistent(j->tables->kind, persist_args(j, j->cur), j->blocks, j->threads, st);
         NH_HIP(hipGetLastError());
         NH_HIP(hipEventRecord(j->ev1, st));
+        launch_soa_compact_gated(j->lin[j->cur].s, j->lin[j->cur ^ 1].s, j->lin[j->cur].cap, &j->d_round->n_in,
+                                 j->d_blk, j->d_ctr, st);
+        NH_HIP(hipGetLastError());
     }
     j->launches++;
     Counters c;
     NH_TRY(read_counters(j, c));
     account_kernel_time(j);
     j->alive -= c.n_dead;
-    // drop dead slots once they are an eighth of the buffer
-    if ((j->cur_n - j->alive) * 8 >= j->cur_n && j->cur_n > j->alive) NH_TRY(compact_lineages(j));
+    // the device dropped the dead slots once they made up an eighth of the
+    // buffer (launch_soa_compact_gated): the live lineages are in the other one
+    if (c.compact_done) {
+        if (c.compact_live != j->alive)
+            return set_error(NH_E_STATE, "compaction found %llu live lineages, expected %llu", c.compact_live,
+                             (unsigned long long)j->alive);
+        j->cur ^= 1;
+        j->cur_n = c.compact_live;
+    }
     return NH_OK;
 }
 

@@ -8,6 +8,8 @@ speciation stream, step count, time and remaining lineages bit for bit; with
 the reference's default cache the plain form does the same and the
 persistent form keeps the order-independent invariants.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -96,3 +98,44 @@ def test_map256_window(form):
     out, ev = hip_run(cogs, lin, 42, form, dedup="none", pause_before=20.0)
     assert_exact(ref, out, ev, "map256 window")
     assert same_remaining(out, ref)
+
+
+def _host_threads():
+    n = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(n, share) if share > 0 else min(n, 16))
+
+
+def oracle_none_partitioned(orc, lin, seed, nu, pause, threads):
+    """The dedup-None oracle over interleaved slices of the lineages, one
+    thread each (the C oracle runs without the GIL).  Lineages never interact
+    and with DedupCache::None nothing depends on which others share a run, so
+    the merged, PackedEvent-sorted stream, the step total and the remaining
+    lineages equal the monolithic run's."""
+    from concurrent.futures import ThreadPoolExecutor
+    parts = [lin[r::threads] for r in range(threads)]
+    with ThreadPoolExecutor(threads) as ex:
+        res = list(ex.map(lambda p: oracle_run(orc, p, seed, nu, dedup="none", pause_before=pause), parts))
+    ev = np.concatenate([r["events"] for r in res])
+    ev = ev[np.lexsort((ev["global_reference"], ev["prior_time"], ev["index"], ev["y"], ev["x"],
+                        ev["event_time"]))]
+    return dict(events=ev, steps=sum(r["steps"] for r in res), time=max(r["time"] for r in res),
+                paused=any(r["paused"] for r in res),
+                remaining=np.concatenate([r["remaining"] for r in res]))
+
+
+def test_config4_sample_heavy_tail_window(config4_sample):
+    """config 4's heavy tail: T = 2e5 (about 1e5 dependent steps per
+    surviving lineage, 100x the headline window) on the 0.1 % sample at
+    nu = 1e-6: the persistent form without dedup reproduces the oracle bit
+    for bit, and with the reference's default cache keeps the species and
+    the unique events"""
+    threads = _host_threads()
+    announce(f"config4 0.1% sample, pause 2e5, nu 1e-6: oracle on {threads} threads, persistent dedup None / 1.0")
+    cogs_nu, orc, lin = config4_sample
+    ref = oracle_none_partitioned(orc, lin, 42, 1e-6, 2e5, threads)
+    out, ev = hip_run(cogs_nu[1e-6], lin, 42, PERSISTENT, dedup="none", pause_before=2e5)
+    assert_exact(ref, out, ev, "config4 sample tail dedup None")
+    assert same_remaining(out, ref)
+    out, ev = hip_run(cogs_nu[1e-6], lin, 42, PERSISTENT, dedup=1.0, pause_before=2e5)
+    assert_invariants(ev, ref["events"], None, "config4 sample tail dedup persistent")
