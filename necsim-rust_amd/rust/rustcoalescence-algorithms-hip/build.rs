@@ -1,7 +1,11 @@
-// Links the prebuilt C-ABI library (necsim-rust_amd/libnecsim_hip.so).
+// Links the prebuilt C-ABI library libnecsim_hip.so (built with
+// necsim-rust_amd/build.py); NECSIM_HIP_LIB_DIR names its directory.
 fn main() {
-    let dir = std::env::var("NECSIM_HIP_LIB_DIR").unwrap_or_else(|_| "../..".into());
+    println!("cargo:rerun-if-env-changed=NECSIM_HIP_LIB_DIR");
+    let dir = std::env::var("NECSIM_HIP_LIB_DIR").unwrap_or_else(|_| {
+        panic!("set NECSIM_HIP_LIB_DIR to the directory that holds libnecsim_hip.so (necsim-rust_amd/)")
+    });
     println!("cargo:rustc-link-search=native={dir}");
     println!("cargo:rustc-link-lib=dylib=necsim_hip");
-    println!("cargo:rerun-if-env-changed=NECSIM_HIP_LIB_DIR");
+    println!("cargo:rustc-link-arg=-Wl,-rpath,{dir}");
 }

@@ -4,6 +4,8 @@
 use core::ffi::{c_char, c_int};
 
 pub const NH_OK: c_int = 0;
+pub const NH_E_INVALID: c_int = -1;
+pub const NH_E_STATE: c_int = -4;
 
 #[repr(C)]
 pub struct nh_ctx {
@@ -59,18 +61,24 @@ pub struct nh_packed_event {
     pub coalescence: u64,
 }
 
+/// Same layout as necsim_impls_no_std::alias::packed::AliasMethodSamplerAtom<usize>.
 #[repr(C)]
+#[derive(Clone, Copy, Default)]
 pub struct nh_alias_atom {
     pub u: f64,
     pub e: u64,
     pub k: u64,
 }
+/// Same layout as .../in_memory/packed_alias::AliasSamplerRange.
 #[repr(C)]
+#[derive(Clone, Copy, Default)]
 pub struct nh_alias_range {
     pub start: u64,
     pub end: u64,
 }
+/// Same layout as .../in_memory/packed_separable_alias::SeparableAliasSelfDispersal.
 #[repr(C)]
+#[derive(Clone, Copy, Default)]
 pub struct nh_self_dispersal {
     pub self_dispersal: f64,
     pub non_self_dispersal_event: u64,
@@ -82,6 +90,7 @@ pub const NH_SCENARIO_SPATIALLY_EXPLICIT: i32 = 2;
 pub const NH_SCENARIO_SPATIALLY_EXPLICIT_PACKED: i32 = 3;
 
 #[repr(C)]
+#[derive(Clone, Copy)]
 pub struct nh_scenario {
     pub kind: i32,
     pub width: u32,
@@ -97,6 +106,25 @@ pub struct nh_scenario {
     pub n_atoms: usize,
 }
 
+impl Default for nh_scenario {
+    fn default() -> Self {
+        Self {
+            kind: NH_SCENARIO_NON_SPATIAL,
+            width: 0,
+            height: 0,
+            origin_x: 0,
+            origin_y: 0,
+            deme: 0,
+            sigma: 0.0,
+            habitat: core::ptr::null(),
+            alias_ranges: core::ptr::null(),
+            self_dispersal: core::ptr::null(),
+            atoms: core::ptr::null(),
+            n_atoms: 0,
+        }
+    }
+}
+
 pub const NH_DEDUP_NONE: i32 = 0;
 pub const NH_DEDUP_RELATIVE: i32 = 1;
 pub const NH_DEDUP_ABSOLUTE: i32 = 2;
@@ -104,9 +132,14 @@ pub const NH_SLICE_RELATIVE: i32 = 1;
 pub const NH_SLICE_ABSOLUTE: i32 = 2;
 pub const NH_FORM_PLAIN: i32 = 0;
 pub const NH_FORM_PERSISTENT: i32 = 1;
+pub const NH_FORM_TASK_LIST: i32 = 2;
+
+pub const NH_LINEAGE_OK: i32 = 0;
+pub const NH_LINEAGE_OUT_OF_DEME: i32 = 1;
+pub const NH_LINEAGE_OUT_OF_HABITAT: i32 = 2;
 
 #[repr(C)]
-#[derive(Default)]
+#[derive(Clone, Copy, Default)]
 pub struct nh_job_params {
     pub rng_seed: u64,
     pub speciation_probability: f64,
@@ -137,7 +170,7 @@ pub struct nh_job_params {
 }
 
 #[repr(C)]
-#[derive(Default)]
+#[derive(Clone, Copy, Default)]
 pub struct nh_outcome {
     pub paused: i32,
     pub time: f64,
@@ -154,12 +187,30 @@ extern "C" {
     pub fn nh_ctx_destroy(ctx: *mut nh_ctx) -> c_int;
     pub fn nh_tables_create(ctx: *mut nh_ctx, s: *const nh_scenario, out: *mut *mut nh_tables) -> c_int;
     pub fn nh_tables_destroy(t: *mut nh_tables) -> c_int;
+    pub fn nh_classify_lineages(
+        t: *const nh_tables,
+        l: *const nh_lineage,
+        n: usize,
+        kinds: *mut i32,
+        n_exceptional: *mut usize,
+    ) -> c_int;
     pub fn nh_job_create(ctx: *mut nh_ctx, t: *const nh_tables, p: *const nh_job_params, out: *mut *mut nh_job) -> c_int;
     pub fn nh_job_add_lineages(job: *mut nh_job, l: *const nh_lineage, n: usize) -> c_int;
+    pub fn nh_job_start(job: *mut nh_job) -> c_int;
     pub fn nh_job_step_round(job: *mut nh_job, finished: *mut i32) -> c_int;
     pub fn nh_job_drain_events(job: *mut nh_job, buf: *mut nh_speciation_event, cap: usize, n_out: *mut usize) -> c_int;
     pub fn nh_job_drain_packed_events(job: *mut nh_job, buf: *mut nh_packed_event, cap: usize, n_out: *mut usize) -> c_int;
     pub fn nh_job_outcome(job: *const nh_job, out: *mut nh_outcome) -> c_int;
     pub fn nh_job_take_lineages(job: *mut nh_job, buf: *mut nh_lineage, cap: usize, n_out: *mut usize) -> c_int;
     pub fn nh_job_destroy(job: *mut nh_job) -> c_int;
+    pub fn nh_autotune(
+        ctx: *mut nh_ctx,
+        t: *const nh_tables,
+        base: *const nh_job_params,
+        sample: *const nh_lineage,
+        n: usize,
+        window: f64,
+        tuned: *mut nh_job_params,
+        best_ms: *mut f64,
+    ) -> c_int;
 }

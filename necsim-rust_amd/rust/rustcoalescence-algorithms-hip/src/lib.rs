@@ -2,26 +2,31 @@
 //!
 //! Mirrors rustcoalescence/algorithms/independent/src/lib.rs:39-173 (CPU
 //! Independent) and rustcoalescence/algorithms/cuda/src/lib.rs:66-593 (the
-//! existing GPU drop-in): same arguments (delta_t, step_slice, dedup_cache,
-//! Monolithic event_slice), same MathsCore (ReproducibleMathsCore) and RNG
-//! (WyHash), same `SimulationOutcome`.  All steps run on the GPU; events come
-//! back sorted per water-level round and are reported through the local
-//! partition like the live water-level reporter
-//! (necsim/impls/no-std/src/parallelisation/independent/monolithic/reporter/live.rs:88-115).
+//! existing GPU drop-in): the CPU algorithm's arguments (delta_t, step_slice,
+//! dedup_cache, Monolithic event_slice), the deterministic MathsCore
+//! (ReproducibleMathsCore) and RNG (WyHash), the same `SimulationOutcome`.
+//! Every step runs on the GPU; each water-level round's events come back
+//! sorted and are reported through the local partition like the live
+//! water-level reporter
+//! (necsim/impls/no-std/src/parallelisation/independent/monolithic/reporter/live.rs:88-115),
+//! and progress (the lineages left) is reported like the monolithic loop does
+//! (.../parallelisation/independent/monolithic/mod.rs:107,244,260), once per
+//! round instead of once per pass.
 
 use std::{ffi::CStr, marker::PhantomData};
 
 use necsim_core::{
-    cogs::{MathsCore, RngCore},
+    cogs::MathsCore,
     event::{PackedEvent, SpeciationEvent, TypedEvent},
-    lineage::{GlobalLineageReference, Lineage},
+    lineage::Lineage,
     reporter::{boolean::Boolean, Reporter},
 };
 use necsim_core_bond::{NonNegativeF64, PositiveF64};
 use necsim_impls_no_std::cogs::{
-    lineage_store::independent::IndependentLineageStore,
-    maths::reproducible::ReproducibleMathsCore,
-    origin_sampler::pre_sampler::OriginPreSampler, rng::wyhash::WyHash,
+    active_lineage_sampler::resuming::lineage::ExceptionalLineage,
+    lineage_store::independent::IndependentLineageStore, maths::reproducible::ReproducibleMathsCore,
+    origin_sampler::{pre_sampler::OriginPreSampler, resuming::ResumingOriginSampler},
+    rng::wyhash::WyHash,
 };
 use necsim_partitioning_core::{
     partition::{Partition, PartitionSize},
@@ -32,14 +37,14 @@ use rustcoalescence_algorithms::{
     strategy::RestartFixUpStrategy,
     Algorithm, AlgorithmDefaults, AlgorithmDispatch, AlgorithmParamters,
 };
-use rustcoalescence_scenarios::{Scenario, ScenarioCogs};
+use rustcoalescence_scenarios::ScenarioCogs;
 
 mod arguments;
 pub mod ffi;
 mod scenario;
 
-pub use arguments::HipArguments;
-pub use scenario::HipScenario;
+pub use arguments::{HipArguments, HipForm};
+pub use scenario::{HipCogs, HipScenario};
 
 #[derive(thiserror::Error, Debug, serde::Serialize, serde::Deserialize)]
 #[error("necsim-hip error {code}: {message}")]
@@ -72,15 +77,10 @@ impl AlgorithmDefaults for HipAlgorithm {
     type Rng<M: MathsCore> = WyHash<M>;
 }
 
-impl<M: MathsCore, O: Scenario<M, WyHash<M>> + HipScenario<M>, R: Reporter>
-    AlgorithmDispatch<M, WyHash<M>, O, R> for HipAlgorithm
-{
+impl<M: MathsCore, O: HipScenario<M>, R: Reporter> AlgorithmDispatch<M, WyHash<M>, O, R> for HipAlgorithm {
     type Algorithm<'p, P: LocalPartition<'p, R>> = Self;
 
-    fn get_logical_partition_size<P: Partitioning>(
-        _args: &Self::Arguments,
-        _partitioning: &P,
-    ) -> PartitionSize {
+    fn get_logical_partition_size<P: Partitioning>(_args: &Self::Arguments, _partitioning: &P) -> PartitionSize {
         PartitionSize::MONOLITHIC
     }
 }
@@ -92,10 +92,54 @@ fn wyhash_seed<M: MathsCore>(rng: &WyHash<M>) -> u64 {
     unsafe { *(rng as *const WyHash<M>).cast::<u64>() }
 }
 
+/// Lineage and nh_lineage share one #[repr(C)] layout
+/// (necsim/core/src/lineage.rs:97-109).
+fn as_nh_lineages(lineages: &[Lineage]) -> *const ffi::nh_lineage {
+    debug_assert_eq!(core::mem::size_of::<Lineage>(), core::mem::size_of::<ffi::nh_lineage>());
+    lineages.as_ptr().cast()
+}
+
+/// One device context with the scenario's resident tables (and the job).
 struct Device {
     ctx: *mut ffi::nh_ctx,
     tables: *mut ffi::nh_tables,
     job: *mut ffi::nh_job,
+}
+
+impl Device {
+    fn open(args: &HipArguments, cogs: &HipCogs) -> Result<Self, HipError> {
+        let mut dev = Device { ctx: core::ptr::null_mut(), tables: core::ptr::null_mut(), job: core::ptr::null_mut() };
+        check(unsafe { ffi::nh_init(args.device, &mut dev.ctx) })?;
+        let desc = cogs.scenario();
+        check(unsafe { ffi::nh_tables_create(dev.ctx, &desc, &mut dev.tables) })?;
+        Ok(dev)
+    }
+
+    /// IndependentActiveLineageSampler::resume_with_store_and_lineages's
+    /// filters (active_lineage_sampler/independent/mod.rs:98-117): the
+    /// lineages that are out of their habitat or deme, in input order.
+    fn exceptional(&self, lineages: &[Lineage]) -> Result<Vec<ExceptionalLineage>, HipError> {
+        let mut kinds = vec![ffi::NH_LINEAGE_OK; lineages.len()];
+        let mut n_bad = 0_usize;
+        check(unsafe {
+            ffi::nh_classify_lineages(
+                self.tables,
+                as_nh_lineages(lineages),
+                lineages.len(),
+                kinds.as_mut_ptr(),
+                &mut n_bad,
+            )
+        })?;
+        let mut out = Vec::with_capacity(n_bad);
+        for (lineage, kind) in lineages.iter().zip(kinds) {
+            match kind {
+                ffi::NH_LINEAGE_OUT_OF_DEME => out.push(ExceptionalLineage::OutOfDeme(lineage.clone())),
+                ffi::NH_LINEAGE_OUT_OF_HABITAT => out.push(ExceptionalLineage::OutOfHabitat(lineage.clone())),
+                _ => (),
+            }
+        }
+        Ok(out)
+    }
 }
 
 impl Drop for Device {
@@ -114,24 +158,33 @@ impl Drop for Device {
     }
 }
 
+/// The autotuner's launch shape for the persistent form (nh_autotune on
+/// every 8th lineage over `window`), kept only for the knobs it tunes.
+fn autotuned(dev: &Device, params: &ffi::nh_job_params, lineages: &[Lineage], window: f64) -> Result<ffi::nh_job_params, HipError> {
+    let sample: Vec<Lineage> = lineages.iter().step_by(8).cloned().collect();
+    let mut tuned = *params;
+    let mut best_ms = 0.0_f64;
+    check(unsafe {
+        ffi::nh_autotune(dev.ctx, dev.tables, params, as_nh_lineages(&sample), sample.len(), window, &mut tuned, &mut best_ms)
+    })?;
+    Ok(tuned)
+}
+
 #[allow(clippy::too_many_arguments)]
-fn simulate<'p, M: MathsCore, O: Scenario<M, WyHash<M>> + HipScenario<M>, R: Reporter, P: LocalPartition<'p, R>>(
+fn simulate<'p, M: MathsCore, R: Reporter, P: LocalPartition<'p, R>>(
     args: &HipArguments,
     rng: WyHash<M>,
-    scenario: &ScenarioCogs<M, WyHash<M>, O>,
+    cogs: &HipCogs,
+    mut dev: Device,
     lineages: &[Lineage],
     resume_after: Option<NonNegativeF64>,
     pause_before: Option<NonNegativeF64>,
     local_partition: &mut P,
 ) -> Result<SimulationOutcome<M, WyHash<M>>, HipError> {
-    let mut dev = Device { ctx: core::ptr::null_mut(), tables: core::ptr::null_mut(), job: core::ptr::null_mut() };
-    check(unsafe { ffi::nh_init(args.device, &mut dev.ctx) })?;
-    let (desc, _keep) = O::hip_scenario(scenario);
-    check(unsafe { ffi::nh_tables_create(dev.ctx, &desc, &mut dev.tables) })?;
     let mut params = args.job_params(
         wyhash_seed(&rng),
-        O::speciation_probability(scenario),
-        O::turnover_rate(scenario),
+        cogs.speciation_probability,
+        cogs.turnover_rate,
         resume_after,
         pause_before,
     );
@@ -139,11 +192,18 @@ fn simulate<'p, M: MathsCore, O: Scenario<M, WyHash<M>> + HipScenario<M>, R: Rep
     // stream (monolithic/mod.rs:139-164)
     let report_dispersal = R::ReportDispersal::VALUE;
     params.report_dispersal = i32::from(report_dispersal);
+    if args.form == HipForm::Persistent && args.autotune_window > 0.0 && !lineages.is_empty() {
+        params = autotuned(&dev, &params, lineages, args.autotune_window)?;
+    }
     check(unsafe { ffi::nh_job_create(dev.ctx, dev.tables, &params, &mut dev.job) })?;
-    // Lineage is #[repr(C)] with the nh_lineage layout (lineage.rs:97-109)
-    check(unsafe { ffi::nh_job_add_lineages(dev.job, lineages.as_ptr().cast(), lineages.len()) })?;
+    check(unsafe { ffi::nh_job_add_lineages(dev.job, as_nh_lineages(lineages), lineages.len()) })?;
+
+    // the progress bar starts at the lineages to simulate (monolithic/mod.rs:107)
+    local_partition.report_progress_sync(lineages.len() as u64);
+
     let mut buf: Vec<ffi::nh_speciation_event> = Vec::new();
     let mut packed: Vec<ffi::nh_packed_event> = Vec::new();
+    let mut o = ffi::nh_outcome::default();
     loop {
         let mut finished = 0_i32;
         check(unsafe { ffi::nh_job_step_round(dev.job, &mut finished) })?;
@@ -172,12 +232,15 @@ fn simulate<'p, M: MathsCore, O: Scenario<M, WyHash<M>> + HipScenario<M>, R: Rep
                 local_partition.get_reporter().report_speciation(event.into());
             }
         }
+        check(unsafe { ffi::nh_job_outcome(dev.job, &mut o) })?;
+        // the lineages left after the round (monolithic/mod.rs:244-250)
+        local_partition.get_reporter().report_progress(&o.n_remaining.into());
         if finished != 0 {
             break;
         }
     }
-    let mut o = ffi::nh_outcome::default();
-    check(unsafe { ffi::nh_job_outcome(dev.job, &mut o) })?;
+    local_partition.report_progress_sync(o.n_remaining);
+
     let time = NonNegativeF64::new(o.time).unwrap_or(NonNegativeF64::zero());
     if o.paused == 0 {
         return Ok(SimulationOutcome::Done { time, steps: o.steps });
@@ -192,13 +255,14 @@ fn simulate<'p, M: MathsCore, O: Scenario<M, WyHash<M>> + HipScenario<M>, R: Rep
     Ok(SimulationOutcome::Paused { time, steps: o.steps, lineages, rng, marker: PhantomData::<M> })
 }
 
-impl<'p, M: MathsCore, O: Scenario<M, WyHash<M>> + HipScenario<M>, R: Reporter, P: LocalPartition<'p, R>>
-    Algorithm<'p, M, WyHash<M>, O, R, P> for HipAlgorithm
+impl<'p, M: MathsCore, O: HipScenario<M>, R: Reporter, P: LocalPartition<'p, R>> Algorithm<'p, M, WyHash<M>, O, R, P>
+    for HipAlgorithm
 {
     type LineageStore = IndependentLineageStore<M, O::Habitat>;
 
-    fn get_logical_partition(_args: &Self::Arguments, local_partition: &P) -> Partition {
-        local_partition.get_partition()
+    fn get_logical_partition(_args: &Self::Arguments, _local_partition: &P) -> Partition {
+        // the monolithic water-level loop (the CPU algorithm's Monolithic mode)
+        Partition::monolithic()
     }
 
     fn initialise_and_simulate<I: Iterator<Item = u64>>(
@@ -209,23 +273,37 @@ impl<'p, M: MathsCore, O: Scenario<M, WyHash<M>> + HipScenario<M>, R: Reporter, 
         pause_before: Option<NonNegativeF64>,
         local_partition: &mut P,
     ) -> Result<SimulationOutcome<M, WyHash<M>>, Self::Error> {
+        let cogs = O::hip_cogs(&scenario)?;
         // the origin sampler stays on the CPU, in the reference's own order
-        let lineages: Vec<Lineage> = O::sample_lineages(&scenario, pre_sampler);
-        simulate(&args, rng, &scenario, &lineages, None, pause_before, local_partition)
+        // (independent/src/launch.rs:78-86)
+        let lineages: Vec<Lineage> =
+            O::sample_habitat(&scenario.habitat, pre_sampler, scenario.origin_sampler_auxiliary.clone()).collect();
+        let dev = Device::open(&args, &cogs)?;
+        simulate(&args, rng, &cogs, dev, &lineages, None, pause_before, local_partition)
     }
 
     fn resume_and_simulate<I: Iterator<Item = u64>, L: ExactSizeIterator<Item = Lineage>>(
         args: Self::Arguments,
         rng: WyHash<M>,
         scenario: ScenarioCogs<M, WyHash<M>, O>,
-        _pre_sampler: OriginPreSampler<M, I>,
+        pre_sampler: OriginPreSampler<M, I>,
         lineages: L,
         resume_after: Option<NonNegativeF64>,
         pause_before: Option<NonNegativeF64>,
         local_partition: &mut P,
     ) -> Result<SimulationOutcome<M, WyHash<M>>, ResumeError<Self::Error>> {
-        let lineages: Vec<Lineage> = lineages.collect();
-        Ok(simulate(&args, rng, &scenario, &lineages, resume_after, pause_before, local_partition)?)
+        let cogs = O::hip_cogs(&scenario)?;
+        // the pre-sampler picks this run's share of the resumed lineages
+        // (independent/src/initialiser/resume.rs:71-74, origin_sampler/resuming.rs:107-120)
+        let lineages: Vec<Lineage> = ResumingOriginSampler::new(&scenario.habitat, pre_sampler, lineages).collect();
+        let dev = Device::open(&args, &cogs)?;
+        // lineages out of their habitat or deme fail the resume
+        // (independent/src/initialiser/resume.rs:83-85)
+        let exceptional = dev.exceptional(&lineages)?;
+        if !exceptional.is_empty() {
+            return Err(ResumeError::Sample(exceptional));
+        }
+        Ok(simulate(&args, rng, &cogs, dev, &lineages, resume_after, pause_before, local_partition)?)
     }
 
     fn fixup_for_restart<I: Iterator<Item = u64>, L: ExactSizeIterator<Item = Lineage>>(
@@ -240,11 +318,9 @@ impl<'p, M: MathsCore, O: Scenario<M, WyHash<M>> + HipScenario<M>, R: Reporter, 
     ) -> Result<SimulationOutcome<M, WyHash<M>>, ResumeError<Self::Error>> {
         // the CPU Independent algorithm's FixUp path (const event-time sampler,
         // trespassing dispersal) is not part of the device library
-        Err(ResumeError::Simulate(HipError { code: -4, message: "fixup_for_restart is not offloaded".into() }))
+        Err(ResumeError::Simulate(HipError {
+            code: ffi::NH_E_STATE,
+            message: "fixup_for_restart is not offloaded to the device".into(),
+        }))
     }
-}
-
-#[allow(dead_code)]
-fn _reference_type_is_u64(r: GlobalLineageReference) -> u64 {
-    unsafe { r.into_inner() }
 }

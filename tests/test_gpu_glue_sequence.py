@@ -1,0 +1,143 @@
+"""The Rust glue's exact C-ABI call sequence, replayed through ctypes.
+
+necsim-rust_amd/rust/rustcoalescence-algorithms-hip/src/lib.rs drives the
+library as follows (the reference's Algorithm::initialise_and_simulate /
+resume_and_simulate, rustcoalescence/algorithms/src/lib.rs:55-112):
+
+  nh_init -> nh_tables_create -> [resume: nh_classify_lineages, and
+  ResumeError::Sample of the exceptional lineages if any] -> nh_job_create ->
+  nh_job_add_lineages -> report_progress_sync(n) -> per round:
+  nh_job_step_round, nh_job_drain_events (count, then fill),
+  nh_job_outcome -> report_progress(n_remaining) -> report_progress_sync ->
+  nh_job_outcome -> [paused: nh_job_take_lineages] -> destroy job, tables,
+  context.
+
+This test makes the same calls in the same order on config 1 and checks the
+events, outcome and remaining lineages against the oracle, the progress
+values (what the Progress and ExecutionTime plugins receive), and that a
+resume with exceptional lineages stops before any job exists.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import necsim_hip as nh
+import oracle as O
+from necsim_hip import _ffi
+from parity_utils import oracle_run
+
+pytestmark = pytest.mark.gpu
+
+L = _ffi.lib
+
+
+def announce(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def ok(rc):
+    _ffi.check(rc)
+
+
+def glue_simulate(scenario_desc, lineages, params, resume=False):
+    """lib.rs `Device::open` + `simulate` (+ the resume classification)"""
+    ctx, tables, job = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    progress, events = [], []
+    try:
+        ok(L().nh_init(0, ctypes.byref(ctx)))
+        ok(L().nh_tables_create(ctx, ctypes.byref(scenario_desc), ctypes.byref(tables)))
+        if resume:
+            kinds = np.zeros(len(lineages), np.int32)
+            n_bad = ctypes.c_size_t(0)
+            ok(L().nh_classify_lineages(tables, _ffi.ptr(lineages), len(lineages), _ffi.ptr(kinds),
+                                        ctypes.byref(n_bad)))
+            bad = [(int(k), lineages[i]) for i, k in enumerate(kinds) if k != 0]
+            assert len(bad) == n_bad.value
+            if bad:
+                return {"exceptional": bad, "job_created": bool(job.value)}
+        ok(L().nh_job_create(ctx, tables, ctypes.byref(params), ctypes.byref(job)))
+        ok(L().nh_job_add_lineages(job, _ffi.ptr(lineages), len(lineages)))
+        progress.append(("sync", len(lineages)))
+        o = _ffi.Outcome()
+        while True:
+            fin = ctypes.c_int32(0)
+            ok(L().nh_job_step_round(job, ctypes.byref(fin)))
+            n = ctypes.c_size_t(0)
+            ok(L().nh_job_drain_events(job, None, 0, ctypes.byref(n)))
+            buf = np.zeros(n.value, nh.EVENT_DTYPE)
+            ok(L().nh_job_drain_events(job, _ffi.ptr(buf), n.value, ctypes.byref(n)))
+            events.append(buf[:n.value])
+            ok(L().nh_job_outcome(job, ctypes.byref(o)))
+            progress.append(("report", int(o.n_remaining)))
+            if fin.value:
+                break
+        progress.append(("sync", int(o.n_remaining)))
+        rem = None
+        if o.paused:
+            rem = np.zeros(o.n_remaining, nh.LINEAGE_DTYPE)
+            k = ctypes.c_size_t(0)
+            ok(L().nh_job_take_lineages(job, _ffi.ptr(rem), len(rem), ctypes.byref(k)))
+            rem = rem[:k.value]
+        return {"events": np.concatenate(events), "steps": int(o.steps), "time": o.time,
+                "paused": bool(o.paused), "remaining": rem, "progress": progress, "rounds": int(o.rounds)}
+    finally:
+        if job.value:
+            L().nh_job_destroy(job)
+        if tables.value:
+            L().nh_tables_destroy(tables)
+        if ctx.value:
+            L().nh_ctx_destroy(ctx)
+
+
+def params_for(seed, nu, pause=None, resume_after=0.0, form=_ffi.FORM_PERSISTENT, dedup=_ffi.DEDUP_NONE):
+    """HipArguments::job_params with the reference defaults"""
+    p = _ffi.JobParams(rng_seed=nh.WyHash.seed_from_u64(seed).seed, speciation_probability=nu,
+                       turnover_rate=0.5, delta_t=2.0, step_slice=10, dedup_mode=dedup, dedup_factor=1.0,
+                       event_slice_mode=_ffi.SLICE_RELATIVE, event_slice_factor=0.2, form=form,
+                       resume_after=resume_after)
+    if pause is not None:
+        p.has_pause_before, p.pause_before = 1, pause
+    return p
+
+
+@pytest.fixture(scope="module")
+def config1():
+    cogs = nh.NonSpatialScenario((100, 100), 1).new(1e-3)
+    orc = O.Scenario.non_spatial(100, 100, 1)
+    return cogs, orc, orc.origins(1.0)
+
+
+@pytest.mark.parametrize("form", [_ffi.FORM_PLAIN, _ffi.FORM_PERSISTENT])
+def test_glue_sequence_initialise_pause_resume(config1, form):
+    announce(f"glue call sequence: initialise to 200, resume to the end, form {form}")
+    cogs, orc, lin = config1
+    ref = oracle_run(orc, lin, 42, 1e-3, dedup="none", event_slice=("relative", 0.2))
+    first = glue_simulate(cogs.desc, lin, params_for(42, 1e-3, pause=200.0, form=form))
+    assert first["paused"] and first["remaining"] is not None
+    kinds = [k for k, _ in first["progress"]]
+    assert kinds[0] == "sync" and kinds[-1] == "sync" and set(kinds[1:-1]) == {"report"}
+    left = [v for _, v in first["progress"]]
+    assert left[0] == len(lin) and left[-1] == len(first["remaining"])
+    assert all(a >= b for a, b in zip(left[1:-1], left[2:-1]))  # lineages only ever leave
+    assert len(left) - 2 == first["rounds"] > 1
+    second = glue_simulate(cogs.desc, first["remaining"], params_for(42, 1e-3, resume_after=200.0, form=form),
+                           resume=True)
+    assert not second["paused"] and second["progress"][-1] == ("sync", 0)
+    ev = np.concatenate([first["events"], second["events"]])
+    assert ev.tobytes() == ref["events"].tobytes()
+    assert first["steps"] + second["steps"] == ref["steps"]
+    assert second["time"] == ref["time"]
+
+
+def test_glue_sequence_resume_rejects_exceptional_lineages(config1):
+    announce("glue call sequence: resume with lineages out of deme / habitat")
+    cogs, orc, lin = config1
+    bad = lin[:6].copy()
+    bad["index"][1] = 1       # deme 1: index 1 is out of the deme
+    bad["x"][4] = 100         # outside the 100x100 extent
+    out = glue_simulate(cogs.desc, bad, params_for(42, 1e-3), resume=True)
+    assert not out["job_created"]
+    assert [k for k, _ in out["exceptional"]] == [1, 2]  # OutOfDeme, OutOfHabitat, input order
+    assert out["exceptional"][0][1].tobytes() == bad[1].tobytes()
+    assert out["exceptional"][1][1].tobytes() == bad[4].tobytes()
