@@ -27,31 +27,42 @@ FLAGS = [
 EXTRA = {"nh_kernels.hip": ["-Xarch_device", "-mllvm=-misched=gcn-max-ilp"]}
 
 
-def build(verbose=False):
+# host-side AddressSanitizer + UndefinedBehaviorSanitizer build (device code
+# unchanged): each -fsanitize= applies to the host compilation only.  The
+# library is linked without the runtime, which the process preloads
+# (oracle/Makefile, target sanitize-check).
+SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=all"]
+
+
+def build(verbose=False, sanitize=False):
     objs = []
-    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    bdir = os.path.join(HERE, "_build", "asan") if sanitize else os.path.join(HERE, "_build")
+    out = os.path.join(bdir, "libnecsim_hip.so") if sanitize else OUT
+    os.makedirs(bdir, exist_ok=True)
     procs = []
     for src in SOURCES:
-        obj = os.path.join(HERE, "_build", src.rsplit(".", 1)[0] + ".o")
+        obj = os.path.join(bdir, src.rsplit(".", 1)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmd = [HIPCC] + FLAGS + EXTRA.get(src, []) + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+        extra = EXTRA.get(src, []) + (SANITIZE if sanitize else [])
+        cmd = [HIPCC] + FLAGS + extra + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
         objs.append(obj)
     for src, p in procs:
-        out = p.communicate()[0].decode()
-        out = "\n".join(l for l in out.splitlines() if "packed-fp32-ops" not in l)
+        log = p.communicate()[0].decode()
+        log = "\n".join(l for l in log.splitlines() if "packed-fp32-ops" not in l)
         if p.returncode != 0:
-            raise RuntimeError(f"hipcc failed on {src}:\n{out}")
-        if verbose and out.strip():
-            print(out)
-    tmp = OUT + ".tmp"
+            raise RuntimeError(f"hipcc failed on {src}:\n{log}")
+        if verbose and log.strip():
+            print(log)
+    tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, sanitize="--sanitize" in sys.argv))

@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libnecsim_oracle.so")
+# NECSIM_ORACLE_LIBRARY selects another build (the sanitizer build of the
+# Makefile's `sanitize` target)
+LIB_PATH = os.environ.get("NECSIM_ORACLE_LIBRARY", os.path.join(HERE, "_build", "libnecsim_oracle.so"))
 
 LINEAGE_DTYPE = np.dtype([("global_reference", "<u8"), ("last_event_time", "<f8"),
                           ("x", "<u4"), ("y", "<u4"), ("index", "<u4"), ("_pad", "<u4")])
