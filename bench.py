@@ -540,6 +540,8 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--inner-steps", type=int, default=0)
     ap.add_argument("--exec-fill", type=int, default=0)
+    ap.add_argument("--pool", type=int, default=0,
+                    help="pooled kernel with this many lineages per wave (128, 192; skips the autotuner)")
     ap.add_argument("--graph", action="store_true",
                     help="capture each water-level round (kernel + event sort) in a hipGraph; the "
                          "kernel timing then covers the whole graph")
@@ -597,11 +599,11 @@ def main():
         print(json.dumps({"part": "tail", "config4_dedup_window": dd, "config4_tail": tail}), flush=True)
         return
     form = nh.PERSISTENT if a.form == "persistent" else nh.PLAIN
-    fixed = a.block_size or a.blocks_per_cu or a.inner_steps or a.exec_fill
+    fixed = a.block_size or a.blocks_per_cu or a.inner_steps or a.exec_fill or a.pool
     args = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=form,
                                    use_graph=a.graph and form == nh.PERSISTENT,
                                    block_size=a.block_size, blocks_per_cu=a.blocks_per_cu,
-                                   inner_steps=a.inner_steps, exec_fill=a.exec_fill)
+                                   inner_steps=a.inner_steps, exec_fill=a.exec_fill, pool=a.pool)
     tuned = None
     if not (a.no_tune or fixed or form == nh.PLAIN):
         t = time.perf_counter()
@@ -609,10 +611,10 @@ def main():
                                  tune_step_slice=False)
         args = nh.IndependentArguments(dedup_cache=args.dedup_cache, form=nh.PERSISTENT, use_graph=args.use_graph,
                                        block_size=targs.block_size, blocks_per_cu=targs.blocks_per_cu,
-                                       inner_steps=targs.inner_steps, exec_fill=targs.exec_fill)
+                                       inner_steps=targs.inner_steps, exec_fill=targs.exec_fill, pool=targs.pool)
         tuned = {"block_size": targs.block_size, "blocks_per_cu": targs.blocks_per_cu,
-                 "inner_steps": targs.inner_steps, "exec_fill": targs.exec_fill, "trial_ms": tms,
-                 "tune_s": time.perf_counter() - t}
+                 "inner_steps": targs.inner_steps, "exec_fill": targs.exec_fill, "pool": targs.pool,
+                 "trial_ms": tms, "tune_s": time.perf_counter() - t}
         log(f"autotuned: {tuned}")
     t = time.perf_counter()
     make_job(ctx, tables, cogs, lineages, args, 42, a.window).close()
@@ -648,10 +650,12 @@ def main():
     mean_launch_ms = kern_ms / nl
     achieved_gbs = per_launch_bytes / (mean_launch_ms * 1e-3) / 1e9
     prof = read_profile_summary(PROFILE_SUMMARY)
-    shape = (args.block_size, args.blocks_per_cu, args.inner_steps, args.exec_fill)
-    prof_shape = tuple(int(prof.get(k, -1)) for k in ("block_size", "blocks_per_cu", "inner_steps", "exec_fill"))
+    shape = (args.block_size, args.blocks_per_cu, args.inner_steps, args.exec_fill, args.pool)
+    prof_shape = tuple(int(prof.get(k, 0 if k == "pool" else -1))
+                       for k in ("block_size", "blocks_per_cu", "inner_steps", "exec_fill", "pool"))
     same_workload = prof.get("window") == a.window and prof.get("form") == a.form
-    kname = "k_persistent<almost-infinite, no dedup>" if form == nh.PERSISTENT else "k_plain<almost-infinite>"
+    kname = ("k_plain<almost-infinite>" if form != nh.PERSISTENT else
+             f"k_pooled<almost-infinite, {args.pool}>" if args.pool else "k_persistent<almost-infinite, no dedup>")
     roofline = {"bound": "valu", "achieved": None, "peak": cus * ENGINE_CLOCK_HZ, "unit": "wave-instructions/s",
                 "frac": None, "traffic": None, "kernel": kname}
     ipl = prof.get("valu_wave_instructions_per_lineage_step") if same_workload else None
@@ -662,7 +666,7 @@ def main():
             # the instruction count was profiled at another launch shape: time
             # that shape here, so that count and time belong to the same launch
             vargs = dataclasses.replace(args, block_size=prof_shape[0], blocks_per_cu=prof_shape[1],
-                                        inner_steps=prof_shape[2], exec_fill=prof_shape[3])
+                                        inner_steps=prof_shape[2], exec_fill=prof_shape[3], pool=prof_shape[4])
             _, vres = timed_runs(ctx, tables, cogs, lineages, vargs, 42, a.window, 1, 0)
             v_steps, v_ms, v_launches = int(vres[0][0].steps), vres[0][1], vres[0][2]
         inst_rate = ipl * v_steps / (v_ms * 1e-3)
@@ -672,7 +676,8 @@ def main():
                                                      * (prof.get("valu_lane_utilization") or 0.0), 4),
                          "wave_instructions_per_lineage_step": ipl,
                          "profiled_launch_shape": {"block_size": prof_shape[0], "blocks_per_cu": prof_shape[1],
-                                                   "inner_steps": prof_shape[2], "exec_fill": prof_shape[3]},
+                                                   "inner_steps": prof_shape[2], "exec_fill": prof_shape[3],
+                                                   "pool": prof_shape[4]},
                          "mean_launch_ms_at_profiled_shape": round(v_ms / max(v_launches, 1), 3),
                          "source": os.path.relpath(PROFILE_SUMMARY, ROOT)})
     roofline["traffic"] = prof.get("hbm_bytes_per_launch") if same_workload and prof_shape == shape else None
@@ -688,7 +693,7 @@ def main():
 
     extra = {"autotuned": tuned,
              "launch": {"form": a.form, "block_size": args.block_size, "blocks_per_cu": args.blocks_per_cu,
-                        "inner_steps": args.inner_steps, "exec_fill": args.exec_fill,
+                        "inner_steps": args.inner_steps, "exec_fill": args.exec_fill, "pool": args.pool,
                         "use_graph": bool(args.use_graph)},
              "copy_kernel": copy,
              "upload_s": upload_s,
@@ -723,7 +728,10 @@ def main():
             extra["config4_tail"] = config4_tail_run(ctx, tables, cogs, lineages, dtuned,
                                                      extra["config4_dedup_window"], a.tail_window,
                                                      completion=not a.no_completion)
-        extra["dispersal_reporting"] = dispersal_run(ctx, tables, cogs, lineages, args, a.dispersal_window,
+        # dispersal reporting runs the one-lineage-per-lane kernel: the
+        # pooled kernel's shape does not carry over
+        dargs = args if not args.pool else dataclasses.replace(args, pool=0, block_size=0, blocks_per_cu=0)
+        extra["dispersal_reporting"] = dispersal_run(ctx, tables, cogs, lineages, dargs, a.dispersal_window,
                                                      a.drain_window)
         del tables
         extra["config2_full_run"] = config2_run(ctx)

@@ -229,12 +229,15 @@ __device__ __forceinline__ void lane_set_location(Lane<KIND> &L, const DevScenar
 //
 // poisson_count: the number of events of the window whose primed RNG is
 // `state`; leaves `state` before the first event's time draw.
-__device__ __forceinline__ uint32_t poisson_count(const DevParams &P, const double *ratio, uint64_t &state) {
+// `tab` points at the table's acc / prod / acc_m part (ratio + RATIO_N unless
+// a kernel keeps only that part in LDS, with `ratio` in global memory).
+__device__ __forceinline__ uint32_t poisson_count(const DevParams &P, const double *ratio, uint64_t &state,
+                                                  const double *tab = nullptr) {
     uint32_t number;
     if (P.p0 > 0.0) {
         // while u > acc_k && prod_k > 0: k += 1 (acc is non-decreasing, so
         // the loop ends at min(first k with u <= acc_k, first k with prod_k == 0))
-        const double *acc_t = ratio + RATIO_N, *prod_t = acc_t + PTBL, *acc_m = prod_t + PTBL;
+        const double *acc_t = tab ? tab : ratio + RATIO_N, *prod_t = acc_t + PTBL, *acc_m = prod_t + PTBL;
         // the draw U = mu * 2^-53 (uniform_closed_open), compared as an integer
         const uint64_t mu = sample_u64(state) >> 11;
         // m = #{k < PTBL : U > acc_k}, by binary search over the sorted table
@@ -343,7 +346,8 @@ __device__ __forceinline__ void lane_peek(Lane<KIND> &L, const DevParams &P, con
 constexpr uint32_t STEP_DRAWS = 2;
 constexpr uint32_t STEP_OPENS = 2;
 template <int KIND>
-__device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams &P, const double *ratio) {
+__device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams &P, const double *ratio,
+                                                 const double *tab = nullptr) {
     if (L.have_next) return;
     // NonNegativeF64::max of last_event_time and min_event_time (sampler.rs:87),
     // recomputed per call instead of held in a register
@@ -357,7 +361,7 @@ __device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams 
 #pragma unroll
     for (uint32_t o = 0; o < STEP_OPENS && L.win_left == 0; o++) {
         uint64_t st = prime_keyed(L.lkey, L.search_ts);
-        L.win_left = poisson_count(P, ratio, st);
+        L.win_left = poisson_count(P, ratio, st, tab);
         L.next_st = st;
         L.win_e = 0;
         L.win_bi = ~0u;

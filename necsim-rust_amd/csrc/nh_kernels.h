@@ -72,6 +72,7 @@ struct PersistArgs {
     DevPacked *packed;             // dispersal reporting: the round's PackedEvents, else null
     unsigned long long packed_cap;
     unsigned coop;                 // lanes that are not searching search ahead for those that are
+    unsigned pool;                 // the pooled kernel's lineages per wave (0: one lineage per lane)
 };
 
 struct PlainArgs {
@@ -138,6 +139,8 @@ int serve_max_blocks_per_cu(int kind, int threads);
 // kernel launchers (all asynchronous on `stream`)
 void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream);
 int persistent_max_blocks_per_cu(int kind, int threads, bool dedup, bool report_dispersal, bool coop);
+// resident one-wave blocks per CU of the pooled kernel with `pool` lineages per wave
+int pooled_max_blocks_per_cu(int kind, unsigned pool);
 void launch_plain(int kind, const PlainArgs &a, int threads, hipStream_t stream);
 
 // m: pass size, or with m_dev the launch bound of a pass whose size is *m_dev

@@ -501,6 +501,206 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
 }
 
 // ---------------------------------------------------------------------------
+// Pooled persistent round kernel (dense rounds: no dedup cache, speciation
+// events only).  Each wave (one per block) keeps POOL lineages in LDS,
+// POOL / 64 per lane, and works in two phases per iteration:
+//   search:  up to 64 of the lineages still looking for their next event
+//            each advance by one lane_search_step, one lineage per lane;
+//   execute: once 64 lineages hold their next event (or, when the queue is
+//            empty, exec_fill eighths of those left), 64 of them execute it,
+//            one per lane.
+// Lanes pick lineages by rank in LDS, so both phases run with full waves
+// however unevenly the searches progress; in the one-lineage-per-lane
+// kernel a lane whose lineage is ready waits for the wave's execution and a
+// lane whose lineage still searches idles through it.  Each lineage makes
+// exactly the computations of k_persistent, so the event stream is the same.
+// Entries are owned by lanes for bookkeeping (entry lane + 64 j), refilled
+// from the same guided work queue.
+constexpr int POOL_MAX = 192;
+enum : uint8_t { PE_EMPTY = 0, PE_SEARCH = 1, PE_READY = 2 };
+constexpr uint32_t PE_NOT_STARTED = 0xffffffffu; // win_left of a search that has not opened a window
+
+template <int POOL> struct PoolLds {
+    uint64_t lkey[POOL];
+    uint64_t st[POOL];    // searching: the open window's RNG state; ready: the event's prime key
+    uint64_t ts[POOL];    // searching: the window
+    double t_last[POOL];
+    double next_t[POOL];  // searching: the earliest valid time of the open window so far; ready: the event
+    uint32_t a[POOL], b[POOL], i[POOL];
+    uint32_t slot[POOL];
+    uint32_t win_left[POOL], win_e[POOL], win_bi[POOL];
+    uint8_t state[POOL];
+    uint8_t assign[64];
+    double tab[3 * PTBL]; // the Poisson table's acc / prod / acc_m
+};
+
+// Lane r (r < the number of selected entries, at most 64) gets the r-th
+// selected entry, counting from word `first` round the words of `sel`.
+template <int NW>
+__device__ __forceinline__ uint32_t pool_pick(const uint64_t (&sel)[NW], int first, uint8_t *assign) {
+    const unsigned lane = lane_id();
+    uint32_t base = 0;
+#pragma unroll
+    for (int k = 0; k < NW; k++) {
+        const int j = (first + k) % NW;
+        if ((sel[j] >> lane) & 1ull) {
+            const uint32_t r = base + (uint32_t)__popcll(sel[j] & lanemask_lt());
+            if (r < 64) assign[r] = (uint8_t)(lane + 64 * j);
+        }
+        base += (uint32_t)__popcll(sel[j]);
+    }
+    wave_sync_lds();
+    return base < 64 ? base : 64;
+}
+
+template <int KIND, int POOL>
+__global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
+    static_assert(POOL % 64 == 0 && POOL <= POOL_MAX, "pool of whole words");
+    constexpr int NW = POOL / 64;
+    __shared__ PoolLds<POOL> p;
+    const unsigned lane = lane_id();
+    if (lane < 3 * PTBL) p.tab[lane] = A.P.ratio[RATIO_N + lane];
+#pragma unroll
+    for (int j = 0; j < NW; j++) p.state[lane + 64 * j] = PE_EMPTY;
+    wave_sync_lds();
+
+    const double level = A.round->level;
+    const unsigned long long n_in = A.round->n_in;
+    WaveQueue q{0, 0, 0, false, false};
+    unsigned long long my_steps = 0, my_dead = 0;
+    double my_max = 0.0;
+    uint32_t iter = 0;
+
+    for (;; iter++) {
+        // refill the lane's empty entries from the work queue, one per lane
+        // and call (a dead slot leaves its entry empty)
+        uint8_t own[NW];
+#pragma unroll
+        for (int j = 0; j < NW; j++) {
+            const int e = lane + 64 * j;
+            own[j] = p.state[e];
+            if (q.exhausted) continue;
+            const bool want = own[j] == PE_EMPTY;
+            const uint64_t need = __ballot(want);
+            if (!need) continue;
+            const unsigned long long w = wave_claim(q, need, want, &A.ctr->head, n_in, A.round->chunk);
+            if (w == ~0ull) continue;
+            Lane<KIND> L;
+            const uint8_t f = lane_load<KIND>(L, A.S, A.lin, w, A.P.seed);
+            if (f & FLAG_DEAD) continue;
+            p.lkey[e] = L.lkey;
+            p.t_last[e] = L.t_last;
+            p.a[e] = L.a;
+            p.b[e] = L.b;
+            p.i[e] = L.i;
+            p.slot[e] = (uint32_t)w;
+            p.next_t[e] = L.next_t;
+            p.st[e] = L.next_st;
+            p.win_left[e] = PE_NOT_STARTED;
+            own[j] = L.have_next ? PE_READY : PE_SEARCH;
+            p.state[e] = own[j];
+        }
+        uint64_t srch[NW], rdy[NW];
+        uint32_t n_occ = 0, n_rdy = 0;
+#pragma unroll
+        for (int j = 0; j < NW; j++) {
+            srch[j] = __ballot(own[j] == PE_SEARCH);
+            rdy[j] = __ballot(own[j] == PE_READY);
+            n_occ += (uint32_t)__popcll(srch[j] | rdy[j]);
+            n_rdy += (uint32_t)__popcll(rdy[j]);
+        }
+        if (n_occ == 0) {
+            if (q.exhausted) break;
+            continue;
+        }
+
+        // search phase: one step for up to 64 searching lineages
+        if (n_occ > n_rdy) {
+            const uint32_t n = pool_pick<NW>(srch, (int)(iter % NW), p.assign);
+            if (lane < n) {
+                const int e = p.assign[lane];
+                Lane<KIND> L;
+                L.lkey = p.lkey[e];
+                L.t_last = p.t_last[e];
+                const uint32_t wl = p.win_left[e];
+                L.searching = wl != PE_NOT_STARTED;
+                L.win_left = L.searching ? wl : 0u;
+                L.win_e = p.win_e[e];
+                L.win_bi = p.win_bi[e];
+                L.search_ts = p.ts[e];
+                L.next_st = p.st[e];
+                L.next_t = p.next_t[e];
+                L.have_next = false;
+                lane_search_step<KIND>(L, A.P, A.P.ratio, p.tab);
+                p.next_t[e] = L.next_t;
+                p.st[e] = L.next_st;
+                if (L.have_next) {
+                    p.state[e] = PE_READY;
+                } else {
+                    p.ts[e] = L.search_ts;
+                    p.win_left[e] = L.win_left;
+                    p.win_e[e] = L.win_e;
+                    p.win_bi[e] = L.win_bi;
+                }
+            }
+            wave_sync_lds();
+            n_rdy = 0;
+#pragma unroll
+            for (int j = 0; j < NW; j++) {
+                rdy[j] = __ballot(p.state[lane + 64 * j] == PE_READY);
+                n_rdy += (uint32_t)__popcll(rdy[j]);
+            }
+        }
+
+        // execute phase: full waves, or the rest once the queue is empty
+        const bool go = n_rdy >= 64 ||
+                        (n_rdy > 0 && (n_rdy == n_occ || (q.exhausted && n_rdy * 8u >= n_occ * A.exec_fill)));
+        if (!go) continue;
+        const uint32_t n = pool_pick<NW>(rdy, (int)(iter % NW), p.assign);
+        if (lane < n) {
+            const int e = p.assign[lane];
+            const uint32_t slot = p.slot[e];
+            Lane<KIND> L;
+            L.lkey = p.lkey[e];
+            L.t_last = p.t_last[e];
+            L.a = p.a[e];
+            L.b = p.b[e];
+            L.i = p.i[e];
+            L.next_t = p.next_t[e];
+            L.next_st = p.st[e];
+            L.have_next = true;
+            if (is_cell_kind(KIND) && KIND != K_NON_SPATIAL) L.rec = A.S.cells[L.a];
+            if (L.next_t >= level) { // the water level: back to the lineage store
+                lane_store<KIND>(L, A.S, A.lin, slot);
+                p.state[e] = PE_EMPTY;
+            } else {
+                DevEvent ev;
+                const int r = lane_execute<KIND, false>(L, A.S, A.P, ev);
+                my_steps++;
+                my_max = L.t_last > my_max ? L.t_last : my_max;
+                if (r == STEP_SPECIATED) {
+                    ev.ref = A.lin.ref[slot];
+                    A.events[atomicAdd(&A.ctr->n_events, 1ull)] = ev;
+                    A.lin.flags[slot] = FLAG_DEAD;
+                    my_dead++;
+                    p.state[e] = PE_EMPTY;
+                } else {
+                    p.lkey[e] = L.lkey;
+                    p.t_last[e] = L.t_last;
+                    p.a[e] = L.a;
+                    p.b[e] = L.b;
+                    p.i[e] = L.i;
+                    p.win_left[e] = PE_NOT_STARTED;
+                    p.state[e] = PE_SEARCH;
+                }
+            }
+        }
+        wave_sync_lds();
+    }
+    block_flush(my_steps, my_max, my_dead, A.ctr);
+}
+
+// ---------------------------------------------------------------------------
 // Serving kernel: the persistent round kernel over the lineages of many
 // requests at once.  Global work index w belongs to the request whose
 // [work_base, work_base + n_in) contains it; each lane keeps its request's
@@ -1472,8 +1672,38 @@ static void launch_persistent_t(int kind, const PersistArgs &a, int blocks, int 
     }
 }
 
-// dispersal reporting has no cooperative variant (its rounds are dense)
+template <int POOL> static void launch_pooled_t(int kind, const PersistArgs &a, int blocks, hipStream_t stream) {
+    switch (kind) {
+    case K_NON_SPATIAL: hipLaunchKernelGGL((k_pooled<K_NON_SPATIAL, POOL>), dim3(blocks), dim3(64), 0, stream, a); break;
+    case K_ALMOST_INFINITE: hipLaunchKernelGGL((k_pooled<K_ALMOST_INFINITE, POOL>), dim3(blocks), dim3(64), 0, stream, a); break;
+    case K_SEPARABLE: hipLaunchKernelGGL((k_pooled<K_SEPARABLE, POOL>), dim3(blocks), dim3(64), 0, stream, a); break;
+    default: hipLaunchKernelGGL((k_pooled<K_PACKED, POOL>), dim3(blocks), dim3(64), 0, stream, a); break;
+    }
+}
+
+template <int POOL> static int pooled_max_blocks_per_cu_t(int kind) {
+    int n = 0;
+    switch (kind) {
+    case K_NON_SPATIAL: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pooled<K_NON_SPATIAL, POOL>, 64, 0); break;
+    case K_ALMOST_INFINITE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pooled<K_ALMOST_INFINITE, POOL>, 64, 0); break;
+    case K_SEPARABLE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pooled<K_SEPARABLE, POOL>, 64, 0); break;
+    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pooled<K_PACKED, POOL>, 64, 0); break;
+    }
+    return n > 0 ? n : 1;
+}
+
+int pooled_max_blocks_per_cu(int kind, unsigned pool) {
+    return pool == 192 ? pooled_max_blocks_per_cu_t<192>(kind) : pooled_max_blocks_per_cu_t<128>(kind);
+}
+
+// dispersal reporting has no cooperative variant (its rounds are dense); the
+// pooled kernel runs dense rounds without dedup cache (one-wave blocks)
 void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream) {
+    if (a.pool && !a.packed && !a.cache && !a.coop) {
+        if (a.pool == 192) launch_pooled_t<192>(kind, a, blocks, stream);
+        else launch_pooled_t<128>(kind, a, blocks, stream);
+        return;
+    }
     if (a.packed) {
         if (a.cache) launch_persistent_t<true, true, false>(kind, a, blocks, threads, stream);
         else launch_persistent_t<false, true, false>(kind, a, blocks, threads, stream);

@@ -480,6 +480,7 @@ struct nh_job {
     int threads = 256;
     int blocks = 0;
     bool coop = false; // cooperative next-event search
+    bool pooled = false; // the pooled kernel (nh_job_params.pool)
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr}; // persistent round, input buffer 0 / 1
@@ -532,6 +533,8 @@ extern "C" int nh_job_create(nh_ctx *ctx, const nh_tables *tables, const nh_job_
     if (!(p->resume_after >= 0.0)) return set_error(NH_E_INVALID, "resume_after must be non-negative");
     if (p->block_size && (p->block_size % 64 || p->block_size > 256))
         return set_error(NH_E_INVALID, "block size must be 64, 128, 192 or 256");
+    if (p->pool != 0 && p->pool != 128 && p->pool != 192)
+        return set_error(NH_E_INVALID, "pool is 0 (off), 128 or 192 lineages per wave");
     NH_HIP(hipSetDevice(ctx->device));
     nh_job *j = new nh_job();
     j->ctx = ctx;
@@ -651,6 +654,15 @@ static int job_start(nh_job *j) {
         else if (p.coop_search == 0) j->coop = !j->disp && (j->cache_cap != 0 || n < lanes);
         if (j->coop) {
             per_cu = persistent_max_blocks_per_cu(j->tables->kind, j->threads, j->cache_cap != 0, false, true);
+            if (p.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)p.blocks_per_cu);
+        }
+        // the pooled kernel: dense rounds without dedup cache or dispersal
+        // reporting, one wave per block
+        j->pooled = p.pool != 0 && j->cache_cap == 0 && !j->disp && p.coop_search != 2;
+        if (j->pooled) {
+            j->coop = false;
+            j->threads = 64;
+            per_cu = pooled_max_blocks_per_cu(j->tables->kind, p.pool);
             if (p.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)p.blocks_per_cu);
         }
         j->blocks = per_cu * (int)j->ctx->cus;
@@ -995,6 +1007,7 @@ static PersistArgs persist_args(nh_job *j, int cur) {
     a.inner_steps = j->p.inner_steps ? j->p.inner_steps : 16;
     a.coop = j->coop ? 1u : 0u;
     a.exec_fill = j->p.exec_fill ? j->p.exec_fill : 5;
+    a.pool = j->pooled ? j->p.pool : 0u;
     return a;
 }
 
@@ -1301,6 +1314,7 @@ extern "C" int nh_autotune(nh_ctx *ctx, const nh_tables *tables, const nh_job_pa
             for (uint64_t slice : slices) {
                 nh_job_params p = *base;
                 p.form = NH_FORM_PERSISTENT;
+                p.pool = 0;
                 p.block_size = threads;
                 p.blocks_per_cu = per_cu;
                 p.step_slice = slice;
@@ -1322,6 +1336,37 @@ extern "C" int nh_autotune(nh_ctx *ctx, const nh_tables *tables, const nh_job_pa
                     best = ms;
                     winner = p;
                 }
+            }
+        }
+    }
+    // the pooled kernel (dense rounds without dedup cache or dispersal
+    // reporting): one-wave blocks at full residency, each pool size
+    if (base->dedup_mode == NH_DEDUP_NONE && !base->report_dispersal && base->coop_search != 2) {
+        for (uint32_t pool : {128u, 192u})
+        for (uint64_t slice : slices) {
+            nh_job_params p = *base;
+            p.form = NH_FORM_PERSISTENT;
+            p.pool = pool;
+            p.block_size = 64;
+            p.blocks_per_cu = (uint32_t)pooled_max_blocks_per_cu(tables->kind, pool);
+            p.step_slice = slice;
+            p.inner_steps = 16;
+            p.exec_fill = 5;
+            p.has_pause_before = 1;
+            p.pause_before = window;
+            p.use_graph = 0;
+            nh_job *j = nullptr;
+            NH_TRY(nh_job_create(ctx, tables, &p, &j));
+            int rc = nh_job_add_lineages(j, sample, n);
+            if (rc == NH_OK) rc = nh_job_start(j);
+            double t0 = now_s();
+            if (rc == NH_OK) rc = nh_job_simulate(j);
+            double ms = (now_s() - t0) * 1e3;
+            nh_job_destroy(j);
+            if (rc != NH_OK) return rc;
+            if (ms < best) {
+                best = ms;
+                winner = p;
             }
         }
     }

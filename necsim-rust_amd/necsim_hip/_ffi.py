@@ -66,7 +66,7 @@ class JobParams(ctypes.Structure):
                 ("form", c_i32), ("block_size", c_u32), ("blocks_per_cu", c_u32),
                 ("use_graph", c_u32), ("inner_steps", c_u32), ("exec_fill", c_u32),
                 ("report_dispersal", c_i32), ("event_buffer", c_u64), ("device_event_counter", c_i32),
-                ("plain_batch", c_u32), ("coop_search", c_u32), ("task_list_size", c_u32)]
+                ("plain_batch", c_u32), ("coop_search", c_u32), ("task_list_size", c_u32), ("pool", c_u32)]
 
 
 class EventCounts(ctypes.Structure):

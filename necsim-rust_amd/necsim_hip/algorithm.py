@@ -97,6 +97,7 @@ class IndependentArguments:
     plain_batch: int = 0  # plain form: passes per host synchronisation (1: one per pass; 0: 16)
     coop_search: int = 0  # persistent form: cooperative next-event search (0: automatic, 1: off, 2: on)
     task_list_size: int = 0  # task-list form: lineages per launch (0: 4096)
+    pool: int = 0  # persistent form without dedup: lineages per wave in the pooled kernel's LDS pool (128, 192)
     device: int = 0
 
 
@@ -585,7 +586,7 @@ def _job_params(args, rng, speciation_probability, turnover_rate, pause_before=N
                           blocks_per_cu=args.blocks_per_cu, use_graph=1 if args.use_graph else 0,
                           inner_steps=args.inner_steps, exec_fill=args.exec_fill,
                           plain_batch=args.plain_batch, coop_search=args.coop_search,
-                          task_list_size=args.task_list_size)
+                          task_list_size=args.task_list_size, pool=args.pool)
 
 
 def autotune(ctx, tables, args: IndependentArguments, rng: WyHash, cogs, sample, window,
@@ -604,7 +605,7 @@ def autotune(ctx, tables, args: IndependentArguments, rng: WyHash, cogs, sample,
                                block_size=int(out.block_size),
                                blocks_per_cu=int(out.blocks_per_cu),
                                inner_steps=int(out.inner_steps),
-                               exec_fill=int(out.exec_fill)), ms.value
+                               exec_fill=int(out.exec_fill), pool=int(out.pool)), ms.value
 
 
 def copy_calibrate(ctx, nbytes, reps=10):

@@ -47,6 +47,9 @@ pub struct HipArguments {
     pub plain_batch: u32,
     /// task-list form: lineages per launch, 0 = 4096
     pub task_list_size: u32,
+    /// persistent form without dedup: lineages per wave in the pooled
+    /// kernel's LDS pool (128 or 192), 0 = one lineage per lane
+    pub pool: u32,
     /// persistent form: pick block size, residency, inner steps and fill with
     /// nh_autotune on every 8th lineage over this pause window (0 = off)
     pub autotune_window: f64,
@@ -73,6 +76,7 @@ impl Default for HipArguments {
             coop_search: 0,
             plain_batch: 0,
             task_list_size: 0,
+            pool: 0,
             autotune_window: 0.0,
         }
     }
@@ -127,6 +131,7 @@ impl HipArguments {
             coop_search: self.coop_search,
             plain_batch: self.plain_batch,
             task_list_size: self.task_list_size,
+            pool: self.pool,
             ..Default::default()
         };
         match self.dedup_cache {

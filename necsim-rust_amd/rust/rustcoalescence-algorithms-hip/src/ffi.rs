@@ -167,6 +167,7 @@ pub struct nh_job_params {
     pub plain_batch: u32,
     pub coop_search: u32,
     pub task_list_size: u32,
+    pub pool: u32,
 }
 
 #[repr(C)]

@@ -37,7 +37,7 @@ def oracle_run(orc_scn, lineages, seed, nu, dedup="none", pause_before=None, ste
 
 def hip_run(cogs, lineages, seed, form, dedup="none", pause_before=None, step_slice=10,
             delta_t=2.0, event_slice=None, resume_after=None, use_graph=False, plain_batch=0, coop_search=0,
-            task_list_size=0):
+            task_list_size=0, pool=0):
     if dedup == "none":
         dc = nh.DedupCache.none()
     elif isinstance(dedup, float):
@@ -47,7 +47,7 @@ def hip_run(cogs, lineages, seed, form, dedup="none", pause_before=None, step_sl
     es = nh.EventSlice.Relative(2.0) if event_slice is None else event_slice
     args = nh.IndependentArguments(delta_t=delta_t, step_slice=step_slice, dedup_cache=dc,
                                    event_slice=es, form=form, use_graph=use_graph, plain_batch=plain_batch,
-                                   coop_search=coop_search, task_list_size=task_list_size)
+                                   coop_search=coop_search, task_list_size=task_list_size, pool=pool)
     rep = nh.EventCollector()
     out = nh.HipIndependentAlgorithm.resume_and_simulate(
         args, nh.WyHash.seed_from_u64(seed), cogs, lineages, resume_after=resume_after,
