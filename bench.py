@@ -607,7 +607,9 @@ def main():
     tuned = None
     if not (a.no_tune or fixed or form == nh.PLAIN):
         t = time.perf_counter()
-        targs, tms = nh.autotune(ctx, tables, args, nh.WyHash.seed_from_u64(42), cogs, lineages[::8], 50.0,
+        # trials on every lineage over a short window: the pooled kernel's
+        # advantage depends on how many lineages each pool entry serves
+        targs, tms = nh.autotune(ctx, tables, args, nh.WyHash.seed_from_u64(42), cogs, lineages, 100.0,
                                  tune_step_slice=False)
         args = nh.IndependentArguments(dedup_cache=args.dedup_cache, form=nh.PERSISTENT, use_graph=args.use_graph,
                                        block_size=targs.block_size, blocks_per_cu=targs.blocks_per_cu,

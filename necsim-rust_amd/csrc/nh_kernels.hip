@@ -571,15 +571,19 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
     double my_max = 0.0;
     uint32_t iter = 0;
 
+    uint32_t n_empty = POOL;
     for (;; iter++) {
         // refill the lane's empty entries from the work queue, one per lane
-        // and call (a dead slot leaves its entry empty)
+        // and call (a dead slot leaves its entry empty).  Refills wait until
+        // a quarter of the pool is empty, the pool runs low or 16 iterations
+        // have passed: every claim is an atomic on the one queue head.
+        const bool refill = !q.exhausted && (n_empty * 4 >= POOL || n_empty + 64 > POOL || (iter & 15) == 0);
         uint8_t own[NW];
 #pragma unroll
         for (int j = 0; j < NW; j++) {
             const int e = lane + 64 * j;
             own[j] = p.state[e];
-            if (q.exhausted) continue;
+            if (!refill || q.exhausted) continue;
             const bool want = own[j] == PE_EMPTY;
             const uint64_t need = __ballot(want);
             if (!need) continue;
@@ -609,6 +613,7 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
             n_occ += (uint32_t)__popcll(srch[j] | rdy[j]);
             n_rdy += (uint32_t)__popcll(rdy[j]);
         }
+        n_empty = POOL - n_occ;
         if (n_occ == 0) {
             if (q.exhausted) break;
             continue;

@@ -1111,8 +1111,12 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
     j->level_time = j->level_time + (double)j->event_slice / total_rate;
     if (p.has_pause_before && p.pause_before < j->level_time) j->level_time = p.pause_before;
     uint64_t ev_start = j->disp ? 0 : j->n_events;
+    // the pooled kernel holds p.pool lineages per wave: claims are sized by
+    // pool entries, not lanes
+    const unsigned long long holders =
+        (unsigned long long)j->blocks * (j->pooled ? (unsigned long long)p.pool : (unsigned long long)j->threads);
     j->round_host = RoundDesc{j->level_time, (unsigned long long)j->cur_n, (unsigned long long)ev_start,
-                              claim_chunk(j->cur_n, (unsigned long long)j->blocks * j->threads)};
+                              claim_chunk(j->cur_n, holders)};
     NH_HIP(hipMemcpyAsync(j->d_round, &j->round_host, sizeof(RoundDesc), hipMemcpyHostToDevice, st));
     if (j->disp && p.form == NH_FORM_PLAIN) {
         NH_HIP(hipMemsetAsync(&j->d_ctr->n_events, 0, sizeof(unsigned long long), st));
