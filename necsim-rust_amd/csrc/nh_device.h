@@ -229,15 +229,16 @@ __device__ __forceinline__ void lane_set_location(Lane<KIND> &L, const DevScenar
 //
 // poisson_count: the number of events of the window whose primed RNG is
 // `state`; leaves `state` before the first event's time draw.
-// `tab` points at the table's acc / prod / acc_m part (ratio + RATIO_N unless
-// a kernel keeps only that part in LDS, with `ratio` in global memory).
+// `acc_t` points at the table's acc / prod / acc_m part: ratio + RATIO_N, or a
+// copy of that part in LDS (one pointer per call site, never a choice made at
+// run time, so that LDS reads compile to LDS instructions, not flat ones).
 __device__ __forceinline__ uint32_t poisson_count(const DevParams &P, const double *ratio, uint64_t &state,
-                                                  const double *tab = nullptr) {
+                                                  const double *acc_t) {
     uint32_t number;
     if (P.p0 > 0.0) {
         // while u > acc_k && prod_k > 0: k += 1 (acc is non-decreasing, so
         // the loop ends at min(first k with u <= acc_k, first k with prod_k == 0))
-        const double *acc_t = tab ? tab : ratio + RATIO_N, *prod_t = acc_t + PTBL, *acc_m = prod_t + PTBL;
+        const double *prod_t = acc_t + PTBL, *acc_m = prod_t + PTBL;
         // the draw U = mu * 2^-53 (uniform_closed_open), compared as an integer
         const uint64_t mu = sample_u64(state) >> 11;
         // m = #{k < PTBL : U > acc_k}, by binary search over the sorted table
@@ -266,6 +267,10 @@ __device__ __forceinline__ uint32_t poisson_count(const DevParams &P, const doub
         number = as_u32(a); // f64::max(a, 0.0) as u32: NaN and negatives give 0
     }
     return number;
+}
+
+__device__ __forceinline__ uint32_t poisson_count(const DevParams &P, const double *ratio, uint64_t &state) {
+    return poisson_count(P, ratio, state, ratio + RATIO_N);
 }
 
 // The window's `number` event times (ts + U[0,1)) * delta_t, drawn in order
@@ -345,9 +350,10 @@ __device__ __forceinline__ void lane_peek(Lane<KIND> &L, const DevParams &P, con
 // so L.next_t / L.next_st end up equal to lane_peek's.
 constexpr uint32_t STEP_DRAWS = 2;
 constexpr uint32_t STEP_OPENS = 2;
+// `acc_t` as for poisson_count.
 template <int KIND>
 __device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams &P, const double *ratio,
-                                                 const double *tab = nullptr) {
+                                                 const double *acc_t) {
     if (L.have_next) return;
     // NonNegativeF64::max of last_event_time and min_event_time (sampler.rs:87),
     // recomputed per call instead of held in a register
@@ -361,7 +367,7 @@ __device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams 
 #pragma unroll
     for (uint32_t o = 0; o < STEP_OPENS && L.win_left == 0; o++) {
         uint64_t st = prime_keyed(L.lkey, L.search_ts);
-        L.win_left = poisson_count(P, ratio, st, tab);
+        L.win_left = poisson_count(P, ratio, st, acc_t);
         L.next_st = st;
         L.win_e = 0;
         L.win_bi = ~0u;
@@ -389,6 +395,11 @@ __device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams 
     } else {
         L.search_ts++;
     }
+}
+
+template <int KIND>
+__device__ __forceinline__ void lane_search_step(Lane<KIND> &L, const DevParams &P, const double *ratio) {
+    lane_search_step(L, P, ratio, ratio + RATIO_N);
 }
 
 enum : int { STEP_DISPERSED = 0, STEP_SPECIATED = 1 };

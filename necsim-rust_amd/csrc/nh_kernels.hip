@@ -680,7 +680,9 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
                 p.state[e] = PE_EMPTY;
             } else {
                 DevEvent ev;
-                const int r = lane_execute<KIND, false>(L, A.S, A.P, ev);
+                // straight-line sin / cos: a full wave of Box-Muller angles
+                // takes every branch of the case-by-case reduction
+                const int r = lane_execute<KIND, false, true>(L, A.S, A.P, ev);
                 my_steps++;
                 my_max = L.t_last > my_max ? L.t_last : my_max;
                 if (r == STEP_SPECIATED) {

@@ -295,8 +295,10 @@ NH_HD void sincos(double x, double &s, double &c) {
 // (near multiples of pi/2, |x| > 9 pi / 4) is a branch.  The bits equal
 // sincos's.  A lone lane (the serial tail of a coalescing simulation) then
 // runs one basic block that the compiler interleaves with the logarithm of
-// the same Box-Muller draw; a full wave of angles runs the case-by-case code
-// about as fast.
+// the same Box-Muller draw.  A full wave of angles takes every branch of the
+// case-by-case code, so the pooled round kernel (whose execution passes run
+// full waves) uses this form too: 5 % less time per launch on the
+// almost-infinite scenario.
 NH_HD void sincos_flat(double x, double &s, double &c) {
     const double pio2_1 = 1.57079632673412561417e+00;
     const double pio2_1t = 6.07710050650619224932e-11;

@@ -91,10 +91,12 @@ def glue_simulate(scenario_desc, lineages, params, resume=False):
 
 
 def params_for(seed, nu, pause=None, resume_after=0.0, form=_ffi.FORM_PERSISTENT, dedup=_ffi.DEDUP_NONE):
-    """HipArguments::job_params with the reference defaults"""
+    """HipArguments::job_params with the reference defaults, but an event
+    slice of 0.02 lineages (200 events on config 1) so that a pause at 200
+    takes several water-level rounds"""
     p = _ffi.JobParams(rng_seed=nh.WyHash.seed_from_u64(seed).seed, speciation_probability=nu,
                        turnover_rate=0.5, delta_t=2.0, step_slice=10, dedup_mode=dedup, dedup_factor=1.0,
-                       event_slice_mode=_ffi.SLICE_RELATIVE, event_slice_factor=0.2, form=form,
+                       event_slice_mode=_ffi.SLICE_RELATIVE, event_slice_factor=0.02, form=form,
                        resume_after=resume_after)
     if pause is not None:
         p.has_pause_before, p.pause_before = 1, pause
@@ -112,7 +114,7 @@ def config1():
 def test_glue_sequence_initialise_pause_resume(config1, form):
     announce(f"glue call sequence: initialise to 200, resume to the end, form {form}")
     cogs, orc, lin = config1
-    ref = oracle_run(orc, lin, 42, 1e-3, dedup="none", event_slice=("relative", 0.2))
+    ref = oracle_run(orc, lin, 42, 1e-3, dedup="none", event_slice=("relative", 0.02))
     first = glue_simulate(cogs.desc, lin, params_for(42, 1e-3, pause=200.0, form=form))
     assert first["paused"] and first["remaining"] is not None
     kinds = [k for k, _ in first["progress"]]
