@@ -40,7 +40,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # instruction per CU per cycle at the 2.4 GHz peak engine clock (the same
 # rate as the 78.6 TFLOP/s f64 vector FMA peak: 64 lanes x 2 flops per CU-cycle)
 ENGINE_CLOCK_HZ = 2.4e9
-PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "k_persistent_config4_window.txt")
+# committed rocprofv3 summaries of the headline window: the pooled kernel and
+# the one-lineage-per-lane kernel (the roofline uses the one of the kernel run)
+PROFILE_SUMMARIES = {"pooled": os.path.join(ROOT, "profiles", "k_pooled_config4_window.txt"),
+                     "lane": os.path.join(ROOT, "profiles", "k_persistent_config4_window.txt")}
 GATHER_PROFILE = os.path.join(ROOT, "profiles", "k_persistent_alias_gather_map256.txt")
 # bytes of lineage state one persistent round reads and writes per lineage
 # (SoA: ref 8, t_last 8, x/y/index 12, next_t 8, next_state 8, flags 1)
@@ -273,18 +276,22 @@ def dispersal_run(ctx, tables, cogs, lineages, args, window, drain_window, seed=
         job.add_lineages(lineages)
         job.start()
         t0 = time.perf_counter()
-        fin, drained = False, 0
+        fin, drained, species = False, 0, set()
         while not fin:
             fin = job.step_round()
             if name == "host_drain":
                 drained += len(job.drain_packed_events())
             elif name == "speciation_only":
-                job.drain_events()
+                species |= species_of(job.drain_events())
         wall = time.perf_counter() - t0
         o = job.outcome()
         ms, k = job.kernel_time()
         r = {"steps": int(o.steps), "events": int(o.n_events), "rounds": int(o.rounds), "wall_ms": wall * 1e3,
              "device_ms": ms, "steps_per_s": o.steps / wall, "device_steps_per_s": o.steps / (ms * 1e-3)}
+        if name == "speciation_only":
+            # coalesced lineages (dedup None) repeat their speciations; the
+            # counter counts each distinct one once (event_counter.rs:57-72)
+            r["distinct_speciations"] = len(species)
         if name == "device_counter":
             r["counts"] = job.event_counts()
             r["counts_raw_total_equals_steps"] = r["counts"]["raw_total"] == o.steps
@@ -295,7 +302,7 @@ def dispersal_run(ctx, tables, cogs, lineages, args, window, drain_window, seed=
         out[name] = r
         log(f"dispersal {name}: {r}")
     sp, dc = out["speciation_only"], out["device_counter"]
-    out["device_counter_species_equal_speciation_only"] = dc["counts"]["speciation"] == sp["events"]
+    out["device_counter_species_equal_speciation_only"] = dc["counts"]["speciation"] == sp["distinct_speciations"]
     out["device_counter_over_speciation_only"] = dc["steps_per_s"] / sp["steps_per_s"]
     return out
 
@@ -651,7 +658,8 @@ def main():
         per_launch_bytes = n_in * (LINEAGE_STATE_BYTES + MOVING_STATE_BYTES)
     mean_launch_ms = kern_ms / nl
     achieved_gbs = per_launch_bytes / (mean_launch_ms * 1e-3) / 1e9
-    prof = read_profile_summary(PROFILE_SUMMARY)
+    profile_path = PROFILE_SUMMARIES["pooled" if args.pool else "lane"]
+    prof = read_profile_summary(profile_path)
     shape = (args.block_size, args.blocks_per_cu, args.inner_steps, args.exec_fill, args.pool)
     prof_shape = tuple(int(prof.get(k, 0 if k == "pool" else -1))
                        for k in ("block_size", "blocks_per_cu", "inner_steps", "exec_fill", "pool"))
@@ -681,7 +689,7 @@ def main():
                                                    "inner_steps": prof_shape[2], "exec_fill": prof_shape[3],
                                                    "pool": prof_shape[4]},
                          "mean_launch_ms_at_profiled_shape": round(v_ms / max(v_launches, 1), 3),
-                         "source": os.path.relpath(PROFILE_SUMMARY, ROOT)})
+                         "source": os.path.relpath(profile_path, ROOT)})
     roofline["traffic"] = prof.get("hbm_bytes_per_launch") if same_workload and prof_shape == shape else None
     roofline.update({"mean_launch_ms": round(mean_launch_ms, 3), "launches": nl,
                      "lineage_steps_per_launch": int(steps_total / nl),
