@@ -628,6 +628,24 @@ def main():
     t = time.perf_counter()
     make_job(ctx, tables, cogs, lineages, args, 42, a.window).close()
     upload_s = time.perf_counter() - t  # host -> HBM upload of the lineages (+ allocation)
+    # origin sampling: the host walk (then the upload above) against the
+    # device sampler writing straight into a job's lineage store
+    t = time.perf_counter()
+    host_lin = cogs.sample_habitat(nh.OriginPreSampler.all())
+    host_walk_s = time.perf_counter() - t
+    oj = nh.Job(ctx, tables, args, nh.WyHash.seed_from_u64(42), cogs.speciation_probability, cogs.turnover_rate,
+                pause_before=a.window)
+    t = time.perf_counter()
+    oj.add_origins(nh.OriginPreSampler.all(), cogs.circle)
+    oj.start()
+    device_origins_s = time.perf_counter() - t
+    origin_sampling = {"lineages": int(len(host_lin)), "host_walk_s": host_walk_s,
+                       "host_walk_and_upload_s": host_walk_s + upload_s,
+                       "device_sampling_into_job_s": device_origins_s,
+                       "same_lineages": oj.take_lineages().tobytes() == host_lin.tobytes()}
+    oj.close()
+    del host_lin
+    log(f"origin sampling: {origin_sampling}")
     wall, results = timed_runs(ctx, tables, cogs, lineages, args, 42, a.window, a.steps, a.warmup)
     steps_total = sum(int(o.steps) for o, _, _ in results)
     value = steps_total / wall
@@ -707,6 +725,7 @@ def main():
                         "use_graph": bool(args.use_graph)},
              "copy_kernel": copy,
              "upload_s": upload_s,
+             "origin_sampling": origin_sampling,
              "pcie_inclusive_value": steps_total / (wall + a.steps * upload_s)}
     if a.part == "all":
         # bit-exact comparison with the plain one-launch-per-slice form

@@ -310,6 +310,16 @@ int nh_classify_lineages(const nh_tables *tables, const nh_lineage *lineages, si
  * that nh_classify_lineages does not class OK fail with NH_E_OUT_OF_DEME
  * (active_lineage_sampler/independent/mod.rs:98-117) and nothing is added. */
 int nh_job_add_lineages(nh_job *job, const nh_lineage *lineages, size_t n);
+/* The job's lineages from the scenario's origin sampler, run on the device
+ * straight into the job's lineage store (nh_sample_origins_device's
+ * lineages, never copied through the host; the task-list form, whose queues
+ * live on the host, downloads them).  Replaces the genesis initialiser's
+ * origin walk (rustcoalescence/algorithms/independent/src/initialiser/
+ * genesis.rs:39-72 with OriginPreSampler::all().percentage(p).partition(rank,
+ * size), pre_sampler.rs:71-114).  A job takes either nh_job_add_lineages or
+ * one nh_job_add_origins (NH_E_STATE otherwise). */
+int nh_job_add_origins(nh_job *job, double percentage, uint32_t rank, uint32_t size,
+                       uint32_t circle_centre_x, uint32_t circle_centre_y, uint32_t circle_radius);
 /* Allocate the job's device state and upload its lineages (done implicitly by
  * the first nh_job_step_round; callable earlier so that the upload stays out
  * of a timed region).  No counterpart in the reference: the CUDA algorithm
@@ -462,6 +472,15 @@ int nh_build_alias_tables(uint32_t width, uint32_t height, const uint32_t *habit
 int nh_sample_origins(const nh_scenario *scenario, double percentage, uint32_t rank,
                       uint32_t size, uint32_t circle_centre_x, uint32_t circle_centre_y,
                       uint32_t circle_radius, nh_lineage *out, size_t cap, size_t *n_out);
+/* The same origin samplers on the device, over resident tables: the lineages
+ * nh_sample_origins gives, in its order, written to the DEVICE buffer d_out
+ * (query the count with d_out == NULL; NH_E_CAPACITY if cap is smaller).
+ * Replaces the origin walk the reference runs before simulating
+ * (rustcoalescence/algorithms/independent/src/initialiser/genesis.rs:39-72, the
+ * samplers above and pre_sampler.rs:71-114).  Circle radii up to 2^22. */
+int nh_sample_origins_device(nh_ctx *ctx, const nh_tables *tables, double percentage, uint32_t rank,
+                             uint32_t size, uint32_t circle_centre_x, uint32_t circle_centre_y,
+                             uint32_t circle_radius, nh_lineage *d_out, size_t cap, size_t *n_out);
 
 /* ---- event log ------------------------------------------------------------ */
 /* EventLogRecorder::sort_and_write_segment
@@ -501,6 +520,13 @@ int nh_lineage_file_read(const char *path, nh_lineage *buf, size_t cap, size_t *
  * WyHash state primed for that event's draws).  in / out are host buffers
  * of 8-byte words; n is the number of outputs. */
 int nh_debug_eval(nh_ctx *ctx, int32_t op, const uint64_t *in, uint64_t *out, size_t n);
+/* Diagnostics, no counterpart in the reference: the first n ranks of the
+ * percentage pre-sampler (OriginPreSampler::all().percentage(p),
+ * pre_sampler.rs:71-101) computed on the device into the host buffer out:
+ * from the closed form of the golden-ratio sequence in parallel (serial = 0,
+ * what the origin samplers use) or by the reference's f64 loop on one thread
+ * (serial = 1).  0 < p < 1. */
+int nh_debug_presample(nh_ctx *ctx, double percentage, uint64_t n, int32_t serial, uint64_t *out);
 
 /* Diagnostics, no counterpart in the reference: times `reps` launches of a
  * plain 16-byte-vector copy kernel (k_copy_calibrate) that reads and writes

@@ -11,7 +11,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libnecsim_hip.so")
-SOURCES = ["nh_kernels.hip", "nh_runtime.hip", "nh_server.hip", "nh_tables.hip", "nh_debug.hip", "nh_host.cpp"]
+SOURCES = ["nh_kernels.hip", "nh_runtime.hip", "nh_server.hip", "nh_tables.hip", "nh_origins.hip", "nh_debug.hip",
+           "nh_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NH_OFFLOAD_ARCH", "gfx950")
 

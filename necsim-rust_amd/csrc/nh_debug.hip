@@ -101,6 +101,15 @@ extern "C" int nh_copy_calibrate(nh_ctx *ctx, size_t bytes, int32_t reps, double
     return NH_OK;
 }
 
+extern "C" int nh_debug_presample(nh_ctx *ctx, double percentage, uint64_t n, int32_t serial, uint64_t *out) {
+    if (!ctx || (!out && n)) return set_error(NH_E_INVALID, "nh_debug_presample: null argument");
+    if (!(percentage > 0.0 && percentage < 1.0))
+        return set_error(NH_E_INVALID, "the percentage pre-sampler takes 0 < p < 1 (got %g)", percentage);
+    if (n == 0) return NH_OK;
+    NH_HIP(hipSetDevice(ctx->device));
+    return debug_presample(percentage, n, serial != 0, out, ctx->stream);
+}
+
 extern "C" int nh_debug_eval(nh_ctx *ctx, int32_t op, const uint64_t *in, uint64_t *out, size_t n) {
     if (!ctx || !in || !out || op < 0 || op > 12 || (op == 12 && n % 2))
         return set_error(NH_E_INVALID, "nh_debug_eval: bad argument");

@@ -248,6 +248,38 @@ struct nh_tables {
     nh::DevScenario dev{};
 };
 
+namespace nh {
+
+struct Ranks;
+
+// The device origin sampler of one scenario (nh_origins.hip): prepare()
+// counts the lineages OriginPreSampler::all().percentage(p).partition(rank,
+// size) gives (generating the percentage pre-sampler's ranks on the device),
+// emit() writes them into a lineage store or a reference-layout array.
+struct OriginPlan {
+    const nh_tables *tables = nullptr;
+    hipStream_t stream = nullptr;
+    uint64_t n = 0; // lineages
+    uint32_t rank = 0, size = 1, cx = 0, cy = 0, radius = 0;
+    Ranks *ranks = nullptr;     // percentage ranks, null for `all`
+    uint64_t *d_rows = nullptr; // circle: cells within the radius before each row
+    uint64_t n_rows = 0;
+    int prepare(const nh_tables *t, double percentage, uint32_t rank, uint32_t size, uint32_t cx, uint32_t cy,
+                uint32_t radius, hipStream_t st);
+    int emit(LineageSoA *soa, nh_lineage *aos) const;
+    OriginPlan() = default;
+    OriginPlan(const OriginPlan &) = delete;
+    OriginPlan &operator=(const OriginPlan &) = delete;
+    ~OriginPlan();
+};
+
+// the first n ranks of the percentage pre-sampler (no partition) into a host
+// buffer, from the closed form of the golden-ratio chain or (serial) by the
+// reference's f64 loop on one thread
+int debug_presample(double percentage, uint64_t n, bool serial, uint64_t *out, hipStream_t st);
+
+} // namespace nh
+
 // NH_LINEAGE_* class of a lineage against the tables' scenario (defined in
 // nh_runtime.hip)
 int nh_lineage_class(const nh_tables *t, const nh_lineage &l);

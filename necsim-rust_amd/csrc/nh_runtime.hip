@@ -417,6 +417,10 @@ struct nh_job {
     DevParams P{};
     double *d_ratio = nullptr;
     std::vector<nh_lineage> pending;
+    // nh_job_add_origins: lineages sampled on the device, waiting for the start
+    LineageBuffers origins{};
+    uint64_t n_origins = 0;
+    bool has_origins = false;
     bool started = false, finished = false;
     uint64_t n_initial = 0;
     uint64_t event_slice = 1;
@@ -490,6 +494,7 @@ struct nh_job {
 
     ~nh_job() {
         for (auto &l : lin) l.release();
+        origins.release();
         dfree(d_ratio); dfree(d_ctr); dfree(d_round); dfree(d_events); dfree(d_sorted);
         dfree(d_packed); dfree(d_packed_sorted); dfree(d_counts);
         dfree(sb.keys[0]); dfree(sb.keys[1]); dfree(sb.vals[0]); dfree(sb.vals[1]);
@@ -548,6 +553,7 @@ extern "C" int nh_job_create(nh_ctx *ctx, const nh_tables *tables, const nh_job_
 extern "C" int nh_job_add_lineages(nh_job *j, const nh_lineage *l, size_t n) {
     if (!j || (!l && n)) return set_error(NH_E_INVALID, "nh_job_add_lineages: null argument");
     if (j->started) return set_error(NH_E_STATE, "lineages must be added before the first round");
+    if (j->has_origins) return set_error(NH_E_STATE, "the job's lineages were sampled on the device");
     for (size_t k = 0; k < n; k++) {
         if (nh_lineage_class(j->tables, l[k]) != NH_LINEAGE_OK)
             return set_error(NH_E_OUT_OF_DEME, "lineage %zu at (%u, %u, %u) is outside the habitat or its deme", k,
@@ -558,10 +564,62 @@ extern "C" int nh_job_add_lineages(nh_job *j, const nh_lineage *l, size_t n) {
     return NH_OK;
 }
 
+extern "C" int nh_job_add_origins(nh_job *j, double percentage, uint32_t rank, uint32_t size, uint32_t cx,
+                                  uint32_t cy, uint32_t radius) {
+    if (!j) return set_error(NH_E_INVALID, "nh_job_add_origins: null job");
+    if (j->started) return set_error(NH_E_STATE, "lineages must be added before the first round");
+    if (j->has_origins || !j->pending.empty())
+        return set_error(NH_E_STATE, "a job takes host lineages or one device origin sample, not both");
+    if (!(percentage == percentage)) return set_error(NH_E_INVALID, "percentage is NaN");
+    NH_HIP(hipSetDevice(j->ctx->device));
+    hipStream_t st = j->ctx->stream;
+    OriginPlan plan;
+    NH_TRY(plan.prepare(j->tables, percentage, rank, size, cx, cy, radius, st));
+    if (j->p.form == NH_FORM_TASK_LIST) {
+        // the task-list form keeps its queues on the host
+        nh_lineage *d = nullptr;
+        NH_TRY(dalloc(&d, plan.n));
+        int rc = plan.emit(nullptr, d);
+        if (rc == NH_OK) {
+            j->pending.resize(plan.n);
+            hipError_t e = hipMemcpy(j->pending.data(), d, plan.n * sizeof(nh_lineage), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = set_error(NH_E_HIP, "origin download: %s", hipGetErrorString(e));
+        }
+        dfree(d);
+        return rc;
+    }
+    LineageBuffers b{};
+    int rc = b.alloc(plan.n);
+    if (rc == NH_OK) rc = plan.emit(&b.s, nullptr);
+    if (rc != NH_OK) {
+        b.release();
+        return rc;
+    }
+    j->origins = b;
+    j->n_origins = plan.n;
+    j->has_origins = true;
+    return NH_OK;
+}
+
+extern "C" int nh_sample_origins_device(nh_ctx *ctx, const nh_tables *tables, double percentage, uint32_t rank,
+                                        uint32_t size, uint32_t cx, uint32_t cy, uint32_t radius, nh_lineage *d_out,
+                                        size_t cap, size_t *n_out) {
+    if (!ctx || !tables || !n_out) return set_error(NH_E_INVALID, "nh_sample_origins_device: null argument");
+    if (!(percentage == percentage)) return set_error(NH_E_INVALID, "percentage is NaN");
+    NH_HIP(hipSetDevice(ctx->device));
+    OriginPlan plan;
+    NH_TRY(plan.prepare(tables, percentage, rank, size, cx, cy, radius, ctx->stream));
+    *n_out = plan.n;
+    if (!d_out) return NH_OK;
+    if (cap < plan.n) return set_error(NH_E_CAPACITY, "origin buffer holds %zu of %llu lineages", cap,
+                                       (unsigned long long)plan.n);
+    return plan.emit(nullptr, d_out);
+}
+
 static int job_start(nh_job *j) {
     hipStream_t st = j->ctx->stream;
     const nh_job_params &p = j->p;
-    uint64_t n = j->pending.size();
+    uint64_t n = j->has_origins ? j->n_origins : j->pending.size();
     j->n_initial = n;
     j->started = true;
 
@@ -593,7 +651,8 @@ static int job_start(nh_job *j) {
     else j->cache_cap = 0;
 
     // max_time / level_time start at the max / min last_event_time (monolithic/mod.rs:120-133)
-    if (n) {
+    // (device-sampled origins all start at 0, the initial max / min)
+    if (!j->pending.empty()) {
         double mx = j->pending[0].last_event_time, mn = mx;
         for (auto &l : j->pending) {
             mx = l.last_event_time > mx ? l.last_event_time : mx;
@@ -619,6 +678,9 @@ static int job_start(nh_job *j) {
         j->task_cap = p.task_list_size ? p.task_list_size : 4096;
         NH_TRY(j->lin[0].alloc(j->task_cap));
         for (auto &l : j->pending) j->slow_q.push_back({l, l.last_event_time});
+    } else if (j->has_origins) {
+        j->lin[0] = j->origins; // already in the lineage store layout
+        j->origins = LineageBuffers{};
     } else {
         NH_TRY(j->lin[0].alloc(n));
         NH_TRY(upload_lineages(j->pending, j->lin[0], st));
@@ -1247,11 +1309,11 @@ extern "C" int nh_job_drain_events(nh_job *j, nh_speciation_event *buf, size_t c
 
 extern "C" int nh_job_outcome(const nh_job *j, nh_outcome *o) {
     if (!j || !o) return set_error(NH_E_INVALID, "null argument");
-    o->paused = j->started ? (j->alive > 0) : (j->pending.size() > 0);
+    o->paused = j->started ? (j->alive > 0) : (j->pending.size() + j->n_origins > 0);
     o->time = j->max_time;
     o->steps = j->steps;
     o->n_events = j->n_events;
-    o->n_remaining = j->started ? j->alive : j->pending.size();
+    o->n_remaining = j->started ? j->alive : j->pending.size() + j->n_origins;
     o->rounds = j->rounds;
     o->launches = j->launches;
     return NH_OK;
@@ -1259,6 +1321,17 @@ extern "C" int nh_job_outcome(const nh_job *j, nh_outcome *o) {
 
 extern "C" int nh_job_take_lineages(nh_job *j, nh_lineage *buf, size_t cap, size_t *n_out) {
     if (!j || !n_out) return set_error(NH_E_INVALID, "null argument");
+    if (!j->started && j->has_origins) {
+        *n_out = j->n_origins;
+        if (!buf) return NH_OK;
+        if (cap < j->n_origins) return set_error(NH_E_CAPACITY, "lineage buffer holds %zu of %llu", cap,
+                                                 (unsigned long long)j->n_origins);
+        NH_HIP(hipSetDevice(j->ctx->device));
+        std::vector<nh_lineage> v;
+        NH_TRY(download_lineages(j->origins, nullptr, j->n_origins, v, j->ctx->stream));
+        std::memcpy(buf, v.data(), v.size() * sizeof(nh_lineage));
+        return NH_OK;
+    }
     if (!j->started) {
         *n_out = j->pending.size();
         if (buf) std::memcpy(buf, j->pending.data(), std::min(cap, j->pending.size()) * sizeof(nh_lineage));

@@ -116,6 +116,7 @@ SIGNATURES = {
     "nh_job_create": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(JobParams), ctypes.POINTER(c_vp)]),
     "nh_classify_lineages": (ctypes.c_int, [c_vp, c_vp, c_sz, c_vp, ctypes.POINTER(c_sz)]),
     "nh_job_add_lineages": (ctypes.c_int, [c_vp, c_vp, c_sz]),
+    "nh_job_add_origins": (ctypes.c_int, [c_vp, c_f64, c_u32, c_u32, c_u32, c_u32, c_u32]),
     "nh_job_start": (ctypes.c_int, [c_vp]),
     "nh_job_step_round": (ctypes.c_int, [c_vp, ctypes.POINTER(c_i32)]),
     "nh_job_simulate": (ctypes.c_int, [c_vp]),
@@ -151,6 +152,9 @@ SIGNATURES = {
     "nh_debug_count_packed_events": (ctypes.c_int, [c_vp, c_vp, c_sz, ctypes.POINTER(EventCounts)]),
     "nh_sample_origins": (ctypes.c_int, [ctypes.POINTER(Scenario), c_f64, c_u32, c_u32, c_u32, c_u32,
                                          c_u32, c_vp, c_sz, ctypes.POINTER(c_sz)]),
+    "nh_sample_origins_device": (ctypes.c_int, [c_vp, c_vp, c_f64, c_u32, c_u32, c_u32, c_u32, c_u32, c_vp,
+                                                c_sz, ctypes.POINTER(c_sz)]),
+    "nh_debug_presample": (ctypes.c_int, [c_vp, c_f64, c_u64, c_i32, c_vp]),
 }
 
 _lib = None

@@ -512,6 +512,14 @@ class Job:
         lineages = np.ascontiguousarray(lineages, dtype=_ffi.LINEAGE_DTYPE)
         check(lib().nh_job_add_lineages(self.handle, ptr(lineages), len(lineages)))
 
+    def add_origins(self, pre_sampler, circle=None):
+        """the scenario's origin sampler run on the device into the job
+        (nh_job_add_origins); `circle` = (centre x, centre y, radius) of the
+        almost-infinite scenarios (ScenarioCogs.circle)"""
+        cx, cy, r = circle if circle else (0, 0, 0)
+        check(lib().nh_job_add_origins(self.handle, float(pre_sampler.proportion), int(pre_sampler.rank),
+                                       int(pre_sampler.size), cx, cy, r))
+
     def start(self):
         check(lib().nh_job_start(self.handle))
 
@@ -640,7 +648,8 @@ class HipIndependentAlgorithm:
     speciation events through report_speciation."""
 
     @staticmethod
-    def _run(args, rng, cogs, lineages, pause_before, resume_after, reporter, check_resume=False):
+    def _run(args, rng, cogs, lineages, pause_before, resume_after, reporter, check_resume=False,
+             pre_sampler=None):
         ctx = Context.get(args.device)
         tables = Tables(ctx, cogs)
         if check_resume:
@@ -656,7 +665,10 @@ class HipIndependentAlgorithm:
         job = Job(ctx, tables, args, rng, cogs.speciation_probability, cogs.turnover_rate,
                   pause_before, resume_after, report_dispersal=disp, device_event_counter=on_device)
         try:
-            job.add_lineages(lineages)
+            if pre_sampler is not None:
+                job.add_origins(pre_sampler, cogs.circle)
+            else:
+                job.add_lineages(lineages)
             finished = False
             while not finished:
                 finished = job.step_round()
@@ -684,8 +696,8 @@ class HipIndependentAlgorithm:
     @classmethod
     def initialise_and_simulate(cls, args: IndependentArguments, rng: WyHash, cogs: ScenarioCogs,
                                 pre_sampler: OriginPreSampler, pause_before=None, reporter=None):
-        lineages = cogs.sample_habitat(pre_sampler)
-        return cls._run(args, rng, cogs, lineages, pause_before, 0.0, reporter)
+        # the genesis initialiser's origin walk, on the device (nh_job_add_origins)
+        return cls._run(args, rng, cogs, None, pause_before, 0.0, reporter, pre_sampler=pre_sampler)
 
     @classmethod
     def resume_and_simulate(cls, args: IndependentArguments, rng: WyHash, cogs: ScenarioCogs,

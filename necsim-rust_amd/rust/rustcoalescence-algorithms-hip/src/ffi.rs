@@ -197,6 +197,15 @@ extern "C" {
     ) -> c_int;
     pub fn nh_job_create(ctx: *mut nh_ctx, t: *const nh_tables, p: *const nh_job_params, out: *mut *mut nh_job) -> c_int;
     pub fn nh_job_add_lineages(job: *mut nh_job, l: *const nh_lineage, n: usize) -> c_int;
+    pub fn nh_job_add_origins(
+        job: *mut nh_job,
+        percentage: f64,
+        rank: u32,
+        size: u32,
+        circle_centre_x: u32,
+        circle_centre_y: u32,
+        circle_radius: u32,
+    ) -> c_int;
     pub fn nh_job_start(job: *mut nh_job) -> c_int;
     pub fn nh_job_step_round(job: *mut nh_job, finished: *mut i32) -> c_int;
     pub fn nh_job_drain_events(job: *mut nh_job, buf: *mut nh_speciation_event, cap: usize, n_out: *mut usize) -> c_int;

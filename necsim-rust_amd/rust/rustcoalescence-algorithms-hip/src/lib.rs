@@ -170,13 +170,22 @@ fn autotuned(dev: &Device, params: &ffi::nh_job_params, lineages: &[Lineage], wi
     Ok(tuned)
 }
 
+/// Where a job's lineages come from.
+enum Origins<'a> {
+    /// lineages on the host: resumed ones, or a sampler the device does not run
+    Host(&'a [Lineage]),
+    /// the scenario's origin sampler run on the device into the job
+    /// (nh_job_add_origins), with the pre-sampler's percentage and the circle
+    Device { percentage: f64, circle: (u32, u32, u32) },
+}
+
 #[allow(clippy::too_many_arguments)]
 fn simulate<'p, M: MathsCore, R: Reporter, P: LocalPartition<'p, R>>(
     args: &HipArguments,
     rng: WyHash<M>,
     cogs: &HipCogs,
     mut dev: Device,
-    lineages: &[Lineage],
+    origins: Origins<'_>,
     resume_after: Option<NonNegativeF64>,
     pause_before: Option<NonNegativeF64>,
     local_partition: &mut P,
@@ -192,14 +201,29 @@ fn simulate<'p, M: MathsCore, R: Reporter, P: LocalPartition<'p, R>>(
     // stream (monolithic/mod.rs:139-164)
     let report_dispersal = R::ReportDispersal::VALUE;
     params.report_dispersal = i32::from(report_dispersal);
-    if args.form == HipForm::Persistent && args.autotune_window > 0.0 && !lineages.is_empty() {
-        params = autotuned(&dev, &params, lineages, args.autotune_window)?;
+    if let Origins::Host(lineages) = origins {
+        if args.form == HipForm::Persistent && args.autotune_window > 0.0 && !lineages.is_empty() {
+            params = autotuned(&dev, &params, lineages, args.autotune_window)?;
+        }
     }
     check(unsafe { ffi::nh_job_create(dev.ctx, dev.tables, &params, &mut dev.job) })?;
-    check(unsafe { ffi::nh_job_add_lineages(dev.job, as_nh_lineages(lineages), lineages.len()) })?;
+    let n_initial = match origins {
+        Origins::Host(lineages) => {
+            check(unsafe { ffi::nh_job_add_lineages(dev.job, as_nh_lineages(lineages), lineages.len()) })?;
+            lineages.len() as u64
+        },
+        Origins::Device { percentage, circle: (cx, cy, radius) } => {
+            // the CLI's OriginPreSampler::all().percentage(p), unpartitioned
+            // (rustcoalescence/src/cli/simulate/dispatch/valid/launch.rs:34-45)
+            check(unsafe { ffi::nh_job_add_origins(dev.job, percentage, 0, 1, cx, cy, radius) })?;
+            let mut o = ffi::nh_outcome::default();
+            check(unsafe { ffi::nh_job_outcome(dev.job, &mut o) })?;
+            o.n_remaining
+        },
+    };
 
     // the progress bar starts at the lineages to simulate (monolithic/mod.rs:107)
-    local_partition.report_progress_sync(lineages.len() as u64);
+    local_partition.report_progress_sync(n_initial);
 
     let mut buf: Vec<ffi::nh_speciation_event> = Vec::new();
     let mut packed: Vec<ffi::nh_packed_event> = Vec::new();
@@ -274,12 +298,24 @@ impl<'p, M: MathsCore, O: HipScenario<M>, R: Reporter, P: LocalPartition<'p, R>>
         local_partition: &mut P,
     ) -> Result<SimulationOutcome<M, WyHash<M>>, Self::Error> {
         let cogs = O::hip_cogs(&scenario)?;
-        // the origin sampler stays on the CPU, in the reference's own order
-        // (independent/src/launch.rs:78-86)
-        let lineages: Vec<Lineage> =
-            O::sample_habitat(&scenario.habitat, pre_sampler, scenario.origin_sampler_auxiliary.clone()).collect();
         let dev = Device::open(&args, &cogs)?;
-        simulate(&args, rng, &cogs, dev, &lineages, None, pause_before, local_partition)
+        // the genesis origin walk (independent/src/launch.rs:78-86) runs on the
+        // device where it has the scenario's sampler; the autotuner, which
+        // times a sample of the lineages, and samplers the device does not
+        // run take the reference's walk on the host
+        match O::device_origins(&scenario) {
+            Some(circle) if args.autotune_window <= 0.0 => {
+                let percentage = pre_sampler.get_sample_proportion().get();
+                simulate(&args, rng, &cogs, dev, Origins::Device { percentage, circle }, None, pause_before,
+                         local_partition)
+            },
+            _ => {
+                let lineages: Vec<Lineage> =
+                    O::sample_habitat(&scenario.habitat, pre_sampler, scenario.origin_sampler_auxiliary.clone())
+                        .collect();
+                simulate(&args, rng, &cogs, dev, Origins::Host(&lineages), None, pause_before, local_partition)
+            },
+        }
     }
 
     fn resume_and_simulate<I: Iterator<Item = u64>, L: ExactSizeIterator<Item = Lineage>>(
@@ -303,7 +339,7 @@ impl<'p, M: MathsCore, O: HipScenario<M>, R: Reporter, P: LocalPartition<'p, R>>
         if !exceptional.is_empty() {
             return Err(ResumeError::Sample(exceptional));
         }
-        Ok(simulate(&args, rng, &cogs, dev, &lineages, resume_after, pause_before, local_partition)?)
+        Ok(simulate(&args, rng, &cogs, dev, Origins::Host(&lineages), resume_after, pause_before, local_partition)?)
     }
 
     fn fixup_for_restart<I: Iterator<Item = u64>, L: ExactSizeIterator<Item = Lineage>>(

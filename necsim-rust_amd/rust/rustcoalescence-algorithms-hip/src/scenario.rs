@@ -77,6 +77,16 @@ pub trait HipScenario<M: MathsCore>: Scenario<M, WyHash<M>> {
     /// Returns `HipError` (NH_E_INVALID) for unsupported scenarios and for
     /// extents the device cannot index.
     fn hip_cogs(cogs: &ScenarioCogs<M, WyHash<M>, Self>) -> Result<HipCogs, HipError>;
+
+    /// The scenario's origin sampler as the device runs it
+    /// (nh_job_add_origins): `Some` with the almost-infinite circle sample's
+    /// (centre x, centre y, radius), or (0, 0, 0) for the non-spatial and
+    /// in-memory samplers; `None` where the device has no such sampler, and
+    /// the reference's own walk runs on the host.
+    fn device_origins(cogs: &ScenarioCogs<M, WyHash<M>, Self>) -> Option<(u32, u32, u32)> {
+        let _ = cogs;
+        None
+    }
 }
 
 fn invalid(message: String) -> HipError {
@@ -141,6 +151,11 @@ impl<M: MathsCore> HipScenario<M> for rustcoalescence_scenarios::non_spatial::No
         };
         Ok(HipCogs::new(desc, nu, lambda))
     }
+
+    fn device_origins(_cogs: &ScenarioCogs<M, WyHash<M>, Self>) -> Option<(u32, u32, u32)> {
+        // NonSpatialOriginSampler (origin_sampler/non_spatial.rs:89-112)
+        Some((0, 0, 0))
+    }
 }
 
 #[cfg(feature = "almost-infinite-normal-dispersal-scenario")]
@@ -156,6 +171,17 @@ impl<M: MathsCore> HipScenario<M>
             ..Default::default()
         };
         Ok(HipCogs::new(desc, nu, lambda))
+    }
+
+    fn device_origins(cogs: &ScenarioCogs<M, WyHash<M>, Self>) -> Option<(u32, u32, u32)> {
+        // SingletonDemesCircleOriginSampler (singleton_demes/circle.rs:52-156);
+        // the rectangle sample has no device sampler
+        match &cogs.origin_sampler_auxiliary.0 {
+            rustcoalescence_scenarios::almost_infinite::Sample::Circle { centre, radius } => {
+                Some((centre.x(), centre.y(), u32::from(*radius)))
+            },
+            rustcoalescence_scenarios::almost_infinite::Sample::Rectangle(_) => None,
+        }
     }
 }
 
@@ -209,6 +235,11 @@ impl<M: MathsCore> HipScenario<M>
             return Err(invalid("the dispersal sampler's tables do not cover the habitat extent".into()));
         }
         Ok(out)
+    }
+
+    fn device_origins(_cogs: &ScenarioCogs<M, WyHash<M>, Self>) -> Option<(u32, u32, u32)> {
+        // InMemoryOriginSampler (origin_sampler/in_memory.rs:89-119)
+        Some((0, 0, 0))
     }
 }
 

@@ -6,7 +6,9 @@ resume_and_simulate, rustcoalescence/algorithms/src/lib.rs:55-112):
 
   nh_init -> nh_tables_create -> [resume: nh_classify_lineages, and
   ResumeError::Sample of the exceptional lineages if any] -> nh_job_create ->
-  nh_job_add_lineages -> report_progress_sync(n) -> per round:
+  nh_job_add_lineages (resume) or nh_job_add_origins (genesis: the scenario's
+  origin sampler on the device) + nh_job_outcome for the count ->
+  report_progress_sync(n) -> per round:
   nh_job_step_round, nh_job_drain_events (count, then fill),
   nh_job_outcome -> report_progress(n_remaining) -> report_progress_sync ->
   nh_job_outcome -> [paused: nh_job_take_lineages] -> destroy job, tables,
@@ -40,8 +42,9 @@ def ok(rc):
     _ffi.check(rc)
 
 
-def glue_simulate(scenario_desc, lineages, params, resume=False):
-    """lib.rs `Device::open` + `simulate` (+ the resume classification)"""
+def glue_simulate(scenario_desc, lineages, params, resume=False, origins=None):
+    """lib.rs `Device::open` + `simulate` (+ the resume classification);
+    origins = (percentage, (cx, cy, radius)) takes the genesis path"""
     ctx, tables, job = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
     progress, events = [], []
     try:
@@ -57,9 +60,15 @@ def glue_simulate(scenario_desc, lineages, params, resume=False):
             if bad:
                 return {"exceptional": bad, "job_created": bool(job.value)}
         ok(L().nh_job_create(ctx, tables, ctypes.byref(params), ctypes.byref(job)))
-        ok(L().nh_job_add_lineages(job, _ffi.ptr(lineages), len(lineages)))
-        progress.append(("sync", len(lineages)))
         o = _ffi.Outcome()
+        if origins is not None:
+            pct, (cx, cy, r) = origins
+            ok(L().nh_job_add_origins(job, pct, 0, 1, cx, cy, r))
+            ok(L().nh_job_outcome(job, ctypes.byref(o)))
+            progress.append(("sync", int(o.n_remaining)))
+        else:
+            ok(L().nh_job_add_lineages(job, _ffi.ptr(lineages), len(lineages)))
+            progress.append(("sync", len(lineages)))
         while True:
             fin = ctypes.c_int32(0)
             ok(L().nh_job_step_round(job, ctypes.byref(fin)))
@@ -143,3 +152,19 @@ def test_glue_sequence_resume_rejects_exceptional_lineages(config1):
     assert [k for k, _ in out["exceptional"]] == [1, 2]  # OutOfDeme, OutOfHabitat, input order
     assert out["exceptional"][0][1].tobytes() == bad[1].tobytes()
     assert out["exceptional"][1][1].tobytes() == bad[4].tobytes()
+
+
+@pytest.mark.parametrize("form", [_ffi.FORM_PLAIN, _ffi.FORM_PERSISTENT])
+def test_glue_sequence_genesis_from_device_origins(form):
+    announce(f"glue call sequence, genesis: config 2 origins at 40 % sampled on the device, form {form}")
+    cogs = nh.AlmostInfiniteNormalDispersalScenario(100, 10.0).new(1e-4)
+    orc = O.Scenario.almost_infinite_normal(10.0)
+    lin = orc.origins(0.4, circle=cogs.circle)
+    ref = oracle_run(orc, lin, 42, 1e-4, dedup="none", event_slice=("relative", 0.02), pause_before=50.0)
+    out = glue_simulate(cogs.desc, None, params_for(42, 1e-4, pause=50.0, form=form), origins=(0.4, cogs.circle))
+    assert out["progress"][0] == ("sync", len(lin))
+    assert out["events"].tobytes() == ref["events"].tobytes()
+    assert out["steps"] == ref["steps"] and out["time"] == ref["time"]
+    def by_ref(a):
+        return np.sort(a, order=["global_reference"]).tobytes()
+    assert by_ref(out["remaining"]) == by_ref(ref["remaining"])
