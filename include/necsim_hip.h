@@ -527,6 +527,12 @@ int nh_debug_eval(nh_ctx *ctx, int32_t op, const uint64_t *in, uint64_t *out, si
  * what the origin samplers use) or by the reference's f64 loop on one thread
  * (serial = 1).  0 < p < 1. */
 int nh_debug_presample(nh_ctx *ctx, double percentage, uint64_t n, int32_t serial, uint64_t *out);
+/* Diagnostics, no counterpart in the reference: *out = the sequential sum
+ * x + x + ... + x of n terms folded from 0.0, as the host evaluates the
+ * water-level rate sum Σλ(ν) over n lineages of one rate
+ * (parallelisation/independent/monolithic/mod.rs:139-164), bit for bit equal
+ * to the fold but computed binade by binade.  Host only. */
+int nh_debug_repeated_sum(double x, uint64_t n, double *out);
 
 /* Diagnostics, no counterpart in the reference: times `reps` launches of a
  * plain 16-byte-vector copy kernel (k_copy_calibrate) that reads and writes

@@ -101,6 +101,12 @@ extern "C" int nh_copy_calibrate(nh_ctx *ctx, size_t bytes, int32_t reps, double
     return NH_OK;
 }
 
+extern "C" int nh_debug_repeated_sum(double x, uint64_t n, double *out) {
+    if (!out) return set_error(NH_E_INVALID, "nh_debug_repeated_sum: null output");
+    *out = repeated_sum(x, n);
+    return NH_OK;
+}
+
 extern "C" int nh_debug_presample(nh_ctx *ctx, double percentage, uint64_t n, int32_t serial, uint64_t *out) {
     if (!ctx || (!out && n)) return set_error(NH_E_INVALID, "nh_debug_presample: null argument");
     if (!(percentage > 0.0 && percentage < 1.0))
@@ -233,7 +239,7 @@ extern "C" int nh_debug_count_packed_events(nh_ctx *ctx, const nh_packed_event *
         hipError_t e = hipMemcpyAsync(d_in, in, n * sizeof(DevPacked), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(d_c, &c, sizeof(c), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) {
-            launch_count_events(d_in, nullptr, n, nullptr, d_c, st);
+            launch_count_events(d_in, nullptr, nullptr, n, nullptr, d_c, st);
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipMemcpyAsync(&c, d_c, sizeof(c), hipMemcpyDeviceToHost, st);

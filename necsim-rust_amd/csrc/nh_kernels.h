@@ -220,11 +220,16 @@ struct EventCounts {
     unsigned long long carry_has;   // an event was counted before this round
     unsigned long long carry_prior; // its prior time (magnitude bits)
 };
-// Counts one round's sorted PackedEvents ev[0, n): with `end`, n is read from
-// *end when the kernels run (clamped to `cap`; capturable in a graph).
-// `ghosts` (or null): the number of trailing ghost records to leave out.
-void launch_count_events(const DevPacked *ev, const unsigned long long *end, uint64_t n_or_cap,
-                         const unsigned long long *ghosts, EventCounts *c, hipStream_t stream);
+// Counts one round's sorted PackedEvents ev[0, n) (or ev[perm[0, n)] in the
+// order of a sort permutation): with `end`, n is read from *end when the
+// kernels run (clamped to `cap`; capturable in a graph).  `ghosts` (or
+// null): the number of trailing ghost records to leave out.
+void launch_count_events(const DevPacked *ev, const uint32_t *perm, const unsigned long long *end,
+                         uint64_t n_or_cap, const unsigned long long *ghosts, EventCounts *c, hipStream_t stream);
+// the round's `slots` records sorted to a permutation and the first n_events
+// of that order counted; `scratch` holds `slots` records of sort scratch
+void sort_and_count_packed(const DevPacked *events, uint64_t slots, uint64_t n_events, DevPacked *scratch,
+                           SortBuffers &sb, EventCounts *c, hipStream_t stream);
 
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream);
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream);

@@ -1425,7 +1425,10 @@ __global__ void __launch_bounds__(KS_THREADS) k_field_minmax(const E *ev0, SortS
 // time fields only; runs with several origins or targets by passes over
 // every field.
 constexpr uint64_t TIE_RUN_MAX = 1024;
-constexpr uint64_t TIE_WINDOW = 1024;
+// A window's span is its positions plus the tail of the run that crosses its
+// end; 896 positions keep spans with tails up to 128 in a 1024-element
+// network (1024 positions made nearly every span need 2048).
+constexpr uint64_t TIE_WINDOW = 896;
 constexpr int TIE_SORT_N = 2048; // >= TIE_WINDOW + TIE_RUN_MAX - 1, a power of two
 constexpr int TIE_THREADS = 512;
 
@@ -1949,13 +1952,16 @@ static int highest_digit(unsigned long long diff) {
 
 // Sort of one round's events by the PackedEvent order: an LSD radix sort of
 // the event times; if two events share a time (duplicate lineages when dedup
-// is off), a full LSD radix sort over every field of the order.  Digits
-// above the highest one in which a key range differs are equal for every key
-// and are skipped.
+// is off), the runs of equal times sorted in LDS windows, or, for runs with
+// several origins or longer than TIE_RUN_MAX, a full LSD radix sort over
+// every field of the order.  Digits above the highest one in which a key
+// range differs are equal for every key and are skipped.  Returns the
+// sorted permutation (one of sb.vals); `scratch` (n events) holds the runs'
+// prior times and references.
 template <class E>
-void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream_t stream, bool fixed_passes) {
+static const uint32_t *sort_permutation(const E *events, uint64_t n, E *scratch, SortBuffers &sb,
+                                        hipStream_t stream, bool fixed_passes) {
     using O = EventOrder<E>;
-    if (n == 0) return;
     SortSize sz{n, nullptr, nullptr, nullptr};
     unsigned g = grid_for(n, 256), gk = grid_for(n, KS_TILE);
     hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, stream, sb.minmax);
@@ -1975,8 +1981,8 @@ void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream
                              stream);
         (void)hipStreamSynchronize(stream);
         if (mm[MM_TIES] && !fixed_passes) {
-            // rank each run of equal times in place (prior / reference scratch in `out`)
-            unsigned long long *prior_ws = reinterpret_cast<unsigned long long *>(out);
+            // rank each run of equal times in place (prior / reference in `scratch`)
+            unsigned long long *prior_ws = reinterpret_cast<unsigned long long *>(scratch);
             unsigned long long *ref_ws = prior_ws + n;
             static_assert(sizeof(E) >= 16, "scratch of two words per event");
             static_assert(MM_TIE_LONG == MM_TIE_MIXED + 1, "tie flags are adjacent");
@@ -1988,9 +1994,7 @@ void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream
             if (!mm[MM_TIE_MIXED] && !mm[MM_TIE_LONG]) {
                 hipLaunchKernelGGL(k_tie_sort_windows, dim3(grid_for(n, TIE_WINDOW)), dim3(TIE_THREADS), 0, stream,
                                    sb.keys[cur], sb.vals[cur], n, prior_ws, ref_ws, sb.vals[cur ^ 1]);
-                hipLaunchKernelGGL(k_gather_events<E>, dim3(g), dim3(256), 0, stream, events, sb.vals[cur ^ 1], out,
-                                   sz);
-                return;
+                return sb.vals[cur ^ 1];
             }
         }
         if (mm[MM_TIES]) {
@@ -2012,7 +2016,15 @@ void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream
             }
         }
     }
-    hipLaunchKernelGGL(k_gather_events<E>, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], out, sz);
+    return sb.vals[cur];
+}
+
+template <class E>
+void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream_t stream, bool fixed_passes) {
+    if (n == 0) return;
+    const uint32_t *perm = sort_permutation(events, n, out, sb, stream, fixed_passes);
+    SortSize sz{n, nullptr, nullptr, nullptr};
+    hipLaunchKernelGGL(k_gather_events<E>, dim3(grid_for(n, 256)), dim3(256), 0, stream, events, perm, out, sz);
 }
 
 // Capturable variant: the event range is read from device memory, grids are
@@ -2071,8 +2083,10 @@ __device__ __forceinline__ bool same_event_key(const DevPacked &a, const DevPack
            a.oindex == b.oindex && (speciation || (a.tx == b.tx && a.ty == b.ty && a.tindex == b.tindex));
 }
 
-__global__ void __launch_bounds__(256) k_count_events(const DevPacked *ev, SortSize sz,
+__global__ void __launch_bounds__(256) k_count_events(const DevPacked *ev, const uint32_t *perm, SortSize sz,
                                                      const unsigned long long *ghosts, EventCounts *c) {
+    // the k-th event in sorted order: ev[k], or ev[perm[k]] from the sort's permutation
+    auto at = [&](uint64_t k) -> const DevPacked & { return ev[perm ? perm[k] : k]; };
     __shared__ unsigned long long s_cnt[EC_COUNTERS];
     if (threadIdx.x < EC_COUNTERS) s_cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -2080,13 +2094,13 @@ __global__ void __launch_bounds__(256) k_count_events(const DevPacked *ev, SortS
     const uint64_t n = sz.get() - (ghosts ? *ghosts : 0ull);
     int which = -1; // the counter this event adds to, besides the raw total
     if (i < n) {
-        const DevPacked e = ev[i];
+        const DevPacked e = at(i);
         const bool sp = packed_is_speciation(e);
         const bool has_prev = i > 0 || c->carry_has != 0;
-        const uint64_t prev_prior = i > 0 ? (bits(ev[i - 1].prior_time) & ABS_BITS) : c->carry_prior;
+        const uint64_t prev_prior = i > 0 ? (bits(at(i - 1).prior_time) & ABS_BITS) : c->carry_prior;
         bool dup = false;
         for (uint64_t j = i; j-- > 0;) {
-            const DevPacked p = ev[j];
+            const DevPacked p = at(j);
             if (!same_event_key(p, e, sp)) break;
             if (packed_is_speciation(p) == sp) {
                 dup = true;
@@ -2121,21 +2135,31 @@ __global__ void __launch_bounds__(256) k_count_events(const DevPacked *ev, SortS
 }
 
 // after the round's count: the last prior time for the next round
-__global__ void k_count_carry(const DevPacked *ev, SortSize sz, const unsigned long long *ghosts,
-                              EventCounts *c) {
+__global__ void k_count_carry(const DevPacked *ev, const uint32_t *perm, SortSize sz,
+                              const unsigned long long *ghosts, EventCounts *c) {
     const uint64_t n = sz.get() - (ghosts ? *ghosts : 0ull);
     if (n) {
         c->carry_has = 1;
-        c->carry_prior = bits(ev[n - 1].prior_time) & ABS_BITS;
+        c->carry_prior = bits(ev[perm ? perm[n - 1] : n - 1].prior_time) & ABS_BITS;
     }
 }
 
-void launch_count_events(const DevPacked *ev, const unsigned long long *end, uint64_t n_or_cap,
-                         const unsigned long long *ghosts, EventCounts *c, hipStream_t stream) {
+void launch_count_events(const DevPacked *ev, const uint32_t *perm, const unsigned long long *end,
+                         uint64_t n_or_cap, const unsigned long long *ghosts, EventCounts *c, hipStream_t stream) {
     if (n_or_cap == 0) return;
     SortSize sz{n_or_cap, end, nullptr, nullptr};
-    hipLaunchKernelGGL(k_count_events, dim3(grid_for(n_or_cap, 256)), dim3(256), 0, stream, ev, sz, ghosts, c);
-    hipLaunchKernelGGL(k_count_carry, dim3(1), dim3(1), 0, stream, ev, sz, ghosts, c);
+    hipLaunchKernelGGL(k_count_events, dim3(grid_for(n_or_cap, 256)), dim3(256), 0, stream, ev, perm, sz, ghosts,
+                       c);
+    hipLaunchKernelGGL(k_count_carry, dim3(1), dim3(1), 0, stream, ev, perm, sz, ghosts, c);
+}
+
+// A round of a device_event_counter job: the events' sorted permutation, then
+// the counters read the events through it (no sorted copy is written)
+void sort_and_count_packed(const DevPacked *events, uint64_t slots, uint64_t n_events, DevPacked *scratch,
+                           SortBuffers &sb, EventCounts *c, hipStream_t stream) {
+    if (slots == 0 || n_events == 0) return;
+    const uint32_t *perm = sort_permutation(events, slots, scratch, sb, stream, false);
+    launch_count_events(events, perm, nullptr, n_events, nullptr, c, stream);
 }
 
 uint32_t soa_compact_blocks(uint64_t n) { return (uint32_t)((n + SC_TILE - 1) / SC_TILE); }

@@ -361,10 +361,58 @@ NH_HD double max_after(double before, double value) {
 
 // Repeated sequential sum `x + x + ... + x` (n terms, folded from 0.0), the
 // value of the water-level event-rate sum over n lineages with a uniform rate
-// (parallelisation/independent/monolithic/mod.rs:150-164).
-inline double repeated_sum(double x, uint64_t n) {
+// (parallelisation/independent/monolithic/mod.rs:150-164), equal bit for bit
+// to the fold but computed binade by binade.  While s and s + x stay in one
+// binade [2^(e-1), 2^e) with ulp u, every addition adds the same multiple D
+// of u: the multiple nearest to x, since s is a multiple of u.  (When x / u
+// is a half-integer, ties go to the even neighbour; the increment is then
+// even after at most one step, which keeps the parity of s / u and the
+// increment fixed.)  So a run of steps inside a binade is one integer
+// multiply-add; a step that reaches the next binade, and a step whose
+// successor adds a different increment, are taken one at a time.  A fold of
+// 12.6M terms takes milliseconds of host time per water-level round; this
+// takes a few dozen iterations.
+inline double repeated_sum_fold(double x, uint64_t n) {
     double s = 0.0;
     for (uint64_t j = 0; j < n; j++) s = s + x;
+    return s;
+}
+
+inline double repeated_sum(double x, uint64_t n) {
+    if (n == 0) return 0.0;
+    if (!(x > 0.0) || x == __builtin_inf()) return repeated_sum_fold(x, n);
+    const uint64_t two53 = 1ull << 53;
+    double s = x; // 0.0 + x
+    uint64_t k = 1;
+    while (k < n) {
+        int e = 0;
+        (void)__builtin_frexp(s, &e);          // s in [2^(e-1), 2^e)
+        if (e < -1021) e = -1021;              // subnormals share the ulp 2^-1074
+        const double u = __builtin_ldexp(1.0, e - 53);
+        const uint64_t S = (uint64_t)(s / u);  // exact: s is a multiple of u, below 2^53 u
+        const uint64_t xi = (uint64_t)__builtin_floor(x / u);
+        // the steps j = 1, 2, ... stay in the binade while S + (j-1) D + x / u < 2^53,
+        // i.e. S + (j-1) D <= B
+        const uint64_t B = two53 - 1 - xi;
+        const double s1 = s + x;
+        if (S > B) { // this step reaches the next binade
+            s = s1;
+            k++;
+            continue;
+        }
+        const double d1 = s1 - s; // exact (x <= s, so s <= s1 <= 2s)
+        if (d1 == 0.0) return s;  // x is below half an ulp: s no longer changes
+        const uint64_t D = (uint64_t)(d1 / u);
+        if (S + D <= B && (s1 + x) - s1 != d1) { // the tie's parity settles first
+            s = s1;
+            k++;
+            continue;
+        }
+        uint64_t m = (B - S) / D + 1;
+        if (m > n - k) m = n - k;
+        s = (double)(S + m * D) * u; // at most 2^53 units: exact
+        k += m;
+    }
     return s;
 }
 

@@ -1216,10 +1216,9 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
     // the live water-level reporter sorts each round's events (live.rs:88-115)
     // (ghost records sort after every event)
     if (ne && j->disp && j->d_counts) {
-        if (!sorted) {
-            sort_events(j->d_packed, slots, j->d_packed_sorted, j->sb, st, false);
-            launch_count_events(j->d_packed_sorted, nullptr, ne, nullptr, j->d_counts, st);
-        }
+        // the counters read the round through the sort's permutation: no
+        // sorted copy (nothing drains a counted round)
+        if (!sorted) sort_and_count_packed(j->d_packed, slots, ne, j->d_packed_sorted, j->sb, j->d_counts, st);
         NH_HIP(hipGetLastError());
     } else if (ne && j->disp) {
         if (!sorted) sort_events(j->d_packed, slots, j->d_packed_sorted, j->sb, st, false);

@@ -155,6 +155,7 @@ SIGNATURES = {
     "nh_sample_origins_device": (ctypes.c_int, [c_vp, c_vp, c_f64, c_u32, c_u32, c_u32, c_u32, c_u32, c_vp,
                                                 c_sz, ctypes.POINTER(c_sz)]),
     "nh_debug_presample": (ctypes.c_int, [c_vp, c_f64, c_u64, c_i32, c_vp]),
+    "nh_debug_repeated_sum": (ctypes.c_int, [c_f64, c_u64, ctypes.POINTER(c_f64)]),
 }
 
 _lib = None
