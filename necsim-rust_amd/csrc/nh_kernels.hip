@@ -1363,8 +1363,10 @@ __global__ void __launch_bounds__(KS_THREADS) k_event_keys(const E *ev0, SortSiz
             const unsigned long long k = EventOrder<E>::field(ev[p], EventOrder<E>::TIME);
             keys[p] = k;
             vals[p] = (uint32_t)p;
-            lo = k < lo ? k : lo;
-            hi = k > hi ? k : hi;
+            if (k != GHOST_TIME_BITS) { // unused slots sort last, outside the range
+                lo = k < lo ? k : lo;
+                hi = k > hi ? k : hi;
+            }
         }
     }
     block_minmax(lo, hi, mm + MM_TIME);
@@ -1577,8 +1579,17 @@ __global__ void k_field_keys(const E *ev0, const uint32_t *vals, unsigned long l
     if (p < sz.get()) keys[p] = EventOrder<E>::field(ev[vals[p]], f);
 }
 
+// Digits are those of min(key, top) - base: a key range [base, top) sorted
+// in fewer passes, keys at or above top (unused slots) after it; base 0 and
+// top ~0 sort the keys themselves.
+__device__ __forceinline__ uint32_t radix_digit(unsigned long long key, unsigned long long base,
+                                                unsigned long long top, int shift) {
+    return (uint32_t)(((key < top ? key : top) - base) >> shift) & 255u;
+}
+
 __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const unsigned long long *keys, SortSize sz,
-                                                           int shift, uint32_t *hist) {
+                                                           int shift, unsigned long long key_base,
+                                                           unsigned long long top, uint32_t *hist) {
     __shared__ uint32_t h[256];
     if (sz.skip()) return;
     const uint64_t n = sz.get();
@@ -1589,7 +1600,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const unsigned long l
     uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
     for (int j = 0; j < RS_ITEMS; j++) {
         uint64_t p = base + (uint64_t)j * RS_THREADS + threadIdx.x;
-        if (p < n) atomicAdd(&h[(keys[p] >> shift) & 255u], 1u);
+        if (p < n) atomicAdd(&h[radix_digit(keys[p], key_base, top, shift)], 1u);
     }
     __syncthreads();
     hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
@@ -1598,8 +1609,8 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const unsigned long l
 __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const unsigned long long *kin,
                                                               const uint32_t *vin,
                                                               unsigned long long *kout, uint32_t *vout,
-                                                              SortSize sz, int shift,
-                                                              const uint32_t *offsets) {
+                                                              SortSize sz, int shift, unsigned long long key_base,
+                                                              unsigned long long top, const uint32_t *offsets) {
     constexpr int W = RS_THREADS / 64;
     if (sz.skip()) return;
     const uint64_t n = sz.get();
@@ -1618,7 +1629,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const unsigned lon
         bool valid = p < n;
         unsigned long long key = valid ? kin[p] : 0;
         uint32_t val = valid ? vin[p] : 0;
-        uint32_t digit = (uint32_t)(key >> shift) & 255u;
+        uint32_t digit = radix_digit(key, key_base, top, shift);
         uint64_t peers = __ballot(valid);
         for (int bit = 0; bit < 8; bit++) {
             bool set = (digit >> bit) & 1u;
@@ -1928,17 +1939,18 @@ size_t sort_hist_words(uint64_t n) {
 }
 
 static int radix_passes(SortSize sz, uint64_t cap, SortBuffers &sb, hipStream_t stream, int first, int last,
-                        int cur) {
+                        int cur, unsigned long long base = 0, unsigned long long top = ~0ull) {
     uint32_t nb = sort_blocks(cap);
     uint32_t nc = (uint32_t)(((uint64_t)256 * nb + HS_CHUNK - 1) / HS_CHUNK);
     uint32_t *chunk_sums = sb.hist + (size_t)256 * nb + 1;
     for (int d = first; d <= last; d++) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sz, 8 * d, sb.hist);
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sz, 8 * d, base, top,
+                           sb.hist);
         hipLaunchKernelGGL(k_hist_chunk_sums, dim3(nc), dim3(HS_THREADS), 0, stream, sb.hist, sz, chunk_sums);
         hipLaunchKernelGGL(k_scan_chunk_sums, dim3(1), dim3(1024), 0, stream, chunk_sums, sz);
         hipLaunchKernelGGL(k_hist_scan_apply, dim3(nc), dim3(HS_THREADS), 0, stream, sb.hist, sz, chunk_sums);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sb.vals[cur],
-                           sb.keys[cur ^ 1], sb.vals[cur ^ 1], sz, 8 * d, sb.hist);
+                           sb.keys[cur ^ 1], sb.vals[cur ^ 1], sz, 8 * d, base, top, sb.hist);
         cur ^= 1;
     }
     return cur;
@@ -1969,12 +1981,22 @@ static const uint32_t *sort_permutation(const E *events, uint64_t n, E *scratch,
                        sb.minmax);
     unsigned long long mm[MM_WORDS];
     int last = 7;
+    unsigned long long base = 0, top = ~0ull;
     if (!fixed_passes) {
         (void)hipMemcpyAsync(mm, sb.minmax, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, stream);
         (void)hipStreamSynchronize(stream);
-        last = highest_digit(mm[MM_TIME] ^ mm[MM_TIME + 1]);
+        // the times' range [lo, hi] (unused slots excluded): digits of
+        // key - lo, unused slots as hi + 1
+        const unsigned long long lo = mm[MM_TIME], hi = mm[MM_TIME + 1];
+        if (lo > hi) {
+            last = -1; // only unused slots
+        } else {
+            base = lo;
+            top = hi + 1;
+            last = highest_digit(top - base);
+        }
     }
-    int cur = radix_passes(sz, n, sb, stream, 0, last, 0);
+    int cur = radix_passes(sz, n, sb, stream, 0, last, 0, base, top);
     if (n > 1) {
         hipLaunchKernelGGL(k_tie_count, dim3(gk), dim3(KS_THREADS), 0, stream, sb.keys[cur], sz, sb.minmax + MM_TIES);
         (void)hipMemcpyAsync(&mm[MM_TIES], sb.minmax + MM_TIES, sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -2083,41 +2105,66 @@ __device__ __forceinline__ bool same_event_key(const DevPacked &a, const DevPack
            a.oindex == b.oindex && (speciation || (a.tx == b.tx && a.ty == b.ty && a.tindex == b.tindex));
 }
 
+// the fields of a PackedEvent the counter reads (all but the reference and
+// the coalescence)
+struct CountRec {
+    uint64_t prior, time; // bit patterns, signs included
+    uint32_t ox, oy, oi, tx, ty, ti;
+};
+
+__device__ __forceinline__ CountRec count_rec(const DevPacked &e) {
+    return CountRec{bits(e.prior_time), bits(e.event_time), e.ox, e.oy, e.oindex, e.tx, e.ty, e.tindex};
+}
+__device__ __forceinline__ bool rec_is_speciation(const CountRec &e) {
+    return (e.prior >> 63) != 0 && (e.time >> 63) == 0; // p(-), e(+)
+}
+__device__ __forceinline__ bool rec_same_key(const CountRec &a, const CountRec &b, bool speciation) {
+    return (a.time & ABS_BITS) == (b.time & ABS_BITS) && a.ox == b.ox && a.oy == b.oy && a.oi == b.oi &&
+           (speciation || (a.tx == b.tx && a.ty == b.ty && a.ti == b.ti));
+}
+
+// Each thread reads its own event once (through the permutation, a random
+// read) and shares it in LDS: the previous events in sorted order come from
+// the neighbouring threads, and from memory only across the block's start.
 __global__ void __launch_bounds__(256) k_count_events(const DevPacked *ev, const uint32_t *perm, SortSize sz,
                                                      const unsigned long long *ghosts, EventCounts *c) {
     // the k-th event in sorted order: ev[k], or ev[perm[k]] from the sort's permutation
-    auto at = [&](uint64_t k) -> const DevPacked & { return ev[perm ? perm[k] : k]; };
+    auto at = [&](uint64_t k) -> CountRec { return count_rec(ev[perm ? perm[k] : k]); };
     __shared__ unsigned long long s_cnt[EC_COUNTERS];
+    __shared__ CountRec s_rec[256];
     if (threadIdx.x < EC_COUNTERS) s_cnt[threadIdx.x] = 0;
-    __syncthreads();
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t first = (uint64_t)blockIdx.x * blockDim.x;
+    const uint64_t i = first + threadIdx.x;
     const uint64_t n = sz.get() - (ghosts ? *ghosts : 0ull);
+    CountRec e{};
+    if (i < n) s_rec[threadIdx.x] = e = at(i);
+    __syncthreads();
+    auto rec = [&](uint64_t k) -> CountRec { return k >= first ? s_rec[k - first] : at(k); };
     int which = -1; // the counter this event adds to, besides the raw total
     if (i < n) {
-        const DevPacked e = at(i);
-        const bool sp = packed_is_speciation(e);
+        const bool sp = rec_is_speciation(e);
         const bool has_prev = i > 0 || c->carry_has != 0;
-        const uint64_t prev_prior = i > 0 ? (bits(at(i - 1).prior_time) & ABS_BITS) : c->carry_prior;
+        const uint64_t prev_prior = i > 0 ? (rec(i - 1).prior & ABS_BITS) : c->carry_prior;
         bool dup = false;
         for (uint64_t j = i; j-- > 0;) {
-            const DevPacked p = at(j);
-            if (!same_event_key(p, e, sp)) break;
-            if (packed_is_speciation(p) == sp) {
+            const CountRec p = rec(j);
+            if (!rec_same_key(p, e, sp)) break;
+            if (rec_is_speciation(p) == sp) {
                 dup = true;
                 break;
             }
         }
         if (dup) {
-            if (!has_prev || prev_prior != (bits(e.prior_time) & ABS_BITS)) which = EC_LATE_COALESCENCE;
+            if (!has_prev || prev_prior != (e.prior & ABS_BITS)) which = EC_LATE_COALESCENCE;
         } else if (sp) {
             which = EC_SPECIATION;
         } else {
             // LineageInteraction from the signs (event.rs:186-229)
-            const bool prior_pos = (bits(e.prior_time) >> 63) == 0, event_pos = (bits(e.event_time) >> 63) == 0;
+            const bool prior_pos = (e.prior >> 63) == 0, event_pos = (e.time >> 63) == 0;
             if (prior_pos && !event_pos) {
                 which = EC_LATE_DISPERSAL_COALESCENCE; // Maybe
             } else {
-                const bool self = e.ox == e.tx && e.oy == e.ty && e.oindex == e.tindex;
+                const bool self = e.ox == e.tx && e.oy == e.ty && e.oi == e.ti;
                 const bool coal = prior_pos && event_pos;
                 which = self ? (coal ? EC_SELF_COALESCENCE : EC_SELF_DISPERSAL)
                              : (coal ? EC_OUT_COALESCENCE : EC_OUT_DISPERSAL);
