@@ -262,8 +262,9 @@ def dispersal_run(ctx, tables, cogs, lineages, args, window, drain_window, seed=
     event_slice / sum(lambda), and each round's events are sorted on the
     device.  Three ways to consume them: the speciation-only run as the
     yardstick, the device EventCounterReporter (events stay in HBM; each
-    round is one hipGraph of step kernel, sort and count), and draining every
-    sorted event to the host (bounded by the device-to-host copy)."""
+    round's step kernel is one hipGraph, then the sort to a permutation and
+    the count through it), and draining every sorted event to the host
+    (bounded by the device-to-host copy)."""
     import dataclasses
     gargs = dataclasses.replace(args, use_graph=True)
     out = {"window": window, "host_drain_window": drain_window}
