@@ -223,6 +223,12 @@ __device__ __forceinline__ unsigned long long wave_claim(WaveQueue &q, uint64_t 
             g = g < chunk ? g : chunk;
             take = g > idle ? g : idle;
         }
+        // A sparse round spreads its lineages over every wave: the lanes of
+        // a wave step in lockstep, so lineages sharing a wave add up their
+        // dependent chains, while a wave with one lineage runs at the
+        // latency of one chain.
+        const unsigned long long per_wave = (n_total + grid_waves() - 1) / grid_waves();
+        if (per_wave < take) take = per_wave > 0 ? per_wave : 1;
         unsigned long long base = 0;
         if (lane == leader) base = atomicAdd(head, take);
         base = __shfl(base, leader);
