@@ -1440,14 +1440,36 @@ constexpr uint64_t TIE_WINDOW = 896;
 constexpr int TIE_SORT_N = 2048; // >= TIE_WINDOW + TIE_RUN_MAX - 1, a power of two
 constexpr int TIE_THREADS = 512;
 
-// (prior, reference) of every event in time order, into scratch; flags runs
-// whose members differ in origin or target (flags[0]) and runs longer than
-// TIE_RUN_MAX (flags[1], checked at each run's head)
+// An event's class for the device EventCounterReporter: bit 0 prior time
+// negative, bit 1 event time negative, bit 2 origin == target.
+__device__ __forceinline__ uint8_t event_class(const DevPacked &e) {
+    const bool self = e.ox == e.tx && e.oy == e.ty && e.oindex == e.tindex;
+    return (uint8_t)(((bits(e.prior_time) >> 63) ? 1 : 0) | ((bits(e.event_time) >> 63) ? 2 : 0) | (self ? 4 : 0));
+}
+__device__ __forceinline__ uint8_t event_class(const DevEvent &) { return 0; }
+
+// The counter an event that is not a duplicate adds to (event_counter.rs:
+// 58-109 and the LineageInteraction of the signs, event.rs:186-229)
+__device__ __forceinline__ int first_event_counter(bool speciation, bool prior_pos, bool event_pos, bool self) {
+    if (speciation) return EC_SPECIATION;
+    if (prior_pos && !event_pos) return EC_LATE_DISPERSAL_COALESCENCE; // Maybe
+    const bool coal = prior_pos && event_pos;
+    return self ? (coal ? EC_SELF_COALESCENCE : EC_SELF_DISPERSAL) : (coal ? EC_OUT_COALESCENCE : EC_OUT_DISPERSAL);
+}
+__device__ __forceinline__ int first_event_counter(uint8_t cls) {
+    const bool prior_neg = cls & 1, event_neg = cls & 2;
+    return first_event_counter(prior_neg && !event_neg, !prior_neg, !event_neg, (cls & 4) != 0);
+}
+
+// (prior, reference) of every event in time order, into scratch, and (with
+// cls_ws) its counter class; flags runs whose members differ in origin or
+// target (flags[0]) and runs longer than TIE_RUN_MAX (flags[1], checked at
+// each run's head)
 template <class E>
 __global__ void __launch_bounds__(KS_THREADS) k_tie_gather(const E *ev, const unsigned long long *keys,
                                                            const uint32_t *vals, uint64_t n,
                                                            unsigned long long *prior_ws, unsigned long long *ref_ws,
-                                                           unsigned long long *flags) {
+                                                           uint8_t *cls_ws, unsigned long long *flags) {
     using O = EventOrder<E>;
     constexpr int NF = O::N - 3; // the origin / target fields 2 .. N - 2 (32-bit values)
     // each event's origin / target fields, for the thread that compares the
@@ -1465,6 +1487,7 @@ __global__ void __launch_bounds__(KS_THREADS) k_tie_gather(const E *ev, const un
             const E e = ev[vals[p]];
             prior_ws[p] = O::field(e, 1);
             ref_ws[p] = O::field(e, 0);
+            if (cls_ws) cls_ws[p] = event_class(e);
 #pragma unroll
             for (int k = 0; k < NF; k++) f[k] = s_f[j % 3][threadIdx.x][k] = (uint32_t)O::field(e, 2 + k);
         }
@@ -1514,14 +1537,27 @@ __device__ __forceinline__ uint64_t block_first_head(const unsigned long long *k
 // Sorts the window's runs by (time key, prior, reference, time-sorted
 // position) with a bitonic sort in LDS over the next power of two >= the
 // window's span and writes their permutation; a window without equal times
-// is copied.  Unused event slots (ghost keys) sort after every event and
-// count as runs of one.
+// keeps its order.  Unused event slots (ghost keys) sort after every event
+// and count as runs of one.
+// With `c` (a device_event_counter round of uniform runs), the window also
+// counts its events in the sorted order, as k_count_events would: a run's
+// first event adds to its class's counter, every later one is a duplicate
+// and a late coalescence when its prior time differs from the previous
+// event's (event_counter.rs:58-109).  Events [0, n_real) are real.  A run
+// whose events differ in class sets c->round_mixed, and the round is then
+// counted through the permutation instead.
 __global__ void __launch_bounds__(TIE_THREADS) k_tie_sort_windows(const unsigned long long *keys, const uint32_t *vals,
                                                                   uint64_t n, const unsigned long long *prior_ws,
                                                                   const unsigned long long *ref_ws,
-                                                                  uint32_t *out_vals) {
+                                                                  uint32_t *out_vals, const uint8_t *cls_ws,
+                                                                  unsigned long long n_real, EventCounts *c) {
     __shared__ unsigned long long s_t[TIE_SORT_N], s_p[TIE_SORT_N], s_r[TIE_SORT_N];
+    // window position (11 bits) and, counting, the event's class (bits 11-13):
+    // one array keeps the LDS at three workgroups per CU
     __shared__ uint16_t s_i[TIE_SORT_N];
+    __shared__ unsigned long long s_cnt[EC_COUNTERS];
+    constexpr uint16_t POS = 0x7ff;
+    static_assert(TIE_SORT_N <= POS + 1, "window positions in 11 bits");
     const uint64_t lo = (uint64_t)blockIdx.x * TIE_WINDOW;
     if (lo >= n) return;
     const uint64_t hi = lo + TIE_WINDOW < n ? lo + TIE_WINDOW : n;
@@ -1543,38 +1579,68 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_sort_windows(const unsigned
             s_p[i] = ~0ull;
             s_r[i] = ~0ull;
         }
-        s_i[i] = (uint16_t)i;
+        s_i[i] = (uint16_t)(i | (c && i < len ? (int)cls_ws[s + i] << 11 : 0));
     }
+    if (c && threadIdx.x < EC_COUNTERS) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     // a window without equal times keeps its order
     int tie = 0;
     for (int i = threadIdx.x + 1; i < len; i += TIE_THREADS)
         tie |= (s_t[i] == s_t[i - 1] && s_t[i] != GHOST_TIME_BITS) ? 1 : 0;
-    if (!__syncthreads_or(tie)) {
-        for (int i = threadIdx.x; i < len; i += TIE_THREADS) out_vals[s + i] = vals[s + i];
-        return;
-    }
-    // bitonic network: in stage (k, j), pair q compares i and i + j
-    for (int k = 2; k <= nsort; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int q = threadIdx.x; q < nsort / 2; q += TIE_THREADS) {
-                const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
-                const int l = i + j;
-                const bool ascending = (i & k) == 0;
-                const unsigned long long ti = s_t[i], tl = s_t[l], pi = s_p[i], pl = s_p[l], ri = s_r[i], rl = s_r[l];
-                const uint16_t ii = s_i[i], il = s_i[l];
-                const bool l_less = tl < ti || (tl == ti && (pl < pi || (pl == pi && (rl < ri || (rl == ri && il < ii)))));
-                if (l_less == ascending) {
-                    s_t[i] = tl; s_t[l] = ti;
-                    s_p[i] = pl; s_p[l] = pi;
-                    s_r[i] = rl; s_r[l] = ri;
-                    s_i[i] = il; s_i[l] = ii;
+    if (__syncthreads_or(tie)) {
+        // bitonic network: in stage (k, j), pair q compares i and i + j
+        for (int k = 2; k <= nsort; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int q = threadIdx.x; q < nsort / 2; q += TIE_THREADS) {
+                    const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                    const int l = i + j;
+                    const bool ascending = (i & k) == 0;
+                    const unsigned long long ti = s_t[i], tl = s_t[l], pi = s_p[i], pl = s_p[l], ri = s_r[i],
+                                             rl = s_r[l];
+                    const uint16_t ii = s_i[i], il = s_i[l];
+                    const bool l_less =
+                        tl < ti ||
+                        (tl == ti && (pl < pi || (pl == pi && (rl < ri || (rl == ri && (il & POS) < (ii & POS))))));
+                    if (l_less == ascending) {
+                        s_t[i] = tl; s_t[l] = ti;
+                        s_p[i] = pl; s_p[l] = pi;
+                        s_r[i] = rl; s_r[l] = ri;
+                        s_i[i] = il; s_i[l] = ii;
+                    }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
-    for (int i = threadIdx.x; i < len; i += TIE_THREADS) out_vals[s + i] = vals[s + s_i[i]];
+    for (int i = threadIdx.x; i < len; i += TIE_THREADS) out_vals[s + i] = vals[s + (s_i[i] & POS)];
+    if (!c) return;
+    // the events of the span in sorted order; position 0 (s) starts a run
+    const int real = n_real > s ? (int)(n_real - s < (uint64_t)len ? n_real - s : (uint64_t)len) : 0;
+    unsigned long long cnt[EC_COUNTERS];
+#pragma unroll
+    for (int k = 0; k < EC_COUNTERS; k++) cnt[k] = 0;
+    bool mixed = false;
+    for (int i = threadIdx.x; i < real; i += TIE_THREADS) {
+        const uint8_t ci = (uint8_t)(s_i[i] >> 11);
+        cnt[EC_RAW_TOTAL]++;
+        if (i == 0 || s_t[i - 1] != s_t[i]) {
+            const int w = first_event_counter(ci);
+#pragma unroll
+            for (int k = EC_SPECIATION; k < EC_COUNTERS; k++) cnt[k] += k == w ? 1 : 0;
+        } else {
+            mixed |= (s_i[i - 1] >> 11) != ci;
+            cnt[EC_LATE_COALESCENCE] += s_p[i - 1] != s_p[i] ? 1 : 0;
+        }
+        if (s + (uint64_t)i == n_real - 1) c->round_carry_prior = s_p[i];
+    }
+#pragma unroll
+    for (int k = 0; k < EC_COUNTERS; k++) {
+        unsigned long long v = cnt[k];
+        for (int off = 32; off; off >>= 1) v += __shfl_xor(v, off);
+        if (lane_id() == 0 && v) atomicAdd(&s_cnt[k], v);
+    }
+    if (__syncthreads_or(mixed ? 1 : 0) && threadIdx.x == 0) c->round_mixed = 1;
+    if (threadIdx.x < EC_COUNTERS && s_cnt[threadIdx.x]) atomicAdd(&c->round_count[threadIdx.x], s_cnt[threadIdx.x]);
 }
 
 template <class E>
@@ -1977,8 +2043,14 @@ static int highest_digit(unsigned long long diff) {
 // sorted permutation (one of sb.vals); `scratch` (n events) holds the runs'
 // prior times and references.
 template <class E>
+// With `counts` (PackedEvents of a device_event_counter round, n_real of the
+// n records real), a round whose runs of equal times go through the LDS
+// windows is counted there, into counts->round_*, and *counted is set; the
+// caller then commits it (k_count_finish).
 static const uint32_t *sort_permutation(const E *events, uint64_t n, E *scratch, SortBuffers &sb,
-                                        hipStream_t stream, bool fixed_passes) {
+                                        hipStream_t stream, bool fixed_passes, EventCounts *counts = nullptr,
+                                        uint64_t n_real = 0, bool *counted = nullptr) {
+    if (counted) *counted = false;
     using O = EventOrder<E>;
     SortSize sz{n, nullptr, nullptr, nullptr};
     unsigned g = grid_for(n, 256), gk = grid_for(n, KS_TILE);
@@ -2012,16 +2084,19 @@ static const uint32_t *sort_permutation(const E *events, uint64_t n, E *scratch,
             // rank each run of equal times in place (prior / reference in `scratch`)
             unsigned long long *prior_ws = reinterpret_cast<unsigned long long *>(scratch);
             unsigned long long *ref_ws = prior_ws + n;
-            static_assert(sizeof(E) >= 16, "scratch of two words per event");
+            uint8_t *cls_ws = counts ? reinterpret_cast<uint8_t *>(ref_ws + n) : nullptr;
+            static_assert(sizeof(E) >= 17, "scratch of two words and a byte per event");
             static_assert(MM_TIE_LONG == MM_TIE_MIXED + 1, "tie flags are adjacent");
             hipLaunchKernelGGL(k_tie_gather<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sb.keys[cur],
-                               sb.vals[cur], n, prior_ws, ref_ws, sb.minmax + MM_TIE_MIXED);
+                               sb.vals[cur], n, prior_ws, ref_ws, cls_ws, sb.minmax + MM_TIE_MIXED);
             (void)hipMemcpyAsync(&mm[MM_TIE_MIXED], sb.minmax + MM_TIE_MIXED, 2 * sizeof(unsigned long long),
                                  hipMemcpyDeviceToHost, stream);
             (void)hipStreamSynchronize(stream);
             if (!mm[MM_TIE_MIXED] && !mm[MM_TIE_LONG]) {
                 hipLaunchKernelGGL(k_tie_sort_windows, dim3(grid_for(n, TIE_WINDOW)), dim3(TIE_THREADS), 0, stream,
-                                   sb.keys[cur], sb.vals[cur], n, prior_ws, ref_ws, sb.vals[cur ^ 1]);
+                                   sb.keys[cur], sb.vals[cur], n, prior_ws, ref_ws, sb.vals[cur ^ 1],
+                                   (const uint8_t *)cls_ws, (unsigned long long)n_real, counts);
+                if (counted) *counted = counts != nullptr;
                 return sb.vals[cur ^ 1];
             }
         }
@@ -2133,7 +2208,9 @@ __device__ __forceinline__ bool rec_same_key(const CountRec &a, const CountRec &
 // read) and shares it in LDS: the previous events in sorted order come from
 // the neighbouring threads, and from memory only across the block's start.
 __global__ void __launch_bounds__(256) k_count_events(const DevPacked *ev, const uint32_t *perm, SortSize sz,
-                                                     const unsigned long long *ghosts, EventCounts *c) {
+                                                     const unsigned long long *ghosts, EventCounts *c,
+                                                     const unsigned long long *gate) {
+    if (gate && !*gate) return; // counted in the sort's windows
     // the k-th event in sorted order: ev[k], or ev[perm[k]] from the sort's permutation
     auto at = [&](uint64_t k) -> CountRec { return count_rec(ev[perm ? perm[k] : k]); };
     __shared__ unsigned long long s_cnt[EC_COUNTERS];
@@ -2189,7 +2266,8 @@ __global__ void __launch_bounds__(256) k_count_events(const DevPacked *ev, const
 
 // after the round's count: the last prior time for the next round
 __global__ void k_count_carry(const DevPacked *ev, const uint32_t *perm, SortSize sz,
-                              const unsigned long long *ghosts, EventCounts *c) {
+                              const unsigned long long *ghosts, EventCounts *c, const unsigned long long *gate) {
+    if (gate && !*gate) return;
     const uint64_t n = sz.get() - (ghosts ? *ghosts : 0ull);
     if (n) {
         c->carry_has = 1;
@@ -2202,17 +2280,42 @@ void launch_count_events(const DevPacked *ev, const uint32_t *perm, const unsign
     if (n_or_cap == 0) return;
     SortSize sz{n_or_cap, end, nullptr, nullptr};
     hipLaunchKernelGGL(k_count_events, dim3(grid_for(n_or_cap, 256)), dim3(256), 0, stream, ev, perm, sz, ghosts,
-                       c);
-    hipLaunchKernelGGL(k_count_carry, dim3(1), dim3(1), 0, stream, ev, perm, sz, ghosts, c);
+                       c, (const unsigned long long *)nullptr);
+    hipLaunchKernelGGL(k_count_carry, dim3(1), dim3(1), 0, stream, ev, perm, sz, ghosts, c,
+                       (const unsigned long long *)nullptr);
 }
+
+// commits a round counted in the sort's windows, unless a run mixed classes
+__global__ void k_count_finish(EventCounts *c) {
+    if (!c->round_mixed) {
+        for (int k = 0; k < EC_COUNTERS; k++) c->count[k] += c->round_count[k];
+        c->carry_has = 1;
+        c->carry_prior = c->round_carry_prior;
+    }
+    for (int k = 0; k < EC_COUNTERS; k++) c->round_count[k] = 0;
+}
+__global__ void k_count_gate_clear(EventCounts *c) { c->round_mixed = 0; }
 
 // A round of a device_event_counter job: the events' sorted permutation, then
 // the counters read the events through it (no sorted copy is written)
 void sort_and_count_packed(const DevPacked *events, uint64_t slots, uint64_t n_events, DevPacked *scratch,
                            SortBuffers &sb, EventCounts *c, hipStream_t stream) {
     if (slots == 0 || n_events == 0) return;
-    const uint32_t *perm = sort_permutation(events, slots, scratch, sb, stream, false);
-    launch_count_events(events, perm, nullptr, n_events, nullptr, c, stream);
+    bool counted = false;
+    const uint32_t *perm = sort_permutation(events, slots, scratch, sb, stream, false, c, n_events, &counted);
+    if (!counted) {
+        launch_count_events(events, perm, nullptr, n_events, nullptr, c, stream);
+        return;
+    }
+    // counted in the windows; a round with a run of mixed classes (gate set)
+    // is counted again through the permutation
+    hipLaunchKernelGGL(k_count_finish, dim3(1), dim3(1), 0, stream, c);
+    SortSize sz{n_events, nullptr, nullptr, nullptr};
+    hipLaunchKernelGGL(k_count_events, dim3(grid_for(n_events, 256)), dim3(256), 0, stream, events, perm, sz,
+                       (const unsigned long long *)nullptr, c, (const unsigned long long *)&c->round_mixed);
+    hipLaunchKernelGGL(k_count_carry, dim3(1), dim3(1), 0, stream, events, perm, sz,
+                       (const unsigned long long *)nullptr, c, (const unsigned long long *)&c->round_mixed);
+    hipLaunchKernelGGL(k_count_gate_clear, dim3(1), dim3(1), 0, stream, c);
 }
 
 uint32_t soa_compact_blocks(uint64_t n) { return (uint32_t)((n + SC_TILE - 1) / SC_TILE); }
