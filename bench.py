@@ -187,6 +187,31 @@ def map_window_run(ctx, size, nu, window, dedup, seed=42):
                                 "steps_per_s_frac_of_roofline": round(o.steps / (ms * 1e-3) / bound, 5)}}
 
 
+def chain_latency_run(ctx, window=20000.0):
+    """The latency of dependent steps: lineages that never speciate in the
+    window (almost-infinite sigma 50, nu 1e-9) run window / 2 steps each; one
+    lineage, and 64 (which sparse rounds spread one per wave), per form."""
+    out = {"window": window}
+    for n_lin, radius in ((1, 0), (64, 5)):
+        cogs = nh.AlmostInfiniteNormalDispersalScenario(radius, 50.0).new(1e-9)
+        tables = nh.Tables(ctx, cogs)
+        lin = cogs.sample_habitat(nh.OriginPreSampler.all())[:n_lin]
+        for name, kw in (("persistent", {}), ("persistent_64_thread_blocks", dict(block_size=64, blocks_per_cu=1)),
+                         ("plain_batched", dict(form=nh.PLAIN)), ("plain", dict(form=nh.PLAIN, plain_batch=1))):
+            args = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(),
+                                           event_slice=nh.EventSlice.Absolute(10**9), **kw)
+            job = make_job(ctx, tables, cogs, lin, args, 5, window)
+            t = time.perf_counter()
+            run_collect(job)
+            wall = time.perf_counter() - t
+            o = job.outcome()
+            job.close()
+            per = o.steps / n_lin
+            out[f"{n_lin}_lineages_{name}"] = {"steps": int(o.steps), "ms": wall * 1e3,
+                                               "us_per_dependent_step": wall / per * 1e6}
+    return out
+
+
 def config3_run(ctx, window):
     """config 3 (128x128 map, nu 1e-5, the reference's default dedup)"""
     return map_window_run(ctx, 128, 1e-5, window, nh.DedupCache.Relative(1.0))
@@ -766,7 +791,13 @@ def main():
         del tables
         extra["config2_full_run"] = config2_run(ctx)
         log(f"config 2: {extra['config2_full_run']}")
+        extra["chain_latency"] = chain_latency_run(ctx)
+        log(f"chain latency: {extra['chain_latency']}")
         extra["config3_window"] = config3_run(ctx, 2000.0)
+        # the same map without dedup: the gathers from its tables (61 MB,
+        # resident in the 256 MB last-level cache) over the whole window
+        extra["config3_gather_window"] = map_window_run(ctx, 128, 1e-5, 2000.0, nh.DedupCache.none())
+        log(f"config 3 without dedup: {extra['config3_gather_window']}")
         log(f"config 3: {extra['config3_window']}")
         extra["gather_window"] = gather_run(ctx, 2000.0)
         log(f"gather: {extra['gather_window']}")
