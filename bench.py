@@ -50,6 +50,9 @@ GATHER_PROFILE = os.path.join(ROOT, "profiles", "k_persistent_alias_gather_map25
 LINEAGE_STATE_BYTES = 45
 MOVING_STATE_BYTES = LINEAGE_STATE_BYTES - 8  # all but the reference, which never changes
 EVENT_BYTES = 40
+# lineages per wave of the pooled kernel in the dispersal-reporting rounds
+# (128 and 192 measured equal)
+DISPERSAL_POOL = 128
 CACHE_BUCKET_BYTES = 32
 
 
@@ -573,8 +576,9 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--inner-steps", type=int, default=0)
     ap.add_argument("--exec-fill", type=int, default=0)
-    ap.add_argument("--pool", type=int, default=0,
-                    help="pooled kernel with this many lineages per wave (128, 192; skips the autotuner)")
+    ap.add_argument("--pool", type=int, default=-1,
+                    help="pooled kernel with this many lineages per wave (128, 192; skips the autotuner); "
+                         "0 = the one-lineage-per-lane kernel")
     ap.add_argument("--graph", action="store_true",
                     help="capture each water-level round (kernel + event sort) in a hipGraph; the "
                          "kernel timing then covers the whole graph")
@@ -613,7 +617,7 @@ def main():
         tables = nh.Tables(ctx, cogs)
         dargs = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), block_size=a.block_size,
                                         blocks_per_cu=a.blocks_per_cu, inner_steps=a.inner_steps,
-                                        exec_fill=a.exec_fill)
+                                        exec_fill=a.exec_fill, pool=DISPERSAL_POOL if a.pool < 0 else a.pool)
         r = dispersal_run(ctx, tables, cogs, lineages, dargs, a.dispersal_window, a.drain_window)
         print(json.dumps({"part": "dispersal", "dispersal_windows": r, "copy_kernel": copy}), flush=True)
         return
@@ -632,11 +636,12 @@ def main():
         print(json.dumps({"part": "tail", "config4_dedup_window": dd, "config4_tail": tail}), flush=True)
         return
     form = nh.PERSISTENT if a.form == "persistent" else nh.PLAIN
-    fixed = a.block_size or a.blocks_per_cu or a.inner_steps or a.exec_fill or a.pool
+    pool = max(a.pool, 0)
+    fixed = a.block_size or a.blocks_per_cu or a.inner_steps or a.exec_fill or pool
     args = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=form,
                                    use_graph=a.graph and form == nh.PERSISTENT,
                                    block_size=a.block_size, blocks_per_cu=a.blocks_per_cu,
-                                   inner_steps=a.inner_steps, exec_fill=a.exec_fill, pool=a.pool)
+                                   inner_steps=a.inner_steps, exec_fill=a.exec_fill, pool=pool)
     tuned = None
     if not (a.no_tune or fixed or form == nh.PLAIN):
         t = time.perf_counter()
@@ -783,9 +788,10 @@ def main():
             extra["config4_tail"] = config4_tail_run(ctx, tables, cogs, lineages, dtuned,
                                                      extra["config4_dedup_window"], a.tail_window,
                                                      completion=not a.no_completion)
-        # dispersal reporting runs the one-lineage-per-lane kernel: the
-        # pooled kernel's shape does not carry over
-        dargs = args if not args.pool else dataclasses.replace(args, pool=0, block_size=0, blocks_per_cu=0)
+        # dispersal reporting runs the pooled kernel at full residency (its
+        # lineages enter and leave the pool every two steps or so)
+        dargs = dataclasses.replace(args, pool=DISPERSAL_POOL, block_size=0, blocks_per_cu=0, inner_steps=0,
+                                    exec_fill=0)
         extra["dispersal_reporting"] = dispersal_run(ctx, tables, cogs, lineages, dargs, a.dispersal_window,
                                                      a.drain_window)
         del tables

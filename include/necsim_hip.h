@@ -198,8 +198,8 @@ typedef struct {
                                       2 = on.  Results do not depend on it. */
     uint32_t task_list_size;       /* NH_FORM_TASK_LIST: lineages per launch; 0 = 4096, the reference
                                       CUDA algorithm's default grid x block of 64 x 64 */
-    uint32_t pool;                 /* persistent form without dedup cache or dispersal reporting:
-                                      lineages per wave kept in LDS by the pooled kernel, whose lanes
+    uint32_t pool;                 /* persistent form without dedup cache: lineages per wave kept
+                                      in LDS by the pooled kernel, whose lanes
                                       take whichever of them needs a search step or an execution
                                       (128 or 192; one-wave blocks, blocks_per_cu counts waves);
                                       0 = the one-lineage-per-lane kernel.  Results do not depend on it. */
@@ -362,8 +362,8 @@ int nh_job_kernel_time(const nh_job *job, double *total_ms, uint64_t *launches);
  * block size (128/256), resident blocks per CU (occupancy limit and half of
  * it), iterations per lane between refills (4, 16), execution fill (1, 5, 8
  * eighths) and, when base->step_slice == 0, step slice (5, 10, 20, 50, 150);
- * without dedup cache or dispersal reporting also the pooled kernel with 128
- * and 192 lineages per wave; and writes the fastest combination (least time
+ * without dedup cache also the pooled kernel with 128 and 192 lineages per
+ * wave; and writes the fastest combination (least time
  * for the window) to *tuned (a copy of *base with block_size, blocks_per_cu,
  * inner_steps, exec_fill, step_slice and pool set).
  * *best_ms receives the winner's window time. */

@@ -140,7 +140,7 @@ int serve_max_blocks_per_cu(int kind, int threads);
 void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream);
 int persistent_max_blocks_per_cu(int kind, int threads, bool dedup, bool report_dispersal, bool coop);
 // resident one-wave blocks per CU of the pooled kernel with `pool` lineages per wave
-int pooled_max_blocks_per_cu(int kind, unsigned pool);
+int pooled_max_blocks_per_cu(int kind, unsigned pool, bool report_dispersal);
 void launch_plain(int kind, const PlainArgs &a, int threads, hipStream_t stream);
 
 // m: pass size, or with m_dev the launch bound of a pass whose size is *m_dev

@@ -507,8 +507,8 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
 }
 
 // ---------------------------------------------------------------------------
-// Pooled persistent round kernel (dense rounds: no dedup cache, speciation
-// events only).  Each wave (one per block) keeps POOL lineages in LDS,
+// Pooled persistent round kernel (dense rounds: no dedup cache).  Each wave
+// (one per block) keeps POOL lineages in LDS,
 // POOL / 64 per lane, and works in two phases per iteration:
 //   search:  up to 64 of the lineages still looking for their next event
 //            each advance by one lane_search_step, one lineage per lane;
@@ -521,7 +521,11 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
 // lane whose lineage still searches idles through it.  Each lineage makes
 // exactly the computations of k_persistent, so the event stream is the same.
 // Entries are owned by lanes for bookkeeping (entry lane + 64 j), refilled
-// from the same guided work queue.
+// from the same guided work queue.  The DISP variant reports every executed
+// step as a PackedEvent (dispersal reporting): a dispersal-reporting round
+// runs each lineage for about two steps, so lineages enter and leave the
+// pool all the time, which the one-lineage-per-lane kernel pays for with
+// idle lanes.
 constexpr int POOL_MAX = 192;
 enum : uint8_t { PE_EMPTY = 0, PE_SEARCH = 1, PE_READY = 2 };
 constexpr uint32_t PE_NOT_STARTED = 0xffffffffu; // win_left of a search that has not opened a window
@@ -559,12 +563,18 @@ __device__ __forceinline__ uint32_t pool_pick(const uint64_t (&sel)[NW], int fir
     return base < 64 ? base : 64;
 }
 
-template <int KIND, int POOL>
+template <int KIND, int POOL, bool DISP>
 __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
     static_assert(POOL % 64 == 0 && POOL <= POOL_MAX, "pool of whole words");
     constexpr int NW = POOL / 64;
     __shared__ PoolLds<POOL> p;
+    __shared__ unsigned long long s_slots[2]; // DISP: the wave's event chunk [next, end)
+    volatile unsigned long long *ws = s_slots;
     const unsigned lane = lane_id();
+    if (DISP && lane == 0) {
+        ws[0] = 0;
+        ws[1] = 0;
+    }
     if (lane < 3 * PTBL) p.tab[lane] = A.P.ratio[RATIO_N + lane];
 #pragma unroll
     for (int j = 0; j < NW; j++) p.state[lane + 64 * j] = PE_EMPTY;
@@ -681,7 +691,20 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
             L.next_st = p.st[e];
             L.have_next = true;
             if (is_cell_kind(KIND) && KIND != K_NON_SPATIAL) L.rec = A.S.cells[L.a];
-            if (L.next_t >= level) { // the water level: back to the lineage store
+            bool leave = L.next_t >= level; // the water level: back to the lineage store
+            unsigned long long eslot = 0;
+            Origin o;
+            if (DISP && !leave) {
+                eslot = wave_take_slot(ws, &A.ctr->n_events);
+                if (eslot >= A.packed_cap) { // the round's buffer is full: not executed
+                    A.ctr->ev_overflow = 1;
+                    leave = true;
+                } else {
+                    L.ref = A.lin.ref[slot];
+                    o = lane_origin<KIND>(L, A.S);
+                }
+            }
+            if (leave) {
                 lane_store<KIND>(L, A.S, A.lin, slot);
                 p.state[e] = PE_EMPTY;
             } else {
@@ -691,9 +714,12 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
                 const int r = lane_execute<KIND, false, true>(L, A.S, A.P, ev);
                 my_steps++;
                 my_max = L.t_last > my_max ? L.t_last : my_max;
+                if (DISP) emit_packed<KIND>(A.packed + eslot, o, L, A.S, r == STEP_SPECIATED);
                 if (r == STEP_SPECIATED) {
-                    ev.ref = A.lin.ref[slot];
-                    A.events[atomicAdd(&A.ctr->n_events, 1ull)] = ev;
+                    if (!DISP) {
+                        ev.ref = A.lin.ref[slot];
+                        A.events[atomicAdd(&A.ctr->n_events, 1ull)] = ev;
+                    }
                     A.lin.flags[slot] = FLAG_DEAD;
                     my_dead++;
                     p.state[e] = PE_EMPTY;
@@ -710,6 +736,7 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
         }
         wave_sync_lds();
     }
+    if (DISP) wave_fill_ghosts(ws, A.packed, A.packed_cap, &A.ctr->n_ghost);
     block_flush(my_steps, my_max, my_dead, A.ctr);
 }
 
@@ -1765,36 +1792,44 @@ static void launch_persistent_t(int kind, const PersistArgs &a, int blocks, int 
     }
 }
 
-template <int POOL> static void launch_pooled_t(int kind, const PersistArgs &a, int blocks, hipStream_t stream) {
+template <int POOL, bool DISP>
+static void launch_pooled_t(int kind, const PersistArgs &a, int blocks, hipStream_t stream) {
     switch (kind) {
-    case K_NON_SPATIAL: hipLaunchKernelGGL((k_pooled<K_NON_SPATIAL, POOL>), dim3(blocks), dim3(64), 0, stream, a); break;
-    case K_ALMOST_INFINITE: hipLaunchKernelGGL((k_pooled<K_ALMOST_INFINITE, POOL>), dim3(blocks), dim3(64), 0, stream, a); break;
-    case K_SEPARABLE: hipLaunchKernelGGL((k_pooled<K_SEPARABLE, POOL>), dim3(blocks), dim3(64), 0, stream, a); break;
-    default: hipLaunchKernelGGL((k_pooled<K_PACKED, POOL>), dim3(blocks), dim3(64), 0, stream, a); break;
+    case K_NON_SPATIAL: hipLaunchKernelGGL((k_pooled<K_NON_SPATIAL, POOL, DISP>), dim3(blocks), dim3(64), 0, stream, a); break;
+    case K_ALMOST_INFINITE: hipLaunchKernelGGL((k_pooled<K_ALMOST_INFINITE, POOL, DISP>), dim3(blocks), dim3(64), 0, stream, a); break;
+    case K_SEPARABLE: hipLaunchKernelGGL((k_pooled<K_SEPARABLE, POOL, DISP>), dim3(blocks), dim3(64), 0, stream, a); break;
+    default: hipLaunchKernelGGL((k_pooled<K_PACKED, POOL, DISP>), dim3(blocks), dim3(64), 0, stream, a); break;
     }
 }
 
-template <int POOL> static int pooled_max_blocks_per_cu_t(int kind) {
+template <int POOL, bool DISP> static int pooled_max_blocks_per_cu_t(int kind) {
     int n = 0;
     switch (kind) {
-    case K_NON_SPATIAL: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pooled<K_NON_SPATIAL, POOL>, 64, 0); break;
-    case K_ALMOST_INFINITE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pooled<K_ALMOST_INFINITE, POOL>, 64, 0); break;
-    case K_SEPARABLE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pooled<K_SEPARABLE, POOL>, 64, 0); break;
-    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pooled<K_PACKED, POOL>, 64, 0); break;
+    case K_NON_SPATIAL: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pooled<K_NON_SPATIAL, POOL, DISP>, 64, 0); break;
+    case K_ALMOST_INFINITE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pooled<K_ALMOST_INFINITE, POOL, DISP>, 64, 0); break;
+    case K_SEPARABLE: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pooled<K_SEPARABLE, POOL, DISP>, 64, 0); break;
+    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pooled<K_PACKED, POOL, DISP>, 64, 0); break;
     }
     return n > 0 ? n : 1;
 }
 
-int pooled_max_blocks_per_cu(int kind, unsigned pool) {
-    return pool == 192 ? pooled_max_blocks_per_cu_t<192>(kind) : pooled_max_blocks_per_cu_t<128>(kind);
+int pooled_max_blocks_per_cu(int kind, unsigned pool, bool report_dispersal) {
+    if (report_dispersal)
+        return pool == 192 ? pooled_max_blocks_per_cu_t<192, true>(kind) : pooled_max_blocks_per_cu_t<128, true>(kind);
+    return pool == 192 ? pooled_max_blocks_per_cu_t<192, false>(kind) : pooled_max_blocks_per_cu_t<128, false>(kind);
 }
 
 // dispersal reporting has no cooperative variant (its rounds are dense); the
 // pooled kernel runs dense rounds without dedup cache (one-wave blocks)
 void launch_persistent(int kind, const PersistArgs &a, int blocks, int threads, hipStream_t stream) {
-    if (a.pool && !a.packed && !a.cache && !a.coop) {
-        if (a.pool == 192) launch_pooled_t<192>(kind, a, blocks, stream);
-        else launch_pooled_t<128>(kind, a, blocks, stream);
+    if (a.pool && !a.cache && !a.coop) {
+        if (a.packed) {
+            if (a.pool == 192) launch_pooled_t<192, true>(kind, a, blocks, stream);
+            else launch_pooled_t<128, true>(kind, a, blocks, stream);
+        } else {
+            if (a.pool == 192) launch_pooled_t<192, false>(kind, a, blocks, stream);
+            else launch_pooled_t<128, false>(kind, a, blocks, stream);
+        }
         return;
     }
     if (a.packed) {

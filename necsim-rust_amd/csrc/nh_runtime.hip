@@ -718,13 +718,12 @@ static int job_start(nh_job *j) {
             per_cu = persistent_max_blocks_per_cu(j->tables->kind, j->threads, j->cache_cap != 0, false, true);
             if (p.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)p.blocks_per_cu);
         }
-        // the pooled kernel: dense rounds without dedup cache or dispersal
-        // reporting, one wave per block
-        j->pooled = p.pool != 0 && j->cache_cap == 0 && !j->disp && p.coop_search != 2;
+        // the pooled kernel: dense rounds without dedup cache, one wave per block
+        j->pooled = p.pool != 0 && j->cache_cap == 0 && p.coop_search != 2;
         if (j->pooled) {
             j->coop = false;
             j->threads = 64;
-            per_cu = pooled_max_blocks_per_cu(j->tables->kind, p.pool);
+            per_cu = pooled_max_blocks_per_cu(j->tables->kind, p.pool, j->disp);
             if (p.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)p.blocks_per_cu);
         }
         j->blocks = per_cu * (int)j->ctx->cus;
@@ -1415,16 +1414,16 @@ extern "C" int nh_autotune(nh_ctx *ctx, const nh_tables *tables, const nh_job_pa
             }
         }
     }
-    // the pooled kernel (dense rounds without dedup cache or dispersal
-    // reporting): one-wave blocks at full residency, each pool size
-    if (base->dedup_mode == NH_DEDUP_NONE && !base->report_dispersal && base->coop_search != 2) {
+    // the pooled kernel (dense rounds without dedup cache): one-wave blocks
+    // at full residency, each pool size
+    if (base->dedup_mode == NH_DEDUP_NONE && base->coop_search != 2) {
         for (uint32_t pool : {128u, 192u})
         for (uint64_t slice : slices) {
             nh_job_params p = *base;
             p.form = NH_FORM_PERSISTENT;
             p.pool = pool;
             p.block_size = 64;
-            p.blocks_per_cu = (uint32_t)pooled_max_blocks_per_cu(tables->kind, pool);
+            p.blocks_per_cu = (uint32_t)pooled_max_blocks_per_cu(tables->kind, pool, base->report_dispersal != 0);
             p.step_slice = slice;
             p.inner_steps = 16;
             p.exec_fill = 5;

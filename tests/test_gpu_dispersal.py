@@ -39,11 +39,11 @@ def _dedup(dedup):
 
 
 def hip_packed(cogs, lineages, seed, form, dedup="none", pause_before=None, step_slice=10,
-               event_slice=None, use_graph=False, resume_after=None):
+               event_slice=None, use_graph=False, resume_after=None, pool=0):
     dc, _ = _dedup(dedup)
     es = nh.EventSlice.Relative(2.0) if event_slice is None else event_slice
     args = nh.IndependentArguments(step_slice=step_slice, dedup_cache=dc, event_slice=es, form=form,
-                                   use_graph=use_graph)
+                                   use_graph=use_graph, pool=pool)
     rep = nh.PackedEventCollector()
     out = nh.HipIndependentAlgorithm.resume_and_simulate(args, nh.WyHash.seed_from_u64(seed), cogs, lineages,
                                                          resume_after=resume_after, pause_before=pause_before,
@@ -125,6 +125,15 @@ def test_dedup_none_bit_exact(case, form):
     assert_packed_exact(ref, out, ev, f"{name} form {form}")
 
 
+@pytest.mark.parametrize("pool", [128, 192])
+def test_pooled_dedup_none_bit_exact(case, pool):
+    """the pooled round kernel reporting every step: the same stream"""
+    name, cogs, orc, lin, seed, nu, pause, ref = case
+    announce(f"dispersal reporting {name} dedup None, pooled kernel {pool}")
+    out, ev = hip_packed(cogs, lin, seed, PERSISTENT, pause_before=pause, pool=pool)
+    assert_packed_exact(ref, out, ev, f"{name} pool {pool}")
+
+
 def test_speciations_equal_the_speciation_only_run(case):
     name, cogs, orc, lin, seed, nu, pause, ref = case
     announce(f"dispersal reporting {name}: speciation subset vs speciation-only run")
@@ -152,41 +161,42 @@ def test_dedup_plain_bit_exact_and_persistent_invariants(case):
     assert out.steps <= ref["steps"]
 
 
-def test_graph_replays_match_eager():
-    announce("dispersal reporting non-spatial 30x30, rounds replayed from hipGraphs")
+@pytest.mark.parametrize("pool", [0, 128])
+def test_graph_replays_match_eager(pool):
+    announce(f"dispersal reporting non-spatial 30x30, rounds replayed from hipGraphs, pool {pool}")
     name, cogs, orc, _, seed, nu, pause = _non_spatial()
     lin = orc.origins(1.0)
     ref = oracle_packed(orc, lin, seed, nu)
     few = nh.EventSlice.Relative(0.25)  # many rounds; the sorted stream does not depend on them
-    out, ev = hip_packed(cogs, lin, seed, PERSISTENT, event_slice=few, use_graph=True)
+    out, ev = hip_packed(cogs, lin, seed, PERSISTENT, event_slice=few, use_graph=True, pool=pool)
     assert out.rounds >= 20
     assert_packed_exact(ref, out, ev, "graph")
 
 
-@pytest.mark.parametrize("form", [PLAIN, PERSISTENT])
-def test_pause_resume_continuity(form):
-    announce(f"dispersal reporting pause at 150 and resume, form {form}")
+@pytest.mark.parametrize("form,pool", [(PLAIN, 0), (PERSISTENT, 0), (PERSISTENT, 128)])
+def test_pause_resume_continuity(form, pool):
+    announce(f"dispersal reporting pause at 150 and resume, form {form} pool {pool}")
     name, cogs, orc, _, seed, nu, _p = _non_spatial()
     lin = orc.origins(1.0)
     ref = oracle_packed(orc, lin, seed, nu)
-    out1, ev1 = hip_packed(cogs, lin, seed, form, pause_before=150.0)
+    out1, ev1 = hip_packed(cogs, lin, seed, form, pause_before=150.0, pool=pool)
     assert out1.paused and len(out1.lineages)
-    out2, ev2 = hip_packed(cogs, out1.lineages, seed, form, resume_after=150.0)
+    out2, ev2 = hip_packed(cogs, out1.lineages, seed, form, resume_after=150.0, pool=pool)
     assert not out2.paused
     both = np.concatenate([ev1, ev2])
     assert packed_bytes(both) == packed_bytes(ref["events"])
     assert out1.steps + out2.steps == ref["steps"]
 
 
-@pytest.mark.parametrize("form", [PLAIN, PERSISTENT])
-def test_event_buffer_overflow_is_an_error(ctx, form):
+@pytest.mark.parametrize("form,pool", [(PLAIN, 0), (PERSISTENT, 0), (PERSISTENT, 128)])
+def test_event_buffer_overflow_is_an_error(ctx, form, pool):
     """a round that reports more events than the buffer holds fails with
     NH_E_CAPACITY, writes nothing past the buffer and leaves the job failed"""
-    announce(f"dispersal event buffer of 64 entries, form {form}")
+    announce(f"dispersal event buffer of 64 entries, form {form} pool {pool}")
     cogs = nh.NonSpatialScenario((30, 30), 2).new(1e-2)
     lin = cogs.sample_habitat(nh.OriginPreSampler.all())
     tables = nh.Tables(ctx, cogs)
-    job = nh.Job(ctx, tables, nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=form),
+    job = nh.Job(ctx, tables, nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=form, pool=pool),
                  nh.WyHash.seed_from_u64(5), 1e-2, 0.5, report_dispersal=True, event_buffer=64)
     job.add_lineages(lin)
     with pytest.raises(_ffi.NecsimHipError) as e:
@@ -301,13 +311,14 @@ def test_device_counter_on_hand_and_coalescing_streams(ctx):
     assert _device_counts(ctx, ev) == reference_counts([ev])
 
 
-@pytest.mark.parametrize("form,graph", [(PLAIN, False), (PERSISTENT, False), (PERSISTENT, True)])
-def test_device_counter_job_matches_host_counter(form, graph):
-    announce(f"device_event_counter job vs host EventCounterReporter, form {form} graph {graph}")
+@pytest.mark.parametrize("form,graph,pool", [(PLAIN, False, 0), (PERSISTENT, False, 0), (PERSISTENT, True, 0),
+                                             (PERSISTENT, False, 128), (PERSISTENT, True, 192)])
+def test_device_counter_job_matches_host_counter(form, graph, pool):
+    announce(f"device_event_counter job vs host EventCounterReporter, form {form} graph {graph} pool {pool}")
     cogs = nh.NonSpatialScenario((12, 12), 2).new(5e-3)
     lin = cogs.sample_habitat(nh.OriginPreSampler.all())
     args = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), form=form, use_graph=graph,
-                                   event_slice=nh.EventSlice.Relative(0.5))
+                                   event_slice=nh.EventSlice.Relative(0.5), pool=pool)
     host, dev = nh.EventCounterReporter(), nh.EventCounterReporter(on_device=True)
     col = nh.PackedEventCollector()
     outs = [nh.HipIndependentAlgorithm.resume_and_simulate(args, nh.WyHash.seed_from_u64(2), cogs, lin, reporter=r)
