@@ -179,12 +179,14 @@ constexpr int SORT_MINMAX_WORDS = 32;
 struct SortBuffers {
     unsigned long long *keys[2];
     uint32_t *vals[2];
-    uint32_t *hist;         // [sort_hist_words(capacity)]: digit counts, then scan chunk sums
+    uint32_t *hist;         // [hist_words]: digit counts, then scan chunk sums
     unsigned long long *minmax; // [SORT_MINMAX_WORDS] device: key ranges, tie count
     uint64_t capacity;
+    uint64_t hist_words;    // sort_hist_words(capacity), or sort_hist_words_wide(capacity) for wide-digit passes
 };
 uint32_t sort_blocks(uint64_t n);
-size_t sort_hist_words(uint64_t n); // words of SortBuffers::hist for n events
+size_t sort_hist_words(uint64_t n);      // words of SortBuffers::hist for n events (8-bit digits)
+size_t sort_hist_words_wide(uint64_t n); // ... with room for the wide-digit passes of eager sorts
 template <class E>
 void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream_t stream, bool fixed_passes);
 // Same, with the event range [*start, *end) read from device memory when the
@@ -232,9 +234,9 @@ struct EventCounts {
 void launch_count_events(const DevPacked *ev, const uint32_t *perm, const unsigned long long *end,
                          uint64_t n_or_cap, const unsigned long long *ghosts, EventCounts *c, hipStream_t stream);
 // the round's `slots` records sorted to a permutation and the first n_events
-// of that order counted; `scratch` holds `slots` records of sort scratch
-void sort_and_count_packed(const DevPacked *events, uint64_t slots, uint64_t n_events, DevPacked *scratch,
-                           SortBuffers &sb, EventCounts *c, hipStream_t stream);
+// of that order counted
+void sort_and_count_packed(const DevPacked *events, uint64_t slots, uint64_t n_events, SortBuffers &sb,
+                           EventCounts *c, hipStream_t stream);
 
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream);
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream);

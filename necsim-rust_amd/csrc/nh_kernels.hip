@@ -1453,12 +1453,13 @@ __global__ void __launch_bounds__(KS_THREADS) k_field_minmax(const E *ev0, SortS
 // from lineages that share a location and a Poisson event (coalesced
 // duplicates), so they share origin and target, and the PackedEvent order
 // within the run reduces to (prior time, reference).  Runs of up to
-// TIE_RUN_MAX events are sorted in LDS: the workgroup of window w sorts the
-// runs that start in positions [w * TIE_WINDOW, (w + 1) * TIE_WINDOW), which
-// with their tails span fewer than TIE_WINDOW + TIE_RUN_MAX positions.
-// Longer runs are ordered by radix passes over the reference, prior and
-// time fields only; runs with several origins or targets by passes over
-// every field.
+// TIE_RUN_MAX events are sorted in LDS (k_tie_runs): the workgroup of window
+// w sorts the runs that start in positions [w * TIE_WINDOW, (w + 1) *
+// TIE_WINDOW), which with their tails span fewer than TIE_WINDOW +
+// TIE_RUN_MAX positions, reading each event's record once and checking that
+// assumption as it goes.  Longer runs are ordered by radix passes over the
+// reference, prior and time fields only;
+// runs with several origins or targets by passes over every field.
 constexpr uint64_t TIE_RUN_MAX = 1024;
 // A window's span is its positions plus the tail of the run that crosses its
 // end; 896 positions keep spans with tails up to 128 in a 1024-element
@@ -1466,6 +1467,11 @@ constexpr uint64_t TIE_RUN_MAX = 1024;
 constexpr uint64_t TIE_WINDOW = 896;
 constexpr int TIE_SORT_N = 2048; // >= TIE_WINDOW + TIE_RUN_MAX - 1, a power of two
 constexpr int TIE_THREADS = 512;
+// a window element is (run << 11 | span position, prior, reference), 20 B:
+// the span's run index orders like its time (the span is time-sorted)
+constexpr int TIE_POS_BITS = 11;
+constexpr uint32_t TIE_POS_MASK = (1u << TIE_POS_BITS) - 1;
+static_assert(TIE_SORT_N <= (1 << TIE_POS_BITS), "span positions and run indices in 11 bits");
 
 // An event's class for the device EventCounterReporter: bit 0 prior time
 // negative, bit 1 event time negative, bit 2 origin == target.
@@ -1488,15 +1494,13 @@ __device__ __forceinline__ int first_event_counter(uint8_t cls) {
     return first_event_counter(prior_neg && !event_neg, !prior_neg, !event_neg, (cls & 4) != 0);
 }
 
-// (prior, reference) of every event in time order, into scratch, and (with
-// cls_ws) its counter class; flags runs whose members differ in origin or
-// target (flags[0]) and runs longer than TIE_RUN_MAX (flags[1], checked at
-// each run's head)
+// Flags runs whose members differ in origin or target (flags[0]) and runs
+// longer than TIE_RUN_MAX (flags[1], checked at each run's head): the
+// check of a round whose runs k_tie_runs could not take
 template <class E>
-__global__ void __launch_bounds__(KS_THREADS) k_tie_gather(const E *ev, const unsigned long long *keys,
-                                                           const uint32_t *vals, uint64_t n,
-                                                           unsigned long long *prior_ws, unsigned long long *ref_ws,
-                                                           uint8_t *cls_ws, unsigned long long *flags) {
+__global__ void __launch_bounds__(KS_THREADS) k_tie_check(const E *ev, const unsigned long long *keys,
+                                                          const uint32_t *vals, uint64_t n,
+                                                          unsigned long long *flags) {
     using O = EventOrder<E>;
     constexpr int NF = O::N - 3; // the origin / target fields 2 .. N - 2 (32-bit values)
     // each event's origin / target fields, for the thread that compares the
@@ -1512,9 +1516,6 @@ __global__ void __launch_bounds__(KS_THREADS) k_tie_gather(const E *ev, const un
         uint32_t f[NF];
         if (have) {
             const E e = ev[vals[p]];
-            prior_ws[p] = O::field(e, 1);
-            ref_ws[p] = O::field(e, 0);
-            if (cls_ws) cls_ws[p] = event_class(e);
 #pragma unroll
             for (int k = 0; k < NF; k++) f[k] = s_f[j % 3][threadIdx.x][k] = (uint32_t)O::field(e, 2 + k);
         }
@@ -1561,30 +1562,38 @@ __device__ __forceinline__ uint64_t block_first_head(const unsigned long long *k
     return r;
 }
 
-// Sorts the window's runs by (time key, prior, reference, time-sorted
-// position) with a bitonic sort in LDS over the next power of two >= the
-// window's span and writes their permutation; a window without equal times
-// keeps its order.  Unused event slots (ghost keys) sort after every event
-// and count as runs of one.
-// With `c` (a device_event_counter round of uniform runs), the window also
-// counts its events in the sorted order, as k_count_events would: a run's
-// first event adds to its class's counter, every later one is a duplicate
-// and a late coalescence when its prior time differs from the previous
-// event's (event_counter.rs:58-109).  Events [0, n_real) are real.  A run
-// whose events differ in class sets c->round_mixed, and the round is then
-// counted through the permutation instead.
-__global__ void __launch_bounds__(TIE_THREADS) k_tie_sort_windows(const unsigned long long *keys, const uint32_t *vals,
-                                                                  uint64_t n, const unsigned long long *prior_ws,
-                                                                  const unsigned long long *ref_ws,
-                                                                  uint32_t *out_vals, const uint8_t *cls_ws,
-                                                                  unsigned long long n_real, EventCounts *c) {
-    __shared__ unsigned long long s_t[TIE_SORT_N], s_p[TIE_SORT_N], s_r[TIE_SORT_N];
-    // window position (11 bits) and, counting, the event's class (bits 11-13):
-    // one array keeps the LDS at three workgroups per CU
-    __shared__ uint16_t s_i[TIE_SORT_N];
+// Sorts the runs of window w (positions [w * TIE_WINDOW, (w + 1) *
+// TIE_WINDOW) and the tail of the run crossing its end) by (prior,
+// reference) and writes their permutation.  Each position's record is read
+// once, through the time sort's permutation: prior, reference, counter class
+// and the origin / target fields, which must equal those of the previous
+// position of the run.  A window whose runs break an assumption (several
+// origins or targets: flags[0]; a run longer than TIE_RUN_MAX: flags[1])
+// writes nothing and the round is sorted by field passes instead.  A span
+// without equal times keeps its order; unused event slots (ghost keys) sort
+// after every event and are runs of one.
+// With `c` (a device_event_counter round), the window also counts its events
+// in the sorted order, as k_count_events would: a run's first event adds to
+// its class's counter, every later one is a duplicate and a late coalescence
+// when its prior time differs from the previous event's
+// (event_counter.rs:58-109).  Events [0, n_real) are real.  A run whose
+// events differ in class sets c->round_mixed, and the round is then counted
+// through the permutation instead.
+template <class E>
+__global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const unsigned long long *keys,
+                                                          const uint32_t *vals, uint64_t n, uint32_t *out_vals,
+                                                          unsigned long long n_real, EventCounts *c,
+                                                          unsigned long long *flags) {
+    using O = EventOrder<E>;
+    constexpr int NF = O::N - 3; // the origin / target fields 2 .. N - 2 (32-bit values)
+    constexpr int NWV = TIE_THREADS / 64;
+    __shared__ unsigned long long s_p[TIE_SORT_N], s_r[TIE_SORT_N];
+    __shared__ uint32_t s_m[TIE_SORT_N];     // run << 11 | span position
+    __shared__ uint8_t s_cls[TIE_SORT_N];    // counter class, by span position
+    __shared__ uint32_t s_edge[NWV + 1][NF]; // origin / target of each wave's last lane; [NWV]: the chunk's last
+    __shared__ uint32_t s_wsum[NWV];
     __shared__ unsigned long long s_cnt[EC_COUNTERS];
-    constexpr uint16_t POS = 0x7ff;
-    static_assert(TIE_SORT_N <= POS + 1, "window positions in 11 bits");
+    __shared__ unsigned s_bad; // bit 0: several origins / targets in a run; bit 1: a long run
     const uint64_t lo = (uint64_t)blockIdx.x * TIE_WINDOW;
     if (lo >= n) return;
     const uint64_t hi = lo + TIE_WINDOW < n ? lo + TIE_WINDOW : n;
@@ -1592,29 +1601,91 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_sort_windows(const unsigned
     if (s >= hi) return; // every position of the window continues an earlier run
     const uint64_t tail_hi = hi + TIE_RUN_MAX < n ? hi + TIE_RUN_MAX : n;
     const uint64_t e = hi < n ? block_first_head(keys, hi, tail_hi) : n;
-    const int len = (int)(e - s);
-    // the network spans the next power of two >= len (padded with ~0 keys)
-    int nsort = 64;
-    while (nsort < len) nsort <<= 1;
-    for (int i = threadIdx.x; i < nsort; i += TIE_THREADS) {
-        if (i < len) {
-            s_t[i] = keys[s + i];
-            s_p[i] = prior_ws[s + i];
-            s_r[i] = ref_ws[s + i];
-        } else {
-            s_t[i] = ~0ull;
-            s_p[i] = ~0ull;
-            s_r[i] = ~0ull;
-        }
-        s_i[i] = (uint16_t)(i | (c && i < len ? (int)cls_ws[s + i] << 11 : 0));
+    if (e == tail_hi && tail_hi < n && keys[tail_hi] == keys[tail_hi - 1] && keys[tail_hi] != GHOST_TIME_BITS) {
+        // the run crossing the window's end goes on past its tail
+        if (threadIdx.x == 0) atomicAdd(&flags[1], 1ull);
+        return;
     }
+    const int len = (int)(e - s);
+    int nsort = 64; // the network spans the next power of two >= len (padded with ~0 keys)
+    while (nsort < len) nsort <<= 1;
+    const unsigned lane = lane_id(), wave = threadIdx.x / 64;
+    if (threadIdx.x == 0) s_bad = 0;
     if (c && threadIdx.x < EC_COUNTERS) s_cnt[threadIdx.x] = 0;
-    __syncthreads();
-    // a window without equal times keeps its order
     int tie = 0;
-    for (int i = threadIdx.x + 1; i < len; i += TIE_THREADS)
-        tie |= (s_t[i] == s_t[i - 1] && s_t[i] != GHOST_TIME_BITS) ? 1 : 0;
-    if (__syncthreads_or(tie)) {
+    uint32_t run_base = 0; // runs that started in earlier chunks
+    for (int c0 = 0; c0 < len; c0 += TIE_THREADS) {
+        const int i = c0 + (int)threadIdx.x;
+        const bool have = i < len;
+        unsigned long long prior = 0, ref = 0;
+        uint32_t f[NF];
+        uint8_t cls = 0;
+        bool head = true;
+        if (have) {
+            const uint64_t p = s + (uint64_t)i;
+            const E rec = ev[vals[p]];
+            prior = O::field(rec, 1);
+            ref = O::field(rec, 0);
+            cls = event_class(rec);
+#pragma unroll
+            for (int k = 0; k < NF; k++) f[k] = (uint32_t)O::field(rec, 2 + k);
+            const unsigned long long k = keys[p];
+            head = i == 0 || k == GHOST_TIME_BITS || keys[p - 1] != k;
+        } else {
+#pragma unroll
+            for (int k = 0; k < NF; k++) f[k] = 0;
+        }
+        // the previous position's origin / target: the lane below, the last
+        // lane of the wave below, or the previous chunk's last position
+        uint32_t q[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) q[k] = (uint32_t)__shfl_up((int)f[k], 1);
+        if (lane == 63) {
+#pragma unroll
+            for (int k = 0; k < NF; k++) s_edge[wave][k] = f[k];
+        }
+        const uint64_t hb = __ballot(have && head);
+        if (lane == 0) s_wsum[wave] = (uint32_t)__popcll(hb);
+        __syncthreads();
+        if (have && !head) { // the previous position is in this run: same origin and target
+            const uint32_t *qe = wave > 0 ? s_edge[wave - 1] : s_edge[NWV];
+            bool mixed = false;
+#pragma unroll
+            for (int k = 0; k < NF; k++) mixed |= f[k] != (lane > 0 ? q[k] : qe[k]);
+            if (mixed) atomicOr(&s_bad, 1u);
+            tie = 1;
+        }
+        // the position's run: heads of the span up to it, minus one
+        const uint32_t incl = (uint32_t)__popcll(hb & (lanemask_lt() | (1ull << lane)));
+        uint32_t before = run_base, chunk = 0;
+        for (unsigned w = 0; w < NWV; w++) {
+            before += w < wave ? s_wsum[w] : 0u;
+            chunk += s_wsum[w];
+        }
+        if (have) {
+            s_p[i] = prior;
+            s_r[i] = ref;
+            s_m[i] = ((before + incl - 1) << TIE_POS_BITS) | (uint32_t)i;
+            s_cls[i] = cls;
+        }
+        run_base += chunk;
+        __syncthreads(); // s_edge and s_wsum are rewritten by the next chunk
+        if (threadIdx.x == TIE_THREADS - 1) {
+#pragma unroll
+            for (int k = 0; k < NF; k++) s_edge[NWV][k] = f[k];
+        }
+    }
+    for (int i = len + (int)threadIdx.x; i < nsort; i += TIE_THREADS) {
+        s_p[i] = ~0ull;
+        s_r[i] = ~0ull;
+        s_m[i] = ~0u;
+    }
+    tie = __syncthreads_or(tie);
+    if (s_bad) {
+        if (threadIdx.x == 0) atomicAdd(&flags[(s_bad & 1u) ? 0 : 1], 1ull);
+        return;
+    }
+    if (tie) {
         // bitonic network: in stage (k, j), pair q compares i and i + j
         for (int k = 2; k <= nsort; k <<= 1) {
             for (int j = k >> 1; j > 0; j >>= 1) {
@@ -1622,24 +1693,22 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_sort_windows(const unsigned
                     const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
                     const int l = i + j;
                     const bool ascending = (i & k) == 0;
-                    const unsigned long long ti = s_t[i], tl = s_t[l], pi = s_p[i], pl = s_p[l], ri = s_r[i],
-                                             rl = s_r[l];
-                    const uint16_t ii = s_i[i], il = s_i[l];
+                    const uint32_t mi = s_m[i], ml = s_m[l];
+                    const unsigned long long pi = s_p[i], pl = s_p[l], ri = s_r[i], rl = s_r[l];
+                    const uint32_t ui = mi >> TIE_POS_BITS, ul = ml >> TIE_POS_BITS;
                     const bool l_less =
-                        tl < ti ||
-                        (tl == ti && (pl < pi || (pl == pi && (rl < ri || (rl == ri && (il & POS) < (ii & POS))))));
+                        ul < ui || (ul == ui && (pl < pi || (pl == pi && (rl < ri || (rl == ri && ml < mi)))));
                     if (l_less == ascending) {
-                        s_t[i] = tl; s_t[l] = ti;
                         s_p[i] = pl; s_p[l] = pi;
                         s_r[i] = rl; s_r[l] = ri;
-                        s_i[i] = il; s_i[l] = ii;
+                        s_m[i] = ml; s_m[l] = mi;
                     }
                 }
                 __syncthreads();
             }
         }
     }
-    for (int i = threadIdx.x; i < len; i += TIE_THREADS) out_vals[s + i] = vals[s + (s_i[i] & POS)];
+    for (int i = threadIdx.x; i < len; i += TIE_THREADS) out_vals[s + i] = vals[s + (s_m[i] & TIE_POS_MASK)];
     if (!c) return;
     // the events of the span in sorted order; position 0 (s) starts a run
     const int real = n_real > s ? (int)(n_real - s < (uint64_t)len ? n_real - s : (uint64_t)len) : 0;
@@ -1648,14 +1717,14 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_sort_windows(const unsigned
     for (int k = 0; k < EC_COUNTERS; k++) cnt[k] = 0;
     bool mixed = false;
     for (int i = threadIdx.x; i < real; i += TIE_THREADS) {
-        const uint8_t ci = (uint8_t)(s_i[i] >> 11);
+        const uint8_t ci = s_cls[s_m[i] & TIE_POS_MASK];
         cnt[EC_RAW_TOTAL]++;
-        if (i == 0 || s_t[i - 1] != s_t[i]) {
+        if (i == 0 || (s_m[i - 1] >> TIE_POS_BITS) != (s_m[i] >> TIE_POS_BITS)) {
             const int w = first_event_counter(ci);
 #pragma unroll
             for (int k = EC_SPECIATION; k < EC_COUNTERS; k++) cnt[k] += k == w ? 1 : 0;
         } else {
-            mixed |= (s_i[i - 1] >> 11) != ci;
+            mixed |= s_cls[s_m[i - 1] & TIE_POS_MASK] != ci;
             cnt[EC_LATE_COALESCENCE] += s_p[i - 1] != s_p[i] ? 1 : 0;
         }
         if (s + (uint64_t)i == n_real - 1) c->round_carry_prior = s_p[i];
@@ -1664,7 +1733,7 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_sort_windows(const unsigned
     for (int k = 0; k < EC_COUNTERS; k++) {
         unsigned long long v = cnt[k];
         for (int off = 32; off; off >>= 1) v += __shfl_xor(v, off);
-        if (lane_id() == 0 && v) atomicAdd(&s_cnt[k], v);
+        if (lane == 0 && v) atomicAdd(&s_cnt[k], v);
     }
     if (__syncthreads_or(mixed ? 1 : 0) && threadIdx.x == 0) c->round_mixed = 1;
     if (threadIdx.x < EC_COUNTERS && s_cnt[threadIdx.x]) atomicAdd(&c->round_count[threadIdx.x], s_cnt[threadIdx.x]);
@@ -1749,6 +1818,191 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const unsigned lon
         for (int w = 0; w < W; w++) add += wcnt[w][threadIdx.x];
         running[threadIdx.x] += add;
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wide-digit LSD passes for eager sorts (the time keys of a round): 10-bit
+// digits, so a round's 47-50-bit key range takes 5 passes instead of 6-7.
+// A tile of WR_TILE keys is ranked by digit in LDS (stable: wave-major, then
+// round, then lane, which is the tile's own order) and written out in digit
+// order, so each digit's run of the tile (8 keys on average) goes to
+// consecutive addresses.  Measured on 25.1M-event rounds, per round: 8-bit
+// digits 2.08 ms, 9-bit 2.05, 10-bit 2.01, 11-bit 2.31 (its runs of 4 keys
+// write partial lines); the per-pass 8-bit sort without staging 2.60.
+constexpr int WR_BITS = 10;
+constexpr int WR_BINS = 1 << WR_BITS;
+constexpr int WR_THREADS = 512;
+constexpr int WR_WAVES = WR_THREADS / 64;
+constexpr int WR_ITEMS = 16;
+constexpr int WR_TILE = WR_THREADS * WR_ITEMS;
+constexpr int WR_DPT = WR_BINS >= WR_THREADS ? WR_BINS / WR_THREADS : 1; // digits per thread in the tile scan
+static_assert(64 * WR_ITEMS < (1 << 16) && WR_BINS <= (1 << 16), "ranks and digits packed in 16 bits each");
+static_assert(WR_BINS % WR_THREADS == 0 || WR_THREADS % WR_BINS == 0, "digits spread evenly over threads");
+
+__device__ __forceinline__ uint32_t wradix_digit(unsigned long long key, unsigned long long base,
+                                                 unsigned long long top, int shift) {
+    return (uint32_t)(((key < top ? key : top) - base) >> shift) & (uint32_t)(WR_BINS - 1);
+}
+
+// lanes of the wave whose digit equals this lane's (valid lanes only)
+__device__ __forceinline__ uint64_t wradix_peers(uint32_t digit, bool valid) {
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < WR_BITS; bit++) {
+        const bool set = (digit >> bit) & 1u;
+        const uint64_t mset = __ballot(valid && set);
+        peers &= set ? mset : ~mset;
+    }
+    return peers;
+}
+
+__device__ __forceinline__ uint32_t wradix_tiles(uint64_t n) { return (uint32_t)((n + WR_TILE - 1) / WR_TILE); }
+
+// Workgroups go to the 8 XCDs round-robin, each XCD with its own L2.  Block
+// b takes tile (b % 8) * per + b / 8, so one XCD works through consecutive
+// tiles and the digit-major histogram words of neighbouring tiles (adjacent
+// in memory) meet in one L2.  The grid has 8 * per blocks; those past the
+// last tile exit.
+constexpr uint32_t XCDS = 8;
+__host__ __device__ __forceinline__ uint32_t xcd_grid(uint32_t tiles) { return XCDS * ((tiles + XCDS - 1) / XCDS); }
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t tiles) {
+    return (b % XCDS) * ((tiles + XCDS - 1) / XCDS) + b / XCDS;
+}
+
+__global__ void __launch_bounds__(WR_THREADS) k_wradix_hist(const unsigned long long *keys, SortSize sz, int shift,
+                                                            unsigned long long key_base, unsigned long long top,
+                                                            uint32_t *hist) {
+    __shared__ uint32_t h[WR_BINS];
+    if (sz.skip()) return;
+    const uint64_t n = sz.get();
+    const uint32_t nb = wradix_tiles(n);
+    const uint32_t tile = xcd_tile(blockIdx.x, nb);
+    if (tile >= nb) return;
+    for (int d = threadIdx.x; d < WR_BINS; d += WR_THREADS) h[d] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)tile * WR_TILE;
+    unsigned long long key[WR_ITEMS]; // every load in flight before the first is used
+#pragma unroll
+    for (int j = 0; j < WR_ITEMS; j++) {
+        const uint64_t p = base + (uint64_t)j * WR_THREADS + threadIdx.x;
+        key[j] = p < n ? keys[p] : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < WR_ITEMS; j++) {
+        const uint64_t p = base + (uint64_t)j * WR_THREADS + threadIdx.x;
+        const bool valid = p < n;
+        const uint32_t digit = valid ? wradix_digit(key[j], key_base, top, shift) : 0u;
+        // one LDS atomic per distinct digit of the wave (sorted keys repeat digits)
+        const uint64_t peers = wradix_peers(digit, valid);
+        if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&h[digit], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < WR_BINS; d += WR_THREADS) hist[(uint64_t)d * nb + tile] = h[d];
+}
+
+// `offsets`: the exclusive scan of the digit-major histogram of k_wradix_hist
+__global__ void __launch_bounds__(WR_THREADS) k_wradix_scatter(const unsigned long long *kin, const uint32_t *vin,
+                                                               unsigned long long *kout, uint32_t *vout,
+                                                               SortSize sz, int shift, unsigned long long key_base,
+                                                               unsigned long long top, const uint32_t *offsets) {
+    __shared__ unsigned long long s_key[WR_TILE];
+    __shared__ uint32_t s_val[WR_TILE];
+    __shared__ uint16_t s_wc[WR_WAVES][WR_BINS]; // per wave: digit counts, then the counts of the waves before
+    __shared__ uint32_t s_start[WR_BINS];        // the digit's first position in the tile
+    __shared__ uint32_t s_dst[WR_BINS];          // global position of the tile's position 0 in the digit's run
+    __shared__ uint32_t s_wsum[WR_WAVES];
+    if (sz.skip()) return;
+    const uint64_t n = sz.get();
+    const uint32_t nb = wradix_tiles(n);
+    const uint32_t tile = xcd_tile(blockIdx.x, nb);
+    if (tile >= nb) return;
+    const unsigned lane = lane_id(), wave = threadIdx.x / 64;
+    // the global offsets of the thread's digits, read while the tile is ranked
+    const bool has_digits = (int)threadIdx.x * WR_DPT < WR_BINS;
+    uint32_t goff[WR_DPT];
+#pragma unroll
+    for (int k = 0; k < WR_DPT; k++)
+        goff[k] = has_digits ? offsets[(uint64_t)(WR_DPT * threadIdx.x + k) * nb + tile] : 0u;
+    for (int i = threadIdx.x; i < WR_WAVES * WR_BINS; i += WR_THREADS) (&s_wc[0][0])[i] = 0;
+    __syncthreads();
+    const uint64_t tbase = (uint64_t)tile * WR_TILE;
+    const uint64_t wbase = tbase + (uint64_t)wave * (64 * WR_ITEMS);
+    unsigned long long key[WR_ITEMS];
+    uint32_t val[WR_ITEMS], dr[WR_ITEMS];
+#pragma unroll
+    for (int j = 0; j < WR_ITEMS; j++) {
+        const uint64_t p = wbase + (uint64_t)j * 64 + lane;
+        key[j] = p < n ? kin[p] : 0ull;
+        val[j] = p < n ? vin[p] : 0u;
+    }
+    // rank of each key among the wave's keys of its digit, in wave order
+#pragma unroll
+    for (int j = 0; j < WR_ITEMS; j++) {
+        const bool valid = wbase + (uint64_t)j * 64 + lane < n;
+        const uint32_t digit = valid ? wradix_digit(key[j], key_base, top, shift) : 0u;
+        const uint64_t peers = wradix_peers(digit, valid);
+        uint32_t r = 0;
+        if (valid) {
+            const uint32_t c = s_wc[wave][digit];
+            r = c + (uint32_t)__popcll(peers & lanemask_lt());
+            if ((peers & lanemask_lt()) == 0) s_wc[wave][digit] = (uint16_t)(c + (uint32_t)__popcll(peers));
+        }
+        dr[j] = digit | (r << 16);
+    }
+    __syncthreads();
+    // digits DPT t .. DPT t + DPT - 1: counts of the waves before each wave,
+    // tile totals, and their exclusive scan over the tile's digits
+    uint32_t tot[WR_DPT], sum = 0;
+#pragma unroll
+    for (int k = 0; k < WR_DPT; k++) {
+        const int d = WR_DPT * threadIdx.x + k;
+        uint32_t run = 0;
+        if (has_digits) {
+            for (int w = 0; w < WR_WAVES; w++) {
+                const uint32_t c = s_wc[w][d];
+                s_wc[w][d] = (uint16_t)run;
+                run += c;
+            }
+        }
+        tot[k] = run;
+        sum += run;
+    }
+    uint32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off);
+        if (lane >= (unsigned)off) incl += y;
+    }
+    if (lane == 63) s_wsum[wave] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum;
+    for (unsigned w = 0; w < wave; w++) run += s_wsum[w];
+#pragma unroll
+    for (int k = 0; k < WR_DPT; k++) {
+        const int d = WR_DPT * threadIdx.x + k;
+        if (has_digits) {
+            s_start[d] = run;
+            s_dst[d] = goff[k] - run;
+        }
+        run += tot[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < WR_ITEMS; j++) {
+        if (wbase + (uint64_t)j * 64 + lane < n) {
+            const uint32_t d = dr[j] & 0xffffu;
+            const uint32_t pos = s_start[d] + s_wc[wave][d] + (dr[j] >> 16);
+            s_key[pos] = key[j];
+            s_val[pos] = val[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t cnt = (uint32_t)(n - tbase < (uint64_t)WR_TILE ? n - tbase : (uint64_t)WR_TILE);
+    for (uint32_t i = threadIdx.x; i < cnt; i += WR_THREADS) {
+        const unsigned long long k = s_key[i];
+        const uint32_t dst = s_dst[wradix_digit(k, key_base, top, shift)] + i;
+        kout[dst] = k;
+        vout[dst] = s_val[i];
     }
 }
 
@@ -1964,11 +2218,15 @@ constexpr int HS_THREADS = 256;
 constexpr int HS_ITEMS = 16;
 constexpr int HS_CHUNK = HS_THREADS * HS_ITEMS;
 
-__device__ __forceinline__ uint64_t hist_words(const SortSize &sz) { return (uint64_t)256 * sz.tiles(); }
+// digit-major histogram words of a sort: BINS digits x tiles of TILE keys
+template <int BINS, int TILE> __device__ __forceinline__ uint64_t hist_words(const SortSize &sz) {
+    return (uint64_t)BINS * ((sz.get() + TILE - 1) / TILE);
+}
 
+template <int BINS = 256, int TILE = RS_TILE>
 __global__ void __launch_bounds__(HS_THREADS) k_hist_chunk_sums(const uint32_t *h, SortSize sz, uint32_t *sums) {
     if (sz.skip()) return;
-    const uint64_t n = hist_words(sz);
+    const uint64_t n = hist_words<BINS, TILE>(sz);
     const uint64_t base = (uint64_t)blockIdx.x * HS_CHUNK;
     if (base >= n) return;
     unsigned long long s = 0;
@@ -1985,10 +2243,11 @@ __global__ void __launch_bounds__(HS_THREADS) k_hist_chunk_sums(const uint32_t *
 }
 
 // one block: exclusive scan of the chunk sums in place
+template <int BINS = 256, int TILE = RS_TILE>
 __global__ void __launch_bounds__(1024) k_scan_chunk_sums(uint32_t *sums, SortSize sz) {
     __shared__ uint32_t partial[1024];
     if (sz.skip()) return;
-    const uint64_t n = (hist_words(sz) + HS_CHUNK - 1) / HS_CHUNK;
+    const uint64_t n = (hist_words<BINS, TILE>(sz) + HS_CHUNK - 1) / HS_CHUNK;
     const uint64_t per = (n + 1023) / 1024;
     const uint64_t lo = per * threadIdx.x, hi = lo + per < n ? lo + per : n;
     uint32_t sum = 0;
@@ -2011,10 +2270,11 @@ __global__ void __launch_bounds__(1024) k_scan_chunk_sums(uint32_t *sums, SortSi
 
 // each thread scans HS_ITEMS consecutive counts; the block scans the thread
 // totals (wave shuffles, then the wave totals)
+template <int BINS = 256, int TILE = RS_TILE>
 __global__ void __launch_bounds__(HS_THREADS) k_hist_scan_apply(uint32_t *h, SortSize sz, const uint32_t *sums) {
     __shared__ uint32_t s_wave[HS_THREADS / 64];
     if (sz.skip()) return;
-    const uint64_t n = hist_words(sz);
+    const uint64_t n = hist_words<BINS, TILE>(sz);
     const uint64_t base = (uint64_t)blockIdx.x * HS_CHUNK;
     if (base >= n) return;
     const uint64_t first = base + (uint64_t)threadIdx.x * HS_ITEMS;
@@ -2045,6 +2305,16 @@ size_t sort_hist_words(uint64_t n) {
     return (size_t)(h + 1 + (h + HS_CHUNK - 1) / HS_CHUNK + 1);
 }
 
+static size_t wide_hist_words(uint64_t n) {
+    uint64_t h = (uint64_t)WR_BINS * ((n + WR_TILE - 1) / WR_TILE);
+    return (size_t)(h + 1 + (h + HS_CHUNK - 1) / HS_CHUNK + 1);
+}
+
+size_t sort_hist_words_wide(uint64_t n) {
+    const size_t a = sort_hist_words(n), b = wide_hist_words(n);
+    return a > b ? a : b;
+}
+
 static int radix_passes(SortSize sz, uint64_t cap, SortBuffers &sb, hipStream_t stream, int first, int last,
                         int cur, unsigned long long base = 0, unsigned long long top = ~0ull) {
     uint32_t nb = sort_blocks(cap);
@@ -2053,11 +2323,34 @@ static int radix_passes(SortSize sz, uint64_t cap, SortBuffers &sb, hipStream_t 
     for (int d = first; d <= last; d++) {
         hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sz, 8 * d, base, top,
                            sb.hist);
-        hipLaunchKernelGGL(k_hist_chunk_sums, dim3(nc), dim3(HS_THREADS), 0, stream, sb.hist, sz, chunk_sums);
-        hipLaunchKernelGGL(k_scan_chunk_sums, dim3(1), dim3(1024), 0, stream, chunk_sums, sz);
-        hipLaunchKernelGGL(k_hist_scan_apply, dim3(nc), dim3(HS_THREADS), 0, stream, sb.hist, sz, chunk_sums);
+        hipLaunchKernelGGL((k_hist_chunk_sums<256, RS_TILE>), dim3(nc), dim3(HS_THREADS), 0, stream, sb.hist, sz, chunk_sums);
+        hipLaunchKernelGGL((k_scan_chunk_sums<256, RS_TILE>), dim3(1), dim3(1024), 0, stream, chunk_sums, sz);
+        hipLaunchKernelGGL((k_hist_scan_apply<256, RS_TILE>), dim3(nc), dim3(HS_THREADS), 0, stream, sb.hist, sz, chunk_sums);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, stream, sb.keys[cur], sb.vals[cur],
                            sb.keys[cur ^ 1], sb.vals[cur ^ 1], sz, 8 * d, base, top, sb.hist);
+        cur ^= 1;
+    }
+    return cur;
+}
+
+// `passes` WR_BITS-bit digit passes from bit 0 (eager sorts: n known on the host)
+static int radix_passes_wide(uint64_t n, SortBuffers &sb, hipStream_t stream, int passes, int cur,
+                             unsigned long long base, unsigned long long top) {
+    SortSize sz{n, nullptr, nullptr, nullptr};
+    const uint32_t nb = (uint32_t)((n + WR_TILE - 1) / WR_TILE), grid = xcd_grid(nb);
+    const uint64_t words = (uint64_t)WR_BINS * nb;
+    const uint32_t nc = (uint32_t)((words + HS_CHUNK - 1) / HS_CHUNK);
+    uint32_t *chunk_sums = sb.hist + words + 1;
+    for (int d = 0; d < passes; d++) {
+        hipLaunchKernelGGL(k_wradix_hist, dim3(grid), dim3(WR_THREADS), 0, stream, sb.keys[cur], sz, WR_BITS * d, base,
+                           top, sb.hist);
+        hipLaunchKernelGGL((k_hist_chunk_sums<WR_BINS, WR_TILE>), dim3(nc), dim3(HS_THREADS), 0, stream, sb.hist, sz,
+                           chunk_sums);
+        hipLaunchKernelGGL((k_scan_chunk_sums<WR_BINS, WR_TILE>), dim3(1), dim3(1024), 0, stream, chunk_sums, sz);
+        hipLaunchKernelGGL((k_hist_scan_apply<WR_BINS, WR_TILE>), dim3(nc), dim3(HS_THREADS), 0, stream, sb.hist, sz,
+                           chunk_sums);
+        hipLaunchKernelGGL(k_wradix_scatter, dim3(grid), dim3(WR_THREADS), 0, stream, sb.keys[cur], sb.vals[cur],
+                           sb.keys[cur ^ 1], sb.vals[cur ^ 1], sz, WR_BITS * d, base, top, sb.hist);
         cur ^= 1;
     }
     return cur;
@@ -2069,20 +2362,21 @@ static int highest_digit(unsigned long long diff) {
     return -1;
 }
 
+__global__ void k_count_round_clear(EventCounts *c);
+
 // Sort of one round's events by the PackedEvent order: an LSD radix sort of
 // the event times; if two events share a time (duplicate lineages when dedup
 // is off), the runs of equal times sorted in LDS windows, or, for runs with
 // several origins or longer than TIE_RUN_MAX, a full LSD radix sort over
 // every field of the order.  Digits above the highest one in which a key
 // range differs are equal for every key and are skipped.  Returns the
-// sorted permutation (one of sb.vals); `scratch` (n events) holds the runs'
-// prior times and references.
+// sorted permutation (one of sb.vals).
 template <class E>
 // With `counts` (PackedEvents of a device_event_counter round, n_real of the
 // n records real), a round whose runs of equal times go through the LDS
 // windows is counted there, into counts->round_*, and *counted is set; the
 // caller then commits it (k_count_finish).
-static const uint32_t *sort_permutation(const E *events, uint64_t n, E *scratch, SortBuffers &sb,
+static const uint32_t *sort_permutation(const E *events, uint64_t n, SortBuffers &sb,
                                         hipStream_t stream, bool fixed_passes, EventCounts *counts = nullptr,
                                         uint64_t n_real = 0, bool *counted = nullptr) {
     if (counted) *counted = false;
@@ -2093,7 +2387,7 @@ static const uint32_t *sort_permutation(const E *events, uint64_t n, E *scratch,
     hipLaunchKernelGGL(k_event_keys<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sz, sb.keys[0], sb.vals[0],
                        sb.minmax);
     unsigned long long mm[MM_WORDS];
-    int last = 7;
+    int last = 7, wide = -1; // wide >= 0: that many WR_BITS-bit passes instead
     unsigned long long base = 0, top = ~0ull;
     if (!fixed_passes) {
         (void)hipMemcpyAsync(mm, sb.minmax, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, stream);
@@ -2107,33 +2401,44 @@ static const uint32_t *sort_permutation(const E *events, uint64_t n, E *scratch,
             base = lo;
             top = hi + 1;
             last = highest_digit(top - base);
+            // with room for their histogram, wide digits: ceil(bits / WR_BITS) passes
+            if (sb.hist_words >= wide_hist_words(n) && n < (1ull << 32)) {
+                const int bits = 64 - __builtin_clzll(top - base);
+                wide = (bits + WR_BITS - 1) / WR_BITS;
+            }
         }
     }
-    int cur = radix_passes(sz, n, sb, stream, 0, last, 0, base, top);
+    int cur = wide >= 0 ? radix_passes_wide(n, sb, stream, wide, 0, base, top)
+                        : radix_passes(sz, n, sb, stream, 0, last, 0, base, top);
     if (n > 1) {
         hipLaunchKernelGGL(k_tie_count, dim3(gk), dim3(KS_THREADS), 0, stream, sb.keys[cur], sz, sb.minmax + MM_TIES);
         (void)hipMemcpyAsync(&mm[MM_TIES], sb.minmax + MM_TIES, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                              stream);
         (void)hipStreamSynchronize(stream);
+        bool runs_checked = false; // mm[MM_TIE_MIXED] covers every run
         if (mm[MM_TIES] && !fixed_passes) {
-            // rank each run of equal times in place (prior / reference in `scratch`)
-            unsigned long long *prior_ws = reinterpret_cast<unsigned long long *>(scratch);
-            unsigned long long *ref_ws = prior_ws + n;
-            uint8_t *cls_ws = counts ? reinterpret_cast<uint8_t *>(ref_ws + n) : nullptr;
-            static_assert(sizeof(E) >= 17, "scratch of two words and a byte per event");
+            // the runs of equal times sorted in LDS windows
             static_assert(MM_TIE_LONG == MM_TIE_MIXED + 1, "tie flags are adjacent");
-            hipLaunchKernelGGL(k_tie_gather<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sb.keys[cur],
-                               sb.vals[cur], n, prior_ws, ref_ws, cls_ws, sb.minmax + MM_TIE_MIXED);
+            hipLaunchKernelGGL(k_tie_runs<E>, dim3(grid_for(n, TIE_WINDOW)), dim3(TIE_THREADS), 0, stream, events,
+                               sb.keys[cur], sb.vals[cur], n, sb.vals[cur ^ 1], (unsigned long long)n_real, counts,
+                               sb.minmax + MM_TIE_MIXED);
             (void)hipMemcpyAsync(&mm[MM_TIE_MIXED], sb.minmax + MM_TIE_MIXED, 2 * sizeof(unsigned long long),
                                  hipMemcpyDeviceToHost, stream);
             (void)hipStreamSynchronize(stream);
             if (!mm[MM_TIE_MIXED] && !mm[MM_TIE_LONG]) {
-                hipLaunchKernelGGL(k_tie_sort_windows, dim3(grid_for(n, TIE_WINDOW)), dim3(TIE_THREADS), 0, stream,
-                                   sb.keys[cur], sb.vals[cur], n, prior_ws, ref_ws, sb.vals[cur ^ 1],
-                                   (const uint8_t *)cls_ws, (unsigned long long)n_real, counts);
                 if (counted) *counted = counts != nullptr;
                 return sb.vals[cur ^ 1];
             }
+            // windows that stopped at a long run did not check the rest
+            if (counts) hipLaunchKernelGGL(k_count_round_clear, dim3(1), dim3(1), 0, stream, counts);
+            if (!mm[MM_TIE_MIXED]) {
+                hipLaunchKernelGGL(k_tie_check<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sb.keys[cur],
+                                   sb.vals[cur], n, sb.minmax + MM_TIE_MIXED);
+                (void)hipMemcpyAsync(&mm[MM_TIE_MIXED], sb.minmax + MM_TIE_MIXED, sizeof(unsigned long long),
+                                     hipMemcpyDeviceToHost, stream);
+                (void)hipStreamSynchronize(stream);
+            }
+            runs_checked = true;
         }
         if (mm[MM_TIES]) {
             if (!fixed_passes) {
@@ -2142,7 +2447,7 @@ static const uint32_t *sort_permutation(const E *events, uint64_t n, E *scratch,
                 (void)hipStreamSynchronize(stream);
             }
             // runs sharing origin and target need the reference, prior and time passes only
-            const bool uniform_runs = !fixed_passes && !mm[MM_TIE_MIXED];
+            const bool uniform_runs = runs_checked && !mm[MM_TIE_MIXED];
             for (int f = 0; f < O::N; f++) {
                 if (uniform_runs && f >= 2 && f < O::N - 1) continue;
                 int lf = fixed_passes ? O::digits(f) - 1
@@ -2160,7 +2465,7 @@ static const uint32_t *sort_permutation(const E *events, uint64_t n, E *scratch,
 template <class E>
 void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream_t stream, bool fixed_passes) {
     if (n == 0) return;
-    const uint32_t *perm = sort_permutation(events, n, out, sb, stream, fixed_passes);
+    const uint32_t *perm = sort_permutation(events, n, sb, stream, fixed_passes);
     SortSize sz{n, nullptr, nullptr, nullptr};
     hipLaunchKernelGGL(k_gather_events<E>, dim3(grid_for(n, 256)), dim3(256), 0, stream, events, perm, out, sz);
 }
@@ -2330,14 +2635,19 @@ __global__ void k_count_finish(EventCounts *c) {
     for (int k = 0; k < EC_COUNTERS; k++) c->round_count[k] = 0;
 }
 __global__ void k_count_gate_clear(EventCounts *c) { c->round_mixed = 0; }
+// a round whose window counts are dropped (its runs went to field passes)
+__global__ void k_count_round_clear(EventCounts *c) {
+    for (int k = 0; k < EC_COUNTERS; k++) c->round_count[k] = 0;
+    c->round_mixed = 0;
+}
 
 // A round of a device_event_counter job: the events' sorted permutation, then
 // the counters read the events through it (no sorted copy is written)
-void sort_and_count_packed(const DevPacked *events, uint64_t slots, uint64_t n_events, DevPacked *scratch,
+void sort_and_count_packed(const DevPacked *events, uint64_t slots, uint64_t n_events,
                            SortBuffers &sb, EventCounts *c, hipStream_t stream) {
     if (slots == 0 || n_events == 0) return;
     bool counted = false;
-    const uint32_t *perm = sort_permutation(events, slots, scratch, sb, stream, false, c, n_events, &counted);
+    const uint32_t *perm = sort_permutation(events, slots, sb, stream, false, c, n_events, &counted);
     if (!counted) {
         launch_count_events(events, perm, nullptr, n_events, nullptr, c, stream);
         return;

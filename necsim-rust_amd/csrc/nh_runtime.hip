@@ -149,6 +149,7 @@ uint64_t carve_sort_scratch(uintptr_t base, uint64_t n, SortBuffers &sb, unsigne
     sb.vals[0] = reinterpret_cast<uint32_t *>(take(4 * n));
     sb.vals[1] = reinterpret_cast<uint32_t *>(take(4 * n));
     sb.hist = reinterpret_cast<uint32_t *>(take(4 * (uint64_t)sort_hist_words(n)));
+    sb.hist_words = sort_hist_words(n);
     sb.minmax = reinterpret_cast<unsigned long long *>(take(8 * (uint64_t)SORT_MINMAX_WORDS));
     *range = reinterpret_cast<unsigned long long *>(take(16));
     sb.capacity = n;
@@ -779,7 +780,8 @@ static int job_start(nh_job *j) {
     NH_TRY(dalloc(&j->sb.keys[1], sort_cap));
     NH_TRY(dalloc(&j->sb.vals[0], sort_cap));
     NH_TRY(dalloc(&j->sb.vals[1], sort_cap));
-    NH_TRY(dalloc(&j->sb.hist, sort_hist_words(sort_cap)));
+    j->sb.hist_words = sort_hist_words_wide(sort_cap);
+    NH_TRY(dalloc(&j->sb.hist, j->sb.hist_words));
     NH_TRY(dalloc(&j->sb.minmax, SORT_MINMAX_WORDS));
     j->pending.clear();
     j->pending.shrink_to_fit();
@@ -1217,7 +1219,7 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
     if (ne && j->disp && j->d_counts) {
         // the counters read the round through the sort's permutation: no
         // sorted copy (nothing drains a counted round)
-        if (!sorted) sort_and_count_packed(j->d_packed, slots, ne, j->d_packed_sorted, j->sb, j->d_counts, st);
+        if (!sorted) sort_and_count_packed(j->d_packed, slots, ne, j->sb, j->d_counts, st);
         NH_HIP(hipGetLastError());
     } else if (ne && j->disp) {
         if (!sorted) sort_events(j->d_packed, slots, j->d_packed_sorted, j->sb, st, false);
