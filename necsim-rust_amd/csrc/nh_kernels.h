@@ -73,6 +73,11 @@ struct PersistArgs {
     unsigned long long packed_cap;
     unsigned coop;                 // lanes that are not searching search ahead for those that are
     unsigned pool;                 // the pooled kernel's lineages per wave (0: one lineage per lane)
+    // dispersal reporting, or null: each PackedEvent's time key and slot for
+    // the round's sort, and the keys' [min, max] (ghost slots excluded)
+    unsigned long long *ev_keys;
+    uint32_t *ev_vals;
+    unsigned long long *ev_range;
 };
 
 struct PlainArgs {
@@ -187,8 +192,10 @@ struct SortBuffers {
 uint32_t sort_blocks(uint64_t n);
 size_t sort_hist_words(uint64_t n);      // words of SortBuffers::hist for n events (8-bit digits)
 size_t sort_hist_words_wide(uint64_t n); // ... with room for the wide-digit passes of eager sorts
+// keys_ready: the round's kernel wrote the time keys, identity values and
+// key range into sb (k_sort_init first; dispersal-reporting persistent rounds)
 template <class E>
-void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream_t stream, bool fixed_passes);
+void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream_t stream, bool keys_ready);
 // Same, with the event range [*start, *end) read from device memory when the
 // kernels run (fixed grid for up to `cap` events; capturable in a graph).
 template <class E>
@@ -235,9 +242,11 @@ void launch_count_events(const DevPacked *ev, const uint32_t *perm, const unsign
                          uint64_t n_or_cap, const unsigned long long *ghosts, EventCounts *c, hipStream_t stream);
 // the round's `slots` records sorted to a permutation and the first n_events
 // of that order counted
-void sort_and_count_packed(const DevPacked *events, uint64_t slots, uint64_t n_events, SortBuffers &sb,
-                           EventCounts *c, hipStream_t stream);
+void sort_and_count_packed(const DevPacked *events, uint64_t slots, uint64_t n_events, bool keys_ready,
+                           SortBuffers &sb, EventCounts *c, hipStream_t stream);
 
+// resets a sort's range words (before a kernel that writes keys_ready keys)
+void launch_sort_init(SortBuffers &sb, hipStream_t stream);
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream);
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream);
 // reset_events: also the event counter and its overflow flag (dispersal

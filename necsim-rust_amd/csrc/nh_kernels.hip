@@ -130,16 +130,48 @@ __device__ __forceinline__ unsigned long long wave_take_slot(volatile unsigned l
     return rank < rem ? old_base + rank : new_base + (rank - rem);
 }
 
-// at wave exit (all lanes): ghost records into the unused slots below cap
+// at wave exit (all lanes): ghost records into the unused slots below cap,
+// with their sort keys (time +inf) when the kernel writes them
 __device__ __forceinline__ void wave_fill_ghosts(volatile unsigned long long *ws, DevPacked *out,
-                                                 unsigned long long cap, unsigned long long *n_ghost) {
+                                                 unsigned long long cap, unsigned long long *n_ghost,
+                                                 unsigned long long *keys, uint32_t *vals) {
     const unsigned long long nx = ws[0], en0 = ws[1];
     const unsigned long long en = en0 < cap ? en0 : cap;
     if (en <= nx) return;
     DevPacked g{};
     g.event_time = from_bits(0x7ff0000000000000ull);
-    for (unsigned long long s = nx + lane_id(); s < en; s += 64) out[s] = g;
+    for (unsigned long long s = nx + lane_id(); s < en; s += 64) {
+        out[s] = g;
+        if (keys) {
+            keys[s] = 0x7ff0000000000000ull;
+            vals[s] = (uint32_t)s;
+        }
+    }
     if (lane_id() == 0) atomicAdd(n_ghost, en - nx);
+}
+
+// a reported event's sort key (the magnitude of its time) and slot
+__device__ __forceinline__ void emit_key(const PersistArgs &A, unsigned long long eslot, double t,
+                                         unsigned long long &lo, unsigned long long &hi) {
+    const unsigned long long k = bits(t) & 0x7fffffffffffffffull;
+    A.ev_keys[eslot] = k;
+    A.ev_vals[eslot] = (uint32_t)eslot;
+    lo = k < lo ? k : lo;
+    hi = k > hi ? k : hi;
+}
+
+// the wave's key range into A.ev_range (one atomic pair per wave)
+__device__ __forceinline__ void wave_flush_range(const PersistArgs &A, unsigned long long lo,
+                                                 unsigned long long hi) {
+    for (int off = 32; off; off >>= 1) {
+        const unsigned long long l2 = __shfl_xor(lo, off), h2 = __shfl_xor(hi, off);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    if (lane_id() == 0 && lo <= hi) {
+        atomicMin(&A.ev_range[0], lo);
+        atomicMax(&A.ev_range[1], hi);
+    }
 }
 
 struct Origin {
@@ -406,6 +438,7 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
     uint64_t slice_steps = 0;
     unsigned long long my_steps = 0, my_dead = 0;
     double my_max = 0.0;
+    unsigned long long key_lo = ~0ull, key_hi = 0ull; // DISP: the range of the keys this lane wrote
 
     for (;;) {
         if (!q.exhausted) {
@@ -487,7 +520,10 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
                 my_steps++;
                 slice_steps++;
                 my_max = L.t_last > my_max ? L.t_last : my_max;
-                if (DISP) emit_packed<KIND>(A.packed + eslot, o, L, A.S, r == STEP_SPECIATED);
+                if (DISP) {
+                    emit_packed<KIND>(A.packed + eslot, o, L, A.S, r == STEP_SPECIATED);
+                    if (A.ev_keys) emit_key(A, eslot, L.t_last, key_lo, key_hi);
+                }
                 if (r == STEP_SPECIATED) {
                     if (!DISP) { // rare
                         ev.ref = A.lin.ref[slot];
@@ -502,7 +538,10 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
             }
         }
     }
-    if (DISP) wave_fill_ghosts(ws, A.packed, A.packed_cap, &A.ctr->n_ghost);
+    if (DISP) {
+        wave_fill_ghosts(ws, A.packed, A.packed_cap, &A.ctr->n_ghost, A.ev_keys, A.ev_vals);
+        if (A.ev_keys) wave_flush_range(A, key_lo, key_hi);
+    }
     block_flush(my_steps, my_max, my_dead, A.ctr);
 }
 
@@ -585,6 +624,7 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
     WaveQueue q{0, 0, 0, false, false};
     unsigned long long my_steps = 0, my_dead = 0;
     double my_max = 0.0;
+    unsigned long long key_lo = ~0ull, key_hi = 0ull; // DISP: the range of the keys this lane wrote
     uint32_t iter = 0;
 
     uint32_t n_empty = POOL;
@@ -714,7 +754,10 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
                 const int r = lane_execute<KIND, false, true>(L, A.S, A.P, ev);
                 my_steps++;
                 my_max = L.t_last > my_max ? L.t_last : my_max;
-                if (DISP) emit_packed<KIND>(A.packed + eslot, o, L, A.S, r == STEP_SPECIATED);
+                if (DISP) {
+                    emit_packed<KIND>(A.packed + eslot, o, L, A.S, r == STEP_SPECIATED);
+                    if (A.ev_keys) emit_key(A, eslot, L.t_last, key_lo, key_hi);
+                }
                 if (r == STEP_SPECIATED) {
                     if (!DISP) {
                         ev.ref = A.lin.ref[slot];
@@ -736,7 +779,10 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
         }
         wave_sync_lds();
     }
-    if (DISP) wave_fill_ghosts(ws, A.packed, A.packed_cap, &A.ctr->n_ghost);
+    if (DISP) {
+        wave_fill_ghosts(ws, A.packed, A.packed_cap, &A.ctr->n_ghost, A.ev_keys, A.ev_vals);
+        if (A.ev_keys) wave_flush_range(A, key_lo, key_hi);
+    }
     block_flush(my_steps, my_max, my_dead, A.ctr);
 }
 
@@ -1548,6 +1594,25 @@ __global__ void __launch_bounds__(KS_THREADS) k_tie_check(const E *ev, const uns
     block_add(longer, flags + 1);
 }
 
+// first run heads in [lo, hi) and [hi, tail) (hi and tail if none), found
+// by the whole block in one pass over the keys
+__device__ __forceinline__ void block_first_heads(const unsigned long long *keys, uint64_t lo, uint64_t hi,
+                                                  uint64_t tail, uint64_t &s, uint64_t &e) {
+    __shared__ unsigned long long s_first[2];
+    if (threadIdx.x == 0) {
+        s_first[0] = hi;
+        s_first[1] = tail;
+    }
+    __syncthreads();
+    for (uint64_t p = lo + threadIdx.x; p < tail; p += blockDim.x)
+        if (p == 0 || keys[p] != keys[p - 1] || keys[p] == GHOST_TIME_BITS)
+            atomicMin(&s_first[p < hi ? 0 : 1], (unsigned long long)p);
+    __syncthreads();
+    s = s_first[0];
+    e = s_first[1];
+    __syncthreads();
+}
+
 // first run head in [lo, hi) (hi if none), found by the whole block
 __device__ __forceinline__ uint64_t block_first_head(const unsigned long long *keys, uint64_t lo, uint64_t hi) {
     __shared__ unsigned long long s_first;
@@ -1597,10 +1662,10 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const uns
     const uint64_t lo = (uint64_t)blockIdx.x * TIE_WINDOW;
     if (lo >= n) return;
     const uint64_t hi = lo + TIE_WINDOW < n ? lo + TIE_WINDOW : n;
-    const uint64_t s = block_first_head(keys, lo, hi);
-    if (s >= hi) return; // every position of the window continues an earlier run
     const uint64_t tail_hi = hi + TIE_RUN_MAX < n ? hi + TIE_RUN_MAX : n;
-    const uint64_t e = hi < n ? block_first_head(keys, hi, tail_hi) : n;
+    uint64_t s, e;
+    block_first_heads(keys, lo, hi, tail_hi, s, e);
+    if (s >= hi) return; // every position of the window continues an earlier run
     if (e == tail_hi && tail_hi < n && keys[tail_hi] == keys[tail_hi - 1] && keys[tail_hi] != GHOST_TIME_BITS) {
         // the run crossing the window's end goes on past its tail
         if (threadIdx.x == 0) atomicAdd(&flags[1], 1ull);
@@ -1612,46 +1677,62 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const uns
     const unsigned lane = lane_id(), wave = threadIdx.x / 64;
     if (threadIdx.x == 0) s_bad = 0;
     if (c && threadIdx.x < EC_COUNTERS) s_cnt[threadIdx.x] = 0;
+    // the thread's positions i = threadIdx.x + TIE_THREADS * k: every index,
+    // then every record, in flight at once (positions past the span read
+    // its last one and are ignored)
+    constexpr int CH = TIE_SORT_N / TIE_THREADS;
+    uint32_t vi[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        const int i = (int)threadIdx.x + TIE_THREADS * k;
+        vi[k] = vals[s + (uint64_t)(i < len ? i : len - 1)];
+    }
+    unsigned long long pr[CH], rf[CH];
+    uint32_t fl[CH][NF];
+    uint8_t cl[CH];
+    bool hd[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        const int i = (int)threadIdx.x + TIE_THREADS * k;
+        const uint64_t p = s + (uint64_t)(i < len ? i : len - 1);
+        const E rec = ev[vi[k]];
+        pr[k] = O::field(rec, 1);
+        rf[k] = O::field(rec, 0);
+        cl[k] = event_class(rec);
+#pragma unroll
+        for (int t = 0; t < NF; t++) fl[k][t] = (uint32_t)O::field(rec, 2 + t);
+        const unsigned long long key = keys[p], before = keys[p > 0 ? p - 1 : 0];
+        hd[k] = i == 0 || key == GHOST_TIME_BITS || before != key;
+    }
     int tie = 0;
     uint32_t run_base = 0; // runs that started in earlier chunks
-    for (int c0 = 0; c0 < len; c0 += TIE_THREADS) {
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        const int c0 = TIE_THREADS * k;
+        if (c0 >= len) break;
         const int i = c0 + (int)threadIdx.x;
         const bool have = i < len;
-        unsigned long long prior = 0, ref = 0;
         uint32_t f[NF];
-        uint8_t cls = 0;
-        bool head = true;
-        if (have) {
-            const uint64_t p = s + (uint64_t)i;
-            const E rec = ev[vals[p]];
-            prior = O::field(rec, 1);
-            ref = O::field(rec, 0);
-            cls = event_class(rec);
 #pragma unroll
-            for (int k = 0; k < NF; k++) f[k] = (uint32_t)O::field(rec, 2 + k);
-            const unsigned long long k = keys[p];
-            head = i == 0 || k == GHOST_TIME_BITS || keys[p - 1] != k;
-        } else {
-#pragma unroll
-            for (int k = 0; k < NF; k++) f[k] = 0;
-        }
+        for (int t = 0; t < NF; t++) f[t] = have ? fl[k][t] : 0u;
+        const bool head = !have || hd[k];
         // the previous position's origin / target: the lane below, the last
         // lane of the wave below, or the previous chunk's last position
-        uint32_t q[NF];
+        uint32_t below[NF];
 #pragma unroll
-        for (int k = 0; k < NF; k++) q[k] = (uint32_t)__shfl_up((int)f[k], 1);
+        for (int t = 0; t < NF; t++) below[t] = (uint32_t)__shfl_up((int)f[t], 1);
         if (lane == 63) {
 #pragma unroll
-            for (int k = 0; k < NF; k++) s_edge[wave][k] = f[k];
+            for (int t = 0; t < NF; t++) s_edge[wave][t] = f[t];
         }
         const uint64_t hb = __ballot(have && head);
         if (lane == 0) s_wsum[wave] = (uint32_t)__popcll(hb);
         __syncthreads();
         if (have && !head) { // the previous position is in this run: same origin and target
-            const uint32_t *qe = wave > 0 ? s_edge[wave - 1] : s_edge[NWV];
+            const uint32_t *edge = wave > 0 ? s_edge[wave - 1] : s_edge[NWV];
             bool mixed = false;
 #pragma unroll
-            for (int k = 0; k < NF; k++) mixed |= f[k] != (lane > 0 ? q[k] : qe[k]);
+            for (int t = 0; t < NF; t++) mixed |= f[t] != (lane > 0 ? below[t] : edge[t]);
             if (mixed) atomicOr(&s_bad, 1u);
             tie = 1;
         }
@@ -1663,16 +1744,16 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const uns
             chunk += s_wsum[w];
         }
         if (have) {
-            s_p[i] = prior;
-            s_r[i] = ref;
+            s_p[i] = pr[k];
+            s_r[i] = rf[k];
             s_m[i] = ((before + incl - 1) << TIE_POS_BITS) | (uint32_t)i;
-            s_cls[i] = cls;
+            s_cls[i] = cl[k];
         }
         run_base += chunk;
         __syncthreads(); // s_edge and s_wsum are rewritten by the next chunk
         if (threadIdx.x == TIE_THREADS - 1) {
 #pragma unroll
-            for (int k = 0; k < NF; k++) s_edge[NWV][k] = f[k];
+            for (int t = 0; t < NF; t++) s_edge[NWV][t] = f[t];
         }
     }
     for (int i = len + (int)threadIdx.x; i < nsort; i += TIE_THREADS) {
@@ -1882,11 +1963,13 @@ __global__ void __launch_bounds__(WR_THREADS) k_wradix_hist(const unsigned long 
     for (int d = threadIdx.x; d < WR_BINS; d += WR_THREADS) h[d] = 0;
     __syncthreads();
     const uint64_t base = (uint64_t)tile * WR_TILE;
-    unsigned long long key[WR_ITEMS]; // every load in flight before the first is used
+    // every load in flight before the first is used: positions past the end
+    // read the last key (unconditional loads are not waited for one by one)
+    unsigned long long key[WR_ITEMS];
 #pragma unroll
     for (int j = 0; j < WR_ITEMS; j++) {
         const uint64_t p = base + (uint64_t)j * WR_THREADS + threadIdx.x;
-        key[j] = p < n ? keys[p] : 0ull;
+        key[j] = keys[p < n ? p : n - 1];
     }
 #pragma unroll
     for (int j = 0; j < WR_ITEMS; j++) {
@@ -2366,106 +2449,85 @@ __global__ void k_count_round_clear(EventCounts *c);
 
 // Sort of one round's events by the PackedEvent order: an LSD radix sort of
 // the event times; if two events share a time (duplicate lineages when dedup
-// is off), the runs of equal times sorted in LDS windows, or, for runs with
-// several origins or longer than TIE_RUN_MAX, a full LSD radix sort over
-// every field of the order.  Digits above the highest one in which a key
-// range differs are equal for every key and are skipped.  Returns the
-// sorted permutation (one of sb.vals).
-template <class E>
+// is off), the runs of equal times sorted in LDS windows (k_tie_runs), or,
+// for runs with several origins or longer than TIE_RUN_MAX, a full LSD radix
+// sort over every field of the order.  Digits above the highest one in which
+// a key range differs are equal for every key and are skipped.  Returns the
+// sorted permutation (one of sb.vals).  With `keys_ready` the round's kernel
+// has already written the time keys, the identity values and the key range
+// (sb.keys[0], sb.vals[0], sb.minmax, initialised by k_sort_init).
 // With `counts` (PackedEvents of a device_event_counter round, n_real of the
 // n records real), a round whose runs of equal times go through the LDS
 // windows is counted there, into counts->round_*, and *counted is set; the
 // caller then commits it (k_count_finish).
-static const uint32_t *sort_permutation(const E *events, uint64_t n, SortBuffers &sb,
-                                        hipStream_t stream, bool fixed_passes, EventCounts *counts = nullptr,
-                                        uint64_t n_real = 0, bool *counted = nullptr) {
+template <class E>
+static const uint32_t *sort_permutation(const E *events, uint64_t n, SortBuffers &sb, hipStream_t stream,
+                                        bool keys_ready, EventCounts *counts = nullptr, uint64_t n_real = 0,
+                                        bool *counted = nullptr) {
     if (counted) *counted = false;
     using O = EventOrder<E>;
     SortSize sz{n, nullptr, nullptr, nullptr};
     unsigned g = grid_for(n, 256), gk = grid_for(n, KS_TILE);
-    hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, stream, sb.minmax);
-    hipLaunchKernelGGL(k_event_keys<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sz, sb.keys[0], sb.vals[0],
-                       sb.minmax);
-    unsigned long long mm[MM_WORDS];
-    int last = 7, wide = -1; // wide >= 0: that many WR_BITS-bit passes instead
-    unsigned long long base = 0, top = ~0ull;
-    if (!fixed_passes) {
-        (void)hipMemcpyAsync(mm, sb.minmax, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, stream);
-        (void)hipStreamSynchronize(stream);
-        // the times' range [lo, hi] (unused slots excluded): digits of
-        // key - lo, unused slots as hi + 1
-        const unsigned long long lo = mm[MM_TIME], hi = mm[MM_TIME + 1];
-        if (lo > hi) {
-            last = -1; // only unused slots
-        } else {
-            base = lo;
-            top = hi + 1;
-            last = highest_digit(top - base);
-            // with room for their histogram, wide digits: ceil(bits / WR_BITS) passes
-            if (sb.hist_words >= wide_hist_words(n) && n < (1ull << 32)) {
-                const int bits = 64 - __builtin_clzll(top - base);
-                wide = (bits + WR_BITS - 1) / WR_BITS;
-            }
-        }
+    if (!keys_ready) {
+        hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, stream, sb.minmax);
+        hipLaunchKernelGGL(k_event_keys<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sz, sb.keys[0],
+                           sb.vals[0], sb.minmax);
     }
-    int cur = wide >= 0 ? radix_passes_wide(n, sb, stream, wide, 0, base, top)
-                        : radix_passes(sz, n, sb, stream, 0, last, 0, base, top);
-    if (n > 1) {
-        hipLaunchKernelGGL(k_tie_count, dim3(gk), dim3(KS_THREADS), 0, stream, sb.keys[cur], sz, sb.minmax + MM_TIES);
-        (void)hipMemcpyAsync(&mm[MM_TIES], sb.minmax + MM_TIES, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                             stream);
-        (void)hipStreamSynchronize(stream);
-        bool runs_checked = false; // mm[MM_TIE_MIXED] covers every run
-        if (mm[MM_TIES] && !fixed_passes) {
-            // the runs of equal times sorted in LDS windows
-            static_assert(MM_TIE_LONG == MM_TIE_MIXED + 1, "tie flags are adjacent");
-            hipLaunchKernelGGL(k_tie_runs<E>, dim3(grid_for(n, TIE_WINDOW)), dim3(TIE_THREADS), 0, stream, events,
-                               sb.keys[cur], sb.vals[cur], n, sb.vals[cur ^ 1], (unsigned long long)n_real, counts,
-                               sb.minmax + MM_TIE_MIXED);
-            (void)hipMemcpyAsync(&mm[MM_TIE_MIXED], sb.minmax + MM_TIE_MIXED, 2 * sizeof(unsigned long long),
-                                 hipMemcpyDeviceToHost, stream);
-            (void)hipStreamSynchronize(stream);
-            if (!mm[MM_TIE_MIXED] && !mm[MM_TIE_LONG]) {
-                if (counted) *counted = counts != nullptr;
-                return sb.vals[cur ^ 1];
-            }
-            // windows that stopped at a long run did not check the rest
-            if (counts) hipLaunchKernelGGL(k_count_round_clear, dim3(1), dim3(1), 0, stream, counts);
-            if (!mm[MM_TIE_MIXED]) {
-                hipLaunchKernelGGL(k_tie_check<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sb.keys[cur],
-                                   sb.vals[cur], n, sb.minmax + MM_TIE_MIXED);
-                (void)hipMemcpyAsync(&mm[MM_TIE_MIXED], sb.minmax + MM_TIE_MIXED, sizeof(unsigned long long),
-                                     hipMemcpyDeviceToHost, stream);
-                (void)hipStreamSynchronize(stream);
-            }
-            runs_checked = true;
-        }
-        if (mm[MM_TIES]) {
-            if (!fixed_passes) {
-                hipLaunchKernelGGL(k_field_minmax<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sz, sb.minmax);
-                (void)hipMemcpyAsync(mm, sb.minmax, sizeof(mm), hipMemcpyDeviceToHost, stream);
-                (void)hipStreamSynchronize(stream);
-            }
-            // runs sharing origin and target need the reference, prior and time passes only
-            const bool uniform_runs = runs_checked && !mm[MM_TIE_MIXED];
-            for (int f = 0; f < O::N; f++) {
-                if (uniform_runs && f >= 2 && f < O::N - 1) continue;
-                int lf = fixed_passes ? O::digits(f) - 1
-                                      : highest_digit(mm[MM_FIELDS + 2 * f] ^ mm[MM_FIELDS + 2 * f + 1]);
-                if (lf < 0) continue;
-                hipLaunchKernelGGL(k_field_keys<E>, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], sb.keys[cur],
-                                   f, sz);
-                cur = radix_passes(sz, n, sb, stream, 0, lf, cur);
-            }
-        }
+    unsigned long long mm[MM_WORDS];
+    (void)hipMemcpyAsync(mm, sb.minmax, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, stream);
+    (void)hipStreamSynchronize(stream);
+    // the times' range [lo, hi] (unused slots excluded): digits of key - lo,
+    // unused slots as hi + 1
+    const unsigned long long lo = mm[MM_TIME], hi = mm[MM_TIME + 1];
+    if (lo > hi) return sb.vals[0]; // only unused slots, in slot order
+    const unsigned long long base = lo, top = hi + 1;
+    int cur;
+    if (sb.hist_words >= wide_hist_words(n) && n < (1ull << 32)) {
+        // with room for their histogram, wide digits: ceil(bits / WR_BITS) passes
+        const int bits = 64 - __builtin_clzll(top - base);
+        cur = radix_passes_wide(n, sb, stream, (bits + WR_BITS - 1) / WR_BITS, 0, base, top);
+    } else {
+        cur = radix_passes(sz, n, sb, stream, 0, highest_digit(top - base), 0, base, top);
+    }
+    if (n < 2) return sb.vals[cur];
+    // the runs of equal times sorted in LDS windows (a span without equal
+    // times keeps its order)
+    static_assert(MM_TIE_LONG == MM_TIE_MIXED + 1, "tie flags are adjacent");
+    hipLaunchKernelGGL(k_tie_runs<E>, dim3(grid_for(n, TIE_WINDOW)), dim3(TIE_THREADS), 0, stream, events,
+                       sb.keys[cur], sb.vals[cur], n, sb.vals[cur ^ 1], (unsigned long long)n_real, counts,
+                       sb.minmax + MM_TIE_MIXED);
+    (void)hipMemcpyAsync(&mm[MM_TIE_MIXED], sb.minmax + MM_TIE_MIXED, 2 * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, stream);
+    (void)hipStreamSynchronize(stream);
+    if (!mm[MM_TIE_MIXED] && !mm[MM_TIE_LONG]) {
+        if (counted) *counted = counts != nullptr;
+        return sb.vals[cur ^ 1];
+    }
+    // field passes: the window counts are dropped, and windows that stopped
+    // at a long run did not check the rest for several origins / targets
+    if (counts) hipLaunchKernelGGL(k_count_round_clear, dim3(1), dim3(1), 0, stream, counts);
+    if (!mm[MM_TIE_MIXED])
+        hipLaunchKernelGGL(k_tie_check<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sb.keys[cur], sb.vals[cur],
+                           n, sb.minmax + MM_TIE_MIXED);
+    hipLaunchKernelGGL(k_field_minmax<E>, dim3(gk), dim3(KS_THREADS), 0, stream, events, sz, sb.minmax);
+    (void)hipMemcpyAsync(mm, sb.minmax, sizeof(mm), hipMemcpyDeviceToHost, stream);
+    (void)hipStreamSynchronize(stream);
+    // runs sharing origin and target need the reference, prior and time passes only
+    const bool uniform_runs = !mm[MM_TIE_MIXED];
+    for (int f = 0; f < O::N; f++) {
+        if (uniform_runs && f >= 2 && f < O::N - 1) continue;
+        const int lf = highest_digit(mm[MM_FIELDS + 2 * f] ^ mm[MM_FIELDS + 2 * f + 1]);
+        if (lf < 0) continue;
+        hipLaunchKernelGGL(k_field_keys<E>, dim3(g), dim3(256), 0, stream, events, sb.vals[cur], sb.keys[cur], f, sz);
+        cur = radix_passes(sz, n, sb, stream, 0, lf, cur);
     }
     return sb.vals[cur];
 }
 
 template <class E>
-void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream_t stream, bool fixed_passes) {
+void sort_events(const E *events, uint64_t n, E *out, SortBuffers &sb, hipStream_t stream, bool keys_ready) {
     if (n == 0) return;
-    const uint32_t *perm = sort_permutation(events, n, sb, stream, fixed_passes);
+    const uint32_t *perm = sort_permutation(events, n, sb, stream, keys_ready);
     SortSize sz{n, nullptr, nullptr, nullptr};
     hipLaunchKernelGGL(k_gather_events<E>, dim3(grid_for(n, 256)), dim3(256), 0, stream, events, perm, out, sz);
 }
@@ -2643,11 +2705,11 @@ __global__ void k_count_round_clear(EventCounts *c) {
 
 // A round of a device_event_counter job: the events' sorted permutation, then
 // the counters read the events through it (no sorted copy is written)
-void sort_and_count_packed(const DevPacked *events, uint64_t slots, uint64_t n_events,
+void sort_and_count_packed(const DevPacked *events, uint64_t slots, uint64_t n_events, bool keys_ready,
                            SortBuffers &sb, EventCounts *c, hipStream_t stream) {
     if (slots == 0 || n_events == 0) return;
     bool counted = false;
-    const uint32_t *perm = sort_permutation(events, slots, sb, stream, false, c, n_events, &counted);
+    const uint32_t *perm = sort_permutation(events, slots, sb, stream, keys_ready, c, n_events, &counted);
     if (!counted) {
         launch_count_events(events, perm, nullptr, n_events, nullptr, c, stream);
         return;
@@ -2690,6 +2752,10 @@ void launch_soa_compact_gated(const LineageSoA &in, const LineageSoA &out, uint6
     hipLaunchKernelGGL(k_soa_gate, dim3(1), dim3(1), 0, stream, blk, nb, cap, n_dev, ctr);
     hipLaunchKernelGGL(k_soa_scatter, dim3(nb), dim3(SC_THREADS), 0, stream, in, out, cap, n_dev,
                        (const Counters *)ctr, blk);
+}
+
+void launch_sort_init(SortBuffers &sb, hipStream_t stream) {
+    hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, stream, sb.minmax);
 }
 
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream) {

@@ -1071,6 +1071,11 @@ static PersistArgs persist_args(nh_job *j, int cur) {
     a.coop = j->coop ? 1u : 0u;
     a.exec_fill = j->p.exec_fill ? j->p.exec_fill : 5;
     a.pool = j->pooled ? j->p.pool : 0u;
+    if (j->disp) { // the kernel writes the round sort's keys, values and key range
+        a.ev_keys = j->sb.keys[0];
+        a.ev_vals = j->sb.vals[0];
+        a.ev_range = j->sb.minmax;
+    }
     return a;
 }
 
@@ -1091,6 +1096,7 @@ static int persistent_round(nh_job *j, bool *sorted) {
             hipGraph_t graph = nullptr;
             NH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             launch_reset_counters(j->d_ctr, j->disp, st);
+            if (j->disp) launch_sort_init(j->sb, st);
             launch_persistent(j->tables->kind, persist_args(j, g), j->blocks, j->threads, st);
             if (!j->disp)
                 sort_events_device_sized(j->d_events, &j->d_ctr->n_events, &j->d_round->ev_start, j->n_initial,
@@ -1109,6 +1115,7 @@ static int persistent_round(nh_job *j, bool *sorted) {
         *sorted = !j->disp;
     } else {
         launch_reset_counters(j->d_ctr, j->disp, st);
+        if (j->disp) launch_sort_init(j->sb, st);
         NH_HIP(hipEventRecord(j->ev0, st));
         launch_persistent(j->tables->kind, persist_args(j, j->cur), j->blocks, j->threads, st);
         NH_HIP(hipGetLastError());
@@ -1215,14 +1222,16 @@ extern "C" int nh_job_step_round(nh_job *j, int32_t *finished) {
     j->rounds++;
 
     // the live water-level reporter sorts each round's events (live.rs:88-115)
-    // (ghost records sort after every event)
+    // (ghost records sort after every event); the persistent form's kernels
+    // wrote the sort's time keys as they reported the events
+    const bool keys_ready = j->p.form == NH_FORM_PERSISTENT;
     if (ne && j->disp && j->d_counts) {
         // the counters read the round through the sort's permutation: no
         // sorted copy (nothing drains a counted round)
-        if (!sorted) sort_and_count_packed(j->d_packed, slots, ne, j->sb, j->d_counts, st);
+        if (!sorted) sort_and_count_packed(j->d_packed, slots, ne, keys_ready, j->sb, j->d_counts, st);
         NH_HIP(hipGetLastError());
     } else if (ne && j->disp) {
-        if (!sorted) sort_events(j->d_packed, slots, j->d_packed_sorted, j->sb, st, false);
+        if (!sorted) sort_events(j->d_packed, slots, j->d_packed_sorted, j->sb, st, keys_ready);
         NH_HIP(hipGetLastError());
         NH_HIP(hipStreamSynchronize(st));
         j->dev_pos = 0;
