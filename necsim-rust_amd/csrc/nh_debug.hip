@@ -179,12 +179,12 @@ static int debug_sort(nh_ctx *ctx, const E *in, size_t n, int32_t device_sized, 
     unsigned long long *d_range = nullptr;
     int rc = NH_OK;
     if ((rc = dalloc(&sb.keys[0], n)) || (rc = dalloc(&sb.keys[1], n)) || (rc = dalloc(&sb.vals[0], n)) ||
-        (rc = dalloc(&sb.vals[1], n)) || (rc = dalloc(&sb.hist, sort_hist_words_wide(n))) ||
+        (rc = dalloc(&sb.vals[1], n)) || (rc = dalloc(&sb.hist, sort_hist_words_staged(n))) ||
         (rc = dalloc(&sb.minmax, SORT_MINMAX_WORDS)) || (rc = dalloc(&d_in, n)) || (rc = dalloc(&d_out, n)) ||
         (rc = dalloc(&d_range, 2))) {
     } else {
         sb.capacity = n;
-        sb.hist_words = sort_hist_words_wide(n);
+        sb.hist_words = sort_hist_words_staged(n);
         unsigned long long range[2] = {0ull, (unsigned long long)n}; // start, end
         hipError_t e = hipMemcpyAsync(d_in, in, n * sizeof(E), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(d_range, range, sizeof(range), hipMemcpyHostToDevice, st);

@@ -187,11 +187,11 @@ struct SortBuffers {
     uint32_t *hist;         // [hist_words]: digit counts, then scan chunk sums
     unsigned long long *minmax; // [SORT_MINMAX_WORDS] device: key ranges, tie count
     uint64_t capacity;
-    uint64_t hist_words;    // sort_hist_words(capacity), or sort_hist_words_wide(capacity) for wide-digit passes
+    uint64_t hist_words;    // sort_hist_words(capacity), or sort_hist_words_staged(capacity) for staged passes
 };
 uint32_t sort_blocks(uint64_t n);
 size_t sort_hist_words(uint64_t n);      // words of SortBuffers::hist for n events (8-bit digits)
-size_t sort_hist_words_wide(uint64_t n); // ... with room for the wide-digit passes of eager sorts
+size_t sort_hist_words_staged(uint64_t n); // ... with room for the staged passes of eager sorts
 // keys_ready: the round's kernel wrote the time keys, identity values and
 // key range into sb (k_sort_init first; dispersal-reporting persistent rounds)
 template <class E>

@@ -1903,34 +1903,35 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const unsigned lon
 }
 
 // ---------------------------------------------------------------------------
-// Wide-digit LSD passes for eager sorts (the time keys of a round): 10-bit
-// digits, so a round's 47-50-bit key range takes 5 passes instead of 6-7.
-// A tile of WR_TILE keys is ranked by digit in LDS (stable: wave-major, then
-// round, then lane, which is the tile's own order) and written out in digit
-// order, so each digit's run of the tile (8 keys on average) goes to
-// consecutive addresses.  Measured on 25.1M-event rounds, per round: 8-bit
-// digits 2.08 ms, 9-bit 2.05, 10-bit 2.01, 11-bit 2.31 (its runs of 4 keys
-// write partial lines); the per-pass 8-bit sort without staging 2.60.
-constexpr int WR_BITS = 10;
-constexpr int WR_BINS = 1 << WR_BITS;
-constexpr int WR_THREADS = 512;
-constexpr int WR_WAVES = WR_THREADS / 64;
-constexpr int WR_ITEMS = 16;
-constexpr int WR_TILE = WR_THREADS * WR_ITEMS;
-constexpr int WR_DPT = WR_BINS >= WR_THREADS ? WR_BINS / WR_THREADS : 1; // digits per thread in the tile scan
-static_assert(64 * WR_ITEMS < (1 << 16) && WR_BINS <= (1 << 16), "ranks and digits packed in 16 bits each");
-static_assert(WR_BINS % WR_THREADS == 0 || WR_THREADS % WR_BINS == 0, "digits spread evenly over threads");
+// Staged LSD passes for eager sorts (the time keys of a round).  A tile of
+// SR_TILE keys is ranked by digit in LDS (stable: wave-major, then round,
+// then lane, which is the tile's own order) and written out in digit order,
+// so each digit's run of the tile (16 keys on average) goes to consecutive
+// addresses.  Measured on 25.1M-event rounds (radix time per round, the
+// digit width and tile chosen from): 8-bit digits in tiles of 4096 keys
+// (two workgroups per CU) 1.48 ms; 9-bit 1.52; 10-bit 1.59; 10-bit in tiles
+// of 8192 (one workgroup per CU) 2.01; 11-bit 2.31; the per-pass 8-bit sort
+// without staging 2.60.
+constexpr int SR_BITS = 8;
+constexpr int SR_BINS = 1 << SR_BITS;
+constexpr int SR_THREADS = 512;
+constexpr int SR_WAVES = SR_THREADS / 64;
+constexpr int SR_ITEMS = 8;
+constexpr int SR_TILE = SR_THREADS * SR_ITEMS;
+constexpr int SR_DPT = SR_BINS >= SR_THREADS ? SR_BINS / SR_THREADS : 1; // digits per thread in the tile scan
+static_assert(64 * SR_ITEMS < (1 << 16) && SR_BINS <= (1 << 16), "ranks and digits packed in 16 bits each");
+static_assert(SR_BINS % SR_THREADS == 0 || SR_THREADS % SR_BINS == 0, "digits spread evenly over threads");
 
-__device__ __forceinline__ uint32_t wradix_digit(unsigned long long key, unsigned long long base,
+__device__ __forceinline__ uint32_t sradix_digit(unsigned long long key, unsigned long long base,
                                                  unsigned long long top, int shift) {
-    return (uint32_t)(((key < top ? key : top) - base) >> shift) & (uint32_t)(WR_BINS - 1);
+    return (uint32_t)(((key < top ? key : top) - base) >> shift) & (uint32_t)(SR_BINS - 1);
 }
 
 // lanes of the wave whose digit equals this lane's (valid lanes only)
-__device__ __forceinline__ uint64_t wradix_peers(uint32_t digit, bool valid) {
+__device__ __forceinline__ uint64_t sradix_peers(uint32_t digit, bool valid) {
     uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (int bit = 0; bit < WR_BITS; bit++) {
+    for (int bit = 0; bit < SR_BITS; bit++) {
         const bool set = (digit >> bit) & 1u;
         const uint64_t mset = __ballot(valid && set);
         peers &= set ? mset : ~mset;
@@ -1938,7 +1939,7 @@ __device__ __forceinline__ uint64_t wradix_peers(uint32_t digit, bool valid) {
     return peers;
 }
 
-__device__ __forceinline__ uint32_t wradix_tiles(uint64_t n) { return (uint32_t)((n + WR_TILE - 1) / WR_TILE); }
+__device__ __forceinline__ uint32_t sradix_tiles(uint64_t n) { return (uint32_t)((n + SR_TILE - 1) / SR_TILE); }
 
 // Workgroups go to the 8 XCDs round-robin, each XCD with its own L2.  Block
 // b takes tile (b % 8) * per + b / 8, so one XCD works through consecutive
@@ -1951,80 +1952,80 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t tiles) {
     return (b % XCDS) * ((tiles + XCDS - 1) / XCDS) + b / XCDS;
 }
 
-__global__ void __launch_bounds__(WR_THREADS) k_wradix_hist(const unsigned long long *keys, SortSize sz, int shift,
+__global__ void __launch_bounds__(SR_THREADS) k_sradix_hist(const unsigned long long *keys, SortSize sz, int shift,
                                                             unsigned long long key_base, unsigned long long top,
                                                             uint32_t *hist) {
-    __shared__ uint32_t h[WR_BINS];
+    __shared__ uint32_t h[SR_BINS];
     if (sz.skip()) return;
     const uint64_t n = sz.get();
-    const uint32_t nb = wradix_tiles(n);
+    const uint32_t nb = sradix_tiles(n);
     const uint32_t tile = xcd_tile(blockIdx.x, nb);
     if (tile >= nb) return;
-    for (int d = threadIdx.x; d < WR_BINS; d += WR_THREADS) h[d] = 0;
+    for (int d = threadIdx.x; d < SR_BINS; d += SR_THREADS) h[d] = 0;
     __syncthreads();
-    const uint64_t base = (uint64_t)tile * WR_TILE;
+    const uint64_t base = (uint64_t)tile * SR_TILE;
     // every load in flight before the first is used: positions past the end
     // read the last key (unconditional loads are not waited for one by one)
-    unsigned long long key[WR_ITEMS];
+    unsigned long long key[SR_ITEMS];
 #pragma unroll
-    for (int j = 0; j < WR_ITEMS; j++) {
-        const uint64_t p = base + (uint64_t)j * WR_THREADS + threadIdx.x;
+    for (int j = 0; j < SR_ITEMS; j++) {
+        const uint64_t p = base + (uint64_t)j * SR_THREADS + threadIdx.x;
         key[j] = keys[p < n ? p : n - 1];
     }
 #pragma unroll
-    for (int j = 0; j < WR_ITEMS; j++) {
-        const uint64_t p = base + (uint64_t)j * WR_THREADS + threadIdx.x;
+    for (int j = 0; j < SR_ITEMS; j++) {
+        const uint64_t p = base + (uint64_t)j * SR_THREADS + threadIdx.x;
         const bool valid = p < n;
-        const uint32_t digit = valid ? wradix_digit(key[j], key_base, top, shift) : 0u;
+        const uint32_t digit = valid ? sradix_digit(key[j], key_base, top, shift) : 0u;
         // one LDS atomic per distinct digit of the wave (sorted keys repeat digits)
-        const uint64_t peers = wradix_peers(digit, valid);
+        const uint64_t peers = sradix_peers(digit, valid);
         if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&h[digit], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < WR_BINS; d += WR_THREADS) hist[(uint64_t)d * nb + tile] = h[d];
+    for (int d = threadIdx.x; d < SR_BINS; d += SR_THREADS) hist[(uint64_t)d * nb + tile] = h[d];
 }
 
-// `offsets`: the exclusive scan of the digit-major histogram of k_wradix_hist
-__global__ void __launch_bounds__(WR_THREADS) k_wradix_scatter(const unsigned long long *kin, const uint32_t *vin,
+// `offsets`: the exclusive scan of the digit-major histogram of k_sradix_hist
+__global__ void __launch_bounds__(SR_THREADS) k_sradix_scatter(const unsigned long long *kin, const uint32_t *vin,
                                                                unsigned long long *kout, uint32_t *vout,
                                                                SortSize sz, int shift, unsigned long long key_base,
                                                                unsigned long long top, const uint32_t *offsets) {
-    __shared__ unsigned long long s_key[WR_TILE];
-    __shared__ uint32_t s_val[WR_TILE];
-    __shared__ uint16_t s_wc[WR_WAVES][WR_BINS]; // per wave: digit counts, then the counts of the waves before
-    __shared__ uint32_t s_start[WR_BINS];        // the digit's first position in the tile
-    __shared__ uint32_t s_dst[WR_BINS];          // global position of the tile's position 0 in the digit's run
-    __shared__ uint32_t s_wsum[WR_WAVES];
+    __shared__ unsigned long long s_key[SR_TILE];
+    __shared__ uint32_t s_val[SR_TILE];
+    __shared__ uint16_t s_wc[SR_WAVES][SR_BINS]; // per wave: digit counts, then the counts of the waves before
+    __shared__ uint32_t s_start[SR_BINS];        // the digit's first position in the tile
+    __shared__ uint32_t s_dst[SR_BINS];          // global position of the tile's position 0 in the digit's run
+    __shared__ uint32_t s_wsum[SR_WAVES];
     if (sz.skip()) return;
     const uint64_t n = sz.get();
-    const uint32_t nb = wradix_tiles(n);
+    const uint32_t nb = sradix_tiles(n);
     const uint32_t tile = xcd_tile(blockIdx.x, nb);
     if (tile >= nb) return;
     const unsigned lane = lane_id(), wave = threadIdx.x / 64;
     // the global offsets of the thread's digits, read while the tile is ranked
-    const bool has_digits = (int)threadIdx.x * WR_DPT < WR_BINS;
-    uint32_t goff[WR_DPT];
+    const bool has_digits = (int)threadIdx.x * SR_DPT < SR_BINS;
+    uint32_t goff[SR_DPT];
 #pragma unroll
-    for (int k = 0; k < WR_DPT; k++)
-        goff[k] = has_digits ? offsets[(uint64_t)(WR_DPT * threadIdx.x + k) * nb + tile] : 0u;
-    for (int i = threadIdx.x; i < WR_WAVES * WR_BINS; i += WR_THREADS) (&s_wc[0][0])[i] = 0;
+    for (int k = 0; k < SR_DPT; k++)
+        goff[k] = has_digits ? offsets[(uint64_t)(SR_DPT * threadIdx.x + k) * nb + tile] : 0u;
+    for (int i = threadIdx.x; i < SR_WAVES * SR_BINS; i += SR_THREADS) (&s_wc[0][0])[i] = 0;
     __syncthreads();
-    const uint64_t tbase = (uint64_t)tile * WR_TILE;
-    const uint64_t wbase = tbase + (uint64_t)wave * (64 * WR_ITEMS);
-    unsigned long long key[WR_ITEMS];
-    uint32_t val[WR_ITEMS], dr[WR_ITEMS];
+    const uint64_t tbase = (uint64_t)tile * SR_TILE;
+    const uint64_t wbase = tbase + (uint64_t)wave * (64 * SR_ITEMS);
+    unsigned long long key[SR_ITEMS];
+    uint32_t val[SR_ITEMS], dr[SR_ITEMS];
 #pragma unroll
-    for (int j = 0; j < WR_ITEMS; j++) {
+    for (int j = 0; j < SR_ITEMS; j++) {
         const uint64_t p = wbase + (uint64_t)j * 64 + lane;
         key[j] = p < n ? kin[p] : 0ull;
         val[j] = p < n ? vin[p] : 0u;
     }
     // rank of each key among the wave's keys of its digit, in wave order
 #pragma unroll
-    for (int j = 0; j < WR_ITEMS; j++) {
+    for (int j = 0; j < SR_ITEMS; j++) {
         const bool valid = wbase + (uint64_t)j * 64 + lane < n;
-        const uint32_t digit = valid ? wradix_digit(key[j], key_base, top, shift) : 0u;
-        const uint64_t peers = wradix_peers(digit, valid);
+        const uint32_t digit = valid ? sradix_digit(key[j], key_base, top, shift) : 0u;
+        const uint64_t peers = sradix_peers(digit, valid);
         uint32_t r = 0;
         if (valid) {
             const uint32_t c = s_wc[wave][digit];
@@ -2036,13 +2037,13 @@ __global__ void __launch_bounds__(WR_THREADS) k_wradix_scatter(const unsigned lo
     __syncthreads();
     // digits DPT t .. DPT t + DPT - 1: counts of the waves before each wave,
     // tile totals, and their exclusive scan over the tile's digits
-    uint32_t tot[WR_DPT], sum = 0;
+    uint32_t tot[SR_DPT], sum = 0;
 #pragma unroll
-    for (int k = 0; k < WR_DPT; k++) {
-        const int d = WR_DPT * threadIdx.x + k;
+    for (int k = 0; k < SR_DPT; k++) {
+        const int d = SR_DPT * threadIdx.x + k;
         uint32_t run = 0;
         if (has_digits) {
-            for (int w = 0; w < WR_WAVES; w++) {
+            for (int w = 0; w < SR_WAVES; w++) {
                 const uint32_t c = s_wc[w][d];
                 s_wc[w][d] = (uint16_t)run;
                 run += c;
@@ -2061,8 +2062,8 @@ __global__ void __launch_bounds__(WR_THREADS) k_wradix_scatter(const unsigned lo
     uint32_t run = incl - sum;
     for (unsigned w = 0; w < wave; w++) run += s_wsum[w];
 #pragma unroll
-    for (int k = 0; k < WR_DPT; k++) {
-        const int d = WR_DPT * threadIdx.x + k;
+    for (int k = 0; k < SR_DPT; k++) {
+        const int d = SR_DPT * threadIdx.x + k;
         if (has_digits) {
             s_start[d] = run;
             s_dst[d] = goff[k] - run;
@@ -2071,7 +2072,7 @@ __global__ void __launch_bounds__(WR_THREADS) k_wradix_scatter(const unsigned lo
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < WR_ITEMS; j++) {
+    for (int j = 0; j < SR_ITEMS; j++) {
         if (wbase + (uint64_t)j * 64 + lane < n) {
             const uint32_t d = dr[j] & 0xffffu;
             const uint32_t pos = s_start[d] + s_wc[wave][d] + (dr[j] >> 16);
@@ -2080,10 +2081,10 @@ __global__ void __launch_bounds__(WR_THREADS) k_wradix_scatter(const unsigned lo
         }
     }
     __syncthreads();
-    const uint32_t cnt = (uint32_t)(n - tbase < (uint64_t)WR_TILE ? n - tbase : (uint64_t)WR_TILE);
-    for (uint32_t i = threadIdx.x; i < cnt; i += WR_THREADS) {
+    const uint32_t cnt = (uint32_t)(n - tbase < (uint64_t)SR_TILE ? n - tbase : (uint64_t)SR_TILE);
+    for (uint32_t i = threadIdx.x; i < cnt; i += SR_THREADS) {
         const unsigned long long k = s_key[i];
-        const uint32_t dst = s_dst[wradix_digit(k, key_base, top, shift)] + i;
+        const uint32_t dst = s_dst[sradix_digit(k, key_base, top, shift)] + i;
         kout[dst] = k;
         vout[dst] = s_val[i];
     }
@@ -2388,13 +2389,13 @@ size_t sort_hist_words(uint64_t n) {
     return (size_t)(h + 1 + (h + HS_CHUNK - 1) / HS_CHUNK + 1);
 }
 
-static size_t wide_hist_words(uint64_t n) {
-    uint64_t h = (uint64_t)WR_BINS * ((n + WR_TILE - 1) / WR_TILE);
+static size_t staged_hist_words(uint64_t n) {
+    uint64_t h = (uint64_t)SR_BINS * ((n + SR_TILE - 1) / SR_TILE);
     return (size_t)(h + 1 + (h + HS_CHUNK - 1) / HS_CHUNK + 1);
 }
 
-size_t sort_hist_words_wide(uint64_t n) {
-    const size_t a = sort_hist_words(n), b = wide_hist_words(n);
+size_t sort_hist_words_staged(uint64_t n) {
+    const size_t a = sort_hist_words(n), b = staged_hist_words(n);
     return a > b ? a : b;
 }
 
@@ -2416,24 +2417,24 @@ static int radix_passes(SortSize sz, uint64_t cap, SortBuffers &sb, hipStream_t 
     return cur;
 }
 
-// `passes` WR_BITS-bit digit passes from bit 0 (eager sorts: n known on the host)
-static int radix_passes_wide(uint64_t n, SortBuffers &sb, hipStream_t stream, int passes, int cur,
+// `passes` SR_BITS-bit digit passes from bit 0 (eager sorts: n known on the host)
+static int radix_passes_staged(uint64_t n, SortBuffers &sb, hipStream_t stream, int passes, int cur,
                              unsigned long long base, unsigned long long top) {
     SortSize sz{n, nullptr, nullptr, nullptr};
-    const uint32_t nb = (uint32_t)((n + WR_TILE - 1) / WR_TILE), grid = xcd_grid(nb);
-    const uint64_t words = (uint64_t)WR_BINS * nb;
+    const uint32_t nb = (uint32_t)((n + SR_TILE - 1) / SR_TILE), grid = xcd_grid(nb);
+    const uint64_t words = (uint64_t)SR_BINS * nb;
     const uint32_t nc = (uint32_t)((words + HS_CHUNK - 1) / HS_CHUNK);
     uint32_t *chunk_sums = sb.hist + words + 1;
     for (int d = 0; d < passes; d++) {
-        hipLaunchKernelGGL(k_wradix_hist, dim3(grid), dim3(WR_THREADS), 0, stream, sb.keys[cur], sz, WR_BITS * d, base,
+        hipLaunchKernelGGL(k_sradix_hist, dim3(grid), dim3(SR_THREADS), 0, stream, sb.keys[cur], sz, SR_BITS * d, base,
                            top, sb.hist);
-        hipLaunchKernelGGL((k_hist_chunk_sums<WR_BINS, WR_TILE>), dim3(nc), dim3(HS_THREADS), 0, stream, sb.hist, sz,
+        hipLaunchKernelGGL((k_hist_chunk_sums<SR_BINS, SR_TILE>), dim3(nc), dim3(HS_THREADS), 0, stream, sb.hist, sz,
                            chunk_sums);
-        hipLaunchKernelGGL((k_scan_chunk_sums<WR_BINS, WR_TILE>), dim3(1), dim3(1024), 0, stream, chunk_sums, sz);
-        hipLaunchKernelGGL((k_hist_scan_apply<WR_BINS, WR_TILE>), dim3(nc), dim3(HS_THREADS), 0, stream, sb.hist, sz,
+        hipLaunchKernelGGL((k_scan_chunk_sums<SR_BINS, SR_TILE>), dim3(1), dim3(1024), 0, stream, chunk_sums, sz);
+        hipLaunchKernelGGL((k_hist_scan_apply<SR_BINS, SR_TILE>), dim3(nc), dim3(HS_THREADS), 0, stream, sb.hist, sz,
                            chunk_sums);
-        hipLaunchKernelGGL(k_wradix_scatter, dim3(grid), dim3(WR_THREADS), 0, stream, sb.keys[cur], sb.vals[cur],
-                           sb.keys[cur ^ 1], sb.vals[cur ^ 1], sz, WR_BITS * d, base, top, sb.hist);
+        hipLaunchKernelGGL(k_sradix_scatter, dim3(grid), dim3(SR_THREADS), 0, stream, sb.keys[cur], sb.vals[cur],
+                           sb.keys[cur ^ 1], sb.vals[cur ^ 1], sz, SR_BITS * d, base, top, sb.hist);
         cur ^= 1;
     }
     return cur;
@@ -2482,10 +2483,10 @@ static const uint32_t *sort_permutation(const E *events, uint64_t n, SortBuffers
     if (lo > hi) return sb.vals[0]; // only unused slots, in slot order
     const unsigned long long base = lo, top = hi + 1;
     int cur;
-    if (sb.hist_words >= wide_hist_words(n) && n < (1ull << 32)) {
-        // with room for their histogram, wide digits: ceil(bits / WR_BITS) passes
+    if (sb.hist_words >= staged_hist_words(n) && n < (1ull << 32)) {
+        // with room for their histogram, staged passes: ceil(bits / SR_BITS) of them
         const int bits = 64 - __builtin_clzll(top - base);
-        cur = radix_passes_wide(n, sb, stream, (bits + WR_BITS - 1) / WR_BITS, 0, base, top);
+        cur = radix_passes_staged(n, sb, stream, (bits + SR_BITS - 1) / SR_BITS, 0, base, top);
     } else {
         cur = radix_passes(sz, n, sb, stream, 0, highest_digit(top - base), 0, base, top);
     }

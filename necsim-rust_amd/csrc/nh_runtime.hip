@@ -780,7 +780,7 @@ static int job_start(nh_job *j) {
     NH_TRY(dalloc(&j->sb.keys[1], sort_cap));
     NH_TRY(dalloc(&j->sb.vals[0], sort_cap));
     NH_TRY(dalloc(&j->sb.vals[1], sort_cap));
-    j->sb.hist_words = sort_hist_words_wide(sort_cap);
+    j->sb.hist_words = sort_hist_words_staged(sort_cap);
     NH_TRY(dalloc(&j->sb.hist, j->sb.hist_words));
     NH_TRY(dalloc(&j->sb.minmax, SORT_MINMAX_WORDS));
     j->pending.clear();

@@ -231,7 +231,7 @@ extern "C" int nh_server_create(nh_ctx *ctx, const nh_tables *tables, const nh_s
     if ((rc = dalloc(&s->d_blk, soa_compact_blocks(cap) + 1))) return fail(rc);
     if ((rc = dalloc(&s->d_sorted, cap))) return fail(rc);
     s->sb.capacity = cap;
-    s->sb.hist_words = sort_hist_words_wide(cap);
+    s->sb.hist_words = sort_hist_words_staged(cap);
     if ((rc = dalloc(&s->sb.keys[0], cap)) || (rc = dalloc(&s->sb.keys[1], cap)) ||
         (rc = dalloc(&s->sb.vals[0], cap)) || (rc = dalloc(&s->sb.vals[1], cap)) ||
         (rc = dalloc(&s->sb.hist, s->sb.hist_words)) || (rc = dalloc(&s->sb.minmax, SORT_MINMAX_WORDS)))
