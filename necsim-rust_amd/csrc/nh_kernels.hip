@@ -571,7 +571,6 @@ constexpr uint32_t PE_NOT_STARTED = 0xffffffffu; // win_left of a search that ha
 
 template <int POOL, bool REF> struct PoolLds {
     uint64_t lkey[POOL];
-    uint64_t ref[REF ? POOL : 1]; // dispersal reporting: each event carries the lineage's reference
     uint64_t st[POOL];    // searching: the open window's RNG state; ready: the event's prime key
     uint64_t ts[POOL];    // searching: the window
     double t_last[POOL];
@@ -582,6 +581,9 @@ template <int POOL, bool REF> struct PoolLds {
     uint8_t state[POOL];
     uint8_t assign[64];
     double tab[3 * PTBL]; // the Poisson table's acc / prod / acc_m
+    // dispersal reporting: each event carries the lineage's reference (last,
+    // so that the other arrays keep offsets that pair LDS accesses)
+    uint64_t ref[REF ? POOL : 1];
 };
 
 // Lane r (r < the number of selected entries, at most 64) gets the r-th
@@ -632,12 +634,14 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
     for (;; iter++) {
         // refill the lane's empty entries from the work queue, one per lane
         // and call (a dead slot leaves its entry empty).  Refills wait until
-        // half the pool is empty, the pool runs low or 16 iterations have
-        // passed: every claim is an atomic on the one queue head, and a
-        // refill waits for its loads, which on this hardware also waits for
-        // every store issued before them (the event records of dispersal
-        // reporting).  A quarter measured 4 % slower on dispersal rounds.
-        const bool refill = !q.exhausted && (n_empty * 2 >= POOL || n_empty + 64 > POOL || (iter & 15) == 0);
+        // a quarter of the pool is empty (dispersal reporting: half), the
+        // pool runs low or 16 iterations have passed: every claim is an
+        // atomic on the one queue head, and a refill waits for its loads,
+        // which on this hardware also waits for every store issued before
+        // them (the event records of dispersal reporting, where a quarter
+        // measured 4 % slower; half was 1 % slower on the dense headline).
+        const bool refill = !q.exhausted && (n_empty * (DISP ? 2u : 4u) >= POOL || n_empty + 64 > POOL ||
+                                             (iter & 15) == 0);
         uint8_t own[NW];
 #pragma unroll
         for (int j = 0; j < NW; j++) {
