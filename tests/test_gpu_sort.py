@@ -74,6 +74,14 @@ CASES = {
     "coalesced_runs": lambda rng: coalesced_runs(rng, 3000, 40),
     # runs longer than the in-place ranking handles (per-field passes)
     "long_duplicate_runs": lambda rng: duplicate_runs(rng, 3, 5000),
+    # the staged radix passes rank tiles of 4096 keys, and their workgroups
+    # take tiles in an order that spreads the 8 XCDs over consecutive tiles:
+    # one full tile, one key short of it, and a tile count that is not a
+    # multiple of 8 (grid blocks past the last tile exit)
+    "one_tile": lambda rng: random_events(rng, 4096, n_times=3000),
+    "one_tile_short": lambda rng: random_events(rng, 4095, n_times=3000),
+    "nine_tiles_and_a_bit": lambda rng: np.concatenate([random_events(rng, 9 * 4096 + 17 - 600, n_times=20000),
+                                                        duplicate_runs(rng, 20, 30)]),
 }
 
 
