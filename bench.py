@@ -158,10 +158,11 @@ def config2_run(ctx):
 GATHER_BYTES_PER_STEP = 48  # cell record 32 B + alias atom 16 B per dispersal step
 
 
-def map_window_run(ctx, size, nu, window, dedup, seed=42):
+def map_window_run(ctx, size, nu, window, dedup, seed=42, tune=False):
     """A window of the spatially explicit map scenario (the generator of
     configs 3 and 5): every dispersal gathers the target cell's record and
-    one alias atom.  Returns timings and the gather-bound roofline."""
+    one alias atom.  Returns timings and the gather-bound roofline.  With
+    tune, the launch shape is chosen by untimed runs of the same window."""
     h, rp, c, w = nh.generate_map(size, size, seed=42)
     scn = nh.SpatiallyExplicitUniformTurnoverScenario(size, size, h, rp, c, w)
     # resident tables: alias build on the host + upload, or on the GPU
@@ -175,12 +176,34 @@ def map_window_run(ctx, size, nu, window, dedup, seed=42):
     device_s = time.perf_counter() - t
     lin = cogs.sample_habitat(nh.OriginPreSampler.all())
     atoms = int(len(scn.tables()[2]))
-    wall, res = timed_runs(ctx, tables, cogs, lin, nh.IndependentArguments(dedup_cache=dedup), seed, window, 1, 1)
+    args = nh.IndependentArguments(dedup_cache=dedup)
+    tuned = None
+    if tune:
+        # launch shape chosen on the window itself: one untimed run with the
+        # default shape and one per residency of 256-thread blocks.  These
+        # windows hold 0.4-1.6 lineages per lane and end in serial chains,
+        # which can run faster with fewer resident waves per SIMD.
+        t = time.perf_counter()
+        trials = {"default": args}
+        for bpc in (1, 2, 3, 4):
+            trials[f"256x{bpc}"] = dataclasses.replace(args, block_size=256, blocks_per_cu=bpc)
+        trial_ms = {}
+        for name, targs in trials.items():
+            tj = make_job(ctx, tables, cogs, lin, targs, seed, window)
+            tt = time.perf_counter()
+            run_collect(tj)
+            trial_ms[name] = round((time.perf_counter() - tt) * 1e3, 3)
+            tj.close()
+        best = min(trial_ms, key=trial_ms.get)
+        args = trials[best]
+        tuned = {"chosen": best, "trial_ms": trial_ms, "tune_s": round(time.perf_counter() - t, 3)}
+    wall, res = timed_runs(ctx, tables, cogs, lin, args, seed, window, 1, 1)
     o, ms, k = res[0]
     achieved = o.steps * GATHER_BYTES_PER_STEP / (ms * 1e-3) / 1e9 if ms else 0.0
     bound = HBM_PEAK_GBS * 1e9 / GATHER_BYTES_PER_STEP
     return {"map": f"{size}x{size}", "lineages": int(len(lin)), "window": window, "speciation": nu,
-            "dedup": "none" if dedup.mode == 0 else "relative 1.0", "steps": int(o.steps), "ms": wall * 1e3,
+            "dedup": "none" if dedup.mode == 0 else "relative 1.0", "launch_shape": tuned or "default",
+            "steps": int(o.steps), "ms": wall * 1e3,
             "value": o.steps / wall, "kernel_ms": ms, "launches": int(k), "atoms": atoms,
             "table_bytes": atoms * 16 + size * size * 32,
             "tables_build_s": {"host_alias_build_and_upload": host_s, "device_alias_build": device_s},
@@ -217,7 +240,7 @@ def chain_latency_run(ctx, window=20000.0):
 
 def config3_run(ctx, window):
     """config 3 (128x128 map, nu 1e-5, the reference's default dedup)"""
-    return map_window_run(ctx, 128, 1e-5, window, nh.DedupCache.Relative(1.0))
+    return map_window_run(ctx, 128, 1e-5, window, nh.DedupCache.Relative(1.0), tune=True)
 
 
 def gather_run(ctx, window):
