@@ -74,6 +74,21 @@ def test_pooled_map_window(pool, packed):
 
 
 @pytest.mark.parametrize("pool", POOLS)
+@pytest.mark.parametrize("packed", [False, True])
+def test_pooled_map_to_completion(pool, packed):
+    # the alias kinds split the execution pass around the search pass; run
+    # every lineage to its speciation (no water level) on a small map
+    announce(f"pooled {pool}: 16x16 map packed={packed}, nu 0.02, to completion")
+    hip, orc = map_scenarios(16, 16, seed=5, packed=packed)
+    cogs = hip.new(0.02)
+    lin = orc.origins(1.0)
+    ref = oracle_run(orc, lin, 3, 0.02, dedup="none")
+    out, ev = hip_run(cogs, lin, 3, PERSISTENT, dedup="none", pool=pool)
+    assert_exact(ref, out, ev, f"map16 pool {pool}")
+    assert not out.paused and len(ev) == len(lin)
+
+
+@pytest.mark.parametrize("pool", POOLS)
 def test_pooled_config4_sample_window(pool):
     announce(f"pooled {pool}: config4 0.1% sample, pause 400")
     cogs = nh.AlmostInfiniteNormalDispersalScenario(2000, 50.0).new(1e-3)
