@@ -318,12 +318,18 @@ def dispersal_run(ctx, tables, cogs, lineages, args, window, drain_window, seed=
     (bounded by the device-to-host copy)."""
     import dataclasses
     gargs = dataclasses.replace(args, use_graph=True)
-    out = {"window": window, "host_drain_window": drain_window}
-    modes = (("speciation_only", window, {}),
-             ("device_counter", window, dict(report_dispersal=True, device_event_counter=True)),
-             ("host_drain", drain_window, dict(report_dispersal=True)))
-    for name, win, kw in modes:
-        job = nh.Job(ctx, tables, gargs, nh.WyHash.seed_from_u64(seed), cogs.speciation_probability,
+    # the reference CUDA algorithm's default event slice, Relative(20.0)
+    # (cuda/src/arguments.rs:124-138): ten times fewer, larger rounds (the
+    # sort buffers for 2 x 20 x 12.6M events take about 70 GB of HBM)
+    wide = dataclasses.replace(gargs, event_slice=nh.EventSlice.Relative(20.0))
+    out = {"window": window, "host_drain_window": drain_window, "event_slice": "relative 2.0",
+           "device_counter_event_slice_20": {"event_slice": "relative 20.0"}}
+    modes = (("speciation_only", window, gargs, {}),
+             ("device_counter", window, gargs, dict(report_dispersal=True, device_event_counter=True)),
+             ("device_counter_event_slice_20", window, wide, dict(report_dispersal=True, device_event_counter=True)),
+             ("host_drain", drain_window, gargs, dict(report_dispersal=True)))
+    for name, win, margs, kw in modes:
+        job = nh.Job(ctx, tables, margs, nh.WyHash.seed_from_u64(seed), cogs.speciation_probability,
                      cogs.turnover_rate, pause_before=win, **kw)
         job.add_lineages(lineages)
         job.start()
@@ -344,16 +350,18 @@ def dispersal_run(ctx, tables, cogs, lineages, args, window, drain_window, seed=
             # coalesced lineages (dedup None) repeat their speciations; the
             # counter counts each distinct one once (event_counter.rs:57-72)
             r["distinct_speciations"] = len(species)
-        if name == "device_counter":
+        if name.startswith("device_counter"):
             r["counts"] = job.event_counts()
             r["counts_raw_total_equals_steps"] = r["counts"]["raw_total"] == o.steps
         if name == "host_drain":
             r["drained_events"] = drained
             r["drained_GB_per_s"] = drained * PACKED_EVENT_BYTES / wall / 1e9
         job.close()
-        out[name] = r
+        out[name] = {**out.get(name, {}), **r}
         log(f"dispersal {name}: {r}")
     sp, dc = out["speciation_only"], out["device_counter"]
+    out["device_counter_event_slice_20"]["counts_equal_event_slice_2"] = \
+        out["device_counter_event_slice_20"]["counts"] == dc["counts"]
     out["device_counter_species_equal_speciation_only"] = dc["counts"]["speciation"] == sp["distinct_speciations"]
     out["device_counter_over_speciation_only"] = dc["steps_per_s"] / sp["steps_per_s"]
     return out
