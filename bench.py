@@ -307,7 +307,7 @@ def serving_run(ctx, n_requests=150, rate=50.0, cap=1000.0, tick_steps=1000, see
 PACKED_EVENT_BYTES = 56
 
 
-def dispersal_run(ctx, tables, cogs, lineages, args, window, drain_window, seed=42):
+def dispersal_run(ctx, tables, cogs, lineages, args, window, drain_window, seed=42, wide_slice=20.0):
     """Dispersal reporting (a reporter with ReportDispersal) on the headline
     workload: every step is a 56-byte PackedEvent, the water level advances by
     event_slice / sum(lambda), and each round's events are sorted on the
@@ -321,13 +321,15 @@ def dispersal_run(ctx, tables, cogs, lineages, args, window, drain_window, seed=
     # the reference CUDA algorithm's default event slice, Relative(20.0)
     # (cuda/src/arguments.rs:124-138): ten times fewer, larger rounds (the
     # sort buffers for 2 x 20 x 12.6M events take about 70 GB of HBM)
-    wide = dataclasses.replace(gargs, event_slice=nh.EventSlice.Relative(20.0))
-    out = {"window": window, "host_drain_window": drain_window, "event_slice": "relative 2.0",
-           "device_counter_event_slice_20": {"event_slice": "relative 20.0"}}
-    modes = (("speciation_only", window, gargs, {}),
-             ("device_counter", window, gargs, dict(report_dispersal=True, device_event_counter=True)),
-             ("device_counter_event_slice_20", window, wide, dict(report_dispersal=True, device_event_counter=True)),
-             ("host_drain", drain_window, gargs, dict(report_dispersal=True)))
+    wide = dataclasses.replace(gargs, event_slice=nh.EventSlice.Relative(wide_slice or 20.0))
+    out = {"window": window, "host_drain_window": drain_window, "event_slice": "relative 2.0"}
+    modes = [("speciation_only", window, gargs, {}),
+             ("device_counter", window, gargs, dict(report_dispersal=True, device_event_counter=True))]
+    if wide_slice:
+        out["device_counter_event_slice_20"] = {"event_slice": f"relative {wide_slice}"}
+        modes.append(("device_counter_event_slice_20", window, wide,
+                      dict(report_dispersal=True, device_event_counter=True)))
+    modes.append(("host_drain", drain_window, gargs, dict(report_dispersal=True)))
     for name, win, margs, kw in modes:
         job = nh.Job(ctx, tables, margs, nh.WyHash.seed_from_u64(seed), cogs.speciation_probability,
                      cogs.turnover_rate, pause_before=win, **kw)
@@ -360,8 +362,9 @@ def dispersal_run(ctx, tables, cogs, lineages, args, window, drain_window, seed=
         out[name] = {**out.get(name, {}), **r}
         log(f"dispersal {name}: {r}")
     sp, dc = out["speciation_only"], out["device_counter"]
-    out["device_counter_event_slice_20"]["counts_equal_event_slice_2"] = \
-        out["device_counter_event_slice_20"]["counts"] == dc["counts"]
+    if wide_slice:
+        out["device_counter_event_slice_20"]["counts_equal_event_slice_2"] = \
+            out["device_counter_event_slice_20"]["counts"] == dc["counts"]
     out["device_counter_species_equal_speciation_only"] = dc["counts"]["speciation"] == sp["distinct_speciations"]
     out["device_counter_over_speciation_only"] = dc["steps_per_s"] / sp["steps_per_s"]
     return out
@@ -596,6 +599,8 @@ def main():
                     help="pause_before of the dispersal-reporting runs on the device")
     ap.add_argument("--task-window", type=float, default=200.0,
                     help="pause_before of the persistent vs task-list (reference CUDA structure) comparison")
+    ap.add_argument("--dispersal-wide", type=float, default=20.0,
+                    help="event slice factor of a second counted dispersal run (0: skip it)")
     ap.add_argument("--drain-window", type=float, default=20.0,
                     help="pause_before of the dispersal-reporting run drained to the host")
     ap.add_argument("--no-extra", action="store_true", help="same as --part headline")
@@ -649,7 +654,8 @@ def main():
         dargs = nh.IndependentArguments(dedup_cache=nh.DedupCache.none(), block_size=a.block_size,
                                         blocks_per_cu=a.blocks_per_cu, inner_steps=a.inner_steps,
                                         exec_fill=a.exec_fill, pool=DISPERSAL_POOL if a.pool < 0 else a.pool)
-        r = dispersal_run(ctx, tables, cogs, lineages, dargs, a.dispersal_window, a.drain_window)
+        r = dispersal_run(ctx, tables, cogs, lineages, dargs, a.dispersal_window, a.drain_window,
+                          wide_slice=a.dispersal_wide)
         print(json.dumps({"part": "dispersal", "dispersal_windows": r, "copy_kernel": copy}), flush=True)
         return
 
@@ -824,7 +830,7 @@ def main():
         dargs = dataclasses.replace(args, pool=DISPERSAL_POOL, block_size=0, blocks_per_cu=0, inner_steps=0,
                                     exec_fill=0)
         extra["dispersal_reporting"] = dispersal_run(ctx, tables, cogs, lineages, dargs, a.dispersal_window,
-                                                     a.drain_window)
+                                                     a.drain_window, wide_slice=a.dispersal_wide)
         del tables
         extra["config2_full_run"] = config2_run(ctx)
         log(f"config 2: {extra['config2_full_run']}")

@@ -1655,7 +1655,11 @@ __device__ __forceinline__ uint64_t block_first_head(const unsigned long long *k
 // (event_counter.rs:58-109).  Events [0, n_real) are real.  A run whose
 // events differ in class sets c->round_mixed, and the round is then counted
 // through the permutation instead.
-template <class E>
+// REF = false (counted rounds, whose permutation only the counters read):
+// runs are ordered by prior time alone, ties kept in position order.  The
+// counters see the same sequence of prior times either way: events of a run
+// differ only in prior time and reference.
+template <class E, bool REF = true>
 __global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const unsigned long long *keys,
                                                           const uint32_t *vals, uint64_t n, uint32_t *out_vals,
                                                           unsigned long long n_real, EventCounts *c,
@@ -1663,7 +1667,7 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const uns
     using O = EventOrder<E>;
     constexpr int NF = O::N - 3; // the origin / target fields 2 .. N - 2 (32-bit values)
     constexpr int NWV = TIE_THREADS / 64;
-    __shared__ unsigned long long s_p[TIE_SORT_N], s_r[TIE_SORT_N];
+    __shared__ unsigned long long s_p[TIE_SORT_N], s_r[REF ? TIE_SORT_N : 1];
     __shared__ uint32_t s_m[TIE_SORT_N];     // run << 11 | span position
     __shared__ uint8_t s_cls[TIE_SORT_N];    // counter class, by span position
     __shared__ uint32_t s_edge[NWV + 1][NF]; // origin / target of each wave's last lane; [NWV]: the chunk's last
@@ -1708,7 +1712,7 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const uns
         const uint64_t p = s + (uint64_t)(i < len ? i : len - 1);
         const E rec = ev[vi[k]];
         pr[k] = O::field(rec, 1);
-        rf[k] = O::field(rec, 0);
+        rf[k] = REF ? O::field(rec, 0) : 0ull;
         cl[k] = event_class(rec);
 #pragma unroll
         for (int t = 0; t < NF; t++) fl[k][t] = (uint32_t)O::field(rec, 2 + t);
@@ -1756,7 +1760,7 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const uns
         }
         if (have) {
             s_p[i] = pr[k];
-            s_r[i] = rf[k];
+            if (REF) s_r[i] = rf[k];
             s_m[i] = ((before + incl - 1) << TIE_POS_BITS) | (uint32_t)i;
             s_cls[i] = cl[k];
         }
@@ -1769,7 +1773,7 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const uns
     }
     for (int i = len + (int)threadIdx.x; i < nsort; i += TIE_THREADS) {
         s_p[i] = ~0ull;
-        s_r[i] = ~0ull;
+        if (REF) s_r[i] = ~0ull;
         s_m[i] = ~0u;
     }
     tie = __syncthreads_or(tie);
@@ -1786,13 +1790,17 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const uns
                     const int l = i + j;
                     const bool ascending = (i & k) == 0;
                     const uint32_t mi = s_m[i], ml = s_m[l];
-                    const unsigned long long pi = s_p[i], pl = s_p[l], ri = s_r[i], rl = s_r[l];
+                    const unsigned long long pi = s_p[i], pl = s_p[l];
+                    const unsigned long long ri = REF ? s_r[i] : 0ull, rl = REF ? s_r[l] : 0ull;
                     const uint32_t ui = mi >> TIE_POS_BITS, ul = ml >> TIE_POS_BITS;
                     const bool l_less =
                         ul < ui || (ul == ui && (pl < pi || (pl == pi && (rl < ri || (rl == ri && ml < mi)))));
                     if (l_less == ascending) {
                         s_p[i] = pl; s_p[l] = pi;
-                        s_r[i] = rl; s_r[l] = ri;
+                        if (REF) {
+                            s_r[i] = rl;
+                            s_r[l] = ri;
+                        }
                         s_m[i] = ml; s_m[l] = mi;
                     }
                 }
@@ -2505,9 +2513,16 @@ static const uint32_t *sort_permutation(const E *events, uint64_t n, SortBuffers
     // the runs of equal times sorted in LDS windows (a span without equal
     // times keeps its order)
     static_assert(MM_TIE_LONG == MM_TIE_MIXED + 1, "tie flags are adjacent");
-    hipLaunchKernelGGL(k_tie_runs<E>, dim3(grid_for(n, TIE_WINDOW)), dim3(TIE_THREADS), 0, stream, events,
-                       sb.keys[cur], sb.vals[cur], n, sb.vals[cur ^ 1], (unsigned long long)n_real, counts,
-                       sb.minmax + MM_TIE_MIXED);
+    // a counted round's permutation is read by the counters only: its runs
+    // are ordered by prior time alone (k_tie_runs<E, false>)
+    if (counts)
+        hipLaunchKernelGGL((k_tie_runs<E, false>), dim3(grid_for(n, TIE_WINDOW)), dim3(TIE_THREADS), 0, stream,
+                           events, sb.keys[cur], sb.vals[cur], n, sb.vals[cur ^ 1], (unsigned long long)n_real, counts,
+                           sb.minmax + MM_TIE_MIXED);
+    else
+        hipLaunchKernelGGL((k_tie_runs<E, true>), dim3(grid_for(n, TIE_WINDOW)), dim3(TIE_THREADS), 0, stream,
+                           events, sb.keys[cur], sb.vals[cur], n, sb.vals[cur ^ 1], (unsigned long long)n_real, counts,
+                           sb.minmax + MM_TIE_MIXED);
     (void)hipMemcpyAsync(&mm[MM_TIE_MIXED], sb.minmax + MM_TIE_MIXED, 2 * sizeof(unsigned long long),
                          hipMemcpyDeviceToHost, stream);
     (void)hipStreamSynchronize(stream);
