@@ -568,6 +568,13 @@ int nh_debug_sort_packed_events(nh_ctx *ctx, const nh_packed_event *in, size_t n
 /* The device EventCounterReporter of device_event_counter jobs over n
  * PackedEvents in PackedEvent order, from a fresh reporter. */
 int nh_debug_count_packed_events(nh_ctx *ctx, const nh_packed_event *in, size_t n, nh_event_counts *out);
+/* n PackedEvents in any order, sorted and counted as a round of a
+ * device_event_counter job is (the water-level sort's permutation, the counts
+ * taken in its windows of equal event times, or through the permutation where
+ * a run mixes classes), from a fresh reporter.  Diagnostic: the reporter side
+ * of LiveWaterLevelReporterProxy::advance_water_level (live.rs:88-115) feeding
+ * EventCounterReporter (event_counter.rs:58-109). */
+int nh_debug_sort_count_packed_events(nh_ctx *ctx, const nh_packed_event *in, size_t n, nh_event_counts *out);
 
 #ifdef __cplusplus
 }

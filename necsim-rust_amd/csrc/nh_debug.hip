@@ -223,6 +223,51 @@ extern "C" int nh_debug_sort_packed_events(nh_ctx *ctx, const nh_packed_event *i
                       reinterpret_cast<DevPacked *>(out), "nh_debug_sort_packed_events");
 }
 
+// Sorts n host PackedEvents (in any order) and counts them as the round of a
+// device_event_counter job does (the sort's permutation, the counts taken in
+// its run windows or through the permutation), from a fresh reporter.
+extern "C" int nh_debug_sort_count_packed_events(nh_ctx *ctx, const nh_packed_event *in, size_t n,
+                                                 nh_event_counts *out) {
+    const char *what = "nh_debug_sort_count_packed_events";
+    if (!ctx || !out || (n && !in)) return set_error(NH_E_INVALID, "%s: bad argument", what);
+    std::memset(out, 0, sizeof(*out));
+    if (n == 0) return NH_OK;
+    if (n > 0xffffffffull) return set_error(NH_E_CAPACITY, "%s: at most 2^32 - 1 events", what);
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_error(NH_E_HIP, "%s: hipSetDevice", what);
+    hipStream_t st = ctx->stream;
+    SortBuffers sb{};
+    DevPacked *d_in = nullptr;
+    EventCounts *d_c = nullptr;
+    int rc = NH_OK;
+    if ((rc = dalloc(&sb.keys[0], n)) || (rc = dalloc(&sb.keys[1], n)) || (rc = dalloc(&sb.vals[0], n)) ||
+        (rc = dalloc(&sb.vals[1], n)) || (rc = dalloc(&sb.hist, sort_hist_words_staged(n))) ||
+        (rc = dalloc(&sb.minmax, SORT_MINMAX_WORDS)) || (rc = dalloc(&d_in, n)) || (rc = dalloc(&d_c, 1))) {
+    } else {
+        sb.capacity = n;
+        sb.hist_words = sort_hist_words_staged(n);
+        EventCounts c{};
+        hipError_t e = hipMemcpyAsync(d_in, in, n * sizeof(DevPacked), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_c, &c, sizeof(c), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) {
+            sort_and_count_packed(d_in, n, n, false, sb, d_c, st);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(&c, d_c, sizeof(c), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = set_error(NH_E_HIP, "%s: %s", what, hipGetErrorString(e));
+        else std::memcpy(out, c.count, sizeof(*out));
+    }
+    dfree(sb.keys[0]);
+    dfree(sb.keys[1]);
+    dfree(sb.vals[0]);
+    dfree(sb.vals[1]);
+    dfree(sb.hist);
+    dfree(sb.minmax);
+    dfree(d_in);
+    dfree(d_c);
+    return rc;
+}
+
 // Counts n host PackedEvents (in PackedEvent order) with the device
 // EventCounterReporter of device_event_counter jobs, from a fresh reporter.
 extern "C" int nh_debug_count_packed_events(nh_ctx *ctx, const nh_packed_event *in, size_t n, nh_event_counts *out) {

@@ -150,6 +150,7 @@ SIGNATURES = {
     "nh_sort_events_noalloc": (ctypes.c_int, [c_vp, c_vp, c_sz, c_i32, c_vp]),
     "nh_debug_sort_packed_events": (ctypes.c_int, [c_vp, c_vp, c_sz, c_i32, c_vp]),
     "nh_debug_count_packed_events": (ctypes.c_int, [c_vp, c_vp, c_sz, ctypes.POINTER(EventCounts)]),
+    "nh_debug_sort_count_packed_events": (ctypes.c_int, [c_vp, c_vp, c_sz, ctypes.POINTER(EventCounts)]),
     "nh_sample_origins": (ctypes.c_int, [ctypes.POINTER(Scenario), c_f64, c_u32, c_u32, c_u32, c_u32,
                                          c_u32, c_vp, c_sz, ctypes.POINTER(c_sz)]),
     "nh_sample_origins_device": (ctypes.c_int, [c_vp, c_vp, c_f64, c_u32, c_u32, c_u32, c_u32, c_u32, c_vp,

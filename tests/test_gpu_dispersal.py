@@ -311,6 +311,69 @@ def test_device_counter_on_hand_and_coalescing_streams(ctx):
     assert _device_counts(ctx, ev) == reference_counts([ev])
 
 
+def _device_sort_counts(ctx, ev):
+    ev = np.ascontiguousarray(ev, dtype=PK)
+    c = _ffi.EventCounts()
+    _ffi.check(_ffi.lib().nh_debug_sort_count_packed_events(ctx.handle, _ffi.ptr(ev), len(ev), ctypes.byref(c)))
+    return {f: int(getattr(c, f)) for f, _ in _ffi.EventCounts._fields_}
+
+
+def _duplicate_runs(rng, n_runs, max_len, priors):
+    """runs of coalesced duplicates (one time, origin and target each; Maybe
+    dispersals), prior times from a small set, references in random order"""
+    rows = []
+    ref = 1
+    for k in range(n_runs):
+        t = 1.0 + k * 0.001 + float(rng.uniform(0.0, 1e-4))
+        org = (int(rng.integers(0, 50)), int(rng.integers(0, 50)), 0)
+        tgt = (int(rng.integers(0, 50)), int(rng.integers(0, 50)), 0)
+        for _ in range(int(rng.integers(1, max_len + 1))):
+            rows.append((ref, float(rng.choice(priors)), t, org, tgt))
+            ref += 1
+    ev = np.zeros(len(rows), PK)
+    refs = rng.permutation(len(rows)).astype(np.uint64) + 1
+    for i, (_, p, t, org, tgt) in enumerate(rows):
+        ev[i]["global_reference"] = refs[i]
+        ev[i]["prior_time"] = p
+        ev[i]["event_time"] = -t  # LineageInteraction::Maybe
+        ev[i]["origin_x"], ev[i]["origin_y"], ev[i]["origin_index"] = org
+        ev[i]["target_x"], ev[i]["target_y"], ev[i]["target_index"] = tgt
+        ev[i]["coalescence"] = refs[i]
+    return ev
+
+
+def test_device_sort_and_count_of_unsorted_streams(ctx):
+    """A counted round's windows order each run by prior time alone (its
+    counters read nothing else); runs whose equal prior times carry
+    references in another order, runs that mix classes (counted through the
+    permutation), runs longer than a window (field passes) and the hand-made
+    and coalescing streams, each shuffled, must count as the sequential
+    restatement of event_counter.rs:58-109 counts the sorted stream."""
+    announce("device sort + count of shuffled streams (counted-round windows)")
+    from test_reporters import hand_stream, reference_counts
+    rng = np.random.default_rng(21)
+    s = O.Scenario.non_spatial(6, 6, 2)
+    full = O.simulate_packed(s, O.make_params(11, 0.02, dedup=None, report_dispersal=True), s.origins(1.0))
+    cases = {
+        "hand": hand_stream(),
+        "coalescing": np.ascontiguousarray(full["events"]).view(PK),
+        "equal_priors": _duplicate_runs(rng, 600, 80, [0.25, 0.5, 0.75]),
+        "one_prior": _duplicate_runs(rng, 300, 200, [0.5]),
+        "long_runs": _duplicate_runs(rng, 3, 2500, [0.25, 0.5]),
+    }
+    # a run of one time and location that mixes speciations with self-dispersals
+    mixed = hand_stream()[:3].copy()
+    extra = mixed.copy()
+    extra["prior_time"] = 0.5
+    extra["event_time"] = 1.0
+    extra["global_reference"] = [40, 41, 42]
+    cases["mixed_classes"] = np.concatenate([mixed, extra, _duplicate_runs(rng, 100, 30, [0.5, 0.75])])
+    for name, ev in cases.items():
+        want = reference_counts([ev[nh.packed_order(ev)]])
+        got = _device_sort_counts(ctx, ev[rng.permutation(len(ev))])
+        assert got == want, (name, got, want)
+
+
 @pytest.mark.parametrize("form,graph,pool", [(PLAIN, False, 0), (PERSISTENT, False, 0), (PERSISTENT, True, 0),
                                              (PERSISTENT, False, 128), (PERSISTENT, True, 192)])
 def test_device_counter_job_matches_host_counter(form, graph, pool):
