@@ -229,10 +229,9 @@ struct EventCounts {
     unsigned long long carry_has;   // an event was counted before this round
     unsigned long long carry_prior; // its prior time (magnitude bits)
     // the round counted inside the sort's run windows, committed afterwards
-    // unless a run held events of different classes (round_mixed)
+    // (k_count_finish)
     unsigned long long round_count[EC_COUNTERS];
     unsigned long long round_carry_prior;
-    unsigned long long round_mixed;
 };
 // Counts one round's sorted PackedEvents ev[0, n) (or ev[perm[0, n)] in the
 // order of a sort permutation): with `end`, n is read from *end when the

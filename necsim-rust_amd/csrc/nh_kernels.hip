@@ -1648,17 +1648,14 @@ __device__ __forceinline__ uint64_t block_first_head(const unsigned long long *k
 // writes nothing and the round is sorted by field passes instead.  A span
 // without equal times keeps its order; unused event slots (ghost keys) sort
 // after every event and are runs of one.
-// With `c` (a device_event_counter round), the window also counts its events
-// in the sorted order, as k_count_events would: a run's first event adds to
-// its class's counter, every later one is a duplicate and a late coalescence
-// when its prior time differs from the previous event's
-// (event_counter.rs:58-109).  Events [0, n_real) are real.  A run whose
-// events differ in class sets c->round_mixed, and the round is then counted
-// through the permutation instead.
-// REF = false (counted rounds, whose permutation only the counters read):
-// runs are ordered by prior time alone, ties kept in position order.  The
-// counters see the same sequence of prior times either way: events of a run
-// differ only in prior time and reference.
+// With `c` (a device_event_counter round, REF = false) the window counts its
+// events instead of writing the permutation: a run's first event adds to its
+// class's counter, every later one is a duplicate, and the run adds one late
+// coalescence per distinct prior time after the first (event_counter.rs:
+// 58-109 over the run in PackedEvent order, where equal prior times are
+// adjacent).  Events [0, n_real) are real.  A run whose events differ in
+// class is flagged like one with several origins (flags[0]), and the round
+// is sorted by field passes and counted through that permutation.
 template <class E, bool REF = true>
 __global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const unsigned long long *keys,
                                                           const uint32_t *vals, uint64_t n, uint32_t *out_vals,
@@ -1670,6 +1667,9 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const uns
     __shared__ unsigned long long s_p[TIE_SORT_N], s_r[REF ? TIE_SORT_N : 1];
     __shared__ uint32_t s_m[TIE_SORT_N];     // run << 11 | span position
     __shared__ uint8_t s_cls[TIE_SORT_N];    // counter class, by span position
+    __shared__ uint32_t s_hpos[REF ? 1 : TIE_SORT_N]; // !REF: span position of each run's first event
+    __shared__ unsigned s_nodd;
+    __shared__ unsigned long long s_carry;
     __shared__ uint32_t s_edge[NWV + 1][NF]; // origin / target of each wave's last lane; [NWV]: the chunk's last
     __shared__ uint32_t s_wsum[NWV];
     __shared__ unsigned long long s_cnt[EC_COUNTERS];
@@ -1763,6 +1763,7 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const uns
             if (REF) s_r[i] = rf[k];
             s_m[i] = ((before + incl - 1) << TIE_POS_BITS) | (uint32_t)i;
             s_cls[i] = cl[k];
+            if (!REF && head) s_hpos[before + incl - 1] = (uint32_t)i;
         }
         run_base += chunk;
         __syncthreads(); // s_edge and s_wsum are rewritten by the next chunk
@@ -1770,6 +1771,109 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const uns
 #pragma unroll
             for (int t = 0; t < NF; t++) s_edge[NWV][t] = f[t];
         }
+    }
+    if constexpr (!REF) {
+        // A counted round: the counters need per run only its length, its
+        // class and the number of distinct prior times (the later events of
+        // a run are duplicates, each a late coalescence where its prior time
+        // differs from the one before it in sorted order: one per distinct
+        // prior time after the first).  Events whose prior time differs from
+        // their run's first event are few; they alone are sorted, by (run,
+        // prior time), and their distinct pairs counted.  A run whose events
+        // differ in class goes, with the round, to the field passes.
+        __syncthreads();
+        if (s_bad) {
+            if (threadIdx.x == 0) atomicAdd(&flags[(s_bad & 1u) ? 0 : 1], 1ull);
+            return;
+        }
+        const int real = n_real > s ? (int)(n_real - s < (uint64_t)len ? n_real - s : (uint64_t)len) : 0;
+        const int last_run = (s + (uint64_t)real == n_real && real > 0) ? (int)(s_m[real - 1] >> TIE_POS_BITS) : -1;
+        if (threadIdx.x == 0) {
+            s_nodd = 0;
+            s_carry = 0;
+        }
+        __syncthreads();
+        unsigned long long cnt[EC_COUNTERS];
+#pragma unroll
+        for (int k = 0; k < EC_COUNTERS; k++) cnt[k] = 0;
+        bool mixed = false;
+        unsigned long long op[CH];
+        uint32_t orun[CH];
+        bool odd[CH];
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            const int i = (int)threadIdx.x + TIE_THREADS * k;
+            odd[k] = false;
+            if (i >= real) continue;
+            const uint32_t run = s_m[i] >> TIE_POS_BITS, h = s_hpos[run];
+            const unsigned long long pi = s_p[i];
+            cnt[EC_RAW_TOTAL]++;
+            if (h == (uint32_t)i) {
+                const int w = first_event_counter(s_cls[i]);
+#pragma unroll
+                for (int q = EC_SPECIATION; q < EC_COUNTERS; q++) cnt[q] += q == w ? 1 : 0;
+            } else {
+                mixed |= s_cls[i] != s_cls[h];
+                odd[k] = pi != s_p[h];
+            }
+            op[k] = pi;
+            orun[k] = run;
+            if ((int)run == last_run) atomicMax(&s_carry, pi);
+        }
+        if (__syncthreads_or(mixed ? 1 : 0)) {
+            if (threadIdx.x == 0) atomicAdd(&flags[0], 1ull);
+            return;
+        }
+        // every read of the span is done: the odd events go to the front
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            if (!odd[k]) continue;
+            const unsigned q = atomicAdd(&s_nodd, 1u);
+            s_p[q] = op[k];
+            s_m[q] = (orun[k] << TIE_POS_BITS) | q;
+        }
+        __syncthreads();
+        const int nodd = (int)s_nodd;
+        if (nodd > 1) {
+            int nso = 2;
+            while (nso < nodd) nso <<= 1;
+            for (int i = nodd + (int)threadIdx.x; i < nso; i += TIE_THREADS) {
+                s_p[i] = ~0ull;
+                s_m[i] = ~0u;
+            }
+            __syncthreads();
+            for (int k = 2; k <= nso; k <<= 1) {
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int q = threadIdx.x; q < nso / 2; q += TIE_THREADS) {
+                        const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                        const int l = i + j;
+                        const bool ascending = (i & k) == 0;
+                        const uint32_t mi = s_m[i], ml = s_m[l];
+                        const unsigned long long pi = s_p[i], pl = s_p[l];
+                        const uint32_t ui = mi >> TIE_POS_BITS, ul = ml >> TIE_POS_BITS;
+                        const bool l_less = ul < ui || (ul == ui && (pl < pi || (pl == pi && ml < mi)));
+                        if (l_less == ascending) {
+                            s_p[i] = pl; s_p[l] = pi;
+                            s_m[i] = ml; s_m[l] = mi;
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        for (int q = threadIdx.x; q < nodd; q += TIE_THREADS)
+            if (q == 0 || (s_m[q] >> TIE_POS_BITS) != (s_m[q - 1] >> TIE_POS_BITS) || s_p[q] != s_p[q - 1])
+                cnt[EC_LATE_COALESCENCE]++;
+#pragma unroll
+        for (int k = 0; k < EC_COUNTERS; k++) {
+            unsigned long long v = cnt[k];
+            for (int off = 32; off; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0 && v) atomicAdd(&s_cnt[k], v);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && last_run >= 0) c->round_carry_prior = s_carry;
+        if (threadIdx.x < EC_COUNTERS && s_cnt[threadIdx.x]) atomicAdd(&c->round_count[threadIdx.x], s_cnt[threadIdx.x]);
+        return;
     }
     for (int i = len + (int)threadIdx.x; i < nsort; i += TIE_THREADS) {
         s_p[i] = ~0ull;
@@ -1809,34 +1913,6 @@ __global__ void __launch_bounds__(TIE_THREADS) k_tie_runs(const E *ev, const uns
         }
     }
     for (int i = threadIdx.x; i < len; i += TIE_THREADS) out_vals[s + i] = vals[s + (s_m[i] & TIE_POS_MASK)];
-    if (!c) return;
-    // the events of the span in sorted order; position 0 (s) starts a run
-    const int real = n_real > s ? (int)(n_real - s < (uint64_t)len ? n_real - s : (uint64_t)len) : 0;
-    unsigned long long cnt[EC_COUNTERS];
-#pragma unroll
-    for (int k = 0; k < EC_COUNTERS; k++) cnt[k] = 0;
-    bool mixed = false;
-    for (int i = threadIdx.x; i < real; i += TIE_THREADS) {
-        const uint8_t ci = s_cls[s_m[i] & TIE_POS_MASK];
-        cnt[EC_RAW_TOTAL]++;
-        if (i == 0 || (s_m[i - 1] >> TIE_POS_BITS) != (s_m[i] >> TIE_POS_BITS)) {
-            const int w = first_event_counter(ci);
-#pragma unroll
-            for (int k = EC_SPECIATION; k < EC_COUNTERS; k++) cnt[k] += k == w ? 1 : 0;
-        } else {
-            mixed |= s_cls[s_m[i - 1] & TIE_POS_MASK] != ci;
-            cnt[EC_LATE_COALESCENCE] += s_p[i - 1] != s_p[i] ? 1 : 0;
-        }
-        if (s + (uint64_t)i == n_real - 1) c->round_carry_prior = s_p[i];
-    }
-#pragma unroll
-    for (int k = 0; k < EC_COUNTERS; k++) {
-        unsigned long long v = cnt[k];
-        for (int off = 32; off; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0 && v) atomicAdd(&s_cnt[k], v);
-    }
-    if (__syncthreads_or(mixed ? 1 : 0) && threadIdx.x == 0) c->round_mixed = 1;
-    if (threadIdx.x < EC_COUNTERS && s_cnt[threadIdx.x]) atomicAdd(&c->round_count[threadIdx.x], s_cnt[threadIdx.x]);
 }
 
 template <class E>
@@ -2714,20 +2790,16 @@ void launch_count_events(const DevPacked *ev, const uint32_t *perm, const unsign
                        (const unsigned long long *)nullptr);
 }
 
-// commits a round counted in the sort's windows, unless a run mixed classes
+// commits a round counted in the sort's windows
 __global__ void k_count_finish(EventCounts *c) {
-    if (!c->round_mixed) {
-        for (int k = 0; k < EC_COUNTERS; k++) c->count[k] += c->round_count[k];
-        c->carry_has = 1;
-        c->carry_prior = c->round_carry_prior;
-    }
+    for (int k = 0; k < EC_COUNTERS; k++) c->count[k] += c->round_count[k];
+    c->carry_has = 1;
+    c->carry_prior = c->round_carry_prior;
     for (int k = 0; k < EC_COUNTERS; k++) c->round_count[k] = 0;
 }
-__global__ void k_count_gate_clear(EventCounts *c) { c->round_mixed = 0; }
 // a round whose window counts are dropped (its runs went to field passes)
 __global__ void k_count_round_clear(EventCounts *c) {
     for (int k = 0; k < EC_COUNTERS; k++) c->round_count[k] = 0;
-    c->round_mixed = 0;
 }
 
 // A round of a device_event_counter job: the events' sorted permutation, then
@@ -2741,15 +2813,8 @@ void sort_and_count_packed(const DevPacked *events, uint64_t slots, uint64_t n_e
         launch_count_events(events, perm, nullptr, n_events, nullptr, c, stream);
         return;
     }
-    // counted in the windows; a round with a run of mixed classes (gate set)
-    // is counted again through the permutation
+    // counted in the windows
     hipLaunchKernelGGL(k_count_finish, dim3(1), dim3(1), 0, stream, c);
-    SortSize sz{n_events, nullptr, nullptr, nullptr};
-    hipLaunchKernelGGL(k_count_events, dim3(grid_for(n_events, 256)), dim3(256), 0, stream, events, perm, sz,
-                       (const unsigned long long *)nullptr, c, (const unsigned long long *)&c->round_mixed);
-    hipLaunchKernelGGL(k_count_carry, dim3(1), dim3(1), 0, stream, events, perm, sz,
-                       (const unsigned long long *)nullptr, c, (const unsigned long long *)&c->round_mixed);
-    hipLaunchKernelGGL(k_count_gate_clear, dim3(1), dim3(1), 0, stream, c);
 }
 
 uint32_t soa_compact_blocks(uint64_t n) { return (uint32_t)((n + SC_TILE - 1) / SC_TILE); }
