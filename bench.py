@@ -156,13 +156,19 @@ def config2_run(ctx):
 
 
 GATHER_BYTES_PER_STEP = 48  # cell record 32 B + alias atom 16 B per dispersal step
+# the alias-gather window's launch shape, the autotuner's choice on it
+# (nh.autotune over windows of 100 and 500: execution once an eighth of a
+# wave is ready, 13.4 ms against 14.3 ms at the default five eighths); fixed
+# here so that its profile holds the window's launches only
+GATHER_SHAPE = dict(block_size=256, blocks_per_cu=4, inner_steps=16, exec_fill=1)
 
 
-def map_window_run(ctx, size, nu, window, dedup, seed=42, tune=False):
+def map_window_run(ctx, size, nu, window, dedup, seed=42, tune=False, shape=None):
     """A window of the spatially explicit map scenario (the generator of
     configs 3 and 5): every dispersal gathers the target cell's record and
     one alias atom.  Returns timings and the gather-bound roofline.  With
-    tune, the launch shape is chosen by untimed runs of the same window."""
+    tune, the launch shape is chosen by untimed runs of the same window; with
+    shape (IndependentArguments fields), that launch shape is used."""
     h, rp, c, w = nh.generate_map(size, size, seed=42)
     scn = nh.SpatiallyExplicitUniformTurnoverScenario(size, size, h, rp, c, w)
     # resident tables: alias build on the host + upload, or on the GPU
@@ -176,8 +182,8 @@ def map_window_run(ctx, size, nu, window, dedup, seed=42, tune=False):
     device_s = time.perf_counter() - t
     lin = cogs.sample_habitat(nh.OriginPreSampler.all())
     atoms = int(len(scn.tables()[2]))
-    args = nh.IndependentArguments(dedup_cache=dedup)
-    tuned = None
+    args = nh.IndependentArguments(dedup_cache=dedup, **(shape or {}))
+    tuned = dict(shape) if shape else None
     if tune:
         # launch shape chosen on the window itself: one untimed run with the
         # default shape and one per residency of 256-thread blocks.  These
@@ -247,7 +253,7 @@ def gather_run(ctx, window):
     """the alias-gather workload: 256x256 map (config 5's generator; its
     tables fill the 256 MB last-level cache), no dedup.  Adds the HBM
     figures of the committed rocprofv3 summary of the same window."""
-    r = map_window_run(ctx, 256, 1e-5, window, nh.DedupCache.none())
+    r = map_window_run(ctx, 256, 1e-5, window, nh.DedupCache.none(), shape=GATHER_SHAPE)
     prof = read_profile_summary(GATHER_PROFILE)
     if prof.get("window") == window:
         r["rocprof"] = {k: prof.get(k) for k in ("kernel_mean_ms", "hbm_bytes_per_launch", "hbm_rate_GBps",
