@@ -177,7 +177,7 @@ typedef struct {
     uint32_t inner_steps;          /* persistent form: iterations per lane between refills, 0 = default 16 */
     uint32_t exec_fill;            /* persistent form: a wave executes its lanes' next events once at least
                                       exec_fill eighths of its active lanes have found theirs (1..8);
-                                      0 = default 5 */
+                                      0 = default 5, or 1 for the spatially explicit (alias) scenarios */
     int32_t report_dispersal;      /* the reporter's ReportDispersal: every event is reported as a
                                       PackedEvent (nh_job_drain_packed_events) and the water level
                                       advances by event_slice / sum(lambda) (monolithic/mod.rs:139-148);
