@@ -586,6 +586,148 @@ def summarize(dirs):
                 print(f"- {kname[:90]} (x{calls[kname]}): {cs}")
 
 
+def _kt_stats(d):
+    import csv
+    import glob
+    f = glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True)[0]
+    return {r["Name"]: r for r in csv.DictReader(open(f))}
+
+
+def _pmc_means(d):
+    """per kernel name: each counter's mean over its dispatches"""
+    import csv
+    import glob
+    from collections import defaultdict
+    per, names = defaultdict(lambda: defaultdict(float)), {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            k = int(r["Dispatch_Id"])
+            per[k][r["Counter_Name"]] += float(r["Counter_Value"])
+            names[k] = r["Kernel_Name"]
+    agg = defaultdict(lambda: defaultdict(list))
+    for k, cs in per.items():
+        for c, v in cs.items():
+            agg[names[k]][c].append(v)
+    return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in agg.items()}
+
+
+def _of(table, pattern, counter):
+    return [v for k, v in table.items() if pattern in k][0][counter]
+
+
+def derive(kind, log, dirs):
+    """The derived figures of a committed profile, from the bench output
+    `log` and the rocprofv3 directories `dirs` (kernel trace, FETCH_SIZE,
+    WRITE_SIZE, then for headline and gather the SQ pass and for headline
+    the instruction-mix pass).  Byte counters are scaled by what they read
+    for k_copy_calibrate (1 GiB read + 1 GiB written per dispatch)."""
+    kt, fe, wr = _kt_stats(dirs[0]), _pmc_means(dirs[1]), _pmc_means(dirs[2])
+    ff = (1 << 30) / (_of(fe, "k_copy_calibrate", "FETCH_SIZE") * 1024)
+    wf = (1 << 30) / (_of(wr, "k_copy_calibrate", "WRITE_SIZE") * 1024)
+    lines = [l for l in open(log).read().splitlines() if l.startswith("{")]
+    bj = json.loads(lines[-1])
+    out = {}
+    if kind in ("headline", "gather"):
+        sq = _pmc_means(dirs[3])
+        if kind == "headline":
+            la = bj["extra"]["launch"]
+            K = f"k_pooled<1, {la['pool']}, false>" if la["pool"] else "k_persistent<1, false, false, false>"
+            steps = bj["roofline"]["lineage_steps_per_launch"]
+        else:
+            K = "k_persistent<2, false, false, false>"
+            steps = bj["gather_window"]["steps"]
+        mean_ms = float([r for n, r in kt.items() if K in n][0]["AverageNs"]) / 1e6
+        fb, wb = _of(fe, K, "FETCH_SIZE") * 1024 * ff, _of(wr, K, "WRITE_SIZE") * 1024 * wf
+        valu, grbm = _of(sq, K, "SQ_INSTS_VALU"), _of(sq, K, "GRBM_GUI_ACTIVE")
+        sec = mean_ms * 1e-3
+        out["window"] = 2000
+        if kind == "headline":
+            out.update({"form": la["form"], "block_size": la["block_size"], "blocks_per_cu": la["blocks_per_cu"],
+                        "inner_steps": la["inner_steps"], "exec_fill": la["exec_fill"], "pool": la["pool"]})
+        out["kernel_mean_ms"] = round(mean_ms, 4)
+        out["lineage_steps_per_launch"] = steps
+        if kind == "headline":
+            algo = bj["roofline"]["hbm"]["algorithmic_bytes_per_launch"]
+            out.update({"algorithmic_bytes_per_launch": algo, "hbm_fetch_bytes_per_launch": round(fb),
+                        "hbm_write_bytes_per_launch": round(wb), "hbm_bytes_per_launch": round(fb + wb),
+                        "traffic_over_algorithmic": round((fb + wb) / algo, 3)})
+        else:
+            gb = steps * GATHER_BYTES_PER_STEP
+            out.update({"gather_bytes_per_launch": gb, "gather_rate_GBps": round(gb / sec / 1e9, 1),
+                        "gather_frac_of_hbm_peak": round(gb / sec / (HBM_PEAK_GBS * 1e9), 4),
+                        "hbm_fetch_bytes_per_launch": round(fb), "hbm_write_bytes_per_launch": round(wb),
+                        "hbm_bytes_per_launch": round(fb + wb), "hbm_rate_GBps": round((fb + wb) / sec / 1e9, 1),
+                        "hbm_frac_of_peak": round((fb + wb) / sec / (HBM_PEAK_GBS * 1e9), 4),
+                        "fetch_over_gather_bytes": round(fb / gb, 3),
+                        "lineage_steps_per_s": float(f"{steps / sec:.4e}"),
+                        "gather_roofline_steps_per_s": float(f"{HBM_PEAK_GBS * 1e9 / GATHER_BYTES_PER_STEP:.4e}")})
+        out["valu_wave_instructions_per_lineage_step"] = round(valu / steps, 3)
+        out["valu_issue_fraction"] = round(valu / (grbm / 8 * 256), 4)
+        out["valu_lane_utilization"] = round(_of(sq, K, "SQ_THREAD_CYCLES_VALU") /
+                                             (64 * _of(sq, K, "SQ_ACTIVE_INST_VALU")), 4)
+        if kind == "headline":
+            mix = _pmc_means(dirs[4])
+            out.update({
+                "valu_int64_per_step": round(_of(mix, K, "SQ_INSTS_VALU_INT64") / steps, 3),
+                "valu_int32_per_step": round(_of(mix, K, "SQ_INSTS_VALU_INT32") / steps, 3),
+                "valu_f64_add_mul_fma_per_step": round(sum(_of(mix, K, c) for c in (
+                    "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_ADD_F64")) / steps, 3),
+                "valu_f64_trans_per_step": round(_of(mix, K, "SQ_INSTS_VALU_TRANS_F64") / steps, 4),
+                "valu_cvt_per_step": round(_of(mix, K, "SQ_INSTS_VALU_CVT") / steps, 3),
+                "vmem_per_step": round(_of(mix, K, "SQ_INSTS_VMEM") / steps, 4),
+                "lds_per_step": round(_of(sq, K, "SQ_INSTS_LDS") / steps, 3)})
+        for k, v in out.items():
+            print(f"{k}: {v}")
+        return
+    # dispersal: one counted round of the device-counter mode, per kernel
+    dc = bj["dispersal_windows"]
+    counted, drained = dc["device_counter"]["rounds"], dc["host_drain"]["rounds"]
+    ev_round = dc["device_counter"]["steps"] / counted
+    sorted_rounds = counted + drained
+    rows = [("k_pooled<1, 128, true>", "step kernel (pooled), every step a 56 B PackedEvent + its sort key and slot",
+             sorted_rounds, True),
+            ("k_sradix_hist", "staged radix: digit histogram per 4096-key tile (per pass)", sorted_rounds, True),
+            ("k_hist_chunk_sums<256, 4096>", "histogram scan, phase 1 (per pass)", sorted_rounds, True),
+            ("k_scan_chunk_sums<256, 4096>", "histogram scan, phase 2 (per pass)", sorted_rounds, True),
+            ("k_hist_scan_apply<256, 4096>", "histogram scan, phase 3 (per pass)", sorted_rounds, True),
+            ("k_sradix_scatter", "staged radix: tile ranked in LDS, written out in digit order (per pass)",
+             sorted_rounds, True),
+            ("k_tie_runs<nh::DevPacked, false>", "windows of a counted round: each record read once, runs counted, "
+             "only events whose prior time differs from their run's first sorted", counted, True),
+            ("k_tie_runs<nh::DevPacked, true>", "(drained rounds only, not in the total) windows ordering runs by "
+             "(prior, reference)", drained, False)]
+    passes = int([r for n, r in kt.items() if "k_sradix_scatter" in n][0]["Calls"]) / sorted_rounds
+    head = {"events_per_round": round(ev_round), "radix_digit_passes_per_round": f"{passes:.4f}",
+            "device_counter_events_per_s": f"{dc['device_counter']['steps_per_s']:.4e}",
+            "device_counter_round_ms": f"{dc['device_counter']['wall_ms'] / counted:.4f}",
+            "step_kernel_events_per_s": f"{dc['device_counter']['device_steps_per_s']:.4e}",
+            "speciation_only_steps_per_s": f"{dc['speciation_only']['steps_per_s']:.4e}",
+            "host_drain_events_per_s": f"{dc['host_drain']['steps_per_s']:.4e}",
+            "host_drain_GB_per_s": f"{dc['host_drain']['drained_GB_per_s']:.4e}",
+            "hbm_bound_events_per_s": f"{HBM_PEAK_GBS * 1e9 / 280:.4e}",
+            "fraction_of_hbm_bound": f"{dc['device_counter']['steps_per_s'] / (HBM_PEAK_GBS * 1e9 / 280):.4f}"}
+    for k, v in head.items():
+        print(f"{k}: {v}")
+    print("\n## per round (kernel mean x calls per round; HBM bytes per event = calibrated FETCH_SIZE + "
+          "WRITE_SIZE / events per round; the scan kernels are shared with the drain and speciation-only sorts)\n")
+    print("| kernel | work | mean ms | calls/round | ms/round | HBM B/event |")
+    print("|---|---|---|---|---|---|")
+    total = 0.0
+    for pat, work, per, in_total in rows:
+        r = [x for n, x in kt.items() if pat in n]
+        if not r:
+            continue
+        r = r[0]
+        mean, cpr = float(r["AverageNs"]) / 1e6, int(r["Calls"]) / per
+        total += mean * cpr if in_total else 0.0
+        fk = [v for k, v in fe.items() if pat in k]
+        wk = [v for k, v in wr.items() if pat in k]
+        b = ((fk[0]["FETCH_SIZE"] * ff + wk[0]["WRITE_SIZE"] * wf) * 1024 / ev_round * cpr) if fk and wk else \
+            float("nan")
+        print(f"| {r['Name'][:60]} | {work} | {mean:.3f} | {cpr:.1f} | {mean * cpr:.3f} | {b:.1f} |")
+    print(f"| total | | | | {total:.3f} | |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -626,9 +768,16 @@ def main():
                          "kernel timing then covers the whole graph")
     ap.add_argument("--summarize", nargs="+", metavar="DIR",
                     help="print a summary of rocprofv3 output directories and exit (no GPU)")
+    ap.add_argument("--derive", nargs="+", metavar="ARG",
+                    help="KIND BENCH_LOG DIR...: the derived figures of a committed profile (KIND headline: "
+                         "kernel trace, FETCH_SIZE, WRITE_SIZE, SQ and instruction-mix directories; gather: the "
+                         "first four; dispersal: the first three); prints text for profiles/ (no GPU)")
     a = ap.parse_args()
     if a.summarize:
         summarize(a.summarize)
+        return
+    if a.derive:
+        derive(a.derive[0], a.derive[1], a.derive[2:])
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for {a.gpus})",
