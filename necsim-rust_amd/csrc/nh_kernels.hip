@@ -566,7 +566,9 @@ __global__ void __launch_bounds__(256, (DEDUP || DISP) ? 3 : 4) k_persistent(Per
 // pool all the time, which the one-lineage-per-lane kernel pays for with
 // idle lanes.
 constexpr int POOL_MAX = 192;
-enum : uint8_t { PE_EMPTY = 0, PE_SEARCH = 1, PE_READY = 2 };
+// PE_LEAVE (dispersal reporting): the next event passes the water level; the
+// entry waits for a leave pass, which stores it to the lineage store
+enum : uint8_t { PE_EMPTY = 0, PE_SEARCH = 1, PE_READY = 2, PE_LEAVE = 3 };
 constexpr uint32_t PE_NOT_STARTED = 0xffffffffu; // win_left of a search that has not opened a window
 
 template <int POOL, bool REF> struct PoolLds {
@@ -666,17 +668,19 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
             p.next_t[e] = L.next_t;
             p.st[e] = L.next_st;
             p.win_left[e] = PE_NOT_STARTED;
-            own[j] = L.have_next ? PE_READY : PE_SEARCH;
+            own[j] = L.have_next ? ((DISP && L.next_t >= level) ? PE_LEAVE : PE_READY) : PE_SEARCH;
             p.state[e] = own[j];
         }
-        uint64_t srch[NW], rdy[NW];
-        uint32_t n_occ = 0, n_rdy = 0;
+        uint64_t srch[NW], rdy[NW], lv[NW];
+        uint32_t n_occ = 0, n_rdy = 0, n_lv = 0;
 #pragma unroll
         for (int j = 0; j < NW; j++) {
             srch[j] = __ballot(own[j] == PE_SEARCH);
             rdy[j] = __ballot(own[j] == PE_READY);
-            n_occ += (uint32_t)__popcll(srch[j] | rdy[j]);
+            lv[j] = DISP ? __ballot(own[j] == PE_LEAVE) : 0ull;
+            n_occ += (uint32_t)__popcll(srch[j] | rdy[j] | lv[j]);
             n_rdy += (uint32_t)__popcll(rdy[j]);
+            n_lv += (uint32_t)__popcll(lv[j]);
         }
         n_empty = POOL - n_occ;
         if (n_occ == 0) {
@@ -685,7 +689,7 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
         }
 
         // search phase: one step for up to 64 searching lineages
-        if (n_occ > n_rdy) {
+        if (n_occ > n_rdy + n_lv) {
             const uint32_t n = pool_pick<NW>(srch, (int)(iter % NW), p.assign);
             if (lane < n) {
                 const int e = p.assign[lane];
@@ -705,7 +709,7 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
                 p.next_t[e] = L.next_t;
                 p.st[e] = L.next_st;
                 if (L.have_next) {
-                    p.state[e] = PE_READY;
+                    p.state[e] = (DISP && L.next_t >= level) ? PE_LEAVE : PE_READY;
                 } else {
                     p.ts[e] = L.search_ts;
                     p.win_left[e] = L.win_left;
@@ -715,16 +719,49 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
             }
             wave_sync_lds();
             n_rdy = 0;
+            n_lv = 0;
 #pragma unroll
             for (int j = 0; j < NW; j++) {
-                rdy[j] = __ballot(p.state[lane + 64 * j] == PE_READY);
+                const uint8_t st = p.state[lane + 64 * j];
+                rdy[j] = __ballot(st == PE_READY);
                 n_rdy += (uint32_t)__popcll(rdy[j]);
+                if (DISP) {
+                    lv[j] = __ballot(st == PE_LEAVE);
+                    n_lv += (uint32_t)__popcll(lv[j]);
+                }
+            }
+        }
+
+        // leave phase (dispersal reporting, where lineages reach the water
+        // level every few steps): the entries whose next event passes it are
+        // stored in a pass of their own, so execution passes do not run with
+        // lanes that only store
+        if constexpr (DISP) {
+            if (n_lv >= 32 || (n_lv > 0 && (n_lv + n_rdy == n_occ || q.exhausted))) {
+                const uint32_t n = pool_pick<NW>(lv, (int)(iter % NW), p.assign);
+                if (lane < n) {
+                    const int e = p.assign[lane];
+                    Lane<KIND> L;
+                    L.t_last = p.t_last[e];
+                    L.a = p.a[e];
+                    L.b = p.b[e];
+                    L.i = p.i[e];
+                    L.next_t = p.next_t[e];
+                    L.next_st = p.st[e];
+                    L.have_next = true;
+                    lane_store<KIND>(L, A.S, A.lin, p.slot[e]);
+                    p.state[e] = PE_EMPTY;
+                }
+                wave_sync_lds();
+                n_occ -= n;
+                n_lv -= n;
             }
         }
 
         // execute phase: full waves, or the rest once the queue is empty
+        const uint32_t n_act = n_occ - n_lv; // searching or ready
         const bool go = n_rdy >= 64 ||
-                        (n_rdy > 0 && (n_rdy == n_occ || (q.exhausted && n_rdy * 8u >= n_occ * A.exec_fill)));
+                        (n_rdy > 0 && (n_rdy == n_act || (q.exhausted && n_rdy * 8u >= n_act * A.exec_fill)));
         if (!go) continue;
         const uint32_t n = pool_pick<NW>(rdy, (int)(iter % NW), p.assign);
         if (lane < n) {
