@@ -735,9 +735,10 @@ __global__ void __launch_bounds__(64) k_pooled(PersistArgs A) {
         // leave phase (dispersal reporting, where lineages reach the water
         // level every few steps): the entries whose next event passes it are
         // stored in a pass of their own, so execution passes do not run with
-        // lanes that only store
+        // lanes that only store (full waves of 64: measured faster than 16 or
+        // 32 per pass)
         if constexpr (DISP) {
-            if (n_lv >= 32 || (n_lv > 0 && (n_lv + n_rdy == n_occ || q.exhausted))) {
+            if (n_lv >= 64 || (n_lv > 0 && (n_lv + n_rdy == n_occ || q.exhausted))) {
                 const uint32_t n = pool_pick<NW>(lv, (int)(iter % NW), p.assign);
                 if (lane < n) {
                     const int e = p.assign[lane];
